@@ -1,0 +1,156 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X Hydro ADI step (plain FP64 forward, exact line solvers).
+
+Workload: the 2048 x 4096 tide beach (dx = dy = 2 m, seeded bar-trough bathymetry,
+1 m / 12.42 h tide through the offshore flux hook, Chezy 65, BreakingHrms viscosity),
+one GPU, inputs resident in HBM before the timed region. A "step" is one Hydro ADI step
+(solveQx + solveQy of the reference, src/Hydro.cpp:196-201); the timed region is one
+Run(K) call: the per-Run drag/viscosity/forcing pre-loop plus K steps.
+
+Prints one JSON line (see the `record` dict at the end).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E, GB/s
+B_ALG_STEP = 360.0          # algorithmic bytes per cell-step of the whole step
+
+# Algorithmic bytes per cell of each kernel: its compulsory FP64 inputs read once and
+# outputs written once (masks as 1 byte). Derivation per kernel in DESIGN.md.
+KERNEL_BYTES_PER_CELL = {
+    "qxx_assemble": 8 * 14 + 8 * 5 + 1,     # 14 fields in; a,b,c,f,drag out; mask out
+    "thomas_qx": 8 * 5,                     # a,b,c,f in; Qx* out
+    "qxy_assemble": 8 * 8 + 1 + 8 * 4,      # D,Vc,Qx*,nuD_x,dnuDx_dy,drag (+2 rows) in, mask in; a,b,c,f out
+    "periodic_qx": 8 * 5,                   # a,b,c,f in; Qx out
+    "qx_tail": 8 * 4 + 8 * 6,               # Qx,D,dQy,Zs in; Qx0,dQx,gZs,gD,U,Uc out
+    "qyy_assemble": 8 * 17 + 8 * 5 + 1,     # gD,gZs,dQy,Zs,dQx,U(2),Qy,Uc,V,Fy,nuD_y,dnuDy_dx,dnuD_dy,cD_y,Urms_y in
+    "periodic_qy": 8 * 5,
+    "qyx_assemble": 8 * 11 + 1 + 8 * 4,
+    "thomas_qy": 8 * 5,
+    "qy_tail": 8 * 5 + 8 * 5,               # Qy,dQx,D,Zs in; dQy,Zs,D,V,Vc out
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nx", type=int, default=2048)
+    ap.add_argument("--ny", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def cpu_baseline(sc, steps: int, threads: int) -> dict:
+    """The oracle (CPU restatement of the reference, OpenMP over independent lines) on
+    the host cores, on a bounded sample of the same workload."""
+    import oracle
+    threads = threads or min(16, os.cpu_count() or 1)
+    o = oracle.OracleHydro(sc, threads=threads)
+    o.run(1)                               # untimed: per-Run setup + one step
+    t0 = time.perf_counter()
+    o.run(steps)
+    dt = time.perf_counter() - t0
+    del o
+    rate = sc.nx * sc.ny * steps / dt
+    return {"value": rate, "unit": "cell-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{sc.nx}x{sc.ny} grid, one Run({steps}) after one warm step, "
+                      f"{threads} OpenMP threads, {dt:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
+                                   "described in DESIGN.md but not launched",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+    from morfo70_amd import scenarios as S
+    from morfo70_amd.hydro import from_scenario
+
+    total = args.warmup + args.steps + 1
+    sc = S.seeded_beach(args.nx, args.ny, 2.0, 2.0, args.seed, total, tide=True,
+                        name=f"beach_tide_{args.nx}x{args.ny}_s{args.seed}")
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    g = from_scenario(sc)
+    stream = torch.cuda.Stream(device=dev)          # default priority
+    g.set_stream(stream.cuda_stream)
+    g.set_forcing()
+    # warm-up (untimed)
+    g.run_async(args.warmup)
+    g.synchronize()
+    # timed region: one Run(K)
+    g.set_profiling(True)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    g.run_async(args.steps)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    prof = g.profile()
+    g.set_profiling(False)
+    finite = g.check_finite()
+    cells = sc.nx * sc.ny
+    value = cells * args.steps / (ms * 1e-3)
+    # dominant kernel
+    per_kernel = {k: (v[0] / v[1] if v[1] else 0.0, v[1], v[0]) for k, v in prof.items()}
+    dom = max((k for k in per_kernel if k in KERNEL_BYTES_PER_CELL), key=lambda k: per_kernel[k][2])
+    avg_ms = per_kernel[dom][0]
+    rows = sc.nx - 1 if dom not in ("qx_tail", "qy_tail") else sc.nx
+    dom_bytes = KERNEL_BYTES_PER_CELL[dom] * rows * sc.ny
+    achieved = dom_bytes / (avg_ms * 1e-3) / 1e9
+    step_gbs = B_ALG_STEP * value / 1e9
+    record = {
+        "metric": "ADI cell-steps/s vs plain and CPU at 1e-10 rel err; adjoint/forward time ratio",
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded bar-trough beach, closed-form wave forcing, tide)",
+        "config": {"workload": sc.name, "nx": sc.nx, "ny": sc.ny, "dx": 2.0, "dy": 2.0,
+                   "dt": sc.physics.dt, "tide": True, "solver": "exact",
+                   "timed": f"one Run({args.steps}) incl. per-Run pre-loop"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "avg_launch_ms": avg_ms,
+                     "alg_bytes_per_launch": dom_bytes},
+        "step_roofline": {"bytes_per_cell_step": B_ALG_STEP, "achieved": step_gbs,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
+        "kernels_avg_ms": {k: round(v[0], 5) for k, v in per_kernel.items()},
+        "finite": finite,
+    }
+    g.close()
+    if not args.no_cpu:
+        record["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps, args.cpu_threads)
+        record["gpu_over_cpu"] = value / record["cpu_baseline"]["value"]
+    else:
+        record["cpu_baseline"] = None
+    print(json.dumps(record), flush=True)
+
+
+if __name__ == "__main__":
+    main()

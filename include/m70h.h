@@ -1,0 +1,150 @@
+/*
+ * m70h.h -- C ABI of the MI355X-native Morfo70 Hydro ADI step (libm70h.so).
+ *
+ * The reference exposes the step as a plain C++ class with no FFI: `class Hydro`
+ * (include/Hydro.h:24-478) with ctor Hydro(Grid*,Bathymetry*,Wave*) (:103), ~Hydro()
+ * (:109), setHydroFromResumeFile(string) (:117), Run(int nIter) (:124) and public state
+ * arrays U, U_c, V, V_c, Zs, Qx, Qy, dQx_dx, dQy_dy (:26-34). Each entry point below
+ * replaces one of those members; the C++ facade (morfo70_amd/host/Hydro.h) rebuilds the
+ * reference class on top of them.
+ *
+ * Conventions:
+ *   - Arrays are row-major float64, index i*ny + j; i cross-shore (0 = offshore), j
+ *     alongshore (periodic). F = nx*ny; "F+" fields (Qx, Qy, U, V) have nx+1 rows: row 0
+ *     is the offshore ghost, row nx the onshore wall.
+ *   - Every call returns 0 (M70H_OK) or an M70H_E* code; m70h_last_error() has the text.
+ *   - A handle owns its device buffers and stream; it is not re-entrant. All device work
+ *     is enqueued on the handle's stream (m70h_set_stream), default: its own stream.
+ *   - Pointers named *_dev are device pointers on the handle's device; all others are
+ *     host pointers. Any state pointer may be NULL to leave that field unchanged.
+ */
+#ifndef M70H_H_
+#define M70H_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define M70H_OK 0
+#define M70H_EINVAL 1     /* bad argument (dims < 3, NULL handle, bad enum, ...) */
+#define M70H_EHIP 2       /* HIP runtime error; text in m70h_last_error */
+#define M70H_ENONFINITE 3 /* NaN/Inf found in the state by m70h_check_finite */
+#define M70H_ESTATE 4     /* call out of order (e.g. run before forcing is set) */
+
+/* Options::Hydro enums, include/OptionsParametersConstants.h:50-64 */
+#define M70H_DRAG_CONSTANT 0
+#define M70H_DRAG_LOGDEPTH 1
+#define M70H_DRAG_MANNING 2
+#define M70H_VISC_CONSTANT 0
+#define M70H_VISC_ROLLERS 1
+#define M70H_VISC_DEPTH 2
+#define M70H_VISC_HRMS 3
+#define M70H_BC_SPONGE 0
+#define M70H_BC_CONSTANT_ZS 1
+
+/* Line-solver families (m70h_set_solver). EXACT runs the reference's own Thomas and
+ * El-Mikkawy recurrences (src/Math.cpp:15-118) one line per lane, in the reference's
+ * operation order: results are bitwise equal to the reference's serial CPU step. */
+#define M70H_SOLVER_EXACT 0
+
+/* Parameters read by Hydro at call time in the reference (Parameters::Hydro::dt, Dmin,
+ * alpha_f, cD0, nu0, M, ka; Parameters::Bottom::z0; Parameters::Rollers::alpha_r;
+ * Options::Hydro::DragModel/ViscosityModel/OffshoreBC), passed explicitly here. */
+typedef struct m70h_params {
+    int drag_model;
+    int visc_model;
+    int offshore_bc;
+    double dt;
+    double Dmin;
+    double alpha_f;
+    double cD0;
+    double nu0;
+    double M;
+    double ka;
+    double z0;
+    double alpha_r;
+} m70h_params;
+
+typedef struct m70h m70h;
+
+/* Replaces Hydro::Hydro (include/Hydro.h:103; src/Hydro.cpp:17-133) together with the
+ * Grid metric set-up it depends on (src/Grid.cpp:32-92): x[nx] are the cross-shore node
+ * coordinates, dy the alongshore spacing. Allocates all device buffers and sets the
+ * offshore boundary constants (src/Hydro.cpp:44-60). State starts at zero. */
+int m70h_create(int nx, int ny, const double *x, double dy, const m70h_params *p, m70h **out);
+
+/* Replaces Hydro::~Hydro (include/Hydro.h:109). */
+void m70h_destroy(m70h *h);
+
+/* Text of the last error on this handle (or of the last failed m70h_create if h is NULL). */
+const char *m70h_last_error(const m70h *h);
+
+/* Use `hip_stream` (a hipStream_t) for all device work of this handle; NULL restores
+ * the handle's own stream. */
+int m70h_set_stream(m70h *h, void *hip_stream);
+
+/* Initial state of the reference ctor: D = bathy->D, Zs = D - Z + h (src/Hydro.cpp:35-37),
+ * all other state fields zero. D, Z, hb are host arrays of F doubles. */
+int m70h_init_from_bathymetry(m70h *h, const double *D, const double *Z, const double *hb);
+
+/* Full state exchange; replaces Hydro::setHydroFromResumeFile (include/Hydro.h:117) and
+ * the public arrays (include/Hydro.h:26-34) plus Bathymetry::D, which Hydro mutates
+ * (src/Hydro.cpp:267). Zs, D, U_c, V_c, dQx_dx, dQy_dy: F; Qx, Qy, U, V: F+. */
+int m70h_set_state(m70h *h, const double *Zs, const double *D, const double *Qx, const double *Qy,
+                   const double *U, const double *V, const double *U_c, const double *V_c,
+                   const double *dQx_dx, const double *dQy_dy);
+int m70h_get_state(m70h *h, double *Zs, double *D, double *Qx, double *Qy, double *U, double *V,
+                   double *U_c, double *V_c, double *dQx_dx, double *dQy_dy);
+int m70h_set_state_dev(m70h *h, const double *Zs_dev, const double *D_dev, const double *Qx_dev,
+                       const double *Qy_dev, const double *U_dev, const double *V_dev,
+                       const double *U_c_dev, const double *V_c_dev, const double *dQx_dx_dev,
+                       const double *dQy_dy_dev);
+int m70h_get_state_dev(m70h *h, double *Zs_dev, double *D_dev, double *Qx_dev, double *Qy_dev,
+                       double *U_dev, double *V_dev, double *U_c_dev, double *V_c_dev,
+                       double *dQx_dx_dev, double *dQy_dy_dev);
+
+/* Wave forcing read by Run's pre-loop: Wave::Sxx, Sxy, Syy, Urms, Dw, D_r, H
+ * (include/Wave.h:36-53; read at src/Hydro.cpp:175-181). F doubles each. */
+int m70h_set_forcing(m70h *h, const double *Sxx, const double *Sxy, const double *Syy,
+                     const double *Urms, const double *Dw, const double *D_r, const double *H);
+int m70h_set_forcing_dev(m70h *h, const double *Sxx_dev, const double *Sxy_dev,
+                         const double *Syy_dev, const double *Urms_dev, const double *Dw_dev,
+                         const double *D_r_dev, const double *H_dev);
+
+/* Offshore flux hook Qx0 = f1*Qx1 + f0 (include/Hydro.h:89-90, used at
+ * src/HydroBoundaryConditions.cpp:13-39). The reference fixes f0 = 0
+ * (src/Hydro.cpp:52,56); a series f0[k] gives a time-varying boundary (tide): the k-th
+ * step run after the call uses f0[k] (the last value repeats past the end). */
+int m70h_set_offshore_f0(m70h *h, double f0);
+int m70h_set_offshore_f0_series(m70h *h, const double *f0, long n);
+int m70h_bc_constants(const m70h *h, double *f1, double *f0, double *bcQy);
+
+/* Replaces Hydro::Run (include/Hydro.h:124; src/Hydro.cpp:170-202): per-call drag,
+ * viscosity and forcing fields from the current D and forcing, then nIter ADI steps.
+ * m70h_run returns when the steps are done; m70h_run_async only enqueues them on the
+ * handle's stream (m70h_synchronize waits and reports errors). */
+int m70h_run(m70h *h, int nIter);
+int m70h_run_async(m70h *h, int nIter);
+int m70h_synchronize(m70h *h);
+
+/* Line-solver family, M70H_SOLVER_* (default EXACT). */
+int m70h_set_solver(m70h *h, int solver);
+
+/* Scans the state for NaN/Inf on the device; *finite = 1 if none. Returns
+ * M70H_ENONFINITE when a non-finite value is found. */
+int m70h_check_finite(m70h *h, int *finite);
+
+/* Per-kernel timing with HIP events on the handle's stream. When enabled, run records an
+ * event pair around every launch; m70h_profile reports kernel k's name, summed
+ * milliseconds and launch count (returns M70H_EINVAL past the last kernel). */
+int m70h_set_profiling(m70h *h, int enable);
+int m70h_profile(m70h *h, int k, const char **name, double *ms, long *launches);
+
+/* Device pointer of the current state field `name` ("Zs","D","Qx","Qy","U","V","U_c",
+ * "V_c","dQx_dx","dQy_dy"), valid until the next run. */
+int m70h_state_ptr(m70h *h, const char *name, double **dev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

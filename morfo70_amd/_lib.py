@@ -1,0 +1,85 @@
+"""ctypes loader of libm70h.so (include/m70h.h). Fails loudly when the library is missing:
+there is no CPU fallback for the Hydro step."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libm70h.so")
+
+M70H_OK, M70H_EINVAL, M70H_EHIP, M70H_ENONFINITE, M70H_ESTATE = 0, 1, 2, 3, 4
+SOLVER_EXACT = 0
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_lib = None
+
+# every symbol include/m70h.h declares, with (restype, argtypes)
+SIGNATURES = {
+    "m70h_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, _vp,
+                                   ctypes.POINTER(_vp)]),
+    "m70h_destroy": (None, [_vp]),
+    "m70h_last_error": (ctypes.c_char_p, [_vp]),
+    "m70h_set_stream": (ctypes.c_int, [_vp, _vp]),
+    "m70h_init_from_bathymetry": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "m70h_set_state": (ctypes.c_int, [_vp] + [_dp] * 10),
+    "m70h_get_state": (ctypes.c_int, [_vp] + [_dp] * 10),
+    "m70h_set_state_dev": (ctypes.c_int, [_vp] + [_vp] * 10),
+    "m70h_get_state_dev": (ctypes.c_int, [_vp] + [_vp] * 10),
+    "m70h_set_forcing": (ctypes.c_int, [_vp] + [_dp] * 7),
+    "m70h_set_forcing_dev": (ctypes.c_int, [_vp] + [_vp] * 7),
+    "m70h_set_offshore_f0": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "m70h_set_offshore_f0_series": (ctypes.c_int, [_vp, _dp, ctypes.c_long]),
+    "m70h_bc_constants": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
+    "m70h_run": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "m70h_run_async": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "m70h_synchronize": (ctypes.c_int, [_vp]),
+    "m70h_set_solver": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "m70h_check_finite": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
+    "m70h_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "m70h_profile": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _dp,
+                                    ctypes.POINTER(ctypes.c_long)]),
+    "m70h_state_ptr": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),
+}
+
+
+class M70HError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"m70h error {code}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    """m70h_params (include/m70h.h)."""
+    _fields_ = [("drag_model", ctypes.c_int), ("visc_model", ctypes.c_int),
+                ("offshore_bc", ctypes.c_int), ("dt", ctypes.c_double), ("Dmin", ctypes.c_double),
+                ("alpha_f", ctypes.c_double), ("cD0", ctypes.c_double), ("nu0", ctypes.c_double),
+                ("M", ctypes.c_double), ("ka", ctypes.c_double), ("z0", ctypes.c_double),
+                ("alpha_r", ctypes.c_double)]
+
+
+def load(path: str | None = None):
+    """Loads libm70h.so, raising if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} is missing: build it with `python -m morfo70_amd.build` "
+                           "(the Hydro step has no CPU fallback)")
+    L = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = L
+    return L
+
+
+def check(L, h, rc: int):
+    if rc != M70H_OK:
+        msg = L.m70h_last_error(h)
+        raise M70HError(rc, msg.decode() if msg else "")
+    return rc

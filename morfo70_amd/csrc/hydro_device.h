@@ -1,0 +1,108 @@
+// hydro_device.h -- device-side data structures and helpers of the Hydro ADI step.
+//
+// Every arithmetic expression in the kernels keeps the operand order of the reference's
+// C++ (src/HydroQx.cpp, src/HydroQy.cpp, src/Hydro.cpp, src/HydroParametrizations.cpp,
+// src/HydroMeshOperations.cpp, src/HydroBoundaryConditions.cpp) and the library is built
+// with -ffp-contract=off, so each value is the IEEE double the reference computes.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace m70 {
+
+// include/OptionsParametersConstants.h:14-19
+constexpr double G = 9.8100000000000000;
+constexpr double RHO = 1025.00000000000;
+constexpr double SQRT_2 = 1.414213562373095;
+
+// Per-row grid metrics (src/Grid.cpp:32-92) plus the per-row reciprocals the reference
+// forms inline (1/dx[i] at src/HydroQx.cpp:105, 1/dx_x[i] at src/HydroQy.cpp:129,266 and
+// src/HydroMeshOperations.cpp:106, inv_rho/dx[i] at src/HydroQx.cpp:372).
+struct Metrics {
+    const double *inv_dx;     // [nx-1]
+    const double *inv_dxx;    // [nx]
+    const double *inv_rhodx;  // [nx-1]
+    const double *difx_x_up, *difx_x_d, *difx_x_low;     // [nx-1]
+    const double *dif2x_x_up, *dif2x_x_d, *dif2x_x_low;  // [nx-1]
+    const double *difx_c_up, *difx_c_d, *difx_c_low;     // [nx]
+    const double *dif2x_c_up, *dif2x_c_d, *dif2x_c_low;  // [nx]
+    const double *x2c_up, *x2c_d;                        // [nx]
+};
+
+// Scalars. Each derived constant is formed on the host with the expression the reference
+// uses at the cited line, so it holds the same bits.
+struct Consts {
+    int nx, ny;
+    int drag_model, visc_model;
+    double dt;
+    double inv_dt;        // 1/dt                       src/HydroQx.cpp:103
+    double inv_dy;        // 1/dy                       src/HydroQx.cpp:104
+    double inv_2dy;       // 0.5/dy                     src/HydroQx.cpp:301
+    double inv_dy2_xy;    // 4*inv_2dy*inv_2dy          src/HydroQx.cpp:302
+    double inv_dy2_yy;    // inv_dy*inv_dy              src/HydroQy.cpp:187
+    double two_inv_dy2;   // 2/(dy*dy)                  src/HydroQx.cpp:282
+    double alpha_f2;      // alpha_f*alpha_f            src/HydroParametrizations.cpp:20
+    double inv_sqrt2;     // 1/sqrt_2                   src/HydroParametrizations.cpp:22
+    double Dmask;         // Dmin+1E-4                  src/HydroQx.cpp:236
+    double f1;            // bcOffshoreQx_f1            src/Hydro.cpp:51,55
+    double bcQy;          // bcOffshoreQy               src/Hydro.cpp:60
+    double inv_3;         // 1.0/3.0                    src/HydroBoundaryConditions.cpp:54
+    double inv_rho;       // 1/rho                      src/HydroQx.cpp:360
+    double inv_4dy_rho;   // (0.25/dy)*inv_rho          src/HydroQx.cpp:361-362
+    double inv_dy_rho;    // (1/dy)*inv_rho             src/HydroQy.cpp:372-373
+    double inv_2dy_c2c;   // 0.5/dy                     src/HydroMeshOperations.cpp:204
+    double cD0, nu0, M, ka, z0, alpha_r;
+    float inv_rho_f;      // float 1/rho                src/HydroParametrizations.cpp:107
+};
+
+struct Forcing {
+    const double *Sxx, *Sxy, *Syy, *Urms, *Dw, *D_r, *H;
+};
+
+// Fields computed once per Run (src/Hydro.cpp:175-192).
+struct RunFields {
+    double *cD, *nuD;                  // F
+    double *Fx, *Urms_x, *cD_x, *nuD_x, *dnuD_dx, *dnuDx_dy;   // rows 0..nx-2
+    double *Fy, *Urms_y, *cD_y, *nuD_y, *dnuD_dy, *dnuDy_dx;   // rows 0..nx-1
+};
+
+// State (include/Hydro.h:26-34 plus Bathymetry::D). D is double-buffered: the Qy tail
+// reads D and writes Dn, then the host swaps them.
+struct State {
+    double *Zs, *D, *Dn, *Qx, *Qy, *U, *V, *Uc, *Vc, *dQx, *dQy;
+};
+
+// Per-step work arrays.
+struct Work {
+    double *a, *b, *c, *f;     // tridiagonal coefficients (diag, low, up, rhs), row-major F
+    double *gZs, *gD;          // guess_Zs, guess_D (src/Hydro.cpp:230-250)
+    double *drag;              // DragTerm of the predictor, reused by the projector
+    uint8_t *w;                // IsWater of the predictor, reused by the projector
+};
+
+__device__ __forceinline__ int wrap_inc(int j, int ny) { return j + 1 == ny ? 0 : j + 1; }
+__device__ __forceinline__ int wrap_dec(int j, int ny) { return j == 0 ? ny - 1 : j - 1; }
+
+// Body of Hydro::setWaterNodes_x/_y (src/HydroQx.cpp:245-253, src/HydroQy.cpp:103-109):
+// integer arithmetic on the six comparisons, then conversion to bool.
+__device__ __forceinline__ int wet_rule(double d, double dup, double zs, double zsup, double s,
+                                        double sup, double Dmin)
+{
+    int dup_gt_min = dup > Dmin;
+    int d_gt_min = d > Dmin;
+    int zsup_gt_zs = zsup > zs;
+    int zs_gt_zsup = zs > zsup;
+    int sink = s > 0;
+    int sinkup = sup > 0;
+    return (d_gt_min * dup_gt_min +
+            (dup_gt_min * (!d_gt_min) * zsup_gt_zs + d_gt_min * (!dup_gt_min) * zs_gt_zsup) * !(sink + sinkup)) != 0;
+}
+
+// Hydro::calculateDragTerm, src/HydroParametrizations.cpp:19-28 (one element).
+__device__ __forceinline__ double drag_term(double cd, double urms, double u, double v, double inv_d,
+                                            const Consts &k)
+{
+    return cd * urms * k.inv_sqrt2 * sqrt(k.alpha_f2 + 2 * (v * v + u * u) / (urms * urms)) * inv_d;
+}
+
+}  // namespace m70

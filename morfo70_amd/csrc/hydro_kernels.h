@@ -1,0 +1,32 @@
+// hydro_kernels.h -- host launchers of the Hydro ADI kernels (internal to libm70h.so).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "hydro_device.h"
+
+namespace m70 {
+
+hipError_t launch_drag_viscosity(hipStream_t st, const Consts &k, const double *D, const Forcing &fw,
+                                 const RunFields &rf);
+hipError_t launch_run_fields(hipStream_t st, const Consts &k, const Metrics &m, const Forcing &fw,
+                             const RunFields &rf);
+hipError_t launch_qxx_assemble(hipStream_t st, const Consts &k, const Metrics &m, const State &s,
+                               const RunFields &rf, const Work &w, double f0);
+hipError_t launch_qxy_assemble(hipStream_t st, const Consts &k, const State &s, const RunFields &rf,
+                               const Work &w);
+hipError_t launch_qx_tail(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const Work &w,
+                          double f0);
+hipError_t launch_qyy_assemble(hipStream_t st, const Consts &k, const Metrics &m, const State &s,
+                               const RunFields &rf, const Work &w);
+hipError_t launch_qyx_assemble(hipStream_t st, const Consts &k, const Metrics &m, const State &s,
+                               const RunFields &rf, const Work &w);
+hipError_t launch_qy_tail(hipStream_t st, const Consts &k, const State &s);
+hipError_t launch_nonfinite(hipStream_t st, const double *x, long n, int *flag);
+hipError_t launch_init_zs(hipStream_t st, double *Zs, const double *D, const double *Z, const double *hb, long n);
+
+hipError_t launch_thomas_cols(hipStream_t st, const double *A, const double *B, const double *C, const double *F,
+                              double *Y, double *V, int n, int m, long ld);
+int periodic_rows_padded_lines(int nL);
+hipError_t launch_periodic_rows(hipStream_t st, const double *A, const double *B, const double *C, const double *F,
+                                long ld, double *Y, long ldy, double *Sr, double *Sv, double *Sk, int nL, int n);
+
+}  // namespace m70

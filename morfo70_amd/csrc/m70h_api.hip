@@ -1,0 +1,614 @@
+// m70h_api.hip -- C ABI of the Hydro ADI step (include/m70h.h): handle, device buffers,
+// the per-Run pre-loop and the per-step launch sequence of Hydro::Run
+// (src/Hydro.cpp:170-202).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/m70h.h"
+#include "hydro_device.h"
+#include "hydro_kernels.h"
+
+using namespace m70;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+enum KernelId {
+    K_DRAG_VISC, K_RUN_FIELDS, K_QXX_ASM, K_THOMAS_QX, K_QXY_ASM, K_PERIODIC_QX, K_QX_TAIL,
+    K_QYY_ASM, K_PERIODIC_QY, K_QYX_ASM, K_THOMAS_QY, K_QY_TAIL, K_COUNT
+};
+const char *const kKernelNames[K_COUNT] = {
+    "drag_viscosity", "run_fields", "qxx_assemble", "thomas_qx", "qxy_assemble", "periodic_qx", "qx_tail",
+    "qyy_assemble", "periodic_qy", "qyx_assemble", "thomas_qy", "qy_tail"};
+
+}  // namespace
+
+struct m70h {
+    int nx = 0, ny = 0;
+    long F = 0, Fp = 0;
+    m70h_params p{};
+    Consts k{};
+    double f0 = 0.0;
+    std::vector<double> f0_series;
+    long f0_pos = 0;
+    int device = 0;
+    hipStream_t own_stream = nullptr, stream = nullptr;
+    int solver = M70H_SOLVER_EXACT;
+    std::string err;
+    std::vector<void *> allocs;
+    double *metric_buf = nullptr;
+    Metrics m{};
+    State s{};
+    Forcing fw{};
+    double *fw_buf[7] = {};
+    bool forcing_set = false;
+    RunFields rf{};
+    Work w{};
+    double *sr = nullptr, *sv = nullptr, *sk = nullptr;
+    int *flag = nullptr;
+    // profiling
+    bool prof = false;
+    struct Rec { int kid; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    double prof_ms[K_COUNT] = {};
+    long prof_n[K_COUNT] = {};
+};
+
+namespace {
+
+int fail(m70h *h, int code, const std::string &msg)
+{
+    if (h) h->err = msg; else g_create_error = msg;
+    return code;
+}
+
+int hip_fail(m70h *h, hipError_t e, const char *what)
+{
+    return fail(h, M70H_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(h, expr)                                   \
+    do {                                                  \
+        hipError_t e_ = (expr);                           \
+        if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(m70h *h, T **ptr, size_t count)
+{
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, count * sizeof(T) + 64);
+    if (e != hipSuccess) return e;
+    h->allocs.push_back(p);
+    *ptr = (T *)p;
+    return hipMemset(p, 0, count * sizeof(T) + 64);
+}
+
+hipEvent_t next_event(m70h *h)
+{
+    if (h->ev_used == h->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        h->ev_pool.push_back(e);
+    }
+    return h->ev_pool[h->ev_used++];
+}
+
+// Runs one launch, bracketed by events when profiling is on.
+template <typename Fn>
+int timed(m70h *h, int kid, Fn &&fn)
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    if (h->prof) {
+        a = next_event(h);
+        b = next_event(h);
+        if (a && b) (void)hipEventRecord(a, h->stream);
+    }
+    hipError_t e = fn();
+    if (e != hipSuccess) return hip_fail(h, e, kKernelNames[kid]);
+    if (h->prof && a && b) {
+        (void)hipEventRecord(b, h->stream);
+        h->recs.push_back({kid, a, b});
+    }
+    return M70H_OK;
+}
+
+int collect_profile(m70h *h)
+{
+    if (h->recs.empty()) return M70H_OK;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (auto &r : h->recs) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            h->prof_ms[r.kid] += ms;
+            h->prof_n[r.kid] += 1;
+        }
+    }
+    h->recs.clear();
+    h->ev_used = 0;
+    return M70H_OK;
+}
+
+double f0_for_step(m70h *h)
+{
+    if (!h->f0_series.empty()) {
+        long q = h->f0_pos < (long)h->f0_series.size() ? h->f0_pos : (long)h->f0_series.size() - 1;
+        return h->f0_series[q];
+    }
+    return h->f0;
+}
+
+// Per-Run pre-loop, src/Hydro.cpp:175-192.
+int run_preloop(m70h *h)
+{
+    int rc = timed(h, K_DRAG_VISC, [&] { return launch_drag_viscosity(h->stream, h->k, h->s.D, h->fw, h->rf); });
+    if (rc) return rc;
+    return timed(h, K_RUN_FIELDS, [&] { return launch_run_fields(h->stream, h->k, h->m, h->fw, h->rf); });
+}
+
+// One ADI step: solveQx (src/HydroQx.cpp:18-93) then solveQy (src/HydroQy.cpp:17-85).
+int run_step(m70h *h)
+{
+    const double f0 = f0_for_step(h);
+    h->f0_pos++;
+    const int nx = h->nx, ny = h->ny;
+    const Consts &k = h->k;
+    State &s = h->s;
+    Work &w = h->w;
+    int rc;
+    hipStream_t st = h->stream;
+    // Qx predictor: assembly + Thomas along x (src/HydroQx.cpp:29-51)
+    if ((rc = timed(h, K_QXX_ASM, [&] { return launch_qxx_assemble(st, k, h->m, s, h->rf, w, f0); }))) return rc;
+    if ((rc = timed(h, K_THOMAS_QX,
+                    [&] { return launch_thomas_cols(st, w.a, w.b, w.c, w.f, s.Qx + ny, h->sv, nx - 1, ny, ny); })))
+        return rc;
+    // Qx projector: assembly + periodic solve along y per row (src/HydroQx.cpp:59-70)
+    if ((rc = timed(h, K_QXY_ASM, [&] { return launch_qxy_assemble(st, k, s, h->rf, w); }))) return rc;
+    if ((rc = timed(h, K_PERIODIC_QX, [&] {
+             return launch_periodic_rows(st, w.a, w.b, w.c, w.f, ny, s.Qx + ny, ny, h->sr, h->sv, h->sk, nx - 1, ny);
+         })))
+        return rc;
+    // BC, dQx_dx, guess, U, U_c (src/HydroQx.cpp:76-92)
+    if ((rc = timed(h, K_QX_TAIL, [&] { return launch_qx_tail(st, k, h->m, s, w, f0); }))) return rc;
+    // Qy predictor (src/HydroQy.cpp:26-42)
+    if ((rc = timed(h, K_QYY_ASM, [&] { return launch_qyy_assemble(st, k, h->m, s, h->rf, w); }))) return rc;
+    if ((rc = timed(h, K_PERIODIC_QY, [&] {
+             return launch_periodic_rows(st, w.a, w.b, w.c, w.f, ny, s.Qy + ny, ny, h->sr, h->sv, h->sk, nx - 1, ny);
+         })))
+        return rc;
+    // Qy projector (src/HydroQy.cpp:52-67)
+    if ((rc = timed(h, K_QYX_ASM, [&] { return launch_qyx_assemble(st, k, h->m, s, h->rf, w); }))) return rc;
+    if ((rc = timed(h, K_THOMAS_QY,
+                    [&] { return launch_thomas_cols(st, w.a, w.b, w.c, w.f, s.Qy + ny, h->sv, nx - 1, ny, ny); })))
+        return rc;
+    // BC, dQy_dy, Zs, D, V, V_c (src/HydroQy.cpp:73-84)
+    if ((rc = timed(h, K_QY_TAIL, [&] { return launch_qy_tail(st, k, s); }))) return rc;
+    std::swap(s.D, s.Dn);
+    return M70H_OK;
+}
+
+// Grid metrics as Grid::Grid (src/Grid.cpp:32-92, closed forms :127-145).
+void grid_metrics(int nx, const double *x, std::vector<double> &dx, std::vector<double> &dx_x,
+                  std::vector<double> (&xx)[6], std::vector<double> (&cc)[6], std::vector<double> &x2c_up,
+                  std::vector<double> &x2c_d)
+{
+    auto dif = [](double dx0, double dx1, double &up, double &d, double &low) {
+        double dx02 = dx0 * dx0;
+        double dx12 = dx1 * dx1;
+        double dx0dx1 = dx0 * dx1;
+        double dx0_p_dx1 = dx0 + dx1;
+        d = (dx12 - dx02) / (dx0_p_dx1 * dx0dx1);
+        up = dx02 / (dx0_p_dx1 * dx0dx1);
+        low = -dx12 / (dx0_p_dx1 * dx0dx1);
+    };
+    auto dif2 = [](double dx0, double dx1, double &up, double &d, double &low) {
+        double dx0dx1 = dx0 * dx1;
+        double dx0_p_dx1 = dx0 + dx1;
+        d = -2 / (dx0dx1);
+        up = 2 / (dx0_p_dx1 * dx1);
+        low = 2 / (dx0_p_dx1 * dx0);
+    };
+    dx.assign(nx - 1, 0.0);
+    dx_x.assign(nx, 0.0);
+    for (auto &v : xx) v.assign(nx - 1, 0.0);
+    for (auto &v : cc) v.assign(nx, 0.0);
+    x2c_up.assign(nx, 0.0);
+    x2c_d.assign(nx, 0.0);
+    for (int i = 0; i < nx - 1; i++) dx[i] = x[i + 1] - x[i];
+    dx_x[0] = dx[0];
+    for (int i = 1; i < nx - 1; i++) dx_x[i] = (dx[i] + dx[i - 1]) / 2;
+    dx_x[nx - 1] = dx[nx - 2];
+    // xx: up, d, low, up2, d2, low2 at x-faces; cc: same at centres
+    for (int i = 0; i < nx - 1; i++) {
+        dif(dx_x[i], dx_x[i + 1], xx[0][i], xx[1][i], xx[2][i]);
+        dif2(dx_x[i], dx_x[i + 1], xx[3][i], xx[4][i], xx[5][i]);
+    }
+    dif(dx[0], dx[1], cc[0][0], cc[1][0], cc[2][0]);
+    dif2(dx[0], dx[1], cc[3][0], cc[4][0], cc[5][0]);
+    for (int i = 1; i < nx - 1; i++) {
+        dif(dx[i - 1], dx[i], cc[0][i], cc[1][i], cc[2][i]);
+        dif2(dx[i - 1], dx[i], cc[3][i], cc[4][i], cc[5][i]);
+    }
+    dif(dx[nx - 2], dx[nx - 2] / 2, cc[0][nx - 1], cc[1][nx - 1], cc[2][nx - 1]);
+    dif2(dx[nx - 2], dx[nx - 2] / 2, cc[3][nx - 1], cc[4][nx - 1], cc[5][nx - 1]);
+    double dx0 = dx[0];
+    double dx1 = dx0;
+    double inv_dx1pdx0 = 1 / (dx1 + dx0);
+    x2c_up[0] = dx0 * inv_dx1pdx0;
+    x2c_d[0] = dx1 * inv_dx1pdx0;
+    for (int i = 1; i < nx - 1; i++) {
+        dx0 = dx1;
+        dx1 = dx[i];
+        inv_dx1pdx0 = 1 / (dx1 + dx0);
+        x2c_up[i] = dx0 * inv_dx1pdx0;
+        x2c_d[i] = dx1 * inv_dx1pdx0;
+    }
+    x2c_up[nx - 1] = 0.5;
+    x2c_d[nx - 1] = 0.5;
+}
+
+int copy_fields(m70h *h, double *const *dst, const double *const *src, const long *n, int count, hipMemcpyKind kind)
+{
+    for (int q = 0; q < count; ++q) {
+        if (!src[q] || !dst[q]) continue;
+        HIPCHK(h, hipMemcpyAsync(dst[q], src[q], (size_t)n[q] * sizeof(double), kind, h->stream));
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return M70H_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *m70h_last_error(const m70h *h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int m70h_create(int nx, int ny, const double *x, double dy, const m70h_params *p, m70h **out)
+{
+    if (!out) return fail(nullptr, M70H_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (nx < 3 || ny < 3) return fail(nullptr, M70H_EINVAL, "nx and ny must be >= 3");
+    if (!x || !p) return fail(nullptr, M70H_EINVAL, "x and p must be non-NULL");
+    if (!(dy > 0.0)) return fail(nullptr, M70H_EINVAL, "dy must be > 0");
+    if (p->drag_model < 0 || p->drag_model > 2 || p->visc_model < 0 || p->visc_model > 3 || p->offshore_bc < 0 ||
+        p->offshore_bc > 1)
+        return fail(nullptr, M70H_EINVAL, "bad model enum");
+    for (int i = 0; i < nx - 1; ++i)
+        if (!(x[i + 1] > x[i])) return fail(nullptr, M70H_EINVAL, "x must be strictly increasing");
+    m70h *h = new m70h();
+    h->nx = nx;
+    h->ny = ny;
+    h->F = (long)nx * ny;
+    h->Fp = h->F + ny;
+    h->p = *p;
+    auto bail = [&](int rc) {
+        g_create_error = h->err;
+        m70h_destroy(h);
+        return rc;
+    };
+    hipError_t e;
+    if ((e = hipGetDevice(&h->device)) != hipSuccess) return bail(hip_fail(h, e, "hipGetDevice"));
+    if ((e = hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking)) != hipSuccess)
+        return bail(hip_fail(h, e, "hipStreamCreate"));
+    h->stream = h->own_stream;
+
+    // metrics
+    std::vector<double> dx, dx_x, xx[6], cc[6], x2c_up, x2c_d;
+    grid_metrics(nx, x, dx, dx_x, xx, cc, x2c_up, x2c_d);
+    const double inv_rho = 1 / RHO;
+    std::vector<double> host((size_t)22 * nx, 0.0);
+    double *hp = host.data();
+    auto put = [&](const std::vector<double> &v) { double *at = hp; std::copy(v.begin(), v.end(), hp); hp += nx; return at - host.data(); };
+    std::vector<double> inv_dx(nx - 1), inv_dxx(nx), inv_rhodx(nx - 1);
+    for (int i = 0; i < nx - 1; ++i) { inv_dx[i] = 1 / dx[i]; inv_rhodx[i] = inv_rho / dx[i]; }
+    for (int i = 0; i < nx; ++i) inv_dxx[i] = 1 / dx_x[i];
+    long o_inv_dx = put(inv_dx), o_inv_dxx = put(inv_dxx), o_inv_rhodx = put(inv_rhodx);
+    long o_xx[6], o_cc[6];
+    for (int q = 0; q < 6; ++q) o_xx[q] = put(xx[q]);
+    for (int q = 0; q < 6; ++q) o_cc[q] = put(cc[q]);
+    long o_x2c_up = put(x2c_up), o_x2c_d = put(x2c_d);
+    if ((e = dalloc(h, &h->metric_buf, host.size())) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc"));
+    if ((e = hipMemcpy(h->metric_buf, host.data(), host.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+        return bail(hip_fail(h, e, "hipMemcpy"));
+    const double *mb = h->metric_buf;
+    h->m.inv_dx = mb + o_inv_dx;
+    h->m.inv_dxx = mb + o_inv_dxx;
+    h->m.inv_rhodx = mb + o_inv_rhodx;
+    h->m.difx_x_up = mb + o_xx[0]; h->m.difx_x_d = mb + o_xx[1]; h->m.difx_x_low = mb + o_xx[2];
+    h->m.dif2x_x_up = mb + o_xx[3]; h->m.dif2x_x_d = mb + o_xx[4]; h->m.dif2x_x_low = mb + o_xx[5];
+    h->m.difx_c_up = mb + o_cc[0]; h->m.difx_c_d = mb + o_cc[1]; h->m.difx_c_low = mb + o_cc[2];
+    h->m.dif2x_c_up = mb + o_cc[3]; h->m.dif2x_c_d = mb + o_cc[4]; h->m.dif2x_c_low = mb + o_cc[5];
+    h->m.x2c_up = mb + o_x2c_up;
+    h->m.x2c_d = mb + o_x2c_d;
+
+    // constants (src/Hydro.cpp:44-60 for the boundary constants)
+    Consts &k = h->k;
+    k.nx = nx; k.ny = ny;
+    k.drag_model = p->drag_model; k.visc_model = p->visc_model;
+    k.dt = p->dt;
+    k.inv_dt = 1 / p->dt;
+    k.inv_dy = 1 / dy;
+    k.inv_2dy = 0.5 / dy;
+    k.inv_dy2_xy = 4 * k.inv_2dy * k.inv_2dy;
+    k.inv_dy2_yy = k.inv_dy * k.inv_dy;
+    k.two_inv_dy2 = 2 / (dy * dy);
+    k.alpha_f2 = (p->alpha_f * p->alpha_f);
+    k.inv_sqrt2 = 1 / SQRT_2;
+    k.Dmask = p->Dmin + 1E-4;
+    {
+        const double lSponge = dx_x[0] * 5;
+        const double exp_dx_2ls = std::exp(-0.5 * dx_x[0] / lSponge);
+        const double exp_dx_ls = std::exp(-dx_x[0] / lSponge);
+        k.f1 = (p->offshore_bc == M70H_BC_SPONGE) ? 0.5 * exp_dx_2ls / (1 - 0.5 * exp_dx_2ls) : 1;
+        k.bcQy = exp_dx_ls;
+    }
+    k.inv_3 = 1.0 / 3.0;
+    k.inv_rho = inv_rho;
+    {
+        double inv_4dy = 0.25 / dy;
+        inv_4dy *= inv_rho;
+        k.inv_4dy_rho = inv_4dy;
+        double inv_dyr = 1 / dy;
+        inv_dyr *= inv_rho;
+        k.inv_dy_rho = inv_dyr;
+    }
+    k.inv_2dy_c2c = 0.5 / dy;
+    k.cD0 = p->cD0; k.nu0 = p->nu0; k.M = p->M; k.ka = p->ka; k.z0 = p->z0; k.alpha_r = p->alpha_r;
+    k.inv_rho_f = (float)(1 / RHO);
+
+    // buffers
+    const long F = h->F, Fp = h->Fp;
+    double **stF[] = {&h->s.Zs, &h->s.D, &h->s.Dn, &h->s.Uc, &h->s.Vc, &h->s.dQx, &h->s.dQy};
+    for (auto q : stF)
+        if ((e = dalloc(h, q, F)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc state"));
+    double **stFp[] = {&h->s.Qx, &h->s.Qy, &h->s.U, &h->s.V};
+    for (auto q : stFp)
+        if ((e = dalloc(h, q, Fp)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc state"));
+    for (int q = 0; q < 7; ++q)
+        if ((e = dalloc(h, &h->fw_buf[q], F)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc forcing"));
+    h->fw = Forcing{h->fw_buf[0], h->fw_buf[1], h->fw_buf[2], h->fw_buf[3], h->fw_buf[4], h->fw_buf[5], h->fw_buf[6]};
+    double **rfp[] = {&h->rf.cD, &h->rf.nuD, &h->rf.Fx, &h->rf.Urms_x, &h->rf.cD_x, &h->rf.nuD_x, &h->rf.dnuD_dx,
+                      &h->rf.dnuDx_dy, &h->rf.Fy, &h->rf.Urms_y, &h->rf.cD_y, &h->rf.nuD_y, &h->rf.dnuD_dy,
+                      &h->rf.dnuDy_dx};
+    for (auto q : rfp)
+        if ((e = dalloc(h, q, F)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc run fields"));
+    double **wk[] = {&h->w.a, &h->w.b, &h->w.c, &h->w.f, &h->w.gZs, &h->w.gD, &h->w.drag};
+    for (auto q : wk)
+        if ((e = dalloc(h, q, F)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc work"));
+    if ((e = dalloc(h, &h->w.w, F)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc mask"));
+    const long nLp = periodic_rows_padded_lines(nx - 1);
+    const long scr = (long)ny * nLp > F ? (long)ny * nLp : F;
+    if ((e = dalloc(h, &h->sr, scr)) != hipSuccess || (e = dalloc(h, &h->sv, scr)) != hipSuccess ||
+        (e = dalloc(h, &h->sk, scr)) != hipSuccess)
+        return bail(hip_fail(h, e, "hipMalloc scratch"));
+    if ((e = dalloc(h, &h->flag, 1)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc flag"));
+    *out = h;
+    return M70H_OK;
+}
+
+void m70h_destroy(m70h *h)
+{
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void *q : h->allocs) (void)hipFree(q);
+    for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+    if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+    delete h;
+}
+
+int m70h_set_stream(m70h *h, void *hip_stream)
+{
+    if (!h) return M70H_EINVAL;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->stream = hip_stream ? (hipStream_t)hip_stream : h->own_stream;
+    return M70H_OK;
+}
+
+int m70h_init_from_bathymetry(m70h *h, const double *D, const double *Z, const double *hb)
+{
+    if (!h || !D || !Z || !hb) return h ? fail(h, M70H_EINVAL, "NULL array") : M70H_EINVAL;
+    State &s = h->s;
+    double *zero[] = {s.Qx, s.Qy, s.U, s.V};
+    for (double *q : zero) HIPCHK(h, hipMemsetAsync(q, 0, (size_t)h->Fp * sizeof(double), h->stream));
+    double *zeroF[] = {s.Uc, s.Vc, s.dQx, s.dQy};
+    for (double *q : zeroF) HIPCHK(h, hipMemsetAsync(q, 0, (size_t)h->F * sizeof(double), h->stream));
+    const size_t nb = (size_t)h->F * sizeof(double);
+    // Z and h pass through the work arrays a, b
+    HIPCHK(h, hipMemcpyAsync(s.D, D, nb, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->w.a, Z, nb, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipMemcpyAsync(h->w.b, hb, nb, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, launch_init_zs(h->stream, s.Zs, s.D, h->w.a, h->w.b, h->F));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return M70H_OK;
+}
+
+static int state_xfer(m70h *h, double *const *hostside, bool to_device, hipMemcpyKind kind)
+{
+    State &s = h->s;
+    double *dev[10] = {s.Zs, s.D, s.Qx, s.Qy, s.U, s.V, s.Uc, s.Vc, s.dQx, s.dQy};
+    const long n[10] = {h->F, h->F, h->Fp, h->Fp, h->Fp, h->Fp, h->F, h->F, h->F, h->F};
+    if (to_device) return copy_fields(h, dev, hostside, n, 10, kind);
+    return copy_fields(h, hostside, dev, n, 10, kind);
+}
+
+int m70h_set_state(m70h *h, const double *Zs, const double *D, const double *Qx, const double *Qy, const double *U,
+                   const double *V, const double *U_c, const double *V_c, const double *dQx_dx, const double *dQy_dy)
+{
+    if (!h) return M70H_EINVAL;
+    double *src[10] = {(double *)Zs, (double *)D, (double *)Qx, (double *)Qy, (double *)U, (double *)V,
+                       (double *)U_c, (double *)V_c, (double *)dQx_dx, (double *)dQy_dy};
+    return state_xfer(h, src, true, hipMemcpyHostToDevice);
+}
+
+int m70h_get_state(m70h *h, double *Zs, double *D, double *Qx, double *Qy, double *U, double *V, double *U_c,
+                   double *V_c, double *dQx_dx, double *dQy_dy)
+{
+    if (!h) return M70H_EINVAL;
+    double *dst[10] = {Zs, D, Qx, Qy, U, V, U_c, V_c, dQx_dx, dQy_dy};
+    return state_xfer(h, dst, false, hipMemcpyDeviceToHost);
+}
+
+int m70h_set_state_dev(m70h *h, const double *Zs, const double *D, const double *Qx, const double *Qy,
+                       const double *U, const double *V, const double *U_c, const double *V_c, const double *dQx_dx,
+                       const double *dQy_dy)
+{
+    if (!h) return M70H_EINVAL;
+    double *src[10] = {(double *)Zs, (double *)D, (double *)Qx, (double *)Qy, (double *)U, (double *)V,
+                       (double *)U_c, (double *)V_c, (double *)dQx_dx, (double *)dQy_dy};
+    return state_xfer(h, src, true, hipMemcpyDeviceToDevice);
+}
+
+int m70h_get_state_dev(m70h *h, double *Zs, double *D, double *Qx, double *Qy, double *U, double *V, double *U_c,
+                       double *V_c, double *dQx_dx, double *dQy_dy)
+{
+    if (!h) return M70H_EINVAL;
+    double *dst[10] = {Zs, D, Qx, Qy, U, V, U_c, V_c, dQx_dx, dQy_dy};
+    return state_xfer(h, dst, false, hipMemcpyDeviceToDevice);
+}
+
+static int forcing_xfer(m70h *h, const double *const *src, hipMemcpyKind kind)
+{
+    for (int q = 0; q < 7; ++q)
+        if (!src[q]) return fail(h, M70H_EINVAL, "all seven forcing arrays are required");
+    long n[7];
+    for (auto &v : n) v = h->F;
+    int rc = copy_fields(h, h->fw_buf, src, n, 7, kind);
+    if (rc == M70H_OK) h->forcing_set = true;
+    return rc;
+}
+
+int m70h_set_forcing(m70h *h, const double *Sxx, const double *Sxy, const double *Syy, const double *Urms,
+                     const double *Dw, const double *D_r, const double *H)
+{
+    if (!h) return M70H_EINVAL;
+    const double *src[7] = {Sxx, Sxy, Syy, Urms, Dw, D_r, H};
+    return forcing_xfer(h, src, hipMemcpyHostToDevice);
+}
+
+int m70h_set_forcing_dev(m70h *h, const double *Sxx, const double *Sxy, const double *Syy, const double *Urms,
+                         const double *Dw, const double *D_r, const double *H)
+{
+    if (!h) return M70H_EINVAL;
+    const double *src[7] = {Sxx, Sxy, Syy, Urms, Dw, D_r, H};
+    return forcing_xfer(h, src, hipMemcpyDeviceToDevice);
+}
+
+int m70h_set_offshore_f0(m70h *h, double f0)
+{
+    if (!h) return M70H_EINVAL;
+    h->f0 = f0;
+    h->f0_series.clear();
+    h->f0_pos = 0;
+    return M70H_OK;
+}
+
+int m70h_set_offshore_f0_series(m70h *h, const double *f0, long n)
+{
+    if (!h) return M70H_EINVAL;
+    if (n < 0 || (n > 0 && !f0)) return fail(h, M70H_EINVAL, "bad f0 series");
+    h->f0_series.assign(f0, f0 + n);
+    h->f0_pos = 0;
+    return M70H_OK;
+}
+
+int m70h_bc_constants(const m70h *h, double *f1, double *f0, double *bcQy)
+{
+    if (!h) return M70H_EINVAL;
+    if (f1) *f1 = h->k.f1;
+    if (f0) *f0 = h->f0;
+    if (bcQy) *bcQy = h->k.bcQy;
+    return M70H_OK;
+}
+
+int m70h_run_async(m70h *h, int nIter)
+{
+    if (!h) return M70H_EINVAL;
+    if (nIter < 0) return fail(h, M70H_EINVAL, "nIter < 0");
+    if (!h->forcing_set) return fail(h, M70H_ESTATE, "forcing not set");
+    int rc = run_preloop(h);
+    if (rc) return rc;
+    for (int it = 0; it < nIter; ++it)
+        if ((rc = run_step(h))) return rc;
+    return M70H_OK;
+}
+
+int m70h_synchronize(m70h *h)
+{
+    if (!h) return M70H_EINVAL;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->prof) return collect_profile(h);
+    return M70H_OK;
+}
+
+int m70h_run(m70h *h, int nIter)
+{
+    int rc = m70h_run_async(h, nIter);
+    if (rc) return rc;
+    return m70h_synchronize(h);
+}
+
+int m70h_set_solver(m70h *h, int solver)
+{
+    if (!h) return M70H_EINVAL;
+    if (solver != M70H_SOLVER_EXACT) return fail(h, M70H_EINVAL, "unknown solver");
+    h->solver = solver;
+    return M70H_OK;
+}
+
+int m70h_check_finite(m70h *h, int *finite)
+{
+    if (!h) return M70H_EINVAL;
+    HIPCHK(h, hipMemsetAsync(h->flag, 0, sizeof(int), h->stream));
+    State &s = h->s;
+    const double *f[] = {s.Zs, s.D, s.Qx, s.Qy, s.U, s.V, s.Uc, s.Vc, s.dQx, s.dQy};
+    const long n[] = {h->F, h->F, h->Fp, h->Fp, h->Fp, h->Fp, h->F, h->F, h->F, h->F};
+    for (int q = 0; q < 10; ++q) HIPCHK(h, launch_nonfinite(h->stream, f[q], n[q], h->flag));
+    int v = 0;
+    HIPCHK(h, hipMemcpyAsync(&v, h->flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (finite) *finite = v ? 0 : 1;
+    return v ? fail(h, M70H_ENONFINITE, "non-finite value in the state") : M70H_OK;
+}
+
+int m70h_set_profiling(m70h *h, int enable)
+{
+    if (!h) return M70H_EINVAL;
+    int rc = collect_profile(h);
+    h->prof = enable != 0;
+    for (int q = 0; q < K_COUNT; ++q) { h->prof_ms[q] = 0; h->prof_n[q] = 0; }
+    return rc;
+}
+
+int m70h_profile(m70h *h, int kk, const char **name, double *ms, long *launches)
+{
+    if (!h) return M70H_EINVAL;
+    if (kk < 0 || kk >= K_COUNT) return M70H_EINVAL;
+    int rc = collect_profile(h);
+    if (rc) return rc;
+    if (name) *name = kKernelNames[kk];
+    if (ms) *ms = h->prof_ms[kk];
+    if (launches) *launches = h->prof_n[kk];
+    return M70H_OK;
+}
+
+int m70h_state_ptr(m70h *h, const char *name, double **dev)
+{
+    if (!h || !name || !dev) return M70H_EINVAL;
+    State &s = h->s;
+    struct { const char *n; double *p; } tab[] = {{"Zs", s.Zs}, {"D", s.D}, {"Qx", s.Qx}, {"Qy", s.Qy}, {"U", s.U},
+                                                  {"V", s.V}, {"U_c", s.Uc}, {"V_c", s.Vc}, {"dQx_dx", s.dQx},
+                                                  {"dQy_dy", s.dQy}};
+    for (auto &t : tab)
+        if (std::strcmp(t.n, name) == 0) { *dev = t.p; return M70H_OK; }
+    return fail(h, M70H_EINVAL, std::string("unknown field ") + name);
+}
+
+}  // extern "C"

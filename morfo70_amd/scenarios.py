@@ -1,0 +1,238 @@
+"""Seeded beach scenarios: grid, bathymetry, wave forcing and tide for the Hydro step.
+
+The reference reads its bathymetry from NetCDF (src/Grid.cpp:16-29,
+src/Bathymetry.cpp:13-31) and gets its forcing from the row-marching wave driver
+(src/Wave.cpp:80-172). Neither is on the Hydro hot path, so this module defines the
+inputs in closed form instead. The same arrays feed the HIP path and the CPU oracle.
+
+Conventions (those of the reference):
+  * x index 0 is offshore; x increases onshore. ``Z`` is the still-water depth of the
+    alongshore-uniform profile (Z > 0 in water, Z < 0 on land), ``h`` is the bed
+    perturbation (positive raises the bed), ``Zb = Z - h`` (include/Bathymetry.h:29-32).
+  * Arrays are (nx, ny) float64, C order: index i*ny + j, j alongshore and periodic.
+
+Physics choices for inputs the reference does not define:
+  * Chezy friction C maps to the constant drag model with cD0 = g / C**2
+    (include/OptionsParametersConstants.h:50-54 has no Chezy model).
+  * The tide enters through the offshore flux hook Qx0 = f1*Qx1 + f0
+    (include/Hydro.h:89-90) with f1 = 1 (ConstantZs) and
+    f0(t) = dx_x[0] * d(eta0)/dt, the long-wave boundary condition of the model's
+    equations document (eq. 2.93).
+  * Wave forcing: linear dispersion, Snell refraction and energy-flux shoaling from the
+    offshore row, depth-limited saturation H <= gamma_b * D (gamma_b = 0.475), radiation
+    stresses E*((1+cos^2)n - 1/2), E*n*sin*cos, E*((2-cos^2)n - 1/2) (the per-unit-energy
+    forms of src/WaveRow.cpp:200-205), breaking dissipation Dw = max(0, -d(E cg cos)/dx),
+    rollers off (D_r = 0) and a floor on Urms so that the drag term stays finite.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+G = 9.81          # include/OptionsParametersConstants.h:14
+RHO = 1025.0      # include/OptionsParametersConstants.h:16
+GAMMA_B = 0.475   # src/OptionsParametersConstants.cpp:48
+URMS_FLOOR = 1e-2  # m/s
+TIDE_PERIOD = 12.42 * 3600.0
+
+DRAG = {"constant": 0, "logdepth": 1, "manning": 2}
+VISC = {"constant": 0, "rollers": 1, "depth": 2, "hrms": 3}
+OFFSHORE_BC = {"sponge": 0, "constant_zs": 1}
+
+
+@dataclasses.dataclass
+class Physics:
+    """Hydro parameters (Parameters::Hydro::*, Parameters::Bottom::z0,
+    Parameters::Rollers::alpha_r and Options::Hydro::* of
+    include/OptionsParametersConstants.h:49-113)."""
+
+    drag_model: int = DRAG["constant"]
+    visc_model: int = VISC["hrms"]
+    offshore_bc: int = OFFSHORE_BC["sponge"]
+    dt: float = 0.5
+    Dmin: float = 0.01
+    alpha_f: float = 1.16
+    cD0: float = G / 65.0 ** 2
+    nu0: float = 1e-4
+    M: float = 1.0
+    ka: float = 0.0219
+    z0: float = 0.001
+    alpha_r: float = 1.0
+
+    def as_tuple(self):
+        return (self.drag_model, self.visc_model, self.offshore_bc, self.dt, self.Dmin,
+                self.alpha_f, self.cD0, self.nu0, self.M, self.ka, self.z0, self.alpha_r)
+
+
+@dataclasses.dataclass
+class Scenario:
+    name: str
+    nx: int
+    ny: int
+    dx: float
+    dy: float
+    x: np.ndarray          # (nx,) cross-shore coordinates
+    Z: np.ndarray          # (nx, ny) profile depth
+    h: np.ndarray          # (nx, ny) bed perturbation
+    zs0: float             # initial still-water level
+    physics: Physics
+    forcing: dict          # Sxx, Sxy, Syy, Urms, Dw, D_r, H -> (nx, ny)
+    steps: int
+    f0_series: np.ndarray | None = None   # per-step offshore flux offset (tide)
+    meta: dict = dataclasses.field(default_factory=dict)
+
+    @property
+    def cells(self) -> int:
+        return self.nx * self.ny
+
+    def initial_depth(self) -> np.ndarray:
+        """D = max(Z - h + zs0, Dmin), as src/Bathymetry.cpp:33-39."""
+        d = (self.Z - self.h) + self.zs0
+        return np.where(d < self.physics.Dmin, self.physics.Dmin, d)
+
+
+def _dispersion_k(omega: float, depth: np.ndarray) -> np.ndarray:
+    """Wavenumber from omega^2 = g k tanh(k h) by Newton iteration (h > 0)."""
+    depth = np.maximum(depth, 1e-3)
+    k0 = omega * omega / G
+    k = k0 / np.sqrt(np.tanh(k0 * depth))
+    for _ in range(12):
+        th = np.tanh(k * depth)
+        f = G * k * th - omega * omega
+        df = G * th + G * k * depth * (1.0 - th * th)
+        k = k - f / df
+    return k
+
+
+def wave_forcing(D: np.ndarray, x: np.ndarray, Hrms: float, T: float, theta_deg: float,
+                 Dmin: float) -> dict:
+    """Closed-form wave forcing on the water depth field D (nx, ny)."""
+    omega = 2.0 * math.pi / T
+    wet = D > Dmin + 1e-4
+    depth = np.where(wet, D, 1e-3)
+    k = _dispersion_k(omega, depth)
+    kh = k * depth
+    c = omega / k
+    n = 0.5 * (1.0 + 2.0 * kh / np.sinh(2.0 * kh))
+    cg = n * c
+    c0, cg0 = c[0:1, :], cg[0:1, :]
+    th0 = math.radians(theta_deg)
+    sin_t = np.clip(math.sin(th0) * c / c0, -0.999, 0.999)
+    cos_t = np.sqrt(1.0 - sin_t * sin_t)
+    Hs = Hrms * np.sqrt((cg0 * math.cos(th0)) / (cg * cos_t))
+    H = np.minimum(Hs, GAMMA_B * depth)
+    H = np.where(wet, H, 0.0)
+    E = 0.125 * RHO * G * H * H
+    Sxx = E * ((1.0 + cos_t * cos_t) * n - 0.5)
+    Syy = E * ((2.0 - cos_t * cos_t) * n - 0.5)
+    Sxy = E * (sin_t * cos_t * n)
+    Urms = math.pi * H / (T * np.sinh(kh)) / math.sqrt(2.0)
+    Urms = np.maximum(np.where(wet, Urms, 0.0), URMS_FLOOR)
+    Ef = E * cg * cos_t
+    dEf = np.gradient(Ef, x, axis=0)
+    Dw = np.where(wet & (Hs > GAMMA_B * depth), np.maximum(-dEf, 0.0), 0.0)
+    D_r = np.zeros_like(D)
+    f = {"Sxx": Sxx, "Sxy": Sxy, "Syy": Syy, "Urms": Urms, "Dw": Dw, "D_r": D_r, "H": H}
+    return {kk: np.ascontiguousarray(v, dtype=np.float64) for kk, v in f.items()}
+
+
+def _bar_beach(nx, ny, dx, dy, slope, bar_amp, bar_mod, bar_centre, bar_width, wavelength,
+               dry_fraction=0.08, phase=0.0):
+    x = np.arange(nx, dtype=np.float64) * dx
+    y = np.arange(ny, dtype=np.float64) * dy
+    x_shore = x[-1] - dry_fraction * x[-1]
+    s = x_shore - x                       # distance seaward of the shoreline
+    Z = np.repeat((slope * s)[:, None], ny, axis=1)
+    amp = bar_amp + bar_mod * np.sin(2.0 * math.pi * y / wavelength + phase)
+    h = np.exp(-((s - bar_centre) / bar_width) ** 2)[:, None] * amp[None, :]
+    return x, np.ascontiguousarray(Z), np.ascontiguousarray(h)
+
+
+def _periodic_wavelength(Ly: float, lam: float) -> float:
+    m = max(1, int(round(Ly / lam)))
+    return Ly / m
+
+
+def tide_f0_series(steps: int, dt: float, dx0: float, amp: float = 1.0,
+                   period: float = TIDE_PERIOD, t0: float = 0.0) -> np.ndarray:
+    """f0 for step k: dx_x[0] * d/dt [amp*sin(2 pi t/period)] at t = t0 + k*dt."""
+    w = 2.0 * math.pi / period
+    t = t0 + np.arange(steps, dtype=np.float64) * dt
+    return dx0 * amp * w * np.cos(w * t)
+
+
+def config1(steps: int = 1000) -> Scenario:
+    """Plain Hydro ADI, 120 x 200 cells, dx=dy=5 m: planar beach slope 0.02 plus a
+    Gaussian bar (1.0 m, 150 m from shore, width 30 m) modulated alongshore by a 0.3 m
+    sinusoid of 250 m wavelength; Hrms 1 m, T 8 s, 10 deg; Chezy 65; dt 0.5 s; sponge."""
+    nx, ny, dx, dy = 120, 200, 5.0, 5.0
+    x, Z, h = _bar_beach(nx, ny, dx, dy, 0.02, 1.0, 0.3, 150.0, 30.0, 250.0)
+    ph = Physics(offshore_bc=OFFSHORE_BC["sponge"], dt=0.5)
+    sc = Scenario("beach_120x200", nx, ny, dx, dy, x, Z, h, 0.0, ph, {}, steps)
+    sc.forcing = wave_forcing(sc.initial_depth(), x, 1.0, 8.0, 10.0, ph.Dmin)
+    sc.meta = dict(slope=0.02, bar_amp=1.0, wavelength=250.0, Hrms=1.0, T=8.0, theta=10.0)
+    return sc
+
+
+def seeded_beach(nx: int, ny: int, dx: float, dy: float, seed: int, steps: int,
+                 tide: bool = False, dt: float = 0.5, random_waves: bool = False,
+                 name: str | None = None) -> Scenario:
+    """Bar-trough beach drawn from ``seed``: slope U[0.01,0.04], bar amplitude
+    U[0.5,1.5] m, alongshore wavelength U[100,400] m (rounded to divide the domain).
+    With ``random_waves`` the forcing is drawn too: Hrms U[0.5,3] m, T U[6,14] s,
+    angle U[-30,30] deg; otherwise Hrms 1 m, T 8 s, 10 deg. With ``tide`` the offshore
+    boundary is ConstantZs and carries a 1 m, 12.42 h tide through f0."""
+    rng = np.random.default_rng(seed)
+    slope = rng.uniform(0.01, 0.04)
+    amp = rng.uniform(0.5, 1.5)
+    lam = _periodic_wavelength(ny * dy, rng.uniform(100.0, 400.0))
+    phase = rng.uniform(0.0, 2.0 * math.pi)
+    # bar crest in water about 2.5x the largest bar height deep
+    centre = 2.5 * (amp * 1.3) / slope
+    x, Z, h = _bar_beach(nx, ny, dx, dy, slope, amp, 0.3 * amp, centre, 30.0, lam, phase=phase)
+    if random_waves:
+        Hrms, T, theta = rng.uniform(0.5, 3.0), rng.uniform(6.0, 14.0), rng.uniform(-30.0, 30.0)
+    else:
+        Hrms, T, theta = 1.0, 8.0, 10.0
+    bc = OFFSHORE_BC["constant_zs"] if tide else OFFSHORE_BC["sponge"]
+    ph = Physics(offshore_bc=bc, dt=dt)
+    sc = Scenario(name or f"beach_{nx}x{ny}_s{seed}", nx, ny, dx, dy, x, Z, h, 0.0, ph, {}, steps)
+    sc.forcing = wave_forcing(sc.initial_depth(), x, Hrms, T, theta, ph.Dmin)
+    if tide:
+        sc.f0_series = tide_f0_series(steps, dt, dx)  # uniform grid: dx_x[0] = dx
+    sc.meta = dict(seed=seed, slope=slope, bar_amp=amp, wavelength=lam, Hrms=Hrms, T=T,
+                   theta=theta, tide=tide)
+    return sc
+
+
+def config2(steps: int = 2000, seed: int = 2, nx: int = 2048, ny: int = 4096) -> Scenario:
+    """Plain Hydro ADI with tide: 2048 x 4096 cells, dx=dy=2 m, seeded bathymetry."""
+    return seeded_beach(nx, ny, 2.0, 2.0, seed, steps, tide=True,
+                        name=f"beach_tide_{nx}x{ny}_s{seed}")
+
+
+def batch_scenarios(count: int = 64, seed: int = 3, steps: int = 500,
+                    nx_range=(128, 1024), ny_range=(256, 2048), dx: float = 2.0):
+    """Uneven scenarios for the batched form: nx ~ U[nx_range], ny ~ U[ny_range]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        nx = int(rng.integers(nx_range[0], nx_range[1] + 1))
+        ny = int(rng.integers(ny_range[0], ny_range[1] + 1))
+        out.append(seeded_beach(nx, ny, dx, dx, seed * 1000 + k, steps, random_waves=True,
+                                name=f"batch{k}_{nx}x{ny}"))
+    return out
+
+
+def lake_at_rest(nx=24, ny=16, dx=5.0, dy=5.0, slope=0.02) -> Scenario:
+    """Still water on a bar beach with zero forcing: an exact fixed point of the step."""
+    x, Z, h = _bar_beach(nx, ny, dx, dy, slope, 0.5, 0.1, 40.0, 15.0, ny * dy / 2)
+    ph = Physics(dt=0.5)
+    sc = Scenario("lake", nx, ny, dx, dy, x, Z, h, 0.0, ph, {}, 10)
+    D = sc.initial_depth()
+    z = np.zeros_like(D)
+    sc.forcing = {"Sxx": z.copy(), "Sxy": z.copy(), "Syy": z.copy(),
+                  "Urms": np.full_like(D, 0.1), "Dw": z.copy(), "D_r": z.copy(), "H": z.copy()}
+    return sc

@@ -1,0 +1,194 @@
+"""ctypes binding of the CPU oracle (TEST INFRASTRUCTURE).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+package; the product (morfo70_amd) never does. See m70_oracle.h for what the oracle
+restates.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+REF_LIB_PATH = os.path.join(_HERE, "_ref", "libm70ref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("drag_model", ctypes.c_int), ("visc_model", ctypes.c_int),
+                ("offshore_bc", ctypes.c_int), ("dt", ctypes.c_double), ("Dmin", ctypes.c_double),
+                ("alpha_f", ctypes.c_double), ("cD0", ctypes.c_double), ("nu0", ctypes.c_double),
+                ("M", ctypes.c_double), ("ka", ctypes.c_double), ("z0", ctypes.c_double),
+                ("alpha_r", ctypes.c_double)]
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double,
+                                 ctypes.POINTER(_Params), _dp, _dp, ctypes.c_double]
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_forcing.argtypes = [ctypes.c_void_p] + [_dp] * 7
+        L.orc_set_f0.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_set_f0_series.argtypes = [ctypes.c_void_p, _dp, ctypes.c_long]
+        L.orc_bc_constants.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_get_state.argtypes = [ctypes.c_void_p] + [_dp] * 10
+        L.orc_set_state.argtypes = [ctypes.c_void_p] + [_dp] * 10
+        L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_run.restype = ctypes.c_int
+        L.orc_get_field.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _dp]
+        L.orc_get_field.restype = ctypes.c_long
+        L.orc_grid_metrics.argtypes = [ctypes.c_int, _dp, _dp]
+        L.orc_tridiag.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]
+        L.orc_tridiag_periodic.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int, _dp]
+        L.orc_wet_rule.argtypes = [ctypes.c_double] * 6 + [ctypes.c_double]
+        L.orc_wet_rule.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def ref_lib():
+    """The reference's own Math.cpp/MeshOperations.h built by oracle/Makefile, or None."""
+    if not os.path.exists(REF_LIB_PATH):
+        return None
+    L = ctypes.CDLL(REF_LIB_PATH)
+    L.ref_tridiag.argtypes = [_dp] * 6 + [ctypes.c_int, ctypes.c_int]
+    L.ref_tridiag_periodic.argtypes = [_dp] * 4 + [ctypes.c_int, _dp]
+    L.ref_tridiag_periodic_batched.argtypes = [_dp] * 5 + [ctypes.c_int, ctypes.c_int]
+    L.ref_c2x.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int]
+    L.ref_x2c.argtypes = [_dp] * 4 + [ctypes.c_int, ctypes.c_int]
+    return L
+
+
+def ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+STATE_FIELDS = ("Zs", "D", "Qx", "Qy", "U", "V", "U_c", "V_c", "dQx_dx", "dQy_dy")
+PLUS_FIELDS = ("Qx", "Qy", "U", "V")
+FORCING_FIELDS = ("Sxx", "Sxy", "Syy", "Urms", "Dw", "D_r", "H")
+
+
+def params_struct(physics) -> _Params:
+    return _Params(*physics.as_tuple())
+
+
+def grid_metrics(x: np.ndarray) -> dict:
+    nx = x.shape[0]
+    out = np.zeros(20 * nx, dtype=np.float64)
+    lib().orc_grid_metrics(nx, ptr(np.ascontiguousarray(x, dtype=np.float64)), ptr(out))
+    names = [("dx", nx - 1), ("dx_x", nx)] + [(n, nx - 1) for n in (
+        "difx_x_up", "difx_x_d", "difx_x_low", "dif2x_x_up", "dif2x_x_d", "dif2x_x_low")] + \
+        [(n, nx) for n in ("difx_c_up", "difx_c_d", "difx_c_low", "dif2x_c_up", "dif2x_c_d",
+                           "dif2x_c_low", "x2c_up", "x2c_d")]
+    res, off = {}, 0
+    for n, ln in names:
+        res[n] = out[off:off + ln].copy()
+        off += ln
+    return res
+
+
+class OracleHydro:
+    """The reference Hydro object restated on the CPU (see m70_oracle.h)."""
+
+    def __init__(self, sc, threads: int = 0):
+        L = lib()
+        if threads:
+            L.orc_set_threads(threads)
+        self.nx, self.ny = sc.nx, sc.ny
+        self._p = params_struct(sc.physics)
+        x = np.ascontiguousarray(sc.x, dtype=np.float64)
+        Z = np.ascontiguousarray(sc.Z, dtype=np.float64)
+        h = np.ascontiguousarray(sc.h, dtype=np.float64)
+        self.h = L.orc_create(sc.nx, sc.ny, ptr(x), sc.dy, ctypes.byref(self._p), ptr(Z), ptr(h),
+                              sc.zs0)
+        if not self.h:
+            raise ValueError("orc_create failed")
+        self.set_forcing(sc.forcing)
+        if sc.f0_series is not None:
+            self.set_f0_series(sc.f0_series)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def set_forcing(self, f: dict):
+        arrs = [np.ascontiguousarray(f[k], dtype=np.float64) for k in FORCING_FIELDS]
+        lib().orc_set_forcing(self.h, *[ptr(a) for a in arrs])
+
+    def set_f0(self, f0: float):
+        lib().orc_set_f0(self.h, f0)
+
+    def set_f0_series(self, s):
+        s = np.ascontiguousarray(s, dtype=np.float64)
+        self._f0 = s
+        lib().orc_set_f0_series(self.h, ptr(s), s.size)
+
+    def bc_constants(self):
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        lib().orc_bc_constants(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return a.value, b.value, c.value
+
+    def run(self, n: int):
+        lib().orc_run(self.h, int(n))
+
+    def _shape(self, name):
+        return (self.nx + 1, self.ny) if name in PLUS_FIELDS else (self.nx, self.ny)
+
+    def get_state(self) -> dict:
+        out = {k: np.zeros(self._shape(k)) for k in STATE_FIELDS}
+        lib().orc_get_state(self.h, *[ptr(out[k]) for k in STATE_FIELDS])
+        return out
+
+    def set_state(self, st: dict):
+        arrs = []
+        for k in STATE_FIELDS:
+            a = st.get(k)
+            arrs.append(None if a is None else np.ascontiguousarray(a, dtype=np.float64))
+        lib().orc_set_state(self.h, *[ptr(a) if a is not None else None for a in arrs])
+
+    def field(self, name: str) -> np.ndarray:
+        n = lib().orc_get_field(self.h, name.encode(), None)
+        if n < 0:
+            raise KeyError(name)
+        out = np.zeros(n)
+        lib().orc_get_field(self.h, name.encode(), ptr(out))
+        return out.reshape(-1, self.ny)
+
+
+def tridiag(a, b, c, f):
+    """Batched Thomas (n, m) arrays as src/Math.cpp:15-65: a=diag, b=low, c=up."""
+    a, b, c, f = (np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, c, f))
+    n, m = a.shape
+    Y = np.zeros((n, m))
+    V = np.zeros((n, m))
+    lib().orc_tridiag(ptr(Y), ptr(V), ptr(a), ptr(b), ptr(c), ptr(f), n, m)
+    return Y
+
+
+def tridiag_periodic(a, b, c, f):
+    a, b, c, f = (np.ascontiguousarray(v, dtype=np.float64) for v in (a, b, c, f))
+    y = np.zeros(a.shape[0])
+    lib().orc_tridiag_periodic(ptr(a), ptr(b), ptr(c), ptr(f), a.shape[0], ptr(y))
+    return y
+
+
+def wet_rule(d, dup, zs, zsup, s, sup, Dmin):
+    return lib().orc_wet_rule(d, dup, zs, zsup, s, sup, Dmin)
