@@ -50,8 +50,11 @@ def parse():
     ap.add_argument("--ny", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=120)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--solver", choices=["partition", "exact"], default="partition")
+    ap.add_argument("--tile-cols", type=int, default=0)
+    ap.add_argument("--tile-rows", type=int, default=0)
     return ap.parse_args()
 
 
@@ -88,7 +91,10 @@ def main():
                         name=f"beach_tide_{args.nx}x{args.ny}_s{args.seed}")
     dev = torch.device("cuda:0")
     torch.cuda.init()
-    g = from_scenario(sc)
+    from morfo70_amd import _lib
+    solver = _lib.SOLVER_PARTITION if args.solver == "partition" else _lib.SOLVER_EXACT
+    g = from_scenario(sc, solver=solver)
+    g._chk(g._L.m70h_set_solver_tiles(g.handle, args.tile_cols, args.tile_rows))
     stream = torch.cuda.Stream(device=dev)          # default priority
     g.set_stream(stream.cuda_stream)
     g.set_forcing()
@@ -132,7 +138,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded bar-trough beach, closed-form wave forcing, tide)",
         "config": {"workload": sc.name, "nx": sc.nx, "ny": sc.ny, "dx": 2.0, "dy": 2.0,
-                   "dt": sc.physics.dt, "tide": True, "solver": "exact",
+                   "dt": sc.physics.dt, "tide": True, "solver": args.solver,
+                   "tiles": [args.tile_cols, args.tile_rows],
                    "timed": f"one Run({args.steps}) incl. per-Run pre-loop"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -46,6 +46,11 @@ extern "C" {
  * El-Mikkawy recurrences (src/Math.cpp:15-118) one line per lane, in the reference's
  * operation order: results are bitwise equal to the reference's serial CPU step. */
 #define M70H_SOLVER_EXACT 0
+/* PARTITION splits every line over the threads of a workgroup: two-sided elimination of
+ * per-thread segments in registers, cyclic reduction of the segment-boundary system in
+ * LDS, back-substitution. Same systems, different rounding: results agree with EXACT to
+ * a few ulps per solve. Line lengths up to 4096. */
+#define M70H_SOLVER_PARTITION 1
 
 /* Parameters read by Hydro at call time in the reference (Parameters::Hydro::dt, Dmin,
  * alpha_f, cD0, nu0, M, ka; Parameters::Bottom::z0; Parameters::Rollers::alpha_r;
@@ -129,6 +134,24 @@ int m70h_synchronize(m70h *h);
 
 /* Line-solver family, M70H_SOLVER_* (default EXACT). */
 int m70h_set_solver(m70h *h, int solver);
+
+/* PARTITION solver tiles: unknowns per thread (8 or 16; 0 = default) for the lines along
+ * x (columns) and along y (rows). This is the knob the autotuner sets. */
+int m70h_set_solver_tiles(m70h *h, int tile_cols, int tile_rows);
+
+/* Batched tridiagonal solve on device arrays, the counterpart of Math::TridiagonalSolver
+ * (src/Math.cpp:15-65, lines strided as in Hydro's x-sweeps) and
+ * Math::TridiagonalSolverPeriodic (src/Math.cpp:68-186, contiguous lines as in Hydro's
+ * y-sweeps). Line L, element k lives at L*ld + k when lines_are_rows, else at k*ld + L.
+ * a = diagonal, b = lower (b of element 0 is the periodic corner), c = upper (c of element
+ * n-1 is the corner), f = right-hand side; y receives the solution with stride ldy.
+ * EXACT supports rows+periodic (scratch >= 3*n*roundup(nlines,64) doubles) and
+ * columns+non-periodic (ldy == ld, scratch >= n*ld doubles); PARTITION supports rows
+ * (periodic or not) and columns (non-periodic), n <= 4096, no scratch. Enqueued on
+ * hip_stream (a hipStream_t); returns without waiting. */
+int m70h_tridiag_solve(void *hip_stream, int solver, int lines_are_rows, int periodic, int n, int nlines, long ld,
+                       const double *a_dev, const double *b_dev, const double *c_dev, const double *f_dev,
+                       double *y_dev, long ldy, double *scratch_dev, long scratch_len, int tile);
 
 /* Scans the state for NaN/Inf on the device; *finite = 1 if none. Returns
  * M70H_ENONFINITE when a non-finite value is found. */

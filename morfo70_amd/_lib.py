@@ -10,6 +10,7 @@ LIB_PATH = os.path.join(HERE, "libm70h.so")
 
 M70H_OK, M70H_EINVAL, M70H_EHIP, M70H_ENONFINITE, M70H_ESTATE = 0, 1, 2, 3, 4
 SOLVER_EXACT = 0
+SOLVER_PARTITION = 1
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
@@ -36,6 +37,10 @@ SIGNATURES = {
     "m70h_run_async": (ctypes.c_int, [_vp, ctypes.c_int]),
     "m70h_synchronize": (ctypes.c_int, [_vp]),
     "m70h_set_solver": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "m70h_set_solver_tiles": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "m70h_tridiag_solve": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_long] + [_vp] * 5 +
+                           [ctypes.c_long, _vp, ctypes.c_long, ctypes.c_int]),
     "m70h_check_finite": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "m70h_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
     "m70h_profile": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _dp,
@@ -65,6 +70,13 @@ def load(path: str | None = None):
     if _lib is not None and path is None:
         return _lib
     p = path or LIB_PATH
+    # torch's wheel ships its own libamdhip64.so.7 with the same SONAME as ROCm's; the first
+    # copy loaded serves the whole process. Load torch's first (when installed) so that a
+    # process using both torch and this library runs on one HIP runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(p):
         raise RuntimeError(f"{p} is missing: build it with `python -m morfo70_amd.build` "
                            "(the Hydro step has no CPU fallback)")

@@ -29,4 +29,10 @@ int periodic_rows_padded_lines(int nL);
 hipError_t launch_periodic_rows(hipStream_t st, const double *A, const double *B, const double *C, const double *F,
                                 long ld, double *Y, long ldy, double *Sr, double *Sv, double *Sk, int nL, int n);
 
+bool partition_supported(int n);
+hipError_t launch_partition_cols(hipStream_t st, const double *A, const double *B, const double *C, const double *F,
+                                 double *Y, int n, int m, long ld, int tile);
+hipError_t launch_partition_rows(hipStream_t st, const double *A, const double *B, const double *C, const double *F,
+                                 long ld, double *Y, long ldy, int nL, int n, bool periodic, int tile);
+
 }  // namespace m70

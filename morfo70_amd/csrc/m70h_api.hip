@@ -40,6 +40,7 @@ struct m70h {
     int device = 0;
     hipStream_t own_stream = nullptr, stream = nullptr;
     int solver = M70H_SOLVER_EXACT;
+    int tile_cols = 0, tile_rows = 0;   // partition-solver unknowns per thread (0 = default)
     std::string err;
     std::vector<void *> allocs;
     double *metric_buf = nullptr;
@@ -167,13 +168,18 @@ int run_step(m70h *h)
     hipStream_t st = h->stream;
     // Qx predictor: assembly + Thomas along x (src/HydroQx.cpp:29-51)
     if ((rc = timed(h, K_QXX_ASM, [&] { return launch_qxx_assemble(st, k, h->m, s, h->rf, w, f0); }))) return rc;
-    if ((rc = timed(h, K_THOMAS_QX,
-                    [&] { return launch_thomas_cols(st, w.a, w.b, w.c, w.f, s.Qx + ny, h->sv, nx - 1, ny, ny); })))
+    const bool fast = h->solver == M70H_SOLVER_PARTITION;
+    if ((rc = timed(h, K_THOMAS_QX, [&] {
+             return fast ? launch_partition_cols(st, w.a, w.b, w.c, w.f, s.Qx + ny, nx - 1, ny, ny, h->tile_cols)
+                         : launch_thomas_cols(st, w.a, w.b, w.c, w.f, s.Qx + ny, h->sv, nx - 1, ny, ny);
+         })))
         return rc;
     // Qx projector: assembly + periodic solve along y per row (src/HydroQx.cpp:59-70)
     if ((rc = timed(h, K_QXY_ASM, [&] { return launch_qxy_assemble(st, k, s, h->rf, w); }))) return rc;
     if ((rc = timed(h, K_PERIODIC_QX, [&] {
-             return launch_periodic_rows(st, w.a, w.b, w.c, w.f, ny, s.Qx + ny, ny, h->sr, h->sv, h->sk, nx - 1, ny);
+             return fast ? launch_partition_rows(st, w.a, w.b, w.c, w.f, ny, s.Qx + ny, ny, nx - 1, ny, true, h->tile_rows)
+                         : launch_periodic_rows(st, w.a, w.b, w.c, w.f, ny, s.Qx + ny, ny, h->sr, h->sv, h->sk,
+                                                nx - 1, ny);
          })))
         return rc;
     // BC, dQx_dx, guess, U, U_c (src/HydroQx.cpp:76-92)
@@ -181,13 +187,17 @@ int run_step(m70h *h)
     // Qy predictor (src/HydroQy.cpp:26-42)
     if ((rc = timed(h, K_QYY_ASM, [&] { return launch_qyy_assemble(st, k, h->m, s, h->rf, w); }))) return rc;
     if ((rc = timed(h, K_PERIODIC_QY, [&] {
-             return launch_periodic_rows(st, w.a, w.b, w.c, w.f, ny, s.Qy + ny, ny, h->sr, h->sv, h->sk, nx - 1, ny);
+             return fast ? launch_partition_rows(st, w.a, w.b, w.c, w.f, ny, s.Qy + ny, ny, nx - 1, ny, true, h->tile_rows)
+                         : launch_periodic_rows(st, w.a, w.b, w.c, w.f, ny, s.Qy + ny, ny, h->sr, h->sv, h->sk,
+                                                nx - 1, ny);
          })))
         return rc;
     // Qy projector (src/HydroQy.cpp:52-67)
     if ((rc = timed(h, K_QYX_ASM, [&] { return launch_qyx_assemble(st, k, h->m, s, h->rf, w); }))) return rc;
-    if ((rc = timed(h, K_THOMAS_QY,
-                    [&] { return launch_thomas_cols(st, w.a, w.b, w.c, w.f, s.Qy + ny, h->sv, nx - 1, ny, ny); })))
+    if ((rc = timed(h, K_THOMAS_QY, [&] {
+             return fast ? launch_partition_cols(st, w.a, w.b, w.c, w.f, s.Qy + ny, nx - 1, ny, ny, h->tile_cols)
+                         : launch_thomas_cols(st, w.a, w.b, w.c, w.f, s.Qy + ny, h->sv, nx - 1, ny, ny);
+         })))
         return rc;
     // BC, dQy_dy, Zs, D, V, V_c (src/HydroQy.cpp:73-84)
     if ((rc = timed(h, K_QY_TAIL, [&] { return launch_qy_tail(st, k, s); }))) return rc;
@@ -558,8 +568,56 @@ int m70h_run(m70h *h, int nIter)
 int m70h_set_solver(m70h *h, int solver)
 {
     if (!h) return M70H_EINVAL;
-    if (solver != M70H_SOLVER_EXACT) return fail(h, M70H_EINVAL, "unknown solver");
+    if (solver != M70H_SOLVER_EXACT && solver != M70H_SOLVER_PARTITION) return fail(h, M70H_EINVAL, "unknown solver");
+    if (solver == M70H_SOLVER_PARTITION && (!partition_supported(h->nx - 1) || !partition_supported(h->ny)))
+        return fail(h, M70H_EINVAL, "partition solver supports line lengths up to 4096");
     h->solver = solver;
+    return M70H_OK;
+}
+
+int m70h_set_solver_tiles(m70h *h, int tile_cols, int tile_rows)
+{
+    if (!h) return M70H_EINVAL;
+    auto ok = [](int t) { return t == 0 || t == 8 || t == 16; };
+    if (!ok(tile_cols) || !ok(tile_rows)) return fail(h, M70H_EINVAL, "tiles must be 0, 8 or 16");
+    h->tile_cols = tile_cols;
+    h->tile_rows = tile_rows;
+    return M70H_OK;
+}
+
+int m70h_tridiag_solve(void *hip_stream, int solver, int lines_are_rows, int periodic, int n, int nlines, long ld,
+                       const double *a_dev, const double *b_dev, const double *c_dev, const double *f_dev,
+                       double *y_dev, long ldy, double *scratch_dev, long scratch_len, int tile)
+{
+    hipStream_t st = (hipStream_t)hip_stream;
+    if (n < 3 || nlines < 1 || !a_dev || !b_dev || !c_dev || !f_dev || !y_dev) return M70H_EINVAL;
+    hipError_t e;
+    if (solver == M70H_SOLVER_PARTITION) {
+        if (!partition_supported(n)) return M70H_EINVAL;
+        e = lines_are_rows ? launch_partition_rows(st, a_dev, b_dev, c_dev, f_dev, ld, y_dev, ldy, nlines, n,
+                                                   periodic != 0, tile)
+                           : (periodic ? hipErrorInvalidValue
+                                       : launch_partition_cols(st, a_dev, b_dev, c_dev, f_dev, y_dev, n, nlines, ld,
+                                                               tile));
+    } else if (solver == M70H_SOLVER_EXACT) {
+        if (lines_are_rows && periodic) {
+            const long nLp = periodic_rows_padded_lines(nlines);
+            if (!scratch_dev || scratch_len < 3 * (long)n * nLp) return M70H_EINVAL;
+            e = launch_periodic_rows(st, a_dev, b_dev, c_dev, f_dev, ld, y_dev, ldy, scratch_dev,
+                                     scratch_dev + (long)n * nLp, scratch_dev + 2L * n * nLp, nlines, n);
+        } else if (!lines_are_rows && !periodic) {
+            if (ldy != ld || !scratch_dev || scratch_len < (long)n * ld) return M70H_EINVAL;
+            e = launch_thomas_cols(st, a_dev, b_dev, c_dev, f_dev, y_dev, scratch_dev, n, nlines, ld);
+        } else {
+            return M70H_EINVAL;
+        }
+    } else {
+        return M70H_EINVAL;
+    }
+    if (e != hipSuccess) {
+        g_create_error = std::string("m70h_tridiag_solve: ") + hipGetErrorString(e);
+        return e == hipErrorInvalidValue ? M70H_EINVAL : M70H_EHIP;
+    }
     return M70H_OK;
 }
 

@@ -21,12 +21,12 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-10
 
 
-def run_both(sc, steps, runs=1, oracle=None):
+def run_both(sc, steps, runs=1, solver=0):
     import oracle as orc
     from morfo70_amd.hydro import from_scenario
     o = orc.OracleHydro(sc)
-    g = from_scenario(sc)
-    print(f"[parity] {sc.name} nx={sc.nx} ny={sc.ny} steps={steps}x{runs}", flush=True)
+    g = from_scenario(sc, solver=solver)
+    print(f"[parity] {sc.name} nx={sc.nx} ny={sc.ny} steps={steps}x{runs} solver={solver}", flush=True)
     for _ in range(runs):
         o.run(steps)
         g.Run(steps)
@@ -91,6 +91,24 @@ def test_config1_1000_steps():
     print({k: (v["rel"], v["mismatch"]) for k, v in res.items()})
 
 
+@pytest.mark.parametrize("case", ["models", "shapes", "tide", "config1"])
+def test_partition_solver_parity(case):
+    """The PARTITION line solvers against the oracle at the 1e-10 bar."""
+    if case == "models":
+        ph = S.Physics(drag_model=S.DRAG["logdepth"], visc_model=S.VISC["hrms"], dt=0.25)
+        sc, steps, runs = nonuniform_beach(physics=ph, seed=11), 25, 2
+    elif case == "shapes":
+        sc, steps, runs = nonuniform_beach(nx=67, ny=70, seed=4), 20, 1
+    elif case == "tide":
+        ph = S.Physics(offshore_bc=S.OFFSHORE_BC["constant_zs"], dt=0.25)
+        sc, steps, runs = nonuniform_beach(physics=ph, tide=True, steps=40, seed=6), 20, 2
+    else:
+        sc, steps, runs = S.config1(steps=1000), 1000, 1
+    got, ref = run_both(sc, steps, runs=runs, solver=1)
+    res = assert_parity(got, ref, exact=False)
+    print({k: v["rel"] for k, v in res.items()})
+
+
 def test_lake_at_rest_matches_oracle_and_stays_at_rest():
     """Still water, zero forcing: Zs = D - Z + h is zero only to rounding, so the step
     moves it at the 1e-16 level; the GPU must match the oracle bit for bit and the
@@ -108,7 +126,7 @@ def test_state_roundtrip_and_finite_check():
     from morfo70_amd.hydro import from_scenario
     g = from_scenario(sc)
     g.Run(3)
-    st = g.state()
+    st = {k: v.copy() for k, v in g.state().items()}
     assert g.check_finite()
     g.Qx[5, 3] = np.nan
     g.push()
