@@ -38,6 +38,9 @@ KERNEL_BYTES_PER_CELL = {
     "qyx_assemble": 8 * 11 + 1 + 8 * 4,
     "thomas_qy": 8 * 5,
     "qy_tail": 8 * 5 + 8 * 5,               # Qy,dQx,D,Zs in; dQy,Zs,D,V,Vc out
+    # fused sweeps: the assembly inputs plus the solution (and side outputs) only
+    "fused_qxy": 8 * 8 + 1 + 8,             # D(2),Vc(2),Qx*,nuD_x,dnuDx_dy,drag in, mask in; Qx out
+    "fused_qyy": 8 * 17 + 8 * 2 + 1,        # the qyy_assemble inputs; Qy*, drag out; mask out
 }
 
 
@@ -52,7 +55,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=120)
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--solver", choices=["partition", "exact"], default="partition")
+    ap.add_argument("--solver", choices=["fused", "partition", "exact"], default="fused")
     ap.add_argument("--tile-cols", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
     return ap.parse_args()
@@ -92,7 +95,8 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.init()
     from morfo70_amd import _lib
-    solver = _lib.SOLVER_PARTITION if args.solver == "partition" else _lib.SOLVER_EXACT
+    solver = {"fused": _lib.SOLVER_FUSED, "partition": _lib.SOLVER_PARTITION,
+              "exact": _lib.SOLVER_EXACT}[args.solver]
     g = from_scenario(sc, solver=solver)
     g._chk(g._L.m70h_set_solver_tiles(g.handle, args.tile_cols, args.tile_rows))
     stream = torch.cuda.Stream(device=dev)          # default priority

@@ -51,6 +51,10 @@ extern "C" {
  * LDS, back-substitution. Same systems, different rounding: results agree with EXACT to
  * a few ulps per solve. Line lengths up to 4096. */
 #define M70H_SOLVER_PARTITION 1
+/* FUSED is PARTITION with the y-sweeps (lines along rows) assembled and solved in one
+ * launch, the row's coefficients staged through LDS so they never reach HBM; bitwise
+ * equal to PARTITION. ny up to 4096. */
+#define M70H_SOLVER_FUSED 2
 
 /* Parameters read by Hydro at call time in the reference (Parameters::Hydro::dt, Dmin,
  * alpha_f, cD0, nu0, M, ka; Parameters::Bottom::z0; Parameters::Rollers::alpha_r;
@@ -83,6 +87,11 @@ void m70h_destroy(m70h *h);
 
 /* Text of the last error on this handle (or of the last failed m70h_create if h is NULL). */
 const char *m70h_last_error(const m70h *h);
+
+/* Replaces the reference's call-time reads of Parameters::Hydro::* and Options::Hydro::*
+ * (e.g. src/HydroQx.cpp:102): new parameters for the following runs. The offshore
+ * boundary constants keep the values fixed at construction (src/Hydro.cpp:44-60). */
+int m70h_set_params(m70h *h, const m70h_params *p);
 
 /* Use `hip_stream` (a hipStream_t) for all device work of this handle; NULL restores
  * the handle's own stream. */

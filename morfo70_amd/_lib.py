@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(HERE, "libm70h.so")
 M70H_OK, M70H_EINVAL, M70H_EHIP, M70H_ENONFINITE, M70H_ESTATE = 0, 1, 2, 3, 4
 SOLVER_EXACT = 0
 SOLVER_PARTITION = 1
+SOLVER_FUSED = 2
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
@@ -22,6 +23,7 @@ SIGNATURES = {
                                    ctypes.POINTER(_vp)]),
     "m70h_destroy": (None, [_vp]),
     "m70h_last_error": (ctypes.c_char_p, [_vp]),
+    "m70h_set_params": (ctypes.c_int, [_vp, _vp]),
     "m70h_set_stream": (ctypes.c_int, [_vp, _vp]),
     "m70h_init_from_bathymetry": (ctypes.c_int, [_vp, _dp, _dp, _dp]),
     "m70h_set_state": (ctypes.c_int, [_vp] + [_dp] * 10),

@@ -33,6 +33,9 @@ def hipcc() -> str:
     return "hipcc"
 
 
+FACADE = os.path.join(HERE, "host", "facade_run")
+
+
 def build(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
     tmp = LIB + ".tmp"
@@ -41,6 +44,13 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # C++ facade (the reference's Hydro call surface) and its driver, plain g++
+    host = os.path.join(HERE, "host")
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", FACADE, os.path.join(host, "facade_run.cpp"),
+           os.path.join(host, "Hydro.cpp"), "-L" + HERE, "-lm70h", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
     return LIB
 
 

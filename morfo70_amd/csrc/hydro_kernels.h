@@ -35,4 +35,9 @@ hipError_t launch_partition_cols(hipStream_t st, const double *A, const double *
 hipError_t launch_partition_rows(hipStream_t st, const double *A, const double *B, const double *C, const double *F,
                                  long ld, double *Y, long ldy, int nL, int n, bool periodic, int tile);
 
+bool rows_lds_supported(int n);
+hipError_t launch_lds_qxy(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w);
+hipError_t launch_lds_qyy(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
+                          const Work &w);
+
 }  // namespace m70

@@ -8,6 +8,7 @@
 // barrier between producer and consumer.
 #include "hydro_device.h"
 #include "hydro_kernels.h"
+#include "assembly.h"
 
 namespace m70 {
 
@@ -130,108 +131,23 @@ __global__ void __launch_bounds__(256) k_run_fields(Consts k, Metrics m, Forcing
 // :99-256 and setSolverQxx_BC (src/HydroBoundaryConditions.cpp:25-39). Rows 0..nx-2.
 // Writes a (d), b (low), c (up), f plus IsWater and DragTerm for the projector.
 // ------------------------------------------------------------------------------------
+
 __global__ void __launch_bounds__(256) k_qxx_assemble(Consts k, Metrics m, State s, RunFields rf,
                                                       Work w, double f0)
 {
     CELL_IJ(k.nx - 1)
-    const int nx = k.nx;
-    const int jm = wrap_dec(j, ny);
-    const double dt = k.dt;
-    const double *D = s.D, *Zs = s.Zs, *dQx = s.dQx;
-    const double Di = D[r + j], Di1 = D[r + ny + j];
-    // getD_x(i): Di row i+1 (:403-418)
-    const double Dx1 = 0.5 * (Di + Di1);
-    const double invDx1 = 1 / Dx1;
-    // setWaterNodes_x(i) (:231-256)
-    const int wi = wet_rule(Di, Di1, Zs[r + j], Zs[r + ny + j], dQx[r + j], dQx[r + ny + j], k.Dmask);
-    // Ui_j = C2X(V_c) (:23, src/HydroMeshOperations.cpp:15-28) and drag (:32,42)
-    const double vx = 0.5 * (s.Vc[r + j] + s.Vc[r + ny + j]);
-    const double u1 = s.U[r + ny + j];
-    const double drag = drag_term(rf.cD_x[r + j], rf.Urms_x[r + j], u1, vx, invDx1, k);
-    // setSolverQxx_f (:99-133)
-    double fv;
-    {
-        const double inv_dx = m.inv_dx[i];
-        const double *zs = Zs + r, *dqy_dy = s.dQy + r, *v = s.V + r + ny;
-        const double dzs_dx = inv_dx * (zs[ny + j] - zs[j]);
-        const double d2qy_dxy = inv_dx * (dqy_dy[ny + j] - dqy_dy[j]);
-        const double dV_dx = 0.5 * inv_dx * ((v[ny + j] + v[ny + jm]) - (v[j] + v[jm]));
-        const double d2V_dxy = inv_dx * k.inv_dy * ((v[ny + j] - v[ny + jm]) - (v[j] - v[jm]));
-        fv = wi * (rf.Fx[r + j] + s.Qx[r + ny + j] * k.inv_dt + G * Dx1 * (dt * d2qy_dxy - dzs_dx) +
-                   rf.dnuDx_dy[r + j] * dV_dx + rf.nuD_x[r + j] * d2V_dxy);
-    }
-    const double dnud_dx = rf.dnuD_dx[r + j], nud_x = rf.nuD_x[r + j];
-    // setSolverQxx_d (:135-162)
-    const int isl = !wi;
-    double dv = (k.inv_dt + drag + m.difx_x_d[i] * (u1 - 2 * dnud_dx * invDx1) -
-                 m.dif2x_x_d[i] * (G * Dx1 * dt + 2 * nud_x * invDx1)) * wi + isl;
-    // setSolverQxx0_low (:214-229) / setSolverQxx_low (:190-212)
-    double lv;
-    if (i == 0) {
-        const double Dx0 = 1.5 * Di - 0.5 * Di1;     // getD_x0 (:390-401)
-        const double invDx0 = 1.0 / Dx0;
-        lv = (m.difx_x_low[0] * (s.U[j] - 2 * dnud_dx * invDx0) -
-              m.dif2x_x_low[0] * (G * Dx1 * dt + 2 * nud_x * invDx0));
-    } else {
-        const double Dm = D[r - ny + j];
-        const double Dxi = 0.5 * (Dm + Di);
-        const double invDxi = 1 / Dxi;
-        const int wm = wet_rule(Dm, Di, Zs[r - ny + j], Zs[r + j], dQx[r - ny + j], dQx[r + j], k.Dmask);
-        const double ww = wi && wm;
-        lv = (m.difx_x_low[i] * (s.U[r + j] - 2 * dnud_dx * invDxi) -
-              m.dif2x_x_low[i] * (G * Dx1 * dt + 2 * nud_x * invDxi)) * ww;
-    }
-    // setSolverQxx_up (:164-188). Row nx-2's up is never read by the Thomas solve
-    // (src/Math.cpp:38-44 uses c rows 0..n-2), so it is left zero.
-    double uv = 0.0;
-    if (i < nx - 2) {
-        const double Dp = D[r + 2L * ny + j];
-        const double Dx2 = 0.5 * (Di1 + Dp);
-        const double invDx2 = 1 / Dx2;
-        const int wp = wet_rule(Di1, Dp, Zs[r + ny + j], Zs[r + 2L * ny + j], dQx[r + ny + j],
-                                dQx[r + 2L * ny + j], k.Dmask);
-        const double ww = wi && wp;
-        uv = (m.difx_x_up[i] * (s.U[r + 2L * ny + j] - 2 * dnud_dx * invDx2) -
-              m.dif2x_x_up[i] * (G * Dx1 * dt + 2 * nud_x * invDx2)) * ww;
-    }
-    if (i == 0) {   // setSolverQxx_BC
-        dv += k.f1 * lv;
-        fv -= f0 * lv;
-    }
-    w.a[r + j] = dv;
-    w.b[r + j] = lv;
-    w.c[r + j] = uv;
-    w.f[r + j] = fv;
-    w.w[r + j] = (uint8_t)wi;
-    w.drag[r + j] = drag;
+    asm_qxx(k, m, s, rf, w, f0, i, j, w.a[r + j], w.b[r + j], w.c[r + j], w.f[r + j]);
 }
 
 // ------------------------------------------------------------------------------------
 // Qx projector assembly: setSolverQxy_f/_d/_up/_low, src/HydroQx.cpp:262-349. Rows
 // 0..nx-2. Reads the predictor's Qx* (Qx rows 1..nx-1), IsWater and DragTerm.
 // ------------------------------------------------------------------------------------
+
 __global__ void __launch_bounds__(256) k_qxy_assemble(Consts k, State s, RunFields rf, Work w)
 {
     CELL_IJ(k.nx - 1)
-    const int jp = wrap_inc(j, ny), jm = wrap_dec(j, ny);
-    const double dt = k.dt;
-    const double *D = s.D;
-    const int wj = w.w[r + j], wp = w.w[r + jp], wm = w.w[r + jm];
-    const double drag = w.drag[r + j];
-    const double nud_x = rf.nuD_x[r + j], dnudx_dy = rf.dnuDx_dy[r + j];
-    const double inv_j = 1 / (0.5 * (D[r + j] + D[r + ny + j]));
-    const double inv_p = 1 / (0.5 * (D[r + jp] + D[r + ny + jp]));
-    const double inv_m = 1 / (0.5 * (D[r + jm] + D[r + ny + jm]));
-    const double v_p = 0.5 * (s.Vc[r + jp] + s.Vc[r + ny + jp]);
-    const double v_m = 0.5 * (s.Vc[r + jm] + s.Vc[r + ny + jm]);
-    const double qx = s.Qx[r + ny + j];
-    w.f[r + j] = wj * qx * (1 + dt * drag);
-    const int isl = !wj;
-    w.a[r + j] = wj * ((1 + dt * drag) + dt * inv_j * nud_x * k.two_inv_dy2) + isl;
-    const double wwp = wj && wp;
-    w.c[r + j] = dt * (v_p * k.inv_2dy - inv_p * (nud_x * k.inv_dy2_xy + dnudx_dy * k.inv_2dy)) * wwp;
-    const double wwm = wj && wm;
-    w.b[r + j] = dt * (-v_m * k.inv_2dy - inv_m * (nud_x * k.inv_dy2_xy - dnudx_dy * k.inv_2dy)) * wwm;
+    asm_qxy(k, s, rf, w, i, j, w.a[r + j], w.b[r + j], w.c[r + j], w.f[r + j]);
 }
 
 // ------------------------------------------------------------------------------------
@@ -296,129 +212,24 @@ __global__ void __launch_bounds__(256) k_qx_tail(Consts k, Metrics m, State s, W
 // (:442-456), setSolverQyy_f/_d/_up/_low (:124-230), C2Y of U_c (:34) and the drag
 // (:35). Rows 0..nx-2. f and d use IsWater row 0 (src/HydroQy.cpp:140,170).
 // ------------------------------------------------------------------------------------
+
 __global__ void __launch_bounds__(256) k_qyy_assemble(Consts k, Metrics m, State s, RunFields rf,
                                                       Work w)
 {
     CELL_IJ(k.nx - 1)
-    const int jp = wrap_inc(j, ny), jm = wrap_dec(j, ny), jpp = wrap_inc(jp, ny);
-    const double dt = k.dt;
-    const double *gD = w.gD + r, *gZs = w.gZs + r, *dqy = s.dQy + r;
-    // masks of row i at faces jm, j, jp
-    const double g_m = gD[jm], g_j = gD[j], g_p = gD[jp], g_pp = gD[jpp];
-    const int wy_j = wet_rule(g_j, g_p, gZs[j], gZs[jp], dqy[j], dqy[jp], k.Dmask);
-    const int wy_m = wet_rule(g_m, g_j, gZs[jm], gZs[j], dqy[jm], dqy[j], k.Dmask);
-    const int wy_p = wet_rule(g_p, g_pp, gZs[jp], gZs[jpp], dqy[jp], dqy[jpp], k.Dmask);
-    // mask of row 0 at face j (Q1)
-    const int w0 = wet_rule(w.gD[j], w.gD[jp], w.gZs[j], w.gZs[jp], s.dQy[j], s.dQy[jp], k.Dmask);
-    // Di row i+1 (getD_y(i)) at j, jm, jp
-    const double Dy = 0.5 * (g_p + g_j);
-    const double invDy = 1 / Dy;
-    const double invDy_m = 1 / (0.5 * (g_j + g_m));
-    const double invDy_p = 1 / (0.5 * (g_pp + g_p));
-    const double nud_y = rf.nuD_y[r + j];
-    double fv;
-    {   // setSolverQyy_f (:124-158)
-        const double inv_dx = m.inv_dxx[i];
-        const double x2c_d = m.x2c_d[i], x2c_up = m.x2c_up[i];
-        const double *zs = s.Zs + r, *dqx_dx = s.dQx + r, *u = s.U + r;
-        const double dzs_dy = k.inv_dy * (zs[jp] - zs[j]);
-        const double d2qx_dxy = k.inv_dy * (dqx_dx[jp] - dqx_dx[j]);
-        const double du_dy = k.inv_dy * (x2c_up * (u[ny + jp] - u[ny + j]) + x2c_d * (u[jp] - u[j]));
-        const double d2u_dxy = inv_dx * k.inv_dy * ((u[ny + jp] - u[ny + j]) - (u[jp] - u[j]));
-        fv = w0 * (rf.Fy[r + j] + s.Qy[r + ny + j] * k.inv_dt + G * Dy * (dt * d2qx_dxy - dzs_dy) +
-                   rf.dnuDy_dx[r + j] * du_dy + nud_y * d2u_dxy);
-    }
-    // Ui_j = C2Y(U_c row i) and the drag with V row i+1 (old)
-    const double uy = 0.5 * (s.Uc[r + jp] + s.Uc[r + j]);
-    const double *v = s.V + r + ny;
-    const double drag = drag_term(rf.cD_y[r + j], rf.Urms_y[r + j], uy, v[j], invDy, k);
-    // setSolverQyy_d (:160-181)
-    const int isl = !w0;
-    const double dv = (k.inv_dt + drag + G * Dy * dt * k.two_inv_dy2 + 2 * nud_y * invDy * k.two_inv_dy2) * w0 + isl;
-    const double dnud_dy = rf.dnuD_dy[r + j];
-    // setSolverQyy_low (:206-230)
-    const double wwm = wy_j && wy_m;
-    const double lv = (0.5 * k.inv_dy * (-v[jm] + 2 * dnud_dy * invDy_m) -
-                       k.inv_dy2_yy * (G * Dy * dt + 2 * nud_y * invDy_m)) * wwm;
-    // setSolverQyy_up (:183-204)
-    const double wwp = wy_j && wy_p;
-    const double uv = (0.5 * k.inv_dy * (v[jp] - 2 * dnud_dy * invDy_p) -
-                       k.inv_dy2_yy * (G * Dy * dt + 2 * nud_y * invDy_p)) * wwp;
-    w.a[r + j] = dv;
-    w.b[r + j] = lv;
-    w.c[r + j] = uv;
-    w.f[r + j] = fv;
-    w.w[r + j] = (uint8_t)wy_j;
-    w.drag[r + j] = drag;
+    asm_qyy(k, m, s, rf, w, i, j, w.a[r + j], w.b[r + j], w.c[r + j], w.f[r + j]);
 }
 
 // ------------------------------------------------------------------------------------
 // Qy projector assembly: setSolverQyx_f/_d/_up/_low/0_low (src/HydroQy.cpp:235-362) and
 // setSolverQyx_BC (src/HydroBoundaryConditions.cpp:41-60). Rows 0..nx-2.
 // ------------------------------------------------------------------------------------
+
 __global__ void __launch_bounds__(256) k_qyx_assemble(Consts k, Metrics m, State s, RunFields rf,
                                                       Work w)
 {
     CELL_IJ(k.nx - 1)
-    const int nx = k.nx;
-    const int jp = wrap_inc(j, ny);
-    const double dt = k.dt;
-    const double *gD = w.gD;
-    const int wi = w.w[r + j];
-    const double drag = w.drag[r + j];
-    const double nud = rf.nuD_y[r + j], dnud_dx = rf.dnuDy_dx[r + j];
-    // setSolverQyx_f (:235-248)
-    const double fv = wi * s.Qy[r + ny + j] * (1 + dt * drag);
-    // setSolverQyx_d (:250-285)
-    double dv;
-    {
-        const double x2c_d = m.x2c_d[i], x2c_up = m.x2c_up[i];
-        const double *u = s.U + r;
-        const double dx_d = m.difx_c_d[i], d2x_d = m.dif2x_c_d[i];
-        const double inv_dx = m.inv_dxx[i];
-        const double inv_d = 1 / (0.5 * (gD[r + jp] + gD[r + j]));   // inv_Di row i+1
-        const int isl = !wi;
-        const double du_dx = 0.5 * inv_dx * ((u[ny + jp] + u[ny + j]) - (u[jp] + u[j]));
-        const double u_y = 0.5 * (x2c_up * (u[ny + jp] + u[ny + j]) + x2c_d * (u[jp] + u[j]));
-        dv = wi * ((1 + dt * drag) + dt * (u_y * dx_d + du_dx) - dt * (inv_d * nud * d2x_d + inv_d * dnud_dx * dx_d)) + isl;
-    }
-    const double uy = 0.5 * (s.Uc[r + jp] + s.Uc[r + j]);   // Ui_j row i (C2Y of U_c)
-    // setSolverQyx_up (:287-311); inv_Di row i+2 = getD_y(i+1); IsWater row nx-1 is false
-    double uv;
-    {
-        const double inv_up = 1 / (0.5 * (gD[r + ny + jp] + gD[r + ny + j]));
-        const int wu = (i + 1 <= nx - 2) ? w.w[r + ny + j] : 0;
-        const double ww = wi && wu;
-        const double dx_up = m.difx_c_up[i], d2x_up = m.dif2x_c_up[i];
-        uv = ww * dt * (uy * dx_up - nud * d2x_up * inv_up - dnud_dx * dx_up * inv_up);
-    }
-    // setSolverQyx0_low (:340-362) / setSolverQyx_low (:313-338)
-    double lv;
-    {
-        const double dx_low = m.difx_c_low[i], d2x_low = m.dif2x_c_low[i];
-        if (i == 0) {
-            // getD_y0 (:426-440); the wrap element reads d[0] where the interior reads
-            // row 1 (:438)
-            double Dy0;
-            if (j < ny - 1)
-                Dy0 = 1 * (gD[j + 1] + gD[j]) - 0.5 * (gD[ny + j + 1] + gD[ny + j]);
-            else
-                Dy0 = 1 * (gD[0] + gD[ny - 1]) - 0.5 * (gD[0] + gD[2 * ny - 1]);
-            const double inv_low = 1 / Dy0;
-            lv = wi * dt * (uy * dx_low - nud * d2x_low * inv_low - dnud_dx * dx_low * inv_low);
-        } else {
-            const double inv_low = 1 / (0.5 * (gD[r - ny + jp] + gD[r - ny + j]));
-            const int wl = w.w[r - ny + j];
-            const double ww = wi && wl;
-            lv = ww * dt * (uy * dx_low - nud * d2x_low * inv_low - dnud_dx * dx_low * inv_low);
-        }
-    }
-    if (i == 0) dv += k.bcQy * lv;
-    if (i == nx - 2) dv += uv * k.inv_3;
-    w.a[r + j] = dv;
-    w.b[r + j] = lv;
-    w.c[r + j] = uv;
-    w.f[r + j] = fv;
+    asm_qyx(k, m, s, rf, w, i, j, w.a[r + j], w.b[r + j], w.c[r + j], w.f[r + j]);
 }
 
 // ------------------------------------------------------------------------------------

@@ -21,16 +21,17 @@ thread_local std::string g_create_error;
 
 enum KernelId {
     K_DRAG_VISC, K_RUN_FIELDS, K_QXX_ASM, K_THOMAS_QX, K_QXY_ASM, K_PERIODIC_QX, K_QX_TAIL,
-    K_QYY_ASM, K_PERIODIC_QY, K_QYX_ASM, K_THOMAS_QY, K_QY_TAIL, K_COUNT
+    K_QYY_ASM, K_PERIODIC_QY, K_QYX_ASM, K_THOMAS_QY, K_QY_TAIL, K_FUSED_QXY, K_FUSED_QYY, K_COUNT
 };
 const char *const kKernelNames[K_COUNT] = {
     "drag_viscosity", "run_fields", "qxx_assemble", "thomas_qx", "qxy_assemble", "periodic_qx", "qx_tail",
-    "qyy_assemble", "periodic_qy", "qyx_assemble", "thomas_qy", "qy_tail"};
+    "qyy_assemble", "periodic_qy", "qyx_assemble", "thomas_qy", "qy_tail", "fused_qxy", "fused_qyy"};
 
 }  // namespace
 
 struct m70h {
     int nx = 0, ny = 0;
+    double dy = 0.0;
     long F = 0, Fp = 0;
     m70h_params p{};
     Consts k{};
@@ -166,6 +167,26 @@ int run_step(m70h *h)
     Work &w = h->w;
     int rc;
     hipStream_t st = h->stream;
+    if (h->solver == M70H_SOLVER_FUSED) {
+        // x-sweeps: coalesced assembly kernel + partition solver on the columns;
+        // y-sweeps: one launch each, the row's coefficients staged through LDS.
+        if ((rc = timed(h, K_QXX_ASM, [&] { return launch_qxx_assemble(st, k, h->m, s, h->rf, w, f0); }))) return rc;
+        if ((rc = timed(h, K_THOMAS_QX, [&] {
+                 return launch_partition_cols(st, w.a, w.b, w.c, w.f, s.Qx + ny, nx - 1, ny, ny, h->tile_cols);
+             })))
+            return rc;
+        if ((rc = timed(h, K_FUSED_QXY, [&] { return launch_lds_qxy(st, k, s, h->rf, w); }))) return rc;
+        if ((rc = timed(h, K_QX_TAIL, [&] { return launch_qx_tail(st, k, h->m, s, w, f0); }))) return rc;
+        if ((rc = timed(h, K_FUSED_QYY, [&] { return launch_lds_qyy(st, k, h->m, s, h->rf, w); }))) return rc;
+        if ((rc = timed(h, K_QYX_ASM, [&] { return launch_qyx_assemble(st, k, h->m, s, h->rf, w); }))) return rc;
+        if ((rc = timed(h, K_THOMAS_QY, [&] {
+                 return launch_partition_cols(st, w.a, w.b, w.c, w.f, s.Qy + ny, nx - 1, ny, ny, h->tile_cols);
+             })))
+            return rc;
+        if ((rc = timed(h, K_QY_TAIL, [&] { return launch_qy_tail(st, k, s); }))) return rc;
+        std::swap(s.D, s.Dn);
+        return M70H_OK;
+    }
     // Qx predictor: assembly + Thomas along x (src/HydroQx.cpp:29-51)
     if ((rc = timed(h, K_QXX_ASM, [&] { return launch_qxx_assemble(st, k, h->m, s, h->rf, w, f0); }))) return rc;
     const bool fast = h->solver == M70H_SOLVER_PARTITION;
@@ -203,6 +224,50 @@ int run_step(m70h *h)
     if ((rc = timed(h, K_QY_TAIL, [&] { return launch_qy_tail(st, k, s); }))) return rc;
     std::swap(s.D, s.Dn);
     return M70H_OK;
+}
+
+// Scalars of the kernels, each formed with the reference's expression (see Consts).
+// The boundary constants depend on the offshore option at construction only
+// (src/Hydro.cpp:44-60), as in the reference.
+void set_consts(m70h *h, const m70h_params &pp, double dx0, double dy, bool with_bc)
+{
+    const m70h_params *p = &pp;
+    const int nx = h->nx, ny = h->ny;
+    const double inv_rho = 1 / RHO;
+    Consts &k = h->k;
+    h->p = pp;
+    k.nx = nx; k.ny = ny;
+    k.drag_model = p->drag_model; k.visc_model = p->visc_model;
+    k.dt = p->dt;
+    k.inv_dt = 1 / p->dt;
+    k.inv_dy = 1 / dy;
+    k.inv_2dy = 0.5 / dy;
+    k.inv_dy2_xy = 4 * k.inv_2dy * k.inv_2dy;
+    k.inv_dy2_yy = k.inv_dy * k.inv_dy;
+    k.two_inv_dy2 = 2 / (dy * dy);
+    k.alpha_f2 = (p->alpha_f * p->alpha_f);
+    k.inv_sqrt2 = 1 / SQRT_2;
+    k.Dmask = p->Dmin + 1E-4;
+    if (with_bc) {
+        const double lSponge = dx0 * 5;
+        const double exp_dx_2ls = std::exp(-0.5 * dx0 / lSponge);
+        const double exp_dx_ls = std::exp(-dx0 / lSponge);
+        k.f1 = (p->offshore_bc == M70H_BC_SPONGE) ? 0.5 * exp_dx_2ls / (1 - 0.5 * exp_dx_2ls) : 1;
+        k.bcQy = exp_dx_ls;
+    }
+    k.inv_3 = 1.0 / 3.0;
+    k.inv_rho = inv_rho;
+    {
+        double inv_4dy = 0.25 / dy;
+        inv_4dy *= inv_rho;
+        k.inv_4dy_rho = inv_4dy;
+        double inv_dyr = 1 / dy;
+        inv_dyr *= inv_rho;
+        k.inv_dy_rho = inv_dyr;
+    }
+    k.inv_2dy_c2c = 0.5 / dy;
+    k.cD0 = p->cD0; k.nu0 = p->nu0; k.M = p->M; k.ka = p->ka; k.z0 = p->z0; k.alpha_r = p->alpha_r;
+    k.inv_rho_f = (float)(1 / RHO);
 }
 
 // Grid metrics as Grid::Grid (src/Grid.cpp:32-92, closed forms :127-145).
@@ -296,6 +361,7 @@ int m70h_create(int nx, int ny, const double *x, double dy, const m70h_params *p
     m70h *h = new m70h();
     h->nx = nx;
     h->ny = ny;
+    h->dy = dy;
     h->F = (long)nx * ny;
     h->Fp = h->F + ny;
     h->p = *p;
@@ -340,40 +406,7 @@ int m70h_create(int nx, int ny, const double *x, double dy, const m70h_params *p
     h->m.x2c_d = mb + o_x2c_d;
 
     // constants (src/Hydro.cpp:44-60 for the boundary constants)
-    Consts &k = h->k;
-    k.nx = nx; k.ny = ny;
-    k.drag_model = p->drag_model; k.visc_model = p->visc_model;
-    k.dt = p->dt;
-    k.inv_dt = 1 / p->dt;
-    k.inv_dy = 1 / dy;
-    k.inv_2dy = 0.5 / dy;
-    k.inv_dy2_xy = 4 * k.inv_2dy * k.inv_2dy;
-    k.inv_dy2_yy = k.inv_dy * k.inv_dy;
-    k.two_inv_dy2 = 2 / (dy * dy);
-    k.alpha_f2 = (p->alpha_f * p->alpha_f);
-    k.inv_sqrt2 = 1 / SQRT_2;
-    k.Dmask = p->Dmin + 1E-4;
-    {
-        const double lSponge = dx_x[0] * 5;
-        const double exp_dx_2ls = std::exp(-0.5 * dx_x[0] / lSponge);
-        const double exp_dx_ls = std::exp(-dx_x[0] / lSponge);
-        k.f1 = (p->offshore_bc == M70H_BC_SPONGE) ? 0.5 * exp_dx_2ls / (1 - 0.5 * exp_dx_2ls) : 1;
-        k.bcQy = exp_dx_ls;
-    }
-    k.inv_3 = 1.0 / 3.0;
-    k.inv_rho = inv_rho;
-    {
-        double inv_4dy = 0.25 / dy;
-        inv_4dy *= inv_rho;
-        k.inv_4dy_rho = inv_4dy;
-        double inv_dyr = 1 / dy;
-        inv_dyr *= inv_rho;
-        k.inv_dy_rho = inv_dyr;
-    }
-    k.inv_2dy_c2c = 0.5 / dy;
-    k.cD0 = p->cD0; k.nu0 = p->nu0; k.M = p->M; k.ka = p->ka; k.z0 = p->z0; k.alpha_r = p->alpha_r;
-    k.inv_rho_f = (float)(1 / RHO);
-
+    set_consts(h, *p, dx_x[0], dy, true);
     // buffers
     const long F = h->F, Fp = h->Fp;
     double **stF[] = {&h->s.Zs, &h->s.D, &h->s.Dn, &h->s.Uc, &h->s.Vc, &h->s.dQx, &h->s.dQy};
@@ -412,6 +445,20 @@ void m70h_destroy(m70h *h)
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     delete h;
+}
+
+int m70h_set_params(m70h *h, const m70h_params *p)
+{
+    if (!h || !p) return M70H_EINVAL;
+    if (p->drag_model < 0 || p->drag_model > 2 || p->visc_model < 0 || p->visc_model > 3)
+        return fail(h, M70H_EINVAL, "bad model enum");
+    const double f1 = h->k.f1, bcQy = h->k.bcQy;
+    const int bc = h->p.offshore_bc;
+    set_consts(h, *p, 0.0, h->dy, false);
+    h->k.f1 = f1;
+    h->k.bcQy = bcQy;
+    h->p.offshore_bc = bc;
+    return M70H_OK;
 }
 
 int m70h_set_stream(m70h *h, void *hip_stream)
@@ -568,9 +615,12 @@ int m70h_run(m70h *h, int nIter)
 int m70h_set_solver(m70h *h, int solver)
 {
     if (!h) return M70H_EINVAL;
-    if (solver != M70H_SOLVER_EXACT && solver != M70H_SOLVER_PARTITION) return fail(h, M70H_EINVAL, "unknown solver");
-    if (solver == M70H_SOLVER_PARTITION && (!partition_supported(h->nx - 1) || !partition_supported(h->ny)))
+    if (solver != M70H_SOLVER_EXACT && solver != M70H_SOLVER_PARTITION && solver != M70H_SOLVER_FUSED)
+        return fail(h, M70H_EINVAL, "unknown solver");
+    if (solver != M70H_SOLVER_EXACT && (!partition_supported(h->nx - 1) || !partition_supported(h->ny)))
         return fail(h, M70H_EINVAL, "partition solver supports line lengths up to 4096");
+    if (solver == M70H_SOLVER_FUSED && !rows_lds_supported(h->ny))
+        return fail(h, M70H_EINVAL, "fused solver supports ny up to 4096");
     h->solver = solver;
     return M70H_OK;
 }

@@ -17,6 +17,7 @@
 // The matrices are diagonally dominant (the 1/dt term), so no pivoting is needed. The
 // rounding differs from the serial recurrences of the reference; results agree to a few
 // ulps per solve.
+#include "assembly.h"
 #include "hydro_device.h"
 #include "hydro_kernels.h"
 
@@ -41,16 +42,169 @@ __device__ __forceinline__ long elem_index(const LineGeom &g, int line, int k, l
     return COLS ? (long)k * ld + line : (long)line * ld + k;
 }
 
+// Coefficient providers: get(line, k, a, b, c, f) yields element k of line `line`.
+template <bool COLS>
+struct ProvArrays {   // coefficients stored in four arrays
+    const double *A, *B, *C, *F;
+    long ld;
+    __device__ __forceinline__ void get(int line, int k, double &a, double &b, double &c, double &f) const
+    {
+        const long e = COLS ? (long)k * ld + line : (long)line * ld + k;
+        a = A[e]; b = B[e]; c = C[e]; f = F[e];
+    }
+};
+struct ProvQxy {      // Qx projector, lines along y (rows): line = i, k = j
+    Consts k; State s; RunFields rf; Work w;
+    __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
+    { asm_qxy(k, s, rf, w, line, kk, a, b, c, f); }
+};
+struct ProvQyy {      // Qy predictor, rows
+    Consts k; Metrics m; State s; RunFields rf; Work w;
+    __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
+    { asm_qyy(k, m, s, rf, w, line, kk, a, b, c, f); }
+};
+
 // One block: lines_per_block lines x S segments. COLS: lines are columns of row-major
 // arrays (element (k, line) at k*ld + line) and thread t maps to (line t % L, segment
 // t / L) so that a wavefront reads runs of consecutive columns; ROWS: lines are rows
 // (element (line, k) at line*ld + k) and t maps to (line t / S, segment t % S).
-template <bool COLS, int BLOCK, int SEG_M>
-__global__ void __launch_bounds__(BLOCK) k_partition_solve(const double *__restrict__ A,
-                                                           const double *__restrict__ B,
-                                                           const double *__restrict__ C,
-                                                           const double *__restrict__ F, double *__restrict__ Y,
-                                                           LineGeom g)
+// ---------------------------------------------------------------------------------
+// Building blocks shared by the partition kernels.
+// ---------------------------------------------------------------------------------
+struct RedRows {   // the two reduced rows of one segment
+    double A_F, C_F, R_F, A_L, B_L, R_L, lo0, upL;
+};
+
+// Two-sided elimination of one segment of m (2..SEG_M) unknowns held in registers:
+// lo/di/up/rh are the lower, diagonal, upper and right-hand side. On return lo[q] is the
+// fill-in on the segment's first unknown, up[q] the fill-in on its last unknown, rh[q]
+// the modified right-hand side and di[q] the inverse pivot of interior row q.
+template <int SEG_M>
+__device__ __forceinline__ RedRows seg_eliminate(int m, double (&lo)[SEG_M], double (&di)[SEG_M],
+                                                 double (&up)[SEG_M], double (&rh)[SEG_M])
+{
+    RedRows r;
+    // downward pass: row q -> s_q x_0 + d_q x_q + up_q x_{q+1} = g_q (q >= 1)
+    r.lo0 = lo[0];
+    double dLast = di[1];
+    di[1] = 1.0 / di[1];
+#pragma unroll
+    for (int q = 2; q < SEG_M; ++q) {
+        if (q < m) {
+            const double tq = lo[q] * di[q - 1];
+            lo[q] = -tq * lo[q - 1];
+            const double dq = di[q] - tq * up[q - 1];
+            rh[q] = rh[q] - tq * rh[q - 1];
+            di[q] = 1.0 / dq;
+            if (q == m - 1) dLast = dq;
+        }
+    }
+    {   // row L (q = m-1): A_L x_0 + B_L x_{m-1} + upL x_m = R_L
+        double sL = lo[1], gL = rh[1], uL = up[1];
+#pragma unroll
+        for (int q = 2; q < SEG_M; ++q)
+            if (q == m - 1) { sL = lo[q]; gL = rh[q]; uL = up[q]; }
+        r.A_L = sL; r.B_L = dLast; r.R_L = gL; r.upL = uL;
+    }
+    // upward pass: row q -> s'_q x_0 + d_q x_q + q_q x_{m-1} = g'_q, q = m-2..1
+#pragma unroll
+    for (int q = SEG_M - 2; q >= 1; --q) {
+        if (q < m - 2) {
+            const double tq = up[q] * di[q + 1];
+            lo[q] = lo[q] - tq * lo[q + 1];
+            up[q] = -tq * up[q + 1];
+            rh[q] = rh[q] - tq * rh[q + 1];
+        }
+    }
+    // row F (q = 0): lo0 x_{-1} + A_F x_0 + C_F x_{m-1} = R_F
+    if (m >= 3) {
+        const double tq = up[0] * di[1];
+        r.A_F = di[0] - tq * lo[1];
+        r.C_F = -tq * up[1];
+        r.R_F = rh[0] - tq * rh[1];
+    } else {
+        r.A_F = di[0];
+        r.C_F = up[0];
+        r.R_F = rh[0];
+    }
+    return r;
+}
+
+// Cyclic PCR on N = 2S reduced rows of one line stored at red0..red3[base + i] (lower,
+// diagonal, upper, rhs). Every thread of the block must call it (it holds barriers);
+// `active` threads own rows 2*seg and 2*seg+1. Returns the solution of those two rows.
+__device__ __forceinline__ void pcr_cyclic(double *red0, double *red1, double *red2, double *red3, int base, int seg,
+                                           int S, bool active, const RedRows &r, double &xF, double &xL)
+{
+    const int N = 2 * S;
+    const int rF = base + 2 * seg, rL = rF + 1;
+    if (active) {
+        red0[rF] = r.lo0; red1[rF] = r.A_F; red2[rF] = r.C_F; red3[rF] = r.R_F;
+        red0[rL] = r.A_L; red1[rL] = r.B_L; red2[rL] = r.upL; red3[rL] = r.R_L;
+    }
+    __syncthreads();
+    for (int st = 1; st < S; st <<= 1) {   // strides 1, 2, ..., N/4
+        double na[2], nb[2], nc[2], nd[2];
+        if (active) {
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                const int i = 2 * seg + w;
+                const int im = base + ((i - st) & (N - 1)), ip = base + ((i + st) & (N - 1));
+                const int ii = base + i;
+                const double k1m = red0[ii] / red1[im];
+                const double k2p = red2[ii] / red1[ip];
+                na[w] = -red0[im] * k1m;
+                nc[w] = -red2[ip] * k2p;
+                nb[w] = red1[ii] - red2[im] * k1m - red0[ip] * k2p;
+                nd[w] = red3[ii] - red3[im] * k1m - red3[ip] * k2p;
+            }
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                const int ii = base + 2 * seg + w;
+                red0[ii] = na[w]; red1[ii] = nb[w]; red2[ii] = nc[w]; red3[ii] = nd[w];
+            }
+        }
+        __syncthreads();
+    }
+    xF = 0.0;
+    xL = 0.0;
+    if (active) {
+        // rows i and j = i + N/2 are coupled only to each other:
+        //   b_i x_i + e_i x_j = d_i,  e_j x_i + b_j x_j = d_j,  e = lower + upper
+        double xs[2];
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+            const int i = 2 * seg + w;
+            const int ii = base + i, jj = base + ((i + S) & (N - 1));
+            const double bi = red1[ii], ei = red0[ii] + red2[ii], di_ = red3[ii];
+            const double bj = red1[jj], ej = red0[jj] + red2[jj], dj = red3[jj];
+            xs[w] = (di_ * bj - ei * dj) / (bi * bj - ei * ej);
+        }
+        xF = xs[0];
+        xL = xs[1];
+    }
+}
+
+// Interior unknown q of an eliminated segment.
+template <int SEG_M>
+__device__ __forceinline__ double seg_value(int q, int m, const double (&lo)[SEG_M], const double (&di)[SEG_M],
+                                            const double (&up)[SEG_M], const double (&rh)[SEG_M], double xF,
+                                            double xL)
+{
+    if (q == 0) return xF;
+    if (q == m - 1) return xL;
+    return (rh[q] - lo[q] * xF - up[q] * xL) * di[q];
+}
+
+// One block: lines_per_block lines x S segments. COLS: lines are columns of row-major
+// arrays (element (k, line) at k*ld + line) and thread t maps to (line t % L, segment
+// t / L) so that a wavefront reads runs of consecutive columns; ROWS: lines are rows
+// (element (line, k) at line*ld + k) and t maps to (line t / S, segment t % S).
+template <bool COLS, int BLOCK, int SEG_M, class Prov>
+__global__ void __launch_bounds__(BLOCK) k_partition_solve(Prov prov, double *__restrict__ Y, LineGeom g)
 {
     __shared__ double red[4][BLOCK * 2];   // reduced rows: lower, diag, upper, rhs
     const int t = threadIdx.x;
@@ -70,138 +224,84 @@ __global__ void __launch_bounds__(BLOCK) k_partition_solve(const double *__restr
     const int m = k1 - k0;   // 2..SEG_M
 
     double lo[SEG_M], di[SEG_M], up[SEG_M], rh[SEG_M];
+    RedRows rr{};
     if (active) {
 #pragma unroll
-        for (int q = 0; q < SEG_M; ++q) {
-            if (q < m) {
-                const long e = elem_index<COLS>(g, line, k0 + q, g.ld);
-                di[q] = A[e];
-                lo[q] = B[e];
-                up[q] = C[e];
-                rh[q] = F[e];
-            }
-        }
+        for (int q = 0; q < SEG_M; ++q)
+            if (q < m) prov.get(line, k0 + q, di[q], lo[q], up[q], rh[q]);
         if (!g.periodic) {
             if (k0 == 0) lo[0] = 0.0;
             if (k1 == n) up[m - 1] = 0.0;
         }
+        rr = seg_eliminate<SEG_M>(m, lo, di, up, rh);
     }
-    // 1a. downward pass: row q -> s_q x_0 + d_q x_q + up_q x_{q+1} = g_q (q >= 1).
-    //     Registers are recycled as the pass consumes them: lo[q] becomes s_q, di[q]
-    //     becomes 1/d_q, rh[q] becomes g_q.
-    double A_F = 0, C_F = 0, R_F = 0, A_L = 0, B_L = 0, R_L = 0, lo0 = 0, upL = 0;
+    double xF, xL;
+    pcr_cyclic(red[0], red[1], red[2], red[3], lloc * 2 * S, seg, S, active, rr, xF, xL);
     if (active) {
-        lo0 = lo[0];
-        double dLast = di[1];         // d_{m-1}, kept un-inverted for the reduced row
-        di[1] = 1.0 / di[1];          // s_1 = lo[1] already in place
 #pragma unroll
-        for (int q = 2; q < SEG_M; ++q) {
-            if (q < m) {
-                const double tq = lo[q] * di[q - 1];
-                lo[q] = -tq * lo[q - 1];
-                const double dq = di[q] - tq * up[q - 1];
-                rh[q] = rh[q] - tq * rh[q - 1];
-                di[q] = 1.0 / dq;
-                if (q == m - 1) dLast = dq;
-            }
-        }
-        // row L (q = m-1): A_L x_0 + B_L x_{m-1} + upL x_m = R_L
-        {
-            double sL = lo[1], gL = rh[1], uL = up[1];
-#pragma unroll
-            for (int q = 2; q < SEG_M; ++q)
-                if (q == m - 1) { sL = lo[q]; gL = rh[q]; uL = up[q]; }
-            A_L = sL; B_L = dLast; R_L = gL; upL = uL;
-        }
-        // 1b. upward pass: row q -> s'_q x_0 + d_q x_q + q_q x_{m-1} = g'_q for
-        //     q = m-2..1; up[q] becomes the fill-in q_q on x_{m-1}.
-#pragma unroll
-        for (int q = SEG_M - 2; q >= 1; --q) {
-            if (q < m - 2) {
-                const double tq = up[q] * di[q + 1];
-                lo[q] = lo[q] - tq * lo[q + 1];
-                up[q] = -tq * up[q + 1];
-                rh[q] = rh[q] - tq * rh[q + 1];
-            }
-        }
-        // row F (q = 0): lo0 x_{-1} + A_F x_0 + C_F x_{m-1} = R_F
-        if (m >= 3) {
-            const double tq = up[0] * di[1];
-            A_F = di[0] - tq * lo[1];
-            C_F = -tq * up[1];
-            R_F = rh[0] - tq * rh[1];
-        } else {
-            A_F = di[0];
-            C_F = up[0];
-            R_F = rh[0];
-        }
+        for (int q = 0; q < SEG_M; ++q)
+            if (q < m) Y[elem_index<COLS>(g, line, k0 + q, g.ldy)] = seg_value<SEG_M>(q, m, lo, di, up, rh, xF, xL);
     }
+}
 
-    // 2. cyclic PCR on the 2S reduced rows of each line, in LDS.
-    const int N = 2 * S;
-    const int base = lloc * N;   // rows of this line in red[][]
-    const int rF = base + 2 * seg, rL = rF + 1;
-    if (active) {
-        red[0][rF] = lo0; red[1][rF] = A_F; red[2][rF] = C_F; red[3][rF] = R_F;
-        red[0][rL] = A_L; red[1][rL] = B_L; red[2][rL] = upL; red[3][rL] = R_L;
+// Row lines staged through LDS: one line (n <= 4096) per block of RB threads. The block
+// assembles the line's coefficients in coalesced order into LDS (element e at
+// e + e/8, so that the per-thread segment reads below are free of bank conflicts),
+// each thread then takes its segment, and the solution goes back through LDS to a
+// coalesced store.
+constexpr int RB = 512;
+constexpr int RM = 8;
+constexpr int RNMAX = RB * RM;
+constexpr int RLDS = RNMAX + RNMAX / 8;
+__device__ __forceinline__ int lpad(int e) { return e + (e >> 3); }
+
+template <class Prov>
+__global__ void __launch_bounds__(RB) k_rows_lds(Prov prov, double *__restrict__ Y, long ldy, int n, int S,
+                                                 int periodic)
+{
+    __shared__ double sh[4][RLDS];
+    const int t = threadIdx.x;
+    const int line = blockIdx.x;
+    for (int e = t; e < n; e += RB) {
+        double a, b, c, f;
+        prov.get(line, e, a, b, c, f);
+        const int p = lpad(e);
+        sh[0][p] = a; sh[1][p] = b; sh[2][p] = c; sh[3][p] = f;
     }
     __syncthreads();
-    for (int st = 1; st < S; st <<= 1) {   // strides 1, 2, ..., N/4
-        double na[2], nb[2], nc[2], nd[2];
-        if (active) {
-#pragma unroll
-            for (int w = 0; w < 2; ++w) {
-                const int i = 2 * seg + w;
-                const int im = base + ((i - st) & (N - 1)), ip = base + ((i + st) & (N - 1));
-                const int ii = base + i;
-                const double k1m = red[0][ii] / red[1][im];
-                const double k2p = red[2][ii] / red[1][ip];
-                na[w] = -red[0][im] * k1m;
-                nc[w] = -red[2][ip] * k2p;
-                nb[w] = red[1][ii] - red[2][im] * k1m - red[0][ip] * k2p;
-                nd[w] = red[3][ii] - red[3][im] * k1m - red[3][ip] * k2p;
-            }
-        }
-        __syncthreads();
-        if (active) {
-#pragma unroll
-            for (int w = 0; w < 2; ++w) {
-                const int ii = base + 2 * seg + w;
-                red[0][ii] = na[w]; red[1][ii] = nb[w]; red[2][ii] = nc[w]; red[3][ii] = nd[w];
-            }
-        }
-        __syncthreads();
-    }
-    double xF = 0, xL = 0;
-    if (active) {
-        // rows i and j = i + N/2 are coupled only to each other:
-        //   b_i x_i + e_i x_j = d_i,  e_j x_i + b_j x_j = d_j,  e = lower + upper
-        double xs[2];
-#pragma unroll
-        for (int w = 0; w < 2; ++w) {
-            const int i = 2 * seg + w;
-            const int ii = base + i, jj = base + ((i + S) & (N - 1));
-            const double bi = red[1][ii], ei = red[0][ii] + red[2][ii], di_ = red[3][ii];
-            const double bj = red[1][jj], ej = red[0][jj] + red[2][jj], dj = red[3][jj];
-            xs[w] = (di_ * bj - ei * dj) / (bi * bj - ei * ej);
-        }
-        xF = xs[0];
-        xL = xs[1];
-    }
-
-    // 3. back-substitution and store
+    const bool active = t < S;
+    const int seg = t;
+    const int k0 = (int)(((long)seg * n) / S);
+    const int k1 = (int)(((long)(seg + 1) * n) / S);
+    const int m = k1 - k0;
+    double lo[RM], di[RM], up[RM], rh[RM];
     if (active) {
 #pragma unroll
-        for (int q = 0; q < SEG_M; ++q) {
+        for (int q = 0; q < RM; ++q) {
             if (q < m) {
-                double x;
-                if (q == 0) x = xF;
-                else if (q == m - 1) x = xL;
-                else x = (rh[q] - lo[q] * xF - up[q] * xL) * di[q];
-                Y[elem_index<COLS>(g, line, k0 + q, g.ldy)] = x;
+                const int p = lpad(k0 + q);
+                di[q] = sh[0][p]; lo[q] = sh[1][p]; up[q] = sh[2][p]; rh[q] = sh[3][p];
             }
         }
+        if (!periodic) {
+            if (k0 == 0) lo[0] = 0.0;
+            if (k1 == n) up[m - 1] = 0.0;
+        }
     }
+    __syncthreads();   // the coefficient tile is free from here on
+    RedRows rr{};
+    if (active) rr = seg_eliminate<RM>(m, lo, di, up, rh);
+    double xF, xL;
+    pcr_cyclic(sh[0], sh[1], sh[2], sh[3], 0, seg, S, active, rr, xF, xL);
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < RM; ++q)
+            if (q < m) sh[0][lpad(k0 + q)] = seg_value<RM>(q, m, lo, di, up, rh, xF, xL);
+    }
+    __syncthreads();
+    double *yl = Y + (long)line * ldy;
+    for (int e = t; e < n; e += RB) yl[e] = sh[0][lpad(e)];
 }
 
 // Picks S (power of two, n >= 2S, ceil(n/S) <= M) for a line length n; 0 if none.
@@ -216,12 +316,11 @@ static int pick_segments(int n, int M, int smax)
 
 bool partition_supported(int n) { return pick_segments(n, 16, 256) > 0 || pick_segments(n, 8, 512) > 0; }
 
-template <bool COLS, int BLOCK, int M>
-static hipError_t launch_partition(hipStream_t st, const double *A, const double *B, const double *C,
-                                   const double *F, double *Y, long ld, long ldy, int n, int nL, bool periodic)
+template <bool COLS, int BLOCK, int M, class Prov>
+static hipError_t launch_partition(hipStream_t st, const Prov &prov, double *Y, long ldy, int n, int nL, bool periodic)
 {
     LineGeom g{};
-    g.ld = ld; g.ldy = ldy; g.n = n; g.nL = nL; g.periodic = periodic ? 1 : 0;
+    g.ld = 0; g.ldy = ldy; g.n = n; g.nL = nL; g.periodic = periodic ? 1 : 0;
     // columns: keep at least 4 lines per block so a row access covers 32 contiguous bytes
     g.S = pick_segments(n, M, COLS ? BLOCK / 4 : BLOCK);
     if (g.S == 0) return hipErrorInvalidValue;
@@ -231,29 +330,63 @@ static hipError_t launch_partition(hipStream_t st, const double *A, const double
     g.lines_per_block = BLOCK / g.S;
     const int blocks = (nL + g.lines_per_block - 1) / g.lines_per_block;
     g.xcd_remap = COLS ? 1 : 0;
-    hipLaunchKernelGGL((k_partition_solve<COLS, BLOCK, M>), dim3((unsigned)blocks), dim3(BLOCK), 0, st, A, B, C, F,
-                       Y, g);
+    hipLaunchKernelGGL((k_partition_solve<COLS, BLOCK, M, Prov>), dim3((unsigned)blocks), dim3(BLOCK), 0, st, prov, Y,
+                       g);
     return hipGetLastError();
 }
 
-// tile: unknowns per thread (8 or 16); 0 picks the default for the layout.
+// Column lines (x-sweeps): tile = unknowns per thread (8 or 16; 0 = default 8).
+template <class Prov>
+static hipError_t dispatch_cols(hipStream_t st, const Prov &prov, double *Y, long ldy, int n, int nL, int tile)
+{
+    if (tile == 16 && pick_segments(n, 16, 64) > 0) return launch_partition<true, 256, 16>(st, prov, Y, ldy, n, nL, false);
+    if (pick_segments(n, 8, 256) > 0) return launch_partition<true, 1024, 8>(st, prov, Y, ldy, n, nL, false);
+    return launch_partition<true, 256, 16>(st, prov, Y, ldy, n, nL, false);
+}
+
+// Row lines (y-sweeps): tile = unknowns per thread (8 or 16; 0 = default 8).
+template <class Prov>
+static hipError_t dispatch_rows(hipStream_t st, const Prov &prov, double *Y, long ldy, int n, int nL, bool periodic,
+                                int tile)
+{
+    if (tile == 16 && pick_segments(n, 16, 256) > 0)
+        return launch_partition<false, 256, 16>(st, prov, Y, ldy, n, nL, periodic);
+    if (pick_segments(n, 8, 512) > 0) return launch_partition<false, 512, 8>(st, prov, Y, ldy, n, nL, periodic);
+    return launch_partition<false, 256, 16>(st, prov, Y, ldy, n, nL, periodic);
+}
+
 hipError_t launch_partition_cols(hipStream_t st, const double *A, const double *B, const double *C, const double *F,
                                  double *Y, int n, int m, long ld, int tile)
 {
-    if (tile == 16 && pick_segments(n, 16, 64) > 0)
-        return launch_partition<true, 256, 16>(st, A, B, C, F, Y, ld, ld, n, m, false);
-    if (pick_segments(n, 8, 256) > 0) return launch_partition<true, 1024, 8>(st, A, B, C, F, Y, ld, ld, n, m, false);
-    return launch_partition<true, 256, 16>(st, A, B, C, F, Y, ld, ld, n, m, false);
+    return dispatch_cols(st, ProvArrays<true>{A, B, C, F, ld}, Y, ld, n, m, tile);
 }
 
 hipError_t launch_partition_rows(hipStream_t st, const double *A, const double *B, const double *C, const double *F,
                                  long ld, double *Y, long ldy, int nL, int n, bool periodic, int tile)
 {
-    if (tile == 8 && pick_segments(n, 8, 512) > 0)
-        return launch_partition<false, 512, 8>(st, A, B, C, F, Y, ld, ldy, n, nL, periodic);
-    if (pick_segments(n, 16, 256) > 0)
-        return launch_partition<false, 256, 16>(st, A, B, C, F, Y, ld, ldy, n, nL, periodic);
-    return launch_partition<false, 512, 8>(st, A, B, C, F, Y, ld, ldy, n, nL, periodic);
+    return dispatch_rows(st, ProvArrays<false>{A, B, C, F, ld}, Y, ldy, n, nL, periodic, tile);
+}
+
+bool rows_lds_supported(int n) { return pick_segments(n, RM, RB) > 0; }
+
+template <class Prov>
+static hipError_t launch_rows_lds(hipStream_t st, const Prov &prov, double *Y, long ldy, int n, int nL, bool periodic)
+{
+    const int S = pick_segments(n, RM, RB);
+    if (S == 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_rows_lds<Prov>), dim3((unsigned)nL), dim3(RB), 0, st, prov, Y, ldy, n, S, periodic ? 1 : 0);
+    return hipGetLastError();
+}
+
+// Row sweeps with the assembly staged through LDS (one row per block).
+hipError_t launch_lds_qxy(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w)
+{
+    return launch_rows_lds(st, ProvQxy{k, s, rf, w}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true);
+}
+hipError_t launch_lds_qyy(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
+                          const Work &w)
+{
+    return launch_rows_lds(st, ProvQyy{k, m, s, rf, w}, s.Qy + k.ny, k.ny, k.ny, k.nx - 1, true);
 }
 
 }  // namespace m70

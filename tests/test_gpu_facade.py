@@ -1,0 +1,58 @@
+"""The C++ drop-in Hydro (morfo70_amd/host/Hydro.h) driven like the reference's main
+loop (Grid, Bathymetry, Wave, Hydro(grid,bathy,wave), Run(n) several times), against
+the oracle on the same inputs."""
+from __future__ import annotations
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from morfo70_amd import scenarios as S
+from morfo70_amd.build import FACADE
+from tests.helpers import FIELDS, compare_states, nonuniform_beach
+
+pytestmark = pytest.mark.gpu
+
+
+def write_input(path, sc, runs, steps, solver):
+    ph = sc.physics
+    with open(path, "wb") as f:
+        np.array([sc.nx, sc.ny, runs, steps, ph.drag_model, ph.visc_model, ph.offshore_bc, solver],
+                 dtype=np.int32).tofile(f)
+        np.array([sc.dy, sc.zs0, ph.dt, ph.Dmin, ph.alpha_f, ph.cD0, ph.nu0, ph.M, ph.ka, ph.z0, ph.alpha_r],
+                 dtype=np.float64).tofile(f)
+        for a in [sc.x, sc.Z, sc.h] + [sc.forcing[k] for k in ("Sxx", "Sxy", "Syy", "Urms", "Dw", "D_r", "H")]:
+            np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+
+
+def read_output(path, nx, ny):
+    raw = np.fromfile(path, dtype=np.float64)
+    out, off = {}, 0
+    for k in FIELDS:
+        rows = nx + 1 if k in ("Qx", "Qy", "U", "V") else nx
+        out[k] = raw[off:off + rows * ny].reshape(rows, ny)
+        off += rows * ny
+    assert off == raw.size
+    return out
+
+
+@pytest.mark.parametrize("solver,exact", [(0, True), (2, False)])
+def test_facade_matches_oracle(tmp_path, solver, exact):
+    import oracle
+    assert os.path.exists(FACADE), "build first: python -m morfo70_amd.build"
+    ph = S.Physics(drag_model=S.DRAG["constant"], visc_model=S.VISC["constant"], dt=0.25)
+    sc = nonuniform_beach(nx=50, ny=44, physics=ph, seed=21)
+    inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_input(inp, sc, runs=3, steps=7, solver=solver)
+    print(f"[facade] solver={solver} 50x44 3x7 steps", flush=True)
+    subprocess.run([FACADE, str(inp), str(outp)], check=True, timeout=120)
+    got = read_output(outp, sc.nx, sc.ny)
+    o = oracle.OracleHydro(sc)
+    for _ in range(3):
+        o.run(7)
+    res = compare_states(got, o.get_state())
+    for k, r in res.items():
+        assert r["nan"] == 0, k
+        assert (r["exact"] if exact else r["rel"] <= 1e-10), (k, r)

@@ -43,12 +43,25 @@ def test_partition_vs_exact_full_size(c2):
         assert r["rel"] <= 1e-10, (k, r)
 
 
+def test_fused_equals_partition_bitwise(c2):
+    """FUSED assembles the same coefficients inside the solver threads: same bits."""
+    from morfo70_amd.hydro import from_scenario
+    out = []
+    for solver in (_lib.SOLVER_PARTITION, _lib.SOLVER_FUSED):
+        g = from_scenario(c2, solver=solver)
+        g.Run(30)
+        out.append(g.state())
+        g.close()
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k
+
+
 def test_mass_budget_full_size(c2):
     """Volume change over one step equals dt * (inflow through the offshore face):
     sum_i,j dx_x[i]*dy*(Zs_new - Zs_old) = -dt*dy*sum_j sum_i (Qx[i+1]-Qx[i] + dx_x[i]*dQy_dy)
     = dt*dy*sum_j (Qx[0] - Qx[nx]) since the alongshore differences telescope to zero."""
     from morfo70_amd.hydro import from_scenario
-    g = from_scenario(c2, solver=_lib.SOLVER_PARTITION)
+    g = from_scenario(c2, solver=_lib.SOLVER_FUSED)
     g.Run(20)
     dxx = np.concatenate([[c2.x[1] - c2.x[0]], 0.5 * (np.diff(c2.x)[1:] + np.diff(c2.x)[:-1]),
                           [c2.x[-1] - c2.x[-2]]])

@@ -91,9 +91,10 @@ def test_config1_1000_steps():
     print({k: (v["rel"], v["mismatch"]) for k, v in res.items()})
 
 
+@pytest.mark.parametrize("solver", [1, 2])
 @pytest.mark.parametrize("case", ["models", "shapes", "tide", "config1"])
-def test_partition_solver_parity(case):
-    """The PARTITION line solvers against the oracle at the 1e-10 bar."""
+def test_partition_solver_parity(case, solver):
+    """The PARTITION (1) and FUSED (2) line solvers against the oracle at the 1e-10 bar."""
     if case == "models":
         ph = S.Physics(drag_model=S.DRAG["logdepth"], visc_model=S.VISC["hrms"], dt=0.25)
         sc, steps, runs = nonuniform_beach(physics=ph, seed=11), 25, 2
@@ -104,7 +105,7 @@ def test_partition_solver_parity(case):
         sc, steps, runs = nonuniform_beach(physics=ph, tide=True, steps=40, seed=6), 20, 2
     else:
         sc, steps, runs = S.config1(steps=1000), 1000, 1
-    got, ref = run_both(sc, steps, runs=runs, solver=1)
+    got, ref = run_both(sc, steps, runs=runs, solver=solver)
     res = assert_parity(got, ref, exact=False)
     print({k: v["rel"] for k, v in res.items()})
 
