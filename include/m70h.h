@@ -51,9 +51,12 @@ extern "C" {
  * LDS, back-substitution. Same systems, different rounding: results agree with EXACT to
  * a few ulps per solve. Line lengths up to 4096. */
 #define M70H_SOLVER_PARTITION 1
-/* FUSED is PARTITION with the y-sweeps (lines along rows) assembled and solved in one
- * launch, the row's coefficients staged through LDS so they never reach HBM; bitwise
- * equal to PARTITION. ny up to 4096. */
+/* FUSED keeps the coefficients out of HBM where it can: the y-sweeps (lines along rows)
+ * are assembled and solved in one launch with the row staged through LDS, and the
+ * x-sweeps assemble and eliminate 8-row column segments in registers, solve the reduced
+ * boundary systems and back-substitute, every access coalesced across 64 columns.
+ * Same partition algorithm as PARTITION, different segment sizes on the columns. ny up
+ * to 4096. */
 #define M70H_SOLVER_FUSED 2
 
 /* Parameters read by Hydro at call time in the reference (Parameters::Hydro::dt, Dmin,

@@ -80,6 +80,15 @@ struct Work {
     uint8_t *w;                // IsWater of the predictor, reused by the projector
 };
 
+// Work arrays of the three-pass column solver: eliminated segments E0..E3 (F each),
+// reduced rows R0..R3 and their solution X (2*nseg rows of ny each).
+struct ColsWork {
+    double *E0, *E1, *E2, *E3;
+    double *R0, *R1, *R2, *R3;
+    double *X;
+    int nseg;
+};
+
 __device__ __forceinline__ int wrap_inc(int j, int ny) { return j + 1 == ny ? 0 : j + 1; }
 __device__ __forceinline__ int wrap_dec(int j, int ny) { return j == 0 ? ny - 1 : j - 1; }
 

@@ -35,6 +35,11 @@ hipError_t launch_partition_cols(hipStream_t st, const double *A, const double *
 hipError_t launch_partition_rows(hipStream_t st, const double *A, const double *B, const double *C, const double *F,
                                  long ld, double *Y, long ldy, int nL, int n, bool periodic, int tile);
 
+int cols_segments(int n);
+hipError_t launch_cols3_qxx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
+                            const Work &w, double f0, const ColsWork &cw, int tile);
+hipError_t launch_cols3_qyx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
+                            const Work &w, const ColsWork &cw, int tile);
 bool rows_lds_supported(int n);
 hipError_t launch_lds_qxy(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w);
 hipError_t launch_lds_qyy(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,

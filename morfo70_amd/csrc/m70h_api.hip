@@ -21,11 +21,11 @@ thread_local std::string g_create_error;
 
 enum KernelId {
     K_DRAG_VISC, K_RUN_FIELDS, K_QXX_ASM, K_THOMAS_QX, K_QXY_ASM, K_PERIODIC_QX, K_QX_TAIL,
-    K_QYY_ASM, K_PERIODIC_QY, K_QYX_ASM, K_THOMAS_QY, K_QY_TAIL, K_FUSED_QXY, K_FUSED_QYY, K_COUNT
+    K_QYY_ASM, K_PERIODIC_QY, K_QYX_ASM, K_THOMAS_QY, K_QY_TAIL, K_FUSED_QXY, K_FUSED_QYY, K_COLS_QX, K_COLS_QY, K_COUNT
 };
 const char *const kKernelNames[K_COUNT] = {
     "drag_viscosity", "run_fields", "qxx_assemble", "thomas_qx", "qxy_assemble", "periodic_qx", "qx_tail",
-    "qyy_assemble", "periodic_qy", "qyx_assemble", "thomas_qy", "qy_tail", "fused_qxy", "fused_qyy"};
+    "qyy_assemble", "periodic_qy", "qyx_assemble", "thomas_qy", "qy_tail", "fused_qxy", "fused_qyy", "cols3_qx", "cols3_qy"};
 
 }  // namespace
 
@@ -53,6 +53,7 @@ struct m70h {
     RunFields rf{};
     Work w{};
     double *sr = nullptr, *sv = nullptr, *sk = nullptr;
+    ColsWork cw{};
     int *flag = nullptr;
     // profiling
     bool prof = false;
@@ -168,20 +169,14 @@ int run_step(m70h *h)
     int rc;
     hipStream_t st = h->stream;
     if (h->solver == M70H_SOLVER_FUSED) {
-        // x-sweeps: coalesced assembly kernel + partition solver on the columns;
+        // x-sweeps: three coalesced passes (assemble+eliminate, reduced solve, back-sub);
         // y-sweeps: one launch each, the row's coefficients staged through LDS.
-        if ((rc = timed(h, K_QXX_ASM, [&] { return launch_qxx_assemble(st, k, h->m, s, h->rf, w, f0); }))) return rc;
-        if ((rc = timed(h, K_THOMAS_QX, [&] {
-                 return launch_partition_cols(st, w.a, w.b, w.c, w.f, s.Qx + ny, nx - 1, ny, ny, h->tile_cols);
-             })))
+        if ((rc = timed(h, K_COLS_QX, [&] { return launch_cols3_qxx(st, k, h->m, s, h->rf, w, f0, h->cw, h->tile_cols); })))
             return rc;
         if ((rc = timed(h, K_FUSED_QXY, [&] { return launch_lds_qxy(st, k, s, h->rf, w); }))) return rc;
         if ((rc = timed(h, K_QX_TAIL, [&] { return launch_qx_tail(st, k, h->m, s, w, f0); }))) return rc;
         if ((rc = timed(h, K_FUSED_QYY, [&] { return launch_lds_qyy(st, k, h->m, s, h->rf, w); }))) return rc;
-        if ((rc = timed(h, K_QYX_ASM, [&] { return launch_qyx_assemble(st, k, h->m, s, h->rf, w); }))) return rc;
-        if ((rc = timed(h, K_THOMAS_QY, [&] {
-                 return launch_partition_cols(st, w.a, w.b, w.c, w.f, s.Qy + ny, nx - 1, ny, ny, h->tile_cols);
-             })))
+        if ((rc = timed(h, K_COLS_QY, [&] { return launch_cols3_qyx(st, k, h->m, s, h->rf, w, h->cw, h->tile_cols); })))
             return rc;
         if ((rc = timed(h, K_QY_TAIL, [&] { return launch_qy_tail(st, k, s); }))) return rc;
         std::swap(s.D, s.Dn);
@@ -433,6 +428,15 @@ int m70h_create(int nx, int ny, const double *x, double dy, const m70h_params *p
         (e = dalloc(h, &h->sk, scr)) != hipSuccess)
         return bail(hip_fail(h, e, "hipMalloc scratch"));
     if ((e = dalloc(h, &h->flag, 1)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc flag"));
+    {   // three-pass column solver: E0..E3 reuse the coefficient arrays a, b, c, f
+        ColsWork &cw = h->cw;
+        cw.nseg = cols_segments(nx - 1);
+        cw.E0 = h->w.a; cw.E1 = h->w.b; cw.E2 = h->w.c; cw.E3 = h->w.f;
+        const long nr = 2L * cw.nseg * ny;
+        double **rp[] = {&cw.R0, &cw.R1, &cw.R2, &cw.R3, &cw.X};
+        for (auto q : rp)
+            if ((e = dalloc(h, q, nr)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc column work"));
+    }
     *out = h;
     return M70H_OK;
 }

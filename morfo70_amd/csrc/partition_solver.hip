@@ -58,6 +58,16 @@ struct ProvQxy {      // Qx projector, lines along y (rows): line = i, k = j
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qxy(k, s, rf, w, line, kk, a, b, c, f); }
 };
+struct ProvQxx {      // Qx predictor, lines along x (columns): line = j, k = i
+    Consts k; Metrics m; State s; RunFields rf; Work w; double f0;
+    __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
+    { asm_qxx(k, m, s, rf, w, f0, kk, line, a, b, c, f); }
+};
+struct ProvQyx {      // Qy projector, columns
+    Consts k; Metrics m; State s; RunFields rf; Work w;
+    __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
+    { asm_qyx(k, m, s, rf, w, kk, line, a, b, c, f); }
+};
 struct ProvQyy {      // Qy predictor, rows
     Consts k; Metrics m; State s; RunFields rf; Work w;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
@@ -365,6 +375,100 @@ hipError_t launch_partition_rows(hipStream_t st, const double *A, const double *
                                  long ld, double *Y, long ldy, int nL, int n, bool periodic, int tile)
 {
     return dispatch_rows(st, ProvArrays<false>{A, B, C, F, ld}, Y, ldy, n, nL, periodic, tile);
+}
+
+// ---------------------------------------------------------------------------------
+// Column lines (x-sweeps) in three coalesced passes. Thread (column j, segment s) with
+// 64 consecutive columns per wavefront, so every access is a 512-byte run:
+//   A. assemble the segment's CM rows, eliminate it, store the eliminated segment and
+//      its two reduced rows;
+//   B. solve the 2*nseg-row reduced systems of all columns (partition solver);
+//   C. back-substitute the segment from its two boundary values.
+// ---------------------------------------------------------------------------------
+constexpr int CM = 8;
+
+int cols_segments(int n) { return (n + CM - 1) / CM; }
+
+template <class Prov>
+__global__ void __launch_bounds__(256) k_cols_elim(Prov prov, ColsWork cw, int n, int ny)
+{
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int sg = blockIdx.y * 4 + threadIdx.y;
+    const int nseg = cw.nseg;
+    if (j >= ny || sg >= nseg) return;
+    const int k0 = (int)(((long)sg * n) / nseg);
+    const int k1 = (int)(((long)(sg + 1) * n) / nseg);
+    const int m = k1 - k0;   // 2..CM
+    double lo[CM], di[CM], up[CM], rh[CM];
+#pragma unroll
+    for (int q = 0; q < CM; ++q)
+        if (q < m) prov.get(j, k0 + q, di[q], lo[q], up[q], rh[q]);
+    if (k0 == 0) lo[0] = 0.0;
+    if (k1 == n) up[m - 1] = 0.0;
+    const RedRows r = seg_eliminate<CM>(m, lo, di, up, rh);
+#pragma unroll
+    for (int q = 1; q < CM - 1; ++q) {
+        if (q < m - 1) {
+            const long o = (long)(k0 + q) * ny + j;
+            cw.E0[o] = lo[q];
+            cw.E1[o] = up[q];
+            cw.E2[o] = rh[q];
+            cw.E3[o] = di[q];
+        }
+    }
+    const long oF = (long)(2 * sg) * ny + j, oL = oF + ny;
+    cw.R0[oF] = r.lo0; cw.R1[oF] = r.A_F; cw.R2[oF] = r.C_F; cw.R3[oF] = r.R_F;
+    cw.R0[oL] = r.A_L; cw.R1[oL] = r.B_L; cw.R2[oL] = r.upL; cw.R3[oL] = r.R_L;
+}
+
+__global__ void __launch_bounds__(256) k_cols_backsub(ColsWork cw, double *__restrict__ Y, int n, int ny)
+{
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int sg = blockIdx.y * 4 + threadIdx.y;
+    const int nseg = cw.nseg;
+    if (j >= ny || sg >= nseg) return;
+    const int k0 = (int)(((long)sg * n) / nseg);
+    const int k1 = (int)(((long)(sg + 1) * n) / nseg);
+    const int m = k1 - k0;
+    const double xF = cw.X[(long)(2 * sg) * ny + j];
+    const double xL = cw.X[(long)(2 * sg + 1) * ny + j];
+#pragma unroll
+    for (int q = 0; q < CM; ++q) {
+        if (q < m) {
+            const long o = (long)(k0 + q) * ny + j;
+            double x;
+            if (q == 0) x = xF;
+            else if (q == m - 1) x = xL;
+            else x = (cw.E2[o] - cw.E0[o] * xF - cw.E1[o] * xL) * cw.E3[o];
+            Y[o] = x;
+        }
+    }
+}
+
+template <class Prov>
+static hipError_t launch_cols3(hipStream_t st, const Prov &prov, const ColsWork &cw, double *Y, int n, int ny, int tile)
+{
+    if (n < 3 || cols_segments(n) != cw.nseg) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)((ny + 63) / 64), (unsigned)((cw.nseg + 3) / 4)), block(64, 4);
+    hipLaunchKernelGGL((k_cols_elim<Prov>), grid, block, 0, st, prov, cw, n, ny);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // reduced systems: diagonal R1, lower R0, upper R2, rhs R3; 2*nseg rows per column
+    e = dispatch_cols(st, ProvArrays<true>{cw.R1, cw.R0, cw.R2, cw.R3, (long)ny}, cw.X, ny, 2 * cw.nseg, ny, tile);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cols_backsub, grid, block, 0, st, cw, Y, n, ny);
+    return hipGetLastError();
+}
+
+hipError_t launch_cols3_qxx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
+                            const Work &w, double f0, const ColsWork &cw, int tile)
+{
+    return launch_cols3(st, ProvQxx{k, m, s, rf, w, f0}, cw, s.Qx + k.ny, k.nx - 1, k.ny, tile);
+}
+hipError_t launch_cols3_qyx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
+                            const Work &w, const ColsWork &cw, int tile)
+{
+    return launch_cols3(st, ProvQyx{k, m, s, rf, w}, cw, s.Qy + k.ny, k.nx - 1, k.ny, tile);
 }
 
 bool rows_lds_supported(int n) { return pick_segments(n, RM, RB) > 0; }
