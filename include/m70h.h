@@ -179,6 +179,28 @@ int m70h_profile(m70h *h, int k, const char **name, double *ms, long *launches);
  * "V_c","dQx_dx","dQy_dy"), valid until the next run. */
 int m70h_state_ptr(m70h *h, const char *name, double **dev);
 
+/* ---------------------------------------------------------------------------------
+ * Batches. The reference advances one beach per process (src/main.cpp:17-29); a batch
+ * advances many independent handles (uneven grids, each with its own state, forcing and
+ * parameters) together: every stage of the step is one launch covering all of them, and
+ * with the graph on (default) the two-step launch sequence is captured once in a hipGraph
+ * and replayed. Results are bitwise those of m70h_run with M70H_SOLVER_FUSED on each
+ * handle. The handles keep owning their buffers; get/set their state as usual between
+ * batch runs. Requirements: ny and nx-1 up to 4096; a time series of f0 is not applied
+ * step by step (its last value is used).
+ * --------------------------------------------------------------------------------- */
+typedef struct m70h_batch m70h_batch;
+int m70h_batch_create(int nscen, m70h *const *handles, m70h_batch **out);
+void m70h_batch_destroy(m70h_batch *b);
+const char *m70h_batch_last_error(const m70h_batch *b);
+int m70h_batch_set_graph(m70h_batch *b, int use_graph);
+/* The batch's hipStream_t (for events around m70h_batch_run_async). */
+int m70h_batch_stream(m70h_batch *b, void **hip_stream);
+/* Run(nIter) of every handle: per-Run pre-loop, then nIter steps. */
+int m70h_batch_run(m70h_batch *b, int nIter);
+int m70h_batch_run_async(m70h_batch *b, int nIter);
+int m70h_batch_synchronize(m70h_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,6 +48,14 @@ SIGNATURES = {
     "m70h_profile": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _dp,
                                     ctypes.POINTER(ctypes.c_long)]),
     "m70h_state_ptr": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "m70h_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "m70h_batch_destroy": (None, [_vp]),
+    "m70h_batch_last_error": (ctypes.c_char_p, [_vp]),
+    "m70h_batch_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "m70h_batch_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "m70h_batch_run": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "m70h_batch_run_async": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "m70h_batch_synchronize": (ctypes.c_int, [_vp]),
 }
 
 

@@ -89,6 +89,38 @@ struct ColsWork {
     int nseg;
 };
 
+// One scenario of a grouped (multi-scenario) launch: everything a stage needs, read by
+// the blocks of that scenario from a device-side table.
+struct ScenDesc {
+    Consts k;
+    Metrics m;
+    State s;
+    RunFields rf;
+    Forcing fw;
+    Work w;
+    ColsWork cw;
+    double f0;
+    int S_rows;   // segments per row line in the LDS row solver
+    int S_red;    // segments per reduced column system
+    int L_red;    // reduced column systems per block
+    int pad;
+};
+
+// Stages of a grouped step; start[stage][s] is the first block of scenario s.
+enum GroupStage { GS_CELLS = 0, GS_ROWS = 1, GS_COLS = 2, GS_RED = 3, GS_COUNT = 4 };
+
+// Scenario owning block b: start[] has nscen+1 ascending entries, start[0] = 0.
+__device__ __forceinline__ int scen_of(const int *__restrict__ start, int nscen, int b)
+{
+    int lo = 0, hi = nscen - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ int wrap_inc(int j, int ny) { return j + 1 == ny ? 0 : j + 1; }
 __device__ __forceinline__ int wrap_dec(int j, int ny) { return j == 0 ? ny - 1 : j - 1; }
 

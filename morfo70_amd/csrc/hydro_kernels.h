@@ -45,4 +45,12 @@ hipError_t launch_lds_qxy(hipStream_t st, const Consts &k, const State &s, const
 hipError_t launch_lds_qyy(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
                           const Work &w);
 
+// grouped (multi-scenario) launches
+int grouped_cell_blocks(int nx, int ny);
+hipError_t launch_cells_grouped(hipStream_t st, int op, const ScenDesc *D, const int *start, int nscen, int blocks);
+void grouped_stage_blocks(int nx, int ny, int *rows, int *cols, int *red, int *S_rows, int *S_red, int *L_red);
+hipError_t launch_rows_grouped(hipStream_t st, int which, const ScenDesc *D, const int *start, int nscen, int blocks);
+hipError_t launch_cols3_grouped(hipStream_t st, int which, const ScenDesc *D, const int *start_cols,
+                                const int *start_red, int nscen, int blocks_cols, int blocks_red);
+
 }  // namespace m70

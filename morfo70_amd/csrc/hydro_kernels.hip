@@ -19,16 +19,23 @@ namespace m70 {
     const int ny = k.ny;                                 \
     const long r = (long)i * ny;
 
+#define CELL_IJ_ONLY(nrows)                              \
+    const int j = blockIdx.x * blockDim.x + threadIdx.x; \
+    const int i = blockIdx.y;                            \
+    if (j >= k.ny || i >= (nrows)) return;
+
 // ------------------------------------------------------------------------------------
 // Per-Run fields, src/Hydro.cpp:175-192
 // ------------------------------------------------------------------------------------
 
 // Hydro::calculateDragAndViscosity with the model functions,
 // src/HydroParametrizations.cpp:30-113. Rows 0..nx-1.
-__global__ void __launch_bounds__(256) k_drag_viscosity(Consts k, const double *__restrict__ D,
-                                                        Forcing fw, RunFields rf)
+__device__ __forceinline__ void drag_viscosity_body(const Consts &k, const double *__restrict__ D, const Forcing &fw,
+                                                    const RunFields &rf, int i, int j)
 {
-    CELL_IJ(k.nx)
+    const int ny = k.ny;
+    const long r = (long)i * ny;
+    (void)r;
     const long q = r + j;
     const double d = D[q];
     double cd;
@@ -56,11 +63,20 @@ __global__ void __launch_bounds__(256) k_drag_viscosity(Consts k, const double *
     rf.nuD[q] = nud;
 }
 
+__global__ void __launch_bounds__(256) k_drag_viscosity(Consts k, const double *__restrict__ D, Forcing fw, RunFields rf)
+{
+    CELL_IJ_ONLY(k.nx)
+    drag_viscosity_body(k, D, fw, rf, i, j);
+}
+
 // Forcing, face averages and viscosity derivatives. Rows 0..nx-1 (x-face fields only
 // for rows 0..nx-2).
-__global__ void __launch_bounds__(256) k_run_fields(Consts k, Metrics m, Forcing fw, RunFields rf)
+__device__ __forceinline__ void run_fields_body(const Consts &k, const Metrics &m, const Forcing &fw, const RunFields &rf,
+                                                int i, int j)
 {
-    CELL_IJ(k.nx)
+    const int ny = k.ny;
+    const long r = (long)i * ny;
+    (void)r;
     const int nx = k.nx;
     const int jp = wrap_inc(j, ny), jm = wrap_dec(j, ny);
     const double *nuD = rf.nuD;
@@ -126,6 +142,12 @@ __global__ void __launch_bounds__(256) k_run_fields(Consts k, Metrics m, Forcing
     }
 }
 
+__global__ void __launch_bounds__(256) k_run_fields(Consts k, Metrics m, Forcing fw, RunFields rf)
+{
+    CELL_IJ_ONLY(k.nx)
+    run_fields_body(k, m, fw, rf, i, j);
+}
+
 // ------------------------------------------------------------------------------------
 // Qx predictor assembly: src/HydroQx.cpp:23-48 with the coefficient functions of
 // :99-256 and setSolverQxx_BC (src/HydroBoundaryConditions.cpp:25-39). Rows 0..nx-2.
@@ -161,9 +183,12 @@ __device__ __forceinline__ double qx_at(const State &s, const Consts &k, long ro
     return row == 0 ? k.f1 * s.Qx[k.ny + j] + f0 : s.Qx[rowoff + j];
 }
 
-__global__ void __launch_bounds__(256) k_qx_tail(Consts k, Metrics m, State s, Work w, double f0)
+__device__ __forceinline__ void qx_tail_body(const Consts &k, const Metrics &m, const State &s, const Work &w, double f0,
+                                             int i, int j)
 {
-    CELL_IJ(k.nx)
+    const int ny = k.ny;
+    const long r = (long)i * ny;
+    (void)r;
     const int nx = k.nx;
     const double dt = k.dt;
     // Qx rows i-1..i+2 (row 0 through the BC, row nx is the wall row held in memory)
@@ -205,6 +230,12 @@ __global__ void __launch_bounds__(256) k_qx_tail(Consts k, Metrics m, State s, W
     s.U[r + j] = ui;
     s.Uc[r + j] = m.x2c_d[i] * ui + m.x2c_up[i] * ui1;
     if (i == 0) s.Qx[j] = q[1];
+}
+
+__global__ void __launch_bounds__(256) k_qx_tail(Consts k, Metrics m, State s, Work w, double f0)
+{
+    CELL_IJ_ONLY(k.nx)
+    qx_tail_body(k, m, s, w, f0, i, j);
 }
 
 // ------------------------------------------------------------------------------------
@@ -256,9 +287,11 @@ __device__ __forceinline__ double new_depth(const State &s, const Consts &k, int
     return s.D[q] + incrZs;
 }
 
-__global__ void __launch_bounds__(256) k_qy_tail(Consts k, State s)
+__device__ __forceinline__ void qy_tail_body(const Consts &k, const State &s, int i, int j)
 {
-    CELL_IJ(k.nx)
+    const int ny = k.ny;
+    const long r = (long)i * ny;
+    (void)r;
     const int nx = k.nx;
     const int jp = wrap_inc(j, ny), jm = wrap_dec(j, ny);
     const double dq = dqy_at(s, k, i, j);
@@ -283,6 +316,12 @@ __global__ void __launch_bounds__(256) k_qy_tail(Consts k, State s)
     if (i == nx - 1) s.Qy[(long)nx * ny + j] = s.Qy[(long)(nx - 1) * ny + j] * k.inv_3;
 }
 
+__global__ void __launch_bounds__(256) k_qy_tail(Consts k, State s)
+{
+    CELL_IJ_ONLY(k.nx)
+    qy_tail_body(k, s, i, j);
+}
+
 // ------------------------------------------------------------------------------------
 // Non-finite scan of a field; sets *flag to 1 if any element is NaN or Inf.
 // ------------------------------------------------------------------------------------
@@ -304,6 +343,42 @@ __global__ void __launch_bounds__(256) k_init_zs(double *Zs, const double *D, co
 {
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < n) Zs[q] = D[q] - Z[q] + hb[q];
+}
+
+// ------------------------------------------------------------------------------------
+// Grouped cell kernels: one launch covers every scenario of a batch. Block b belongs to
+// scenario scen_of(start, nscen, b); within it blocks tile (ceil(ny/256) x nx).
+// ------------------------------------------------------------------------------------
+template <int OP>
+__global__ void __launch_bounds__(256) k_cells_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
+                                                       int nscen)
+{
+    const int b = blockIdx.x;
+    const int sid = scen_of(start, nscen, b);
+    const ScenDesc &d = D[sid];
+    const int ny = d.k.ny;
+    const int lb = b - start[sid];
+    const int gx = (ny + 255) / 256;
+    const int i = lb / gx;
+    const int j = (lb - i * gx) * 256 + threadIdx.x;
+    if (j >= ny || i >= d.k.nx) return;
+    if (OP == 0) drag_viscosity_body(d.k, d.s.D, d.fw, d.rf, i, j);
+    else if (OP == 1) run_fields_body(d.k, d.m, d.fw, d.rf, i, j);
+    else if (OP == 2) qx_tail_body(d.k, d.m, d.s, d.w, d.f0, i, j);
+    else qy_tail_body(d.k, d.s, i, j);
+}
+
+int grouped_cell_blocks(int nx, int ny) { return ((ny + 255) / 256) * nx; }
+
+hipError_t launch_cells_grouped(hipStream_t st, int op, const ScenDesc *D, const int *start, int nscen, int blocks)
+{
+    switch (op) {
+    case 0: hipLaunchKernelGGL(k_cells_grouped<0>, dim3(blocks), dim3(256), 0, st, D, start, nscen); break;
+    case 1: hipLaunchKernelGGL(k_cells_grouped<1>, dim3(blocks), dim3(256), 0, st, D, start, nscen); break;
+    case 2: hipLaunchKernelGGL(k_cells_grouped<2>, dim3(blocks), dim3(256), 0, st, D, start, nscen); break;
+    default: hipLaunchKernelGGL(k_cells_grouped<3>, dim3(blocks), dim3(256), 0, st, D, start, nscen); break;
+    }
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------------------

@@ -724,3 +724,194 @@ int m70h_state_ptr(m70h *h, const char *name, double **dev)
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------------------
+// Batches: many handles advanced by grouped launches (see include/m70h.h).
+// ------------------------------------------------------------------------------------
+struct m70h_batch {
+    std::vector<m70h *> hs;
+    hipStream_t stream = nullptr;
+    ScenDesc *d_desc = nullptr;   // [2][nscen]: even steps (D, Dn) and odd steps (swapped)
+    int *d_start = nullptr;       // [GS_COUNT][nscen+1]
+    int blocks[GS_COUNT] = {};
+    int cell_blocks = 0;
+    bool use_graph = true;
+    hipGraphExec_t gexec = nullptr;
+    std::string err;
+};
+
+static int bfail(m70h_batch *b, int code, const std::string &msg)
+{
+    if (b) b->err = msg; else g_create_error = msg;
+    return code;
+}
+
+#define BHIPCHK(b, expr)                                                                      \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess) return bfail(b, M70H_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+static void fill_desc(const m70h *h, ScenDesc &d, bool odd)
+{
+    int rows, cols, red, S_rows, S_red, L_red;
+    grouped_stage_blocks(h->nx, h->ny, &rows, &cols, &red, &S_rows, &S_red, &L_red);
+    d.k = h->k; d.m = h->m; d.s = h->s; d.rf = h->rf; d.fw = h->fw; d.w = h->w; d.cw = h->cw;
+    if (odd) std::swap(d.s.D, d.s.Dn);
+    d.f0 = h->f0_series.empty() ? h->f0 : h->f0_series.back();
+    d.S_rows = S_rows; d.S_red = S_red; d.L_red = L_red; d.pad = 0;
+}
+
+static int batch_upload(m70h_batch *b)
+{
+    const int n = (int)b->hs.size();
+    std::vector<ScenDesc> host(2 * (size_t)n);
+    for (int q = 0; q < n; ++q) {
+        fill_desc(b->hs[q], host[q], false);
+        fill_desc(b->hs[q], host[n + q], true);
+    }
+    BHIPCHK(b, hipMemcpyAsync(b->d_desc, host.data(), host.size() * sizeof(ScenDesc), hipMemcpyHostToDevice, b->stream));
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    return M70H_OK;
+}
+
+static hipError_t batch_step(m70h_batch *b, int odd)
+{
+    const int n = (int)b->hs.size();
+    const ScenDesc *D = b->d_desc + (odd ? n : 0);
+    const int *st_cells = b->d_start + GS_CELLS * (n + 1), *st_rows = b->d_start + GS_ROWS * (n + 1);
+    const int *st_cols = b->d_start + GS_COLS * (n + 1), *st_red = b->d_start + GS_RED * (n + 1);
+    hipStream_t st = b->stream;
+    hipError_t e;
+    if ((e = launch_cols3_grouped(st, 0, D, st_cols, st_red, n, b->blocks[GS_COLS], b->blocks[GS_RED])) != hipSuccess)
+        return e;
+    if ((e = launch_rows_grouped(st, 0, D, st_rows, n, b->blocks[GS_ROWS])) != hipSuccess) return e;
+    if ((e = launch_cells_grouped(st, 2, D, st_cells, n, b->blocks[GS_CELLS])) != hipSuccess) return e;
+    if ((e = launch_rows_grouped(st, 1, D, st_rows, n, b->blocks[GS_ROWS])) != hipSuccess) return e;
+    if ((e = launch_cols3_grouped(st, 1, D, st_cols, st_red, n, b->blocks[GS_COLS], b->blocks[GS_RED])) != hipSuccess)
+        return e;
+    return launch_cells_grouped(st, 3, D, st_cells, n, b->blocks[GS_CELLS]);
+}
+
+extern "C" {
+
+const char *m70h_batch_last_error(const m70h_batch *b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+int m70h_batch_create(int nscen, m70h *const *handles, m70h_batch **out)
+{
+    if (!out) return bfail(nullptr, M70H_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (nscen < 1 || !handles) return bfail(nullptr, M70H_EINVAL, "need at least one handle");
+    m70h_batch *b = new m70h_batch();
+    auto bail = [&](int rc) { g_create_error = b->err; m70h_batch_destroy(b); return rc; };
+    int dev0 = -1;
+    std::vector<int> starts((size_t)GS_COUNT * (nscen + 1), 0);
+    for (int q = 0; q < nscen; ++q) {
+        m70h *h = handles[q];
+        if (!h) return bail(bfail(b, M70H_EINVAL, "NULL handle"));
+        if (dev0 < 0) dev0 = h->device;
+        if (h->device != dev0) return bail(bfail(b, M70H_EINVAL, "handles on different devices"));
+        if (!rows_lds_supported(h->ny) || !partition_supported(h->nx - 1))
+            return bail(bfail(b, M70H_EINVAL, "batched step supports ny and nx-1 up to 4096"));
+        int rows, cols, red, S_rows, S_red, L_red;
+        grouped_stage_blocks(h->nx, h->ny, &rows, &cols, &red, &S_rows, &S_red, &L_red);
+        if (S_rows <= 0 || S_red <= 0) return bail(bfail(b, M70H_EINVAL, "grid too small for the batched step"));
+        const int per[GS_COUNT] = {grouped_cell_blocks(h->nx, h->ny), rows, cols, red};
+        for (int gs = 0; gs < GS_COUNT; ++gs)
+            starts[(size_t)gs * (nscen + 1) + q + 1] = starts[(size_t)gs * (nscen + 1) + q] + per[gs];
+        b->hs.push_back(h);
+    }
+    for (int gs = 0; gs < GS_COUNT; ++gs) b->blocks[gs] = starts[(size_t)gs * (nscen + 1) + nscen];
+    hipError_t e;
+    if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipMalloc(&b->d_desc, 2 * (size_t)nscen * sizeof(ScenDesc))) != hipSuccess ||
+        (e = hipMalloc(&b->d_start, starts.size() * sizeof(int))) != hipSuccess ||
+        (e = hipMemcpy(b->d_start, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
+        return bail(bfail(b, M70H_EHIP, std::string("batch allocation: ") + hipGetErrorString(e)));
+    *out = b;
+    return M70H_OK;
+}
+
+void m70h_batch_destroy(m70h_batch *b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->d_desc) (void)hipFree(b->d_desc);
+    if (b->d_start) (void)hipFree(b->d_start);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int m70h_batch_set_graph(m70h_batch *b, int use_graph)
+{
+    if (!b) return M70H_EINVAL;
+    b->use_graph = use_graph != 0;
+    return M70H_OK;
+}
+
+int m70h_batch_stream(m70h_batch *b, void **hip_stream)
+{
+    if (!b || !hip_stream) return M70H_EINVAL;
+    *hip_stream = (void *)b->stream;
+    return M70H_OK;
+}
+
+int m70h_batch_run_async(m70h_batch *b, int nIter)
+{
+    if (!b) return M70H_EINVAL;
+    if (nIter < 0) return bfail(b, M70H_EINVAL, "nIter < 0");
+    for (m70h *h : b->hs) {
+        if (!h->forcing_set) return bfail(b, M70H_ESTATE, "forcing not set on a handle");
+        BHIPCHK(b, hipStreamSynchronize(h->stream));   // the handles' own queued work first
+    }
+    int rc = batch_upload(b);
+    if (rc) return rc;
+    const int n = (int)b->hs.size();
+    hipStream_t st = b->stream;
+    // per-Run pre-loop (src/Hydro.cpp:175-192) of every scenario
+    BHIPCHK(b, launch_cells_grouped(st, 0, b->d_desc, b->d_start + GS_CELLS * (n + 1), n, b->blocks[GS_CELLS]));
+    BHIPCHK(b, launch_cells_grouped(st, 1, b->d_desc, b->d_start + GS_CELLS * (n + 1), n, b->blocks[GS_CELLS]));
+    int it = 0;
+    if (b->use_graph && nIter >= 2) {
+        if (!b->gexec) {   // two steps: even (D -> Dn) and odd (Dn -> D)
+            hipGraph_t g = nullptr;
+            BHIPCHK(b, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            hipError_t e1 = batch_step(b, 0);
+            hipError_t e2 = e1 == hipSuccess ? batch_step(b, 1) : e1;
+            hipError_t e3 = hipStreamEndCapture(st, &g);
+            if (e2 != hipSuccess || e3 != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return bfail(b, M70H_EHIP, std::string("graph capture: ") + hipGetErrorString(e2 != hipSuccess ? e2 : e3));
+            }
+            hipError_t e4 = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e4 != hipSuccess) {
+                b->gexec = nullptr;
+                return bfail(b, M70H_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e4));
+            }
+        }
+        for (; it + 2 <= nIter; it += 2) BHIPCHK(b, hipGraphLaunch(b->gexec, st));
+    }
+    for (; it < nIter; ++it) BHIPCHK(b, batch_step(b, it & 1));
+    if (nIter & 1)
+        for (m70h *h : b->hs) std::swap(h->s.D, h->s.Dn);
+    for (m70h *h : b->hs) h->f0_pos += nIter;
+    return M70H_OK;
+}
+
+int m70h_batch_synchronize(m70h_batch *b)
+{
+    if (!b) return M70H_EINVAL;
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    return M70H_OK;
+}
+
+int m70h_batch_run(m70h_batch *b, int nIter)
+{
+    int rc = m70h_batch_run_async(b, nIter);
+    if (rc) return rc;
+    return m70h_batch_synchronize(b);
+}
+
+}  // extern "C"

@@ -214,16 +214,11 @@ __device__ __forceinline__ double seg_value(int q, int m, const double (&lo)[SEG
 // t / L) so that a wavefront reads runs of consecutive columns; ROWS: lines are rows
 // (element (line, k) at line*ld + k) and t maps to (line t / S, segment t % S).
 template <bool COLS, int BLOCK, int SEG_M, class Prov>
-__global__ void __launch_bounds__(BLOCK) k_partition_solve(Prov prov, double *__restrict__ Y, LineGeom g)
+__device__ __forceinline__ void partition_body(const Prov &prov, double *__restrict__ Y, const LineGeom &g, int blk,
+                                               double (*red)[BLOCK * 2])
 {
-    __shared__ double red[4][BLOCK * 2];   // reduced rows: lower, diag, upper, rhs
     const int t = threadIdx.x;
     const int S = g.S, L = g.lines_per_block;
-    int blk = blockIdx.x;
-    if (g.xcd_remap) {
-        const int nb = gridDim.x;
-        if ((nb & 7) == 0) blk = (blk & 7) * (nb >> 3) + (blk >> 3);
-    }
     const int lloc = COLS ? (t % L) : (t / S);
     const int seg = COLS ? (t / L) : (t % S);
     const int line = blk * L + lloc;
@@ -254,6 +249,18 @@ __global__ void __launch_bounds__(BLOCK) k_partition_solve(Prov prov, double *__
     }
 }
 
+template <bool COLS, int BLOCK, int SEG_M, class Prov>
+__global__ void __launch_bounds__(BLOCK) k_partition_solve(Prov prov, double *__restrict__ Y, LineGeom g)
+{
+    __shared__ double red[4][BLOCK * 2];   // reduced rows: lower, diag, upper, rhs
+    int blk = blockIdx.x;
+    if (g.xcd_remap) {
+        const int nb = gridDim.x;
+        if ((nb & 7) == 0) blk = (blk & 7) * (nb >> 3) + (blk >> 3);
+    }
+    partition_body<COLS, BLOCK, SEG_M>(prov, Y, g, blk, red);
+}
+
 // Row lines staged through LDS: one line (n <= 4096) per block of RB threads. The block
 // assembles the line's coefficients in coalesced order into LDS (element e at
 // e + e/8, so that the per-thread segment reads below are free of bank conflicts),
@@ -266,12 +273,10 @@ constexpr int RLDS = RNMAX + RNMAX / 8;
 __device__ __forceinline__ int lpad(int e) { return e + (e >> 3); }
 
 template <class Prov>
-__global__ void __launch_bounds__(RB) k_rows_lds(Prov prov, double *__restrict__ Y, long ldy, int n, int S,
-                                                 int periodic)
+__device__ __forceinline__ void rows_lds_body(const Prov &prov, double *__restrict__ Y, long ldy, int n, int S,
+                                              int periodic, int line, double (*sh)[RLDS])
 {
-    __shared__ double sh[4][RLDS];
     const int t = threadIdx.x;
-    const int line = blockIdx.x;
     for (int e = t; e < n; e += RB) {
         double a, b, c, f;
         prov.get(line, e, a, b, c, f);
@@ -312,6 +317,14 @@ __global__ void __launch_bounds__(RB) k_rows_lds(Prov prov, double *__restrict__
     __syncthreads();
     double *yl = Y + (long)line * ldy;
     for (int e = t; e < n; e += RB) yl[e] = sh[0][lpad(e)];
+}
+
+template <class Prov>
+__global__ void __launch_bounds__(RB) k_rows_lds(Prov prov, double *__restrict__ Y, long ldy, int n, int S,
+                                                 int periodic)
+{
+    __shared__ double sh[4][RLDS];
+    rows_lds_body(prov, Y, ldy, n, S, periodic, (int)blockIdx.x, sh);
 }
 
 // Picks S (power of two, n >= 2S, ceil(n/S) <= M) for a line length n; 0 if none.
@@ -390,10 +403,10 @@ constexpr int CM = 8;
 int cols_segments(int n) { return (n + CM - 1) / CM; }
 
 template <class Prov>
-__global__ void __launch_bounds__(256) k_cols_elim(Prov prov, ColsWork cw, int n, int ny)
+__device__ __forceinline__ void cols_elim_body(const Prov &prov, const ColsWork &cw, int n, int ny, int bx, int by)
 {
-    const int j = blockIdx.x * 64 + threadIdx.x;
-    const int sg = blockIdx.y * 4 + threadIdx.y;
+    const int j = bx * 64 + threadIdx.x;
+    const int sg = by * 4 + threadIdx.y;
     const int nseg = cw.nseg;
     if (j >= ny || sg >= nseg) return;
     const int k0 = (int)(((long)sg * n) / nseg);
@@ -421,10 +434,11 @@ __global__ void __launch_bounds__(256) k_cols_elim(Prov prov, ColsWork cw, int n
     cw.R0[oL] = r.A_L; cw.R1[oL] = r.B_L; cw.R2[oL] = r.upL; cw.R3[oL] = r.R_L;
 }
 
-__global__ void __launch_bounds__(256) k_cols_backsub(ColsWork cw, double *__restrict__ Y, int n, int ny)
+__device__ __forceinline__ void cols_backsub_body(const ColsWork &cw, double *__restrict__ Y, int n, int ny, int bx,
+                                                  int by)
 {
-    const int j = blockIdx.x * 64 + threadIdx.x;
-    const int sg = blockIdx.y * 4 + threadIdx.y;
+    const int j = bx * 64 + threadIdx.x;
+    const int sg = by * 4 + threadIdx.y;
     const int nseg = cw.nseg;
     if (j >= ny || sg >= nseg) return;
     const int k0 = (int)(((long)sg * n) / nseg);
@@ -443,6 +457,17 @@ __global__ void __launch_bounds__(256) k_cols_backsub(ColsWork cw, double *__res
             Y[o] = x;
         }
     }
+}
+
+template <class Prov>
+__global__ void __launch_bounds__(256) k_cols_elim(Prov prov, ColsWork cw, int n, int ny)
+{
+    cols_elim_body(prov, cw, n, ny, (int)blockIdx.x, (int)blockIdx.y);
+}
+
+__global__ void __launch_bounds__(256) k_cols_backsub(ColsWork cw, double *__restrict__ Y, int n, int ny)
+{
+    cols_backsub_body(cw, Y, n, ny, (int)blockIdx.x, (int)blockIdx.y);
 }
 
 template <class Prov>
@@ -472,6 +497,124 @@ hipError_t launch_cols3_qyx(hipStream_t st, const Consts &k, const Metrics &m, c
 }
 
 bool rows_lds_supported(int n) { return pick_segments(n, RM, RB) > 0; }
+
+// ---------------------------------------------------------------------------------
+// Grouped (multi-scenario) versions of the FUSED stages. Providers read the scenario's
+// descriptor in place.
+// ---------------------------------------------------------------------------------
+struct GProvQxx {
+    const ScenDesc *d;
+    __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
+    { asm_qxx(d->k, d->m, d->s, d->rf, d->w, d->f0, kk, line, a, b, c, f); }
+};
+struct GProvQyx {
+    const ScenDesc *d;
+    __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
+    { asm_qyx(d->k, d->m, d->s, d->rf, d->w, kk, line, a, b, c, f); }
+};
+struct GProvQxy {
+    const ScenDesc *d;
+    __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
+    { asm_qxy(d->k, d->s, d->rf, d->w, line, kk, a, b, c, f); }
+};
+struct GProvQyy {
+    const ScenDesc *d;
+    __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
+    { asm_qyy(d->k, d->m, d->s, d->rf, d->w, line, kk, a, b, c, f); }
+};
+
+template <int WHICH>   // 0: Qx projector rows, 1: Qy predictor rows
+__global__ void __launch_bounds__(RB) k_rows_lds_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
+                                                         int nscen)
+{
+    __shared__ double sh[4][RLDS];
+    const int b = blockIdx.x;
+    const int sid = scen_of(start, nscen, b);
+    const ScenDesc *d = D + sid;
+    const int ny = d->k.ny;
+    const int line = b - start[sid];
+    if (WHICH == 0) rows_lds_body(GProvQxy{d}, d->s.Qx + ny, ny, ny, d->S_rows, 1, line, sh);
+    else rows_lds_body(GProvQyy{d}, d->s.Qy + ny, ny, ny, d->S_rows, 1, line, sh);
+}
+
+template <int WHICH>   // 0: Qx predictor columns, 1: Qy projector columns
+__global__ void __launch_bounds__(256) k_cols_elim_grouped(const ScenDesc *__restrict__ D,
+                                                           const int *__restrict__ start, int nscen)
+{
+    const int b = blockIdx.x;
+    const int sid = scen_of(start, nscen, b);
+    const ScenDesc *d = D + sid;
+    const int ny = d->k.ny;
+    const int lb = b - start[sid];
+    const int gx = (ny + 63) / 64;
+    const int by = lb / gx, bx = lb - by * gx;
+    if (WHICH == 0) cols_elim_body(GProvQxx{d}, d->cw, d->k.nx - 1, ny, bx, by);
+    else cols_elim_body(GProvQyx{d}, d->cw, d->k.nx - 1, ny, bx, by);
+}
+
+template <int WHICH>
+__global__ void __launch_bounds__(256) k_cols_backsub_grouped(const ScenDesc *__restrict__ D,
+                                                              const int *__restrict__ start, int nscen)
+{
+    const int b = blockIdx.x;
+    const int sid = scen_of(start, nscen, b);
+    const ScenDesc *d = D + sid;
+    const int ny = d->k.ny;
+    const int lb = b - start[sid];
+    const int gx = (ny + 63) / 64;
+    const int by = lb / gx, bx = lb - by * gx;
+    cols_backsub_body(d->cw, (WHICH == 0 ? d->s.Qx : d->s.Qy) + ny, d->k.nx - 1, ny, bx, by);
+}
+
+// Reduced column systems of every scenario (partition solver, 8 unknowns per thread).
+__global__ void __launch_bounds__(1024) k_red_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
+                                                      int nscen)
+{
+    __shared__ double red[4][1024 * 2];
+    const int b = blockIdx.x;
+    const int sid = scen_of(start, nscen, b);
+    const ScenDesc *d = D + sid;
+    const int ny = d->k.ny;
+    LineGeom g{};
+    g.ld = ny; g.ldy = ny; g.n = 2 * d->cw.nseg; g.nL = ny; g.S = d->S_red; g.lines_per_block = d->L_red;
+    g.periodic = 0;
+    partition_body<true, 1024, 8>(ProvArrays<true>{d->cw.R1, d->cw.R0, d->cw.R2, d->cw.R3, (long)ny}, d->cw.X, g,
+                                  b - start[sid], red);
+}
+
+// Block counts of the grouped stages for one scenario.
+void grouped_stage_blocks(int nx, int ny, int *rows, int *cols, int *red, int *S_rows, int *S_red, int *L_red)
+{
+    *rows = nx - 1;
+    *S_rows = pick_segments(ny, RM, RB);
+    const int nseg = cols_segments(nx - 1);
+    *cols = ((ny + 63) / 64) * ((nseg + 3) / 4);
+    *S_red = pick_segments(2 * nseg, 8, 256);
+    *L_red = *S_red > 0 ? 1024 / *S_red : 0;
+    *red = *L_red > 0 ? (ny + *L_red - 1) / *L_red : 0;
+}
+
+hipError_t launch_rows_grouped(hipStream_t st, int which, const ScenDesc *D, const int *start, int nscen, int blocks)
+{
+    if (which == 0) hipLaunchKernelGGL(k_rows_lds_grouped<0>, dim3(blocks), dim3(RB), 0, st, D, start, nscen);
+    else hipLaunchKernelGGL(k_rows_lds_grouped<1>, dim3(blocks), dim3(RB), 0, st, D, start, nscen);
+    return hipGetLastError();
+}
+
+hipError_t launch_cols3_grouped(hipStream_t st, int which, const ScenDesc *D, const int *start_cols,
+                                const int *start_red, int nscen, int blocks_cols, int blocks_red)
+{
+    if (which == 0) hipLaunchKernelGGL(k_cols_elim_grouped<0>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
+    else hipLaunchKernelGGL(k_cols_elim_grouped<1>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_red_grouped, dim3(blocks_red), dim3(1024), 0, st, D, start_red, nscen);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (which == 0) hipLaunchKernelGGL(k_cols_backsub_grouped<0>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
+    else hipLaunchKernelGGL(k_cols_backsub_grouped<1>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
+    return hipGetLastError();
+}
 
 template <class Prov>
 static hipError_t launch_rows_lds(hipStream_t st, const Prov &prov, double *Y, long ldy, int n, int nL, bool periodic)
