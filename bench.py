@@ -26,21 +26,33 @@ HBM_PEAK_GBS = 8000.0       # MI355X HBM3E, GB/s
 B_ALG_STEP = 360.0          # algorithmic bytes per cell-step of the whole step
 
 # Algorithmic bytes per cell of each kernel: its compulsory FP64 inputs read once and
-# outputs written once (masks as 1 byte). Derivation per kernel in DESIGN.md.
+# outputs written once (masks as 1 byte); intermediates a kernel hands to the next one
+# (eliminated segments E0..E3, reduced rows, their solution X) count as its outputs and
+# as the next kernel's inputs. Cells are the (nx-1) x ny flux faces unless noted.
 KERNEL_BYTES_PER_CELL = {
     "qxx_assemble": 8 * 14 + 8 * 5 + 1,     # 14 fields in; a,b,c,f,drag out; mask out
     "thomas_qx": 8 * 5,                     # a,b,c,f in; Qx* out
     "qxy_assemble": 8 * 8 + 1 + 8 * 4,      # D,Vc,Qx*,nuD_x,dnuDx_dy,drag (+2 rows) in, mask in; a,b,c,f out
     "periodic_qx": 8 * 5,                   # a,b,c,f in; Qx out
-    "qx_tail": 8 * 4 + 8 * 6,               # Qx,D,dQy,Zs in; Qx0,dQx,gZs,gD,U,Uc out
+    "qx_tail": 8 * 4 + 8 * 6,               # Qx,D,dQy,Zs in; Qx0,dQx,gZs,gD,U,Uc out (nx rows)
     "qyy_assemble": 8 * 17 + 8 * 5 + 1,     # gD,gZs,dQy,Zs,dQx,U(2),Qy,Uc,V,Fy,nuD_y,dnuDy_dx,dnuD_dy,cD_y,Urms_y in
     "periodic_qy": 8 * 5,
     "qyx_assemble": 8 * 11 + 1 + 8 * 4,
     "thomas_qy": 8 * 5,
-    "qy_tail": 8 * 5 + 8 * 5,               # Qy,dQx,D,Zs in; dQy,Zs,D,V,Vc out
-    # fused sweeps: the assembly inputs plus the solution (and side outputs) only
+    "qy_tail": 8 * 5 + 8 * 5,               # Qy,dQx,D,Zs in; dQy,Zs,D,V,Vc out (nx rows)
+    # FUSED family: y-sweeps in one launch (assembly inputs in, solution out)
     "fused_qxy": 8 * 8 + 1 + 8,             # D(2),Vc(2),Qx*,nuD_x,dnuDx_dy,drag in, mask in; Qx out
     "fused_qyy": 8 * 17 + 8 * 2 + 1,        # the qyy_assemble inputs; Qy*, drag out; mask out
+    # FUSED x-sweeps, pass 1: assembly inputs in; drag, mask, E0..E3 out, plus the two
+    # reduced rows (4 values each) per 8-row segment = 8 B per cell
+    "cols_qx_elim": 8 * 14 + 8 + 1 + 8 * 4 + 8,
+    "cols_qy_elim": 8 * 11 + 1 + 8 * 4 + 8,
+    # pass 3: E0..E3 and the two interface values per segment in, the solution out
+    "cols_qx_backsub": 8 * 4 + 2 + 8,
+    "cols_qy_backsub": 8 * 4 + 2 + 8,
+    # pass 2: 2 reduced rows per 8-row segment, (lo, diag, up, rhs) in and x out = 10 B per cell
+    "cols_qx_reduced": 10,
+    "cols_qy_reduced": 10,
 }
 
 
@@ -53,12 +65,42 @@ def parse():
     ap.add_argument("--ny", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=120)
+    ap.add_argument("--cpu-steps", type=int, default=300)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--solver", choices=["fused", "partition", "exact"], default="fused")
     ap.add_argument("--tile-cols", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--workload", choices=["tide2048x4096", "batch64"], default="tide2048x4096")
+    ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-sized device buffer 5 times (reference kernel "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
     return ap.parse_args()
+
+
+def copy_calibration(mib: int):
+    """A plain device-to-device copy of known size, for calibrating HBM counters."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 5}), flush=True)
+
+
+def measured_traffic(kernel: str, nx: int, ny: int, solver: str):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes (each scaled by what the counter reads for a plain copy of known
+    size); None when no such measurement exists for this workload."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_tide{nx}x{ny}.json")
+    if solver != "fused" or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f)["kernels"].get(kernel)
+    return None if k is None else k["hbm_bytes"]
 
 
 def cpu_baseline(sc, steps: int, threads: int) -> dict:
@@ -78,8 +120,85 @@ def cpu_baseline(sc, steps: int, threads: int) -> dict:
                       f"{threads} OpenMP threads, {dt:.2f} s"}
 
 
+def bench_batch(args):
+    """64 uneven beaches (nx ~ U[128,1024], ny ~ U[256,2048], seeded bathymetry and
+    forcing): aggregate cell-steps/s of looping the plain step over them one after the
+    other, against the batched step (grouped launches, hipGraph replay)."""
+    import torch
+    from morfo70_amd import _lib
+    from morfo70_amd import scenarios as S
+    from morfo70_amd.batch import Batch
+    from morfo70_amd.hydro import from_scenario
+
+    t0 = time.perf_counter()
+    scs = S.batch_scenarios(args.batch_count, seed=3, steps=args.steps + args.warmup + 1)
+    gen_s = time.perf_counter() - t0
+    cells = sum(sc.cells for sc in scs)
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    stream = torch.cuda.Stream(device=dev)
+    hs = [from_scenario(sc, solver=_lib.SOLVER_FUSED) for sc in scs]
+    for h in hs:
+        h.set_stream(stream.cuda_stream)
+        h.set_forcing()
+    # looping the plain step: each scenario's Run(K), one after the other, one stream
+    for h in hs:
+        h.run_async(args.warmup)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for h in hs:
+        h.run_async(args.steps)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    loop_ms = ev0.elapsed_time(ev1)
+    # batched, schedule picked by the autotuner (trial steps advance the state)
+    from morfo70_amd import autotune
+    b = Batch(hs, graph=True)
+    tuned = autotune.tune_batch(b, steps=3)
+    b.run(args.warmup)
+    bstream = torch.cuda.ExternalStream(b.stream, device=dev)
+    torch.cuda.synchronize()
+    ev2, ev3 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev2.record(bstream)
+    b.run_async(args.steps)
+    ev3.record(bstream)
+    b.synchronize()
+    torch.cuda.synchronize()
+    batch_ms = ev2.elapsed_time(ev3)
+    finite = all(h.check_finite() for h in hs)
+    b.close()
+    for h in hs:
+        h.close()
+    loop_rate = cells * args.steps / (loop_ms * 1e-3)
+    batch_rate = cells * args.steps / (batch_ms * 1e-3)
+    rec = {
+        "metric": "batched ADI cell-steps/s (64 uneven scenarios) vs looping the plain step",
+        "value": batch_rate, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": batch_ms / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (64 seeded beaches, random waves)",
+        "config": {"workload": f"batch{len(scs)}_uneven", "cells_total": cells,
+                   "nx_range": [min(s.nx for s in scs), max(s.nx for s in scs)],
+                   "ny_range": [min(s.ny for s in scs), max(s.ny for s in scs)],
+                   "timed": f"one Run({args.steps}) of every scenario"},
+        "looped_value": loop_rate, "looped_ms_per_step": loop_ms / args.steps,
+        "batched_over_looped": batch_rate / loop_rate, "finite": finite, "autotune": tuned,
+        "generation_s": gen_s,
+    }
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.workload == "batch64":
+        if args.gpus != 1:
+            sys.exit(2)
+        bench_batch(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU sharding is "
                                    "described in DESIGN.md but not launched",
@@ -128,6 +247,7 @@ def main():
     dom_bytes = KERNEL_BYTES_PER_CELL[dom] * rows * sc.ny
     achieved = dom_bytes / (avg_ms * 1e-3) / 1e9
     step_gbs = B_ALG_STEP * value / 1e9
+    traffic = measured_traffic(dom, args.nx, args.ny, args.solver)
     record = {
         "metric": "ADI cell-steps/s vs plain and CPU at 1e-10 rel err; adjoint/forward time ratio",
         "value": value,
@@ -147,7 +267,7 @@ def main():
                    "timed": f"one Run({args.steps}) incl. per-Run pre-loop"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "avg_launch_ms": avg_ms,
+                     "traffic": traffic, "avg_launch_ms": avg_ms,
                      "alg_bytes_per_launch": dom_bytes},
         "step_roofline": {"bytes_per_cell_step": B_ALG_STEP, "achieved": step_gbs,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},

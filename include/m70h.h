@@ -151,6 +151,10 @@ int m70h_set_solver(m70h *h, int solver);
  * x (columns) and along y (rows). This is the knob the autotuner sets. */
 int m70h_set_solver_tiles(m70h *h, int tile_cols, int tile_rows);
 
+/* FUSED row-solver block: 256 threads (ny up to 2048; two blocks per CU) or 512 (ny up
+ * to 4096); 0 picks the smallest that fits. An autotuner knob. */
+int m70h_set_rows_block(m70h *h, int threads);
+
 /* Batched tridiagonal solve on device arrays, the counterpart of Math::TridiagonalSolver
  * (src/Math.cpp:15-65, lines strided as in Hydro's x-sweeps) and
  * Math::TridiagonalSolverPeriodic (src/Math.cpp:68-186, contiguous lines as in Hydro's
@@ -194,6 +198,9 @@ int m70h_batch_create(int nscen, m70h *const *handles, m70h_batch **out);
 void m70h_batch_destroy(m70h_batch *b);
 const char *m70h_batch_last_error(const m70h_batch *b);
 int m70h_batch_set_graph(m70h_batch *b, int use_graph);
+/* Row-solver block size (256 or 512) for the whole batch; default the smallest that fits
+ * the largest ny. An autotuner knob. */
+int m70h_batch_set_rows_block(m70h_batch *b, int threads);
 /* The batch's hipStream_t (for events around m70h_batch_run_async). */
 int m70h_batch_stream(m70h_batch *b, void **hip_stream);
 /* Run(nIter) of every handle: per-Run pre-loop, then nIter steps. */

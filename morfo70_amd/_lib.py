@@ -40,6 +40,7 @@ SIGNATURES = {
     "m70h_synchronize": (ctypes.c_int, [_vp]),
     "m70h_set_solver": (ctypes.c_int, [_vp, ctypes.c_int]),
     "m70h_set_solver_tiles": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "m70h_set_rows_block": (ctypes.c_int, [_vp, ctypes.c_int]),
     "m70h_tridiag_solve": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_long] + [_vp] * 5 +
                            [ctypes.c_long, _vp, ctypes.c_long, ctypes.c_int]),
@@ -52,6 +53,7 @@ SIGNATURES = {
     "m70h_batch_destroy": (None, [_vp]),
     "m70h_batch_last_error": (ctypes.c_char_p, [_vp]),
     "m70h_batch_set_graph": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "m70h_batch_set_rows_block": (ctypes.c_int, [_vp, ctypes.c_int]),
     "m70h_batch_stream": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
     "m70h_batch_run": (ctypes.c_int, [_vp, ctypes.c_int]),
     "m70h_batch_run_async": (ctypes.c_int, [_vp, ctypes.c_int]),

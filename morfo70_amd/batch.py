@@ -25,6 +25,9 @@ class Batch:
         if rc != _lib.M70H_OK:
             raise _lib.M70HError(rc, self._L.m70h_batch_last_error(self._b).decode())
 
+    def set_rows_block(self, threads: int):
+        self._chk(self._L.m70h_batch_set_rows_block(self._b, int(threads)))
+
     def set_graph(self, on: bool):
         self._chk(self._L.m70h_batch_set_graph(self._b, 1 if on else 0))
 

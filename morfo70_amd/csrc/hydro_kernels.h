@@ -36,20 +36,23 @@ hipError_t launch_partition_rows(hipStream_t st, const double *A, const double *
                                  long ld, double *Y, long ldy, int nL, int n, bool periodic, int tile);
 
 int cols_segments(int n);
+enum ColsPass { COLS_ELIM = 0, COLS_REDUCED = 1, COLS_BACKSUB = 2 };
 hipError_t launch_cols3_qxx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                            const Work &w, double f0, const ColsWork &cw, int tile);
+                            const Work &w, double f0, const ColsWork &cw, int tile, int pass);
 hipError_t launch_cols3_qyx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                            const Work &w, const ColsWork &cw, int tile);
+                            const Work &w, const ColsWork &cw, int tile, int pass);
 bool rows_lds_supported(int n);
-hipError_t launch_lds_qxy(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w);
+int rows_block_for(int n, int rb);
+hipError_t launch_lds_qxy(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w, int rb);
 hipError_t launch_lds_qyy(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                          const Work &w);
+                          const Work &w, int rb);
 
 // grouped (multi-scenario) launches
 int grouped_cell_blocks(int nx, int ny);
 hipError_t launch_cells_grouped(hipStream_t st, int op, const ScenDesc *D, const int *start, int nscen, int blocks);
-void grouped_stage_blocks(int nx, int ny, int *rows, int *cols, int *red, int *S_rows, int *S_red, int *L_red);
-hipError_t launch_rows_grouped(hipStream_t st, int which, const ScenDesc *D, const int *start, int nscen, int blocks);
+void grouped_stage_blocks(int nx, int ny, int rb, int *rows, int *cols, int *red, int *S_rows, int *S_red, int *L_red);
+hipError_t launch_rows_grouped(hipStream_t st, int which, int rb, const ScenDesc *D, const int *start, int nscen,
+                               int blocks);
 hipError_t launch_cols3_grouped(hipStream_t st, int which, const ScenDesc *D, const int *start_cols,
                                 const int *start_red, int nscen, int blocks_cols, int blocks_red);
 

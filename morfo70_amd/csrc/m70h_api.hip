@@ -21,11 +21,13 @@ thread_local std::string g_create_error;
 
 enum KernelId {
     K_DRAG_VISC, K_RUN_FIELDS, K_QXX_ASM, K_THOMAS_QX, K_QXY_ASM, K_PERIODIC_QX, K_QX_TAIL,
-    K_QYY_ASM, K_PERIODIC_QY, K_QYX_ASM, K_THOMAS_QY, K_QY_TAIL, K_FUSED_QXY, K_FUSED_QYY, K_COLS_QX, K_COLS_QY, K_COUNT
+    K_QYY_ASM, K_PERIODIC_QY, K_QYX_ASM, K_THOMAS_QY, K_QY_TAIL, K_FUSED_QXY, K_FUSED_QYY,
+    K_COLS_QX_ELIM, K_COLS_QX_RED, K_COLS_QX_BACK, K_COLS_QY_ELIM, K_COLS_QY_RED, K_COLS_QY_BACK, K_COUNT
 };
 const char *const kKernelNames[K_COUNT] = {
     "drag_viscosity", "run_fields", "qxx_assemble", "thomas_qx", "qxy_assemble", "periodic_qx", "qx_tail",
-    "qyy_assemble", "periodic_qy", "qyx_assemble", "thomas_qy", "qy_tail", "fused_qxy", "fused_qyy", "cols3_qx", "cols3_qy"};
+    "qyy_assemble", "periodic_qy", "qyx_assemble", "thomas_qy", "qy_tail", "fused_qxy", "fused_qyy",
+    "cols_qx_elim", "cols_qx_reduced", "cols_qx_backsub", "cols_qy_elim", "cols_qy_reduced", "cols_qy_backsub"};
 
 }  // namespace
 
@@ -42,6 +44,7 @@ struct m70h {
     hipStream_t own_stream = nullptr, stream = nullptr;
     int solver = M70H_SOLVER_EXACT;
     int tile_cols = 0, tile_rows = 0;   // partition-solver unknowns per thread (0 = default)
+    int rows_block = 0;                 // FUSED row-solver block size (0 = smallest that fits)
     std::string err;
     std::vector<void *> allocs;
     double *metric_buf = nullptr;
@@ -171,13 +174,17 @@ int run_step(m70h *h)
     if (h->solver == M70H_SOLVER_FUSED) {
         // x-sweeps: three coalesced passes (assemble+eliminate, reduced solve, back-sub);
         // y-sweeps: one launch each, the row's coefficients staged through LDS.
-        if ((rc = timed(h, K_COLS_QX, [&] { return launch_cols3_qxx(st, k, h->m, s, h->rf, w, f0, h->cw, h->tile_cols); })))
-            return rc;
-        if ((rc = timed(h, K_FUSED_QXY, [&] { return launch_lds_qxy(st, k, s, h->rf, w); }))) return rc;
+        for (int pass = COLS_ELIM; pass <= COLS_BACKSUB; ++pass)
+            if ((rc = timed(h, K_COLS_QX_ELIM + pass,
+                            [&] { return launch_cols3_qxx(st, k, h->m, s, h->rf, w, f0, h->cw, h->tile_cols, pass); })))
+                return rc;
+        if ((rc = timed(h, K_FUSED_QXY, [&] { return launch_lds_qxy(st, k, s, h->rf, w, h->rows_block); }))) return rc;
         if ((rc = timed(h, K_QX_TAIL, [&] { return launch_qx_tail(st, k, h->m, s, w, f0); }))) return rc;
-        if ((rc = timed(h, K_FUSED_QYY, [&] { return launch_lds_qyy(st, k, h->m, s, h->rf, w); }))) return rc;
-        if ((rc = timed(h, K_COLS_QY, [&] { return launch_cols3_qyx(st, k, h->m, s, h->rf, w, h->cw, h->tile_cols); })))
-            return rc;
+        if ((rc = timed(h, K_FUSED_QYY, [&] { return launch_lds_qyy(st, k, h->m, s, h->rf, w, h->rows_block); }))) return rc;
+        for (int pass = COLS_ELIM; pass <= COLS_BACKSUB; ++pass)
+            if ((rc = timed(h, K_COLS_QY_ELIM + pass,
+                            [&] { return launch_cols3_qyx(st, k, h->m, s, h->rf, w, h->cw, h->tile_cols, pass); })))
+                return rc;
         if ((rc = timed(h, K_QY_TAIL, [&] { return launch_qy_tail(st, k, s); }))) return rc;
         std::swap(s.D, s.Dn);
         return M70H_OK;
@@ -639,6 +646,15 @@ int m70h_set_solver_tiles(m70h *h, int tile_cols, int tile_rows)
     return M70H_OK;
 }
 
+int m70h_set_rows_block(m70h *h, int threads)
+{
+    if (!h) return M70H_EINVAL;
+    if (threads != 0 && threads != 256 && threads != 512) return fail(h, M70H_EINVAL, "rows block must be 0, 256 or 512");
+    if (threads == 256 && rows_block_for(h->ny, 256) != 256) return fail(h, M70H_EINVAL, "ny too large for 256");
+    h->rows_block = threads;
+    return M70H_OK;
+}
+
 int m70h_tridiag_solve(void *hip_stream, int solver, int lines_are_rows, int periodic, int n, int nlines, long ld,
                        const double *a_dev, const double *b_dev, const double *c_dev, const double *f_dev,
                        double *y_dev, long ldy, double *scratch_dev, long scratch_len, int tile)
@@ -736,6 +752,7 @@ struct m70h_batch {
     int blocks[GS_COUNT] = {};
     int cell_blocks = 0;
     bool use_graph = true;
+    int rows_block = 0;   // row-solver block size for every scenario of the batch
     hipGraphExec_t gexec = nullptr;
     std::string err;
 };
@@ -752,10 +769,10 @@ static int bfail(m70h_batch *b, int code, const std::string &msg)
         if (e_ != hipSuccess) return bfail(b, M70H_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
     } while (0)
 
-static void fill_desc(const m70h *h, ScenDesc &d, bool odd)
+static void fill_desc(const m70h *h, ScenDesc &d, bool odd, int rb)
 {
     int rows, cols, red, S_rows, S_red, L_red;
-    grouped_stage_blocks(h->nx, h->ny, &rows, &cols, &red, &S_rows, &S_red, &L_red);
+    grouped_stage_blocks(h->nx, h->ny, rb, &rows, &cols, &red, &S_rows, &S_red, &L_red);
     d.k = h->k; d.m = h->m; d.s = h->s; d.rf = h->rf; d.fw = h->fw; d.w = h->w; d.cw = h->cw;
     if (odd) std::swap(d.s.D, d.s.Dn);
     d.f0 = h->f0_series.empty() ? h->f0 : h->f0_series.back();
@@ -767,8 +784,8 @@ static int batch_upload(m70h_batch *b)
     const int n = (int)b->hs.size();
     std::vector<ScenDesc> host(2 * (size_t)n);
     for (int q = 0; q < n; ++q) {
-        fill_desc(b->hs[q], host[q], false);
-        fill_desc(b->hs[q], host[n + q], true);
+        fill_desc(b->hs[q], host[q], false, b->rows_block);
+        fill_desc(b->hs[q], host[n + q], true, b->rows_block);
     }
     BHIPCHK(b, hipMemcpyAsync(b->d_desc, host.data(), host.size() * sizeof(ScenDesc), hipMemcpyHostToDevice, b->stream));
     BHIPCHK(b, hipStreamSynchronize(b->stream));
@@ -785,9 +802,9 @@ static hipError_t batch_step(m70h_batch *b, int odd)
     hipError_t e;
     if ((e = launch_cols3_grouped(st, 0, D, st_cols, st_red, n, b->blocks[GS_COLS], b->blocks[GS_RED])) != hipSuccess)
         return e;
-    if ((e = launch_rows_grouped(st, 0, D, st_rows, n, b->blocks[GS_ROWS])) != hipSuccess) return e;
+    if ((e = launch_rows_grouped(st, 0, b->rows_block, D, st_rows, n, b->blocks[GS_ROWS])) != hipSuccess) return e;
     if ((e = launch_cells_grouped(st, 2, D, st_cells, n, b->blocks[GS_CELLS])) != hipSuccess) return e;
-    if ((e = launch_rows_grouped(st, 1, D, st_rows, n, b->blocks[GS_ROWS])) != hipSuccess) return e;
+    if ((e = launch_rows_grouped(st, 1, b->rows_block, D, st_rows, n, b->blocks[GS_ROWS])) != hipSuccess) return e;
     if ((e = launch_cols3_grouped(st, 1, D, st_cols, st_red, n, b->blocks[GS_COLS], b->blocks[GS_RED])) != hipSuccess)
         return e;
     return launch_cells_grouped(st, 3, D, st_cells, n, b->blocks[GS_CELLS]);
@@ -805,6 +822,10 @@ int m70h_batch_create(int nscen, m70h *const *handles, m70h_batch **out)
     m70h_batch *b = new m70h_batch();
     auto bail = [&](int rc) { g_create_error = b->err; m70h_batch_destroy(b); return rc; };
     int dev0 = -1;
+    int maxny = 0;
+    for (int q = 0; q < nscen; ++q)
+        if (handles[q] && handles[q]->ny > maxny) maxny = handles[q]->ny;
+    b->rows_block = rows_block_for(maxny, 0);
     std::vector<int> starts((size_t)GS_COUNT * (nscen + 1), 0);
     for (int q = 0; q < nscen; ++q) {
         m70h *h = handles[q];
@@ -814,7 +835,7 @@ int m70h_batch_create(int nscen, m70h *const *handles, m70h_batch **out)
         if (!rows_lds_supported(h->ny) || !partition_supported(h->nx - 1))
             return bail(bfail(b, M70H_EINVAL, "batched step supports ny and nx-1 up to 4096"));
         int rows, cols, red, S_rows, S_red, L_red;
-        grouped_stage_blocks(h->nx, h->ny, &rows, &cols, &red, &S_rows, &S_red, &L_red);
+        grouped_stage_blocks(h->nx, h->ny, b->rows_block, &rows, &cols, &red, &S_rows, &S_red, &L_red);
         if (S_rows <= 0 || S_red <= 0) return bail(bfail(b, M70H_EINVAL, "grid too small for the batched step"));
         const int per[GS_COUNT] = {grouped_cell_blocks(h->nx, h->ny), rows, cols, red};
         for (int gs = 0; gs < GS_COUNT; ++gs)
@@ -841,6 +862,26 @@ void m70h_batch_destroy(m70h_batch *b)
     if (b->d_start) (void)hipFree(b->d_start);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
+}
+
+int m70h_batch_set_rows_block(m70h_batch *b, int threads)
+{
+    if (!b) return M70H_EINVAL;
+    int maxny = 0;
+    for (m70h *h : b->hs) maxny = h->ny > maxny ? h->ny : maxny;
+    if (threads != 256 && threads != 512) return bfail(b, M70H_EINVAL, "rows block must be 256 or 512");
+    if (threads == 256 && rows_block_for(maxny, 256) != 256) return bfail(b, M70H_EINVAL, "ny too large for 256");
+    for (m70h *h : b->hs) {   // segment counts depend on the block size
+        int rows, cols, red, S_rows, S_red, L_red;
+        grouped_stage_blocks(h->nx, h->ny, threads, &rows, &cols, &red, &S_rows, &S_red, &L_red);
+        if (S_rows <= 0) return bfail(b, M70H_EINVAL, "grid too small for the row block");
+    }
+    b->rows_block = threads;
+    if (b->gexec) {
+        (void)hipGraphExecDestroy(b->gexec);
+        b->gexec = nullptr;
+    }
+    return M70H_OK;
 }
 
 int m70h_batch_set_graph(m70h_batch *b, int use_graph)

@@ -266,15 +266,16 @@ __global__ void __launch_bounds__(BLOCK) k_partition_solve(Prov prov, double *__
 // e + e/8, so that the per-thread segment reads below are free of bank conflicts),
 // each thread then takes its segment, and the solution goes back through LDS to a
 // coalesced store.
-constexpr int RB = 512;
-constexpr int RM = 8;
-constexpr int RNMAX = RB * RM;
-constexpr int RLDS = RNMAX + RNMAX / 8;
+constexpr int RM = 8;                  // unknowns per thread
+template <int RB> struct RowsLds {     // RB threads per block, lines up to RB*RM
+    static constexpr int NMAX = RB * RM;
+    static constexpr int LDS = NMAX + NMAX / 8;
+};
 __device__ __forceinline__ int lpad(int e) { return e + (e >> 3); }
 
-template <class Prov>
+template <int RB, class Prov>
 __device__ __forceinline__ void rows_lds_body(const Prov &prov, double *__restrict__ Y, long ldy, int n, int S,
-                                              int periodic, int line, double (*sh)[RLDS])
+                                              int periodic, int line, double (*sh)[RowsLds<RB>::LDS])
 {
     const int t = threadIdx.x;
     for (int e = t; e < n; e += RB) {
@@ -319,12 +320,12 @@ __device__ __forceinline__ void rows_lds_body(const Prov &prov, double *__restri
     for (int e = t; e < n; e += RB) yl[e] = sh[0][lpad(e)];
 }
 
-template <class Prov>
+template <int RB, class Prov>
 __global__ void __launch_bounds__(RB) k_rows_lds(Prov prov, double *__restrict__ Y, long ldy, int n, int S,
                                                  int periodic)
 {
-    __shared__ double sh[4][RLDS];
-    rows_lds_body(prov, Y, ldy, n, S, periodic, (int)blockIdx.x, sh);
+    __shared__ double sh[4][RowsLds<RB>::LDS];
+    rows_lds_body<RB>(prov, Y, ldy, n, S, periodic, (int)blockIdx.x, sh);
 }
 
 // Picks S (power of two, n >= 2S, ceil(n/S) <= M) for a line length n; 0 if none.
@@ -470,33 +471,51 @@ __global__ void __launch_bounds__(256) k_cols_backsub(ColsWork cw, double *__res
     cols_backsub_body(cw, Y, n, ny, (int)blockIdx.x, (int)blockIdx.y);
 }
 
+// One pass of the three-pass column solve: COLS_ELIM (assemble + eliminate the segments),
+// COLS_REDUCED (the reduced systems, 2*nseg rows per column) or COLS_BACKSUB. The passes
+// run in this order on one stream; each is its own launch so it can be timed alone.
 template <class Prov>
-static hipError_t launch_cols3(hipStream_t st, const Prov &prov, const ColsWork &cw, double *Y, int n, int ny, int tile)
+static hipError_t launch_cols3(hipStream_t st, const Prov &prov, const ColsWork &cw, double *Y, int n, int ny, int tile,
+                               int pass)
 {
     if (n < 3 || cols_segments(n) != cw.nseg) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((ny + 63) / 64), (unsigned)((cw.nseg + 3) / 4)), block(64, 4);
-    hipLaunchKernelGGL((k_cols_elim<Prov>), grid, block, 0, st, prov, cw, n, ny);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    // reduced systems: diagonal R1, lower R0, upper R2, rhs R3; 2*nseg rows per column
-    e = dispatch_cols(st, ProvArrays<true>{cw.R1, cw.R0, cw.R2, cw.R3, (long)ny}, cw.X, ny, 2 * cw.nseg, ny, tile);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_cols_backsub, grid, block, 0, st, cw, Y, n, ny);
-    return hipGetLastError();
+    switch (pass) {
+    case COLS_ELIM:
+        hipLaunchKernelGGL((k_cols_elim<Prov>), grid, block, 0, st, prov, cw, n, ny);
+        return hipGetLastError();
+    case COLS_REDUCED:
+        // diagonal R1, lower R0, upper R2, rhs R3
+        return dispatch_cols(st, ProvArrays<true>{cw.R1, cw.R0, cw.R2, cw.R3, (long)ny}, cw.X, ny, 2 * cw.nseg, ny, tile);
+    case COLS_BACKSUB:
+        hipLaunchKernelGGL(k_cols_backsub, grid, block, 0, st, cw, Y, n, ny);
+        return hipGetLastError();
+    default:
+        return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_cols3_qxx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                            const Work &w, double f0, const ColsWork &cw, int tile)
+                            const Work &w, double f0, const ColsWork &cw, int tile, int pass)
 {
-    return launch_cols3(st, ProvQxx{k, m, s, rf, w, f0}, cw, s.Qx + k.ny, k.nx - 1, k.ny, tile);
+    return launch_cols3(st, ProvQxx{k, m, s, rf, w, f0}, cw, s.Qx + k.ny, k.nx - 1, k.ny, tile, pass);
 }
 hipError_t launch_cols3_qyx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                            const Work &w, const ColsWork &cw, int tile)
+                            const Work &w, const ColsWork &cw, int tile, int pass)
 {
-    return launch_cols3(st, ProvQyx{k, m, s, rf, w}, cw, s.Qy + k.ny, k.nx - 1, k.ny, tile);
+    return launch_cols3(st, ProvQyx{k, m, s, rf, w}, cw, s.Qy + k.ny, k.nx - 1, k.ny, tile, pass);
 }
 
-bool rows_lds_supported(int n) { return pick_segments(n, RM, RB) > 0; }
+bool rows_lds_supported(int n) { return pick_segments(n, RM, 512) > 0; }
+
+// Row block: 256 threads (lines up to 2048, 74 KB of LDS, two blocks per CU) or 512
+// threads (lines up to 4096, 147 KB, one block per CU). rb = 0 picks the smallest fit.
+int rows_block_for(int n, int rb)
+{
+    if (rb == 256 && pick_segments(n, RM, 256) > 0 && n <= RowsLds<256>::NMAX) return 256;
+    if (rb == 512) return 512;
+    return n <= RowsLds<256>::NMAX ? 256 : 512;
+}
 
 // ---------------------------------------------------------------------------------
 // Grouped (multi-scenario) versions of the FUSED stages. Providers read the scenario's
@@ -523,18 +542,18 @@ struct GProvQyy {
     { asm_qyy(d->k, d->m, d->s, d->rf, d->w, line, kk, a, b, c, f); }
 };
 
-template <int WHICH>   // 0: Qx projector rows, 1: Qy predictor rows
+template <int WHICH, int RB>   // WHICH 0: Qx projector rows, 1: Qy predictor rows
 __global__ void __launch_bounds__(RB) k_rows_lds_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
                                                          int nscen)
 {
-    __shared__ double sh[4][RLDS];
+    __shared__ double sh[4][RowsLds<RB>::LDS];
     const int b = blockIdx.x;
     const int sid = scen_of(start, nscen, b);
     const ScenDesc *d = D + sid;
     const int ny = d->k.ny;
     const int line = b - start[sid];
-    if (WHICH == 0) rows_lds_body(GProvQxy{d}, d->s.Qx + ny, ny, ny, d->S_rows, 1, line, sh);
-    else rows_lds_body(GProvQyy{d}, d->s.Qy + ny, ny, ny, d->S_rows, 1, line, sh);
+    if (WHICH == 0) rows_lds_body<RB>(GProvQxy{d}, d->s.Qx + ny, ny, ny, d->S_rows, 1, line, sh);
+    else rows_lds_body<RB>(GProvQyy{d}, d->s.Qy + ny, ny, ny, d->S_rows, 1, line, sh);
 }
 
 template <int WHICH>   // 0: Qx predictor columns, 1: Qy projector columns
@@ -583,10 +602,10 @@ __global__ void __launch_bounds__(1024) k_red_grouped(const ScenDesc *__restrict
 }
 
 // Block counts of the grouped stages for one scenario.
-void grouped_stage_blocks(int nx, int ny, int *rows, int *cols, int *red, int *S_rows, int *S_red, int *L_red)
+void grouped_stage_blocks(int nx, int ny, int rb, int *rows, int *cols, int *red, int *S_rows, int *S_red, int *L_red)
 {
     *rows = nx - 1;
-    *S_rows = pick_segments(ny, RM, RB);
+    *S_rows = pick_segments(ny, RM, rb);
     const int nseg = cols_segments(nx - 1);
     *cols = ((ny + 63) / 64) * ((nseg + 3) / 4);
     *S_red = pick_segments(2 * nseg, 8, 256);
@@ -594,10 +613,16 @@ void grouped_stage_blocks(int nx, int ny, int *rows, int *cols, int *red, int *S
     *red = *L_red > 0 ? (ny + *L_red - 1) / *L_red : 0;
 }
 
-hipError_t launch_rows_grouped(hipStream_t st, int which, const ScenDesc *D, const int *start, int nscen, int blocks)
+hipError_t launch_rows_grouped(hipStream_t st, int which, int rb, const ScenDesc *D, const int *start, int nscen,
+                               int blocks)
 {
-    if (which == 0) hipLaunchKernelGGL(k_rows_lds_grouped<0>, dim3(blocks), dim3(RB), 0, st, D, start, nscen);
-    else hipLaunchKernelGGL(k_rows_lds_grouped<1>, dim3(blocks), dim3(RB), 0, st, D, start, nscen);
+    if (rb == 256) {
+        if (which == 0) hipLaunchKernelGGL((k_rows_lds_grouped<0, 256>), dim3(blocks), dim3(256), 0, st, D, start, nscen);
+        else hipLaunchKernelGGL((k_rows_lds_grouped<1, 256>), dim3(blocks), dim3(256), 0, st, D, start, nscen);
+    } else {
+        if (which == 0) hipLaunchKernelGGL((k_rows_lds_grouped<0, 512>), dim3(blocks), dim3(512), 0, st, D, start, nscen);
+        else hipLaunchKernelGGL((k_rows_lds_grouped<1, 512>), dim3(blocks), dim3(512), 0, st, D, start, nscen);
+    }
     return hipGetLastError();
 }
 
@@ -617,23 +642,30 @@ hipError_t launch_cols3_grouped(hipStream_t st, int which, const ScenDesc *D, co
 }
 
 template <class Prov>
-static hipError_t launch_rows_lds(hipStream_t st, const Prov &prov, double *Y, long ldy, int n, int nL, bool periodic)
+static hipError_t launch_rows_lds(hipStream_t st, const Prov &prov, double *Y, long ldy, int n, int nL, bool periodic,
+                                  int rb)
 {
-    const int S = pick_segments(n, RM, RB);
+    rb = rows_block_for(n, rb);
+    const int S = pick_segments(n, RM, rb);
     if (S == 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_rows_lds<Prov>), dim3((unsigned)nL), dim3(RB), 0, st, prov, Y, ldy, n, S, periodic ? 1 : 0);
+    if (rb == 256)
+        hipLaunchKernelGGL((k_rows_lds<256, Prov>), dim3((unsigned)nL), dim3(256), 0, st, prov, Y, ldy, n, S,
+                           periodic ? 1 : 0);
+    else
+        hipLaunchKernelGGL((k_rows_lds<512, Prov>), dim3((unsigned)nL), dim3(512), 0, st, prov, Y, ldy, n, S,
+                           periodic ? 1 : 0);
     return hipGetLastError();
 }
 
 // Row sweeps with the assembly staged through LDS (one row per block).
-hipError_t launch_lds_qxy(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w)
+hipError_t launch_lds_qxy(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w, int rb)
 {
-    return launch_rows_lds(st, ProvQxy{k, s, rf, w}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true);
+    return launch_rows_lds(st, ProvQxy{k, s, rf, w}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true, rb);
 }
 hipError_t launch_lds_qyy(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                          const Work &w)
+                          const Work &w, int rb)
 {
-    return launch_rows_lds(st, ProvQyy{k, m, s, rf, w}, s.Qy + k.ny, k.ny, k.ny, k.nx - 1, true);
+    return launch_rows_lds(st, ProvQyy{k, m, s, rf, w}, s.Qy + k.ny, k.ny, k.ny, k.nx - 1, true, rb);
 }
 
 }  // namespace m70

@@ -174,6 +174,10 @@ class Hydro:
 
     def set_stream(self, stream_ptr: int | None):
         self._chk(self._L.m70h_set_stream(self._h, ctypes.c_void_p(stream_ptr or 0)))
+        self._stream = stream_ptr or 0
+
+    def _stream_ptr(self) -> int:
+        return getattr(self, "_stream", 0)
 
     def set_profiling(self, on: bool):
         self._chk(self._L.m70h_set_profiling(self._h, 1 if on else 0))

@@ -45,7 +45,9 @@ def nonuniform_beach(nx=24, ny=20, seed=0, steps=5, physics=None, tide=False, Hr
     ph = physics or S.Physics(dt=0.25)
     sc = S.Scenario(f"nonuniform_{nx}x{ny}", nx, ny, float(np.mean(np.diff(x))), dy, x,
                     np.ascontiguousarray(Z), np.ascontiguousarray(h), 0.0, ph, {}, steps)
-    sc.forcing = S.wave_forcing(sc.initial_depth(), x, Hrms, 7.0, 15.0, ph.Dmin)
+    # unspread closed form: short runs on these small grids stay stable without it, and
+    # it keeps the offshore rows quiet (the translation pin depends on that)
+    sc.forcing = S.wave_forcing(sc.initial_depth(), x, Hrms, 7.0, 15.0, ph.Dmin, spread_m=0.0)
     if tide:
         sc.f0_series = S.tide_f0_series(steps, ph.dt, x[1] - x[0], amp=0.5, period=600.0)
     return sc

@@ -180,7 +180,9 @@ def test_mass_budget_each_step():
 def _uniform_case(ny=12):
     sc = nonuniform_beach(nx=28, ny=ny, seed=1)
     sc.h[:] = sc.h[:, :1]                       # no alongshore variation
-    sc.forcing = S.wave_forcing(sc.initial_depth(), sc.x, 1.0, 7.0, 12.0, sc.physics.Dmin)
+    # unspread forcing: it leaves the offshore rows at rest, so the reference's getD_y0
+    # seam quirk (src/HydroQy.cpp:438, row-0 depth read at j = ny-1) has nothing to act on
+    sc.forcing = S.wave_forcing(sc.initial_depth(), sc.x, 1.0, 7.0, 12.0, sc.physics.Dmin, spread_m=0.0)
     return sc
 
 
