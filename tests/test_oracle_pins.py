@@ -11,8 +11,11 @@ the oracle instead:
   * the 2^6 truth table of the wet/dry rule, against its boolean statement;
   * OpenMP thread-count invariance (the restatement is race-free, unlike
     src/HydroQx.cpp:36-46 under -fopenmp);
+  * the drag and viscosity models at points where log and cbrt are exact, and the
+    forcing stencils on radiation stresses they differentiate exactly;
   * physical invariants of the composed step: lake at rest, mass budget,
-    alongshore-uniformity, alongshore translation, and the steady set-up balance.
+    alongshore-uniformity, alongshore translation, the steady set-up balance and the
+    steady longshore drag balance.
 """
 from __future__ import annotations
 
@@ -253,3 +256,129 @@ def test_bc_constants_closed_form():
     assert f1 == pytest.approx(0.5 * e / (1 - 0.5 * e), rel=1e-15)
     assert bcQy == pytest.approx(np.exp(-0.2), rel=1e-15)
     assert f0 == 0.0
+
+
+# ----------------------------------------------------------------- per-Run fields
+def _flat_case(D_rows, ny=6, dx=None, physics=None, forcing=None):
+    """Grid whose depth is D_rows[i] on row i (Z = D, h = 0, zs0 = 0)."""
+    nx = len(D_rows)
+    x = np.cumsum(np.r_[0.0, dx if dx is not None else np.full(nx - 1, 3.0)])
+    Z = np.repeat(np.asarray(D_rows, dtype=np.float64)[:, None], ny, axis=1)
+    ph = physics or S.Physics()
+    sc = S.Scenario("flat", nx, ny, 3.0, 2.0, x, np.ascontiguousarray(Z), np.zeros_like(Z), 0.0, ph, {}, 1)
+    z = np.zeros((nx, ny))
+    sc.forcing = {"Sxx": z.copy(), "Sxy": z.copy(), "Syy": z.copy(), "Urms": np.full((nx, ny), 0.3),
+                  "Dw": z.copy(), "D_r": z.copy(), "H": z.copy()}
+    if forcing:
+        sc.forcing.update({k: np.ascontiguousarray(v, dtype=np.float64) for k, v in forcing.items()})
+    return sc
+
+
+def _fields(sc, *names):
+    o = oracle.OracleHydro(sc)
+    o.run(1)
+    return [o.field(n) for n in names]
+
+
+def test_drag_models_at_exact_points():
+    """LogDepth (src/HydroParametrizations.cpp:53-63) returns c at D = z0*exp(1 + 0.4/sqrt(c))
+    and clamps D below z0 + 0.001; ManningStrickler (:65-71) returns 0.015*2 where
+    ka/D = 8 (cbrt exact); Constant (:49-51) returns cD0."""
+    z0, c = 0.001, 0.003
+    D_log = z0 * np.exp(1.0 + 0.4 / np.sqrt(c))
+    ph = S.Physics(drag_model=S.DRAG["logdepth"], z0=z0, Dmin=1e-4)
+    (cD,) = _fields(_flat_case([D_log, D_log, 0.0005, 0.0015, 1.0], physics=ph), "cD")
+    assert cD[0, 0] == pytest.approx(c, rel=1e-13)
+    # both shallow rows clamp to z0 + 0.001 = 2*z0, where ln(D/z0) = ln 2
+    assert np.all(cD[2] == cD[3])
+    assert cD[2, 0] == pytest.approx((0.4 / (np.log(2.0) - 1.0)) ** 2, rel=1e-14)
+    assert cD[4, 0] == pytest.approx((0.4 / (np.log(1.0 / z0) - 1.0)) ** 2, rel=1e-14)
+    ph = S.Physics(drag_model=S.DRAG["manning"], ka=0.08)
+    (cD,) = _fields(_flat_case([0.01, 0.01, 0.08, 0.64], physics=ph), "cD")
+    np.testing.assert_allclose(cD[:, 0], [0.03, 0.03, 0.015, 0.0075], rtol=1e-15)
+    ph = S.Physics(drag_model=S.DRAG["constant"], cD0=0.0042)
+    (cD,) = _fields(_flat_case([0.5, 1.0, 2.0], physics=ph), "cD")
+    assert np.all(cD == 0.0042)
+
+
+def test_viscosity_models_at_exact_points():
+    """nuD = D*nu0 (Constant); D*(nu0 + M*D*cbrt(Dw/rho)) (BreakingDepth); rollers with
+    ((1-alpha_r)*Dw + D_r)/rho; BreakingHrms uses H and a float 1/rho (:107), which shifts
+    the result by ~1e-8 relative. Dw = 8*rho and D_r = 8*rho make every cbrt exactly 2."""
+    rho = 1025.0
+    D = [0.5, 1.0, 2.0, 3.0]
+    nx, ny = len(D), 6
+    F = {"Dw": np.full((nx, ny), 8 * rho), "D_r": np.full((nx, ny), 8 * rho), "H": np.full((nx, ny), 0.7)}
+    Dc = np.asarray(D)
+    (nuD,) = _fields(_flat_case(D, physics=S.Physics(visc_model=S.VISC["constant"], nu0=0.01), forcing=F), "nuD")
+    np.testing.assert_allclose(nuD[:, 0], Dc * 0.01, rtol=1e-15)
+    ph = S.Physics(visc_model=S.VISC["depth"], nu0=0.01, M=0.5)
+    (nuD,) = _fields(_flat_case(D, physics=ph, forcing=F), "nuD")
+    np.testing.assert_allclose(nuD[:, 0], Dc * (0.01 + 0.5 * Dc * 2.0), rtol=1e-15)
+    ph = S.Physics(visc_model=S.VISC["rollers"], nu0=0.01, M=0.5, alpha_r=0.5)
+    F2 = dict(F, D_r=np.full((nx, ny), 4 * rho))   # (1-0.5)*8rho + 4rho = 8rho
+    (nuD,) = _fields(_flat_case(D, physics=ph, forcing=F2), "nuD")
+    np.testing.assert_allclose(nuD[:, 0], Dc * (0.01 + 0.5 * Dc * 2.0), rtol=1e-15)
+    ph = S.Physics(visc_model=S.VISC["hrms"], nu0=0.01, M=0.5)
+    (nuD,) = _fields(_flat_case(D, physics=ph, forcing=F), "nuD")
+    exact = Dc * (0.01 + 0.5 * 0.7 * 2.0)
+    rel = np.abs(nuD[:, 0] - exact) / exact
+    assert np.all(rel < 1e-7) and np.all(rel > 1e-11)   # the float 1/rho is visible, and small
+
+
+def test_forcing_stencils_exact_on_linear_and_sinusoidal_stresses():
+    """Fx (src/HydroQx.cpp:357-388) and Fy (src/HydroQy.cpp:367-424) on a non-uniform
+    grid: Sxx linear in x gives -slope/rho exactly; the 4-point alongshore difference of a
+    sinusoidal Sxy is c2*cos(k y_j)*sin(k dy)/(rho dy) exactly (trigonometric identity);
+    Syy linear in y and Sxy linear in x give -(t + c1)/rho on interior rows, because the
+    three-point x weights are exact on linear functions."""
+    rho = 1025.0
+    nx, ny, dy = 9, 12, 2.0
+    dx = np.array([3.0, 2.5, 2.0, 2.2, 2.9, 3.3, 2.1, 1.7])
+    x = np.r_[0.0, np.cumsum(dx)]
+    y = np.arange(ny) * dy
+    L = ny * dy
+    k = 2 * np.pi / L
+    s, c0, c1, c2, t = 37.0, 5.0, 11.0, 3.0, 7.0
+    Sxx = np.repeat((20.0 + s * x)[:, None], ny, axis=1)
+    Sxy_lin = np.repeat((c0 + c1 * x)[:, None], ny, axis=1)
+    Sxy_sin = c2 * np.sin(k * y)[None, :] * np.ones((nx, 1))
+    Syy = np.repeat((t * y)[None, :], nx, axis=0)
+    D = np.full(nx, 2.0)
+    sc = _flat_case(D, ny=ny, dx=dx, forcing={"Sxx": Sxx, "Sxy": Sxy_lin + Sxy_sin, "Syy": Syy})
+    sc.dy = dy
+    Fx, Fy = _fields(sc, "Forcing_x", "Forcing_y")
+    want_x = -(c2 * np.cos(k * y) * np.sin(k * dy) / (rho * dy))[None, :] - s / rho
+    np.testing.assert_allclose(Fx, np.broadcast_to(want_x, Fx.shape), rtol=1e-11, atol=1e-14)
+    # Fy: the sinusoid enters only through x-derivatives (zero); interior rows and j < ny-1
+    want_y = -(t + c1) / rho
+    np.testing.assert_allclose(Fy[1:nx - 1, :ny - 1], want_y, rtol=1e-11)
+    # the periodic seam of Syy: (Syy[0] - Syy[ny-1]) / dy = -t (ny-1)
+    np.testing.assert_allclose(Fy[1:nx - 1, ny - 1], -(t * (1 - ny) + c1) / rho, rtol=1e-11)
+
+
+def test_steady_longshore_current_balances_drag():
+    """Oblique waves on an alongshore-uniform beach, negligible lateral mixing: at steady
+    state the alongshore momentum balance is drag = forcing, i.e.
+    cD*Urms_y/sqrt2*sqrt(alpha_f^2 + 2(U^2+V^2)/Urms_y^2)*V = Fy at each wet row
+    (calculateDragTerm, src/HydroParametrizations.cpp:19-28, times Qy = V*D)."""
+    nx, ny, dx = 40, 4, 4.0
+    x = np.arange(nx) * dx
+    s = x[-1] * 0.85 - x
+    Z = np.repeat((0.03 * s)[:, None], ny, axis=1)
+    ph = S.Physics(dt=0.5, visc_model=S.VISC["constant"], nu0=1e-6)
+    sc = S.Scenario("longshore", nx, ny, dx, dx, x, Z, np.zeros_like(Z), 0.0, ph, {}, 0)
+    sc.forcing = S.wave_forcing(sc.initial_depth(), x, 1.0, 8.0, 20.0, ph.Dmin)
+    o = oracle.OracleHydro(sc)
+    o.run(8000)
+    st = o.get_state()
+    Fy = o.field("Forcing_y")[:, 0]
+    Urms_y = o.field("Urms_y")[:, 0]
+    V = st["V"][1:, 0]
+    D = st["D"][:, 0]
+    cd = ph.cD0
+    drag = cd * Urms_y / np.sqrt(2.0) * np.sqrt(ph.alpha_f ** 2 + 2 * V * V / Urms_y ** 2) * V
+    # rows where the forcing is strong; near its edges lateral mixing carries a few %
+    wet = (D > 0.8) & (np.abs(Fy) > 0.2 * np.max(np.abs(Fy)))
+    assert wet.sum() >= 5
+    np.testing.assert_allclose(drag[wet], Fy[wet], rtol=0.01)
