@@ -253,11 +253,7 @@ template <bool COLS, int BLOCK, int SEG_M, class Prov>
 __global__ void __launch_bounds__(BLOCK) k_partition_solve(Prov prov, double *__restrict__ Y, LineGeom g)
 {
     __shared__ double red[4][BLOCK * 2];   // reduced rows: lower, diag, upper, rhs
-    int blk = blockIdx.x;
-    if (g.xcd_remap) {
-        const int nb = gridDim.x;
-        if ((nb & 7) == 0) blk = (blk & 7) * (nb >> 3) + (blk >> 3);
-    }
+    const int blk = g.xcd_remap ? xcd_chunk((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
     partition_body<COLS, BLOCK, SEG_M>(prov, Y, g, blk, red);
 }
 

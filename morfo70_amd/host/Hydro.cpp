@@ -12,6 +12,7 @@ HydroConfig &hydroConfig()
 }
 }  // namespace m70
 
+#ifndef M70_USE_REFERENCE_CLASSES
 Grid::Grid(const std::vector<double> &xs, int ny_, double dy_)
 {
     nx = (int)xs.size();
@@ -72,6 +73,8 @@ Wave::~Wave()
 {
     delete[] H; delete[] Urms; delete[] Dw; delete[] D_r; delete[] Sxx; delete[] Sxy; delete[] Syy;
 }
+
+#endif /* M70_USE_REFERENCE_CLASSES */
 
 void Hydro::check(int rc) const
 {

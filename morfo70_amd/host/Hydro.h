@@ -33,6 +33,13 @@ HydroConfig &hydroConfig();
 
 }  // namespace m70
 
+#ifdef M70_USE_REFERENCE_CLASSES
+/* Linked into the reference: its own Grid, Bathymetry and Wave (include/Grid.h:21-54,
+ * include/Bathymetry.h:22-53, include/Wave.h:36-53) provide the members read below. */
+#include "Grid.h"
+#include "Bathymetry.h"
+#include "Wave.h"
+#else
 /* Mesh (include/Grid.h:21-54): node coordinates x[nx], spacing dy, sizes. */
 class Grid {
 public:
@@ -73,6 +80,7 @@ public:
     explicit Wave(Grid *grid);
     ~Wave();
 };
+#endif /* M70_USE_REFERENCE_CLASSES */
 
 class Hydro {
 public:
