@@ -192,3 +192,101 @@ def tridiag_periodic(a, b, c, f):
 
 def wet_rule(d, dup, zs, zsup, s, sup, Dmin):
     return lib().orc_wet_rule(d, dup, zs, zsup, s, sup, Dmin)
+
+
+# ------------------------------------------------------------------ tangent-linear model
+TLM_LIB_PATH = os.path.join(_HERE, "build", "liboracle_tlm.so")
+_tlm = None
+
+
+class _Dual(ctypes.Structure):
+    _fields_ = [("v", ctypes.c_double), ("d", ctypes.c_double)]
+
+
+class _DualParams(ctypes.Structure):
+    _fields_ = [("drag_model", ctypes.c_int), ("visc_model", ctypes.c_int),
+                ("offshore_bc", ctypes.c_int)] + [(n, _Dual) for n in (
+                    "dt", "Dmin", "alpha_f", "cD0", "nu0", "M", "ka", "z0", "alpha_r")]
+
+
+def tlm_lib():
+    """The tangent-linear build (tlm.cpp): the oracle's API over (value, tangent) pairs."""
+    global _tlm
+    if _tlm is None:
+        if not os.path.exists(TLM_LIB_PATH):
+            build()
+        L = ctypes.CDLL(TLM_LIB_PATH)
+        vp = ctypes.c_void_p
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, vp, _Dual, ctypes.POINTER(_DualParams), vp, vp, _Dual]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_forcing.argtypes = [vp] + [vp] * 7
+        L.orc_set_f0_series.argtypes = [vp, vp, ctypes.c_long]
+        L.orc_get_state.argtypes = [vp] + [vp] * 10
+        L.orc_set_state.argtypes = [vp] + [vp] * 10
+        L.orc_run.argtypes = [vp, ctypes.c_int]
+        L.orc_run.restype = ctypes.c_int
+        _tlm = L
+    return _tlm
+
+
+def _pair(v, d=None) -> np.ndarray:
+    v = np.asarray(v, dtype=np.float64)
+    out = np.zeros(v.shape + (2,))
+    out[..., 0] = v
+    if d is not None:
+        out[..., 1] = d
+    return np.ascontiguousarray(out)
+
+
+def _vp(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"] and a.shape[-1] == 2
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class TangentHydro:
+    """Tangent-linear model of the oracle: runs the step on (value, tangent) pairs.
+
+    ``dh`` is the tangent of the bar field h (bathymetry perturbation, Z - h = bed
+    level) and ``dparams`` maps parameter names (cD0, nu0, M, ...) to their tangents.
+    The initial depth D = max(Z - h + zs0, Dmin) and surface Zs = D - Z + h are formed
+    inside the model from h, so their tangents follow from dh. After run(n),
+    get_state() returns (values, tangents) per state field."""
+
+    def __init__(self, sc, dh=None, dparams: dict | None = None, threads: int = 0):
+        L = tlm_lib()
+        if threads:
+            L.orc_set_threads(threads)
+        self.nx, self.ny = sc.nx, sc.ny
+        ph = sc.physics
+        dparams = dparams or {}
+        vals = dict(zip(("dt", "Dmin", "alpha_f", "cD0", "nu0", "M", "ka", "z0", "alpha_r"),
+                        ph.as_tuple()[3:]))
+        self._p = _DualParams(ph.drag_model, ph.visc_model, ph.offshore_bc,
+                              *[_Dual(vals[n], dparams.get(n, 0.0)) for n in vals])
+        self._keep = [_pair(sc.x), _pair(sc.Z), _pair(sc.h, dh)]
+        self.h = L.orc_create(sc.nx, sc.ny, _vp(self._keep[0]), _Dual(sc.dy, 0.0), ctypes.byref(self._p),
+                              _vp(self._keep[1]), _vp(self._keep[2]), _Dual(sc.zs0, 0.0))
+        if not self.h:
+            raise ValueError("orc_create failed")
+        f = [_pair(sc.forcing[k]) for k in FORCING_FIELDS]
+        L.orc_set_forcing(self.h, *[_vp(a) for a in f])
+        if sc.f0_series is not None:
+            self._f0 = _pair(sc.f0_series)
+            L.orc_set_f0_series(self.h, _vp(self._f0), self._f0.shape[0])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            tlm_lib().orc_destroy(self.h)
+            self.h = None
+
+    def run(self, n: int):
+        tlm_lib().orc_run(self.h, int(n))
+
+    def get_state(self):
+        """(values, tangents): two dicts of the state fields."""
+        shp = {k: ((self.nx + 1, self.ny) if k in PLUS_FIELDS else (self.nx, self.ny)) for k in STATE_FIELDS}
+        out = {k: np.zeros(shp[k] + (2,)) for k in STATE_FIELDS}
+        tlm_lib().orc_get_state(self.h, *[_vp(out[k]) for k in STATE_FIELDS])
+        return {k: out[k][..., 0].copy() for k in out}, {k: out[k][..., 1].copy() for k in out}

@@ -415,7 +415,7 @@ static void calculateDragAndViscosity(orc_hydro *H)
             break;
         }
         default: { /* ORC_VISC_HRMS, :103-113; note the float inv_rho at :107 */
-            float inv_rho = 1 / rho;
+            float inv_rho = (float)(1 / rho);
             for (j = 0; j < ny; j++) nuD[j] = D[j] * (p->nu0 + p->M * Hw[j] * cbrt(Dw[j] * inv_rho));
             break;
         }

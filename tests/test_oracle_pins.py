@@ -382,3 +382,45 @@ def test_steady_longshore_current_balances_drag():
     wet = (D > 0.8) & (np.abs(Fy) > 0.2 * np.max(np.abs(Fy)))
     assert wet.sum() >= 5
     np.testing.assert_allclose(drag[wet], Fy[wet], rtol=0.01)
+
+
+# ------------------------------------------------------------- tangent-linear model
+def test_tangent_linear_model_matches_central_differences():
+    """The dual-number build of the oracle (oracle/tlm.cpp) returns the plain oracle's
+    values bit for bit, and its tangents with respect to h and cD0 agree with central
+    differences of the plain oracle (truncation ~eps^2, rounding ~1e-16/eps)."""
+    sc = nonuniform_beach(nx=30, ny=16, seed=3)
+    n = 20
+    o = oracle.OracleHydro(sc)
+    o.run(n)
+    st = o.get_state()
+    dh = np.random.default_rng(0).normal(size=sc.h.shape)
+    t = oracle.TangentHydro(sc, dh=dh)
+    t.run(n)
+    vals, tans = t.get_state()
+    for k in st:
+        assert np.array_equal(vals[k], st[k]), k
+
+    def final(h=None, cD0=None):
+        s2 = nonuniform_beach(nx=30, ny=16, seed=3)
+        if h is not None:
+            s2.h = np.ascontiguousarray(h)
+        if cD0 is not None:
+            s2.physics.cD0 = cD0
+        o2 = oracle.OracleHydro(s2)
+        o2.run(n)
+        return o2.get_state()
+
+    eps = 1e-6
+    sp, sm = final(h=sc.h + eps * dh), final(h=sc.h - eps * dh)
+    for k in ("Zs", "D", "Qx", "Qy", "U", "V"):
+        fd = (sp[k] - sm[k]) / (2 * eps)
+        assert np.max(np.abs(fd - tans[k])) <= 1e-8 * np.max(np.abs(tans[k])), k
+    c0 = sc.physics.cD0
+    tc = oracle.TangentHydro(sc, dparams={"cD0": 1.0})
+    tc.run(n)
+    _, tanc = tc.get_state()
+    e = c0 * 1e-5
+    sp, sm = final(cD0=c0 + e), final(cD0=c0 - e)
+    fd = (sp["Zs"] - sm["Zs"]) / (2 * e)
+    assert np.max(np.abs(fd - tanc["Zs"])) <= 1e-6 * np.max(np.abs(tanc["Zs"]))
