@@ -208,6 +208,34 @@ int m70h_batch_run(m70h_batch *b, int nIter);
 int m70h_batch_run_async(m70h_batch *b, int nIter);
 int m70h_batch_synchronize(m70h_batch *b);
 
+/* ------------------------------------------------------------------------------------
+ * Adjoint of Run. No counterpart in the reference, which has no derivatives: these
+ * entry points serve callers that fit bathymetry or friction to observations (gradient
+ * of a loss of the final state with respect to the initial state and the drag and
+ * viscosity parameters).
+ *
+ * m70h_adjoint_forward runs Run(nIter) like m70h_run (same pre-loop, same steps, same
+ * final state) and keeps `slots` state checkpoints (>= 1; at most nIter are used), placed
+ * by the optimal binomial (Revolve) schedule. m70h_adjoint_reverse then takes the
+ * adjoint of the final state (seed, device pointers, NULL fields = zero) and returns the
+ * adjoint of the state before the Run (grad, device pointers, NULL = not returned) and,
+ * in dparams[0..2] (host), the adjoints of the parameters cD0, nu0 and M. The depth
+ * adjoint includes the per-Run drag and viscosity fields' dependence on that depth.
+ * Steps are recomputed from checkpoints with the intermediates taped; wet/dry masks are
+ * held fixed (their derivative is zero almost everywhere). Afterwards the handle's state
+ * is the final state again. Any Run or forcing change in between makes the reverse fail
+ * with M70H_ESTATE. m70h_adjoint_release frees the checkpoints and work buffers.
+ * ------------------------------------------------------------------------------------ */
+typedef struct m70h_fields {
+    double *Zs, *D, *Qx, *Qy, *U, *V, *U_c, *V_c, *dQx_dx, *dQy_dy;
+} m70h_fields;
+
+int m70h_adjoint_forward(m70h *h, int nIter, int slots);
+int m70h_adjoint_reverse(m70h *h, const m70h_fields *seed_dev, const m70h_fields *grad_dev, double *dparams);
+/* Untaped and taped step evaluations of the last reverse sweep, and buffers in use. */
+int m70h_adjoint_stats(const m70h *h, long *forward_steps, long *taped_steps, int *buffers);
+int m70h_adjoint_release(m70h *h);
+
 #ifdef __cplusplus
 }
 #endif

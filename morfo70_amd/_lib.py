@@ -58,7 +58,19 @@ SIGNATURES = {
     "m70h_batch_run": (ctypes.c_int, [_vp, ctypes.c_int]),
     "m70h_batch_run_async": (ctypes.c_int, [_vp, ctypes.c_int]),
     "m70h_batch_synchronize": (ctypes.c_int, [_vp]),
+    "m70h_adjoint_forward": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "m70h_adjoint_reverse": (ctypes.c_int, [_vp, _vp, _vp, _dp]),
+    "m70h_adjoint_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),
+                                          ctypes.POINTER(ctypes.c_int)]),
+    "m70h_adjoint_release": (ctypes.c_int, [_vp]),
 }
+
+STATE_FIELDS = ("Zs", "D", "Qx", "Qy", "U", "V", "U_c", "V_c", "dQx_dx", "dQy_dy")
+
+
+class Fields(ctypes.Structure):
+    """m70h_fields (include/m70h.h): ten device pointers in state order."""
+    _fields_ = [(k, ctypes.c_void_p) for k in STATE_FIELDS]
 
 
 class M70HError(RuntimeError):

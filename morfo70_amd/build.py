@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libm70h.so")
 
-SOURCES = ["hydro_kernels.hip", "line_solvers.hip", "partition_solver.hip", "m70h_api.hip"]
+SOURCES = ["hydro_kernels.hip", "line_solvers.hip", "partition_solver.hip", "adjoint.hip", "m70h_api.hip"]
 
 # -ffp-contract=off: no fused multiply-adds, so every product and sum rounds as the
 #   reference's x86-64 build does.

@@ -13,7 +13,7 @@ namespace m70 {
 
 // Qx predictor, src/HydroQx.cpp:23-48 and :99-256 with setSolverQxx_BC
 // (src/HydroBoundaryConditions.cpp:25-39). Side outputs: IsWater and DragTerm of row i.
-__device__ __forceinline__ void asm_qxx(const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
+__host__ __device__ __forceinline__ void asm_qxx(const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
                                         const Work &w, double f0, int i, int j, double &a_out, double &b_out,
                                         double &c_out, double &f_out)
 {
@@ -92,7 +92,7 @@ __device__ __forceinline__ void asm_qxx(const Consts &k, const Metrics &m, const
 }
 
 // Qx projector, setSolverQxy_f/_d/_up/_low, src/HydroQx.cpp:262-349.
-__device__ __forceinline__ void asm_qxy(const Consts &k, const State &s, const RunFields &rf, const Work &w, int i,
+__host__ __device__ __forceinline__ void asm_qxy(const Consts &k, const State &s, const RunFields &rf, const Work &w, int i,
                                         int j, double &a_out, double &b_out, double &c_out, double &f_out)
 {
     const int ny = k.ny;
@@ -120,7 +120,7 @@ __device__ __forceinline__ void asm_qxy(const Consts &k, const State &s, const R
 
 // Qy predictor, src/HydroQy.cpp:26-41 with :92-230; f and d use IsWater row 0
 // (src/HydroQy.cpp:140,170). Side outputs: IsWater and DragTerm of row i.
-__device__ __forceinline__ void asm_qyy(const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
+__host__ __device__ __forceinline__ void asm_qyy(const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
                                         const Work &w, int i, int j, double &a_out, double &b_out, double &c_out,
                                         double &f_out)
 {
@@ -180,7 +180,7 @@ __device__ __forceinline__ void asm_qyy(const Consts &k, const Metrics &m, const
 
 // Qy projector, setSolverQyx_* (src/HydroQy.cpp:235-362) and setSolverQyx_BC
 // (src/HydroBoundaryConditions.cpp:41-60).
-__device__ __forceinline__ void asm_qyx(const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
+__host__ __device__ __forceinline__ void asm_qyx(const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
                                         const Work &w, int i, int j, double &a_out, double &b_out, double &c_out,
                                         double &f_out)
 {

@@ -127,18 +127,18 @@ __device__ __forceinline__ int scen_of(const int *__restrict__ start, int nscen,
 // reduced-system column solver; on the streaming row and column kernels it cut fetched
 // bytes by 5-20% but made them slower (they are latency-bound, and eight far-apart
 // streams of rows spread the DRAM pages), so they keep the plain order.
-__device__ __forceinline__ int xcd_chunk(int b, int N)
+__host__ __device__ __forceinline__ int xcd_chunk(int b, int N)
 {
     const int q = N >> 3, r = N & 7, x = b & 7, l = b >> 3;
     return x * q + (x < r ? x : r) + l;
 }
 
-__device__ __forceinline__ int wrap_inc(int j, int ny) { return j + 1 == ny ? 0 : j + 1; }
-__device__ __forceinline__ int wrap_dec(int j, int ny) { return j == 0 ? ny - 1 : j - 1; }
+__host__ __device__ __forceinline__ int wrap_inc(int j, int ny) { return j + 1 == ny ? 0 : j + 1; }
+__host__ __device__ __forceinline__ int wrap_dec(int j, int ny) { return j == 0 ? ny - 1 : j - 1; }
 
 // Body of Hydro::setWaterNodes_x/_y (src/HydroQx.cpp:245-253, src/HydroQy.cpp:103-109):
 // integer arithmetic on the six comparisons, then conversion to bool.
-__device__ __forceinline__ int wet_rule(double d, double dup, double zs, double zsup, double s,
+__host__ __device__ __forceinline__ int wet_rule(double d, double dup, double zs, double zsup, double s,
                                         double sup, double Dmin)
 {
     int dup_gt_min = dup > Dmin;
@@ -152,7 +152,7 @@ __device__ __forceinline__ int wet_rule(double d, double dup, double zs, double 
 }
 
 // Hydro::calculateDragTerm, src/HydroParametrizations.cpp:19-28 (one element).
-__device__ __forceinline__ double drag_term(double cd, double urms, double u, double v, double inv_d,
+__host__ __device__ __forceinline__ double drag_term(double cd, double urms, double u, double v, double inv_d,
                                             const Consts &k)
 {
     return cd * urms * k.inv_sqrt2 * sqrt(k.alpha_f2 + 2 * (v * v + u * u) / (urms * urms)) * inv_d;

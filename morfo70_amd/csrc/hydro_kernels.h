@@ -35,6 +35,12 @@ hipError_t launch_partition_cols(hipStream_t st, const double *A, const double *
 hipError_t launch_partition_rows(hipStream_t st, const double *A, const double *B, const double *C, const double *F,
                                  long ld, double *Y, long ldy, int nL, int n, bool periodic, int tile);
 
+// Transposed solves of the adjoint (A^T x = F for the systems A stored in A, B, C).
+hipError_t launch_transposed_cols(hipStream_t st, const double *A, const double *B, const double *C, const double *F,
+                                  double *Y, int n, int m, long ld, int tile);
+hipError_t launch_transposed_rows(hipStream_t st, const double *A, const double *B, const double *C, const double *F,
+                                  long ld, double *Y, long ldy, int nL, int n, int tile);
+
 int cols_segments(int n);
 enum ColsPass { COLS_ELIM = 0, COLS_REDUCED = 1, COLS_BACKSUB = 2 };
 hipError_t launch_cols3_qxx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,

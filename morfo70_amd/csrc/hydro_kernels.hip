@@ -30,7 +30,7 @@ namespace m70 {
 
 // Hydro::calculateDragAndViscosity with the model functions,
 // src/HydroParametrizations.cpp:30-113. Rows 0..nx-1.
-__device__ __forceinline__ void drag_viscosity_body(const Consts &k, const double *__restrict__ D, const Forcing &fw,
+__host__ __device__ __forceinline__ void drag_viscosity_body(const Consts &k, const double *__restrict__ D, const Forcing &fw,
                                                     const RunFields &rf, int i, int j)
 {
     const int ny = k.ny;
@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(256) k_drag_viscosity(Consts k, const double *
 
 // Forcing, face averages and viscosity derivatives. Rows 0..nx-1 (x-face fields only
 // for rows 0..nx-2).
-__device__ __forceinline__ void run_fields_body(const Consts &k, const Metrics &m, const Forcing &fw, const RunFields &rf,
+__host__ __device__ __forceinline__ void run_fields_body(const Consts &k, const Metrics &m, const Forcing &fw, const RunFields &rf,
                                                 int i, int j)
 {
     const int ny = k.ny;
@@ -177,13 +177,13 @@ __global__ void __launch_bounds__(256) k_qxy_assemble(Consts k, State s, RunFiel
 // (src/HydroQx.cpp:80), solveZs_guess (src/Hydro.cpp:230-250), getU0/getU
 // (src/HydroQx.cpp:420-451), X2C (include/MeshOperations.h:36-47). Rows 0..nx-1.
 // ------------------------------------------------------------------------------------
-__device__ __forceinline__ double qx_at(const State &s, const Consts &k, long rowoff, int row, int j,
+__host__ __device__ __forceinline__ double qx_at(const State &s, const Consts &k, long rowoff, int row, int j,
                                         double f0)
 {
     return row == 0 ? k.f1 * s.Qx[k.ny + j] + f0 : s.Qx[rowoff + j];
 }
 
-__device__ __forceinline__ void qx_tail_body(const Consts &k, const Metrics &m, const State &s, const Work &w, double f0,
+__host__ __device__ __forceinline__ void qx_tail_body(const Consts &k, const Metrics &m, const State &s, const Work &w, double f0,
                                              int i, int j)
 {
     const int ny = k.ny;
@@ -269,25 +269,25 @@ __global__ void __launch_bounds__(256) k_qyx_assemble(Consts k, Metrics m, State
 // (src/HydroQy.cpp:458-487), Y2C (src/HydroMeshOperations.cpp:88-97). Rows 0..nx-1.
 // Reads D, writes the new depth to Dn.
 // ------------------------------------------------------------------------------------
-__device__ __forceinline__ double qy_row(const State &s, const Consts &k, int row, int j)
+__host__ __device__ __forceinline__ double qy_row(const State &s, const Consts &k, int row, int j)
 {   // Qy row `row` in 1..nx after the BC: row nx is Qy[nx-1]*inv_3
     return row < k.nx ? s.Qy[(long)row * k.ny + j] : s.Qy[(long)(k.nx - 1) * k.ny + j] * k.inv_3;
 }
 
-__device__ __forceinline__ double dqy_at(const State &s, const Consts &k, int i, int j)
+__host__ __device__ __forceinline__ double dqy_at(const State &s, const Consts &k, int i, int j)
 {
     const int jm = wrap_dec(j, k.ny);
     return (qy_row(s, k, i + 1, j) - qy_row(s, k, i + 1, jm)) * k.inv_dy;
 }
 
-__device__ __forceinline__ double new_depth(const State &s, const Consts &k, int i, int j)
+__host__ __device__ __forceinline__ double new_depth(const State &s, const Consts &k, int i, int j)
 {
     const long q = (long)i * k.ny + j;
     const double incrZs = -k.dt * (s.dQx[q] + dqy_at(s, k, i, j));
     return s.D[q] + incrZs;
 }
 
-__device__ __forceinline__ void qy_tail_body(const Consts &k, const State &s, int i, int j)
+__host__ __device__ __forceinline__ void qy_tail_body(const Consts &k, const State &s, int i, int j)
 {
     const int ny = k.ny;
     const long r = (long)i * ny;

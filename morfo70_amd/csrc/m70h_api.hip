@@ -10,10 +10,13 @@
 #include <vector>
 
 #include "../../include/m70h.h"
+#include "adjoint_kernels.h"
 #include "hydro_device.h"
 #include "hydro_kernels.h"
 
 using namespace m70;
+
+struct AdjCtx;
 
 namespace {
 
@@ -58,6 +61,8 @@ struct m70h {
     double *sr = nullptr, *sv = nullptr, *sk = nullptr;
     ColsWork cw{};
     int *flag = nullptr;
+    AdjCtx *adj = nullptr;   // checkpoints and work of the adjoint (m70h_adjoint_*)
+    long run_gen = 0;        // bumped by every Run and forcing change
     // profiling
     bool prof = false;
     struct Rec { int kid; hipEvent_t a, b; };
@@ -332,6 +337,8 @@ void grid_metrics(int nx, const double *x, std::vector<double> &dx, std::vector<
     x2c_d[nx - 1] = 0.5;
 }
 
+void adj_release(m70h *h);
+
 int copy_fields(m70h *h, double *const *dst, const double *const *src, const long *n, int count, hipMemcpyKind kind)
 {
     for (int q = 0; q < count; ++q) {
@@ -452,6 +459,7 @@ void m70h_destroy(m70h *h)
 {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    adj_release(h);
     for (void *q : h->allocs) (void)hipFree(q);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
@@ -550,6 +558,7 @@ static int forcing_xfer(m70h *h, const double *const *src, hipMemcpyKind kind)
     for (auto &v : n) v = h->F;
     int rc = copy_fields(h, h->fw_buf, src, n, 7, kind);
     if (rc == M70H_OK) h->forcing_set = true;
+    h->run_gen++;
     return rc;
 }
 
@@ -601,6 +610,7 @@ int m70h_run_async(m70h *h, int nIter)
     if (!h) return M70H_EINVAL;
     if (nIter < 0) return fail(h, M70H_EINVAL, "nIter < 0");
     if (!h->forcing_set) return fail(h, M70H_ESTATE, "forcing not set");
+    h->run_gen++;
     int rc = run_preloop(h);
     if (rc) return rc;
     for (int it = 0; it < nIter; ++it)
@@ -953,6 +963,428 @@ int m70h_batch_run(m70h_batch *b, int nIter)
     int rc = m70h_batch_run_async(b, nIter);
     if (rc) return rc;
     return m70h_batch_synchronize(b);
+}
+
+}  // extern "C"
+
+// =====================================================================================
+// Adjoint of Run (no counterpart in the reference): checkpointed forward, taped step,
+// reverse sweep. The checkpoint schedule is the optimal binomial one (Griewank's
+// Revolve), found by dynamic programming over (steps, free slots).
+// =====================================================================================
+struct AdjCtx {
+    int N = 0;                   // steps of the recorded Run
+    int S = 0;                   // free slots of the schedule (state buffers minus the start)
+    long f0_pos0 = 0;            // offshore-hook position at the start of the Run
+    long gen = 0;                // run generation the recording belongs to
+    std::vector<void *> allocs;
+    std::vector<double *> slot;  // state buffers, 6F + 4F+ doubles each
+    std::vector<int> slot_step;  // step held by each buffer, -1 when free
+    double *s0buf = nullptr;     // state before the step being reversed, when no slot holds it
+    double *final_buf = nullptr; // state after the Run, restored at the end of the reverse sweep
+    std::vector<int> split;      // first advance of the optimal schedule, (N+1) x (S+1)
+    StepTape T{};
+    AdjFields A[2]{};
+    AdjWork W{};
+    AdjRun R{};
+    double *params = nullptr;    // device: adjoints of cD0, nu0, M
+    long fwd_steps = 0, taped_steps = 0;
+    int cur = 0;                 // A[cur] holds the adjoint of the current state
+};
+
+namespace {
+
+void adj_release(m70h *h)
+{
+    if (!h || !h->adj) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (void *q : h->adj->allocs) (void)hipFree(q);
+    delete h->adj;
+    h->adj = nullptr;
+}
+
+hipError_t adj_alloc(AdjCtx *a, double **p, long n)
+{
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, (size_t)n * sizeof(double) + 64);
+    if (e != hipSuccess) return e;
+    a->allocs.push_back(q);
+    *p = (double *)q;
+    return hipMemset(q, 0, (size_t)n * sizeof(double) + 64);
+}
+
+State buf_state(const m70h *h, double *b)
+{
+    const long F = h->F, Fp = h->Fp;
+    State s{};
+    s.Zs = b;
+    s.D = b + F;
+    s.Qx = b + 2 * F;
+    s.Qy = s.Qx + Fp;
+    s.U = s.Qy + Fp;
+    s.V = s.U + Fp;
+    s.Uc = s.V + Fp;
+    s.Vc = s.Uc + F;
+    s.dQx = s.Vc + F;
+    s.dQy = s.dQx + F;
+    return s;
+}
+
+hipError_t copy_state(m70h *h, const State &dst, const State &src)
+{
+    const long F = h->F, Fp = h->Fp;
+    double *d[10] = {dst.Zs, dst.D, dst.Qx, dst.Qy, dst.U, dst.V, dst.Uc, dst.Vc, dst.dQx, dst.dQy};
+    const double *s[10] = {src.Zs, src.D, src.Qx, src.Qy, src.U, src.V, src.Uc, src.Vc, src.dQx, src.dQy};
+    const long n[10] = {F, F, Fp, Fp, Fp, Fp, F, F, F, F};
+    for (int q = 0; q < 10; ++q) {
+        hipError_t e = hipMemcpyAsync(d[q], s[q], (size_t)n[q] * sizeof(double), hipMemcpyDeviceToDevice, h->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t zero_fields(m70h *h, const AdjFields &A)
+{
+    const long F = h->F, Fp = h->Fp;
+    double *d[10] = {A.Zs, A.D, A.Qx, A.Qy, A.U, A.V, A.Uc, A.Vc, A.dQx, A.dQy};
+    const long n[10] = {F, F, Fp, Fp, Fp, Fp, F, F, F, F};
+    for (int q = 0; q < 10; ++q) {
+        hipError_t e = hipMemsetAsync(d[q], 0, (size_t)n[q] * sizeof(double), h->stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+double f0_at(m70h *h, long pos)
+{
+    if (!h->f0_series.empty()) {
+        const long q = pos < (long)h->f0_series.size() ? pos : (long)h->f0_series.size() - 1;
+        return h->f0_series[q];
+    }
+    return h->f0;
+}
+
+// Allocates the context for a Run of N steps with `slots` state buffers.
+int adj_setup(m70h *h, int N, int slots)
+{
+    adj_release(h);
+    AdjCtx *a = new AdjCtx();
+    h->adj = a;
+    a->N = N;
+    a->S = slots - 1 < N - 1 ? slots - 1 : N - 1;
+    if (a->S < 0) a->S = 0;
+    const long F = h->F, Fp = h->Fp, SB = 6 * F + 4 * Fp;
+    hipError_t e;
+    a->slot.assign(a->S + 1, nullptr);
+    a->slot_step.assign(a->S + 1, -1);
+    for (auto &b : a->slot)
+        if ((e = adj_alloc(a, &b, SB)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc checkpoints");
+    if ((e = adj_alloc(a, &a->s0buf, SB)) != hipSuccess || (e = adj_alloc(a, &a->final_buf, SB)) != hipSuccess)
+        return hip_fail(h, e, "adjoint: hipMalloc state");
+    StepTape &T = a->T;
+    double **tp[] = {&T.a1, &T.b1, &T.c1, &T.qxs, &T.a2, &T.b2, &T.c2, &T.a3, &T.b3, &T.c3, &T.qys,
+                     &T.a4, &T.b4, &T.c4, &T.f, &T.dragx, &T.dragy, &T.gD, &T.gZs};
+    for (auto q : tp)
+        if ((e = adj_alloc(a, q, F)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc tape");
+    double *mk = nullptr;
+    if ((e = adj_alloc(a, &mk, F / 4 + 1)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc tape");
+    T.wx = (uint8_t *)mk;
+    if ((e = adj_alloc(a, &mk, F / 4 + 1)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc tape");
+    T.wy = (uint8_t *)mk;
+    for (int q = 0; q < 2; ++q) {
+        AdjFields &A = a->A[q];
+        double **fp[] = {&A.Zs, &A.D, &A.Uc, &A.Vc, &A.dQx, &A.dQy};
+        for (auto p : fp)
+            if ((e = adj_alloc(a, p, F)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc adjoint");
+        double **pp[] = {&A.Qx, &A.Qy, &A.U, &A.V};
+        for (auto p : pp)
+            if ((e = adj_alloc(a, p, Fp)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc adjoint");
+    }
+    AdjWork &W = a->W;
+    double **wp[] = {&W.rhs, &W.lam, &W.dragx, &W.dragy, &W.UcN, &W.dQxN, &W.gD, &W.dQyN, &W.qyv, &W.qU};
+    for (auto p : wp)
+        if ((e = adj_alloc(a, p, F)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc work");
+    if ((e = adj_alloc(a, &W.UN, Fp)) != hipSuccess || (e = adj_alloc(a, &W.qy0v, h->ny)) != hipSuccess)
+        return hip_fail(h, e, "adjoint: hipMalloc work");
+    AdjRun &R = a->R;
+    double **rp[] = {&R.cD_x, &R.nuD_x, &R.dnuD_dx, &R.dnuDx_dy, &R.cD_y, &R.nuD_y, &R.dnuD_dy, &R.dnuDy_dx,
+                     &R.cD, &R.nuD};
+    for (auto p : rp)
+        if ((e = adj_alloc(a, p, F)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc run adjoints");
+    if ((e = adj_alloc(a, &a->params, 4)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc params");
+    // schedule: cost(l, s) = fewest untaped steps to reverse l steps from a stored start
+    // with s free buffers; split(l, s) = the first advance that achieves it
+    const int S = a->S;
+    std::vector<long> cost((size_t)(N + 1) * (S + 1), 0);
+    a->split.assign((size_t)(N + 1) * (S + 1), 1);
+    auto C = [&](int l, int s) -> long & { return cost[(size_t)l * (S + 1) + s]; };
+    for (int l = 2; l <= N; ++l) {
+        C(l, 0) = (long)l * (l - 1) / 2;
+        a->split[(size_t)l * (S + 1)] = l - 1;
+    }
+    for (int s = 1; s <= S; ++s)
+        for (int l = 2; l <= N; ++l) {
+            long best = -1;
+            int bj = 1;
+            for (int j = 1; j < l; ++j) {
+                const long c = j + C(l - j, s - 1) + C(j, s);
+                if (best < 0 || c < best) { best = c; bj = j; }
+            }
+            C(l, s) = best;
+            a->split[(size_t)l * (S + 1) + s] = bj;
+        }
+    return M70H_OK;
+}
+
+int adj_store(m70h *h, int step)
+{
+    AdjCtx *a = h->adj;
+    for (size_t q = 0; q < a->slot.size(); ++q)
+        if (a->slot_step[q] < 0) {
+            HIPCHK(h, copy_state(h, buf_state(h, a->slot[q]), h->s));
+            a->slot_step[q] = step;
+            return M70H_OK;
+        }
+    return fail(h, M70H_ESTATE, "adjoint: no free checkpoint buffer (schedule error)");
+}
+
+int adj_find(m70h *h, int step)
+{
+    AdjCtx *a = h->adj;
+    for (size_t q = 0; q < a->slot.size(); ++q)
+        if (a->slot_step[q] == step) return (int)q;
+    return -1;
+}
+
+// Advances h->s from step `from` by n untaped steps.
+int adj_advance(m70h *h, int from, int n)
+{
+    for (int t = 0; t < n; ++t) {
+        h->f0_pos = h->adj->f0_pos0 + from + t;
+        int rc = run_step(h);
+        if (rc) return rc;
+        h->adj->fwd_steps++;
+    }
+    return M70H_OK;
+}
+
+// One step from h->s with its intermediates taped (assembly kernels + partition solves).
+int adj_taped_step(m70h *h, int t)
+{
+    AdjCtx *a = h->adj;
+    const StepTape &T = a->T;
+    const double f0 = f0_at(h, a->f0_pos0 + t);
+    const int nx = h->nx, ny = h->ny;
+    const Consts &k = h->k;
+    State &s = h->s;
+    hipStream_t st = h->stream;
+    const long nF = (long)(nx - 1) * ny;
+    Work w = h->w;
+    w.gD = T.gD;
+    w.gZs = T.gZs;
+    Work wx = w, wy = w;
+    wx.w = T.wx; wx.drag = T.dragx;
+    wy.w = T.wy; wy.drag = T.dragy;
+    hipError_t e;
+    wx.a = T.a1; wx.b = T.b1; wx.c = T.c1; wx.f = T.f;
+    if ((e = launch_qxx_assemble(st, k, h->m, s, h->rf, wx, f0)) != hipSuccess) return hip_fail(h, e, "adjoint tape qxx");
+    if ((e = launch_partition_cols(st, T.a1, T.b1, T.c1, T.f, s.Qx + ny, nx - 1, ny, ny, h->tile_cols)) != hipSuccess)
+        return hip_fail(h, e, "adjoint tape thomas x");
+    HIPCHK(h, hipMemcpyAsync(T.qxs, s.Qx + ny, (size_t)nF * sizeof(double), hipMemcpyDeviceToDevice, st));
+    wx.a = T.a2; wx.b = T.b2; wx.c = T.c2;
+    if ((e = launch_qxy_assemble(st, k, s, h->rf, wx)) != hipSuccess) return hip_fail(h, e, "adjoint tape qxy");
+    if ((e = launch_partition_rows(st, T.a2, T.b2, T.c2, T.f, ny, s.Qx + ny, ny, nx - 1, ny, true, h->tile_rows)) !=
+        hipSuccess)
+        return hip_fail(h, e, "adjoint tape periodic x");
+    if ((e = launch_qx_tail(st, k, h->m, s, wx, f0)) != hipSuccess) return hip_fail(h, e, "adjoint tape qx tail");
+    wy.a = T.a3; wy.b = T.b3; wy.c = T.c3; wy.f = T.f;
+    if ((e = launch_qyy_assemble(st, k, h->m, s, h->rf, wy)) != hipSuccess) return hip_fail(h, e, "adjoint tape qyy");
+    if ((e = launch_partition_rows(st, T.a3, T.b3, T.c3, T.f, ny, s.Qy + ny, ny, nx - 1, ny, true, h->tile_rows)) !=
+        hipSuccess)
+        return hip_fail(h, e, "adjoint tape periodic y");
+    HIPCHK(h, hipMemcpyAsync(T.qys, s.Qy + ny, (size_t)nF * sizeof(double), hipMemcpyDeviceToDevice, st));
+    wy.a = T.a4; wy.b = T.b4; wy.c = T.c4;
+    if ((e = launch_qyx_assemble(st, k, h->m, s, h->rf, wy)) != hipSuccess) return hip_fail(h, e, "adjoint tape qyx");
+    if ((e = launch_partition_cols(st, T.a4, T.b4, T.c4, T.f, s.Qy + ny, nx - 1, ny, ny, h->tile_cols)) != hipSuccess)
+        return hip_fail(h, e, "adjoint tape thomas y");
+    if ((e = launch_qy_tail(st, k, s)) != hipSuccess) return hip_fail(h, e, "adjoint tape qy tail");
+    std::swap(s.D, s.Dn);
+    a->taped_steps++;
+    return M70H_OK;
+}
+
+// Reverses step t: h->s holds s_t, S0 is a buffer holding s_t that stays untouched.
+int adj_reverse_one(m70h *h, int t, const State &S0)
+{
+    AdjCtx *a = h->adj;
+    int rc = adj_taped_step(h, t);
+    if (rc) return rc;
+    const int out = 1 - a->cur;
+    const AdjFields &Ain = a->A[a->cur], &Aout = a->A[out];
+    HIPCHK(h, zero_fields(h, Aout));
+    const AdjWork &W = a->W;
+    HIPCHK(h, hipMemsetAsync(W.UN, 0, (size_t)h->Fp * sizeof(double), h->stream));
+    HIPCHK(h, hipMemsetAsync(W.UcN, 0, (size_t)h->F * sizeof(double), h->stream));
+    HIPCHK(h, hipMemsetAsync(W.gD, 0, (size_t)h->F * sizeof(double), h->stream));
+    hipError_t e;
+    if ((e = launch_adj_y(h->stream, h->k, h->m, S0, h->s, h->rf, a->T, Ain, Aout, W, a->R, h->tile_cols)) != hipSuccess)
+        return hip_fail(h, e, "adjoint y half-step");
+    const double f0 = f0_at(h, a->f0_pos0 + t);
+    if ((e = launch_adj_x(h->stream, h->k, h->m, S0, h->s, h->rf, a->T, Ain, Aout, W, a->R, f0, h->tile_cols)) !=
+        hipSuccess)
+        return hip_fail(h, e, "adjoint x half-step");
+    a->cur = out;
+    return M70H_OK;
+}
+
+// Reverses steps a0..b0-1; the state of step a0 is held by a checkpoint buffer and s free
+// buffers remain.
+int adj_reverse_range(m70h *h, int a0, int b0, int s)
+{
+    AdjCtx *a = h->adj;
+    const int qa = adj_find(h, a0);
+    if (qa < 0) return fail(h, M70H_ESTATE, "adjoint: missing checkpoint");
+    const State Sa = buf_state(h, a->slot[qa]);
+    int rc;
+    if (b0 - a0 == 1) {
+        HIPCHK(h, copy_state(h, h->s, Sa));
+        return adj_reverse_one(h, a0, Sa);
+    }
+    if (s == 0) {
+        for (int t = b0 - 1; t >= a0; --t) {
+            HIPCHK(h, copy_state(h, h->s, Sa));
+            if (t == a0) {
+                if ((rc = adj_reverse_one(h, t, Sa))) return rc;
+            } else {
+                if ((rc = adj_advance(h, a0, t - a0))) return rc;
+                const State S0 = buf_state(h, a->s0buf);
+                HIPCHK(h, copy_state(h, S0, h->s));
+                if ((rc = adj_reverse_one(h, t, S0))) return rc;
+            }
+        }
+        return M70H_OK;
+    }
+    const int j = a->split[(size_t)(b0 - a0) * (a->S + 1) + s];
+    const int m = a0 + j;
+    if (adj_find(h, m) < 0) {
+        HIPCHK(h, copy_state(h, h->s, Sa));
+        if ((rc = adj_advance(h, a0, j))) return rc;
+        if ((rc = adj_store(h, m))) return rc;
+    }
+    if ((rc = adj_reverse_range(h, m, b0, s - 1))) return rc;
+    a->slot_step[adj_find(h, m)] = -1;
+    return adj_reverse_range(h, a0, m, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int m70h_adjoint_forward(m70h *h, int nIter, int slots)
+{
+    if (!h) return M70H_EINVAL;
+    if (nIter < 1) return fail(h, M70H_EINVAL, "adjoint: nIter must be >= 1");
+    if (slots < 1) return fail(h, M70H_EINVAL, "adjoint: slots must be >= 1");
+    if (!h->forcing_set) return fail(h, M70H_ESTATE, "forcing not set");
+    if ((double)nIter * nIter * (slots < nIter ? slots : nIter) > 4e9)
+        return fail(h, M70H_EINVAL, "adjoint: schedule too large (split the Run or use fewer slots)");
+    int rc = adj_setup(h, nIter, slots);
+    if (rc) { adj_release(h); return rc; }
+    AdjCtx *a = h->adj;
+    h->run_gen++;
+    a->gen = h->run_gen;
+    a->f0_pos0 = h->f0_pos;
+    if ((rc = run_preloop(h))) return rc;
+    if ((rc = adj_store(h, 0))) return rc;
+    // first sweep: store the chain of checkpoints the schedule starts with
+    int pos = 0, l = nIter, s = a->S;
+    while (l > 1 && s > 0) {
+        const int j = a->split[(size_t)l * (a->S + 1) + s];
+        if ((rc = adj_advance(h, pos, j))) return rc;
+        pos += j;
+        if ((rc = adj_store(h, pos))) return rc;
+        l -= j;
+        --s;
+    }
+    if ((rc = adj_advance(h, pos, nIter - pos))) return rc;
+    HIPCHK(h, copy_state(h, buf_state(h, a->final_buf), h->s));
+    h->f0_pos = a->f0_pos0 + nIter;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return M70H_OK;
+}
+
+int m70h_adjoint_reverse(m70h *h, const m70h_fields *seed, const m70h_fields *grad, double *dparams)
+{
+    if (!h) return M70H_EINVAL;
+    AdjCtx *a = h->adj;
+    if (!a) return fail(h, M70H_ESTATE, "adjoint: no recorded Run (call m70h_adjoint_forward)");
+    if (a->gen != h->run_gen) return fail(h, M70H_ESTATE, "adjoint: the handle ran or changed forcing since the recording");
+    const long F = h->F, Fp = h->Fp;
+    a->cur = 0;
+    HIPCHK(h, zero_fields(h, a->A[0]));
+    if (seed) {
+        const double *src[10] = {seed->Zs, seed->D, seed->Qx, seed->Qy, seed->U, seed->V, seed->U_c, seed->V_c,
+                                 seed->dQx_dx, seed->dQy_dy};
+        const AdjFields &A = a->A[0];
+        double *dst[10] = {A.Zs, A.D, A.Qx, A.Qy, A.U, A.V, A.Uc, A.Vc, A.dQx, A.dQy};
+        const long n[10] = {F, F, Fp, Fp, Fp, Fp, F, F, F, F};
+        for (int q = 0; q < 10; ++q)
+            if (src[q])
+                HIPCHK(h, hipMemcpyAsync(dst[q], src[q], (size_t)n[q] * sizeof(double), hipMemcpyDeviceToDevice,
+                                         h->stream));
+    }
+    const AdjRun &R = a->R;
+    double *rz[] = {R.cD_x, R.nuD_x, R.dnuD_dx, R.dnuDx_dy, R.cD_y, R.nuD_y, R.dnuD_dy, R.dnuDy_dx, R.cD, R.nuD};
+    for (double *p : rz) HIPCHK(h, hipMemsetAsync(p, 0, (size_t)F * sizeof(double), h->stream));
+    HIPCHK(h, hipMemsetAsync(a->params, 0, 4 * sizeof(double), h->stream));
+    a->fwd_steps = 0;
+    a->taped_steps = 0;
+    // the first sweep's checkpoints are still held; reverse all steps
+    int rc = adj_reverse_range(h, 0, a->N, a->S);
+    if (rc) return rc;
+    // per-Run pre-loop: drag/viscosity fields were formed from the depth at step 0
+    const int q0 = adj_find(h, 0);
+    if (q0 < 0) return fail(h, M70H_ESTATE, "adjoint: missing start checkpoint");
+    const State S0 = buf_state(h, a->slot[q0]);
+    const AdjFields &L = a->A[a->cur];
+    hipError_t e;
+    if ((e = launch_adj_preloop(h->stream, h->k, h->m, S0.D, h->fw, R, L.D, a->params)) != hipSuccess)
+        return hip_fail(h, e, "adjoint pre-loop");
+    if (grad) {
+        double *dst[10] = {grad->Zs, grad->D, grad->Qx, grad->Qy, grad->U, grad->V, grad->U_c, grad->V_c,
+                           grad->dQx_dx, grad->dQy_dy};
+        const double *src[10] = {L.Zs, L.D, L.Qx, L.Qy, L.U, L.V, L.Uc, L.Vc, L.dQx, L.dQy};
+        const long n[10] = {F, F, Fp, Fp, Fp, Fp, F, F, F, F};
+        for (int q = 0; q < 10; ++q)
+            if (dst[q])
+                HIPCHK(h, hipMemcpyAsync(dst[q], src[q], (size_t)n[q] * sizeof(double), hipMemcpyDeviceToDevice,
+                                         h->stream));
+    }
+    double hp[4] = {0, 0, 0, 0};
+    HIPCHK(h, hipMemcpyAsync(hp, a->params, 4 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    // leave the handle at the state after the Run, as a plain Run would
+    HIPCHK(h, copy_state(h, h->s, buf_state(h, a->final_buf)));
+    h->f0_pos = a->f0_pos0 + a->N;
+    // the first sweep's chain is consumed; a second reverse recomputes it from step 0
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (dparams) { dparams[0] = hp[0]; dparams[1] = hp[1]; dparams[2] = hp[2]; }
+    return M70H_OK;
+}
+
+int m70h_adjoint_stats(const m70h *h, long *forward_steps, long *taped_steps, int *buffers)
+{
+    if (!h || !h->adj) return M70H_EINVAL;
+    if (forward_steps) *forward_steps = h->adj->fwd_steps;
+    if (taped_steps) *taped_steps = h->adj->taped_steps;
+    if (buffers) *buffers = (int)h->adj->slot.size();
+    return M70H_OK;
+}
+
+int m70h_adjoint_release(m70h *h)
+{
+    if (!h) return M70H_EINVAL;
+    adj_release(h);
+    return M70H_OK;
 }
 
 }  // extern "C"

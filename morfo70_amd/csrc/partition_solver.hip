@@ -387,6 +387,47 @@ hipError_t launch_partition_rows(hipStream_t st, const double *A, const double *
     return dispatch_rows(st, ProvArrays<false>{A, B, C, F, ld}, Y, ldy, n, nL, periodic, tile);
 }
 
+// Transposed systems for the adjoint: A^T has diagonal a_k, lower c_{k-1} and upper
+// b_{k+1} (periodic: indices wrap, so the corners swap as well).
+struct ProvTransCols {   // columns, non-periodic; element (k, line) at k*ld + line
+    const double *A, *B, *C, *F;
+    long ld;
+    int n;
+    __device__ __forceinline__ void get(int line, int k, double &a, double &b, double &c, double &f) const
+    {
+        const long e = (long)k * ld + line;
+        a = A[e];
+        b = (k >= 1) ? C[e - ld] : 0.0;
+        c = (k <= n - 2) ? B[e + ld] : 0.0;
+        f = F[e];
+    }
+};
+struct ProvTransRows {   // rows, periodic; element (line, k) at line*ld + k
+    const double *A, *B, *C, *F;
+    long ld;
+    int n;
+    __device__ __forceinline__ void get(int line, int k, double &a, double &b, double &c, double &f) const
+    {
+        const long r = (long)line * ld;
+        a = A[r + k];
+        b = C[r + (k == 0 ? n - 1 : k - 1)];
+        c = B[r + (k == n - 1 ? 0 : k + 1)];
+        f = F[r + k];
+    }
+};
+
+hipError_t launch_transposed_cols(hipStream_t st, const double *A, const double *B, const double *C, const double *F,
+                                  double *Y, int n, int m, long ld, int tile)
+{
+    return dispatch_cols(st, ProvTransCols{A, B, C, F, ld, n}, Y, ld, n, m, tile);
+}
+
+hipError_t launch_transposed_rows(hipStream_t st, const double *A, const double *B, const double *C, const double *F,
+                                  long ld, double *Y, long ldy, int nL, int n, int tile)
+{
+    return dispatch_rows(st, ProvTransRows{A, B, C, F, ld, n}, Y, ldy, n, nL, true, tile);
+}
+
 // ---------------------------------------------------------------------------------
 // Column lines (x-sweeps) in three coalesced passes. Thread (column j, segment s) with
 // 64 consecutive columns per wavefront, so every access is a 512-byte run:

@@ -284,6 +284,19 @@ class TangentHydro:
     def run(self, n: int):
         tlm_lib().orc_run(self.h, int(n))
 
+    def set_state(self, values: dict, tangents: dict | None = None):
+        """Sets the state fields given in ``values`` with the tangents in ``tangents``
+        (missing tangents are zero); fields absent from ``values`` keep theirs."""
+        tangents = tangents or {}
+        arrs = []
+        for k in STATE_FIELDS:
+            if k in values:
+                arrs.append(_pair(values[k], tangents.get(k)))
+            else:
+                arrs.append(None)
+        self._st = arrs
+        tlm_lib().orc_set_state(self.h, *[(_vp(a) if a is not None else None) for a in arrs])
+
     def get_state(self):
         """(values, tangents): two dicts of the state fields."""
         shp = {k: ((self.nx + 1, self.ny) if k in PLUS_FIELDS else (self.nx, self.ny)) for k in STATE_FIELDS}
