@@ -70,7 +70,9 @@ def parse():
     ap.add_argument("--solver", choices=["fused", "partition", "exact"], default="fused")
     ap.add_argument("--tile-cols", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
-    ap.add_argument("--workload", choices=["tide2048x4096", "batch64"], default="tide2048x4096")
+    ap.add_argument("--workload", choices=["tide2048x4096", "batch64", "adjoint512x1024"], default="tide2048x4096")
+    ap.add_argument("--slots", type=int, default=48, help="adjoint checkpoint buffers")
+    ap.add_argument("--adj-steps", type=int, default=400)
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 5 times (reference kernel "
@@ -189,10 +191,75 @@ def bench_batch(args):
     print(json.dumps(rec), flush=True)
 
 
+def bench_adjoint(args):
+    """Gradient of L = 1/2 sum (Zs_N - obs)^2 on the 512 x 1024 inversion beach (400
+    steps): time of the checkpointed forward + reverse sweep against a plain Run(400)."""
+    import torch
+    from morfo70_amd import _lib
+    from morfo70_amd import scenarios as S
+    from morfo70_amd.adjoint import HydroModel
+    from morfo70_amd.hydro import from_scenario
+
+    sc, dh, rng = S.config4(steps=args.adj_steps)
+    n = sc.steps
+    torch.cuda.init()
+    hy = from_scenario(sc, solver=_lib.SOLVER_FUSED)
+    hy.set_forcing()
+    model = HydroModel(hy, sc.Z, sc.zs0, n, slots=args.slots)
+    dev = "cuda"
+    cD0 = S.G / 65.0 ** 2
+    h = torch.as_tensor(np.ascontiguousarray(sc.h), device=dev)
+    obs = model.run_plain(torch.as_tensor(sc.h + dh, device=dev), cD0)
+    obs = obs + torch.as_tensor(rng.normal(0.0, 0.02, size=(sc.nx, sc.ny)), device=dev)
+
+    def t_forward():
+        model.set_drag(cD0)
+        model.load_initial(h)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hy._chk(hy._L.m70h_run(hy.handle, n))
+        return time.perf_counter() - t0
+
+    def t_adjoint():
+        model.set_drag(cD0)
+        model.load_initial(h)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        model.adj.forward(n, args.slots)
+        eta = model.final_zs()
+        grads, dp = model.adj.reverse({"Zs": eta - obs}, want=("Zs", "D"))
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, grads, dp
+
+    t_forward()
+    t_adjoint()
+    tf = min(t_forward() for _ in range(3))
+    ta, grads, dp = min((t_adjoint() for _ in range(2)), key=lambda r: r[0])
+    st = model.adj.stats()
+    finite = bool(torch.isfinite(grads["D"]).all()) and bool(torch.isfinite(grads["Zs"]).all())
+    rec = {
+        "metric": "adjoint (gradient of a final-state loss) time / forward Run time", "value": ta / tf,
+        "unit": "ratio", "n_gpus": 1, "steps": n, "warmup": 1, "ms_per_step": ta / n * 1e3,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded beach, perturbed-bar observations + N(0, 0.02 m) noise)",
+        "config": {"workload": f"adjoint{sc.nx}x{sc.ny}", "steps": n, "checkpoint_buffers": st["buffers"],
+                   "recomputed_steps": st["forward_steps"], "taped_steps": st["taped_steps"]},
+        "forward_s": tf, "adjoint_s": ta, "forward_cell_steps_per_s": sc.nx * sc.ny * n / tf,
+        "dL_dcD0": dp["cD0"], "finite": finite,
+    }
+    hy.close()
+    print(json.dumps(rec), flush=True)
+
+
 def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.workload == "adjoint512x1024":
+        if args.gpus != 1:
+            sys.exit(2)
+        bench_adjoint(args)
         return
     if args.workload == "batch64":
         if args.gpus != 1:

@@ -119,6 +119,13 @@ class HydroModel:
         torch.cuda.synchronize()
         self.hy._chk(self.hy._L.m70h_set_state_dev(self.hy.handle, *[ctypes.c_void_p(t.data_ptr()) for t in keep]))
 
+    def run_plain(self, h, cD0: float):
+        """Forward only (no checkpoints): Zs after nIter steps from the state of h."""
+        self.set_drag(cD0)
+        self.load_initial(h)
+        self.hy._chk(self.hy._L.m70h_run(self.hy.handle, self.nIter))
+        return self.final_zs()
+
     def final_zs(self):
         import torch
         out = torch.empty((self.hy.nx, self.hy.ny), dtype=torch.float64, device="cuda")

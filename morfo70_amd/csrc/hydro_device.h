@@ -85,7 +85,7 @@ struct Work {
 struct ColsWork {
     double *E0, *E1, *E2, *E3;
     double *R0, *R1, *R2, *R3;
-    double *X;
+    double *X, *V;   // reduced solution and the reduced solve's scratch
     int nseg;
 };
 
@@ -107,7 +107,8 @@ struct ScenDesc {
 };
 
 // Stages of a grouped step; start[stage][s] is the first block of scenario s.
-enum GroupStage { GS_CELLS = 0, GS_ROWS = 1, GS_COLS = 2, GS_RED = 3, GS_COUNT = 4 };
+// GS_RED: reduced systems solved one lane per column; GS_REDP: by the partition solver.
+enum GroupStage { GS_CELLS = 0, GS_ROWS = 1, GS_COLS = 2, GS_RED = 3, GS_REDP = 4, GS_COUNT = 5 };
 
 // Scenario owning block b: start[] has nscen+1 ascending entries, start[0] = 0.
 __device__ __forceinline__ int scen_of(const int *__restrict__ start, int nscen, int b)

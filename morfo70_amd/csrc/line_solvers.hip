@@ -92,15 +92,12 @@ __device__ __forceinline__ void tb_step(const TBack &s, double &yn, double *__re
     }
 }
 
-// Solves m systems of n unknowns; system j has diagonal A[p*ld+j], lower B, upper C and
+// Solves system j of n unknowns with diagonal A[p*ld+j], lower B, upper C and
 // right-hand side F, p = 0..n-1. Solution to Y[p*ld+j]; V is n*ld scratch.
-__global__ void __launch_bounds__(64) k_thomas_cols(const double *__restrict__ A, const double *__restrict__ B,
-                                                    const double *__restrict__ C, const double *__restrict__ F,
-                                                    double *__restrict__ Y, double *__restrict__ V, int n, int m,
-                                                    long ld)
+__device__ __forceinline__ void thomas_col(const double *__restrict__ A, const double *__restrict__ B,
+                                           const double *__restrict__ C, const double *__restrict__ F,
+                                           double *__restrict__ Y, double *__restrict__ V, int n, long ld, int j)
 {
-    const int j = blockIdx.x * 64 + threadIdx.x;
-    if (j >= m) return;
     // forward elimination, src/Math.cpp:20-48
     double inv_w = 1 / A[j];
     double y = F[j] * inv_w;
@@ -133,6 +130,37 @@ __global__ void __launch_bounds__(64) k_thomas_cols(const double *__restrict__ A
         tb_step(b1, yn, Y, q0, ld, j);
         q0 -= TCH;
     }
+}
+
+__global__ void __launch_bounds__(64) k_thomas_cols(const double *__restrict__ A, const double *__restrict__ B,
+                                                    const double *__restrict__ C, const double *__restrict__ F,
+                                                    double *__restrict__ Y, double *__restrict__ V, int n, int m,
+                                                    long ld)
+{
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    if (j >= m) return;
+    thomas_col(A, B, C, F, Y, V, n, ld, j);
+}
+
+// Reduced column systems of the FUSED x-sweeps for a batch: one lane per column of each
+// scenario (blocks of 64 columns, scenario s owns blocks start[s]..start[s+1]-1).
+__global__ void __launch_bounds__(64) k_red_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
+                                                    int nscen)
+{
+    const int b = blockIdx.x;
+    const int sid = scen_of(start, nscen, b);
+    const ScenDesc *d = D + sid;
+    const int ny = d->k.ny;
+    const int j = (b - start[sid]) * 64 + threadIdx.x;
+    if (j >= ny) return;
+    const ColsWork &cw = d->cw;
+    thomas_col(cw.R1, cw.R0, cw.R2, cw.R3, cw.X, cw.V, 2 * cw.nseg, ny, j);
+}
+
+hipError_t launch_red_grouped(hipStream_t st, const ScenDesc *D, const int *start_red, int nscen, int blocks_red)
+{
+    hipLaunchKernelGGL(k_red_grouped, dim3(blocks_red), dim3(64), 0, st, D, start_red, nscen);
+    return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------------------
@@ -334,9 +362,11 @@ __global__ void __launch_bounds__(64) k_periodic_rows(const double *__restrict__
 hipError_t launch_thomas_cols(hipStream_t st, const double *A, const double *B, const double *C, const double *F,
                               double *Y, double *V, int n, int m, long ld)
 {
-    hipLaunchKernelGGL(k_thomas_cols, dim3((unsigned)((m + 63) / 64)), dim3(64), 0, st, A, B, C, F, Y, V, n, m, ld);
+    hipLaunchKernelGGL(k_thomas_cols, dim3((unsigned)((m + 63) / 64)), dim3(64), 0, st, A, B, C, F, Y, V, n, m,
+                       ld);
     return hipGetLastError();
 }
+
 
 int periodic_rows_padded_lines(int nL) { return ((nL + LW - 1) / LW) * LW; }
 

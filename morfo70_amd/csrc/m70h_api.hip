@@ -447,7 +447,7 @@ int m70h_create(int nx, int ny, const double *x, double dy, const m70h_params *p
         cw.nseg = cols_segments(nx - 1);
         cw.E0 = h->w.a; cw.E1 = h->w.b; cw.E2 = h->w.c; cw.E3 = h->w.f;
         const long nr = 2L * cw.nseg * ny;
-        double **rp[] = {&cw.R0, &cw.R1, &cw.R2, &cw.R3, &cw.X};
+        double **rp[] = {&cw.R0, &cw.R1, &cw.R2, &cw.R3, &cw.X, &cw.V};
         for (auto q : rp)
             if ((e = dalloc(h, q, nr)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc column work"));
     }
@@ -781,8 +781,8 @@ static int bfail(m70h_batch *b, int code, const std::string &msg)
 
 static void fill_desc(const m70h *h, ScenDesc &d, bool odd, int rb)
 {
-    int rows, cols, red, S_rows, S_red, L_red;
-    grouped_stage_blocks(h->nx, h->ny, rb, &rows, &cols, &red, &S_rows, &S_red, &L_red);
+    int rows, cols, red, redp, S_rows, S_red, L_red;
+    grouped_stage_blocks(h->nx, h->ny, rb, &rows, &cols, &red, &redp, &S_rows, &S_red, &L_red);
     d.k = h->k; d.m = h->m; d.s = h->s; d.rf = h->rf; d.fw = h->fw; d.w = h->w; d.cw = h->cw;
     if (odd) std::swap(d.s.D, d.s.Dn);
     d.f0 = h->f0_series.empty() ? h->f0 : h->f0_series.back();
@@ -808,14 +808,17 @@ static hipError_t batch_step(m70h_batch *b, int odd)
     const ScenDesc *D = b->d_desc + (odd ? n : 0);
     const int *st_cells = b->d_start + GS_CELLS * (n + 1), *st_rows = b->d_start + GS_ROWS * (n + 1);
     const int *st_cols = b->d_start + GS_COLS * (n + 1), *st_red = b->d_start + GS_RED * (n + 1);
+    const int *st_redp = b->d_start + GS_REDP * (n + 1);
     hipStream_t st = b->stream;
     hipError_t e;
-    if ((e = launch_cols3_grouped(st, 0, D, st_cols, st_red, n, b->blocks[GS_COLS], b->blocks[GS_RED])) != hipSuccess)
+    if ((e = launch_cols3_grouped(st, 0, D, st_cols, st_red, st_redp, n, b->blocks[GS_COLS], b->blocks[GS_RED],
+                                  b->blocks[GS_REDP])) != hipSuccess)
         return e;
     if ((e = launch_rows_grouped(st, 0, b->rows_block, D, st_rows, n, b->blocks[GS_ROWS])) != hipSuccess) return e;
     if ((e = launch_cells_grouped(st, 2, D, st_cells, n, b->blocks[GS_CELLS])) != hipSuccess) return e;
     if ((e = launch_rows_grouped(st, 1, b->rows_block, D, st_rows, n, b->blocks[GS_ROWS])) != hipSuccess) return e;
-    if ((e = launch_cols3_grouped(st, 1, D, st_cols, st_red, n, b->blocks[GS_COLS], b->blocks[GS_RED])) != hipSuccess)
+    if ((e = launch_cols3_grouped(st, 1, D, st_cols, st_red, st_redp, n, b->blocks[GS_COLS], b->blocks[GS_RED],
+                                  b->blocks[GS_REDP])) != hipSuccess)
         return e;
     return launch_cells_grouped(st, 3, D, st_cells, n, b->blocks[GS_CELLS]);
 }
@@ -844,10 +847,10 @@ int m70h_batch_create(int nscen, m70h *const *handles, m70h_batch **out)
         if (h->device != dev0) return bail(bfail(b, M70H_EINVAL, "handles on different devices"));
         if (!rows_lds_supported(h->ny) || !partition_supported(h->nx - 1))
             return bail(bfail(b, M70H_EINVAL, "batched step supports ny and nx-1 up to 4096"));
-        int rows, cols, red, S_rows, S_red, L_red;
-        grouped_stage_blocks(h->nx, h->ny, b->rows_block, &rows, &cols, &red, &S_rows, &S_red, &L_red);
+        int rows, cols, red, redp, S_rows, S_red, L_red;
+        grouped_stage_blocks(h->nx, h->ny, b->rows_block, &rows, &cols, &red, &redp, &S_rows, &S_red, &L_red);
         if (S_rows <= 0 || S_red <= 0) return bail(bfail(b, M70H_EINVAL, "grid too small for the batched step"));
-        const int per[GS_COUNT] = {grouped_cell_blocks(h->nx, h->ny), rows, cols, red};
+        const int per[GS_COUNT] = {grouped_cell_blocks(h->nx, h->ny), rows, cols, red, redp};
         for (int gs = 0; gs < GS_COUNT; ++gs)
             starts[(size_t)gs * (nscen + 1) + q + 1] = starts[(size_t)gs * (nscen + 1) + q] + per[gs];
         b->hs.push_back(h);
@@ -882,8 +885,8 @@ int m70h_batch_set_rows_block(m70h_batch *b, int threads)
     if (threads != 256 && threads != 512) return bfail(b, M70H_EINVAL, "rows block must be 256 or 512");
     if (threads == 256 && rows_block_for(maxny, 256) != 256) return bfail(b, M70H_EINVAL, "ny too large for 256");
     for (m70h *h : b->hs) {   // segment counts depend on the block size
-        int rows, cols, red, S_rows, S_red, L_red;
-        grouped_stage_blocks(h->nx, h->ny, threads, &rows, &cols, &red, &S_rows, &S_red, &L_red);
+        int rows, cols, red, redp, S_rows, S_red, L_red;
+        grouped_stage_blocks(h->nx, h->ny, threads, &rows, &cols, &red, &redp, &S_rows, &S_red, &L_red);
         if (S_rows <= 0) return bfail(b, M70H_EINVAL, "grid too small for the row block");
     }
     b->rows_block = threads;

@@ -508,6 +508,12 @@ __global__ void __launch_bounds__(256) k_cols_backsub(ColsWork cw, double *__res
     cols_backsub_body(cw, Y, n, ny, (int)blockIdx.x, (int)blockIdx.y);
 }
 
+// Reduced column systems of up to 256 rows (nx up to 1025) are solved one lane per
+// column (Thomas): the chain is short, and a workgroup-parallel solve of so few rows per
+// line launches too few workgroups to fill the chip. Longer ones use the partition
+// solver, whose cyclic reduction then wins over the lane's serial chain.
+bool red_thomas(int nseg) { return 2 * nseg <= 256; }
+
 // One pass of the three-pass column solve: COLS_ELIM (assemble + eliminate the segments),
 // COLS_REDUCED (the reduced systems, 2*nseg rows per column) or COLS_BACKSUB. The passes
 // run in this order on one stream; each is its own launch so it can be timed alone.
@@ -522,7 +528,8 @@ static hipError_t launch_cols3(hipStream_t st, const Prov &prov, const ColsWork 
         hipLaunchKernelGGL((k_cols_elim<Prov>), grid, block, 0, st, prov, cw, n, ny);
         return hipGetLastError();
     case COLS_REDUCED:
-        // diagonal R1, lower R0, upper R2, rhs R3
+        // diagonal R1, lower R0, upper R2, rhs R3; 2*nseg rows per column
+        if (red_thomas(cw.nseg)) return launch_thomas_cols(st, cw.R1, cw.R0, cw.R2, cw.R3, cw.X, cw.V, 2 * cw.nseg, ny, ny);
         return dispatch_cols(st, ProvArrays<true>{cw.R1, cw.R0, cw.R2, cw.R3, (long)ny}, cw.X, ny, 2 * cw.nseg, ny, tile);
     case COLS_BACKSUB:
         hipLaunchKernelGGL(k_cols_backsub, grid, block, 0, st, cw, Y, n, ny);
@@ -623,8 +630,10 @@ __global__ void __launch_bounds__(256) k_cols_backsub_grouped(const ScenDesc *__
 }
 
 // Reduced column systems of every scenario (partition solver, 8 unknowns per thread).
-__global__ void __launch_bounds__(1024) k_red_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
-                                                      int nscen)
+// Reduced systems solved by the partition solver, for the scenarios of a batch whose
+// systems are longer than red_thomas allows.
+__global__ void __launch_bounds__(1024) k_red_pcr_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
+                                                          int nscen)
 {
     __shared__ double red[4][1024 * 2];
     const int b = blockIdx.x;
@@ -639,7 +648,8 @@ __global__ void __launch_bounds__(1024) k_red_grouped(const ScenDesc *__restrict
 }
 
 // Block counts of the grouped stages for one scenario.
-void grouped_stage_blocks(int nx, int ny, int rb, int *rows, int *cols, int *red, int *S_rows, int *S_red, int *L_red)
+void grouped_stage_blocks(int nx, int ny, int rb, int *rows, int *cols, int *red, int *redp, int *S_rows, int *S_red,
+                          int *L_red)
 {
     *rows = nx - 1;
     *S_rows = pick_segments(ny, RM, rb);
@@ -647,7 +657,13 @@ void grouped_stage_blocks(int nx, int ny, int rb, int *rows, int *cols, int *red
     *cols = ((ny + 63) / 64) * ((nseg + 3) / 4);
     *S_red = pick_segments(2 * nseg, 8, 256);
     *L_red = *S_red > 0 ? 1024 / *S_red : 0;
-    *red = *L_red > 0 ? (ny + *L_red - 1) / *L_red : 0;
+    if (red_thomas(nseg)) {     // one lane per column, blocks of 64 (k_red_grouped)
+        *red = (ny + 63) / 64;
+        *redp = 0;
+    } else {                    // partition solver, blocks of 1024 (k_red_pcr_grouped)
+        *red = 0;
+        *redp = *L_red > 0 ? (ny + *L_red - 1) / *L_red : 0;
+    }
 }
 
 hipError_t launch_rows_grouped(hipStream_t st, int which, int rb, const ScenDesc *D, const int *start, int nscen,
@@ -664,15 +680,18 @@ hipError_t launch_rows_grouped(hipStream_t st, int which, int rb, const ScenDesc
 }
 
 hipError_t launch_cols3_grouped(hipStream_t st, int which, const ScenDesc *D, const int *start_cols,
-                                const int *start_red, int nscen, int blocks_cols, int blocks_red)
+                                const int *start_red, const int *start_redp, int nscen, int blocks_cols, int blocks_red,
+                                int blocks_redp)
 {
     if (which == 0) hipLaunchKernelGGL(k_cols_elim_grouped<0>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
     else hipLaunchKernelGGL(k_cols_elim_grouped<1>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_red_grouped, dim3(blocks_red), dim3(1024), 0, st, D, start_red, nscen);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (blocks_red > 0 && (e = launch_red_grouped(st, D, start_red, nscen, blocks_red)) != hipSuccess) return e;
+    if (blocks_redp > 0) {
+        hipLaunchKernelGGL(k_red_pcr_grouped, dim3(blocks_redp), dim3(1024), 0, st, D, start_redp, nscen);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     if (which == 0) hipLaunchKernelGGL(k_cols_backsub_grouped<0>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
     else hipLaunchKernelGGL(k_cols_backsub_grouped<1>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
     return hipGetLastError();

@@ -235,6 +235,29 @@ def config2(steps: int = 2000, seed: int = 2, nx: int = 2048, ny: int = 4096) ->
                         name=f"beach_tide_{nx}x{ny}_s{seed}")
 
 
+def smooth_field(nx: int, ny: int, rng, amp: float, corr_cells: float = 12.0) -> np.ndarray:
+    """Smooth random field, periodic along y: white noise filtered by a Gaussian of
+    ``corr_cells`` cells (x by convolution with held edges, y spectrally), scaled so its
+    largest magnitude is ``amp``."""
+    w = rng.normal(size=(nx, ny))
+    w = _smooth_x(w, corr_cells)
+    ky = np.fft.rfftfreq(ny)
+    w = np.fft.irfft(np.fft.rfft(w, axis=1) * np.exp(-0.5 * (2 * np.pi * ky * corr_cells) ** 2)[None, :],
+                     n=ny, axis=1)
+    return np.ascontiguousarray(amp * w / np.max(np.abs(w)))
+
+
+def config4(seed: int = 4, steps: int = 400, nx: int = 512, ny: int = 1024, dx: float = 2.0,
+            noise: float = 0.02):
+    """Bathymetry inversion set-up: a seeded beach (Hrms 1 m, T 8 s, 10 deg, dt 0.5 s),
+    the true bar field h_true = h + dh with dh a smooth random field of 0.2 m amplitude,
+    and the observation noise level (0.02 m). Returns (scenario, dh, noise_rng)."""
+    sc = seeded_beach(nx, ny, dx, dx, seed, steps, name=f"inversion_{nx}x{ny}_s{seed}")
+    rng = np.random.default_rng(seed + 1000)
+    dh = smooth_field(nx, ny, rng, 0.2)
+    return sc, dh, rng
+
+
 def batch_scenarios(count: int = 64, seed: int = 3, steps: int = 500,
                     nx_range=(128, 1024), ny_range=(256, 2048), dx: float = 2.0):
     """Uneven scenarios for the batched form: nx ~ U[nx_range], ny ~ U[ny_range]."""

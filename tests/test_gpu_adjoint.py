@@ -128,3 +128,95 @@ def test_checkpoint_schedules_agree():
             scale = max(np.max(np.abs(ref[0][k])), 1e-300)
             assert np.max(np.abs(g[k] - ref[0][k])) <= 1e-12 * scale, (slots, k)
         assert _rel(dp["cD0"], ref[1]["cD0"]) < 1e-12
+
+
+def _wet_interior_cells(D0, count, rng, depth=1.0, margin=3):
+    """Cells at least `depth` deep with no shallower cell within `margin` cells (wet and
+    away from the swash zone, so a small perturbation moves no wet/dry mask)."""
+    deep = D0 > depth
+    ok = deep.copy()
+    for di in range(-margin, margin + 1):
+        for dj in range(-margin, margin + 1):
+            ok &= np.roll(np.roll(deep, di, axis=0), dj, axis=1)
+    ok[:margin + 1, :] = False
+    ok[-margin - 1:, :] = False
+    idx = np.argwhere(ok)
+    return idx[rng.choice(len(idx), size=count, replace=False)]
+
+
+def test_bathymetry_inversion_gradient_c4():
+    """512 x 1024 beach, 400 steps, L = 1/2 sum (Zs_N - Zs_obs)^2 with observations from
+    a perturbed bar field plus N(0, 0.02 m) noise. The gradient with respect to h (all
+    cells) and the Chezy coefficient C, through the torch.autograd wrapper: against
+    central differences of the GPU forward on 32 wet non-swash cells (<= 1e-6), and
+    against the tangent-linear oracle along a random smooth direction of h and along
+    cD0 = g/C^2."""
+    from morfo70_amd.adjoint import HydroModel, hydro_run
+
+    sc, dh, rng = S.config4()
+    n = sc.steps
+    hy = from_scenario(sc, solver=_lib.SOLVER_FUSED)
+    hy.set_forcing()
+    model = HydroModel(hy, sc.Z, sc.zs0, n, slots=48)
+    dev = "cuda"
+    C0 = 65.0
+    cD0 = S.G / C0 ** 2
+    h_true = torch.as_tensor(sc.h + dh, device=dev)
+    obs = model.run_plain(h_true, cD0) + torch.as_tensor(rng.normal(0.0, 0.02, size=(sc.nx, sc.ny)), device=dev)
+    h0 = torch.as_tensor(np.ascontiguousarray(sc.h), device=dev).requires_grad_(True)
+    C = torch.tensor(C0, dtype=torch.float64, device=dev, requires_grad=True)
+    eta = hydro_run(h0, S.G / C ** 2, model)
+    loss = 0.5 * torch.sum((eta - obs) ** 2)
+    loss.backward()
+    gh = h0.grad.detach().cpu().numpy()
+    gC = float(C.grad)
+    assert np.isfinite(gh).all() and np.isfinite(gC)
+
+    def dloss(hp, hm, cp, cm):   # L(+) - L(-) without cancellation
+        ep, em = model.run_plain(hp, cp), model.run_plain(hm, cm)
+        return float(torch.sum(0.5 * (ep - em) * ((ep - obs) + (em - obs))))
+
+    # Fourth-order central differences (Richardson: (4 D(eps) - D(2 eps)) / 3). The step
+    # grows with depth: L changes by ~1e-16 of its size between two runs from rounding
+    # alone, so deep cells, whose gradients are 1e-7..1e-9, need steps of centimetres to
+    # rise above that floor, while in 1-2 m of water the response turns nonlinear beyond
+    # a few tenths of a millimetre.
+    hb = torch.as_tensor(np.ascontiguousarray(sc.h), device=dev)
+    D0 = sc.initial_depth()
+    cells = _wet_interior_cells(D0, 32, np.random.default_rng(44))
+
+    def fd4(step, pert):
+        d1 = dloss(*pert(step)) / (2 * step)
+        d2 = dloss(*pert(2 * step)) / (4 * step)
+        return (4 * d1 - d2) / 3
+
+    for i, j in cells:
+        eps = min(1e-4 * max(1.0, D0[i, j] ** 2), 0.2)
+
+        def pert(st, i=i, j=j):
+            e = torch.zeros_like(hb)
+            e[i, j] = st
+            return hb + e, hb - e, cD0, cD0
+
+        fd = fd4(eps, pert)
+        rel = abs(fd - gh[i, j]) / max(abs(gh[i, j]), 1e-300)
+        assert rel <= 1e-6, (int(i), int(j), float(D0[i, j]), fd, gh[i, j])
+    # Tangent-linear oracle: along a smooth random direction of h, and along cD0. (A
+    # global friction change moves some wet/dry mask in the swash zone over 400 steps, so
+    # L is only piecewise smooth in C and its differences are not a derivative; the TLM
+    # differentiates the same branch as the adjoint.)
+    obs_h = obs.cpu().numpy()
+    v = S.smooth_field(sc.nx, sc.ny, np.random.default_rng(45), 0.01)
+    tl = oracle.TangentHydro(sc, dh=v)
+    tl.run(n)
+    vals, tans = tl.get_state()
+    lhs = float(np.sum((vals["Zs"] - obs_h) * tans["Zs"]))
+    rhs = float(np.sum(gh * v))
+    assert abs(lhs - rhs) <= 1e-8 * max(abs(lhs), abs(rhs)), (lhs, rhs)
+    tc = oracle.TangentHydro(sc, dparams={"cD0": 1.0})
+    tc.run(n)
+    vals, tans = tc.get_state()
+    lhs = float(np.sum((vals["Zs"] - obs_h) * tans["Zs"]))
+    gcD0 = gC / (-2 * S.G / C0 ** 3)   # dcD0/dC = -2 g / C^3
+    assert abs(lhs - gcD0) <= 1e-8 * max(abs(lhs), abs(gcD0)), (lhs, gcD0)
+    hy.close()
