@@ -71,7 +71,8 @@ def parse():
     ap.add_argument("--tile-cols", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--workload", choices=["tide2048x4096", "batch64", "adjoint512x1024"], default="tide2048x4096")
-    ap.add_argument("--slots", type=int, default=48, help="adjoint checkpoint buffers")
+    ap.add_argument("--slots", type=int, default=0,
+                    help="adjoint checkpoint buffers; 0 = tape every step in the forward sweep")
     ap.add_argument("--adj-steps", type=int, default=400)
     ap.add_argument("--batch-count", type=int, default=64)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -242,8 +243,10 @@ def bench_adjoint(args):
         "unit": "ratio", "n_gpus": 1, "steps": n, "warmup": 1, "ms_per_step": ta / n * 1e3,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded beach, perturbed-bar observations + N(0, 0.02 m) noise)",
-        "config": {"workload": f"adjoint{sc.nx}x{sc.ny}", "steps": n, "checkpoint_buffers": st["buffers"],
-                   "recomputed_steps": st["forward_steps"], "taped_steps": st["taped_steps"]},
+        "config": {"workload": f"adjoint{sc.nx}x{sc.ny}", "steps": n,
+                   "schedule": "tape every step" if args.slots == 0 else f"binomial, {args.slots} buffers",
+                   "state_buffers": st["buffers"], "recomputed_steps": st["forward_steps"],
+                   "retaped_steps": st["taped_steps"]},
         "forward_s": tf, "adjoint_s": ta, "forward_cell_steps_per_s": sc.nx * sc.ny * n / tf,
         "dL_dcD0": dp["cD0"], "finite": finite,
     }

@@ -216,7 +216,9 @@ int m70h_batch_synchronize(m70h_batch *b);
  *
  * m70h_adjoint_forward runs Run(nIter) like m70h_run (same pre-loop, same steps, same
  * final state) and keeps `slots` state checkpoints (>= 1; at most nIter are used), placed
- * by the optimal binomial (Revolve) schedule. m70h_adjoint_reverse then takes the
+ * by the optimal binomial (Revolve) schedule. slots = 0 instead tapes every step during
+ * the forward sweep (about 30*nx*ny doubles per step of device memory), so that the
+ * reverse sweep recomputes nothing. m70h_adjoint_reverse then takes the
  * adjoint of the final state (seed, device pointers, NULL fields = zero) and returns the
  * adjoint of the state before the Run (grad, device pointers, NULL = not returned) and,
  * in dparams[0..2] (host), the adjoints of the parameters cD0, nu0 and M. The depth

@@ -96,6 +96,46 @@ __global__ void __launch_bounds__(256) k_adj_drag_visc(Consts k, const double *_
     }
 }
 
+// Zeroes the ten state adjoints and the accumulators of the intermediates (one launch
+// instead of thirteen fills).
+__global__ void __launch_bounds__(256) k_adj_zero(AdjFields A, double *UN, double *UcN, double *gD, long F, long Fp)
+{
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    if (q < F) {
+        A.Zs[q] = 0; A.D[q] = 0; A.Uc[q] = 0; A.Vc[q] = 0; A.dQx[q] = 0; A.dQy[q] = 0;
+        UcN[q] = 0; gD[q] = 0;
+    }
+    if (q < Fp) {
+        A.Qx[q] = 0; A.Qy[q] = 0; A.U[q] = 0; A.V[q] = 0;
+        UN[q] = 0;
+    }
+}
+
+// Copies the ten state fields (one launch instead of ten copies).
+__global__ void __launch_bounds__(256) k_copy_state(State dst, State src, long F, long Fp)
+{
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    if (q < F) {
+        dst.Zs[q] = src.Zs[q]; dst.D[q] = src.D[q]; dst.Uc[q] = src.Uc[q]; dst.Vc[q] = src.Vc[q];
+        dst.dQx[q] = src.dQx[q]; dst.dQy[q] = src.dQy[q];
+    }
+    if (q < Fp) {
+        dst.Qx[q] = src.Qx[q]; dst.Qy[q] = src.Qy[q]; dst.U[q] = src.U[q]; dst.V[q] = src.V[q];
+    }
+}
+
+hipError_t launch_adj_zero(hipStream_t st, const AdjFields &A, const AdjWork &W, long F, long Fp)
+{
+    hipLaunchKernelGGL(k_adj_zero, dim3((unsigned)((Fp + 255) / 256)), dim3(256), 0, st, A, W.UN, W.UcN, W.gD, F, Fp);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_state(hipStream_t st, const State &dst, const State &src, long F, long Fp)
+{
+    hipLaunchKernelGGL(k_copy_state, dim3((unsigned)((Fp + 255) / 256)), dim3(256), 0, st, dst, src, F, Fp);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------------------------
 // Launchers
 // ------------------------------------------------------------------------------------

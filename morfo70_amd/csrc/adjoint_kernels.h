@@ -6,6 +6,11 @@
 
 namespace m70 {
 
+// Zeroes A and the accumulated intermediate adjoints of W (UN, UcN, gD).
+hipError_t launch_adj_zero(hipStream_t st, const AdjFields &A, const AdjWork &W, long F, long Fp);
+// Copies the ten fields of a state (F = nx*ny, Fp = (nx+1)*ny).
+hipError_t launch_copy_state(hipStream_t st, const State &dst, const State &src, long F, long Fp);
+
 // Reverse of solveQy (Y5..Y1): reads Ain (adjoint of the state after the step), adds the
 // y half-step's contributions to A (adjoint of the state before it), W and R.
 hipError_t launch_adj_y(hipStream_t st, const Consts &k, const Metrics &m, const State &S0, const State &S1,

@@ -85,7 +85,7 @@ struct Work {
 struct ColsWork {
     double *E0, *E1, *E2, *E3;
     double *R0, *R1, *R2, *R3;
-    double *X, *V;   // reduced solution and the reduced solve's scratch
+    double *X;
     int nseg;
 };
 
@@ -102,12 +102,12 @@ struct ScenDesc {
     double f0;
     int S_rows;   // segments per row line in the LDS row solver
     int S_red;    // segments per reduced column system
-    int L_red;    // reduced column systems per block
+    int L_red;    // reduced column systems per workgroup
     int pad;
 };
 
 // Stages of a grouped step; start[stage][s] is the first block of scenario s.
-// GS_RED: reduced systems solved one lane per column; GS_REDP: by the partition solver.
+// GS_RED / GS_REDP: reduced column systems in 256- / 1024-thread workgroups.
 enum GroupStage { GS_CELLS = 0, GS_ROWS = 1, GS_COLS = 2, GS_RED = 3, GS_REDP = 4, GS_COUNT = 5 };
 
 // Scenario owning block b: start[] has nscen+1 ascending entries, start[0] = 0.

@@ -23,7 +23,6 @@ hipError_t launch_qy_tail(hipStream_t st, const Consts &k, const State &s);
 hipError_t launch_nonfinite(hipStream_t st, const double *x, long n, int *flag);
 hipError_t launch_init_zs(hipStream_t st, double *Zs, const double *D, const double *Z, const double *hb, long n);
 
-hipError_t launch_red_grouped(hipStream_t st, const ScenDesc *D, const int *start_red, int nscen, int blocks_red);
 hipError_t launch_thomas_cols(hipStream_t st, const double *A, const double *B, const double *C, const double *F,
                               double *Y, double *V, int n, int m, long ld);
 int periodic_rows_padded_lines(int nL);
@@ -59,7 +58,6 @@ int grouped_cell_blocks(int nx, int ny);
 hipError_t launch_cells_grouped(hipStream_t st, int op, const ScenDesc *D, const int *start, int nscen, int blocks);
 void grouped_stage_blocks(int nx, int ny, int rb, int *rows, int *cols, int *red, int *redp, int *S_rows, int *S_red,
                           int *L_red);
-bool red_thomas(int nseg);
 hipError_t launch_rows_grouped(hipStream_t st, int which, int rb, const ScenDesc *D, const int *start, int nscen,
                                int blocks);
 hipError_t launch_cols3_grouped(hipStream_t st, int which, const ScenDesc *D, const int *start_cols,

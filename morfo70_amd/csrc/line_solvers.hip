@@ -142,26 +142,6 @@ __global__ void __launch_bounds__(64) k_thomas_cols(const double *__restrict__ A
     thomas_col(A, B, C, F, Y, V, n, ld, j);
 }
 
-// Reduced column systems of the FUSED x-sweeps for a batch: one lane per column of each
-// scenario (blocks of 64 columns, scenario s owns blocks start[s]..start[s+1]-1).
-__global__ void __launch_bounds__(64) k_red_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
-                                                    int nscen)
-{
-    const int b = blockIdx.x;
-    const int sid = scen_of(start, nscen, b);
-    const ScenDesc *d = D + sid;
-    const int ny = d->k.ny;
-    const int j = (b - start[sid]) * 64 + threadIdx.x;
-    if (j >= ny) return;
-    const ColsWork &cw = d->cw;
-    thomas_col(cw.R1, cw.R0, cw.R2, cw.R3, cw.X, cw.V, 2 * cw.nseg, ny, j);
-}
-
-hipError_t launch_red_grouped(hipStream_t st, const ScenDesc *D, const int *start_red, int nscen, int blocks_red)
-{
-    hipLaunchKernelGGL(k_red_grouped, dim3(blocks_red), dim3(64), 0, st, D, start_red, nscen);
-    return hipGetLastError();
-}
 
 // ------------------------------------------------------------------------------------
 // Periodic (El-Mikkawy) along rows

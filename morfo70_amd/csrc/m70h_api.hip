@@ -447,7 +447,7 @@ int m70h_create(int nx, int ny, const double *x, double dy, const m70h_params *p
         cw.nseg = cols_segments(nx - 1);
         cw.E0 = h->w.a; cw.E1 = h->w.b; cw.E2 = h->w.c; cw.E3 = h->w.f;
         const long nr = 2L * cw.nseg * ny;
-        double **rp[] = {&cw.R0, &cw.R1, &cw.R2, &cw.R3, &cw.X, &cw.V};
+        double **rp[] = {&cw.R0, &cw.R1, &cw.R2, &cw.R3, &cw.X};
         for (auto q : rp)
             if ((e = dalloc(h, q, nr)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc column work"));
     }
@@ -993,6 +993,9 @@ struct AdjCtx {
     double *params = nullptr;    // device: adjoints of cD0, nu0, M
     long fwd_steps = 0, taped_steps = 0;
     int cur = 0;                 // A[cur] holds the adjoint of the current state
+    bool tape_all = false;       // every step's state and intermediates kept by the forward sweep
+    int slots_arg = -1;          // the slots argument the buffers were sized for
+    std::vector<StepTape> tapes; // tape_all: one per step (slot[t] holds the state before step t)
 };
 
 namespace {
@@ -1035,15 +1038,7 @@ State buf_state(const m70h *h, double *b)
 
 hipError_t copy_state(m70h *h, const State &dst, const State &src)
 {
-    const long F = h->F, Fp = h->Fp;
-    double *d[10] = {dst.Zs, dst.D, dst.Qx, dst.Qy, dst.U, dst.V, dst.Uc, dst.Vc, dst.dQx, dst.dQy};
-    const double *s[10] = {src.Zs, src.D, src.Qx, src.Qy, src.U, src.V, src.Uc, src.Vc, src.dQx, src.dQy};
-    const long n[10] = {F, F, Fp, Fp, Fp, Fp, F, F, F, F};
-    for (int q = 0; q < 10; ++q) {
-        hipError_t e = hipMemcpyAsync(d[q], s[q], (size_t)n[q] * sizeof(double), hipMemcpyDeviceToDevice, h->stream);
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
+    return launch_copy_state(h->stream, dst, src, h->F, h->Fp);
 }
 
 hipError_t zero_fields(m70h *h, const AdjFields &A)
@@ -1067,15 +1062,42 @@ double f0_at(m70h *h, long pos)
     return h->f0;
 }
 
-// Allocates the context for a Run of N steps with `slots` state buffers.
-int adj_setup(m70h *h, int N, int slots)
+// Tape buffers of one step (the right-hand side scratch f is shared).
+hipError_t adj_alloc_tape(AdjCtx *a, StepTape &T, long F, double *shared_f)
 {
+    hipError_t e;
+    double **tp[] = {&T.a1, &T.b1, &T.c1, &T.qxs, &T.a2, &T.b2, &T.c2, &T.a3, &T.b3, &T.c3, &T.qys,
+                     &T.a4, &T.b4, &T.c4, &T.dragx, &T.dragy, &T.gD, &T.gZs};
+    for (auto q : tp)
+        if ((e = adj_alloc(a, q, F)) != hipSuccess) return e;
+    double *mk = nullptr;
+    if ((e = adj_alloc(a, &mk, F / 4 + 1)) != hipSuccess) return e;
+    T.wx = (uint8_t *)mk;
+    if ((e = adj_alloc(a, &mk, F / 4 + 1)) != hipSuccess) return e;
+    T.wy = (uint8_t *)mk;
+    T.f = shared_f;
+    return hipSuccess;
+}
+
+// Allocates the context for a Run of N steps with `slots` state buffers.
+int adj_setup(m70h *h, int N, int slots, bool tape_all)
+{
+    if (h->adj && h->adj->N == N && h->adj->tape_all == tape_all && h->adj->slots_arg == slots) {
+        // same Run length and schedule: keep the buffers (every one is rewritten before use)
+        AdjCtx *a = h->adj;
+        for (auto &q : a->slot_step) q = -1;
+        a->fwd_steps = a->taped_steps = 0;
+        return M70H_OK;
+    }
     adj_release(h);
     AdjCtx *a = new AdjCtx();
+    a->slots_arg = slots;
     h->adj = a;
     a->N = N;
+    a->tape_all = tape_all;
     a->S = slots - 1 < N - 1 ? slots - 1 : N - 1;
     if (a->S < 0) a->S = 0;
+    if (tape_all) a->S = N;   // states 0..N
     const long F = h->F, Fp = h->Fp, SB = 6 * F + 4 * Fp;
     hipError_t e;
     a->slot.assign(a->S + 1, nullptr);
@@ -1084,16 +1106,15 @@ int adj_setup(m70h *h, int N, int slots)
         if ((e = adj_alloc(a, &b, SB)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc checkpoints");
     if ((e = adj_alloc(a, &a->s0buf, SB)) != hipSuccess || (e = adj_alloc(a, &a->final_buf, SB)) != hipSuccess)
         return hip_fail(h, e, "adjoint: hipMalloc state");
-    StepTape &T = a->T;
-    double **tp[] = {&T.a1, &T.b1, &T.c1, &T.qxs, &T.a2, &T.b2, &T.c2, &T.a3, &T.b3, &T.c3, &T.qys,
-                     &T.a4, &T.b4, &T.c4, &T.f, &T.dragx, &T.dragy, &T.gD, &T.gZs};
-    for (auto q : tp)
-        if ((e = adj_alloc(a, q, F)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc tape");
-    double *mk = nullptr;
-    if ((e = adj_alloc(a, &mk, F / 4 + 1)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc tape");
-    T.wx = (uint8_t *)mk;
-    if ((e = adj_alloc(a, &mk, F / 4 + 1)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc tape");
-    T.wy = (uint8_t *)mk;
+    double *fbuf = nullptr;
+    if ((e = adj_alloc(a, &fbuf, F)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc tape");
+    if ((e = adj_alloc_tape(a, a->T, F, fbuf)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc tape");
+    if (tape_all) {
+        a->tapes.resize(N);
+        for (int t = 0; t < N; ++t)
+            if ((e = adj_alloc_tape(a, a->tapes[t], F, fbuf)) != hipSuccess)
+                return hip_fail(h, e, "adjoint: hipMalloc per-step tapes (use checkpoints instead)");
+    }
     for (int q = 0; q < 2; ++q) {
         AdjFields &A = a->A[q];
         double **fp[] = {&A.Zs, &A.D, &A.Uc, &A.Vc, &A.dQx, &A.dQy};
@@ -1115,6 +1136,7 @@ int adj_setup(m70h *h, int N, int slots)
     for (auto p : rp)
         if ((e = adj_alloc(a, p, F)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc run adjoints");
     if ((e = adj_alloc(a, &a->params, 4)) != hipSuccess) return hip_fail(h, e, "adjoint: hipMalloc params");
+    if (tape_all) return M70H_OK;
     // schedule: cost(l, s) = fewest untaped steps to reverse l steps from a stored start
     // with s free buffers; split(l, s) = the first advance that achieves it
     const int S = a->S;
@@ -1172,10 +1194,9 @@ int adj_advance(m70h *h, int from, int n)
 }
 
 // One step from h->s with its intermediates taped (assembly kernels + partition solves).
-int adj_taped_step(m70h *h, int t)
+int adj_taped_step(m70h *h, int t, const StepTape &T)
 {
     AdjCtx *a = h->adj;
-    const StepTape &T = a->T;
     const double f0 = f0_at(h, a->f0_pos0 + t);
     const int nx = h->nx, ny = h->ny;
     const Consts &k = h->k;
@@ -1216,28 +1237,30 @@ int adj_taped_step(m70h *h, int t)
     return M70H_OK;
 }
 
-// Reverses step t: h->s holds s_t, S0 is a buffer holding s_t that stays untouched.
-int adj_reverse_one(m70h *h, int t, const State &S0)
+// Reverse kernels of step t: S0/S1 = states before/after it, T = its tape.
+int adj_reverse_kernels(m70h *h, int t, const State &S0, const State &S1, const StepTape &T)
 {
     AdjCtx *a = h->adj;
-    int rc = adj_taped_step(h, t);
-    if (rc) return rc;
     const int out = 1 - a->cur;
     const AdjFields &Ain = a->A[a->cur], &Aout = a->A[out];
-    HIPCHK(h, zero_fields(h, Aout));
     const AdjWork &W = a->W;
-    HIPCHK(h, hipMemsetAsync(W.UN, 0, (size_t)h->Fp * sizeof(double), h->stream));
-    HIPCHK(h, hipMemsetAsync(W.UcN, 0, (size_t)h->F * sizeof(double), h->stream));
-    HIPCHK(h, hipMemsetAsync(W.gD, 0, (size_t)h->F * sizeof(double), h->stream));
+    HIPCHK(h, launch_adj_zero(h->stream, Aout, W, h->F, h->Fp));
     hipError_t e;
-    if ((e = launch_adj_y(h->stream, h->k, h->m, S0, h->s, h->rf, a->T, Ain, Aout, W, a->R, h->tile_cols)) != hipSuccess)
+    if ((e = launch_adj_y(h->stream, h->k, h->m, S0, S1, h->rf, T, Ain, Aout, W, a->R, h->tile_cols)) != hipSuccess)
         return hip_fail(h, e, "adjoint y half-step");
     const double f0 = f0_at(h, a->f0_pos0 + t);
-    if ((e = launch_adj_x(h->stream, h->k, h->m, S0, h->s, h->rf, a->T, Ain, Aout, W, a->R, f0, h->tile_cols)) !=
-        hipSuccess)
+    if ((e = launch_adj_x(h->stream, h->k, h->m, S0, S1, h->rf, T, Ain, Aout, W, a->R, f0, h->tile_cols)) != hipSuccess)
         return hip_fail(h, e, "adjoint x half-step");
     a->cur = out;
     return M70H_OK;
+}
+
+// Reverses step t: h->s holds s_t, S0 is a buffer holding s_t that stays untouched.
+int adj_reverse_one(m70h *h, int t, const State &S0)
+{
+    int rc = adj_taped_step(h, t, h->adj->T);
+    if (rc) return rc;
+    return adj_reverse_kernels(h, t, S0, h->s, h->adj->T);
 }
 
 // Reverses steps a0..b0-1; the state of step a0 is held by a checkpoint buffer and s free
@@ -1287,17 +1310,31 @@ int m70h_adjoint_forward(m70h *h, int nIter, int slots)
 {
     if (!h) return M70H_EINVAL;
     if (nIter < 1) return fail(h, M70H_EINVAL, "adjoint: nIter must be >= 1");
-    if (slots < 1) return fail(h, M70H_EINVAL, "adjoint: slots must be >= 1");
+    if (slots < 0) return fail(h, M70H_EINVAL, "adjoint: slots must be >= 1, or 0 to tape every step");
     if (!h->forcing_set) return fail(h, M70H_ESTATE, "forcing not set");
-    if ((double)nIter * nIter * (slots < nIter ? slots : nIter) > 4e9)
+    const bool tape_all = slots == 0;
+    if (!tape_all && (double)nIter * nIter * (slots < nIter ? slots : nIter) > 4e9)
         return fail(h, M70H_EINVAL, "adjoint: schedule too large (split the Run or use fewer slots)");
-    int rc = adj_setup(h, nIter, slots);
+    int rc = adj_setup(h, nIter, slots, tape_all);
     if (rc) { adj_release(h); return rc; }
     AdjCtx *a = h->adj;
     h->run_gen++;
     a->gen = h->run_gen;
     a->f0_pos0 = h->f0_pos;
     if ((rc = run_preloop(h))) return rc;
+    if (tape_all) {   // every step taped now; the reverse sweep recomputes nothing
+        for (int t = 0; t < nIter; ++t) {
+            HIPCHK(h, copy_state(h, buf_state(h, a->slot[t]), h->s));
+            a->slot_step[t] = t;
+            if ((rc = adj_taped_step(h, t, a->tapes[t]))) return rc;
+        }
+        HIPCHK(h, copy_state(h, buf_state(h, a->slot[nIter]), h->s));
+        a->slot_step[nIter] = nIter;
+        HIPCHK(h, copy_state(h, buf_state(h, a->final_buf), h->s));
+        h->f0_pos = a->f0_pos0 + nIter;
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        return M70H_OK;
+    }
     if ((rc = adj_store(h, 0))) return rc;
     // first sweep: store the chain of checkpoints the schedule starts with
     int pos = 0, l = nIter, s = a->S;
@@ -1342,9 +1379,14 @@ int m70h_adjoint_reverse(m70h *h, const m70h_fields *seed, const m70h_fields *gr
     HIPCHK(h, hipMemsetAsync(a->params, 0, 4 * sizeof(double), h->stream));
     a->fwd_steps = 0;
     a->taped_steps = 0;
-    // the first sweep's checkpoints are still held; reverse all steps
-    int rc = adj_reverse_range(h, 0, a->N, a->S);
-    if (rc) return rc;
+    int rc;
+    if (a->tape_all) {
+        for (int t = a->N - 1; t >= 0; --t)
+            if ((rc = adj_reverse_kernels(h, t, buf_state(h, a->slot[t]), buf_state(h, a->slot[t + 1]), a->tapes[t])))
+                return rc;
+    } else if ((rc = adj_reverse_range(h, 0, a->N, a->S))) {   // the first sweep's checkpoints are still held
+        return rc;
+    }
     // per-Run pre-loop: drag/viscosity fields were formed from the depth at step 0
     const int q0 = adj_find(h, 0);
     if (q0 < 0) return fail(h, M70H_ESTATE, "adjoint: missing start checkpoint");

@@ -355,12 +355,35 @@ static hipError_t launch_partition(hipStream_t st, const Prov &prov, double *Y, 
     return hipGetLastError();
 }
 
+// Workgroup size (1024, 256 or 64 threads) of the column partition solve of nL lines of n
+// unknowns with S segments each.
+static int col_workgroup(int n, int nL, int S)
+{
+    const int sizes[3] = {1024, 256, 64};
+    int best = 1024;
+    for (int B : sizes) {
+        if (pick_segments(n, 8, B / 4) != S || B / S < 1) continue;
+        best = B;
+        if ((nL + B / S - 1) / (B / S) >= 256) break;
+    }
+    return best;
+}
+
 // Column lines (x-sweeps): tile = unknowns per thread (8 or 16; 0 = default 8).
 template <class Prov>
 static hipError_t dispatch_cols(hipStream_t st, const Prov &prov, double *Y, long ldy, int n, int nL, int tile)
 {
     if (tile == 16 && pick_segments(n, 16, 64) > 0) return launch_partition<true, 256, 16>(st, prov, Y, ldy, n, nL, false);
-    if (pick_segments(n, 8, 256) > 0) return launch_partition<true, 1024, 8>(st, prov, Y, ldy, n, nL, false);
+    const int S = pick_segments(n, 8, 256);
+    if (S > 0) {
+        // A workgroup of B threads holds B/S lines. With few lines (columns) 1024-thread
+        // workgroups leave CUs idle, so take the largest B that still gives one
+        // workgroup per CU. The segment count, and so every result, is the same for all B.
+        const int wg = col_workgroup(n, nL, S);
+        if (wg == 64) return launch_partition<true, 64, 8>(st, prov, Y, ldy, n, nL, false);
+        if (wg == 256) return launch_partition<true, 256, 8>(st, prov, Y, ldy, n, nL, false);
+        return launch_partition<true, 1024, 8>(st, prov, Y, ldy, n, nL, false);
+    }
     return launch_partition<true, 256, 16>(st, prov, Y, ldy, n, nL, false);
 }
 
@@ -508,12 +531,6 @@ __global__ void __launch_bounds__(256) k_cols_backsub(ColsWork cw, double *__res
     cols_backsub_body(cw, Y, n, ny, (int)blockIdx.x, (int)blockIdx.y);
 }
 
-// Reduced column systems of up to 256 rows (nx up to 1025) are solved one lane per
-// column (Thomas): the chain is short, and a workgroup-parallel solve of so few rows per
-// line launches too few workgroups to fill the chip. Longer ones use the partition
-// solver, whose cyclic reduction then wins over the lane's serial chain.
-bool red_thomas(int nseg) { return 2 * nseg <= 256; }
-
 // One pass of the three-pass column solve: COLS_ELIM (assemble + eliminate the segments),
 // COLS_REDUCED (the reduced systems, 2*nseg rows per column) or COLS_BACKSUB. The passes
 // run in this order on one stream; each is its own launch so it can be timed alone.
@@ -529,7 +546,6 @@ static hipError_t launch_cols3(hipStream_t st, const Prov &prov, const ColsWork 
         return hipGetLastError();
     case COLS_REDUCED:
         // diagonal R1, lower R0, upper R2, rhs R3; 2*nseg rows per column
-        if (red_thomas(cw.nseg)) return launch_thomas_cols(st, cw.R1, cw.R0, cw.R2, cw.R3, cw.X, cw.V, 2 * cw.nseg, ny, ny);
         return dispatch_cols(st, ProvArrays<true>{cw.R1, cw.R0, cw.R2, cw.R3, (long)ny}, cw.X, ny, 2 * cw.nseg, ny, tile);
     case COLS_BACKSUB:
         hipLaunchKernelGGL(k_cols_backsub, grid, block, 0, st, cw, Y, n, ny);
@@ -630,12 +646,13 @@ __global__ void __launch_bounds__(256) k_cols_backsub_grouped(const ScenDesc *__
 }
 
 // Reduced column systems of every scenario (partition solver, 8 unknowns per thread).
-// Reduced systems solved by the partition solver, for the scenarios of a batch whose
-// systems are longer than red_thomas allows.
-__global__ void __launch_bounds__(1024) k_red_pcr_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
-                                                          int nscen)
+// Reduced systems of the FUSED x-sweeps for a batch: the partition solver with the
+// scenario's segment count, RED_BLOCK-thread workgroups of RED_BLOCK/S lines.
+template <int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_red_pcr_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
+                                                           int nscen)
 {
-    __shared__ double red[4][1024 * 2];
+    __shared__ double red[4][BLOCK * 2];
     const int b = blockIdx.x;
     const int sid = scen_of(start, nscen, b);
     const ScenDesc *d = D + sid;
@@ -643,8 +660,8 @@ __global__ void __launch_bounds__(1024) k_red_pcr_grouped(const ScenDesc *__rest
     LineGeom g{};
     g.ld = ny; g.ldy = ny; g.n = 2 * d->cw.nseg; g.nL = ny; g.S = d->S_red; g.lines_per_block = d->L_red;
     g.periodic = 0;
-    partition_body<true, 1024, 8>(ProvArrays<true>{d->cw.R1, d->cw.R0, d->cw.R2, d->cw.R3, (long)ny}, d->cw.X, g,
-                                  b - start[sid], red);
+    partition_body<true, BLOCK, 8>(ProvArrays<true>{d->cw.R1, d->cw.R0, d->cw.R2, d->cw.R3, (long)ny}, d->cw.X, g,
+                                   b - start[sid], red);
 }
 
 // Block counts of the grouped stages for one scenario.
@@ -656,11 +673,13 @@ void grouped_stage_blocks(int nx, int ny, int rb, int *rows, int *cols, int *red
     const int nseg = cols_segments(nx - 1);
     *cols = ((ny + 63) / 64) * ((nseg + 3) / 4);
     *S_red = pick_segments(2 * nseg, 8, 256);
-    *L_red = *S_red > 0 ? 1024 / *S_red : 0;
-    if (red_thomas(nseg)) {     // one lane per column, blocks of 64 (k_red_grouped)
-        *red = (ny + 63) / 64;
+    // 256-thread workgroups when the segment count allows it (2*nseg <= 512), else 1024
+    const int B = (*S_red > 0 && pick_segments(2 * nseg, 8, 64) == *S_red) ? 256 : 1024;
+    *L_red = *S_red > 0 ? B / *S_red : 0;
+    if (B == 256) {
+        *red = (ny + *L_red - 1) / *L_red;
         *redp = 0;
-    } else {                    // partition solver, blocks of 1024 (k_red_pcr_grouped)
+    } else {
         *red = 0;
         *redp = *L_red > 0 ? (ny + *L_red - 1) / *L_red : 0;
     }
@@ -687,9 +706,12 @@ hipError_t launch_cols3_grouped(hipStream_t st, int which, const ScenDesc *D, co
     else hipLaunchKernelGGL(k_cols_elim_grouped<1>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (blocks_red > 0 && (e = launch_red_grouped(st, D, start_red, nscen, blocks_red)) != hipSuccess) return e;
-    if (blocks_redp > 0) {
-        hipLaunchKernelGGL(k_red_pcr_grouped, dim3(blocks_redp), dim3(1024), 0, st, D, start_redp, nscen);
+    if (blocks_red > 0) {   // scenarios whose reduced solve fits 256-thread workgroups
+        hipLaunchKernelGGL(k_red_pcr_grouped<256>, dim3(blocks_red), dim3(256), 0, st, D, start_red, nscen);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (blocks_redp > 0) {  // the others, 1024-thread workgroups
+        hipLaunchKernelGGL(k_red_pcr_grouped<1024>, dim3(blocks_redp), dim3(1024), 0, st, D, start_redp, nscen);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (which == 0) hipLaunchKernelGGL(k_cols_backsub_grouped<0>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
