@@ -49,9 +49,10 @@ class RunAdjoint:
         self.hy._chk(self._L.m70h_adjoint_forward(self.hy.handle, self.nIter, int(slots)))
 
     def reverse(self, seed: dict, want=("Zs", "D")):
-        """seed: adjoints of the final state (device float64 tensors, by field name).
-        Returns (grads, dparams): adjoints of the initial state for the fields in
-        ``want`` (device tensors) and a dict of the parameter adjoints."""
+        """seed: adjoints of the final state (device float64 tensors, by field name),
+        ready on the handle's stream. Returns (grads, dparams): adjoints of the initial
+        state for the fields in ``want`` (device tensors) and a dict of the parameter
+        adjoints. Returns after the sweep has finished."""
         import torch
         shapes = {k: ((self.hy.nx + 1, self.hy.ny) if k in ("Qx", "Qy", "U", "V") else (self.hy.nx, self.hy.ny))
                   for k in _lib.STATE_FIELDS}
@@ -60,7 +61,7 @@ class RunAdjoint:
         seedc = {k: v.contiguous() for k, v in seed.items()}
         sf, gf = _fields(seedc), _fields(grads)
         dp = (ctypes.c_double * 3)()
-        torch.cuda.synchronize()
+        torch.cuda.current_stream().synchronize()   # seeds and zeroed outputs are ready
         self.hy._chk(self._L.m70h_adjoint_reverse(self.hy.handle, ctypes.byref(sf), ctypes.byref(gf),
                                                   ctypes.cast(dp, ctypes.POINTER(ctypes.c_double))))
         return grads, {"cD0": dp[0], "nu0": dp[1], "M": dp[2]}
@@ -92,13 +93,21 @@ def h_adjoint(Z, h, zs0: float, Dmin: float, gD, gZs):
 
 class HydroModel:
     """A Hydro handle plus what the autograd wrapper needs: the fixed Z, zs0, the run
-    length, the checkpoint budget, and the state to restart from."""
+    length, the checkpoint budget (0 = tape every step), and the state to restart from.
 
-    def __init__(self, hydro: Hydro, Z, zs0: float, nIter: int, slots: int):
+    The handle and the model's torch work share one stream (``self.stream``), so calls
+    are ordered without device-wide synchronisation and several models can run at once
+    from different threads."""
+
+    def __init__(self, hydro: Hydro, Z, zs0: float, nIter: int, slots: int, stream=None):
         import torch
         self.hy = hydro
         self.adj = RunAdjoint(hydro)
-        self.Z = torch.as_tensor(np.ascontiguousarray(Z), dtype=torch.float64, device="cuda")
+        self.stream = stream if stream is not None else torch.cuda.Stream()
+        hydro.set_stream(self.stream.cuda_stream)
+        with torch.cuda.stream(self.stream):
+            self.Z = torch.as_tensor(np.ascontiguousarray(Z), dtype=torch.float64, device="cuda")
+        self.stream.synchronize()
         self.zs0 = float(zs0)
         self.nIter = int(nIter)
         self.slots = int(slots)
@@ -109,14 +118,16 @@ class HydroModel:
         self.hy._chk(self.hy._L.m70h_set_params(self.hy.handle, ctypes.cast(ctypes.pointer(p), ctypes.c_void_p)))
 
     def load_initial(self, h):
-        """Sets the handle's state to the reference's initial state for h."""
+        """Sets the handle's state to the reference's initial state for h (h must be
+        ready on this model's stream)."""
         import torch
-        D, Zs = initial_state(self.Z, h, self.zs0, self.hy._params.Dmin)
-        z = torch.zeros_like(D)
-        zp = torch.zeros((self.hy.nx + 1, self.hy.ny), dtype=torch.float64, device=D.device)
-        keep = [Zs.contiguous(), D.contiguous(), zp, zp.clone(), zp.clone(), zp.clone(), z, z.clone(), z.clone(),
-                z.clone()]
-        torch.cuda.synchronize()
+        with torch.cuda.stream(self.stream):
+            D, Zs = initial_state(self.Z, h, self.zs0, self.hy._params.Dmin)
+            z = torch.zeros_like(D)
+            zp = torch.zeros((self.hy.nx + 1, self.hy.ny), dtype=torch.float64, device=D.device)
+            keep = [Zs.contiguous(), D.contiguous(), zp, zp.clone(), zp.clone(), zp.clone(), z, z.clone(), z.clone(),
+                    z.clone()]
+        # the copy runs on the same stream, after the kernels that produce its sources
         self.hy._chk(self.hy._L.m70h_set_state_dev(self.hy.handle, *[ctypes.c_void_p(t.data_ptr()) for t in keep]))
 
     def run_plain(self, h, cD0: float):
@@ -128,7 +139,8 @@ class HydroModel:
 
     def final_zs(self):
         import torch
-        out = torch.empty((self.hy.nx, self.hy.ny), dtype=torch.float64, device="cuda")
+        with torch.cuda.stream(self.stream):
+            out = torch.empty((self.hy.nx, self.hy.ny), dtype=torch.float64, device="cuda")
         self.hy._chk(self.hy._L.m70h_get_state_dev(self.hy.handle, ctypes.c_void_p(out.data_ptr()),
                                                    *([None] * 9)))
         return out
@@ -143,18 +155,28 @@ def _hydro_run_function():
         @staticmethod
         def forward(ctx, h, cD0, model: HydroModel):
             model.set_drag(float(cD0))
+            cur = torch.cuda.current_stream()
+            model.stream.wait_stream(cur)
             model.load_initial(h.detach())
             model.adj.forward(model.nIter, model.slots)
             ctx.model = model
             ctx.save_for_backward(h.detach())
-            return model.final_zs()
+            out = model.final_zs()
+            cur.wait_stream(model.stream)
+            return out
 
         @staticmethod
         def backward(ctx, gout):
             model = ctx.model
             (h,) = ctx.saved_tensors
-            grads, dp = model.adj.reverse({"Zs": gout.contiguous().to(torch.float64)}, want=("Zs", "D"))
-            gh = h_adjoint(model.Z, h, model.zs0, model.hy._params.Dmin, grads["D"], grads["Zs"])
+            cur = torch.cuda.current_stream()
+            model.stream.wait_stream(cur)
+            with torch.cuda.stream(model.stream):
+                seed = gout.contiguous().to(torch.float64)
+            grads, dp = model.adj.reverse({"Zs": seed}, want=("Zs", "D"))
+            with torch.cuda.stream(model.stream):
+                gh = h_adjoint(model.Z, h, model.zs0, model.hy._params.Dmin, grads["D"], grads["Zs"])
+            cur.wait_stream(model.stream)
             gc = torch.tensor(dp["cD0"], dtype=torch.float64, device=h.device)
             return gh, gc, None
 

@@ -258,6 +258,26 @@ def config4(seed: int = 4, steps: int = 400, nx: int = 512, ny: int = 1024, dx: 
     return sc, dh, rng
 
 
+def config5(count: int = 16, seed: int = 5, steps: int = 300, nx_range=(256, 1024), ny_range=(512, 2048),
+            dx: float = 2.0, noise: float = 0.02):
+    """Batched inversion set-up: ``count`` seeded beaches with nx ~ U[nx_range],
+    ny ~ U[ny_range] (Hrms U[0.5, 1.5] m, T U[6, 12] s, angle U[-20, 20] deg), each
+    with a true bar field h + dh (dh smooth, 0.2 m) and the noise level of its
+    observations. Returns (scenarios, dhs, rng)."""
+    rng = np.random.default_rng(seed)
+    scs, dhs = [], []
+    for k in range(count):
+        nx = int(rng.integers(nx_range[0], nx_range[1] + 1))
+        ny = int(rng.integers(ny_range[0], ny_range[1] + 1))
+        sc = seeded_beach(nx, ny, dx, dx, seed * 1000 + k, steps, name=f"inv{k}_{nx}x{ny}")
+        Hrms, T, theta = rng.uniform(0.5, 1.5), rng.uniform(6.0, 12.0), rng.uniform(-20.0, 20.0)
+        sc.forcing = wave_forcing(sc.initial_depth(), sc.x, Hrms, T, theta, sc.physics.Dmin)
+        sc.meta.update(Hrms=Hrms, T=T, theta=theta)
+        scs.append(sc)
+        dhs.append(smooth_field(nx, ny, rng, 0.2))
+    return scs, dhs, rng
+
+
 def batch_scenarios(count: int = 64, seed: int = 3, steps: int = 500,
                     nx_range=(128, 1024), ny_range=(256, 2048), dx: float = 2.0):
     """Uneven scenarios for the batched form: nx ~ U[nx_range], ny ~ U[ny_range]."""

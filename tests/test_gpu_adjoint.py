@@ -220,3 +220,27 @@ def test_bathymetry_inversion_gradient_c4():
     gcD0 = gC / (-2 * S.G / C0 ** 3)   # dcD0/dC = -2 g / C^3
     assert abs(lhs - gcD0) <= 1e-8 * max(abs(lhs), abs(gcD0)), (lhs, gcD0)
     hy.close()
+
+
+def test_batched_gradients_and_descent():
+    """Gradients of several scenarios issued from concurrent host threads (one stream
+    each) equal the one-after-another ones, and gradient descent on h lowers every
+    scenario's loss."""
+    from morfo70_amd.inversion import BatchInversion
+
+    scs, dhs, rng = S.config5(count=4, steps=60, nx_range=(48, 96), ny_range=(64, 128))
+    inv = BatchInversion(scs, 60, slots=8, cD0=S.G / 65.0 ** 2, workers=4)
+    truth = inv.final_zs([torch.as_tensor(sc.h + dh, device="cuda") for sc, dh in zip(scs, dhs)])
+    inv.set_observations([z + torch.as_tensor(rng.normal(0, 0.02, size=z.shape), device="cuda") for z in truth])
+    hs = [torch.as_tensor(np.ascontiguousarray(sc.h), device="cuda") for sc in scs]
+    l1, g1 = inv.loss_and_grad(hs, concurrent=True)
+    l2, g2 = inv.loss_and_grad(hs, concurrent=False)
+    for a, b, ga, gb in zip(l1, l2, g1, g2):
+        assert a == b
+        ga, gb = ga.cpu().numpy(), gb.cpu().numpy()
+        assert np.max(np.abs(ga - gb)) <= 1e-12 * np.max(np.abs(gb))
+    _, hist = inv.descend(hs, iters=5, max_step=0.01)
+    first, last = hist[0][1], hist[-1][1]
+    for q in range(len(scs)):
+        assert last[q] < first[q], (q, first[q], last[q])
+    inv.close()
