@@ -236,7 +236,11 @@ __device__ __forceinline__ void partition_body(const Prov &prov, double *__restr
             if (q < m) prov.get(line, k0 + q, di[q], lo[q], up[q], rh[q]);
         if (!g.periodic) {
             if (k0 == 0) lo[0] = 0.0;
-            if (k1 == n) up[m - 1] = 0.0;
+            if (k1 == n) {   // static indices keep the segment arrays in registers
+#pragma unroll
+                for (int q = 0; q < SEG_M; ++q)
+                    if (q == m - 1) up[q] = 0.0;
+            }
         }
         rr = seg_eliminate<SEG_M>(m, lo, di, up, rh);
     }
@@ -297,7 +301,11 @@ __device__ __forceinline__ void rows_lds_body(const Prov &prov, double *__restri
         }
         if (!periodic) {
             if (k0 == 0) lo[0] = 0.0;
-            if (k1 == n) up[m - 1] = 0.0;
+            if (k1 == n) {   // static indices keep the segment arrays in registers
+#pragma unroll
+                for (int q = 0; q < RM; ++q)
+                    if (q == m - 1) up[q] = 0.0;
+            }
         }
     }
     __syncthreads();   // the coefficient tile is free from here on
@@ -478,7 +486,11 @@ __device__ __forceinline__ void cols_elim_body(const Prov &prov, const ColsWork 
     for (int q = 0; q < CM; ++q)
         if (q < m) prov.get(j, k0 + q, di[q], lo[q], up[q], rh[q]);
     if (k0 == 0) lo[0] = 0.0;
-    if (k1 == n) up[m - 1] = 0.0;
+    if (k1 == n) {   // static indices keep the segment arrays in registers
+#pragma unroll
+        for (int q = 0; q < CM; ++q)
+            if (q == m - 1) up[q] = 0.0;
+    }
     const RedRows r = seg_eliminate<CM>(m, lo, di, up, rh);
 #pragma unroll
     for (int q = 1; q < CM - 1; ++q) {
