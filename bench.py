@@ -55,6 +55,16 @@ KERNEL_BYTES_PER_CELL = {
     "cols_qy_reduced": 10,
 }
 
+# Sediment transport and bed update (src/Sediment.cpp), per cell of the nx x ny grid
+# (the bed update covers rows 1..nx-1): compulsory FP64 fields read once, written once.
+SED_BYTES_PER_CELL = {
+    "sed_columns": 8 * 2,                   # Dw, D in (running max, shoreline scan; Zs of one row per column)
+    "sed_coeffs": 8 * 2 + 8 * 3,            # D, Zb in; Alpha, gamma, Gamma out
+    "sed_fluxes": 8 * 6 + 8 * 2,            # Alpha, gamma, Gamma, h, U, V in; qx, qy out
+    "sed_update": 8 * 6 + 8 * 4,            # qx, qy, dh_dt, h, Zb, Zs in; h, Zb, Zs, dh_dt out
+}
+B_ALG_SED = float(sum(SED_BYTES_PER_CELL.values()))
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -65,12 +75,14 @@ def parse():
     ap.add_argument("--ny", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-steps", type=int, default=300)
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="CPU baseline sample length (0 = 300 ADI steps / 150 sediment steps)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--solver", choices=["fused", "partition", "exact"], default="fused")
     ap.add_argument("--tile-cols", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
-    ap.add_argument("--workload", choices=["tide2048x4096", "batch64", "adjoint512x1024", "inversion16"],
+    ap.add_argument("--workload", choices=["tide2048x4096", "batch64", "adjoint512x1024", "inversion16",
+                                           "sediment2048x4096"],
                     default="tide2048x4096")
     ap.add_argument("--gd-iters", type=int, default=20)
     ap.add_argument("--slots", type=int, default=0,
@@ -96,11 +108,11 @@ def copy_calibration(mib: int):
     print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 5}), flush=True)
 
 
-def measured_traffic(kernel: str, nx: int, ny: int, solver: str):
+def measured_traffic(kernel: str, nx: int, ny: int, solver: str, workload: str = "tide"):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3 FETCH_SIZE and
     WRITE_SIZE passes (each scaled by what the counter reads for a plain copy of known
     size); None when no such measurement exists for this workload."""
-    path = os.path.join(ROOT, "profiles", f"hbm_traffic_tide{nx}x{ny}.json")
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_{workload}{nx}x{ny}.json")
     if solver != "fused" or not os.path.exists(path):
         return None
     with open(path) as f:
@@ -123,6 +135,101 @@ def cpu_baseline(sc, steps: int, threads: int) -> dict:
     return {"value": rate, "unit": "cell-steps/s", "cores": threads, "kind": "port",
             "sample": f"{sc.nx}x{sc.ny} grid, one Run({steps}) after one warm step, "
                       f"{threads} OpenMP threads, {dt:.2f} s"}
+
+
+def bench_sediment(args):
+    """Sediment transport and bed update on the 2048 x 4096 tide beach: a "step" is one
+    calculateSedimentFluxes + updateBottom (src/Sediment.cpp:47-75) over the whole grid,
+    after a short hydro spin-up; inputs resident in HBM."""
+    import torch
+    from morfo70_amd import _lib
+    from morfo70_amd import scenarios as S
+    from morfo70_amd.hydro import from_scenario
+    from morfo70_amd.sediment import Sediment
+
+    spin = 20
+    sc = S.seeded_beach(args.nx, args.ny, 2.0, 2.0, args.seed, spin + 2, tide=True,
+                        name=f"beach_tide_{args.nx}x{args.ny}_s{args.seed}")
+    sp = S.SedimentParams()
+    dt_bed = 0.1                                   # Parameters::Run::dt default
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    g = from_scenario(sc, solver=_lib.SOLVER_FUSED)
+    stream = torch.cuda.Stream(device=dev)
+    g.set_stream(stream.cuda_stream)
+    g.set_forcing()
+    g.run_async(spin)
+    g.synchronize()
+    sd = Sediment(g, params=sp)
+    for _ in range(args.warmup):
+        g._L.m70s_calculate_fluxes_async(sd._s)
+        g._L.m70s_update_bottom_async(sd._s, dt_bed)
+    g.synchronize()
+    g.set_profiling(True)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        sd._chk(g._L.m70s_calculate_fluxes_async(sd._s))
+        sd._chk(g._L.m70s_update_bottom_async(sd._s, dt_bed))
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    prof = g.profile()
+    g.set_profiling(False)
+    sd.pull()
+    finite = bool(np.all(np.isfinite(sd.bathy.h)) and np.all(np.isfinite(sd.qx)))
+    sd.close()
+    g.close()
+    cells = sc.nx * sc.ny
+    value = cells * args.steps / (ms * 1e-3)
+    per_kernel = {k: (v[0] / v[1], v[1], v[0]) for k, v in prof.items() if k in SED_BYTES_PER_CELL and v[1]}
+    dom = max(per_kernel, key=lambda k: per_kernel[k][2])
+    avg_ms = per_kernel[dom][0]
+    rows = sc.nx - 1 if dom == "sed_update" else sc.nx
+    dom_bytes = SED_BYTES_PER_CELL[dom] * rows * sc.ny
+    achieved = dom_bytes / (avg_ms * 1e-3) / 1e9
+    step_gbs = B_ALG_SED * value / 1e9
+    rec = {
+        "metric": "sediment transport + bed update cell-updates/s (src/Sediment.cpp:47-154)",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded bar-trough beach, closed-form wave forcing, tide; "
+                f"{spin} hydro steps of spin-up)",
+        "config": {"workload": f"sediment{sc.nx}x{sc.ny}", "nx": sc.nx, "ny": sc.ny, "dx": 2.0, "dy": 2.0,
+                   "bed_dt": dt_bed, "timed": f"{args.steps} x (fluxes + bed update), one stream"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom, sc.nx, sc.ny, "fused", "sediment"),
+                     "avg_launch_ms": avg_ms, "alg_bytes_per_launch": dom_bytes},
+        "step_roofline": {"bytes_per_cell_update": B_ALG_SED, "achieved": step_gbs, "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
+        "kernels_avg_ms": {k: round(v[0], 5) for k, v in per_kernel.items()},
+        "finite": finite,
+    }
+    if not args.no_cpu:
+        import oracle
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        n = args.cpu_steps or 150
+        o = oracle.OracleHydro(sc, threads=threads)
+        o.run(1)
+        osd = oracle.OracleSediment(o, sp)
+        osd.fluxes()
+        osd.update_bottom(dt_bed)                  # untimed first step
+        t0 = time.perf_counter()
+        for _ in range(n):
+            osd.fluxes()
+            osd.update_bottom(dt_bed)
+        dt = time.perf_counter() - t0
+        del osd, o
+        rec["cpu_baseline"] = {"value": cells * n / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+                               "sample": f"{sc.nx}x{sc.ny} grid, {n} x (fluxes + bed update) after one untimed, "
+                                         f"{threads} OpenMP threads, {dt:.2f} s"}
+        rec["gpu_over_cpu"] = value / rec["cpu_baseline"]["value"]
+    else:
+        rec["cpu_baseline"] = None
+    print(json.dumps(rec), flush=True)
 
 
 def bench_batch(args):
@@ -318,6 +425,11 @@ def main():
             sys.exit(2)
         bench_adjoint(args)
         return
+    if args.workload == "sediment2048x4096":
+        if args.gpus != 1:
+            sys.exit(2)
+        bench_sediment(args)
+        return
     if args.workload == "batch64":
         if args.gpus != 1:
             sys.exit(2)
@@ -400,7 +512,7 @@ def main():
     }
     g.close()
     if not args.no_cpu:
-        record["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps, args.cpu_threads)
+        record["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps or 300, args.cpu_threads)
         record["gpu_over_cpu"] = value / record["cpu_baseline"]["value"]
     else:
         record["cpu_baseline"] = None

@@ -209,6 +209,50 @@ int m70h_batch_run_async(m70h_batch *b, int nIter);
 int m70h_batch_synchronize(m70h_batch *b);
 
 /* ------------------------------------------------------------------------------------
+ * Sediment transport and bed update: class Sediment (include/Sediment.h:17-115,
+ * src/Sediment.cpp). A sediment handle is attached to a hydro handle and works on its
+ * device state: it reads Zs, D, U, V and the wave dissipation Dw of the hydro's forcing,
+ * owns the bed fields h and Zb (Bathymetry::h, ::Zb, which the reference's Sediment
+ * mutates) and the sediment fields, and its bed update moves the hydro's Zs with the bed
+ * (src/Sediment.cpp:131), as the reference's does. Destroy it before its hydro handle.
+ * ------------------------------------------------------------------------------------ */
+/* Parameters::Bottom::p, Parameters::Sediment::CWS::* and ::Swash::*
+ * (include/OptionsParametersConstants.h:84-138). */
+typedef struct m70s_params {
+    double p;
+    double alpha0_max, alpha0_min, D_alpha0_max, D_alpha0_min, gamma0;
+    double Gamma0, A_swash;
+} m70s_params;
+
+typedef struct m70s m70s;
+
+/* Replaces Sediment::Sediment (include/Sediment.h:66; src/Sediment.cpp:13-31): zeroed
+ * sediment fields; h and Zb (host, F doubles) are the bathymetry's bed fields. */
+int m70s_create(m70h *hydro, const double *h, const double *Zb, const m70s_params *p, m70s **out);
+/* Replaces Sediment::~Sediment (include/Sediment.h:71). */
+void m70s_destroy(m70s *s);
+const char *m70s_last_error(const m70s *s);
+/* Replaces Sediment::calculateSedimentFluxes (include/Sediment.h:76; src/Sediment.cpp:47-65). */
+int m70s_calculate_fluxes(m70s *s);
+/* Replaces Sediment::updateBottom (include/Sediment.h:81; src/Sediment.cpp:67-75,105-151);
+ * dt is Parameters::Run::dt, read at call time in the reference. */
+int m70s_update_bottom(m70s *s, double dt);
+/* The two calls above without the wait: enqueued on the hydro handle's stream after its
+ * pending work (m70h_run_async), completed by m70h_synchronize or any m70s call that
+ * copies to or from the host. */
+int m70s_calculate_fluxes_async(m70s *s);
+int m70s_update_bottom_async(m70s *s, double dt);
+/* Replaces Sediment::setSedimentFluxesFromResumeFile's arithmetic (include/Sediment.h:87;
+ * src/Sediment.cpp:77-103): sets qx, qy (host, F) and the dh_dt they imply. */
+int m70s_set_fluxes(m70s *s, const double *qx, const double *qy);
+/* The public arrays Alpha, gamma, Gamma, qx, qy, dh_dt (include/Sediment.h:21-26) and the
+ * bed fields h, Zb; host pointers of F doubles, NULL = skip. */
+int m70s_get(m70s *s, double *Alpha, double *gamma, double *Gamma, double *qx, double *qy, double *dh_dt, double *h,
+             double *Zb);
+/* Callers' writes to Bathymetry::h / ::Zb between calls (host, F; NULL = unchanged). */
+int m70s_set_bed(m70s *s, const double *h, const double *Zb);
+
+/* ------------------------------------------------------------------------------------
  * Adjoint of Run. No counterpart in the reference, which has no derivatives: these
  * entry points serve callers that fit bathymetry or friction to observations (gradient
  * of a loss of the final state with respect to the initial state and the drag and

@@ -58,6 +58,16 @@ SIGNATURES = {
     "m70h_batch_run": (ctypes.c_int, [_vp, ctypes.c_int]),
     "m70h_batch_run_async": (ctypes.c_int, [_vp, ctypes.c_int]),
     "m70h_batch_synchronize": (ctypes.c_int, [_vp]),
+    "m70s_create": (ctypes.c_int, [_vp, _dp, _dp, _vp, ctypes.POINTER(_vp)]),
+    "m70s_destroy": (None, [_vp]),
+    "m70s_last_error": (ctypes.c_char_p, [_vp]),
+    "m70s_calculate_fluxes": (ctypes.c_int, [_vp]),
+    "m70s_update_bottom": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "m70s_calculate_fluxes_async": (ctypes.c_int, [_vp]),
+    "m70s_update_bottom_async": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "m70s_set_fluxes": (ctypes.c_int, [_vp, _dp, _dp]),
+    "m70s_get": (ctypes.c_int, [_vp] + [_dp] * 8),
+    "m70s_set_bed": (ctypes.c_int, [_vp, _dp, _dp]),
     "m70h_adjoint_forward": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "m70h_adjoint_reverse": (ctypes.c_int, [_vp, _vp, _vp, _dp]),
     "m70h_adjoint_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),

@@ -13,7 +13,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libm70h.so")
 
-SOURCES = ["hydro_kernels.hip", "line_solvers.hip", "partition_solver.hip", "adjoint.hip", "m70h_api.hip"]
+SOURCES = ["hydro_kernels.hip", "line_solvers.hip", "partition_solver.hip", "adjoint.hip", "sediment.hip",
+           "m70h_api.hip"]
 
 # -ffp-contract=off: no fused multiply-adds, so every product and sum rounds as the
 #   reference's x86-64 build does.
@@ -47,7 +48,7 @@ def build(verbose: bool = False) -> str:
     # C++ facade (the reference's Hydro call surface) and its driver, plain g++
     host = os.path.join(HERE, "host")
     cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", FACADE, os.path.join(host, "facade_run.cpp"),
-           os.path.join(host, "Hydro.cpp"), "-L" + HERE, "-lm70h", "-Wl,-rpath,$ORIGIN/.."]
+           os.path.join(host, "Hydro.cpp"), os.path.join(host, "Sediment.cpp"), "-L" + HERE, "-lm70h", "-Wl,-rpath,$ORIGIN/.."]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -13,6 +13,7 @@
 #include "adjoint_kernels.h"
 #include "hydro_device.h"
 #include "hydro_kernels.h"
+#include "sediment.h"
 
 using namespace m70;
 
@@ -25,12 +26,14 @@ thread_local std::string g_create_error;
 enum KernelId {
     K_DRAG_VISC, K_RUN_FIELDS, K_QXX_ASM, K_THOMAS_QX, K_QXY_ASM, K_PERIODIC_QX, K_QX_TAIL,
     K_QYY_ASM, K_PERIODIC_QY, K_QYX_ASM, K_THOMAS_QY, K_QY_TAIL, K_FUSED_QXY, K_FUSED_QYY,
-    K_COLS_QX_ELIM, K_COLS_QX_RED, K_COLS_QX_BACK, K_COLS_QY_ELIM, K_COLS_QY_RED, K_COLS_QY_BACK, K_COUNT
+    K_COLS_QX_ELIM, K_COLS_QX_RED, K_COLS_QX_BACK, K_COLS_QY_ELIM, K_COLS_QY_RED, K_COLS_QY_BACK,
+    K_SED_COLUMNS, K_SED_COEFFS, K_SED_FLUXES, K_SED_UPDATE, K_COUNT
 };
 const char *const kKernelNames[K_COUNT] = {
     "drag_viscosity", "run_fields", "qxx_assemble", "thomas_qx", "qxy_assemble", "periodic_qx", "qx_tail",
     "qyy_assemble", "periodic_qy", "qyx_assemble", "thomas_qy", "qy_tail", "fused_qxy", "fused_qyy",
-    "cols_qx_elim", "cols_qx_reduced", "cols_qx_backsub", "cols_qy_elim", "cols_qy_reduced", "cols_qy_backsub"};
+    "cols_qx_elim", "cols_qx_reduced", "cols_qx_backsub", "cols_qy_elim", "cols_qy_reduced", "cols_qy_backsub",
+    "sed_columns", "sed_coeffs", "sed_fluxes", "sed_update"};
 
 }  // namespace
 
@@ -1429,6 +1432,201 @@ int m70h_adjoint_release(m70h *h)
 {
     if (!h) return M70H_EINVAL;
     adj_release(h);
+    return M70H_OK;
+}
+
+}  // extern "C"
+
+// =====================================================================================
+// Sediment transport and bed update (class Sediment, src/Sediment.cpp)
+// =====================================================================================
+struct m70s {
+    m70h *h = nullptr;
+    m70s_params p{};
+    std::string err;
+    std::vector<void *> allocs;
+    double *Alpha = nullptr, *gamma = nullptr, *Gamma = nullptr, *qx = nullptr, *qy = nullptr, *dh_dt = nullptr;
+    double *bed_h = nullptr, *Zb = nullptr, *maxDw = nullptr, *Zs_sh = nullptr;
+    double *part_max = nullptr;
+    int *part_last = nullptr;
+};
+
+namespace {
+
+int sfail(m70s *s, int code, const std::string &msg)
+{
+    if (s) s->err = msg; else g_create_error = msg;
+    return code;
+}
+
+#define SHIPCHK(s, expr)                                                                       \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return sfail(s, M70H_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+SedArgs sed_args(m70s *s)
+{
+    const m70h *h = s->h;
+    SedArgs a{};
+    a.nx = h->nx;
+    a.ny = h->ny;
+    a.Dmin = h->p.Dmin;
+    a.inv_dy = 1 / h->dy;
+    a.inv_dx = h->m.inv_dx;
+    a.inv_dxx = h->m.inv_dxx;
+    a.p = s->p;
+    a.D = h->s.D; a.U = h->s.U; a.V = h->s.V; a.Dw = h->fw.Dw; a.Zs = h->s.Zs;
+    a.Alpha = s->Alpha; a.gamma = s->gamma; a.Gamma = s->Gamma; a.qx = s->qx; a.qy = s->qy; a.dh_dt = s->dh_dt;
+    a.h = s->bed_h; a.Zb = s->Zb; a.maxDw = s->maxDw; a.Zs_sh = s->Zs_sh;
+    a.part_max = s->part_max; a.part_last = s->part_last;
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *m70s_last_error(const m70s *s) { return s ? s->err.c_str() : g_create_error.c_str(); }
+
+int m70s_create(m70h *hydro, const double *hb, const double *Zb, const m70s_params *p, m70s **out)
+{
+    if (!out) return sfail(nullptr, M70H_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!hydro || !hb || !Zb || !p) return sfail(nullptr, M70H_EINVAL, "hydro, h, Zb and p must be non-NULL");
+    if (!(p->p < 1.0) || !(p->D_alpha0_min != p->D_alpha0_max))
+        return sfail(nullptr, M70H_EINVAL, "need p < 1 and D_alpha0_min != D_alpha0_max");
+    m70s *s = new m70s();
+    s->h = hydro;
+    s->p = *p;
+    const long F = hydro->F;
+    double **fp[] = {&s->Alpha, &s->gamma, &s->Gamma, &s->qx, &s->qy, &s->dh_dt, &s->bed_h, &s->Zb};
+    for (auto q : fp) {
+        void *m = nullptr;
+        hipError_t e = hipMalloc(&m, (size_t)F * sizeof(double));
+        if (e == hipSuccess) e = hipMemset(m, 0, (size_t)F * sizeof(double));
+        if (e != hipSuccess) {
+            g_create_error = std::string("m70s_create: ") + hipGetErrorString(e);
+            m70s_destroy(s);
+            return M70H_EHIP;
+        }
+        s->allocs.push_back(m);
+        *q = (double *)m;
+    }
+    for (double **q : {&s->maxDw, &s->Zs_sh}) {
+        void *m = nullptr;
+        hipError_t e = hipMalloc(&m, (size_t)hydro->ny * sizeof(double));
+        if (e == hipSuccess) e = hipMemset(m, 0, (size_t)hydro->ny * sizeof(double));
+        if (e != hipSuccess) {
+            g_create_error = std::string("m70s_create: ") + hipGetErrorString(e);
+            m70s_destroy(s);
+            return M70H_EHIP;
+        }
+        s->allocs.push_back(m);
+        *q = (double *)m;
+    }
+    for (void **q : {(void **)&s->part_max, (void **)&s->part_last}) {
+        void *m = nullptr;
+        const size_t nb = (size_t)SED_MAX_CHUNKS * hydro->ny * (q == (void **)&s->part_max ? sizeof(double) : sizeof(int));
+        hipError_t e = hipMalloc(&m, nb);
+        if (e != hipSuccess) {
+            g_create_error = std::string("m70s_create: ") + hipGetErrorString(e);
+            m70s_destroy(s);
+            return M70H_EHIP;
+        }
+        s->allocs.push_back(m);
+        *q = m;
+    }
+    int rc = m70s_set_bed(s, hb, Zb);
+    if (rc) {
+        g_create_error = s->err;
+        m70s_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return M70H_OK;
+}
+
+void m70s_destroy(m70s *s)
+{
+    if (!s) return;
+    if (s->h && s->h->stream) (void)hipStreamSynchronize(s->h->stream);
+    for (void *q : s->allocs) (void)hipFree(q);
+    delete s;
+}
+
+int m70s_set_bed(m70s *s, const double *hb, const double *Zb)
+{
+    if (!s) return M70H_EINVAL;
+    const size_t nb = (size_t)s->h->F * sizeof(double);
+    if (hb) SHIPCHK(s, hipMemcpyAsync(s->bed_h, hb, nb, hipMemcpyHostToDevice, s->h->stream));
+    if (Zb) SHIPCHK(s, hipMemcpyAsync(s->Zb, Zb, nb, hipMemcpyHostToDevice, s->h->stream));
+    SHIPCHK(s, hipStreamSynchronize(s->h->stream));
+    return M70H_OK;
+}
+
+int m70s_calculate_fluxes_async(m70s *s)
+{
+    if (!s) return M70H_EINVAL;
+    m70h *h = s->h;
+    if (!h->forcing_set) return sfail(s, M70H_ESTATE, "the hydro handle has no forcing (Dw)");
+    const SedArgs a = sed_args(s);
+    int rc;
+    if ((rc = timed(h, K_SED_COLUMNS, [&] { return launch_sed_columns(h->stream, a); })) ||
+        (rc = timed(h, K_SED_COEFFS, [&] { return launch_sed_coeffs(h->stream, a); })) ||
+        (rc = timed(h, K_SED_FLUXES, [&] { return launch_sed_qxqy(h->stream, a); })))
+        return sfail(s, rc, h->err);
+    return M70H_OK;
+}
+
+int m70s_update_bottom_async(m70s *s, double dt)
+{
+    if (!s) return M70H_EINVAL;
+    m70h *h = s->h;
+    const SedArgs a = sed_args(s);
+    int rc = timed(h, K_SED_UPDATE, [&] { return launch_sed_update(h->stream, a, dt); });
+    if (rc) return sfail(s, rc, h->err);
+    h->run_gen++;   // Zs moved with the bed
+    return M70H_OK;
+}
+
+int m70s_calculate_fluxes(m70s *s)
+{
+    int rc = m70s_calculate_fluxes_async(s);
+    if (rc) return rc;
+    if ((rc = m70h_synchronize(s->h))) return sfail(s, rc, s->h->err);
+    return M70H_OK;
+}
+
+int m70s_update_bottom(m70s *s, double dt)
+{
+    int rc = m70s_update_bottom_async(s, dt);
+    if (rc) return rc;
+    if ((rc = m70h_synchronize(s->h))) return sfail(s, rc, s->h->err);
+    return M70H_OK;
+}
+
+int m70s_set_fluxes(m70s *s, const double *qx, const double *qy)
+{
+    if (!s || !qx || !qy) return M70H_EINVAL;
+    const size_t nb = (size_t)s->h->F * sizeof(double);
+    SHIPCHK(s, hipMemcpyAsync(s->qx, qx, nb, hipMemcpyHostToDevice, s->h->stream));
+    SHIPCHK(s, hipMemcpyAsync(s->qy, qy, nb, hipMemcpyHostToDevice, s->h->stream));
+    SHIPCHK(s, launch_sed_dhdt(s->h->stream, sed_args(s)));
+    SHIPCHK(s, hipStreamSynchronize(s->h->stream));
+    return M70H_OK;
+}
+
+int m70s_get(m70s *s, double *Alpha, double *gamma, double *Gamma, double *qx, double *qy, double *dh_dt, double *hb,
+             double *Zb)
+{
+    if (!s) return M70H_EINVAL;
+    double *dst[8] = {Alpha, gamma, Gamma, qx, qy, dh_dt, hb, Zb};
+    const double *src[8] = {s->Alpha, s->gamma, s->Gamma, s->qx, s->qy, s->dh_dt, s->bed_h, s->Zb};
+    const size_t nb = (size_t)s->h->F * sizeof(double);
+    for (int q = 0; q < 8; ++q)
+        if (dst[q]) SHIPCHK(s, hipMemcpyAsync(dst[q], src[q], nb, hipMemcpyDeviceToHost, s->h->stream));
+    SHIPCHK(s, hipStreamSynchronize(s->h->stream));
     return M70H_OK;
 }
 

@@ -67,6 +67,26 @@ class Physics:
 
 
 @dataclasses.dataclass
+class SedimentParams:
+    """Parameters::Bottom::p and Parameters::Sediment::CWS / ::Swash
+    (include/OptionsParametersConstants.h:84-138), defaults of
+    src/OptionsParametersConstants.cpp:44,76-85."""
+
+    p: float = 0.4
+    alpha0_max: float = 0.002
+    alpha0_min: float = 0.0
+    D_alpha0_max: float = 4.0
+    D_alpha0_min: float = 10.0
+    gamma0: float = 0.5
+    Gamma0: float = 1.0
+    A_swash: float = 0.3
+
+    def as_tuple(self):
+        return (self.p, self.alpha0_max, self.alpha0_min, self.D_alpha0_max, self.D_alpha0_min, self.gamma0,
+                self.Gamma0, self.A_swash)
+
+
+@dataclasses.dataclass
 class Scenario:
     name: str
     nx: int

@@ -303,3 +303,58 @@ class TangentHydro:
         out = {k: np.zeros(shp[k] + (2,)) for k in STATE_FIELDS}
         tlm_lib().orc_get_state(self.h, *[_vp(out[k]) for k in STATE_FIELDS])
         return {k: out[k][..., 0].copy() for k in out}, {k: out[k][..., 1].copy() for k in out}
+
+
+# ------------------------------------------------------------------------ sediment
+class _SedParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("p", "alpha0_max", "alpha0_min", "D_alpha0_max", "D_alpha0_min",
+                                               "gamma0", "Gamma0", "A_swash")]
+
+
+SED_FIELDS = ("Alpha", "gamma", "Gamma", "qx", "qy", "dh_dt", "h", "Zb")
+
+
+class OracleSediment:
+    """Sediment transport and bed update of the oracle (Sediment, src/Sediment.cpp) on an
+    OracleHydro: reads its Zs, D, U, V, Dw; update_bottom mutates its Zs."""
+
+    def __init__(self, hydro: OracleHydro, params):
+        L = lib()
+        if not hasattr(L, "_sed_ready"):
+            L.orc_sed_create.restype = ctypes.c_void_p
+            L.orc_sed_create.argtypes = [ctypes.c_void_p, ctypes.POINTER(_SedParams)]
+            L.orc_sed_destroy.argtypes = [ctypes.c_void_p]
+            L.orc_sed_fluxes.argtypes = [ctypes.c_void_p]
+            L.orc_sed_update_bottom.argtypes = [ctypes.c_void_p, ctypes.c_double]
+            L.orc_sed_set_fluxes.argtypes = [ctypes.c_void_p, _dp, _dp]
+            L.orc_sed_get.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _dp]
+            L.orc_sed_get.restype = ctypes.c_long
+            L._sed_ready = True
+        self.hy = hydro
+        self._p = _SedParams(*params.as_tuple())
+        self.s = L.orc_sed_create(hydro.h, ctypes.byref(self._p))
+        if not self.s:
+            raise ValueError("orc_sed_create failed")
+
+    def __del__(self):
+        if getattr(self, "s", None):
+            lib().orc_sed_destroy(self.s)
+            self.s = None
+
+    def fluxes(self):
+        lib().orc_sed_fluxes(self.s)
+
+    def update_bottom(self, dt: float):
+        lib().orc_sed_update_bottom(self.s, float(dt))
+
+    def set_fluxes(self, qx, qy):
+        qx, qy = (np.ascontiguousarray(a, dtype=np.float64) for a in (qx, qy))
+        lib().orc_sed_set_fluxes(self.s, ptr(qx), ptr(qy))
+
+    def get(self, name: str) -> np.ndarray:
+        n = lib().orc_sed_get(self.s, name.encode(), None)
+        if n < 0:
+            raise KeyError(name)
+        out = np.zeros(n)
+        lib().orc_sed_get(self.s, name.encode(), ptr(out))
+        return out.reshape(-1, self.hy.ny) if n != self.hy.ny else out

@@ -1295,3 +1295,256 @@ long orc_get_field(const orc_hydro *H, const char *name, double *out)
         }
     return -1;
 }
+
+/* ================================================================================== */
+/* Sediment transport and bed update (src/Sediment.cpp)                               */
+/* ================================================================================== */
+/* include/OptionsParametersConstants.h:15,17,22-23 */
+static const double pi_c = 3.141592653589793;
+static const double s_rel = 2.65;
+static const double tanPhi_i = 0.6249;
+
+struct orc_sediment {
+    orc_hydro *H;
+    orc_sed_params sp;
+    double *Alpha, *gamma, *Gamma, *qx, *qy, *dh_dt, *h, *Zb, *maxDw, *Zs_sh;
+};
+
+orc_sediment *orc_sed_create(orc_hydro *H, const orc_sed_params *sp)
+{   /* Sediment::Sediment, src/Sediment.cpp:13-31: zeroed arrays */
+    orc_sediment *S = (orc_sediment *)calloc(1, sizeof(orc_sediment));
+    if (!S) return NULL;
+    const long size = H->size;
+    long q;
+    S->H = H;
+    S->sp = *sp;
+    S->Alpha = zalloc(size); S->gamma = zalloc(size); S->Gamma = zalloc(size);
+    S->qx = zalloc(size); S->qy = zalloc(size); S->dh_dt = zalloc(size);
+    S->h = zalloc(size); S->Zb = zalloc(size);
+    S->maxDw = zalloc(H->ny); S->Zs_sh = zalloc(H->ny);
+    for (q = 0; q < size; q++) {   /* bathy->h, bathy->Zb = Z - h (src/Bathymetry.cpp:34) */
+        S->h[q] = H->h[q];
+        S->Zb[q] = H->Z[q] - H->h[q];
+    }
+    return S;
+}
+
+void orc_sed_destroy(orc_sediment *S)
+{
+    if (!S) return;
+    free(S->Alpha); free(S->gamma); free(S->Gamma); free(S->qx); free(S->qy); free(S->dh_dt);
+    free(S->h); free(S->Zb); free(S->maxDw); free(S->Zs_sh);
+    free(S);
+}
+
+/* Sediment::getMaxWaveBreakingDissipation, :252-262: running maximum over every call */
+static void sed_max_dw(orc_sediment *S)
+{
+    const int nx = S->H->nx, ny = S->H->ny;
+    const double *dw = S->H->Dw;
+    int i, j;
+#pragma omp parallel for private(i) schedule(static)
+    for (j = 0; j < ny; j++)
+        for (i = 0; i < nx; i++) {
+            const double a = S->maxDw[j], b = dw[(long)i * ny + j];
+            S->maxDw[j] = (a < b) ? b : a;   /* std::max(a, b) */
+        }
+}
+
+/* Sediment::getShorelineZs, :264-283: Zs of the last row i >= 1 where rows i-1 and i
+ * are not both dry; the previous value stays otherwise */
+static void sed_shoreline(orc_sediment *S)
+{
+    const int nx = S->H->nx, ny = S->H->ny;
+    const double *zs = S->H->Zs, *d = S->H->D;
+    const double Dmin = S->H->p.Dmin + 1E-4;
+    int i, j;
+#pragma omp parallel for private(i) schedule(static)
+    for (j = 0; j < ny; j++)
+        for (i = 1; i < nx; i++) {
+            const int isDry = (d[(long)i * ny + j] <= Dmin);
+            const int isDryLow = (d[(long)(i - 1) * ny + j] <= Dmin);
+            const int isSh = !(isDry * isDryLow);
+            S->Zs_sh[j] = (1 - isSh) * S->Zs_sh[j] + zs[(long)i * ny + j] * isSh;
+        }
+}
+
+/* Sediment::getWaveStirringCWS, :222-239 */
+static void sed_stirring(orc_sediment *S, int i)
+{
+    const int ny = S->H->ny;
+    const double alpha0_min = S->sp.alpha0_min;
+    const double incrAlpha = S->sp.alpha0_max - alpha0_min;
+    const double Dmax = S->sp.D_alpha0_max;
+    const double Dmax_min = S->sp.D_alpha0_min - Dmax;
+    const double *d = S->H->D + (long)i * ny;
+    double *alpha = S->Alpha + (long)i * ny;
+    int j;
+    for (j = 0; j < ny; j++) {
+        double x = 1 + 2 * (Dmax - d[j]) / Dmax_min;
+        alpha[j] = alpha0_min + incrAlpha * 0.5 * (1 + tanh(pi_c * x));
+    }
+}
+
+/* Sediment::getGammaDownslopeCWS, :194-220 (the EGU form; sf_drySwash is unused) */
+static void sed_downslope(orc_sediment *S, int i)
+{
+    const int ny = S->H->ny;
+    const double inv_tanPhi_i = 1 / tanPhi_i;
+    const double gamma0_tanPhi = S->sp.gamma0 * inv_tanPhi_i;
+    const double *d = S->H->D + (long)i * ny;
+    const double Dmin = S->H->p.Dmin;
+    const double A_swash = S->sp.A_swash;
+    double *g_ = S->gamma + (long)i * ny;
+    int j;
+    for (j = 0; j < ny; j++) {
+        const double u_bsl = (tanh(pi_c * (2 * (d[j] - Dmin) / (A_swash - Dmin) - 1)) + 1) / 2;
+        g_[j] = u_bsl * gamma0_tanPhi;
+    }
+}
+
+/* Sediment::getGammaSwash, :241-266 (abs resolves to the double overload) */
+static void sed_swash(orc_sediment *S, int i)
+{
+    const int ny = S->H->ny;
+    const double G0 = S->sp.Gamma0;
+    const double Dmin = S->H->p.Dmin + 1E-4;
+    const double A_swash = S->sp.A_swash;
+    const double inv_rho_g_sm1 = 1 / (g * rho * (s_rel - 1));
+    const double *d = S->H->D + (long)i * ny, *zb = S->Zb + (long)i * ny;
+    double *G_ = S->Gamma + (long)i * ny;
+    const double _2_Doff_on = 2 / (A_swash - Dmin);
+    int j;
+    for (j = 0; j < ny; j++) {
+        const int isDry = (d[j] <= Dmin);
+        const double d_dry = (1 - isDry) * d[j] + isDry * (fabs(zb[j] + S->Zs_sh[j]));
+        const double sf_swash = 1 - (tanh(pi_c * (_2_Doff_on * (d_dry - Dmin) - 1)) + 1) * 0.5;
+        G_[j] = G0 * inv_rho_g_sm1 * sf_swash * S->maxDw[j];
+    }
+}
+
+/* Sediment::getqx, :156-174: rows 0..nx-2, U row i+1 */
+static void sed_qx(orc_sediment *S, int i)
+{
+    const int ny = S->H->ny;
+    const double inv_dx = 1 / S->H->dx[i];
+    const double *g_ = S->gamma + (long)i * ny, *A = S->Alpha + (long)i * ny, *G_ = S->Gamma + (long)i * ny;
+    const double *h = S->h + (long)i * ny, *U = S->H->U + (long)(i + 1) * ny;
+    double *qx = S->qx + (long)i * ny;
+    int j;
+    for (j = 0; j < ny; j++) {
+        double A_x = 0.5 * (A[j] + A[j + ny]);
+        double g_x = 0.5 * (g_[j] + g_[j + ny]);
+        double G_x = 0.5 * (G_[j] + G_[j + ny]);
+        double dh_dx = (h[j + ny] - h[j]) * inv_dx;
+        qx[j] = A_x * (U[j] - g_x * dh_dx) - G_x * dh_dx;
+    }
+}
+
+/* Sediment::getqy, :176-192: rows 0..nx-1, V row i+1, periodic in j */
+static void sed_qy(orc_sediment *S, int i)
+{
+    const int ny = S->H->ny;
+    const double inv_dy = 1 / S->H->dy;
+    const double *g_ = S->gamma + (long)i * ny, *A = S->Alpha + (long)i * ny, *G_ = S->Gamma + (long)i * ny;
+    const double *h = S->h + (long)i * ny, *V = S->H->V + (long)(i + 1) * ny;
+    double *qy = S->qy + (long)i * ny;
+    int j;
+    for (j = 0; j < ny; j++) {
+        const int jp = (j == ny - 1) ? 0 : j + 1;
+        double A_y = 0.5 * (A[j] + A[jp]);
+        double g_y = 0.5 * (g_[j] + g_[jp]);
+        double G_y = 0.5 * (G_[j] + G_[jp]);
+        double dh_dy = (h[jp] - h[j]) * inv_dy;
+        qy[j] = A_y * (V[j] - g_y * dh_dy) - G_y * dh_dy;
+    }
+}
+
+void orc_sed_fluxes(orc_sediment *S)
+{   /* :47-65. The reference's OpenMP loop computes row i's coefficients and then qx of row
+     * i-1, which reads row i-1's coefficients that another thread may not have written
+     * yet; its serial order is the intended one. Every row's coefficients read only
+     * inputs, so computing all of them first and then the fluxes gives the serial
+     * order's values, race-free in parallel. */
+    const int nx = S->H->nx;
+    int i;
+    sed_max_dw(S);
+    sed_shoreline(S);
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < nx; i++) {
+        sed_stirring(S, i);
+        sed_downslope(S, i);
+        sed_swash(S, i);
+    }
+#pragma omp parallel for schedule(static)
+    for (i = 0; i < nx; i++) {
+        if (i < nx - 1) sed_qx(S, i);
+        sed_qy(S, i);
+    }
+}
+
+void orc_sed_update_bottom(orc_sediment *S, double dt)
+{   /* Sediment::updateBottomColumn, :105-151, rows 1..nx-1 (qx row nx-1 is never
+     * computed and stays zero) */
+    const int nx = S->H->nx, ny = S->H->ny;
+    const double dt_2 = 0.5 * dt;
+    const double inv_dy = 1 / S->H->dy;
+    const double p = S->sp.p;
+    const double _1_m1p = 1 / (1 - p);
+    int i;
+#pragma omp parallel for schedule(static)
+    for (i = 1; i < nx; i++) {
+        const double inv_dx = 1 / S->H->dx_x[i];
+        const double *qx = S->qx + (long)(i - 1) * ny, *qy = S->qy + (long)i * ny;
+        double *h = S->h + (long)i * ny, *Zb = S->Zb + (long)i * ny, *Zs = S->H->Zs + (long)i * ny;
+        double *m1 = S->dh_dt + (long)i * ny;
+        int j;
+        for (j = 0; j < ny; j++) {
+            const int jm = (j == 0) ? ny - 1 : j - 1;
+            double dqy_dy = inv_dy * (qy[j] - qy[jm]);
+            double dqx_dx = inv_dx * (qx[ny + j] - qx[j]);
+            double dh_dt = -_1_m1p * (dqx_dx + dqy_dy);
+            double incr_h = dt_2 * (3 * dh_dt - m1[j]);   /* Adams-Bashforth 2 */
+            h[j] += incr_h;
+            Zb[j] -= incr_h;
+            Zs[j] += incr_h;
+            m1[j] = dh_dt;
+        }
+    }
+}
+
+void orc_sed_set_fluxes(orc_sediment *S, const double *qx, const double *qy)
+{
+    const int nx = S->H->nx, ny = S->H->ny;
+    const double p = S->sp.p;
+    const double _1_m1p = 1 / (1 - p);
+    const double inv_dy = 1 / S->H->dy;
+    int i, j;
+    memcpy(S->qx, qx, sizeof(double) * (size_t)S->H->size);
+    memcpy(S->qy, qy, sizeof(double) * (size_t)S->H->size);
+    for (i = 1; i < nx; i++) {
+        const double inv_dx = 1 / S->H->dx_x[i];
+        for (j = 0; j < ny; j++) {
+            const int jm = (j == 0) ? ny - 1 : j - 1;
+            double dqy_dy = inv_dy * (S->qy[(long)i * ny + j] - S->qy[(long)i * ny + jm]);
+            double dqx_dx = inv_dx * (S->qx[(long)i * ny + j] - S->qx[(long)(i - 1) * ny + j]);
+            S->dh_dt[(long)i * ny + j] = -_1_m1p * (dqx_dx + dqy_dy);
+        }
+    }
+}
+
+long orc_sed_get(orc_sediment *S, const char *name, double *out)
+{
+    const long F = S->H->size, ny = S->H->ny;
+    struct { const char *n; double *p; long len; } tab[] = {
+        {"Alpha", S->Alpha, F}, {"gamma", S->gamma, F}, {"Gamma", S->Gamma, F}, {"qx", S->qx, F},
+        {"qy", S->qy, F}, {"dh_dt", S->dh_dt, F}, {"h", S->h, F}, {"Zb", S->Zb, F},
+        {"maxDw", S->maxDw, ny}, {"Zs_sh", S->Zs_sh, ny}};
+    size_t q;
+    for (q = 0; q < sizeof(tab) / sizeof(tab[0]); q++)
+        if (strcmp(tab[q].n, name) == 0) {
+            if (out) memcpy(out, tab[q].p, sizeof(double) * (size_t)tab[q].len);
+            return tab[q].len;
+        }
+    return -1;
+}

@@ -96,6 +96,31 @@ void orc_tridiag_periodic(const double *a, const double *b, const double *c, con
 int orc_wet_rule(double d, double dup, double zs, double zsup, double s, double sup,
                  double Dmin);
 
+/* ---------------------------------------------------------------------------------
+ * Sediment transport and bed update (class Sediment, include/Sediment.h:17-115,
+ * src/Sediment.cpp). The sediment object reads the hydro object's Zs, D, U, V and Dw and
+ * owns the bed fields h and Zb (copied from the hydro's at creation: h, Z - h).
+ * update_bottom mutates the hydro's Zs, as the reference's does (src/Sediment.cpp:131).
+ * --------------------------------------------------------------------------------- */
+typedef struct orc_sed_params {
+    double p;             /* Parameters::Bottom::p */
+    double alpha0_max, alpha0_min, D_alpha0_max, D_alpha0_min, gamma0; /* Parameters::Sediment::CWS */
+    double Gamma0, A_swash;                                           /* Parameters::Sediment::Swash */
+} orc_sed_params;
+typedef struct orc_sediment orc_sediment;
+
+orc_sediment *orc_sed_create(orc_hydro *H, const orc_sed_params *sp);
+void orc_sed_destroy(orc_sediment *S);
+/* Sediment::calculateSedimentFluxes, src/Sediment.cpp:47-65 */
+void orc_sed_fluxes(orc_sediment *S);
+/* Sediment::updateBottom, src/Sediment.cpp:67-75,105-151; dt = Parameters::Run::dt */
+void orc_sed_update_bottom(orc_sediment *S, double dt);
+/* Sediment::setSedimentFluxesFromResumeFile's arithmetic, src/Sediment.cpp:77-103 */
+void orc_sed_set_fluxes(orc_sediment *S, const double *qx, const double *qy);
+/* Copies a field: Alpha, gamma, Gamma, qx, qy, dh_dt, h, Zb (F) or maxDw, Zs_sh (ny);
+ * returns its length, -1 for an unknown name; out may be NULL to query the length. */
+long orc_sed_get(orc_sediment *S, const char *name, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -72,6 +72,12 @@ inline orc_dual cbrt(const orc_dual &a)
     const double c = std::cbrt(a.v);
     return orc_dual(c, a.d == 0.0 ? 0.0 : a.d / (3.0 * c * c));
 }
+inline orc_dual tanh(const orc_dual &a)
+{
+    const double t = std::tanh(a.v);
+    return orc_dual(t, a.d == 0.0 ? 0.0 : a.d * (1.0 - t * t));
+}
+inline orc_dual fabs(const orc_dual &a) { return a.v < 0 ? orc_dual(-a.v, -a.d) : a; }
 inline orc_dual log(const orc_dual &a) { return orc_dual(std::log(a.v), a.d == 0.0 ? 0.0 : a.d / a.v); }
 inline orc_dual exp(const orc_dual &a)
 {

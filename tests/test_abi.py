@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_symbols():
     txt = open(os.path.join(ROOT, "include", "m70h.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(m70h_[a-z_0-9]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(m70[hs]_[a-z_0-9]+)\s*\(", txt)))
 
 
 def test_header_matches_binding():
@@ -35,3 +35,12 @@ def test_create_rejects_bad_dims_without_device():
     assert rc == _lib.M70H_EINVAL
     assert b"nx and ny" in L.m70h_last_error(None)
     assert not h.value
+
+
+def test_sediment_create_rejects_null_hydro_without_device():
+    L = _lib.load()
+    s = ctypes.c_void_p()
+    rc = L.m70s_create(None, None, None, None, ctypes.byref(s))
+    assert rc == _lib.M70H_EINVAL
+    assert b"non-NULL" in L.m70s_last_error(None)
+    assert not s.value

@@ -16,21 +16,24 @@ from tests.helpers import FIELDS, compare_states, nonuniform_beach
 pytestmark = pytest.mark.gpu
 
 
-def write_input(path, sc, runs, steps, solver):
+def write_input(path, sc, runs, steps, solver, sediment=0, bed_dt=0.1):
     ph = sc.physics
     with open(path, "wb") as f:
-        np.array([sc.nx, sc.ny, runs, steps, ph.drag_model, ph.visc_model, ph.offshore_bc, solver],
+        np.array([sc.nx, sc.ny, runs, steps, ph.drag_model, ph.visc_model, ph.offshore_bc, solver, sediment],
                  dtype=np.int32).tofile(f)
-        np.array([sc.dy, sc.zs0, ph.dt, ph.Dmin, ph.alpha_f, ph.cD0, ph.nu0, ph.M, ph.ka, ph.z0, ph.alpha_r],
-                 dtype=np.float64).tofile(f)
+        np.array([sc.dy, sc.zs0, ph.dt, ph.Dmin, ph.alpha_f, ph.cD0, ph.nu0, ph.M, ph.ka, ph.z0, ph.alpha_r,
+                  bed_dt], dtype=np.float64).tofile(f)
         for a in [sc.x, sc.Z, sc.h] + [sc.forcing[k] for k in ("Sxx", "Sxy", "Syy", "Urms", "Dw", "D_r", "H")]:
             np.ascontiguousarray(a, dtype=np.float64).tofile(f)
 
 
-def read_output(path, nx, ny):
+SED_FIELDS = ("Alpha", "gamma", "Gamma", "qx", "qy", "dh_dt", "h", "Zb")
+
+
+def read_output(path, nx, ny, sediment=False):
     raw = np.fromfile(path, dtype=np.float64)
     out, off = {}, 0
-    for k in FIELDS:
+    for k in FIELDS + (SED_FIELDS if sediment else ()):
         rows = nx + 1 if k in ("Qx", "Qy", "U", "V") else nx
         out[k] = raw[off:off + rows * ny].reshape(rows, ny)
         off += rows * ny
@@ -56,3 +59,28 @@ def test_facade_matches_oracle(tmp_path, solver, exact):
     for k, r in res.items():
         assert r["nan"] == 0, k
         assert (r["exact"] if exact else r["rel"] <= 1e-10), (k, r)
+
+
+def test_facade_sediment_outer_loop(tmp_path):
+    """The C++ Sediment (morfo70_amd/host/Sediment.h) in the outer loop: Run, fluxes, bed
+    update, three times; the bed update's Zs reaches the next Run through the host mirror.
+    Bar: max-norm relative error <= 1e-10 (the coefficients' tanh)."""
+    import oracle
+    ph = S.Physics(drag_model=S.DRAG["constant"], visc_model=S.VISC["constant"], dt=0.25)
+    sc = nonuniform_beach(nx=50, ny=44, physics=ph, seed=22)
+    inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_input(inp, sc, runs=3, steps=5, solver=2, sediment=1, bed_dt=2.0)
+    print("[facade] sediment outer loop 50x44 3x(5 steps + bed)", flush=True)
+    subprocess.run([FACADE, str(inp), str(outp)], check=True, timeout=120)
+    got = read_output(outp, sc.nx, sc.ny, sediment=True)
+    o = oracle.OracleHydro(sc)
+    sd = oracle.OracleSediment(o, S.SedimentParams())
+    for _ in range(3):
+        o.run(5)
+        sd.fluxes()
+        sd.update_bottom(2.0)
+    ref = dict(o.get_state(), **{k: sd.get(k) for k in SED_FIELDS})
+    res = compare_states(got, ref, fields=FIELDS + SED_FIELDS)
+    for k, r in res.items():
+        assert r["nan"] == 0 and r["rel"] <= 1e-10, (k, r)
+    assert np.max(np.abs(got["dh_dt"])) > 0

@@ -424,3 +424,79 @@ def test_tangent_linear_model_matches_central_differences():
     sp, sm = final(cD0=c0 + e), final(cD0=c0 - e)
     fd = (sp["Zs"] - sm["Zs"]) / (2 * e)
     assert np.max(np.abs(fd - tanc["Zs"])) <= 1e-6 * np.max(np.abs(tanc["Zs"]))
+
+
+# ------------------------------------------------------------------------ sediment
+def _sed_case(D_rows, ny=8, forcing=None, sed=None):
+    sc = _flat_case(D_rows, ny=ny, forcing=forcing)
+    o = oracle.OracleHydro(sc)
+    o.run(1)
+    return sc, o, oracle.OracleSediment(o, sed or S.SedimentParams())
+
+
+def test_sediment_coefficients_at_exact_points():
+    """Stirring (src/Sediment.cpp:222-239) is alpha0_min + (max-min)/2 where its tanh
+    argument vanishes, at D = (D_alpha0_max + D_alpha0_min)/2; the downslope coefficient
+    (:194-220) is gamma0/(2 tan(phi)) at D = (Dmin + A_swash)/2, with tan(phi) = 0.6249."""
+    sp = S.SedimentParams()
+    Dmid_a = 0.5 * (sp.D_alpha0_max + sp.D_alpha0_min)
+    Dmid_g = 0.5 * (0.01 + sp.A_swash)
+    sc, o, sd = _sed_case([Dmid_a, Dmid_a, Dmid_g, Dmid_g, 2.0])
+    sd.fluxes()
+    A, g_ = sd.get("Alpha"), sd.get("gamma")
+    assert np.all(A[0] == sp.alpha0_min + 0.5 * (sp.alpha0_max - sp.alpha0_min))
+    np.testing.assert_allclose(g_[2], 0.5 * sp.gamma0 / 0.6249, rtol=1e-15)
+    # deep water: no stirring change with depth beyond D_alpha0_min (tanh saturates)
+    sc2, o2, sd2 = _sed_case([40.0, 50.0, 60.0])
+    sd2.fluxes()
+    np.testing.assert_allclose(sd2.get("Alpha"), sp.alpha0_min, atol=1e-12)
+
+
+def test_sediment_shoreline_and_running_max():
+    """Zs_sh takes Zs of the first dry row below the wet ones (the last row i whose rows
+    i-1 and i are not both dry, :264-283); maxDw is a running maximum over every call."""
+    nx, ny = 12, 6
+    D = [3.0, 2.5, 2.0, 1.5, 1.0, 0.5, 0.2, 0.005, 0.005, 0.005, 0.005, 0.005]
+    rng = np.random.default_rng(4)
+    Dw = rng.uniform(0.0, 50.0, (nx, ny))
+    sc, o, sd = _sed_case(D, ny=ny, forcing={"Dw": Dw})
+    sd.fluxes()
+    Zs = o.get_state()["Zs"]
+    k = 7                                              # first row at D = Dmin
+    np.testing.assert_array_equal(sd.get("Zs_sh"), Zs[k])
+    np.testing.assert_array_equal(sd.get("maxDw"), Dw.max(axis=0))
+    o.set_forcing(dict(sc.forcing, Dw=np.ascontiguousarray(0.5 * Dw)))
+    sd.fluxes()                                        # smaller Dw: the maximum stays
+    np.testing.assert_array_equal(sd.get("maxDw"), Dw.max(axis=0))
+
+
+def test_sediment_bed_update_budget_and_adams_bashforth():
+    """dh/dt = -div(q)/(1-p) (:105-151): summed with the cross-shore cell widths over rows
+    1..nx-1 the divergence telescopes to the offshore flux, sum_ij dx_x[i] dy dh_dt =
+    dy/(1-p) sum_j qx[0][j] (qx of the last row is never computed, so zero; y is
+    periodic). The update is Adams-Bashforth 2, h += dt/2 (3 dh_dt - dh_dt_prev), and moves
+    Zs by the same amount and Zb by its negative."""
+    sc = nonuniform_beach(nx=30, ny=16, seed=3)
+    o = oracle.OracleHydro(sc)
+    o.run(20)
+    sp = S.SedimentParams()
+    sd = oracle.OracleSediment(o, sp)
+    dxx = oracle.grid_metrics(sc.x)["dx_x"]
+    dt = 2.0
+    prev = np.zeros((sc.nx, sc.ny))
+    for it in range(3):
+        sd.fluxes()
+        h0, zb0, zs0 = sd.get("h"), sd.get("Zb"), o.get_state()["Zs"]
+        sd.update_bottom(dt)
+        dhdt, qx = sd.get("dh_dt"), sd.get("qx")
+        assert np.all(qx[-1] == 0.0)
+        lhs = np.sum(dxx[1:, None] * sc.dy * dhdt[1:])
+        rhs = sc.dy / (1 - sp.p) * np.sum(qx[0])
+        assert abs(lhs - rhs) <= 1e-12 * max(abs(rhs), np.max(np.abs(dhdt)) * sc.dy * np.sum(dxx))
+        incr = 0.5 * dt * (3 * dhdt[1:] - prev[1:])
+        np.testing.assert_allclose(sd.get("h")[1:] - h0[1:], incr, rtol=1e-9, atol=1e-18)
+        np.testing.assert_allclose(o.get_state()["Zs"][1:] - zs0[1:], incr, rtol=1e-9, atol=1e-15)
+        np.testing.assert_allclose(sd.get("Zb")[1:] - zb0[1:], -incr, rtol=1e-9, atol=1e-15)
+        assert np.all(sd.get("h")[0] == h0[0])       # row 0 is not updated
+        prev = dhdt
+        o.run(5)
