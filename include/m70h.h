@@ -253,6 +253,61 @@ int m70s_get(m70s *s, double *Alpha, double *gamma, double *Gamma, double *qx, d
 int m70s_set_bed(m70s *s, const double *h, const double *Zb);
 
 /* ------------------------------------------------------------------------------------
+ * Wave and rollers driver: class Wave (include/Wave.h:16-128, src/Wave.cpp) with
+ * class WaveRow (include/WaveRow.h, src/WaveRow.cpp, src/WaveDir.cpp, src/WaveEnergy.cpp,
+ * src/WaveParametrizations.cpp, src/WaveRollers.cpp). It produces the hydro forcing
+ * (Sxx, Sxy, Syy, Urms, Dw, D_r, H) every outer step (src/Morfo70.cpp:141).
+ * Semantics differ from the reference's code in two places where that code cannot be
+ * what was meant: the two row buffers swap after every marching step (the brace-less
+ * `else` at src/Wave.cpp:158-163 aliases them after a mesh row), and bottom friction is
+ * evaluated per node being solved (src/WaveParametrizations.cpp:357-378 index the first
+ * nNodesToSolve nodes). With one OpenMP thread and friction off (the defaults) the
+ * reference computes the same thing.
+ * ------------------------------------------------------------------------------------ */
+enum { M70W_BREAK_CT = 1, M70W_BREAK_TG = 2, M70W_BREAK_REGULAR = 3 };   /* include/OptionsParametersConstants.h:31-36 */
+enum { M70W_FRIC_CONSTANT = 0, M70W_FRIC_SOULSBY97 = 1 };                /* :37-40 */
+
+/* Options::Wave::*, Options::Rollers::CalculateRollers, Parameters::Wave::*,
+ * Parameters::Rollers::*, Parameters::Bottom::z0 (include/OptionsParametersConstants.h).
+ * theta_max in radians. BreakingModel None is rejected (the reference calls through a
+ * NULL member pointer for it, src/WaveRow.cpp:68-80). */
+typedef struct m70w_params {
+    int breaking_model, friction_model, bottom_friction, rollers, max_iterations;
+    double tolerance, B3, gamma_b, m_mon, n_mon, theta_max, fw0, z0, sinBeta_r, alpha_r;
+} m70w_params;
+
+typedef struct m70w m70w;
+
+/* Replaces Wave::Wave (src/Wave.cpp:13-56): grid x[nx] (offshore first), ny, dy. */
+int m70w_create(int nx, int ny, const double *x, double dy, const m70w_params *p, m70w **out);
+void m70w_destroy(m70w *w);
+const char *m70w_last_error(const m70w *w);
+int m70w_set_stream(m70w *w, void *hip_stream);
+/* bathy->D, the depth the field is computed on (host, nx*ny). */
+int m70w_set_depth(m70w *w, const double *D);
+/* Replaces Wave::setUV (src/Wave.cpp:293-302): currents (host, nx*ny) and their
+ * projection on the stored wave direction. */
+int m70w_set_uv(m70w *w, const double *U, const double *V);
+/* Replaces Wave::calculateWaveField (src/Wave.cpp:80-172). */
+int m70w_calculate(m70w *w, double hOff, double thetaOff, double T);
+int m70w_calculate_async(m70w *w, double hOff, double thetaOff, double T);
+/* A public array of Wave (include/Wave.h:33-53) by name: K (log of the wave number),
+ * Theta, Cg, H, E, Urms, Sxx_w, Sxy_w, Syy_w, Dw, E_r, D_r, Sxx_r, Sxy_r, Syy_r, Sxx, Sxy,
+ * Syy, CosTheta, SinTheta, U_w. out: host, nx*ny doubles. */
+int m70w_get(m70w *w, const char *name, double *out);
+/* Marching steps (mesh rows plus intermediate rows) of the last field. */
+int m70w_steps(m70w *w, long *steps);
+/* Device coupling with a hydro handle of the same grid (no counterpart in the reference,
+ * whose objects share host arrays): the field is computed on the hydro's depth D (which
+ * Hydro::Run updates, src/Hydro.cpp:267) and on the hydro's stream. */
+int m70w_couple_hydro(m70w *w, m70h *h);
+/* Wave::setUV(hydro->U_c, hydro->V_c) (src/Morfo70.cpp:151-154) on the device. */
+int m70w_set_uv_from_hydro(m70w *w);
+/* The forcing Hydro::Run reads (Sxx, Sxy, Syy, Urms, Dw, D_r, H) copied device to device
+ * into the coupled hydro handle, in place of m70h_set_forcing. */
+int m70w_push_forcing(m70w *w);
+
+/* ------------------------------------------------------------------------------------
  * Adjoint of Run. No counterpart in the reference, which has no derivatives: these
  * entry points serve callers that fit bathymetry or friction to observations (gradient
  * of a loss of the final state with respect to the initial state and the drag and

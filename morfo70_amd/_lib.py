@@ -68,6 +68,19 @@ SIGNATURES = {
     "m70s_set_fluxes": (ctypes.c_int, [_vp, _dp, _dp]),
     "m70s_get": (ctypes.c_int, [_vp] + [_dp] * 8),
     "m70s_set_bed": (ctypes.c_int, [_vp, _dp, _dp]),
+    "m70w_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, _vp, ctypes.POINTER(_vp)]),
+    "m70w_destroy": (None, [_vp]),
+    "m70w_last_error": (ctypes.c_char_p, [_vp]),
+    "m70w_set_stream": (ctypes.c_int, [_vp, _vp]),
+    "m70w_set_depth": (ctypes.c_int, [_vp, _dp]),
+    "m70w_set_uv": (ctypes.c_int, [_vp, _dp, _dp]),
+    "m70w_calculate": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "m70w_calculate_async": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "m70w_get": (ctypes.c_int, [_vp, ctypes.c_char_p, _dp]),
+    "m70w_steps": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_long)]),
+    "m70w_couple_hydro": (ctypes.c_int, [_vp, _vp]),
+    "m70w_set_uv_from_hydro": (ctypes.c_int, [_vp]),
+    "m70w_push_forcing": (ctypes.c_int, [_vp]),
     "m70h_adjoint_forward": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "m70h_adjoint_reverse": (ctypes.c_int, [_vp, _vp, _vp, _dp]),
     "m70h_adjoint_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),

@@ -14,6 +14,7 @@
 #include "hydro_device.h"
 #include "hydro_kernels.h"
 #include "sediment.h"
+#include "wave.h"
 
 using namespace m70;
 
@@ -1627,6 +1628,266 @@ int m70s_get(m70s *s, double *Alpha, double *gamma, double *Gamma, double *qx, d
     for (int q = 0; q < 8; ++q)
         if (dst[q]) SHIPCHK(s, hipMemcpyAsync(dst[q], src[q], nb, hipMemcpyDeviceToHost, s->h->stream));
     SHIPCHK(s, hipStreamSynchronize(s->h->stream));
+    return M70H_OK;
+}
+
+}  // extern "C"
+
+// =====================================================================================
+// Wave and rollers driver (class Wave, src/Wave.cpp; class WaveRow)
+// =====================================================================================
+struct m70w {
+    int nx = 0, ny = 0;
+    double dy = 0.0;
+    m70w_params p{};
+    std::string err;
+    std::vector<void *> allocs;
+    hipStream_t own_stream = nullptr, stream = nullptr;
+    WaveArgs a{};
+    double *D = nullptr, *U = nullptr, *V = nullptr, *U_w = nullptr;   // owned inputs
+    long *status = nullptr;
+    long last_steps = -1;
+    bool pending = false;
+    m70h *hydro = nullptr;
+};
+
+namespace {
+
+int wfail(m70w *w, int code, const std::string &msg)
+{
+    if (w) w->err = msg; else g_create_error = msg;
+    return code;
+}
+
+#define WHIPCHK(w, expr)                                                                       \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return wfail(w, M70H_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+hipError_t walloc(m70w *w, double **p, size_t n)
+{
+    void *m = nullptr;
+    hipError_t e = hipMalloc(&m, n * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(m, 0, n * sizeof(double));
+    if (e != hipSuccess) return e;
+    w->allocs.push_back(m);
+    *p = (double *)m;
+    return hipSuccess;
+}
+
+// Waits for the last field and reads its step count and status.
+int wave_finish(m70w *w)
+{
+    if (!w->pending) return M70H_OK;
+    long st[2] = {0, 0};
+    WHIPCHK(w, hipMemcpyAsync(st, w->status, sizeof(st), hipMemcpyDeviceToHost, w->stream));
+    WHIPCHK(w, hipStreamSynchronize(w->stream));
+    w->pending = false;
+    w->last_steps = st[0];
+    if (st[1]) return wfail(w, M70H_ENONFINITE, "wave field: the march took more than 1000*nx steps");
+    return M70H_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *m70w_last_error(const m70w *w) { return w ? w->err.c_str() : g_create_error.c_str(); }
+
+int m70w_create(int nx, int ny, const double *x, double dy, const m70w_params *p, m70w **out)
+{
+    if (!out) return wfail(nullptr, M70H_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (nx < 2 || ny < 3) return wfail(nullptr, M70H_EINVAL, "need nx >= 2 and ny >= 3");
+    if (!x || !p || !(dy > 0)) return wfail(nullptr, M70H_EINVAL, "x and p must be non-NULL and dy > 0");
+    if (p->breaking_model < M70W_BREAK_CT || p->breaking_model > M70W_BREAK_REGULAR)
+        return wfail(nullptr, M70H_EINVAL, "breaking_model must be ChurchThornton, ThorntonGuza or RegularWaves");
+    if (p->friction_model != M70W_FRIC_CONSTANT && p->friction_model != M70W_FRIC_SOULSBY97)
+        return wfail(nullptr, M70H_EINVAL, "friction_model must be Constant or Soulsby97");
+    for (int i = 0; i + 1 < nx; ++i)
+        if (!(x[i + 1] > x[i])) return wfail(nullptr, M70H_EINVAL, "x must increase");
+    m70w *w = new m70w();
+    w->nx = nx;
+    w->ny = ny;
+    w->dy = dy;
+    w->p = *p;
+    auto bail = [&](hipError_t e) {
+        g_create_error = std::string("m70w_create: ") + hipGetErrorString(e);
+        m70w_destroy(w);
+        return M70H_EHIP;
+    };
+    hipError_t e = hipStreamCreate(&w->own_stream);
+    if (e != hipSuccess) return bail(e);
+    w->stream = w->own_stream;
+    const size_t F = (size_t)nx * ny;
+    double *xd = nullptr, *dxd = nullptr;
+    if ((e = walloc(w, &xd, nx)) != hipSuccess || (e = walloc(w, &dxd, nx)) != hipSuccess) return bail(e);
+    std::vector<double> dx(nx, 0.0);
+    for (int i = 0; i + 1 < nx; ++i) dx[i] = x[i + 1] - x[i];   // src/Grid.cpp:32-34
+    if ((e = hipMemcpy(xd, x, nx * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) return bail(e);
+    if ((e = hipMemcpy(dxd, dx.data(), nx * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) return bail(e);
+    for (double **q : {&w->D, &w->U, &w->V, &w->U_w})
+        if ((e = walloc(w, q, F)) != hipSuccess) return bail(e);
+    double **fld = &w->a.f.K;
+    for (int q = 0; q < WFIELD_ARRAYS; ++q)
+        if ((e = walloc(w, &fld[q], F)) != hipSuccess) return bail(e);
+    for (int r = 0; r < 2; ++r) {
+        double *buf = nullptr;
+        if ((e = walloc(w, &buf, (size_t)WROW_ARRAYS * ny)) != hipSuccess) return bail(e);
+        double **arr = &w->a.rows[r].D;
+        for (int q = 0; q < WROW_ARRAYS; ++q) arr[q] = buf + (size_t)q * ny;
+    }
+    void *stm = nullptr;
+    if ((e = hipMalloc(&stm, 2 * sizeof(long))) != hipSuccess) return bail(e);
+    w->allocs.push_back(stm);
+    w->status = (long *)stm;
+    WaveArgs &a = w->a;
+    a.nx = nx;
+    a.ny = ny;
+    a.dy = dy;
+    a.inv_2dy = 0.5 / dy;   // WaveRow::WaveRow (src/WaveRow.cpp:63)
+    a.x = xd;
+    a.dx = dxd;
+    a.p = *p;
+    a.D = w->D;
+    a.U = w->U;
+    a.V = w->V;
+    a.U_w = w->U_w;
+    a.status = w->status;
+    *out = w;
+    return M70H_OK;
+}
+
+void m70w_destroy(m70w *w)
+{
+    if (!w) return;
+    if (w->stream) (void)hipStreamSynchronize(w->stream);
+    for (void *q : w->allocs) (void)hipFree(q);
+    if (w->own_stream) (void)hipStreamDestroy(w->own_stream);
+    delete w;
+}
+
+int m70w_set_stream(m70w *w, void *hip_stream)
+{
+    if (!w) return M70H_EINVAL;
+    int rc = wave_finish(w);
+    if (rc) return rc;
+    WHIPCHK(w, hipStreamSynchronize(w->stream));
+    w->stream = hip_stream ? (hipStream_t)hip_stream : w->own_stream;
+    return M70H_OK;
+}
+
+int m70w_set_depth(m70w *w, const double *D)
+{
+    if (!w || !D) return w ? wfail(w, M70H_EINVAL, "D is NULL") : M70H_EINVAL;
+    int rc = wave_finish(w);
+    if (rc) return rc;
+    WHIPCHK(w, hipMemcpyAsync(w->D, D, (size_t)w->nx * w->ny * sizeof(double), hipMemcpyHostToDevice, w->stream));
+    WHIPCHK(w, hipStreamSynchronize(w->stream));
+    w->a.D = w->D;
+    return M70H_OK;
+}
+
+int m70w_set_uv(m70w *w, const double *U, const double *V)
+{
+    if (!w || !U || !V) return w ? wfail(w, M70H_EINVAL, "U and V are required") : M70H_EINVAL;
+    int rc = wave_finish(w);
+    if (rc) return rc;
+    const long F = (long)w->nx * w->ny;
+    double *tU = nullptr, *tV = nullptr;
+    WHIPCHK(w, hipMalloc((void **)&tU, (size_t)F * sizeof(double)));
+    hipError_t e = hipMalloc((void **)&tV, (size_t)F * sizeof(double));
+    if (e == hipSuccess) e = hipMemcpyAsync(tU, U, (size_t)F * sizeof(double), hipMemcpyHostToDevice, w->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(tV, V, (size_t)F * sizeof(double), hipMemcpyHostToDevice, w->stream);
+    if (e == hipSuccess) e = launch_wave_set_uv(w->stream, w->a, w->U, w->V, w->U_w, tU, tV, F);
+    if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+    (void)hipFree(tU);
+    if (tV) (void)hipFree(tV);
+    if (e != hipSuccess) return wfail(w, M70H_EHIP, std::string("m70w_set_uv: ") + hipGetErrorString(e));
+    return M70H_OK;
+}
+
+int m70w_calculate_async(m70w *w, double hOff, double thetaOff, double T)
+{
+    if (!w) return M70H_EINVAL;
+    if (!(T > 0) || !(hOff > 0)) return wfail(w, M70H_EINVAL, "need T > 0 and hOff > 0");
+    int rc = wave_finish(w);
+    if (rc) return rc;
+    if (w->hydro) w->a.D = w->hydro->s.D;
+    WHIPCHK(w, launch_wave_field(w->stream, w->a, hOff, thetaOff, T));
+    w->pending = true;
+    return M70H_OK;
+}
+
+int m70w_calculate(m70w *w, double hOff, double thetaOff, double T)
+{
+    int rc = m70w_calculate_async(w, hOff, thetaOff, T);
+    if (rc) return rc;
+    return wave_finish(w);
+}
+
+int m70w_get(m70w *w, const char *name, double *out)
+{
+    if (!w || !name || !out) return w ? wfail(w, M70H_EINVAL, "name and out are required") : M70H_EINVAL;
+    int rc = wave_finish(w);
+    if (rc) return rc;
+    static const char *const names[WFIELD_ARRAYS] = {"K", "Theta", "Cg", "H", "E", "Urms", "Sxx_w", "Sxy_w",
+                                                     "Syy_w", "Dw", "E_r", "D_r", "Sxx_r", "Sxy_r", "Syy_r", "Sxx",
+                                                     "Sxy", "Syy", "CosTheta", "SinTheta"};
+    const double *src = nullptr;
+    for (int q = 0; q < WFIELD_ARRAYS; ++q)
+        if (std::strcmp(name, names[q]) == 0) src = (&w->a.f.K)[q];
+    if (std::strcmp(name, "U_w") == 0) src = w->U_w;
+    if (!src) return wfail(w, M70H_EINVAL, std::string("unknown wave field ") + name);
+    WHIPCHK(w, hipMemcpyAsync(out, src, (size_t)w->nx * w->ny * sizeof(double), hipMemcpyDeviceToHost, w->stream));
+    WHIPCHK(w, hipStreamSynchronize(w->stream));
+    return M70H_OK;
+}
+
+int m70w_steps(m70w *w, long *steps)
+{
+    if (!w || !steps) return M70H_EINVAL;
+    int rc = wave_finish(w);
+    if (rc) return rc;
+    *steps = w->last_steps;
+    return M70H_OK;
+}
+
+int m70w_couple_hydro(m70w *w, m70h *h)
+{
+    if (!w || !h) return w ? wfail(w, M70H_EINVAL, "hydro is NULL") : M70H_EINVAL;
+    if (h->nx != w->nx || h->ny != w->ny) return wfail(w, M70H_EINVAL, "the hydro handle's grid differs");
+    int rc = wave_finish(w);
+    if (rc) return rc;
+    WHIPCHK(w, hipStreamSynchronize(w->stream));
+    w->hydro = h;
+    w->stream = h->stream;
+    w->a.D = h->s.D;
+    return M70H_OK;
+}
+
+int m70w_set_uv_from_hydro(m70w *w)
+{
+    if (!w) return M70H_EINVAL;
+    if (!w->hydro) return wfail(w, M70H_ESTATE, "no hydro handle coupled");
+    const long F = (long)w->nx * w->ny;
+    WHIPCHK(w, launch_wave_set_uv(w->stream, w->a, w->U, w->V, w->U_w, w->hydro->s.Uc, w->hydro->s.Vc, F));
+    return M70H_OK;
+}
+
+int m70w_push_forcing(m70w *w)
+{
+    if (!w) return M70H_EINVAL;
+    m70h *h = w->hydro;
+    if (!h) return wfail(w, M70H_ESTATE, "no hydro handle coupled");
+    const WField &f = w->a.f;
+    const double *src[7] = {f.Sxx, f.Sxy, f.Syy, f.Urms, f.Dw, f.D_r, f.H};
+    for (int q = 0; q < 7; ++q)
+        WHIPCHK(w, hipMemcpyAsync(h->fw_buf[q], src[q], (size_t)h->F * sizeof(double), hipMemcpyDeviceToDevice,
+                                  w->stream));
+    h->forcing_set = true;
+    h->run_gen++;
     return M70H_OK;
 }
 

@@ -86,6 +86,46 @@ class SedimentParams:
                 self.Gamma0, self.A_swash)
 
 
+BREAKING = {"ChurchThornton": 1, "ThorntonGuza": 2, "RegularWaves": 3}
+FRICTION = {"Constant": 0, "Soulsby97": 1}
+
+
+@dataclasses.dataclass
+class WaveParams:
+    """Options::Wave::*, Options::Rollers::CalculateRollers, Parameters::Wave::*,
+    Parameters::Rollers::* and Parameters::Bottom::z0 (include/OptionsParametersConstants.h),
+    with the compiled-in defaults of src/OptionsParametersConstants.cpp:14-58. theta_max is
+    in radians as the reference stores it (its .ini reader converts degrees,
+    src/MorfoIniFile.cpp:125-127; the compiled-in 85 is taken as radians)."""
+
+    breaking_model: int = 1
+    friction_model: int = 0
+    bottom_friction: int = 0
+    rollers: int = 0
+    max_iterations: int = 10
+    tolerance: float = 1e-10
+    B3: float = 2.0
+    gamma_b: float = 0.475
+    m_mon: float = 20.0
+    n_mon: float = 30.0
+    theta_max: float = 85.0
+    fw0: float = 0.01
+    z0: float = 0.001
+    sinBeta_r: float = 0.05
+    alpha_r: float = 1.0
+
+    @classmethod
+    def base_case(cls) -> "WaveParams":
+        """The wave and roller settings of the reference's test/input/base_case.ini."""
+        return cls(breaking_model=BREAKING["ThorntonGuza"], B3=1.0, m_mon=10.0, n_mon=10.0,
+                   theta_max=86.0 * math.pi / 180.0, z0=0.01)
+
+    def as_tuple(self):
+        return (self.breaking_model, self.friction_model, self.bottom_friction, self.rollers, self.max_iterations,
+                self.tolerance, self.B3, self.gamma_b, self.m_mon, self.n_mon, self.theta_max, self.fw0, self.z0,
+                self.sinBeta_r, self.alpha_r)
+
+
 @dataclasses.dataclass
 class Scenario:
     name: str

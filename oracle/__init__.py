@@ -358,3 +358,90 @@ class OracleSediment:
         out = np.zeros(n)
         lib().orc_sed_get(self.s, name.encode(), ptr(out))
         return out.reshape(-1, self.hy.ny) if n != self.hy.ny else out
+
+
+# ----------------------------------------------------------------------------- wave
+class _WaveParams(ctypes.Structure):
+    _fields_ = [("breaking_model", ctypes.c_int), ("friction_model", ctypes.c_int), ("bottom_friction", ctypes.c_int),
+                ("rollers", ctypes.c_int), ("max_iterations", ctypes.c_int), ("tolerance", ctypes.c_double)] + \
+               [(n, ctypes.c_double) for n in ("B3", "gamma_b", "m_mon", "n_mon", "theta_max", "fw0", "z0",
+                                               "sinBeta_r", "alpha_r")]
+
+
+WAVE_FIELDS = ("K", "Theta", "Cg", "H", "E", "Urms", "Sxx_w", "Sxy_w", "Syy_w", "Dw", "E_r", "D_r", "Sxx_r", "Sxy_r",
+               "Syy_r", "Sxx", "Sxy", "Syy", "CosTheta", "SinTheta", "U_w")
+
+
+def _wave_lib():
+    L = lib()
+    if not hasattr(L, "_wave_ready"):
+        L.orc_wave_create.restype = ctypes.c_void_p
+        L.orc_wave_create.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.POINTER(_WaveParams)]
+        L.orc_wave_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_wave_set_depth.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_wave_set_uv.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_wave_calculate.restype = ctypes.c_long
+        L.orc_wave_calculate.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.orc_wave_get.restype = ctypes.c_long
+        L.orc_wave_get.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _dp]
+        L.orc_wave_dispersion.argtypes = [ctypes.c_double] * 4 + [_dp] * 5
+        L.orc_wave_breaking.argtypes = [ctypes.POINTER(_WaveParams), ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, _dp, _dp]
+        L._wave_ready = True
+    return L
+
+
+class OracleWave:
+    """Wave and rollers driver of the oracle (class Wave, src/Wave.cpp; class WaveRow)."""
+
+    def __init__(self, x, ny: int, dy: float, params):
+        L = _wave_lib()
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.nx, self.ny = int(self.x.shape[0]), int(ny)
+        self._p = _WaveParams(*params.as_tuple())
+        self.w = L.orc_wave_create(self.nx, self.ny, ptr(self.x), float(dy), ctypes.byref(self._p))
+        if not self.w:
+            raise ValueError("orc_wave_create failed")
+
+    def __del__(self):
+        if getattr(self, "w", None):
+            _wave_lib().orc_wave_destroy(self.w)
+            self.w = None
+
+    def set_depth(self, D):
+        D = np.ascontiguousarray(D, dtype=np.float64).reshape(self.nx, self.ny)
+        _wave_lib().orc_wave_set_depth(self.w, ptr(D))
+
+    def set_uv(self, U, V):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.nx, self.ny)
+        V = np.ascontiguousarray(V, dtype=np.float64).reshape(self.nx, self.ny)
+        _wave_lib().orc_wave_set_uv(self.w, ptr(U), ptr(V))
+
+    def calculate(self, hOff: float, thetaOff: float, T: float) -> int:
+        n = _wave_lib().orc_wave_calculate(self.w, float(hOff), float(thetaOff), float(T))
+        if n < 0:
+            raise ValueError(f"orc_wave_calculate: {n}")
+        return int(n)
+
+    def get(self, name: str) -> np.ndarray:
+        out = np.zeros((self.nx, self.ny))
+        if _wave_lib().orc_wave_get(self.w, name.encode(), ptr(out)) < 0:
+            raise KeyError(name)
+        return out
+
+    def fields(self) -> dict:
+        return {k: self.get(k) for k in WAVE_FIELDS}
+
+
+def wave_dispersion(D: float, U_w: float, w: float, z0: float = 0.001) -> dict:
+    out = [ctypes.c_double() for _ in range(5)]
+    _wave_lib().orc_wave_dispersion(D, U_w, w, z0, *[ctypes.cast(ctypes.byref(o), _dp) for o in out])
+    return dict(zip(("lK", "sigma", "C", "Cg", "Urms_H"), (o.value for o in out)))
+
+
+def wave_breaking(params, H: float, sigma: float, D: float):
+    p = _WaveParams(*params.as_tuple())
+    dw, ddw = ctypes.c_double(), ctypes.c_double()
+    _wave_lib().orc_wave_breaking(ctypes.byref(p), H, sigma, D, ctypes.cast(ctypes.byref(dw), _dp),
+                                  ctypes.cast(ctypes.byref(ddw), _dp))
+    return dw.value, ddw.value
