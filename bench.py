@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -82,7 +83,7 @@ def parse():
     ap.add_argument("--tile-cols", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--workload", choices=["tide2048x4096", "batch64", "adjoint512x1024", "inversion16",
-                                           "sediment2048x4096"],
+                                           "sediment2048x4096", "wave2048x4096"],
                     default="tide2048x4096")
     ap.add_argument("--gd-iters", type=int, default=20)
     ap.add_argument("--slots", type=int, default=0,
@@ -226,6 +227,89 @@ def bench_sediment(args):
         rec["cpu_baseline"] = {"value": cells * n / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
                                "sample": f"{sc.nx}x{sc.ny} grid, {n} x (fluxes + bed update) after one untimed, "
                                          f"{threads} OpenMP threads, {dt:.2f} s"}
+        rec["gpu_over_cpu"] = value / rec["cpu_baseline"]["value"]
+    else:
+        rec["cpu_baseline"] = None
+    print(json.dumps(rec), flush=True)
+
+
+WAVE_BYTES_PER_CELL = 8 + 20 * 8   # depth in; the 20 public arrays of Wave out
+
+
+def bench_wave(args):
+    """Wave and rollers field (Wave::calculateWaveField, src/Wave.cpp:80-172) on the
+    2048 x 4096 tide beach's still-water depth: 1 m, 8 s, 10 degrees offshore, the reference's
+    base_case.ini wave settings (Thornton-Guza breaking). A "step" is one whole field."""
+    import torch
+    from morfo70_amd import scenarios as S
+    from morfo70_amd.wave import Wave
+
+    sc = S.seeded_beach(args.nx, args.ny, 2.0, 2.0, args.seed, 2, tide=True,
+                        name=f"beach_tide_{args.nx}x{args.ny}_s{args.seed}")
+    D = np.ascontiguousarray(sc.initial_depth())
+    p = S.WaveParams.base_case()
+    hOff, theta, T = 1.0, math.radians(10.0), 8.0
+
+    class _G:
+        x, nx, ny, dy = sc.x, sc.nx, sc.ny, sc.dy
+
+    class _B:
+        pass
+    _B.D = D
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)
+    w = Wave(_G, _B, p)
+    w.set_stream(stream.cuda_stream)
+    for _ in range(args.warmup):
+        w.calculateWaveField(hOff, theta, T, pull=False)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        w._chk(w._L.m70w_calculate_async(w.handle, hOff, theta, T))
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    steps = w.steps()
+    w.pull(("H", "Sxx"))
+    finite = bool(np.all(np.isfinite(w.H)) and np.all(np.isfinite(w.Sxx)))
+    w.close()
+    cells = sc.nx * sc.ny
+    value = cells / (ms * 1e-3)
+    alg = WAVE_BYTES_PER_CELL * cells
+    achieved = alg / (ms * 1e-3) / 1e9
+    rec = {
+        "metric": "wave and rollers field cell-updates/s (Wave::calculateWaveField)",
+        "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded bar-trough beach still-water depth; 1 m, 8 s, 10 deg offshore waves)",
+        "config": {"workload": f"wave{sc.nx}x{sc.ny}", "nx": sc.nx, "ny": sc.ny, "dx": 2.0, "dy": 2.0,
+                   "breaking": "ThorntonGuza", "marching_steps": steps,
+                   "timed": f"{args.steps} whole fields, one stream"},
+        "roofline": {"bound": "hbm", "kernel": "wave_field", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": ms,
+                     "alg_bytes_per_launch": alg,
+                     "note": "a serial march over the rows, one launch per marching step over ~70 workgroups; "
+                             "a step is bound by the latency of each node's Newton-Raphson chains "
+                             "(dispersion, energy), not by HBM"},
+        "finite": finite,
+    }
+    if not args.no_cpu:
+        import oracle
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        oracle.lib().orc_set_threads(threads)
+        o = oracle.OracleWave(sc.x, sc.ny, sc.dy, p)
+        o.set_depth(D)
+        t0 = time.perf_counter()
+        n = max(1, args.cpu_steps or 3)
+        for _ in range(n):
+            o.calculate(hOff, theta, T)
+        dt = (time.perf_counter() - t0) / n
+        del o
+        rec["cpu_baseline"] = {"value": cells / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+                               "sample": f"{sc.nx}x{sc.ny} field, {n} fields, OpenMP over the nodes of a row "
+                                         f"(up to {threads} threads), {dt:.2f} s per field"}
         rec["gpu_over_cpu"] = value / rec["cpu_baseline"]["value"]
     else:
         rec["cpu_baseline"] = None
@@ -424,6 +508,11 @@ def main():
         if args.gpus != 1:
             sys.exit(2)
         bench_adjoint(args)
+        return
+    if args.workload == "wave2048x4096":
+        if args.gpus != 1:
+            sys.exit(2)
+        bench_wave(args)
         return
     if args.workload == "sediment2048x4096":
         if args.gpus != 1:

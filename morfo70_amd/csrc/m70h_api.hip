@@ -1645,7 +1645,9 @@ struct m70w {
     hipStream_t own_stream = nullptr, stream = nullptr;
     WaveArgs a{};
     double *D = nullptr, *U = nullptr, *V = nullptr, *U_w = nullptr;   // owned inputs
-    long *status = nullptr;
+    WaveCtlBufs cb{};
+    double T = 0.0;      // wave period of the pending field
+    int next_s = 0;      // marching steps launched for the pending field
     long last_steps = -1;
     bool pending = false;
     m70h *hydro = nullptr;
@@ -1676,17 +1678,29 @@ hipError_t walloc(m70w *w, double **p, size_t n)
     return hipSuccess;
 }
 
-// Waits for the last field and reads its step count and status.
+// Completes the pending field: reads the marching state after the last launched step
+// and launches more steps (intermediate rows) until the last mesh row is reached.
 int wave_finish(m70w *w)
 {
     if (!w->pending) return M70H_OK;
-    long st[2] = {0, 0};
-    WHIPCHK(w, hipMemcpyAsync(st, w->status, sizeof(st), hipMemcpyDeviceToHost, w->stream));
-    WHIPCHK(w, hipStreamSynchronize(w->stream));
-    w->pending = false;
-    w->last_steps = st[0];
-    if (st[1]) return wfail(w, M70H_ENONFINITE, "wave field: the march took more than 1000*nx steps");
-    return M70H_OK;
+    const long limit = 1000L * w->nx + 1000 + 64;
+    for (;;) {
+        WaveCtl c{};
+        WHIPCHK(w, hipMemcpyAsync(&c, w->cb.ctl + (w->next_s & 1), sizeof(c), hipMemcpyDeviceToHost, w->stream));
+        WHIPCHK(w, hipStreamSynchronize(w->stream));
+        if (c.done) {
+            w->pending = false;
+            w->last_steps = c.steps;
+            if (c.err) return wfail(w, M70H_ENONFINITE, "wave field: the march took more than 1000*nx steps");
+            return M70H_OK;
+        }
+        if (w->next_s > limit) {
+            w->pending = false;
+            return wfail(w, M70H_ENONFINITE, "wave field: the march did not finish");
+        }
+        WHIPCHK(w, launch_wave_steps(w->stream, w->a, w->T, w->next_s, 64, w->cb));
+        w->next_s += 64;
+    }
 }
 
 }  // namespace
@@ -1738,10 +1752,13 @@ int m70w_create(int nx, int ny, const double *x, double dy, const m70w_params *p
         double **arr = &w->a.rows[r].D;
         for (int q = 0; q < WROW_ARRAYS; ++q) arr[q] = buf + (size_t)q * ny;
     }
-    void *stm = nullptr;
-    if ((e = hipMalloc(&stm, 2 * sizeof(long))) != hipSuccess) return bail(e);
-    w->allocs.push_back(stm);
-    w->status = (long *)stm;
+    void *cbm = nullptr;
+    const size_t cb_bytes = 2 * sizeof(WaveCtl) + 3 * sizeof(unsigned long long) + 3 * sizeof(int);
+    if ((e = hipMalloc(&cbm, cb_bytes)) != hipSuccess) return bail(e);
+    w->allocs.push_back(cbm);
+    w->cb.ctl = (WaveCtl *)cbm;
+    w->cb.acc = (unsigned long long *)((char *)cbm + 2 * sizeof(WaveCtl));
+    w->cb.anyv = (int *)((char *)w->cb.acc + 3 * sizeof(unsigned long long));
     WaveArgs &a = w->a;
     a.nx = nx;
     a.ny = ny;
@@ -1754,7 +1771,6 @@ int m70w_create(int nx, int ny, const double *x, double dy, const m70w_params *p
     a.U = w->U;
     a.V = w->V;
     a.U_w = w->U_w;
-    a.status = w->status;
     *out = w;
     return M70H_OK;
 }
@@ -1815,7 +1831,15 @@ int m70w_calculate_async(m70w *w, double hOff, double thetaOff, double T)
     int rc = wave_finish(w);
     if (rc) return rc;
     if (w->hydro) w->a.D = w->hydro->s.D;
-    WHIPCHK(w, launch_wave_field(w->stream, w->a, hOff, thetaOff, T));
+    // both rows start from zero, as the reference builds them afresh for every field
+    for (int r = 0; r < 2; ++r)
+        WHIPCHK(w, hipMemsetAsync(w->a.rows[r].D, 0, (size_t)WROW_ARRAYS * w->ny * sizeof(double), w->stream));
+    WHIPCHK(w, hipMemsetAsync(w->cb.acc, 0, 3 * sizeof(unsigned long long) + 3 * sizeof(int), w->stream));
+    w->T = T;
+    WHIPCHK(w, launch_wave_first(w->stream, w->a, hOff, thetaOff, T, w->cb));
+    // every field has at least nx-1 steps (one per mesh row); wave_finish adds the rest
+    WHIPCHK(w, launch_wave_steps(w->stream, w->a, T, 0, w->nx - 1, w->cb));
+    w->next_s = w->nx - 1;
     w->pending = true;
     return M70H_OK;
 }
@@ -1881,6 +1905,8 @@ int m70w_push_forcing(m70w *w)
     if (!w) return M70H_EINVAL;
     m70h *h = w->hydro;
     if (!h) return wfail(w, M70H_ESTATE, "no hydro handle coupled");
+    int rc = wave_finish(w);   // a field with intermediate rows needs more steps than nx-1
+    if (rc) return rc;
     const WField &f = w->a.f;
     const double *src[7] = {f.Sxx, f.Sxy, f.Syy, f.Urms, f.Dw, f.D_r, f.H};
     for (int q = 0; q < 7; ++q)

@@ -30,12 +30,35 @@ struct WaveArgs {
     const double *D, *U, *V, *U_w;     // depth and currents, nx*ny
     WField f;
     WRow rows[2];
-    long *status;                      // [0] marching steps, [1] 0 or an error code
 };
 
-// Wave::calculateWaveField (src/Wave.cpp:80-172) on one workgroup: the rows march
-// offshore to onshore with the reference's control flow (mesh rows and intermediate rows).
-hipError_t launch_wave_field(hipStream_t st, const WaveArgs &a, double hOff, double thetaOff, double T);
+// Workgroups of WB threads own WNI consecutive nodes of a row (two halo nodes each side).
+constexpr int WB = 64;
+constexpr int WNI = WB - 4;
+int wave_blocks(int ny);
+
+// Marching state after a step; double-buffered (launch s reads ctl[s&1], writes
+// ctl[(s+1)&1]).
+struct WaveCtl {
+    double x0;      // cross-shore position of the last row
+    long steps;     // marching steps so far
+    int xIndex;     // mesh row of the last mesh row reached
+    int done;       // xIndex == nx-1 (or the step limit was hit)
+    int err;        // the march exceeded 1000*nx + 1000 steps
+};
+struct WaveCtlBufs {
+    WaveCtl *ctl;               // [2]
+    unsigned long long *acc;    // [3] bit patterns of max |tan(theta)| of a row
+    int *anyv;                  // [3] the row had a non-NaN |tan(theta)|
+};
+
+// Wave::calculateWaveField (src/Wave.cpp:80-172): the first row (with ctl[0], acc[0]),
+// then marching steps s0 .. s0+count-1, one launch each. The caller zeroes both row
+// buffers, acc and anyv before the first row, launches at least nx-1 steps and reads
+// ctl[next & 1] to see whether more are needed (intermediate rows).
+hipError_t launch_wave_first(hipStream_t st, const WaveArgs &a, double hOff, double thetaOff, double T,
+                             const WaveCtlBufs &b);
+hipError_t launch_wave_steps(hipStream_t st, const WaveArgs &a, double T, int s0, int count, const WaveCtlBufs &b);
 // Wave::setUV (src/Wave.cpp:293-302): U, V copied in; U_w from the stored direction.
 hipError_t launch_wave_set_uv(hipStream_t st, const WaveArgs &a, double *U, double *V, double *U_w,
                               const double *Usrc, const double *Vsrc, long n);

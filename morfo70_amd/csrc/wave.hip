@@ -2,19 +2,23 @@
 // src/WaveRow.cpp, src/WaveDir.cpp, src/WaveEnergy.cpp, src/WaveParametrizations.cpp,
 // src/WaveRollers.cpp) for gfx950.
 //
-// The field marches offshore to onshore, one row after the other; a row depends on the
-// whole previous row (the step length takes the maximum |tan(theta)| over it), and within
-// a row the direction needs the neighbours' wave numbers and the energy balance the
-// neighbours' directions. So one workgroup owns a field: its threads take the nodes of a
-// row in strides and barriers separate (depth, dispersion) -> (direction) -> (energy,
-// rollers, storage); the row arrays live in global memory, resident in L2 (37 arrays of
-// ny doubles per row, two rows). Control flow (mesh rows and intermediate rows, the step
-// count) follows src/Wave.cpp:80-172, with the two row buffers swapped after every step
-// (the reference's brace-less `else` at :158-163 swaps them only after intermediate rows,
-// aliasing the buffers otherwise). Each node's Newton-Raphson, friction terms included,
-// is its own (src/WaveEnergy.cpp:62-115 with the per-node indexing; friction is off by
-// default). Arithmetic keeps the reference's operation order (-ffp-contract=off);
-// transcendental functions are the device library's.
+// The field marches offshore to onshore, one row after the other: a row needs the whole
+// previous row (the step length takes the largest |tan(theta)| over it). Each marching
+// step is one launch over the row, split across workgroups of WB threads that own WNI
+// consecutive nodes each. Within a row the direction needs the neighbours' wave numbers
+// and the energy balance the neighbours' directions and currents, so a workgroup also
+// computes a halo of two nodes each side for the dispersion and one for the direction
+// (the same arithmetic as the owner's, so the values agree bit for bit) and needs no
+// other workgroup within the step. The largest |tan(theta)| of the new row is reduced by
+// an atomic maximum (non-negative doubles order as their bit patterns) into a three-slot
+// accumulator, and the step's control state (x0, mesh row, done) is double-buffered in
+// device memory, so that launch s reads what launch s-1 wrote. Control flow (mesh rows,
+// intermediate rows, the step count) follows src/Wave.cpp:80-172, with the two row
+// buffers swapped after every step (the reference's brace-less `else` at :158-163 swaps
+// them only after intermediate rows, aliasing the buffers otherwise). Each node's
+// Newton-Raphson, friction terms included, is its own (src/WaveEnergy.cpp:62-115 with the
+// per-node indexing; friction is off by default). Arithmetic keeps the reference's
+// operation order (-ffp-contract=off); transcendental functions are the device library's.
 #include "wave.h"
 
 namespace m70 {
@@ -147,300 +151,388 @@ __device__ void w_friction(const m70w_params &p, double h, double sigma0, double
     }
 }
 
-// Value at the foot of the incident ray (src/WaveEnergy.cpp:188-193).
-__device__ __forceinline__ double at_ray(const double *f, int up, int low, double y_i_dy)
+// Value at the foot of the incident ray: linear interpolation between the neighbours
+// j+1 (up) and j-1 (low) of the previous row (src/WaveEnergy.cpp:188-193).
+__device__ __forceinline__ double at_ray(double f_up, double f_low, double y_i_dy)
 {
-    return (f[up] + f[low]) * 0.5 + (f[up] - f[low]) * y_i_dy;
+    return (f_up + f_low) * 0.5 + (f_up - f_low) * y_i_dy;
 }
 
-// Direction and radiation-stress factors of a node (src/WaveRow.cpp:171-190,
-// src/WaveDir.cpp:74-99).
-__device__ __forceinline__ void dir_terms(const WRow &r, int j, double theta, double cosTheta, double sinTheta,
-                                          double cossinTheta)
+// One node of a row, in registers.
+struct WNode {
+    double D, U, V, U_w, lK, Sigma, C, Cg, Urms_H;
+    double Y_i, inv_s_i, Theta, CosTheta, SinTheta, TanTheta, Cx, Cgx, Cy, Cgy, sxx, syy, sxy, sxx_r, syy_r, sxy_r;
+    double lE, E, H, dw, df, Dw, WC, WC_r, E_r, d_r;
+};
+
+// Direction and radiation-stress factors (src/WaveRow.cpp:171-190, src/WaveDir.cpp:74-99).
+__device__ __forceinline__ void node_dir_terms(WNode &n, double theta, double cosTheta, double sinTheta,
+                                               double cossinTheta)
 {
-    const double c = r.C[j], cg = r.Cg[j];
-    r.Cx[j] = c * cosTheta;
-    r.Cgx[j] = cg * cosTheta;
-    r.Cy[j] = c * sinTheta;
-    r.Cgy[j] = cg * sinTheta;
+    const double c = n.C, cg = n.Cg;
+    n.Cx = c * cosTheta;
+    n.Cgx = cg * cosTheta;
+    n.Cy = c * sinTheta;
+    n.Cgy = cg * sinTheta;
     const double cg_c = cg / c;
     const double cosTheta2 = cosTheta * cosTheta;
-    r.sxx[j] = ((1 + cosTheta2) * cg_c - 0.5);
-    r.syy[j] = ((2 - cosTheta2) * cg_c - 0.5);
-    r.sxy[j] = (cossinTheta * cg_c);
-    r.sxx_r[j] = 2 * cosTheta2;
-    r.syy_r[j] = 2 * (1 - cosTheta2);
-    r.sxy_r[j] = 2 * cossinTheta;
-    r.Theta[j] = theta;
-    r.CosTheta[j] = cosTheta;
-    r.SinTheta[j] = sinTheta;
-    r.TanTheta[j] = sinTheta / cosTheta;
+    n.sxx = ((1 + cosTheta2) * cg_c - 0.5);
+    n.syy = ((2 - cosTheta2) * cg_c - 0.5);
+    n.sxy = (cossinTheta * cg_c);
+    n.sxx_r = 2 * cosTheta2;
+    n.syy_r = 2 * (1 - cosTheta2);
+    n.sxy_r = 2 * cossinTheta;
+    n.Theta = theta;
+    n.CosTheta = cosTheta;
+    n.SinTheta = sinTheta;
+    n.TanTheta = sinTheta / cosTheta;
 }
 
-// Wave::storageRow (src/Wave.cpp:175-264) of node j into row xIndex of the field.
-__device__ __forceinline__ void store_node(const WField &f, long o, const WRow &r, int j)
+// The arrays of a row that the next step reads as the previous row.
+__device__ __forceinline__ void write_row(const WRow &r, int j, const WNode &n)
 {
-    f.Theta[o] = r.Theta[j];
-    f.CosTheta[o] = r.CosTheta[j];
-    f.SinTheta[o] = r.SinTheta[j];
-    f.Cg[o] = r.Cg[j];
-    f.H[o] = r.H[j];
-    f.Dw[o] = r.Dw[j];
-    f.K[o] = r.lK[j];
-    const double e = r.E[j], e_r = r.E_r[j];
-    const double sxx_w = r.sxx[j] * e, sxy_w = r.sxy[j] * e, syy_w = r.syy[j] * e;
-    const double sxx_r = r.sxx_r[j] * e_r, sxy_r = r.sxy_r[j] * e_r, syy_r = r.syy_r[j] * e_r;
+    r.H[j] = n.H;
+    r.lE[j] = n.lE;
+    r.TanTheta[j] = n.TanTheta;
+    r.lK[j] = n.lK;
+    r.Cg[j] = n.Cg;
+    r.C[j] = n.C;
+    r.U[j] = n.U;
+    r.V[j] = n.V;
+    r.dw[j] = n.dw;
+    r.df[j] = n.df;
+    r.Dw[j] = n.Dw;
+    r.WC[j] = n.WC;
+    r.WC_r[j] = n.WC_r;
+    r.Cx[j] = n.Cx;
+    r.Cy[j] = n.Cy;
+    r.Cgx[j] = n.Cgx;
+    r.Cgy[j] = n.Cgy;
+    r.E_r[j] = n.E_r;
+    r.d_r[j] = n.d_r;
+}
+
+// Wave::storageRow (src/Wave.cpp:175-264) of one node into element o of the field.
+__device__ __forceinline__ void store_node(const WField &f, long o, const WNode &n)
+{
+    f.Theta[o] = n.Theta;
+    f.CosTheta[o] = n.CosTheta;
+    f.SinTheta[o] = n.SinTheta;
+    f.Cg[o] = n.Cg;
+    f.H[o] = n.H;
+    f.Dw[o] = n.Dw;
+    f.K[o] = n.lK;
+    const double e = n.E, e_r = n.E_r;
+    const double sxx_w = n.sxx * e, sxy_w = n.sxy * e, syy_w = n.syy * e;
+    const double sxx_r = n.sxx_r * e_r, sxy_r = n.sxy_r * e_r, syy_r = n.syy_r * e_r;
     f.E[o] = e;
     f.Sxx_w[o] = sxx_w;
     f.Sxy_w[o] = sxy_w;
     f.Syy_w[o] = syy_w;
     f.E_r[o] = e_r;
-    f.D_r[o] = r.d_r[j] * e_r;
+    f.D_r[o] = n.d_r * e_r;
     f.Sxx_r[o] = sxx_r;
     f.Sxy_r[o] = sxy_r;
     f.Syy_r[o] = syy_r;
     f.Sxx[o] = sxx_w + sxx_r;
     f.Sxy[o] = sxy_w + sxy_r;
     f.Syy[o] = syy_w + syy_r;
-    f.Urms[o] = r.Urms_H[j] * r.H[j];
+    f.Urms[o] = n.Urms_H * n.H;
 }
 
-template <int BS>
-__global__ void __launch_bounds__(BS) k_wave_field(WaveArgs a, double hOff, double thetaOff, double T)
+// Block maximum of |tan(theta)| over the valid values of the workgroup's nodes, folded
+// into acc (bit pattern of a non-negative double) and any (a value was valid).
+__device__ __forceinline__ void fold_tan_max(double m, int any, unsigned long long *acc, int *anyv)
 {
-    __shared__ double red[BS];
+    __shared__ double s_max[WB];
+    __shared__ int s_any[WB];
     const int t = threadIdx.x;
+    s_max[t] = m;
+    s_any[t] = any;
+    __syncthreads();
+    for (int o = WB / 2; o > 0; o >>= 1) {
+        if (t < o) {
+            if (s_max[t + o] > s_max[t]) s_max[t] = s_max[t + o];
+            s_any[t] |= s_any[t + o];
+        }
+        __syncthreads();
+    }
+    if (t == 0 && s_any[0]) {
+        atomicMax(acc, (unsigned long long)__double_as_longlong(s_max[0]));
+        atomicOr(anyv, 1);
+    }
+}
+
+// WaveRow::initFirstRow (src/WaveRow.cpp:129-232) and storageRow(0): one thread per
+// node of the workgroup's WNI nodes.
+__global__ void __launch_bounds__(WB) k_wave_first(WaveArgs a, double hOff, double thetaOff, double T, WaveCtl *ctl,
+                                                   unsigned long long *acc, int *anyv)
+{
+    __shared__ double s_dw0, s_df0;
+    const int t = threadIdx.x, k = blockIdx.x;
+    const int ny = a.ny;
+    const m70w_params &p = a.p;
+    const double w = 2 * WPI / T;
+    if (t == 0) {   // node 0's depth, the absolute frequency and the offshore height (:143-146)
+        double lK, sg, C, Cg, Uh;
+        w_dispersion(a.D[0], a.U_w[0], w, p.z0, lK, sg, C, Cg, Uh);
+        double dw0 = 0, ddw_dh0 = 0, df0 = 0, ddf_dh0 = 0;
+        w_breaking(p, hOff, w, a.D[0], dw0, ddw_dh0);
+        if (p.bottom_friction) w_friction(p, hOff, w, Uh, df0, ddf_dh0);
+        s_dw0 = dw0;
+        s_df0 = df0;
+    }
+    __syncthreads();
+    const int j0 = k * WNI;
+    const int nint = min(WNI, ny - j0);
+    double m = -999;
+    int any = 0;
+    if (t < nint) {
+        const int j = j0 + t;
+        WNode n{};
+        n.D = a.D[j];
+        n.U = a.U[j];
+        n.V = a.V[j];
+        n.U_w = a.U_w[j];
+        w_dispersion(n.D, n.U_w, w, p.z0, n.lK, n.Sigma, n.C, n.Cg, n.Urms_H);
+        const double cosTheta = cos(thetaOff);
+        const double sinTheta = sin(thetaOff);
+        node_dir_terms(n, thetaOff, cosTheta, sinTheta, sinTheta * cosTheta);
+        const double EOff = hOff * hOff * CTE_ENERGY;
+        n.H = hOff;
+        n.E = EOff;
+        n.Dw = s_dw0 * EOff;
+        n.lE = log(EOff);
+        n.dw = s_dw0;
+        n.df = s_df0;
+        write_row(a.rows[0], j, n);
+        store_node(a.f, j, n);
+        const double tt = fabs(n.TanTheta);
+        if (tt > m) {
+            m = tt;
+            any = 1;
+        }
+    }
+    fold_tan_max(m, any, &acc[0], &anyv[0]);
+    if (k == 0 && t == 0) {
+        WaveCtl c{};
+        c.x0 = a.x[0];
+        c.done = (a.nx - 1 <= 0);
+        ctl[0] = c;
+    }
+}
+
+// One marching step s (rows s&1 -> (s&1)^1): Wave::getDx (src/Wave.cpp:267-290),
+// WaveRow::initRow (src/WaveRow.cpp:235-296), dispersion (src/WaveParametrizations.cpp:
+// 14-60), solveDir (src/WaveDir.cpp:12-126), energyPreprocessor and solveEnergy
+// (src/WaveEnergy.cpp), solveRollers (src/WaveRollers.cpp) and storageRow when the row is a
+// mesh row. Thread t of workgroup k holds node (k*WNI + t - 2) mod ny: slots 0..nint+3
+// take the dispersion, 1..nint+2 the direction, 2..nint+1 (the owned nodes) the rest.
+__global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, WaveCtl *ctl, unsigned long long *acc,
+                                                  int *anyv)
+{
+    const WaveCtl c = ctl[s & 1];
+    const int t = threadIdx.x, k = blockIdx.x;
+    if (c.done) {   // pass the final state on to the launches that follow
+        if (k == 0 && t == 0) ctl[(s + 1) & 1] = c;
+        return;
+    }
+    __shared__ double sl_lK[WB], sl_U[WB], sl_V[WB], sl_Cgy[WB], sl_Cy[WB];
+    __shared__ double s_sigma0;
     const int nx = a.nx, ny = a.ny;
     const m70w_params &p = a.p;
     const double inv_2dy = a.inv_2dy;
     const double w = 2 * WPI / T;
-    // Both rows start from zero, as the reference builds them afresh for every field
-    // (src/Wave.cpp:115-116; WC, WC_r, E_r, d_r of the first row are read as zero).
-    for (int q = 0; q < 2; ++q) {
-        double *const *arr = &a.rows[q].D;
-        for (int k = 0; k < WROW_ARRAYS; ++k)
-            for (int j = t; j < ny; j += BS) arr[k][j] = 0.0;
-    }
-    // WaveRow::initFirstRow (src/WaveRow.cpp:129-232)
-    {
-        const WRow &r = a.rows[0];
-        for (int j = t; j < ny; j += BS) {
-            r.D[j] = a.D[j];
-            r.U[j] = a.U[j];
-            r.V[j] = a.V[j];
-            r.U_w[j] = a.U_w[j];
-            w_dispersion(r.D[j], r.U_w[j], w, p.z0, r.lK[j], r.Sigma[j], r.C[j], r.Cg[j], r.Urms_H[j]);
-        }
-        __syncthreads();
-        double dw0 = 0, ddw_dh0 = 0, df0 = 0, ddf_dh0 = 0;   // node 0, absolute frequency, offshore H
-        w_breaking(p, hOff, w, r.D[0], dw0, ddw_dh0);
-        if (p.bottom_friction) w_friction(p, hOff, w, r.Urms_H[0], df0, ddf_dh0);
-        const double EOff = hOff * hOff * CTE_ENERGY;
-        const double lEOff = log(EOff);
-        for (int j = t; j < ny; j += BS) {
-            const double cosTheta = cos(thetaOff);
-            const double sinTheta = sin(thetaOff);
-            dir_terms(r, j, thetaOff, cosTheta, sinTheta, sinTheta * cosTheta);
-            r.H[j] = hOff;
-            r.E[j] = EOff;
-            r.Dw[j] = dw0 * EOff;
-            r.lE[j] = lEOff;
-            r.dw[j] = dw0;
-            r.df[j] = df0;
-            store_node(a.f, j, r, j);
-        }
-        __syncthreads();
-    }
-    double x0 = a.x[0];
-    int xIndex = 0, prev = 0;
-    long steps = 0, err = 0;
-    const long max_steps = 1000L * nx + 1000;
-    const double logCteEnergy = log(CTE_ENERGY);
-    while (xIndex < nx - 1) {
-        const WRow &rm1 = a.rows[prev];
-        const WRow &r = a.rows[prev ^ 1];
-        // Wave::getDx (src/Wave.cpp:267-290): the largest |tan(theta)| of the previous row
-        double m = -999;
-        for (int j = t; j < ny; j += BS) {
-            const double tt = fabs(rm1.TanTheta[j]);
-            if (tt > m) m = tt;
-        }
-        red[t] = m;
-        __syncthreads();
-        for (int s = BS / 2; s > 0; s >>= 1) {
-            if (t < s && red[t + s] > red[t]) red[t] = red[t + s];
-            __syncthreads();
-        }
-        const double tanThetaMax = red[0];
-        const double dx_remain = a.x[xIndex + 1] - x0;
-        double dx_row;
-        int store;
-        if (tanThetaMax == 0) {
+    // getDx
+    const double tanThetaMax = anyv[s % 3] ? __longlong_as_double((long long)acc[s % 3]) : -999.0;
+    const double dx_remain = a.x[c.xIndex + 1] - c.x0;
+    double dx_row;
+    int store;
+    if (tanThetaMax == 0) {
+        dx_row = dx_remain;
+        store = 1;
+    } else {
+        dx_row = a.dy / tanThetaMax;
+        if (dx_row < dx_remain) {
+            dx_row = dx_row * 0.9;
+            store = 0;
+        } else {
             dx_row = dx_remain;
             store = 1;
-        } else {
-            dx_row = a.dy / tanThetaMax;
-            if (dx_row < dx_remain) {
-                dx_row = dx_row * 0.9;
-                store = 0;
-            } else {
-                dx_row = dx_remain;
-                store = 1;
-            }
-        }
-        // WaveRow::initRow (src/WaveRow.cpp:235-296) and dispersion_parallel (:108-128)
-        const long index = (long)(xIndex + 1) * ny, index_m1 = (long)xIndex * ny;
-        const double dx_dx = dx_row / a.dx[xIndex];
-        for (int j = t; j < ny; j += BS) {
-            r.H[j] = rm1.H[j];
-            r.lE[j] = rm1.lE[j];
-            double d, u, v, uw;
-            if (store) {
-                d = a.D[index + j];
-                u = a.U[index + j];
-                v = a.V[index + j];
-                uw = a.U_w[index + j];
-            } else {
-                d = a.D[index_m1 + j] + (a.D[index + j] - a.D[index_m1 + j]) * dx_dx;
-                u = a.U[index_m1 + j] + (a.U[index + j] - a.U[index_m1 + j]) * dx_dx;
-                v = a.V[index_m1 + j] + (a.V[index + j] - a.V[index_m1 + j]) * dx_dx;
-                uw = a.U_w[index_m1 + j] + (a.U_w[index + j] - a.U_w[index_m1 + j]) * dx_dx;
-            }
-            r.D[j] = d;
-            r.U[j] = u;
-            r.V[j] = v;
-            r.U_w[j] = uw;
-            w_dispersion(d, uw, w, p.z0, r.lK[j], r.Sigma[j], r.C[j], r.Cg[j], r.Urms_H[j]);
-        }
-        __syncthreads();
-        // WaveRow::solveDir (src/WaveDir.cpp:12-126)
-        const double dx = dx_row;
-        const double dx2 = dx * dx;
-        const double inv_dx = 1 / dx;
-        const double thetaMax = p.theta_max;
-        for (int j = t; j < ny; j += BS) {
-            const int jp = (j == ny - 1) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;
-            double tt_p1 = rm1.TanTheta[jp];
-            double tt_m1 = rm1.TanTheta[jm];
-            double dttheta_dy = (tt_p1 - tt_m1) * inv_2dy;
-            double ttheta_med = (tt_p1 + tt_m1) * 0.5;
-            double y_i = -ttheta_med / (dttheta_dy + inv_dx);
-            double s_i = sqrt(dx2 + y_i * y_i);
-            double inv_s_i = 1 / s_i;
-            double sinTheta_i = -y_i * inv_s_i;
-            double cosTheta_i = dx * inv_s_i;
-            double theta_i = asin(sinTheta_i);
-            r.Y_i[j] = y_i;
-            r.inv_S_i[j] = inv_s_i;
-            double dlk_dx = (r.lK[j] - rm1.lK[j]) * inv_dx;
-            double dlk_dy_c = 0.5 * inv_2dy * (r.lK[jp] + rm1.lK[jp] - r.lK[jm] - rm1.lK[jm]);
-            double dlk_ds = dlk_dx * cosTheta_i + dlk_dy_c * sinTheta_i;
-            double dlk_dst = -dlk_dx * sinTheta_i + dlk_dy_c * cosTheta_i;
-            double theta = theta_i + dlk_dst / (inv_s_i + dlk_ds * 0.5);
-            theta = smin_(theta, thetaMax);
-            theta = smax_(theta, -thetaMax);
-            const double sinTheta = sin(theta);
-            const double cosTheta = cos(theta);
-            dir_terms(r, j, theta, cosTheta, sinTheta, cosTheta * sinTheta);
-        }
-        __syncthreads();
-        // energyPreprocessor (src/WaveEnergy.cpp:130-287), solveEnergy (:32-127),
-        // solveRollers (src/WaveRollers.cpp:10-106) and storageRow, node by node
-        const double sigma0 = r.Sigma[0];
-        const double d_r_div_c = 2 * WG * p.sinBeta_r;
-        const double alpha_r = p.alpha_r;
-        for (int i = t; i < ny; i += BS) {
-            const int up = (i == ny - 1) ? 0 : i + 1, low = (i == 0) ? ny - 1 : i - 1;
-            const double y_i_dy = r.Y_i[i] * inv_2dy;
-            const double inv_s_i = r.inv_S_i[i];
-            double G, Gp;
-            {
-                double cg_c = 0.5 * (r.Cg[i] + at_ray(rm1.Cg, up, low, y_i_dy));
-                double u_c = 0.5 * (r.U[i] + at_ray(rm1.U, up, low, y_i_dy));
-                double v_c = 0.5 * (r.V[i] + at_ray(rm1.V, up, low, y_i_dy));
-                double dw_i = at_ray(rm1.dw, up, low, y_i_dy);
-                double df_i = at_ray(rm1.df, up, low, y_i_dy);
-                double le_i = at_ray(rm1.lE, up, low, y_i_dy);
-                double du_dx = (r.U[i] - rm1.U[i]) * inv_dx;
-                double du_dy = (r.U[up] - r.U[low]) * inv_2dy;
-                double dv_dx = (r.V[i] - rm1.V[i]) * inv_dx;
-                double dv_dy = (r.V[up] - r.V[low]) * inv_2dy;
-                double wc = 2 * r.sxx[i] * du_dx + r.sxy[i] * (du_dy + dv_dx) + 2 * r.syy[i] * dv_dy;
-                double wc_c = 0.5 * (wc + at_ray(rm1.WC, up, low, y_i_dy));
-                r.WC[i] = wc;
-                r.WC_r[i] = r.sxx_r[i] * du_dx + r.sxy_r[i] * (du_dy + dv_dx) + r.syy_r[i] * dv_dy;
-                double dcgxu_dx = (r.Cgx[i] - rm1.Cgx[i]) * inv_dx + du_dx;
-                double dcgyv_dy_c = 0.5 * (r.Cgy[up] + rm1.Cgy[up] - r.Cgy[low] - rm1.Cgy[low]) * inv_2dy + dv_dy;
-                double div_cgv_c = dcgxu_dx + dcgyv_dy_c;
-                double dle_dy_c = (rm1.lE[up] - rm1.lE[low]) * inv_2dy;
-                G = v_c * dle_dy_c + wc_c + div_cgv_c + 0.5 * (dw_i + df_i) - cg_c * inv_s_i * le_i -
-                    u_c * rm1.lE[i] * inv_dx;
-                Gp = cg_c * inv_s_i + u_c * inv_dx;
-                r.G[i] = G;
-                r.Gp[i] = Gp;
-            }
-            {   // Newton-Raphson on log(E); the Jacobian's doubled Gp is the reference's (:97)
-                double le_old = rm1.lE[i], H = r.H[i], E = 0.0, Dw = 0.0;
-                double dw = 0.0, ddw_dh = 0.0, df = 0.0, ddf_dh = 0.0;
-                const double sig = r.Sigma[i], Dn = r.D[i], urms_h = r.Urms_H[i];
-                for (int k = 0; k < p.max_iterations; k++) {
-                    w_breaking(p, H, sig, Dn, dw, ddw_dh);
-                    if (p.bottom_friction) w_friction(p, H, sigma0, urms_h, df, ddf_dh);
-                    double gg = G;
-                    double gp = Gp;
-                    double le = le_old;
-                    gg += gp * le + 0.5 * (dw + df);
-                    gp += gp + 0.25 * (ddf_dh + ddw_dh) * H;
-                    le -= gg / gp;
-                    double h = exp((le - logCteEnergy) * 0.5);
-                    H = h;
-                    E = CTE_ENERGY * h * h;
-                    const bool more = fabs(le - le_old) > p.tolerance;
-                    le_old = le;
-                    Dw = dw * E;
-                    if (!more) break;
-                }
-                r.lE[i] = le_old;
-                r.E[i] = E;
-                r.H[i] = H;
-                r.dw[i] = dw;
-                r.df[i] = df;
-                r.Dw[i] = Dw;
-            }
-            if (p.rollers) {
-                double c_c = 0.5 * (r.C[i] + at_ray(rm1.C, up, low, y_i_dy));
-                double u_c = 0.5 * (r.U[i] + at_ray(rm1.U, up, low, y_i_dy));
-                double v_c = 0.5 * (r.V[i] + at_ray(rm1.V, up, low, y_i_dy));
-                double du_dx = (r.U[i] - rm1.U[i]) * inv_dx;
-                double dv_dy = (r.V[up] - r.V[low]) * inv_2dy;
-                double wc_r_c = 0.5 * (r.WC_r[i] + at_ray(rm1.WC_r, up, low, y_i_dy));
-                double dcxu_dx = (r.Cx[i] - rm1.Cx[i]) * inv_dx + du_dx;
-                double dcyv_dy_c = 0.5 * (r.Cy[up] + rm1.Cy[up] - r.Cy[low] - rm1.Cy[low]) * inv_2dy + dv_dy;
-                double div_cv_c = dcxu_dx + dcyv_dy_c;
-                double der_dy_c = (rm1.E_r[up] - rm1.E_r[low]) * inv_2dy;
-                const double d_r = d_r_div_c / r.C[i];
-                r.d_r[i] = d_r;
-                double d_r_c = 0.5 * (d_r + at_ray(rm1.d_r, up, low, y_i_dy));
-                double Dw_c = 0.5 * (r.Dw[i] + at_ray(rm1.Dw, up, low, y_i_dy));
-                double er_i = at_ray(rm1.E_r, up, low, y_i_dy);
-                double upv = c_c * inv_s_i * er_i + u_c * rm1.E_r[i] * inv_dx - v_c * der_dy_c + 0.5 * alpha_r * Dw_c;
-                double lowv = c_c * inv_s_i + u_c * inv_dx + div_cv_c + 0.5 * wc_r_c + 0.5 * d_r_c;
-                r.E_r[i] = upv / lowv;
-            }
-            if (store) store_node(a.f, index + i, r, i);
-        }
-        __syncthreads();
-        prev ^= 1;
-        x0 = x0 + dx_row;
-        if (store) xIndex = xIndex + 1;
-        if (++steps > max_steps) {
-            err = 1;
-            break;
         }
     }
-    if (t == 0) {
-        a.status[0] = steps;
-        a.status[1] = err;
+    const WRow &rm1 = a.rows[s & 1];
+    const WRow &r = a.rows[(s & 1) ^ 1];
+    const long index = (long)(c.xIndex + 1) * ny, index_m1 = (long)c.xIndex * ny;
+    const double dx_dx = dx_row / a.dx[c.xIndex];
+    const int j0 = k * WNI;
+    const int nint = min(WNI, ny - j0);
+    int j = (j0 + t - 2) % ny;
+    if (j < 0) j += ny;
+    const int jp = (j == ny - 1) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;
+    WNode n{};
+    // initRow + dispersion
+    auto init_node = [&](int q, double &d, double &u, double &v, double &uw) {
+        if (store) {
+            d = a.D[index + q];
+            u = a.U[index + q];
+            v = a.V[index + q];
+            uw = a.U_w[index + q];
+        } else {
+            d = a.D[index_m1 + q] + (a.D[index + q] - a.D[index_m1 + q]) * dx_dx;
+            u = a.U[index_m1 + q] + (a.U[index + q] - a.U[index_m1 + q]) * dx_dx;
+            v = a.V[index_m1 + q] + (a.V[index + q] - a.V[index_m1 + q]) * dx_dx;
+            uw = a.U_w[index_m1 + q] + (a.U_w[index + q] - a.U_w[index_m1 + q]) * dx_dx;
+        }
+    };
+    if (t < nint + 4) {
+        init_node(j, n.D, n.U, n.V, n.U_w);
+        w_dispersion(n.D, n.U_w, w, p.z0, n.lK, n.Sigma, n.C, n.Cg, n.Urms_H);
+        sl_lK[t] = n.lK;
+        sl_U[t] = n.U;
+        sl_V[t] = n.V;
+    }
+    if (t == 0 && p.bottom_friction && p.friction_model == M70W_FRIC_SOULSBY97) {
+        double d0, u0, v0, uw0, lK0, sg0, C0, Cg0, Uh0;   // Sigma of the row's first node
+        init_node(0, d0, u0, v0, uw0);
+        w_dispersion(d0, uw0, w, p.z0, lK0, sg0, C0, Cg0, Uh0);
+        s_sigma0 = sg0;
+    }
+    __syncthreads();
+    // solveDir
+    const double dx = dx_row;
+    const double dx2 = dx * dx;
+    const double inv_dx = 1 / dx;
+    if (t >= 1 && t < nint + 3) {
+        double tt_p1 = rm1.TanTheta[jp];
+        double tt_m1 = rm1.TanTheta[jm];
+        double dttheta_dy = (tt_p1 - tt_m1) * inv_2dy;
+        double ttheta_med = (tt_p1 + tt_m1) * 0.5;
+        double y_i = -ttheta_med / (dttheta_dy + inv_dx);
+        double s_i = sqrt(dx2 + y_i * y_i);
+        double inv_s_i = 1 / s_i;
+        double sinTheta_i = -y_i * inv_s_i;
+        double cosTheta_i = dx * inv_s_i;
+        double theta_i = asin(sinTheta_i);
+        n.Y_i = y_i;
+        n.inv_s_i = inv_s_i;
+        double dlk_dx = (n.lK - rm1.lK[j]) * inv_dx;
+        double dlk_dy_c = 0.5 * inv_2dy * (sl_lK[t + 1] + rm1.lK[jp] - sl_lK[t - 1] - rm1.lK[jm]);
+        double dlk_ds = dlk_dx * cosTheta_i + dlk_dy_c * sinTheta_i;
+        double dlk_dst = -dlk_dx * sinTheta_i + dlk_dy_c * cosTheta_i;
+        double theta = theta_i + dlk_dst / (inv_s_i + dlk_ds * 0.5);
+        theta = smin_(theta, p.theta_max);
+        theta = smax_(theta, -p.theta_max);
+        const double sinTheta = sin(theta);
+        const double cosTheta = cos(theta);
+        node_dir_terms(n, theta, cosTheta, sinTheta, cosTheta * sinTheta);
+        sl_Cgy[t] = n.Cgy;
+        sl_Cy[t] = n.Cy;
+    }
+    __syncthreads();
+    double m = -999;
+    int any = 0;
+    if (t >= 2 && t < nint + 2) {
+        const int up = jp, low = jm;
+        const double y_i_dy = n.Y_i * inv_2dy;
+        const double inv_s_i = n.inv_s_i;
+        const double U_up = sl_U[t + 1], U_low = sl_U[t - 1], V_up = sl_V[t + 1], V_low = sl_V[t - 1];
+        double G, Gp;
+        {   // energyPreprocessor (src/WaveEnergy.cpp:181-219)
+            double cg_c = 0.5 * (n.Cg + at_ray(rm1.Cg[up], rm1.Cg[low], y_i_dy));
+            double u_c = 0.5 * (n.U + at_ray(rm1.U[up], rm1.U[low], y_i_dy));
+            double v_c = 0.5 * (n.V + at_ray(rm1.V[up], rm1.V[low], y_i_dy));
+            double dw_i = at_ray(rm1.dw[up], rm1.dw[low], y_i_dy);
+            double df_i = at_ray(rm1.df[up], rm1.df[low], y_i_dy);
+            double le_i = at_ray(rm1.lE[up], rm1.lE[low], y_i_dy);
+            double du_dx = (n.U - rm1.U[j]) * inv_dx;
+            double du_dy = (U_up - U_low) * inv_2dy;
+            double dv_dx = (n.V - rm1.V[j]) * inv_dx;
+            double dv_dy = (V_up - V_low) * inv_2dy;
+            double wc = 2 * n.sxx * du_dx + n.sxy * (du_dy + dv_dx) + 2 * n.syy * dv_dy;
+            double wc_c = 0.5 * (wc + at_ray(rm1.WC[up], rm1.WC[low], y_i_dy));
+            n.WC = wc;
+            n.WC_r = n.sxx_r * du_dx + n.sxy_r * (du_dy + dv_dx) + n.syy_r * dv_dy;
+            double dcgxu_dx = (n.Cgx - rm1.Cgx[j]) * inv_dx + du_dx;
+            double dcgyv_dy_c = 0.5 * (sl_Cgy[t + 1] + rm1.Cgy[up] - sl_Cgy[t - 1] - rm1.Cgy[low]) * inv_2dy + dv_dy;
+            double div_cgv_c = dcgxu_dx + dcgyv_dy_c;
+            double dle_dy_c = (rm1.lE[up] - rm1.lE[low]) * inv_2dy;
+            G = v_c * dle_dy_c + wc_c + div_cgv_c + 0.5 * (dw_i + df_i) - cg_c * inv_s_i * le_i -
+                u_c * rm1.lE[j] * inv_dx;
+            Gp = cg_c * inv_s_i + u_c * inv_dx;
+        }
+        {   // solveEnergy: Newton-Raphson on log(E); the Jacobian's doubled Gp is the
+            // reference's (src/WaveEnergy.cpp:97)
+            const double logCteEnergy = log(CTE_ENERGY);
+            double le_old = rm1.lE[j], H = rm1.H[j], E = 0.0, Dw = 0.0;
+            double dw = 0.0, ddw_dh = 0.0, df = 0.0, ddf_dh = 0.0;
+            for (int it = 0; it < p.max_iterations; it++) {
+                w_breaking(p, H, n.Sigma, n.D, dw, ddw_dh);
+                if (p.bottom_friction) w_friction(p, H, s_sigma0, n.Urms_H, df, ddf_dh);
+                double gg = G;
+                double gp = Gp;
+                double le = le_old;
+                gg += gp * le + 0.5 * (dw + df);
+                gp += gp + 0.25 * (ddf_dh + ddw_dh) * H;
+                le -= gg / gp;
+                double h = exp((le - logCteEnergy) * 0.5);
+                H = h;
+                E = CTE_ENERGY * h * h;
+                const bool more = fabs(le - le_old) > p.tolerance;
+                le_old = le;
+                Dw = dw * E;
+                if (!more) break;
+            }
+            n.lE = le_old;
+            n.E = E;
+            n.H = H;
+            n.dw = dw;
+            n.df = df;
+            n.Dw = Dw;
+        }
+        if (p.rollers) {   // solveRollers (src/WaveRollers.cpp:57-100)
+            const double d_r_div_c = 2 * WG * p.sinBeta_r;
+            double c_c = 0.5 * (n.C + at_ray(rm1.C[up], rm1.C[low], y_i_dy));
+            double u_c = 0.5 * (n.U + at_ray(rm1.U[up], rm1.U[low], y_i_dy));
+            double v_c = 0.5 * (n.V + at_ray(rm1.V[up], rm1.V[low], y_i_dy));
+            double du_dx = (n.U - rm1.U[j]) * inv_dx;
+            double dv_dy = (V_up - V_low) * inv_2dy;
+            double wc_r_c = 0.5 * (n.WC_r + at_ray(rm1.WC_r[up], rm1.WC_r[low], y_i_dy));
+            double dcxu_dx = (n.Cx - rm1.Cx[j]) * inv_dx + du_dx;
+            double dcyv_dy_c = 0.5 * (sl_Cy[t + 1] + rm1.Cy[up] - sl_Cy[t - 1] - rm1.Cy[low]) * inv_2dy + dv_dy;
+            double div_cv_c = dcxu_dx + dcyv_dy_c;
+            double der_dy_c = (rm1.E_r[up] - rm1.E_r[low]) * inv_2dy;
+            const double d_r = d_r_div_c / n.C;
+            n.d_r = d_r;
+            double d_r_c = 0.5 * (d_r + at_ray(rm1.d_r[up], rm1.d_r[low], y_i_dy));
+            double Dw_c = 0.5 * (n.Dw + at_ray(rm1.Dw[up], rm1.Dw[low], y_i_dy));
+            double er_i = at_ray(rm1.E_r[up], rm1.E_r[low], y_i_dy);
+            double upv = c_c * inv_s_i * er_i + u_c * rm1.E_r[j] * inv_dx - v_c * der_dy_c + 0.5 * p.alpha_r * Dw_c;
+            double lowv = c_c * inv_s_i + u_c * inv_dx + div_cv_c + 0.5 * wc_r_c + 0.5 * d_r_c;
+            n.E_r = upv / lowv;
+        }
+        write_row(r, j, n);
+        if (store) store_node(a.f, index + j, n);
+        const double tt = fabs(n.TanTheta);
+        if (tt > m) {
+            m = tt;
+            any = 1;
+        }
+    }
+    fold_tan_max(m, any, &acc[(s + 1) % 3], &anyv[(s + 1) % 3]);
+    if (k == 0 && t == 0) {
+        WaveCtl nc = c;
+        nc.x0 = c.x0 + dx_row;
+        if (store) nc.xIndex = c.xIndex + 1;
+        nc.steps = c.steps + 1;
+        nc.done = (nc.xIndex >= nx - 1);
+        if (nc.steps > 1000L * nx + 1000) {
+            nc.done = 1;
+            nc.err = 1;
+        }
+        ctl[(s + 1) & 1] = nc;
+        acc[(s + 2) % 3] = 0;   // the slot launch s+1 folds into
+        anyv[(s + 2) % 3] = 0;
     }
 }
 
@@ -455,12 +547,21 @@ __global__ void __launch_bounds__(256) k_wave_set_uv(double *U, double *V, doubl
     U_w[q] = cosT[q] * u + sinT[q] * v;
 }
 
-hipError_t launch_wave_field(hipStream_t st, const WaveArgs &a, double hOff, double thetaOff, double T)
+int wave_blocks(int ny) { return (ny + WNI - 1) / WNI; }
+
+hipError_t launch_wave_first(hipStream_t st, const WaveArgs &a, double hOff, double thetaOff, double T,
+                             const WaveCtlBufs &b)
 {
-    // 512 threads at most: the node body needs about 250 VGPRs, two waves per SIMD
-    if (a.ny >= 512) hipLaunchKernelGGL(k_wave_field<512>, dim3(1), dim3(512), 0, st, a, hOff, thetaOff, T);
-    else if (a.ny >= 256) hipLaunchKernelGGL(k_wave_field<256>, dim3(1), dim3(256), 0, st, a, hOff, thetaOff, T);
-    else hipLaunchKernelGGL(k_wave_field<64>, dim3(1), dim3(64), 0, st, a, hOff, thetaOff, T);
+    hipLaunchKernelGGL(k_wave_first, dim3((unsigned)wave_blocks(a.ny)), dim3(WB), 0, st, a, hOff, thetaOff, T, b.ctl,
+                       b.acc, b.anyv);
+    return hipGetLastError();
+}
+
+hipError_t launch_wave_steps(hipStream_t st, const WaveArgs &a, double T, int s0, int count, const WaveCtlBufs &b)
+{
+    for (int s = s0; s < s0 + count; ++s)
+        hipLaunchKernelGGL(k_wave_step, dim3((unsigned)wave_blocks(a.ny)), dim3(WB), 0, st, a, T, s, b.ctl, b.acc,
+                           b.anyv);
     return hipGetLastError();
 }
 
