@@ -308,6 +308,54 @@ int m70w_set_uv_from_hydro(m70w *w);
 int m70w_push_forcing(m70w *w);
 
 /* ------------------------------------------------------------------------------------
+ * Morphodynamic outer loop on the device: Morfo70::Run (src/Morfo70.cpp:131-218).
+ * Each outer step computes the wave field on the hydro's depth, hands it to the hydro as
+ * forcing, runs the hydro, optionally projects its currents on the wave direction, computes
+ * the sediment fluxes and, inside the bed evolution window, updates the bed; then the
+ * largest changes of E, E_r, U_c, V_c, Zs and h since the previous step (the line the
+ * reference prints, :184-199) are reduced on the device into a per-step record, and a
+ * snapshot of the saved fields (saveTimeStep, :225-276) is taken at the save times by a
+ * device copy followed by an asynchronous copy to pinned host memory. The fields stay
+ * resident; the host only evaluates the scalar schedule (time, wave ramp, bed window,
+ * save times) in the reference's arithmetic. The wave handle must be coupled to the
+ * hydro handle (m70w_couple_hydro); the sediment handle (NULL = none) must be attached to
+ * it. Destroy the outer-loop handle before the others.
+ * ------------------------------------------------------------------------------------ */
+/* Morfo70::H_off, T_off, Theta_off (radians), Parameters::Run::warmTime, bedStartTime,
+ * bedStopTime, dt, saveIntervalHydro, saveIntervalMorfo, Parameters::Hydro::runIterations,
+ * runIterationsInBedEvolution, Options::Hydro::CalculateHydro,
+ * Options::Wave::WaveCurrentInteraction, Options::Sediment::CalculateSedimentTransport,
+ * Options::Bottom::CalulateBedEvolution (include/OptionsParametersConstants.h). The
+ * minimum offshore height is 0.01 m, or H_off when warmTime <= 0 (src/Morfo70.cpp:96-98). */
+typedef struct m70o_params {
+    double H_off, T_off, Theta_off;
+    double warmTime, bedStartTime, bedStopTime, dt, saveIntervalHydro, saveIntervalMorfo;
+    int runIterations, runIterationsInBedEvolution;
+    int calculate_hydro, wave_current_interaction, calculate_sediment, calculate_bed_evolution;
+} m70o_params;
+
+typedef struct m70o m70o;
+
+/* time0: the start time (0, or a resume file's time attribute, :104-110). The previous
+ * step's fields for the first record are the handles' current E, E_r, U_c, V_c, Zs, h
+ * (:120-127). max_snapshots: device and pinned host slots kept (the oldest is reused). */
+int m70o_create(m70h *h, m70w *w, m70s *s, const m70o_params *p, double time0, int max_snapshots, m70o **out);
+void m70o_destroy(m70o *o);
+const char *m70o_last_error(const m70o *o);
+/* Outer steps while time < stopTime (:131-218). *steps (may be NULL): steps run. */
+int m70o_run(m70o *o, double stopTime, long *steps);
+int m70o_time(m70o *o, double *time);
+/* Per-step records, 8 doubles each: time, offshore height, dE, dR, dU, dV, dZs, dh
+ * (:194). out = NULL returns the count in *n; otherwise *n records are copied. */
+int m70o_records(m70o *o, long *n, double *out);
+/* Snapshots taken so far (save times, :205-214), and a field of snapshot i by the name it
+ * has in the reference's m70_*.nc files: Zs, D, U (= U_c), V (= V_c), U_full, V_full,
+ * Qx_full, Qy_full ((nx+1)*ny, row 0 the offshore ghost), dQx_dx, dQy_dy, H, E, Cg, lK,
+ * Theta, Dw, E_r, qx, qy, Gamma_Swash, Alpha, gamma, h. Waits for that snapshot's copy. */
+int m70o_snapshot_count(m70o *o, long *total);
+int m70o_snapshot(m70o *o, long i, double *time, const char *field, double *out);
+
+/* ------------------------------------------------------------------------------------
  * Adjoint of Run. No counterpart in the reference, which has no derivatives: these
  * entry points serve callers that fit bathymetry or friction to observations (gradient
  * of a loss of the final state with respect to the initial state and the drag and

@@ -81,6 +81,14 @@ SIGNATURES = {
     "m70w_couple_hydro": (ctypes.c_int, [_vp, _vp]),
     "m70w_set_uv_from_hydro": (ctypes.c_int, [_vp]),
     "m70w_push_forcing": (ctypes.c_int, [_vp]),
+    "m70o_create": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "m70o_destroy": (None, [_vp]),
+    "m70o_last_error": (ctypes.c_char_p, [_vp]),
+    "m70o_run": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.POINTER(ctypes.c_long)]),
+    "m70o_time": (ctypes.c_int, [_vp, _dp]),
+    "m70o_records": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_long), _dp]),
+    "m70o_snapshot_count": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_long)]),
+    "m70o_snapshot": (ctypes.c_int, [_vp, ctypes.c_long, _dp, ctypes.c_char_p, _dp]),
     "m70h_adjoint_forward": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "m70h_adjoint_reverse": (ctypes.c_int, [_vp, _vp, _vp, _dp]),
     "m70h_adjoint_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),

@@ -14,6 +14,7 @@
 #include "hydro_device.h"
 #include "hydro_kernels.h"
 #include "sediment.h"
+#include "outer.h"
 #include "wave.h"
 
 using namespace m70;
@@ -1915,6 +1916,337 @@ int m70w_push_forcing(m70w *w)
     h->forcing_set = true;
     h->run_gen++;
     return M70H_OK;
+}
+
+}  // extern "C"
+
+// =====================================================================================
+// Morphodynamic outer loop on the device (Morfo70::Run, src/Morfo70.cpp:131-218)
+// =====================================================================================
+namespace {
+
+struct SnapField {
+    const char *name;
+    bool plus;   // (nx+1)*ny
+};
+// The arrays saveTimeStep writes (src/Morfo70.cpp:225-276), full-size where the file splits
+// them into a row and the rest (U0/U_x, Qx0/Qx_x, V0/V_y, Qy0/Qy_y).
+const SnapField kSnapFields[] = {
+    {"Zs", false}, {"D", false}, {"U", false}, {"V", false}, {"U_full", true}, {"V_full", true},
+    {"Qx_full", true}, {"Qy_full", true}, {"dQx_dx", false}, {"dQy_dy", false}, {"H", false}, {"E", false},
+    {"Cg", false}, {"lK", false}, {"Theta", false}, {"Dw", false}, {"E_r", false}, {"qx", false},
+    {"qy", false}, {"Gamma_Swash", false}, {"Alpha", false}, {"gamma", false}, {"h", false}};
+constexpr int kSnapCount = sizeof(kSnapFields) / sizeof(kSnapFields[0]);
+
+}  // namespace
+
+struct m70o {
+    m70h *h = nullptr;
+    m70w *w = nullptr;
+    m70s *s = nullptr;
+    m70o_params p{};
+    double H_off0 = 0.01;
+    std::string err;
+    double time = 0.0, lastSavedTime = -1;
+    std::vector<void *> dev;
+    double *prev[6] = {};
+    unsigned long long *dout = nullptr;
+    double *rec = nullptr;   // pinned host, 8 per step
+    long rec_cap = 0, rec_n = 0;
+    hipStream_t copy_stream = nullptr;
+    int max_snap = 0;
+    struct Slot {
+        double time = 0.0;
+        long index = -1;
+        double *dev = nullptr, *host = nullptr;
+        hipEvent_t copied = nullptr;
+    };
+    std::vector<Slot> slots;
+    long snap_total = 0;
+    size_t snap_doubles = 0;
+};
+
+namespace {
+
+int ofail(m70o *o, int code, const std::string &msg)
+{
+    if (o) o->err = msg; else g_create_error = msg;
+    return code;
+}
+
+#define OHIPCHK(o, expr)                                                                       \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return ofail(o, M70H_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// Device sources of the snapshot fields, in kSnapFields order (NULL: zero).
+void snap_sources(m70o *o, const double **src)
+{
+    const State &st = o->h->s;
+    const WField &f = o->w->a.f;
+    const double *hs[] = {st.Zs, st.D, st.Uc, st.Vc, st.U, st.V, st.Qx, st.Qy, st.dQx, st.dQy,
+                          f.H, f.E, f.Cg, f.K, f.Theta, f.Dw, f.E_r};
+    int q = 0;
+    for (const double *p : hs) src[q++] = p;
+    const m70s *s = o->s;
+    src[q++] = s ? s->qx : nullptr;
+    src[q++] = s ? s->qy : nullptr;
+    src[q++] = s ? s->Gamma : nullptr;
+    src[q++] = s ? s->Alpha : nullptr;
+    src[q++] = s ? s->gamma : nullptr;
+    src[q++] = s ? s->bed_h : nullptr;
+}
+
+int take_snapshot(m70o *o)
+{
+    const long F = o->h->F, Fp = o->h->Fp;
+    const int k = (int)(o->snap_total % o->max_snap);
+    m70o::Slot &sl = o->slots[k];
+    if (!sl.dev) {
+        OHIPCHK(o, hipMalloc((void **)&sl.dev, o->snap_doubles * sizeof(double)));
+        OHIPCHK(o, hipHostMalloc((void **)&sl.host, o->snap_doubles * sizeof(double), hipHostMallocDefault));
+        OHIPCHK(o, hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming));
+    } else {
+        OHIPCHK(o, hipStreamWaitEvent(o->h->stream, sl.copied, 0));   // the slot's last copy is out
+    }
+    const double *src[kSnapCount];
+    snap_sources(o, src);
+    size_t off = 0;
+    for (int q = 0; q < kSnapCount; ++q) {
+        const size_t n = (size_t)(kSnapFields[q].plus ? Fp : F);
+        if (src[q])
+            OHIPCHK(o, hipMemcpyAsync(sl.dev + off, src[q], n * sizeof(double), hipMemcpyDeviceToDevice, o->h->stream));
+        else
+            OHIPCHK(o, hipMemsetAsync(sl.dev + off, 0, n * sizeof(double), o->h->stream));
+        off += n;
+    }
+    hipEvent_t ready;
+    OHIPCHK(o, hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    OHIPCHK(o, hipEventRecord(ready, o->h->stream));
+    OHIPCHK(o, hipStreamWaitEvent(o->copy_stream, ready, 0));
+    OHIPCHK(o, hipMemcpyAsync(sl.host, sl.dev, o->snap_doubles * sizeof(double), hipMemcpyDeviceToHost,
+                              o->copy_stream));
+    OHIPCHK(o, hipEventRecord(sl.copied, o->copy_stream));
+    OHIPCHK(o, hipEventDestroy(ready));
+    sl.time = o->time;
+    sl.index = o->snap_total++;
+    return M70H_OK;
+}
+
+int ensure_records(m70o *o, long need)
+{
+    if (need <= o->rec_cap) return M70H_OK;
+    long cap = o->rec_cap ? o->rec_cap : 64;
+    while (cap < need) cap *= 2;
+    OHIPCHK(o, hipStreamSynchronize(o->h->stream));   // pending record copies land first
+    double *nrec = nullptr;
+    OHIPCHK(o, hipHostMalloc((void **)&nrec, (size_t)cap * 8 * sizeof(double), hipHostMallocDefault));
+    if (o->rec) {
+        std::memcpy(nrec, o->rec, (size_t)o->rec_n * 8 * sizeof(double));
+        (void)hipHostFree(o->rec);
+    }
+    o->rec = nrec;
+    o->rec_cap = cap;
+    return M70H_OK;
+}
+
+// The fields the records compare, current device pointers.
+void diff_sources(m70o *o, const double **cur)
+{
+    cur[0] = o->w->a.f.E;
+    cur[1] = o->w->a.f.E_r;
+    cur[2] = o->h->s.Uc;
+    cur[3] = o->h->s.Vc;
+    cur[4] = o->h->s.Zs;
+    cur[5] = o->s ? o->s->bed_h : nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *m70o_last_error(const m70o *o) { return o ? o->err.c_str() : g_create_error.c_str(); }
+
+int m70o_create(m70h *h, m70w *w, m70s *s, const m70o_params *p, double time0, int max_snapshots, m70o **out)
+{
+    if (!out) return ofail(nullptr, M70H_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!h || !w || !p) return ofail(nullptr, M70H_EINVAL, "hydro, wave and params are required");
+    if (w->hydro != h) return ofail(nullptr, M70H_EINVAL, "the wave handle is not coupled to this hydro handle");
+    if (s && s->h != h) return ofail(nullptr, M70H_EINVAL, "the sediment handle belongs to another hydro handle");
+    if (!(p->dt > 0) || !(p->T_off > 0)) return ofail(nullptr, M70H_EINVAL, "need dt > 0 and T_off > 0");
+    if (p->runIterations < 0 || p->runIterationsInBedEvolution < 0)
+        return ofail(nullptr, M70H_EINVAL, "run iterations must be >= 0");
+    m70o *o = new m70o();
+    o->h = h;
+    o->w = w;
+    o->s = s;
+    o->p = *p;
+    o->time = time0;
+    o->lastSavedTime = -1;
+    if (time0 > 0) {   // resumed run (src/Morfo70.cpp:104-110)
+        o->lastSavedTime = std::floor(time0);
+    }
+    o->H_off0 = (p->warmTime <= 0) ? p->H_off : 0.01;   // :96-98
+    o->max_snap = max_snapshots > 0 ? max_snapshots : 1;
+    o->slots.resize(o->max_snap);
+    o->snap_doubles = 0;
+    for (const SnapField &f : kSnapFields) o->snap_doubles += (size_t)(f.plus ? h->Fp : h->F);
+    auto bail = [&](hipError_t e) {
+        g_create_error = std::string("m70o_create: ") + hipGetErrorString(e);
+        m70o_destroy(o);
+        return M70H_EHIP;
+    };
+    hipError_t e = hipStreamCreate(&o->copy_stream);
+    if (e != hipSuccess) return bail(e);
+    const long F = h->F;
+    for (int q = 0; q < 6; ++q) {
+        void *m = nullptr;
+        if ((e = hipMalloc(&m, (size_t)F * sizeof(double))) != hipSuccess) return bail(e);
+        o->dev.push_back(m);
+        o->prev[q] = (double *)m;
+    }
+    void *m = nullptr;
+    if ((e = hipMalloc(&m, 6 * sizeof(unsigned long long))) != hipSuccess) return bail(e);
+    o->dev.push_back(m);
+    o->dout = (unsigned long long *)m;
+    // the previous step's fields of the first record: the current ones (:120-127)
+    int rc = wave_finish(w);
+    if (rc) {
+        g_create_error = w->err;
+        m70o_destroy(o);
+        return rc;
+    }
+    const double *cur[6];
+    diff_sources(o, cur);
+    for (int q = 0; q < 6; ++q) {
+        if (cur[q]) e = hipMemcpyAsync(o->prev[q], cur[q], (size_t)F * sizeof(double), hipMemcpyDeviceToDevice, h->stream);
+        else e = hipMemsetAsync(o->prev[q], 0, (size_t)F * sizeof(double), h->stream);
+        if (e != hipSuccess) return bail(e);
+    }
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return bail(e);
+    *out = o;
+    return M70H_OK;
+}
+
+void m70o_destroy(m70o *o)
+{
+    if (!o) return;
+    if (o->h && o->h->stream) (void)hipStreamSynchronize(o->h->stream);
+    if (o->copy_stream) (void)hipStreamSynchronize(o->copy_stream);
+    for (auto &sl : o->slots) {
+        if (sl.dev) (void)hipFree(sl.dev);
+        if (sl.host) (void)hipHostFree(sl.host);
+        if (sl.copied) (void)hipEventDestroy(sl.copied);
+    }
+    for (void *q : o->dev) (void)hipFree(q);
+    if (o->rec) (void)hipHostFree(o->rec);
+    if (o->copy_stream) (void)hipStreamDestroy(o->copy_stream);
+    delete o;
+}
+
+int m70o_run(m70o *o, double stopTime, long *steps)
+{
+    if (!o) return M70H_EINVAL;
+    const m70o_params &p = o->p;
+    // the step count, from the same time arithmetic (time = time + dt while time < stopTime)
+    long n = 0;
+    for (double t = o->time; t < stopTime; t = t + p.dt) ++n;
+    int rc = ensure_records(o, o->rec_n + n);
+    if (rc) return rc;
+    m70h *h = o->h;
+    for (long it = 0; it < n; ++it) {
+        const double time = o->time;
+        // warm-up ramp of the offshore height (:133-139)
+        double waveFactor = (1 + std::tanh((5 / p.warmTime) * (time - p.warmTime / 2))) / 2;
+        if (p.warmTime <= 0) waveFactor = 1;
+        const double a = o->H_off0, b = waveFactor * p.H_off;
+        const double H_off_t = (a < b) ? b : a;   // std::max
+        if ((rc = m70w_calculate_async(o->w, H_off_t, p.Theta_off, p.T_off))) return ofail(o, rc, o->w->err);
+        const bool updateBottom = (time >= p.bedStartTime && time < p.bedStopTime) && p.calculate_bed_evolution;
+        const int hydroIterations = updateBottom ? p.runIterationsInBedEvolution : p.runIterations;
+        // the forcing also carries Dw to the sediment, which reads wave->Dw in the reference
+        if ((rc = m70w_push_forcing(o->w))) return ofail(o, rc, o->w->err);
+        if (p.calculate_hydro) {
+            if ((rc = m70h_run_async(h, hydroIterations))) return ofail(o, rc, h->err);
+            if (p.wave_current_interaction && (rc = m70w_set_uv_from_hydro(o->w))) return ofail(o, rc, o->w->err);
+        }
+        if (p.calculate_sediment && o->s && (rc = m70s_calculate_fluxes_async(o->s))) return ofail(o, rc, o->s->err);
+        double saveInterval = p.saveIntervalHydro;
+        if (updateBottom) {
+            if (o->s && (rc = m70s_update_bottom_async(o->s, p.dt))) return ofail(o, rc, o->s->err);
+            saveInterval = p.saveIntervalMorfo;
+        }
+        o->time = time + p.dt;
+        // largest changes since the previous step (:184-199), on the device
+        OuterDiffArgs da{};
+        diff_sources(o, da.cur);
+        for (int q = 0; q < 6; ++q) da.prev[q] = o->prev[q];
+        da.n = h->F;
+        da.out = o->dout;
+        OHIPCHK(o, launch_outer_diff(h->stream, da));
+        double *r = o->rec + 8 * o->rec_n;
+        r[0] = o->time;
+        r[1] = H_off_t;
+        OHIPCHK(o, hipMemcpyAsync(r + 2, o->dout, 6 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        o->rec_n++;
+        // snapshots at the save times (:205-214)
+        if (saveInterval > 0 && saveInterval * std::floor(o->time / saveInterval) > o->lastSavedTime) {
+            if ((rc = take_snapshot(o))) return rc;
+            o->lastSavedTime = saveInterval * std::floor(o->time / saveInterval);
+        }
+    }
+    if (steps) *steps = n;
+    return M70H_OK;
+}
+
+int m70o_time(m70o *o, double *time)
+{
+    if (!o || !time) return M70H_EINVAL;
+    *time = o->time;
+    return M70H_OK;
+}
+
+int m70o_records(m70o *o, long *n, double *out)
+{
+    if (!o || !n) return M70H_EINVAL;
+    OHIPCHK(o, hipStreamSynchronize(o->h->stream));
+    if (!out) {
+        *n = o->rec_n;
+        return M70H_OK;
+    }
+    const long k = *n < o->rec_n ? *n : o->rec_n;
+    std::memcpy(out, o->rec, (size_t)k * 8 * sizeof(double));
+    *n = k;
+    return M70H_OK;
+}
+
+int m70o_snapshot_count(m70o *o, long *total)
+{
+    if (!o || !total) return M70H_EINVAL;
+    *total = o->snap_total;
+    return M70H_OK;
+}
+
+int m70o_snapshot(m70o *o, long i, double *time, const char *field, double *out)
+{
+    if (!o || !field) return M70H_EINVAL;
+    if (i < 0 || i >= o->snap_total || i < o->snap_total - o->max_snap)
+        return ofail(o, M70H_EINVAL, "snapshot index out of the kept range");
+    m70o::Slot &sl = o->slots[(int)(i % o->max_snap)];
+    OHIPCHK(o, hipEventSynchronize(sl.copied));
+    if (time) *time = sl.time;
+    size_t off = 0;
+    for (const SnapField &f : kSnapFields) {
+        const size_t len = (size_t)(f.plus ? o->h->Fp : o->h->F);
+        if (std::strcmp(f.name, field) == 0) {
+            if (out) std::memcpy(out, sl.host + off, len * sizeof(double));
+            return M70H_OK;
+        }
+        off += len;
+    }
+    return ofail(o, M70H_EINVAL, std::string("unknown snapshot field ") + field);
 }
 
 }  // extern "C"
