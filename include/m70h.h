@@ -356,6 +356,40 @@ int m70o_snapshot_count(m70o *o, long *total);
 int m70o_snapshot(m70o *o, long i, double *time, const char *field, double *out);
 
 /* ------------------------------------------------------------------------------------
+ * Files: NetCDF classic I/O in place of class MorfoIO::Netcdf (include/Netcdf.h,
+ * src/Netcdf.cpp), which wraps netcdf-c. Files hold dimensions x (nx) and y (ny), a
+ * global attribute `time`, and double variables over (x, y) or (y). The reader also takes
+ * the other numeric classic types; record (unlimited) variables are not supported.
+ * ------------------------------------------------------------------------------------ */
+/* Netcdf::ReadDims (src/Netcdf.cpp:39-50). */
+int m70nc_dims(const char *path, int *nx, int *ny);
+/* Netcdf::ReadVar / ReadVarY / ReadXY (:52-97): variable `var` as doubles; n must be its
+ * element count (nx*ny, ny or nx). */
+int m70nc_read(const char *path, const char *var, double *out, long n);
+/* Netcdf::ReadAtt (:79-82): a global attribute's first value. */
+int m70nc_read_att(const char *path, const char *att, double *value);
+/* Netcdf::WriteXY / WriteVar / WriteVarY / WriteAtt (:99-140) in one call: dimensions and
+ * variables x, y, then nvars variables over (x, y) when rows[q] == nx or (y) when
+ * rows[q] == 1, then the global attributes. The file is CDF-1, or CDF-2 (64-bit offsets)
+ * when it passes 2 GiB. */
+int m70nc_write(const char *path, int nx, int ny, const double *x, const double *y, int nvars,
+                const char *const *names, const double *const *data, const int *rows, int natts,
+                const char *const *att_names, const double *att_values);
+/* Message of the last failed m70nc_* or m70o_save_snapshot / m70_resume call on this thread. */
+const char *m70nc_last_error(void);
+/* Morfo70::saveTimeStep (src/Morfo70.cpp:225-276) of snapshot i of an outer loop: the same
+ * variables in the same order; x, y, Z (host) are the grid's and the bathymetry's. */
+int m70o_save_snapshot(m70o *o, long i, const char *path, const double *x, const double *y, const double *Z);
+/* Morfo70::resume's reads (src/Morfo70.cpp:298-312) from one m70_*.nc file into the
+ * handles: bathymetry D and h (Bathymetry::setBathyFromResumeFile; Zb is not re-read,
+ * as there), wave E, H, Theta and, with the wave-current interaction, U, V and U_w
+ * (Wave::setWaveEnergyFromResumeFile, ::setUVFromResumeFile), the hydro state
+ * (Hydro::setHydroFromResumeFile), the sediment fluxes and their dh_dt
+ * (Sediment::setSedimentFluxesFromResumeFile); *time = the time attribute. w and s may
+ * be NULL. */
+int m70_resume(const char *path, m70h *h, m70w *w, m70s *s, int wave_current_interaction, double *time);
+
+/* ------------------------------------------------------------------------------------
  * Adjoint of Run. No counterpart in the reference, which has no derivatives: these
  * entry points serve callers that fit bathymetry or friction to observations (gradient
  * of a loss of the final state with respect to the initial state and the drag and

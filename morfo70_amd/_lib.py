@@ -89,6 +89,16 @@ SIGNATURES = {
     "m70o_records": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_long), _dp]),
     "m70o_snapshot_count": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_long)]),
     "m70o_snapshot": (ctypes.c_int, [_vp, ctypes.c_long, _dp, ctypes.c_char_p, _dp]),
+    "m70nc_dims": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "m70nc_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _dp, ctypes.c_long]),
+    "m70nc_read_att": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _dp]),
+    "m70nc_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int,
+                                   ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(_dp),
+                                   ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.POINTER(ctypes.c_char_p),
+                                   _dp]),
+    "m70nc_last_error": (ctypes.c_char_p, []),
+    "m70o_save_snapshot": (ctypes.c_int, [_vp, ctypes.c_long, ctypes.c_char_p, _dp, _dp, _dp]),
+    "m70_resume": (ctypes.c_int, [ctypes.c_char_p, _vp, _vp, _vp, ctypes.c_int, _dp]),
     "m70h_adjoint_forward": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "m70h_adjoint_reverse": (ctypes.c_int, [_vp, _vp, _vp, _dp]),
     "m70h_adjoint_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),

@@ -14,6 +14,7 @@
 #include "hydro_device.h"
 #include "hydro_kernels.h"
 #include "sediment.h"
+#include "netcdf3.h"
 #include "outer.h"
 #include "wave.h"
 
@@ -2247,6 +2248,224 @@ int m70o_snapshot(m70o *o, long i, double *time, const char *field, double *out)
         off += len;
     }
     return ofail(o, M70H_EINVAL, std::string("unknown snapshot field ") + field);
+}
+
+}  // extern "C"
+
+// =====================================================================================
+// Files (class MorfoIO::Netcdf): reading, writing, snapshots, resume
+// =====================================================================================
+namespace {
+
+thread_local std::string g_nc_error;
+
+int ncfail(const std::string &msg)
+{
+    g_nc_error = msg;
+    return M70H_EINVAL;
+}
+
+bool nc_var(const m70nc::File &f, const char *name, size_t n, std::vector<double> &out)
+{
+    std::string err;
+    if (!m70nc::read_var(f, name, out, err)) {
+        g_nc_error = err;
+        return false;
+    }
+    if (out.size() != n) {
+        g_nc_error = f.path + ": variable " + name + " has " + std::to_string(out.size()) + " values, expected " +
+                     std::to_string(n);
+        return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *m70nc_last_error(void) { return g_nc_error.c_str(); }
+
+int m70nc_dims(const char *path, int *nx, int *ny)
+{
+    if (!path || !nx || !ny) return ncfail("path, nx and ny are required");
+    m70nc::File f;
+    std::string err;
+    if (!m70nc::read_header(path, f, err)) return ncfail(err);
+    long x = -1, y = -1;
+    for (auto &d : f.dims) {
+        if (d.first == "x") x = (long)d.second;
+        if (d.first == "y") y = (long)d.second;
+    }
+    if (x < 0 || y < 0) return ncfail(std::string(path) + ": dimensions x and y are required");
+    *nx = (int)x;
+    *ny = (int)y;
+    return M70H_OK;
+}
+
+int m70nc_read(const char *path, const char *var, double *out, long n)
+{
+    if (!path || !var || !out) return ncfail("path, var and out are required");
+    m70nc::File f;
+    std::string err;
+    if (!m70nc::read_header(path, f, err)) return ncfail(err);
+    std::vector<double> v;
+    if (!nc_var(f, var, (size_t)n, v)) return M70H_EINVAL;
+    std::memcpy(out, v.data(), (size_t)n * sizeof(double));
+    return M70H_OK;
+}
+
+int m70nc_read_att(const char *path, const char *att, double *value)
+{
+    if (!path || !att || !value) return ncfail("path, att and value are required");
+    m70nc::File f;
+    std::string err;
+    if (!m70nc::read_header(path, f, err) || !m70nc::read_att(f, att, *value, err)) return ncfail(err);
+    return M70H_OK;
+}
+
+int m70nc_write(const char *path, int nx, int ny, const double *x, const double *y, int nvars,
+                const char *const *names, const double *const *data, const int *rows, int natts,
+                const char *const *att_names, const double *att_values)
+{
+    if (!path || nx < 1 || ny < 1 || !x || !y) return ncfail("path, nx, ny, x and y are required");
+    if (nvars > 0 && (!names || !data || !rows)) return ncfail("names, data and rows are required");
+    if (natts > 0 && (!att_names || !att_values)) return ncfail("attribute names and values are required");
+    std::vector<std::pair<std::string, uint64_t>> dims = {{"x", (uint64_t)nx}, {"y", (uint64_t)ny}};
+    std::vector<m70nc::OutVar> vars = {{"x", {0}, x}, {"y", {1}, y}};
+    for (int q = 0; q < nvars; ++q) {
+        if (!names[q] || !data[q]) return ncfail("NULL variable name or data");
+        if (rows[q] == nx) vars.push_back({names[q], {0, 1}, data[q]});
+        else if (rows[q] == 1) vars.push_back({names[q], {1}, data[q]});
+        else return ncfail(std::string("variable ") + names[q] + ": rows must be nx or 1");
+    }
+    std::vector<std::pair<std::string, double>> atts;
+    for (int q = 0; q < natts; ++q) atts.emplace_back(att_names[q], att_values[q]);
+    std::string err;
+    if (!m70nc::write_file(path, dims, atts, vars, false, err)) return ncfail(err);
+    return M70H_OK;
+}
+
+int m70o_save_snapshot(m70o *o, long i, const char *path, const double *x, const double *y, const double *Z)
+{
+    if (!o || !path || !x || !y || !Z) return ncfail("o, path, x, y and Z are required");
+    const int nx = o->h->nx, ny = o->h->ny;
+    const size_t F = (size_t)nx * ny, Fp = F + ny;
+    auto get = [&](const char *name, size_t n, std::vector<double> &v) {
+        v.resize(n);
+        return m70o_snapshot(o, i, nullptr, name, v.data());
+    };
+    double t = 0.0;
+    int rc = m70o_snapshot(o, i, &t, "Zs", nullptr);
+    if (rc) return ncfail(o->err);
+    const char *fnames[] = {"h", "D", "H", "E", "Cg", "lK", "dQx_dx", "dQy_dy", "Theta", "Dw", "E_r", "U", "V",
+                            "Zs", "qx", "qy", "Gamma_Swash", "Alpha", "gamma"};
+    std::vector<std::vector<double>> fv(sizeof(fnames) / sizeof(fnames[0]));
+    for (size_t q = 0; q < fv.size(); ++q)
+        if ((rc = get(fnames[q], F, fv[q]))) return ncfail(o->err);
+    std::vector<double> Uf, Vf, Qxf, Qyf;
+    if ((rc = get("U_full", Fp, Uf)) || (rc = get("V_full", Fp, Vf)) || (rc = get("Qx_full", Fp, Qxf)) ||
+        (rc = get("Qy_full", Fp, Qyf)))
+        return ncfail(o->err);
+    auto f = [&](const char *n) -> const double * {
+        for (size_t q = 0; q < fv.size(); ++q)
+            if (std::strcmp(fnames[q], n) == 0) return fv[q].data();
+        return nullptr;
+    };
+    // Morfo70::saveTimeStep's variables, in its order
+    std::vector<m70nc::OutVar> vars = {{"x", {0}, x}, {"y", {1}, y}};
+    auto v2 = [&](const char *n, const double *p) { vars.push_back({n, {0, 1}, p}); };
+    auto v1 = [&](const char *n, const double *p) { vars.push_back({n, {1}, p}); };
+    v2("Z", Z);
+    v2("h", f("h"));
+    v2("D", f("D"));
+    v2("H", f("H"));
+    v2("E", f("E"));
+    v2("Cg", f("Cg"));
+    v2("lK", f("lK"));
+    v2("dQx_dx", f("dQx_dx"));
+    v2("dQy_dy", f("dQy_dy"));
+    v2("Theta", f("Theta"));
+    v2("Dw", f("Dw"));
+    v2("E_r", f("E_r"));
+    v2("U", f("U"));
+    v1("U0", Uf.data());
+    v1("Qx0", Qxf.data());
+    v2("U_x", Uf.data() + ny);
+    v2("Qx_x", Qxf.data() + ny);
+    v2("V", f("V"));
+    v1("V0", Vf.data());
+    v1("Qy0", Qyf.data());
+    v2("V_y", Vf.data() + ny);
+    v2("Qy_y", Qyf.data() + ny);
+    v2("Zs", f("Zs"));
+    v2("qx", f("qx"));
+    v2("qy", f("qy"));
+    v2("Gamma_Swash", f("Gamma_Swash"));
+    v2("Alpha", f("Alpha"));
+    v2("gamma", f("gamma"));
+    std::string err;
+    if (!m70nc::write_file(path, {{"x", (uint64_t)nx}, {"y", (uint64_t)ny}}, {{"time", t}}, vars, false, err))
+        return ncfail(err);
+    return M70H_OK;
+}
+
+int m70_resume(const char *path, m70h *h, m70w *w, m70s *s, int wave_current_interaction, double *time)
+{
+    if (!path || !h) return ncfail("path and hydro are required");
+    m70nc::File f;
+    std::string err;
+    if (!m70nc::read_header(path, f, err)) return ncfail(err);
+    const int nx = h->nx, ny = h->ny;
+    const size_t F = (size_t)nx * ny, Fp = F + ny;
+    std::vector<double> D, hb, Uc, Vc, Zs, dqx, dqy, U0, Qx0, V0, Qy0, Ux, Qxx, Vy, Qyy;
+    if (!nc_var(f, "D", F, D) || !nc_var(f, "U", F, Uc) || !nc_var(f, "V", F, Vc) || !nc_var(f, "Zs", F, Zs) ||
+        !nc_var(f, "dQx_dx", F, dqx) || !nc_var(f, "dQy_dy", F, dqy) || !nc_var(f, "U0", ny, U0) ||
+        !nc_var(f, "Qx0", ny, Qx0) || !nc_var(f, "V0", ny, V0) || !nc_var(f, "Qy0", ny, Qy0) ||
+        !nc_var(f, "U_x", F, Ux) || !nc_var(f, "Qx_x", F, Qxx) || !nc_var(f, "V_y", F, Vy) ||
+        !nc_var(f, "Qy_y", F, Qyy))
+        return M70H_EINVAL;
+    // Hydro::setHydroFromResumeFile (src/Hydro.cpp:204-227): row 0 and rows 1..nx apart
+    auto join = [&](const std::vector<double> &r0, const std::vector<double> &rest) {
+        std::vector<double> v(Fp);
+        std::memcpy(v.data(), r0.data(), ny * sizeof(double));
+        std::memcpy(v.data() + ny, rest.data(), F * sizeof(double));
+        return v;
+    };
+    std::vector<double> U = join(U0, Ux), Qx = join(Qx0, Qxx), V = join(V0, Vy), Qy = join(Qy0, Qyy);
+    int rc = m70h_set_state(h, Zs.data(), D.data(), Qx.data(), Qy.data(), U.data(), V.data(), Uc.data(), Vc.data(),
+                            dqx.data(), dqy.data());
+    if (rc) return ncfail(h->err);
+    if (s) {   // Bathymetry::setBathyFromResumeFile (h; Zb stays) and the sediment fluxes
+        std::vector<double> qx, qy;
+        if (!nc_var(f, "h", F, hb) || !nc_var(f, "qx", F, qx) || !nc_var(f, "qy", F, qy)) return M70H_EINVAL;
+        if ((rc = m70s_set_bed(s, hb.data(), nullptr)) || (rc = m70s_set_fluxes(s, qx.data(), qy.data())))
+            return ncfail(s->err);
+    }
+    if (w) {   // Wave::setWaveEnergyFromResumeFile, ::setUVFromResumeFile (src/Wave.cpp:305-336)
+        std::vector<double> E, H, Th;
+        if (!nc_var(f, "E", F, E) || !nc_var(f, "H", F, H) || !nc_var(f, "Theta", F, Th)) return M70H_EINVAL;
+        if ((rc = wave_finish(w))) return ncfail(w->err);
+        const size_t nb = F * sizeof(double);
+        hipError_t e = hipMemcpyAsync(w->a.f.E, E.data(), nb, hipMemcpyHostToDevice, w->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(w->a.f.H, H.data(), nb, hipMemcpyHostToDevice, w->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(w->a.f.Theta, Th.data(), nb, hipMemcpyHostToDevice, w->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+        if (e != hipSuccess) return ncfail(std::string("m70_resume: ") + hipGetErrorString(e));
+        if (wave_current_interaction) {
+            std::vector<double> Uw(F);
+            for (size_t q = 0; q < F; ++q) Uw[q] = std::cos(Th[q]) * Uc[q] + std::sin(Th[q]) * Vc[q];
+            e = hipMemcpyAsync(w->U, Uc.data(), nb, hipMemcpyHostToDevice, w->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(w->V, Vc.data(), nb, hipMemcpyHostToDevice, w->stream);
+            if (e == hipSuccess) e = hipMemcpyAsync(w->U_w, Uw.data(), nb, hipMemcpyHostToDevice, w->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+            if (e != hipSuccess) return ncfail(std::string("m70_resume: ") + hipGetErrorString(e));
+        }
+    }
+    double t = 0.0;
+    if (!m70nc::read_att(f, "time", t, err)) return ncfail(err);
+    if (time) *time = t;
+    return M70H_OK;
 }
 
 }  // extern "C"

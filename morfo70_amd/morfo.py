@@ -115,3 +115,27 @@ class Morfo70Run:
         self._chk(self._L.m70o_snapshot(self._o, int(i), ctypes.byref(t), field.encode(),
                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
         return t.value, out
+
+
+def save_snapshot(run: Morfo70Run, i: int, path: str, x, y, Z):
+    """Morfo70::saveTimeStep (src/Morfo70.cpp:225-276) of snapshot i to an m70_*.nc file."""
+    L = run._L
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, Z)]
+    rc = L.m70o_save_snapshot(run._o, int(i), str(path).encode(),
+                              *[a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for a in arrs])
+    if rc != _lib.M70H_OK:
+        raise RuntimeError(f"m70o_save_snapshot: {L.m70nc_last_error().decode()} (rc={rc})")
+
+
+def resume(path: str, hydro, wave=None, sediment=None, wave_current_interaction: bool = False) -> float:
+    """Morfo70::resume's reads (src/Morfo70.cpp:298-312) into the device handles; returns
+    the file's time. The hydro's host mirrors are refreshed."""
+    L = _lib.load()
+    t = ctypes.c_double()
+    rc = L.m70_resume(str(path).encode(), hydro.handle, wave.handle if wave is not None else None,
+                      sediment._s if sediment is not None else None, int(bool(wave_current_interaction)),
+                      ctypes.byref(t))
+    if rc != _lib.M70H_OK:
+        raise RuntimeError(f"m70_resume: {L.m70nc_last_error().decode()} (rc={rc})")
+    hydro.pull()
+    return t.value

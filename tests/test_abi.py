@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_symbols():
     txt = open(os.path.join(ROOT, "include", "m70h.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(m70[hswo]_[a-z_0-9]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(m70[hswo]?(?:nc)?_[a-z_0-9]+)\s*\(", txt)))
 
 
 def test_header_matches_binding():
