@@ -83,7 +83,7 @@ def parse():
     ap.add_argument("--tile-cols", type=int, default=0)
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--workload", choices=["tide2048x4096", "batch64", "adjoint512x1024", "inversion16",
-                                           "sediment2048x4096", "wave2048x4096"],
+                                           "sediment2048x4096", "wave2048x4096", "morpho2048x4096"],
                     default="tide2048x4096")
     ap.add_argument("--gd-iters", type=int, default=20)
     ap.add_argument("--slots", type=int, default=0,
@@ -316,6 +316,81 @@ def bench_wave(args):
     print(json.dumps(rec), flush=True)
 
 
+def bench_morpho(args):
+    """The morphodynamic outer loop on the device (Morfo70::Run, src/Morfo70.cpp:131-218) on
+    the 2048 x 4096 tide beach: a "step" is one outer step - wave field, forcing hand-off,
+    hydro Run(1) (runIterations default), sediment fluxes, bed update, the six change maxima;
+    the reference's base_case wave settings, bed evolution on, dt = 0.1 s."""
+    import torch
+    from morfo70_amd import scenarios as S
+    from morfo70_amd.hydro import from_scenario
+    from morfo70_amd.morfo import Morfo70Run, OuterParams
+    from morfo70_amd.sediment import Sediment
+    from morfo70_amd.wave import Wave
+
+    sc = S.seeded_beach(args.nx, args.ny, 2.0, 2.0, args.seed, 2, tide=False,
+                        name=f"beach_{args.nx}x{args.ny}_s{args.seed}")
+    wp, sp = S.WaveParams.base_case(), S.SedimentParams()
+    op = OuterParams(H_off=1.0, T_off=8.0, Theta_off=math.radians(10.0), dt=0.1, calculate_bed_evolution=1)
+    torch.cuda.init()
+    from morfo70_amd import _lib
+    h = from_scenario(sc, solver=_lib.SOLVER_FUSED)
+    w = Wave(h.grid, h.bathy, wp)
+    w.couple(h)
+    sd = Sediment(h, params=sp)
+    run = Morfo70Run(h, w, sd, op)
+    run.run(op.dt * args.warmup - 1e-9)
+    h.synchronize()
+    t0 = time.perf_counter()
+    n = run.run(op.dt * (args.warmup + args.steps) - 1e-9)
+    recs = run.records()                 # waits for the last step
+    dt = time.perf_counter() - t0
+    finite = bool(np.all(np.isfinite(recs)))
+    ms = dt * 1e3 / n
+    run.close()
+    sd.close()
+    w.close()
+    h.close()
+    cells = sc.nx * sc.ny
+    value = n / dt
+    rec = {
+        "metric": "morphodynamic outer steps/s (Morfo70::Run: wave + hydro + sediment + bed, on the device)",
+        "value": value, "unit": "outer-steps/s", "n_gpus": 1, "steps": n, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded bar-trough beach, computed wave forcing)",
+        "config": {"workload": f"morpho{sc.nx}x{sc.ny}", "nx": sc.nx, "ny": sc.ny, "dx": 2.0, "dy": 2.0,
+                   "runIterations": op.runIterations, "bed_evolution": True,
+                   "timed": f"{n} outer steps, host wall clock around m70o_run"},
+        "roofline": {"bound": "hbm", "kernel": "wave_field", "achieved": None, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": None, "traffic": None,
+                     "note": "the outer step is dominated by the wave march (see the wave2048x4096 workload)"},
+        "finite": finite,
+    }
+    if not args.no_cpu:
+        import oracle
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        o = oracle.OracleHydro(sc, threads=threads)
+        ow = oracle.OracleWave(sc.x, sc.ny, sc.dy, wp)
+        osd = oracle.OracleSediment(o, sp)
+        k = max(1, args.cpu_steps or 2)
+        t1 = time.perf_counter()
+        for _ in range(k):
+            ow.set_depth(o.get_state()["D"])
+            ow.calculate(op.H_off, op.Theta_off, op.T_off)
+            o.set_forcing({f: ow.get(f) for f in ("Sxx", "Sxy", "Syy", "Urms", "Dw", "D_r", "H")})
+            o.run(op.runIterations)
+            osd.fluxes()
+            osd.update_bottom(op.dt)
+        cdt = (time.perf_counter() - t1) / k
+        rec["cpu_baseline"] = {"value": 1.0 / cdt, "unit": "outer-steps/s", "cores": threads, "kind": "port",
+                               "sample": f"{sc.nx}x{sc.ny}, {k} outer steps of the oracle's wave, hydro and "
+                                         f"sediment ({threads} OpenMP threads), {cdt:.2f} s per step"}
+        rec["gpu_over_cpu"] = value * cdt
+    else:
+        rec["cpu_baseline"] = None
+    print(json.dumps(rec), flush=True)
+
+
 def bench_batch(args):
     """64 uneven beaches (nx ~ U[128,1024], ny ~ U[256,2048], seeded bathymetry and
     forcing): aggregate cell-steps/s of looping the plain step over them one after the
@@ -508,6 +583,11 @@ def main():
         if args.gpus != 1:
             sys.exit(2)
         bench_adjoint(args)
+        return
+    if args.workload == "morpho2048x4096":
+        if args.gpus != 1:
+            sys.exit(2)
+        bench_morpho(args)
         return
     if args.workload == "wave2048x4096":
         if args.gpus != 1:
