@@ -1748,6 +1748,8 @@ int m70w_create(int nx, int ny, const double *x, double dy, const m70w_params *p
     double **fld = &w->a.f.K;
     for (int q = 0; q < WFIELD_ARRAYS; ++q)
         if ((e = walloc(w, &fld[q], F)) != hipSuccess) return bail(e);
+    for (double **q : {&w->a.disp.lK, &w->a.disp.Sigma, &w->a.disp.C, &w->a.disp.Cg, &w->a.disp.Urms_H})
+        if ((e = walloc(w, q, F)) != hipSuccess) return bail(e);
     for (int r = 0; r < 2; ++r) {
         double *buf = nullptr;
         if ((e = walloc(w, &buf, (size_t)WROW_ARRAYS * ny)) != hipSuccess) return bail(e);
@@ -1838,6 +1840,7 @@ int m70w_calculate_async(m70w *w, double hOff, double thetaOff, double T)
         WHIPCHK(w, hipMemsetAsync(w->a.rows[r].D, 0, (size_t)WROW_ARRAYS * w->ny * sizeof(double), w->stream));
     WHIPCHK(w, hipMemsetAsync(w->cb.acc, 0, 3 * sizeof(unsigned long long) + 3 * sizeof(int), w->stream));
     w->T = T;
+    WHIPCHK(w, launch_wave_dispersion(w->stream, w->a, T));
     WHIPCHK(w, launch_wave_first(w->stream, w->a, hOff, thetaOff, T, w->cb));
     // every field has at least nx-1 steps (one per mesh row); wave_finish adds the rest
     WHIPCHK(w, launch_wave_steps(w->stream, w->a, T, 0, w->nx - 1, w->cb));

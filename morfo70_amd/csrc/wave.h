@@ -22,6 +22,13 @@ struct WField {
 };
 constexpr int WFIELD_ARRAYS = 20;
 
+// Dispersion of every mesh node (nx*ny each), solved for the whole field before the march:
+// a mesh row's depth and current are known up front, only intermediate rows need their
+// own (interpolated) solve.
+struct WDisp {
+    double *lK, *Sigma, *C, *Cg, *Urms_H;
+};
+
 struct WaveArgs {
     int nx, ny;
     double dy, inv_2dy;
@@ -30,6 +37,7 @@ struct WaveArgs {
     const double *D, *U, *V, *U_w;     // depth and currents, nx*ny
     WField f;
     WRow rows[2];
+    WDisp disp;
 };
 
 // Workgroups of WB threads own WNI consecutive nodes of a row (two halo nodes each side).
@@ -58,6 +66,8 @@ struct WaveCtlBufs {
 // ctl[next & 1] to see whether more are needed (intermediate rows).
 hipError_t launch_wave_first(hipStream_t st, const WaveArgs &a, double hOff, double thetaOff, double T,
                              const WaveCtlBufs &b);
+// WaveRow::dispersion (src/WaveParametrizations.cpp:14-60) of every mesh node into a.disp.
+hipError_t launch_wave_dispersion(hipStream_t st, const WaveArgs &a, double T);
 hipError_t launch_wave_steps(hipStream_t st, const WaveArgs &a, double T, int s0, int count, const WaveCtlBufs &b);
 // Wave::setUV (src/Wave.cpp:293-302): U, V copied in; U_w from the stored direction.
 hipError_t launch_wave_set_uv(hipStream_t st, const WaveArgs &a, double *U, double *V, double *U_w,

@@ -274,8 +274,7 @@ __global__ void __launch_bounds__(WB) k_wave_first(WaveArgs a, double hOff, doub
     const m70w_params &p = a.p;
     const double w = 2 * WPI / T;
     if (t == 0) {   // node 0's depth, the absolute frequency and the offshore height (:143-146)
-        double lK, sg, C, Cg, Uh;
-        w_dispersion(a.D[0], a.U_w[0], w, p.z0, lK, sg, C, Cg, Uh);
+        const double Uh = a.disp.Urms_H[0];
         double dw0 = 0, ddw_dh0 = 0, df0 = 0, ddf_dh0 = 0;
         w_breaking(p, hOff, w, a.D[0], dw0, ddw_dh0);
         if (p.bottom_friction) w_friction(p, hOff, w, Uh, df0, ddf_dh0);
@@ -294,7 +293,11 @@ __global__ void __launch_bounds__(WB) k_wave_first(WaveArgs a, double hOff, doub
         n.U = a.U[j];
         n.V = a.V[j];
         n.U_w = a.U_w[j];
-        w_dispersion(n.D, n.U_w, w, p.z0, n.lK, n.Sigma, n.C, n.Cg, n.Urms_H);
+        n.lK = a.disp.lK[j];
+        n.Sigma = a.disp.Sigma[j];
+        n.C = a.disp.C[j];
+        n.Cg = a.disp.Cg[j];
+        n.Urms_H = a.disp.Urms_H[j];
         const double cosTheta = cos(thetaOff);
         const double sinTheta = sin(thetaOff);
         node_dir_terms(n, thetaOff, cosTheta, sinTheta, sinTheta * cosTheta);
@@ -387,15 +390,27 @@ __global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, W
     };
     if (t < nint + 4) {
         init_node(j, n.D, n.U, n.V, n.U_w);
-        w_dispersion(n.D, n.U_w, w, p.z0, n.lK, n.Sigma, n.C, n.Cg, n.Urms_H);
+        if (store) {   // a mesh row: solved before the march, from the same inputs
+            n.lK = a.disp.lK[index + j];
+            n.Sigma = a.disp.Sigma[index + j];
+            n.C = a.disp.C[index + j];
+            n.Cg = a.disp.Cg[index + j];
+            n.Urms_H = a.disp.Urms_H[index + j];
+        } else {
+            w_dispersion(n.D, n.U_w, w, p.z0, n.lK, n.Sigma, n.C, n.Cg, n.Urms_H);
+        }
         sl_lK[t] = n.lK;
         sl_U[t] = n.U;
         sl_V[t] = n.V;
     }
     if (t == 0 && p.bottom_friction && p.friction_model == M70W_FRIC_SOULSBY97) {
         double d0, u0, v0, uw0, lK0, sg0, C0, Cg0, Uh0;   // Sigma of the row's first node
-        init_node(0, d0, u0, v0, uw0);
-        w_dispersion(d0, uw0, w, p.z0, lK0, sg0, C0, Cg0, Uh0);
+        if (store) {
+            sg0 = a.disp.Sigma[index];
+        } else {
+            init_node(0, d0, u0, v0, uw0);
+            w_dispersion(d0, uw0, w, p.z0, lK0, sg0, C0, Cg0, Uh0);
+        }
         s_sigma0 = sg0;
     }
     __syncthreads();
@@ -536,6 +551,14 @@ __global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, W
     }
 }
 
+__global__ void __launch_bounds__(256) k_wave_dispersion(WaveArgs a, double w, long n)
+{
+    const long q = (long)blockIdx.x * 256 + threadIdx.x;
+    if (q >= n) return;
+    w_dispersion(a.D[q], a.U_w[q], w, a.p.z0, a.disp.lK[q], a.disp.Sigma[q], a.disp.C[q], a.disp.Cg[q],
+                 a.disp.Urms_H[q]);
+}
+
 __global__ void __launch_bounds__(256) k_wave_set_uv(double *U, double *V, double *U_w, const double *Usrc,
                                                      const double *Vsrc, const double *cosT, const double *sinT, long n)
 {
@@ -554,6 +577,13 @@ hipError_t launch_wave_first(hipStream_t st, const WaveArgs &a, double hOff, dou
 {
     hipLaunchKernelGGL(k_wave_first, dim3((unsigned)wave_blocks(a.ny)), dim3(WB), 0, st, a, hOff, thetaOff, T, b.ctl,
                        b.acc, b.anyv);
+    return hipGetLastError();
+}
+
+hipError_t launch_wave_dispersion(hipStream_t st, const WaveArgs &a, double T)
+{
+    const long n = (long)a.nx * a.ny;
+    hipLaunchKernelGGL(k_wave_dispersion, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, 2 * WPI / T, n);
     return hipGetLastError();
 }
 
