@@ -12,40 +12,78 @@
 namespace m70 {
 
 // Qx predictor, src/HydroQx.cpp:23-48 and :99-256 with setSolverQxx_BC
-// (src/HydroBoundaryConditions.cpp:25-39). Side outputs: IsWater and DragTerm of row i.
-__host__ __device__ __forceinline__ void asm_qxx(const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                                        const Work &w, double f0, int i, int j, double &a_out, double &b_out,
-                                        double &c_out, double &f_out)
+// (src/HydroBoundaryConditions.cpp:25-39), in two parts: the values element (i, j) reads
+// (QxxIn, qxx_load) and the arithmetic on them (asm_qxx_vals). The column solver keeps a
+// window of rows in registers and fills QxxIn without reloading shared rows; every caller
+// runs the same arithmetic.
+struct QxxIn {
+    double Dm, D0, D1, D2;       // D rows i-1..i+2 (Dm only for i > 0, D2 only for i < nx-2)
+    double Zm, Z0, Z1, Z2;       // Zs rows i-1..i+2 (same rows as D)
+    double Qm, Q0, Q1, Q2;       // dQx_dx rows i-1..i+2 (same rows as D)
+    double Vc0, Vc1;             // V_c rows i, i+1
+    double U0, U1, U2;           // U rows i, i+1, i+2 (U2 only for i < nx-2)
+    double dqy0, dqy1;           // dQy_dy rows i, i+1
+    double v1j, v1m, v2j, v2m;   // V rows i+1, i+2 at columns j and j-1 (periodic)
+    double Qx1;                  // Qx row i+1
+    double cD, Urms, Fx, dnuDx_dy, nuD_x, dnuD_dx;   // run fields, row i
+};
+
+__host__ __device__ __forceinline__ QxxIn qxx_load(const Consts &k, const State &s, const RunFields &rf, int i, int j)
 {
-    const int ny = k.ny;
+    const int ny = k.ny, nx = k.nx;
     const long r = (long)i * ny;
-    const int nx = k.nx;
     const int jm = wrap_dec(j, ny);
+    QxxIn q{};
+    q.D0 = s.D[r + j]; q.D1 = s.D[r + ny + j];
+    q.Z0 = s.Zs[r + j]; q.Z1 = s.Zs[r + ny + j];
+    q.Q0 = s.dQx[r + j]; q.Q1 = s.dQx[r + ny + j];
+    if (i > 0) {
+        q.Dm = s.D[r - ny + j]; q.Zm = s.Zs[r - ny + j]; q.Qm = s.dQx[r - ny + j];
+    }
+    if (i < nx - 2) {
+        q.D2 = s.D[r + 2L * ny + j]; q.Z2 = s.Zs[r + 2L * ny + j]; q.Q2 = s.dQx[r + 2L * ny + j];
+        q.U2 = s.U[r + 2L * ny + j];
+    }
+    q.Vc0 = s.Vc[r + j]; q.Vc1 = s.Vc[r + ny + j];
+    q.U0 = s.U[r + j]; q.U1 = s.U[r + ny + j];
+    q.dqy0 = s.dQy[r + j]; q.dqy1 = s.dQy[r + ny + j];
+    q.v1j = s.V[r + ny + j]; q.v1m = s.V[r + ny + jm];
+    q.v2j = s.V[r + 2L * ny + j]; q.v2m = s.V[r + 2L * ny + jm];
+    q.Qx1 = s.Qx[r + ny + j];
+    q.cD = rf.cD_x[r + j]; q.Urms = rf.Urms_x[r + j]; q.Fx = rf.Fx[r + j];
+    q.dnuDx_dy = rf.dnuDx_dy[r + j]; q.nuD_x = rf.nuD_x[r + j]; q.dnuD_dx = rf.dnuD_dx[r + j];
+    return q;
+}
+
+// Side outputs: IsWater (wi) and DragTerm (drag) of row i.
+__host__ __device__ __forceinline__ void asm_qxx_vals(const Consts &k, const Metrics &m, const QxxIn &q, double f0,
+                                                      int i, double &a_out, double &b_out, double &c_out,
+                                                      double &f_out, int &wi_out, double &drag_out)
+{
+    const int nx = k.nx;
     const double dt = k.dt;
-    const double *D = s.D, *Zs = s.Zs, *dQx = s.dQx;
-    const double Di = D[r + j], Di1 = D[r + ny + j];
+    const double Di = q.D0, Di1 = q.D1;
     // getD_x(i): Di row i+1 (:403-418)
     const double Dx1 = 0.5 * (Di + Di1);
     const double invDx1 = 1 / Dx1;
     // setWaterNodes_x(i) (:231-256)
-    const int wi = wet_rule(Di, Di1, Zs[r + j], Zs[r + ny + j], dQx[r + j], dQx[r + ny + j], k.Dmask);
+    const int wi = wet_rule(Di, Di1, q.Z0, q.Z1, q.Q0, q.Q1, k.Dmask);
     // Ui_j = C2X(V_c) (:23, src/HydroMeshOperations.cpp:15-28) and drag (:32,42)
-    const double vx = 0.5 * (s.Vc[r + j] + s.Vc[r + ny + j]);
-    const double u1 = s.U[r + ny + j];
-    const double drag = drag_term(rf.cD_x[r + j], rf.Urms_x[r + j], u1, vx, invDx1, k);
+    const double vx = 0.5 * (q.Vc0 + q.Vc1);
+    const double u1 = q.U1;
+    const double drag = drag_term(q.cD, q.Urms, u1, vx, invDx1, k);
     // setSolverQxx_f (:99-133)
     double fv;
     {
         const double inv_dx = m.inv_dx[i];
-        const double *zs = Zs + r, *dqy_dy = s.dQy + r, *v = s.V + r + ny;
-        const double dzs_dx = inv_dx * (zs[ny + j] - zs[j]);
-        const double d2qy_dxy = inv_dx * (dqy_dy[ny + j] - dqy_dy[j]);
-        const double dV_dx = 0.5 * inv_dx * ((v[ny + j] + v[ny + jm]) - (v[j] + v[jm]));
-        const double d2V_dxy = inv_dx * k.inv_dy * ((v[ny + j] - v[ny + jm]) - (v[j] - v[jm]));
-        fv = wi * (rf.Fx[r + j] + s.Qx[r + ny + j] * k.inv_dt + G * Dx1 * (dt * d2qy_dxy - dzs_dx) +
-                   rf.dnuDx_dy[r + j] * dV_dx + rf.nuD_x[r + j] * d2V_dxy);
+        const double dzs_dx = inv_dx * (q.Z1 - q.Z0);
+        const double d2qy_dxy = inv_dx * (q.dqy1 - q.dqy0);
+        const double dV_dx = 0.5 * inv_dx * ((q.v2j + q.v2m) - (q.v1j + q.v1m));
+        const double d2V_dxy = inv_dx * k.inv_dy * ((q.v2j - q.v2m) - (q.v1j - q.v1m));
+        fv = wi * (q.Fx + q.Qx1 * k.inv_dt + G * Dx1 * (dt * d2qy_dxy - dzs_dx) + q.dnuDx_dy * dV_dx +
+                   q.nuD_x * d2V_dxy);
     }
-    const double dnud_dx = rf.dnuD_dx[r + j], nud_x = rf.nuD_x[r + j];
+    const double dnud_dx = q.dnuD_dx, nud_x = q.nuD_x;
     // setSolverQxx_d (:135-162)
     const int isl = !wi;
     double dv = (k.inv_dt + drag + m.difx_x_d[i] * (u1 - 2 * dnud_dx * invDx1) -
@@ -55,28 +93,27 @@ __host__ __device__ __forceinline__ void asm_qxx(const Consts &k, const Metrics 
     if (i == 0) {
         const double Dx0 = 1.5 * Di - 0.5 * Di1;     // getD_x0 (:390-401)
         const double invDx0 = 1.0 / Dx0;
-        lv = (m.difx_x_low[0] * (s.U[j] - 2 * dnud_dx * invDx0) -
+        lv = (m.difx_x_low[0] * (q.U0 - 2 * dnud_dx * invDx0) -
               m.dif2x_x_low[0] * (G * Dx1 * dt + 2 * nud_x * invDx0));
     } else {
-        const double Dm = D[r - ny + j];
+        const double Dm = q.Dm;
         const double Dxi = 0.5 * (Dm + Di);
         const double invDxi = 1 / Dxi;
-        const int wm = wet_rule(Dm, Di, Zs[r - ny + j], Zs[r + j], dQx[r - ny + j], dQx[r + j], k.Dmask);
+        const int wm = wet_rule(Dm, Di, q.Zm, q.Z0, q.Qm, q.Q0, k.Dmask);
         const double ww = wi && wm;
-        lv = (m.difx_x_low[i] * (s.U[r + j] - 2 * dnud_dx * invDxi) -
+        lv = (m.difx_x_low[i] * (q.U0 - 2 * dnud_dx * invDxi) -
               m.dif2x_x_low[i] * (G * Dx1 * dt + 2 * nud_x * invDxi)) * ww;
     }
     // setSolverQxx_up (:164-188). Row nx-2's up is never read by the Thomas solve
     // (src/Math.cpp:38-44 uses c rows 0..n-2), so it is left zero.
     double uv = 0.0;
     if (i < nx - 2) {
-        const double Dp = D[r + 2L * ny + j];
+        const double Dp = q.D2;
         const double Dx2 = 0.5 * (Di1 + Dp);
         const double invDx2 = 1 / Dx2;
-        const int wp = wet_rule(Di1, Dp, Zs[r + ny + j], Zs[r + 2L * ny + j], dQx[r + ny + j],
-                                dQx[r + 2L * ny + j], k.Dmask);
+        const int wp = wet_rule(Di1, Dp, q.Z1, q.Z2, q.Q1, q.Q2, k.Dmask);
         const double ww = wi && wp;
-        uv = (m.difx_x_up[i] * (s.U[r + 2L * ny + j] - 2 * dnud_dx * invDx2) -
+        uv = (m.difx_x_up[i] * (q.U2 - 2 * dnud_dx * invDx2) -
               m.dif2x_x_up[i] * (G * Dx1 * dt + 2 * nud_x * invDx2)) * ww;
     }
     if (i == 0) {   // setSolverQxx_BC
@@ -87,8 +124,20 @@ __host__ __device__ __forceinline__ void asm_qxx(const Consts &k, const Metrics 
     b_out = lv;
     c_out = uv;
     f_out = fv;
-    w.w[r + j] = (uint8_t)wi;
-    w.drag[r + j] = drag;
+    wi_out = wi;
+    drag_out = drag;
+}
+
+__host__ __device__ __forceinline__ void asm_qxx(const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
+                                        const Work &w, double f0, int i, int j, double &a_out, double &b_out,
+                                        double &c_out, double &f_out)
+{
+    int wi;
+    double drag;
+    asm_qxx_vals(k, m, qxx_load(k, s, rf, i, j), f0, i, a_out, b_out, c_out, f_out, wi, drag);
+    const long e = (long)i * k.ny + j;
+    w.w[e] = (uint8_t)wi;
+    w.drag[e] = drag;
 }
 
 // Qx projector, setSolverQxy_f/_d/_up/_low, src/HydroQx.cpp:262-349.

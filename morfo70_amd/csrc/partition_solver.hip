@@ -17,6 +17,8 @@
 // The matrices are diagonally dominant (the 1/dt term), so no pivoting is needed. The
 // rounding differs from the serial recurrences of the reference; results agree to a few
 // ulps per solve.
+#include <type_traits>
+
 #include "assembly.h"
 #include "hydro_device.h"
 #include "hydro_kernels.h"
@@ -58,10 +60,66 @@ struct ProvQxy {      // Qx projector, lines along y (rows): line = i, k = j
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qxy(k, s, rf, w, line, kk, a, b, c, f); }
 };
+constexpr int CM = 8;   // rows per column segment in the three-pass column solver
+
+// Column segment i = k0..k0+mm-1 of the Qx predictor at column j (diagonal di, lower lo,
+// upper up, right-hand side rh), with the rows the elements share held in a register
+// window: each element loads only the rows no earlier element of the segment loaded
+// (15 values instead of about 35). Rows of the nx-row fields past nx-1 are read only
+// where asm_qxx_vals does not use them, so they are clamped to row nx-1.
+__device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, const State &s, const RunFields &rf,
+                                            const Work &w, double f0, int j, int k0, int mm, double (&lo)[CM],
+                                            double (&di)[CM], double (&up)[CM], double (&rh)[CM])
+{
+    const int ny = k.ny, nx = k.nx;
+    const int jm = wrap_dec(j, ny);
+    auto at = [&](int i) { return (long)i * ny + j; };
+    auto atm = [&](int i) { return (long)i * ny + jm; };
+    auto cl = [&](int i) { return i < nx ? i : nx - 1; };
+    QxxIn q;
+    {
+        const int i = k0, im = i > 0 ? i - 1 : 0, i2 = cl(i + 2);
+        q.Dm = s.D[at(im)]; q.D0 = s.D[at(i)]; q.D1 = s.D[at(i + 1)]; q.D2 = s.D[at(i2)];
+        q.Zm = s.Zs[at(im)]; q.Z0 = s.Zs[at(i)]; q.Z1 = s.Zs[at(i + 1)]; q.Z2 = s.Zs[at(i2)];
+        q.Qm = s.dQx[at(im)]; q.Q0 = s.dQx[at(i)]; q.Q1 = s.dQx[at(i + 1)]; q.Q2 = s.dQx[at(i2)];
+        q.Vc0 = s.Vc[at(i)]; q.Vc1 = s.Vc[at(i + 1)];
+        q.U0 = s.U[at(i)]; q.U1 = s.U[at(i + 1)]; q.U2 = s.U[at(i + 2)];
+        q.dqy0 = s.dQy[at(i)]; q.dqy1 = s.dQy[at(i + 1)];
+        q.v1j = s.V[at(i + 1)]; q.v1m = s.V[atm(i + 1)]; q.v2j = s.V[at(i + 2)]; q.v2m = s.V[atm(i + 2)];
+    }
+#pragma unroll
+    for (int t = 0; t < CM; ++t) {
+        if (t < mm) {
+            const int i = k0 + t;
+            if (t > 0) {
+                const int i2 = cl(i + 2);
+                q.Dm = q.D0; q.D0 = q.D1; q.D1 = q.D2; q.D2 = s.D[at(i2)];
+                q.Zm = q.Z0; q.Z0 = q.Z1; q.Z1 = q.Z2; q.Z2 = s.Zs[at(i2)];
+                q.Qm = q.Q0; q.Q0 = q.Q1; q.Q1 = q.Q2; q.Q2 = s.dQx[at(i2)];
+                q.Vc0 = q.Vc1; q.Vc1 = s.Vc[at(i + 1)];
+                q.U0 = q.U1; q.U1 = q.U2; q.U2 = s.U[at(i + 2)];
+                q.dqy0 = q.dqy1; q.dqy1 = s.dQy[at(i + 1)];
+                q.v1j = q.v2j; q.v1m = q.v2m; q.v2j = s.V[at(i + 2)]; q.v2m = s.V[atm(i + 2)];
+            }
+            q.Qx1 = s.Qx[at(i + 1)];
+            q.cD = rf.cD_x[at(i)]; q.Urms = rf.Urms_x[at(i)]; q.Fx = rf.Fx[at(i)];
+            q.dnuDx_dy = rf.dnuDx_dy[at(i)]; q.nuD_x = rf.nuD_x[at(i)]; q.dnuD_dx = rf.dnuD_dx[at(i)];
+            int wi;
+            double drag;
+            asm_qxx_vals(k, mt, q, f0, i, di[t], lo[t], up[t], rh[t], wi, drag);
+            w.w[at(i)] = (uint8_t)wi;
+            w.drag[at(i)] = drag;
+        }
+    }
+}
+
 struct ProvQxx {      // Qx predictor, lines along x (columns): line = j, k = i
     Consts k; Metrics m; State s; RunFields rf; Work w; double f0;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qxx(k, m, s, rf, w, f0, kk, line, a, b, c, f); }
+    __device__ __forceinline__ void segment(int line, int k0, int mm, double (&lo)[CM], double (&di)[CM],
+                                            double (&up)[CM], double (&rh)[CM]) const
+    { qxx_segment(k, m, s, rf, w, f0, line, k0, mm, lo, di, up, rh); }
 };
 struct ProvQyx {      // Qy projector, columns
     Consts k; Metrics m; State s; RunFields rf; Work w;
@@ -74,10 +132,6 @@ struct ProvQyy {      // Qy predictor, rows
     { asm_qyy(k, m, s, rf, w, line, kk, a, b, c, f); }
 };
 
-// One block: lines_per_block lines x S segments. COLS: lines are columns of row-major
-// arrays (element (k, line) at k*ld + line) and thread t maps to (line t % L, segment
-// t / L) so that a wavefront reads runs of consecutive columns; ROWS: lines are rows
-// (element (line, k) at line*ld + k) and t maps to (line t / S, segment t % S).
 // ---------------------------------------------------------------------------------
 // Building blocks shared by the partition kernels.
 // ---------------------------------------------------------------------------------
@@ -140,14 +194,16 @@ __device__ __forceinline__ RedRows seg_eliminate(int m, double (&lo)[SEG_M], dou
     return r;
 }
 
-// Cyclic PCR on N = 2S reduced rows of one line stored at red0..red3[base + i] (lower,
+// Cyclic PCR on N = 2S reduced rows of one line stored at red0..red3[base + i*rs] (lower,
 // diagonal, upper, rhs). Every thread of the block must call it (it holds barriers);
 // `active` threads own rows 2*seg and 2*seg+1. Returns the solution of those two rows.
-__device__ __forceinline__ void pcr_cyclic(double *red0, double *red1, double *red2, double *red3, int base, int seg,
-                                           int S, bool active, const RedRows &r, double &xF, double &xL)
+// Workgroups holding several lines interleave them (rs = lines per workgroup), so that
+// lanes of neighbouring lines touch neighbouring LDS words.
+__device__ __forceinline__ void pcr_cyclic(double *red0, double *red1, double *red2, double *red3, int base, int rs,
+                                           int seg, int S, bool active, const RedRows &r, double &xF, double &xL)
 {
     const int N = 2 * S;
-    const int rF = base + 2 * seg, rL = rF + 1;
+    const int rF = base + 2 * seg * rs, rL = rF + rs;
     if (active) {
         red0[rF] = r.lo0; red1[rF] = r.A_F; red2[rF] = r.C_F; red3[rF] = r.R_F;
         red0[rL] = r.A_L; red1[rL] = r.B_L; red2[rL] = r.upL; red3[rL] = r.R_L;
@@ -159,8 +215,8 @@ __device__ __forceinline__ void pcr_cyclic(double *red0, double *red1, double *r
 #pragma unroll
             for (int w = 0; w < 2; ++w) {
                 const int i = 2 * seg + w;
-                const int im = base + ((i - st) & (N - 1)), ip = base + ((i + st) & (N - 1));
-                const int ii = base + i;
+                const int im = base + ((i - st) & (N - 1)) * rs, ip = base + ((i + st) & (N - 1)) * rs;
+                const int ii = base + i * rs;
                 const double k1m = red0[ii] / red1[im];
                 const double k2p = red2[ii] / red1[ip];
                 na[w] = -red0[im] * k1m;
@@ -173,7 +229,7 @@ __device__ __forceinline__ void pcr_cyclic(double *red0, double *red1, double *r
         if (active) {
 #pragma unroll
             for (int w = 0; w < 2; ++w) {
-                const int ii = base + 2 * seg + w;
+                const int ii = base + (2 * seg + w) * rs;
                 red0[ii] = na[w]; red1[ii] = nb[w]; red2[ii] = nc[w]; red3[ii] = nd[w];
             }
         }
@@ -188,7 +244,7 @@ __device__ __forceinline__ void pcr_cyclic(double *red0, double *red1, double *r
 #pragma unroll
         for (int w = 0; w < 2; ++w) {
             const int i = 2 * seg + w;
-            const int ii = base + i, jj = base + ((i + S) & (N - 1));
+            const int ii = base + i * rs, jj = base + ((i + S) & (N - 1)) * rs;
             const double bi = red1[ii], ei = red0[ii] + red2[ii], di_ = red3[ii];
             const double bj = red1[jj], ej = red0[jj] + red2[jj], dj = red3[jj];
             xs[w] = (di_ * bj - ei * dj) / (bi * bj - ei * ej);
@@ -245,7 +301,8 @@ __device__ __forceinline__ void partition_body(const Prov &prov, double *__restr
         rr = seg_eliminate<SEG_M>(m, lo, di, up, rh);
     }
     double xF, xL;
-    pcr_cyclic(red[0], red[1], red[2], red[3], lloc * 2 * S, seg, S, active, rr, xF, xL);
+    if (COLS) pcr_cyclic(red[0], red[1], red[2], red[3], lloc, L, seg, S, active, rr, xF, xL);
+    else pcr_cyclic(red[0], red[1], red[2], red[3], lloc * 2 * S, 1, seg, S, active, rr, xF, xL);
     if (active) {
 #pragma unroll
         for (int q = 0; q < SEG_M; ++q)
@@ -312,7 +369,7 @@ __device__ __forceinline__ void rows_lds_body(const Prov &prov, double *__restri
     RedRows rr{};
     if (active) rr = seg_eliminate<RM>(m, lo, di, up, rh);
     double xF, xL;
-    pcr_cyclic(sh[0], sh[1], sh[2], sh[3], 0, seg, S, active, rr, xF, xL);
+    pcr_cyclic(sh[0], sh[1], sh[2], sh[3], 0, 1, seg, S, active, rr, xF, xL);
     __syncthreads();
     if (active) {
 #pragma unroll
@@ -467,7 +524,8 @@ hipError_t launch_transposed_rows(hipStream_t st, const double *A, const double 
 //   B. solve the 2*nseg-row reduced systems of all columns (partition solver);
 //   C. back-substitute the segment from its two boundary values.
 // ---------------------------------------------------------------------------------
-constexpr int CM = 8;
+template <class P, class = void> struct HasSegment : std::false_type {};
+template <class P> struct HasSegment<P, std::void_t<decltype(&P::segment)>> : std::true_type {};
 
 int cols_segments(int n) { return (n + CM - 1) / CM; }
 
@@ -482,9 +540,13 @@ __device__ __forceinline__ void cols_elim_body(const Prov &prov, const ColsWork 
     const int k1 = (int)(((long)(sg + 1) * n) / nseg);
     const int m = k1 - k0;   // 2..CM
     double lo[CM], di[CM], up[CM], rh[CM];
+    if constexpr (HasSegment<Prov>::value) {
+        prov.segment(j, k0, m, lo, di, up, rh);
+    } else {
 #pragma unroll
-    for (int q = 0; q < CM; ++q)
-        if (q < m) prov.get(j, k0 + q, di[q], lo[q], up[q], rh[q]);
+        for (int q = 0; q < CM; ++q)
+            if (q < m) prov.get(j, k0 + q, di[q], lo[q], up[q], rh[q]);
+    }
     if (k0 == 0) lo[0] = 0.0;
     if (k1 == n) {   // static indices keep the segment arrays in registers
 #pragma unroll
@@ -597,6 +659,9 @@ struct GProvQxx {
     const ScenDesc *d;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qxx(d->k, d->m, d->s, d->rf, d->w, d->f0, kk, line, a, b, c, f); }
+    __device__ __forceinline__ void segment(int line, int k0, int mm, double (&lo)[CM], double (&di)[CM],
+                                            double (&up)[CM], double (&rh)[CM]) const
+    { qxx_segment(d->k, d->m, d->s, d->rf, d->w, d->f0, line, k0, mm, lo, di, up, rh); }
 };
 struct GProvQyx {
     const ScenDesc *d;
