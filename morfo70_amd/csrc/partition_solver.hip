@@ -194,46 +194,54 @@ __device__ __forceinline__ RedRows seg_eliminate(int m, double (&lo)[SEG_M], dou
     return r;
 }
 
-// Cyclic PCR on N = 2S reduced rows of one line stored at red0..red3[base + i*rs] (lower,
-// diagonal, upper, rhs). Every thread of the block must call it (it holds barriers);
-// `active` threads own rows 2*seg and 2*seg+1. Returns the solution of those two rows.
-// Workgroups holding several lines interleave them (rs = lines per workgroup), so that
-// lanes of neighbouring lines touch neighbouring LDS words.
-__device__ __forceinline__ void pcr_cyclic(double *red0, double *red1, double *red2, double *red3, int base, int rs,
-                                           int seg, int S, bool active, const RedRows &r, double &xF, double &xL)
+// Cyclic PCR on N = 2S reduced rows of one line (lower, diagonal, upper, rhs), row i of
+// the line at base + i*rs of the four arrays P[0..3] or Q[0..3]: the steps alternate
+// between the two buffers, so one barrier per step orders every read before the next
+// overwrite, and each thread keeps its own two rows in registers, reading only the
+// neighbours' from LDS. Workgroups holding several lines interleave them (rs = lines per
+// workgroup), so that lanes of neighbouring lines touch neighbouring LDS words. Every
+// thread of the block must call it (it holds barriers); `active` threads own rows 2*seg
+// and 2*seg+1. Returns the solution of those two rows.
+__device__ __forceinline__ void pcr_cyclic(double *const (&P)[4], double *const (&Q)[4], int base, int rs, int seg,
+                                           int S, bool active, const RedRows &r, double &xF, double &xL)
 {
     const int N = 2 * S;
-    const int rF = base + 2 * seg * rs, rL = rF + rs;
+    double lo[2] = {r.lo0, r.A_L}, dg[2] = {r.A_F, r.B_L}, up[2] = {r.C_F, r.upL}, rh[2] = {r.R_F, r.R_L};
+    const double *c0 = P[0], *c1 = P[1], *c2 = P[2], *c3 = P[3];
+    double *n0 = Q[0], *n1 = Q[1], *n2 = Q[2], *n3 = Q[3];
     if (active) {
-        red0[rF] = r.lo0; red1[rF] = r.A_F; red2[rF] = r.C_F; red3[rF] = r.R_F;
-        red0[rL] = r.A_L; red1[rL] = r.B_L; red2[rL] = r.upL; red3[rL] = r.R_L;
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+            const int ii = base + (2 * seg + w) * rs;
+            P[0][ii] = lo[w]; P[1][ii] = dg[w]; P[2][ii] = up[w]; P[3][ii] = rh[w];
+        }
     }
     __syncthreads();
     for (int st = 1; st < S; st <<= 1) {   // strides 1, 2, ..., N/4
-        double na[2], nb[2], nc[2], nd[2];
         if (active) {
 #pragma unroll
             for (int w = 0; w < 2; ++w) {
                 const int i = 2 * seg + w;
                 const int im = base + ((i - st) & (N - 1)) * rs, ip = base + ((i + st) & (N - 1)) * rs;
-                const int ii = base + i * rs;
-                const double k1m = red0[ii] / red1[im];
-                const double k2p = red2[ii] / red1[ip];
-                na[w] = -red0[im] * k1m;
-                nc[w] = -red2[ip] * k2p;
-                nb[w] = red1[ii] - red2[im] * k1m - red0[ip] * k2p;
-                nd[w] = red3[ii] - red3[im] * k1m - red3[ip] * k2p;
+                const double k1m = lo[w] / c1[im];
+                const double k2p = up[w] / c1[ip];
+                const double na = -c0[im] * k1m;
+                const double nc = -c2[ip] * k2p;
+                const double nb = dg[w] - c2[im] * k1m - c0[ip] * k2p;
+                const double nd = rh[w] - c3[im] * k1m - c3[ip] * k2p;
+                lo[w] = na; dg[w] = nb; up[w] = nc; rh[w] = nd;
             }
-        }
-        __syncthreads();
-        if (active) {
 #pragma unroll
             for (int w = 0; w < 2; ++w) {
                 const int ii = base + (2 * seg + w) * rs;
-                red0[ii] = na[w]; red1[ii] = nb[w]; red2[ii] = nc[w]; red3[ii] = nd[w];
+                n0[ii] = lo[w]; n1[ii] = dg[w]; n2[ii] = up[w]; n3[ii] = rh[w];
             }
         }
         __syncthreads();
+        const double *t0 = n0, *t1 = n1, *t2 = n2, *t3 = n3;
+        n0 = const_cast<double *>(c0); n1 = const_cast<double *>(c1);
+        n2 = const_cast<double *>(c2); n3 = const_cast<double *>(c3);
+        c0 = t0; c1 = t1; c2 = t2; c3 = t3;
     }
     xF = 0.0;
     xL = 0.0;
@@ -244,9 +252,9 @@ __device__ __forceinline__ void pcr_cyclic(double *red0, double *red1, double *r
 #pragma unroll
         for (int w = 0; w < 2; ++w) {
             const int i = 2 * seg + w;
-            const int ii = base + i * rs, jj = base + ((i + S) & (N - 1)) * rs;
-            const double bi = red1[ii], ei = red0[ii] + red2[ii], di_ = red3[ii];
-            const double bj = red1[jj], ej = red0[jj] + red2[jj], dj = red3[jj];
+            const int jj = base + ((i + S) & (N - 1)) * rs;
+            const double bi = dg[w], ei = lo[w] + up[w], di_ = rh[w];
+            const double bj = c1[jj], ej = c0[jj] + c2[jj], dj = c3[jj];
             xs[w] = (di_ * bj - ei * dj) / (bi * bj - ei * ej);
         }
         xF = xs[0];
@@ -271,7 +279,7 @@ __device__ __forceinline__ double seg_value(int q, int m, const double (&lo)[SEG
 // (element (line, k) at line*ld + k) and t maps to (line t / S, segment t % S).
 template <bool COLS, int BLOCK, int SEG_M, class Prov>
 __device__ __forceinline__ void partition_body(const Prov &prov, double *__restrict__ Y, const LineGeom &g, int blk,
-                                               double (*red)[BLOCK * 2])
+                                               double (*red)[4][BLOCK * 2])
 {
     const int t = threadIdx.x;
     const int S = g.S, L = g.lines_per_block;
@@ -301,8 +309,10 @@ __device__ __forceinline__ void partition_body(const Prov &prov, double *__restr
         rr = seg_eliminate<SEG_M>(m, lo, di, up, rh);
     }
     double xF, xL;
-    if (COLS) pcr_cyclic(red[0], red[1], red[2], red[3], lloc, L, seg, S, active, rr, xF, xL);
-    else pcr_cyclic(red[0], red[1], red[2], red[3], lloc * 2 * S, 1, seg, S, active, rr, xF, xL);
+    double *const P[4] = {red[0][0], red[0][1], red[0][2], red[0][3]};
+    double *const Q[4] = {red[1][0], red[1][1], red[1][2], red[1][3]};
+    if (COLS) pcr_cyclic(P, Q, lloc, L, seg, S, active, rr, xF, xL);
+    else pcr_cyclic(P, Q, lloc * 2 * S, 1, seg, S, active, rr, xF, xL);
     if (active) {
 #pragma unroll
         for (int q = 0; q < SEG_M; ++q)
@@ -313,16 +323,20 @@ __device__ __forceinline__ void partition_body(const Prov &prov, double *__restr
 template <bool COLS, int BLOCK, int SEG_M, class Prov>
 __global__ void __launch_bounds__(BLOCK) k_partition_solve(Prov prov, double *__restrict__ Y, LineGeom g)
 {
-    __shared__ double red[4][BLOCK * 2];   // reduced rows: lower, diag, upper, rhs
+    __shared__ double red[2][4][BLOCK * 2];   // reduced rows (lower, diag, upper, rhs), two buffers
     const int blk = g.xcd_remap ? xcd_chunk((int)blockIdx.x, (int)gridDim.x) : (int)blockIdx.x;
     partition_body<COLS, BLOCK, SEG_M>(prov, Y, g, blk, red);
 }
 
-// Row lines staged through LDS: one line (n <= 4096) per block of RB threads. The block
-// assembles the line's coefficients in coalesced order into LDS (element e at
-// e + e/8, so that the per-thread segment reads below are free of bank conflicts),
-// each thread then takes its segment, and the solution goes back through LDS to a
-// coalesced store.
+// Row lines staged through LDS: one line (n <= RB*RM) per block of RB threads. The block
+// assembles the line's coefficients in coalesced order and hands each thread its segment
+// through a two-array LDS tile (element e at e + e/8, so that the per-thread segment reads
+// are free of bank conflicts), in two passes: the diagonal and lower coefficients first,
+// while each thread keeps the upper and right-hand-side values it assembled in registers,
+// then those. The tile (two arrays) then holds the reduced system's two PCR buffers and
+// finally the solution on its way to a coalesced store. At 2*RB*RM*9/8 doubles the tile
+// leaves room for two 512-thread or four 256-thread blocks per CU, so one block's
+// elimination and PCR overlap another block's assembly.
 constexpr int RM = 8;                  // unknowns per thread
 template <int RB> struct RowsLds {     // RB threads per block, lines up to RB*RM
     static constexpr int NMAX = RB * RM;
@@ -335,11 +349,16 @@ __device__ __forceinline__ void rows_lds_body(const Prov &prov, double *__restri
                                               int periodic, int line, double (*sh)[RowsLds<RB>::LDS])
 {
     const int t = threadIdx.x;
-    for (int e = t; e < n; e += RB) {
-        double a, b, c, f;
-        prov.get(line, e, a, b, c, f);
-        const int p = lpad(e);
-        sh[0][p] = a; sh[1][p] = b; sh[2][p] = c; sh[3][p] = f;
+    double cu[RM], cf[RM];   // upper and rhs of the elements this thread assembles
+#pragma unroll
+    for (int it = 0; it < RM; ++it) {
+        const int e = t + it * RB;
+        if (e < n) {
+            double a, b;
+            prov.get(line, e, a, b, cu[it], cf[it]);
+            const int p = lpad(e);
+            sh[0][p] = a; sh[1][p] = b;
+        }
     }
     __syncthreads();
     const bool active = t < S;
@@ -353,7 +372,26 @@ __device__ __forceinline__ void rows_lds_body(const Prov &prov, double *__restri
         for (int q = 0; q < RM; ++q) {
             if (q < m) {
                 const int p = lpad(k0 + q);
-                di[q] = sh[0][p]; lo[q] = sh[1][p]; up[q] = sh[2][p]; rh[q] = sh[3][p];
+                di[q] = sh[0][p]; lo[q] = sh[1][p];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < RM; ++it) {
+        const int e = t + it * RB;
+        if (e < n) {
+            const int p = lpad(e);
+            sh[0][p] = cu[it]; sh[1][p] = cf[it];
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < RM; ++q) {
+            if (q < m) {
+                const int p = lpad(k0 + q);
+                up[q] = sh[0][p]; rh[q] = sh[1][p];
             }
         }
         if (!periodic) {
@@ -369,7 +407,12 @@ __device__ __forceinline__ void rows_lds_body(const Prov &prov, double *__restri
     RedRows rr{};
     if (active) rr = seg_eliminate<RM>(m, lo, di, up, rh);
     double xF, xL;
-    pcr_cyclic(sh[0], sh[1], sh[2], sh[3], 0, 1, seg, S, active, rr, xF, xL);
+    {
+        const int N = 2 * S;   // 4N <= 4*RB*2 < LDS doubles per array
+        double *const P[4] = {sh[0], sh[0] + N, sh[0] + 2 * N, sh[0] + 3 * N};
+        double *const Q[4] = {sh[1], sh[1] + N, sh[1] + 2 * N, sh[1] + 3 * N};
+        pcr_cyclic(P, Q, 0, 1, seg, S, active, rr, xF, xL);
+    }
     __syncthreads();
     if (active) {
 #pragma unroll
@@ -382,10 +425,10 @@ __device__ __forceinline__ void rows_lds_body(const Prov &prov, double *__restri
 }
 
 template <int RB, class Prov>
-__global__ void __launch_bounds__(RB) k_rows_lds(Prov prov, double *__restrict__ Y, long ldy, int n, int S,
-                                                 int periodic)
+__global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_rows_lds(Prov prov, double *__restrict__ Y, long ldy, int n, int S, int periodic)
 {
-    __shared__ double sh[4][RowsLds<RB>::LDS];
+    __shared__ double sh[2][RowsLds<RB>::LDS];
     rows_lds_body<RB>(prov, Y, ldy, n, S, periodic, (int)blockIdx.x, sh);
 }
 
@@ -642,8 +685,8 @@ hipError_t launch_cols3_qyx(hipStream_t st, const Consts &k, const Metrics &m, c
 
 bool rows_lds_supported(int n) { return pick_segments(n, RM, 512) > 0; }
 
-// Row block: 256 threads (lines up to 2048, 74 KB of LDS, two blocks per CU) or 512
-// threads (lines up to 4096, 147 KB, one block per CU). rb = 0 picks the smallest fit.
+// Row block: 256 threads (lines up to 2048, 36 KB of LDS) or 512 threads (lines up to
+// 4096, 72 KB). rb = 0 picks the smallest fit.
 int rows_block_for(int n, int rb)
 {
     if (rb == 256 && pick_segments(n, RM, 256) > 0 && n <= RowsLds<256>::NMAX) return 256;
@@ -680,10 +723,10 @@ struct GProvQyy {
 };
 
 template <int WHICH, int RB>   // WHICH 0: Qx projector rows, 1: Qy predictor rows
-__global__ void __launch_bounds__(RB) k_rows_lds_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
-                                                         int nscen)
+__global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_rows_lds_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start, int nscen)
 {
-    __shared__ double sh[4][RowsLds<RB>::LDS];
+    __shared__ double sh[2][RowsLds<RB>::LDS];
     const int b = blockIdx.x;
     const int sid = scen_of(start, nscen, b);
     const ScenDesc *d = D + sid;
@@ -729,7 +772,7 @@ template <int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_red_pcr_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
                                                            int nscen)
 {
-    __shared__ double red[4][BLOCK * 2];
+    __shared__ double red[2][4][BLOCK * 2];
     const int b = blockIdx.x;
     const int sid = scen_of(start, nscen, b);
     const ScenDesc *d = D + sid;
