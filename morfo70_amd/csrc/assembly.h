@@ -168,7 +168,8 @@ __host__ __device__ __forceinline__ void asm_qxy(const Consts &k, const State &s
 }
 
 // Qy predictor, src/HydroQy.cpp:26-41 with :92-230; f and d use IsWater row 0
-// (src/HydroQy.cpp:140,170). Side outputs: IsWater and DragTerm of row i.
+// (src/HydroQy.cpp:140,170). IsWater (setWaterNodes_y) of every face is in w.w, written by
+// the Qx tail from guess_D and guess_Zs. Side output: DragTerm of row i.
 __host__ __device__ __forceinline__ void asm_qyy(const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
                                         const Work &w, int i, int j, double &a_out, double &b_out, double &c_out,
                                         double &f_out)
@@ -177,14 +178,13 @@ __host__ __device__ __forceinline__ void asm_qyy(const Consts &k, const Metrics 
     const long r = (long)i * ny;
     const int jp = wrap_inc(j, ny), jm = wrap_dec(j, ny), jpp = wrap_inc(jp, ny);
     const double dt = k.dt;
-    const double *gD = w.gD + r, *gZs = w.gZs + r, *dqy = s.dQy + r;
-    // masks of row i at faces jm, j, jp
+    const double *gD = w.gD + r;
     const double g_m = gD[jm], g_j = gD[j], g_p = gD[jp], g_pp = gD[jpp];
-    const int wy_j = wet_rule(g_j, g_p, gZs[j], gZs[jp], dqy[j], dqy[jp], k.Dmask);
-    const int wy_m = wet_rule(g_m, g_j, gZs[jm], gZs[j], dqy[jm], dqy[j], k.Dmask);
-    const int wy_p = wet_rule(g_p, g_pp, gZs[jp], gZs[jpp], dqy[jp], dqy[jpp], k.Dmask);
+    // masks of row i at faces jm, j, jp
+    const uint8_t *wq = w.w + r;
+    const int wy_j = wq[j], wy_m = wq[jm], wy_p = wq[jp];
     // mask of row 0 at face j (Q1)
-    const int w0 = wet_rule(w.gD[j], w.gD[jp], w.gZs[j], w.gZs[jp], s.dQy[j], s.dQy[jp], k.Dmask);
+    const int w0 = w.w[j];
     // Di row i+1 (getD_y(i)) at j, jm, jp
     const double Dy = 0.5 * (g_p + g_j);
     const double invDy = 1 / Dy;
@@ -223,7 +223,6 @@ __host__ __device__ __forceinline__ void asm_qyy(const Consts &k, const Metrics 
     b_out = lv;
     c_out = uv;
     f_out = fv;
-    w.w[r + j] = (uint8_t)wy_j;
     w.drag[r + j] = drag;
 }
 

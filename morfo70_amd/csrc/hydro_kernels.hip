@@ -175,7 +175,8 @@ __global__ void __launch_bounds__(256) k_qxy_assemble(Consts k, State s, RunFiel
 // ------------------------------------------------------------------------------------
 // After the Qx projector: setQx_BC (src/HydroBoundaryConditions.cpp:13-23), difC2X of Qx
 // (src/HydroQx.cpp:80), solveZs_guess (src/Hydro.cpp:230-250), getU0/getU
-// (src/HydroQx.cpp:420-451), X2C (include/MeshOperations.h:36-47). Rows 0..nx-1.
+// (src/HydroQx.cpp:420-451), X2C (include/MeshOperations.h:36-47), and the Qy
+// predictor's IsWater (setWaterNodes_y, src/HydroQy.cpp:92-122) into w.w. Rows 0..nx-1.
 // ------------------------------------------------------------------------------------
 __host__ __device__ __forceinline__ double qx_at(const State &s, const Consts &k, long rowoff, int row, int j,
                                         double f0)
@@ -224,8 +225,18 @@ __host__ __device__ __forceinline__ void qx_tail_body(const Consts &k, const Met
     } else {
         ui1 = s.U[(long)nx * ny + j];   // U row nx is never written by the reference
     }
+    const double gzs_i = s.Zs[r + j] + inc_i;
+    {   // IsWater of face j: guess_D and guess_Zs of column j+1 by the same expressions
+        const int jp = wrap_inc(j, ny);
+        const double qa = qx_at(s, k, r, i, jp, f0), qb = qx_at(s, k, r + ny, i + 1, jp, f0);
+        const double dq = m.inv_dxx[i] * (qb - qa);
+        const double incrZs = -dt * (dq + s.dQy[r + jp]);
+        const double gd_p = s.D[r + jp] + incrZs;
+        const double gzs_p = s.Zs[r + jp] + incrZs;
+        w.w[r + j] = (uint8_t)wet_rule(gd[1], gd_p, gzs_i, gzs_p, s.dQy[r + j], s.dQy[r + jp], k.Dmask);
+    }
     s.dQx[r + j] = dq_i;
-    w.gZs[r + j] = s.Zs[r + j] + inc_i;
+    w.gZs[r + j] = gzs_i;
     w.gD[r + j] = gd[1];
     s.U[r + j] = ui;
     s.Uc[r + j] = m.x2c_d[i] * ui + m.x2c_up[i] * ui1;

@@ -1226,7 +1226,11 @@ int adj_taped_step(m70h *h, int t, const StepTape &T)
     if ((e = launch_partition_rows(st, T.a2, T.b2, T.c2, T.f, ny, s.Qx + ny, ny, nx - 1, ny, true, h->tile_rows)) !=
         hipSuccess)
         return hip_fail(h, e, "adjoint tape periodic x");
-    if ((e = launch_qx_tail(st, k, h->m, s, wx, f0)) != hipSuccess) return hip_fail(h, e, "adjoint tape qx tail");
+    {   // the tail writes the Qy predictor's IsWater: onto the y tape
+        Work wt = wx;
+        wt.w = T.wy;
+        if ((e = launch_qx_tail(st, k, h->m, s, wt, f0)) != hipSuccess) return hip_fail(h, e, "adjoint tape qx tail");
+    }
     wy.a = T.a3; wy.b = T.b3; wy.c = T.c3; wy.f = T.f;
     if ((e = launch_qyy_assemble(st, k, h->m, s, h->rf, wy)) != hipSuccess) return hip_fail(h, e, "adjoint tape qyy");
     if ((e = launch_partition_rows(st, T.a3, T.b3, T.c3, T.f, ny, s.Qy + ny, ny, nx - 1, ny, true, h->tile_rows)) !=
