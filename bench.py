@@ -28,29 +28,29 @@ B_ALG_STEP = 360.0          # algorithmic bytes per cell-step of the whole step
 
 # Algorithmic bytes per cell of each kernel: its compulsory FP64 inputs read once and
 # outputs written once (masks as 1 byte); intermediates a kernel hands to the next one
-# (eliminated segments E0..E3, reduced rows, their solution X) count as its outputs and
+# (eliminated segments E0..E2, reduced rows, their solution X) count as its outputs and
 # as the next kernel's inputs. Cells are the (nx-1) x ny flux faces unless noted.
 KERNEL_BYTES_PER_CELL = {
     "qxx_assemble": 8 * 14 + 8 * 5 + 1,     # 14 fields in; a,b,c,f,drag out; mask out
     "thomas_qx": 8 * 5,                     # a,b,c,f in; Qx* out
     "qxy_assemble": 8 * 8 + 1 + 8 * 4,      # D,Vc,Qx*,nuD_x,dnuDx_dy,drag (+2 rows) in, mask in; a,b,c,f out
     "periodic_qx": 8 * 5,                   # a,b,c,f in; Qx out
-    "qx_tail": 8 * 4 + 8 * 6,               # Qx,D,dQy,Zs in; Qx0,dQx,gZs,gD,U,Uc out (nx rows)
-    "qyy_assemble": 8 * 17 + 8 * 5 + 1,     # gD,gZs,dQy,Zs,dQx,U(2),Qy,Uc,V,Fy,nuD_y,dnuDy_dx,dnuD_dy,cD_y,Urms_y in
+    "qx_tail": 8 * 4 + 8 * 6 + 1,           # Qx,D,dQy,Zs in; Qx0,dQx,gZs,gD,U,Uc out, Qy mask out (nx rows)
+    "qyy_assemble": 8 * 14 + 1 + 8 * 5,     # gD,Zs,dQx,U(2),Qy,Uc,V,Fy,nuD_y,dnuDy_dx,dnuD_dy,cD_y,Urms_y, mask in; a,b,c,f,drag out
     "periodic_qy": 8 * 5,
     "qyx_assemble": 8 * 11 + 1 + 8 * 4,
     "thomas_qy": 8 * 5,
     "qy_tail": 8 * 5 + 8 * 5,               # Qy,dQx,D,Zs in; dQy,Zs,D,V,Vc out (nx rows)
     # FUSED family: y-sweeps in one launch (assembly inputs in, solution out)
     "fused_qxy": 8 * 8 + 1 + 8,             # D(2),Vc(2),Qx*,nuD_x,dnuDx_dy,drag in, mask in; Qx out
-    "fused_qyy": 8 * 17 + 8 * 2 + 1,        # the qyy_assemble inputs; Qy*, drag out; mask out
-    # FUSED x-sweeps, pass 1: assembly inputs in; drag, mask, E0..E3 out, plus the two
+    "fused_qyy": 8 * 14 + 1 + 8 * 2,        # the qyy_assemble inputs; Qy*, drag out
+    # FUSED x-sweeps, pass 1: assembly inputs in; drag, mask, E0..E2 out, plus the two
     # reduced rows (4 values each) per 8-row segment = 8 B per cell
-    "cols_qx_elim": 8 * 14 + 8 + 1 + 8 * 4 + 8,
-    "cols_qy_elim": 8 * 11 + 1 + 8 * 4 + 8,
-    # pass 3: E0..E3 and the two interface values per segment in, the solution out
-    "cols_qx_backsub": 8 * 4 + 2 + 8,
-    "cols_qy_backsub": 8 * 4 + 2 + 8,
+    "cols_qx_elim": 8 * 14 + 8 + 1 + 8 * 3 + 8,
+    "cols_qy_elim": 8 * 11 + 1 + 8 * 3 + 8,
+    # pass 3: E0..E2 and the two interface values per segment in, the solution out
+    "cols_qx_backsub": 8 * 3 + 2 + 8,
+    "cols_qy_backsub": 8 * 3 + 2 + 8,
     # pass 2: 2 reduced rows per 8-row segment, (lo, diag, up, rhs) in and x out = 10 B per cell
     "cols_qx_reduced": 10,
     "cols_qy_reduced": 10,

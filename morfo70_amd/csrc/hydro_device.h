@@ -80,10 +80,11 @@ struct Work {
     uint8_t *w;                // IsWater of the predictor, reused by the projector
 };
 
-// Work arrays of the three-pass column solver: eliminated segments E0..E3 (F each),
-// reduced rows R0..R3 and their solution X (2*nseg rows of ny each).
+// Work arrays of the three-pass column solver: eliminated segments E0..E2 (F each: the
+// fill-ins on the segment's first and last unknowns and the right-hand side, each times
+// the inverse pivot), reduced rows R0..R3 and their solution X (2*nseg rows of ny each).
 struct ColsWork {
-    double *E0, *E1, *E2, *E3;
+    double *E0, *E1, *E2;
     double *R0, *R1, *R2, *R3;
     double *X;
     int nseg;

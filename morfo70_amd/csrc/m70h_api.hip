@@ -448,10 +448,10 @@ int m70h_create(int nx, int ny, const double *x, double dy, const m70h_params *p
         (e = dalloc(h, &h->sk, scr)) != hipSuccess)
         return bail(hip_fail(h, e, "hipMalloc scratch"));
     if ((e = dalloc(h, &h->flag, 1)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc flag"));
-    {   // three-pass column solver: E0..E3 reuse the coefficient arrays a, b, c, f
+    {   // three-pass column solver: E0..E2 reuse the coefficient arrays a, b, c
         ColsWork &cw = h->cw;
         cw.nseg = cols_segments(nx - 1);
-        cw.E0 = h->w.a; cw.E1 = h->w.b; cw.E2 = h->w.c; cw.E3 = h->w.f;
+        cw.E0 = h->w.a; cw.E1 = h->w.b; cw.E2 = h->w.c;
         const long nr = 2L * cw.nseg * ny;
         double **rp[] = {&cw.R0, &cw.R1, &cw.R2, &cw.R3, &cw.X};
         for (auto q : rp)

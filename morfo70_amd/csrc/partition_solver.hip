@@ -601,10 +601,9 @@ __device__ __forceinline__ void cols_elim_body(const Prov &prov, const ColsWork 
     for (int q = 1; q < CM - 1; ++q) {
         if (q < m - 1) {
             const long o = (long)(k0 + q) * ny + j;
-            cw.E0[o] = lo[q];
-            cw.E1[o] = up[q];
-            cw.E2[o] = rh[q];
-            cw.E3[o] = di[q];
+            cw.E0[o] = lo[q] * di[q];
+            cw.E1[o] = up[q] * di[q];
+            cw.E2[o] = rh[q] * di[q];
         }
     }
     const long oF = (long)(2 * sg) * ny + j, oL = oF + ny;
@@ -631,7 +630,7 @@ __device__ __forceinline__ void cols_backsub_body(const ColsWork &cw, double *__
             double x;
             if (q == 0) x = xF;
             else if (q == m - 1) x = xL;
-            else x = (cw.E2[o] - cw.E0[o] * xF - cw.E1[o] * xL) * cw.E3[o];
+            else x = cw.E2[o] - cw.E0[o] * xF - cw.E1[o] * xL;
             Y[o] = x;
         }
     }
