@@ -334,6 +334,139 @@ __global__ void __launch_bounds__(256) k_qy_tail(Consts k, State s)
 }
 
 // ------------------------------------------------------------------------------------
+// The two tails on two adjacent columns per thread (even ny), with 16-byte loads of
+// column pairs: the columns a thread needs either side come from the neighbouring pairs
+// instead of one load per neighbour and column. Every value is formed by the same
+// expressions as qx_tail_body / qy_tail_body, so the results are the same bits.
+// ------------------------------------------------------------------------------------
+__device__ __forceinline__ double2 ld2(const double *p, long e) { return *reinterpret_cast<const double2 *>(p + e); }
+__device__ __forceinline__ void st2(double *p, long e, double a, double b)
+{
+    *reinterpret_cast<double2 *>(p + e) = make_double2(a, b);
+}
+
+__global__ void __launch_bounds__(256) k_qx_tail2(Consts k, Metrics m, State s, Work w, double f0)
+{
+    const int ny = k.ny, nx = k.nx;
+    const int j0 = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = blockIdx.y;
+    if (j0 >= ny || i >= nx) return;
+    const long r = (long)i * ny;
+    const int c2 = (j0 + 2 == ny) ? 0 : j0 + 2;
+    const double dt = k.dt;
+    // Qx rows i-1..i+2 at (j0, j0+1), through the row-0 BC (qx_at)
+    auto qrow = [&](int row, int c) -> double2 {
+        if (row < 0 || row > nx) return make_double2(0.0, 0.0);
+        if (row == 0) {
+            const double2 v = ld2(s.Qx, ny + c);
+            return make_double2(k.f1 * v.x + f0, k.f1 * v.y + f0);
+        }
+        return ld2(s.Qx, (long)row * ny + c);
+    };
+    double2 q[4];
+    for (int t = 0; t < 4; ++t) q[t] = qrow(i - 1 + t, j0);
+    double2 gd[3];
+    double2 dq_i = make_double2(0.0, 0.0), inc_i = make_double2(0.0, 0.0);
+    for (int t = 0; t < 3; ++t) {
+        const int row = i - 1 + t;
+        if (row < 0 || row > nx - 1) { gd[t] = make_double2(0.0, 0.0); continue; }
+        const double idx = m.inv_dxx[row];
+        const double2 dqy = ld2(s.dQy, (long)row * ny + j0), dd = ld2(s.D, (long)row * ny + j0);
+        const double dqa = idx * (q[t + 1].x - q[t].x), dqb = idx * (q[t + 1].y - q[t].y);
+        const double ia = -dt * (dqa + dqy.x), ib = -dt * (dqb + dqy.y);
+        gd[t] = make_double2(dd.x + ia, dd.y + ib);
+        if (t == 1) { dq_i = make_double2(dqa, dqb); inc_i = make_double2(ia, ib); }
+    }
+    double ua, ub;
+    if (i == 0) {
+        ua = q[1].x / (1.5 * gd[1].x - 0.5 * gd[2].x);
+        ub = q[1].y / (1.5 * gd[1].y - 0.5 * gd[2].y);
+    } else {
+        ua = q[1].x / (0.5 * (gd[0].x + gd[1].x));
+        ub = q[1].y / (0.5 * (gd[0].y + gd[1].y));
+    }
+    double u1a, u1b;
+    if (i + 1 <= nx - 1) {
+        u1a = q[2].x / (0.5 * (gd[1].x + gd[2].x));
+        u1b = q[2].y / (0.5 * (gd[1].y + gd[2].y));
+    } else {
+        const double2 un = ld2(s.U, (long)nx * ny + j0);   // U row nx is never written by the reference
+        u1a = un.x;
+        u1b = un.y;
+    }
+    const double2 zs = ld2(s.Zs, r + j0);
+    const double gza = zs.x + inc_i.x, gzb = zs.y + inc_i.y;
+    const double2 dqy_i = ld2(s.dQy, r + j0);
+    // guess_D / guess_Zs of column j0+2 (the right neighbour of j0+1)
+    double gd_c2, gz_c2, dqy_c2;
+    {
+        const double qa = qrow(i, c2).x, qb = qrow(i + 1, c2).x;
+        const double dq = m.inv_dxx[i] * (qb - qa);
+        dqy_c2 = s.dQy[r + c2];
+        const double incrZs = -dt * (dq + dqy_c2);
+        gd_c2 = s.D[r + c2] + incrZs;
+        gz_c2 = s.Zs[r + c2] + incrZs;
+    }
+    const int wa = wet_rule(gd[1].x, gd[1].y, gza, gzb, dqy_i.x, dqy_i.y, k.Dmask);
+    const int wb = wet_rule(gd[1].y, gd_c2, gzb, gz_c2, dqy_i.y, dqy_c2, k.Dmask);
+    w.w[r + j0] = (uint8_t)wa;
+    w.w[r + j0 + 1] = (uint8_t)wb;
+    st2(s.dQx, r + j0, dq_i.x, dq_i.y);
+    st2(w.gZs, r + j0, gza, gzb);
+    st2(w.gD, r + j0, gd[1].x, gd[1].y);
+    st2(s.U, r + j0, ua, ub);
+    st2(s.Uc, r + j0, m.x2c_d[i] * ua + m.x2c_up[i] * u1a, m.x2c_d[i] * ub + m.x2c_up[i] * u1b);
+    if (i == 0) st2(s.Qx, j0, q[1].x, q[1].y);
+}
+
+__global__ void __launch_bounds__(256) k_qy_tail2(Consts k, State s)
+{
+    const int ny = k.ny, nx = k.nx;
+    const int j0 = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = blockIdx.y;
+    if (j0 >= ny || i >= nx) return;
+    const long r = (long)i * ny;
+    const int cm = (j0 == 0) ? ny - 2 : j0 - 2, cp = (j0 + 2 == ny) ? 0 : j0 + 2;
+    // qy_row(i+1, .) at columns cm, cm+1 | j0, j0+1 | cp (row nx is Qy[nx-1]*inv_3)
+    auto qyr = [&](int c) -> double2 {
+        if (i + 1 < nx) return ld2(s.Qy, (long)(i + 1) * ny + c);
+        const double2 v = ld2(s.Qy, (long)(nx - 1) * ny + c);
+        return make_double2(v.x * k.inv_3, v.y * k.inv_3);
+    };
+    const double2 qm = qyr(cm), q0 = qyr(j0), qp = qyr(cp);
+    const double2 xm = ld2(s.dQx, r + cm), x0 = ld2(s.dQx, r + j0), xp = ld2(s.dQx, r + cp);
+    const double2 dm = ld2(s.D, r + cm), d0 = ld2(s.D, r + j0), dp = ld2(s.D, r + cp);
+    // columns j0-1 (m1), j0 (a), j0+1 (b), j0+2 (p1)
+    const double dq_m1 = (qm.y - qm.x) * k.inv_dy;
+    const double dq_a = (q0.x - qm.y) * k.inv_dy;
+    const double dq_b = (q0.y - q0.x) * k.inv_dy;
+    const double dq_p1 = (qp.x - q0.y) * k.inv_dy;
+    const double inc_a = -k.dt * (x0.x + dq_a), inc_b = -k.dt * (x0.y + dq_b);
+    const double dn_m1 = dm.y + -k.dt * (xm.y + dq_m1);
+    const double dn_a = d0.x + inc_a, dn_b = d0.y + inc_b;
+    const double dn_p1 = dp.x + -k.dt * (xp.x + dq_p1);
+    const double v_m1 = qm.y / (0.5 * (dn_m1 + dn_a));
+    const double v_a = q0.x / (0.5 * (dn_a + dn_b));
+    const double v_b = q0.y / (0.5 * (dn_b + dn_p1));
+    st2(s.dQy, r + j0, dq_a, dq_b);
+    const double2 zs = ld2(s.Zs, r + j0);
+    st2(s.Zs, r + j0, zs.x + inc_a, zs.y + inc_b);
+    st2(s.Dn, r + j0, dn_a, dn_b);
+    st2(s.V, r + ny + j0, v_a, v_b);
+    st2(s.Vc, r + j0, 0.5 * (v_a + v_m1), 0.5 * (v_b + v_a));
+    if (i == 0) {
+        const double2 q1 = ld2(s.Qy, ny + j0);
+        const double qa = k.bcQy * q1.x, qb = k.bcQy * q1.y;
+        st2(s.Qy, j0, qa, qb);
+        st2(s.V, j0, qa / (2 * dn_a - new_depth(s, k, 1, j0)), qb / (2 * dn_b - new_depth(s, k, 1, j0 + 1)));
+    }
+    if (i == nx - 1) {
+        const double2 ql = ld2(s.Qy, (long)(nx - 1) * ny + j0);
+        st2(s.Qy, (long)nx * ny + j0, ql.x * k.inv_3, ql.y * k.inv_3);
+    }
+}
+
+// ------------------------------------------------------------------------------------
 // Non-finite scan of a field; sets *flag to 1 if any element is NaN or Inf.
 // ------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_nonfinite(const double *__restrict__ x, long n, int *flag)
@@ -424,7 +557,10 @@ hipError_t launch_qxy_assemble(hipStream_t st, const Consts &k, const State &s, 
 hipError_t launch_qx_tail(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const Work &w,
                           double f0)
 {
-    hipLaunchKernelGGL(k_qx_tail, cell_grid(k.ny, k.nx), dim3(256), 0, st, k, m, s, w, f0);
+    if (k.ny % 2 == 0 && k.ny >= 4)
+        hipLaunchKernelGGL(k_qx_tail2, cell_grid(k.ny / 2, k.nx), dim3(256), 0, st, k, m, s, w, f0);
+    else
+        hipLaunchKernelGGL(k_qx_tail, cell_grid(k.ny, k.nx), dim3(256), 0, st, k, m, s, w, f0);
     return hipGetLastError();
 }
 hipError_t launch_qyy_assemble(hipStream_t st, const Consts &k, const Metrics &m, const State &s,
@@ -441,7 +577,10 @@ hipError_t launch_qyx_assemble(hipStream_t st, const Consts &k, const Metrics &m
 }
 hipError_t launch_qy_tail(hipStream_t st, const Consts &k, const State &s)
 {
-    hipLaunchKernelGGL(k_qy_tail, cell_grid(k.ny, k.nx), dim3(256), 0, st, k, s);
+    if (k.ny % 2 == 0 && k.ny >= 4)
+        hipLaunchKernelGGL(k_qy_tail2, cell_grid(k.ny / 2, k.nx), dim3(256), 0, st, k, s);
+    else
+        hipLaunchKernelGGL(k_qy_tail, cell_grid(k.ny, k.nx), dim3(256), 0, st, k, s);
     return hipGetLastError();
 }
 hipError_t launch_nonfinite(hipStream_t st, const double *x, long n, int *flag)
