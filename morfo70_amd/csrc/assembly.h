@@ -140,24 +140,105 @@ __host__ __device__ __forceinline__ void asm_qxx(const Consts &k, const Metrics 
     w.drag[e] = drag;
 }
 
-// Qx projector, setSolverQxy_f/_d/_up/_low, src/HydroQx.cpp:262-349.
-__host__ __device__ __forceinline__ void asm_qxy(const Consts &k, const State &s, const RunFields &rf, const Work &w, int i,
-                                        int j, double &a_out, double &b_out, double &c_out, double &f_out)
+// Loads of two adjacent doubles (an even column of a row whose length is even: 16-byte
+// aligned) and of two adjacent masks.
+__host__ __device__ __forceinline__ void ld_pair(const double *p, long e, double &a, double &b)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double2 v = *reinterpret_cast<const double2 *>(p + e);
+    a = v.x;
+    b = v.y;
+#else
+    a = p[e];
+    b = p[e + 1];
+#endif
+}
+__host__ __device__ __forceinline__ void ld_mask_pair(const uint8_t *p, long e, int &a, int &b)
+{
+    const uint16_t v = *reinterpret_cast<const uint16_t *>(p + e);
+    a = v & 0xff;
+    b = v >> 8;
+}
+
+// Qx projector, setSolverQxy_f/_d/_up/_low, src/HydroQx.cpp:262-349, in two parts: the
+// values element (i, j) reads (QxyIn) and the arithmetic on them (asm_qxy_vals).
+struct QxyIn {
+    int wj, wp, wm;                  // IsWater of the predictor at columns j, j+1, j-1
+    double drag, nud_x, dnudx_dy;    // DragTerm of the predictor, nuD_x, dnuDx_dy
+    double Dj0, Dj1, Dp0, Dp1, Dm0, Dm1;          // D rows i, i+1 at columns j, j+1, j-1
+    double Vcp0, Vcp1, Vcm0, Vcm1;               // V_c rows i, i+1 at columns j+1, j-1
+    double qx;                       // Qx* row i+1
+};
+
+__host__ __device__ __forceinline__ QxyIn qxy_load(const Consts &k, const State &s, const RunFields &rf, const Work &w,
+                                                   int i, int j)
 {
     const int ny = k.ny;
     const long r = (long)i * ny;
     const int jp = wrap_inc(j, ny), jm = wrap_dec(j, ny);
+    QxyIn q;
+    q.wj = w.w[r + j]; q.wp = w.w[r + jp]; q.wm = w.w[r + jm];
+    q.drag = w.drag[r + j];
+    q.nud_x = rf.nuD_x[r + j]; q.dnudx_dy = rf.dnuDx_dy[r + j];
+    q.Dj0 = s.D[r + j]; q.Dj1 = s.D[r + ny + j];
+    q.Dp0 = s.D[r + jp]; q.Dp1 = s.D[r + ny + jp];
+    q.Dm0 = s.D[r + jm]; q.Dm1 = s.D[r + ny + jm];
+    q.Vcp0 = s.Vc[r + jp]; q.Vcp1 = s.Vc[r + ny + jp];
+    q.Vcm0 = s.Vc[r + jm]; q.Vcm1 = s.Vc[r + ny + jm];
+    q.qx = s.Qx[r + ny + j];
+    return q;
+}
+
+// Elements j = e and e + 1 of row i (e even, ny even), from loads of column pairs.
+__host__ __device__ __forceinline__ void qxy_load2(const Consts &k, const State &s, const RunFields &rf, const Work &w,
+                                                   int i, int e, QxyIn &a, QxyIn &b)
+{
+    const int ny = k.ny;
+    const long r = (long)i * ny;
+    const int em = (e == 0) ? ny - 2 : e - 2, ep = (e + 2 == ny) ? 0 : e + 2;
+    int wmm, wm, w0, w1, wp, wpp;
+    ld_mask_pair(w.w, r + em, wmm, wm);
+    ld_mask_pair(w.w, r + e, w0, w1);
+    ld_mask_pair(w.w, r + ep, wp, wpp);
+    a.wj = w0; a.wp = w1; a.wm = wm;
+    b.wj = w1; b.wp = wp; b.wm = w0;
+    ld_pair(w.drag, r + e, a.drag, b.drag);
+    ld_pair(rf.nuD_x, r + e, a.nud_x, b.nud_x);
+    ld_pair(rf.dnuDx_dy, r + e, a.dnudx_dy, b.dnudx_dy);
+    double d0m, d0, d0n, d0p, d1m, d1, d1n, d1p, x;
+    ld_pair(s.D, r + em, x, d0m);
+    ld_pair(s.D, r + e, d0, d0n);
+    ld_pair(s.D, r + ep, d0p, x);
+    ld_pair(s.D, r + ny + em, x, d1m);
+    ld_pair(s.D, r + ny + e, d1, d1n);
+    ld_pair(s.D, r + ny + ep, d1p, x);
+    a.Dj0 = d0; a.Dj1 = d1; a.Dp0 = d0n; a.Dp1 = d1n; a.Dm0 = d0m; a.Dm1 = d1m;
+    b.Dj0 = d0n; b.Dj1 = d1n; b.Dp0 = d0p; b.Dp1 = d1p; b.Dm0 = d0; b.Dm1 = d1;
+    double v0m, v0, v0n, v0p, v1m, v1, v1n, v1p;
+    ld_pair(s.Vc, r + em, x, v0m);
+    ld_pair(s.Vc, r + e, v0, v0n);
+    ld_pair(s.Vc, r + ep, v0p, x);
+    ld_pair(s.Vc, r + ny + em, x, v1m);
+    ld_pair(s.Vc, r + ny + e, v1, v1n);
+    ld_pair(s.Vc, r + ny + ep, v1p, x);
+    a.Vcp0 = v0n; a.Vcp1 = v1n; a.Vcm0 = v0m; a.Vcm1 = v1m;
+    b.Vcp0 = v0p; b.Vcp1 = v1p; b.Vcm0 = v0; b.Vcm1 = v1;
+    ld_pair(s.Qx, r + ny + e, a.qx, b.qx);
+}
+
+__host__ __device__ __forceinline__ void asm_qxy_vals(const Consts &k, const QxyIn &q, double &a_out, double &b_out,
+                                                      double &c_out, double &f_out)
+{
     const double dt = k.dt;
-    const double *D = s.D;
-    const int wj = w.w[r + j], wp = w.w[r + jp], wm = w.w[r + jm];
-    const double drag = w.drag[r + j];
-    const double nud_x = rf.nuD_x[r + j], dnudx_dy = rf.dnuDx_dy[r + j];
-    const double inv_j = 1 / (0.5 * (D[r + j] + D[r + ny + j]));
-    const double inv_p = 1 / (0.5 * (D[r + jp] + D[r + ny + jp]));
-    const double inv_m = 1 / (0.5 * (D[r + jm] + D[r + ny + jm]));
-    const double v_p = 0.5 * (s.Vc[r + jp] + s.Vc[r + ny + jp]);
-    const double v_m = 0.5 * (s.Vc[r + jm] + s.Vc[r + ny + jm]);
-    const double qx = s.Qx[r + ny + j];
+    const int wj = q.wj, wp = q.wp, wm = q.wm;
+    const double drag = q.drag;
+    const double nud_x = q.nud_x, dnudx_dy = q.dnudx_dy;
+    const double inv_j = 1 / (0.5 * (q.Dj0 + q.Dj1));
+    const double inv_p = 1 / (0.5 * (q.Dp0 + q.Dp1));
+    const double inv_m = 1 / (0.5 * (q.Dm0 + q.Dm1));
+    const double v_p = 0.5 * (q.Vcp0 + q.Vcp1);
+    const double v_m = 0.5 * (q.Vcm0 + q.Vcm1);
+    const double qx = q.qx;
     f_out = wj * qx * (1 + dt * drag);
     const int isl = !wj;
     a_out = wj * ((1 + dt * drag) + dt * inv_j * nud_x * k.two_inv_dy2) + isl;
@@ -167,63 +248,155 @@ __host__ __device__ __forceinline__ void asm_qxy(const Consts &k, const State &s
     b_out = dt * (-v_m * k.inv_2dy - inv_m * (nud_x * k.inv_dy2_xy - dnudx_dy * k.inv_2dy)) * wwm;
 }
 
+__host__ __device__ __forceinline__ void asm_qxy(const Consts &k, const State &s, const RunFields &rf, const Work &w, int i,
+                                        int j, double &a_out, double &b_out, double &c_out, double &f_out)
+{
+    asm_qxy_vals(k, qxy_load(k, s, rf, w, i, j), a_out, b_out, c_out, f_out);
+}
+
 // Qy predictor, src/HydroQy.cpp:26-41 with :92-230; f and d use IsWater row 0
 // (src/HydroQy.cpp:140,170). IsWater (setWaterNodes_y) of every face is in w.w, written by
-// the Qx tail from guess_D and guess_Zs. Side output: DragTerm of row i.
-__host__ __device__ __forceinline__ void asm_qyy(const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                                        const Work &w, int i, int j, double &a_out, double &b_out, double &c_out,
-                                        double &f_out)
+// the Qx tail from guess_D and guess_Zs. In two parts: the values element (i, j) reads
+// (QyyIn) and the arithmetic on them (asm_qyy_vals). Side output: DragTerm of row i.
+struct QyyIn {
+    double g_m, g_j, g_p, g_pp;      // guess_D row i at columns j-1, j, j+1, j+2
+    int wy_m, wy_j, wy_p, w0;        // IsWater row i at faces j-1, j, j+1; row 0 at face j (Q1)
+    double zs0, zs1, dqx0, dqx1;     // Zs, dQx_dx row i at columns j, j+1
+    double u0j, u0p, u1j, u1p;       // U rows i, i+1 at columns j, j+1
+    double uc0, uc1;                 // U_c row i at columns j, j+1
+    double vm, vj, vp;               // V row i+1 at columns j-1, j, j+1
+    double Fy, Qy1, dnuDy_dx, nuD_y, cD, Urms, dnuD_dy;
+};
+
+__host__ __device__ __forceinline__ QyyIn qyy_load(const Consts &k, const State &s, const RunFields &rf, const Work &w,
+                                                   int i, int j)
 {
     const int ny = k.ny;
     const long r = (long)i * ny;
     const int jp = wrap_inc(j, ny), jm = wrap_dec(j, ny), jpp = wrap_inc(jp, ny);
-    const double dt = k.dt;
+    QyyIn q;
     const double *gD = w.gD + r;
-    const double g_m = gD[jm], g_j = gD[j], g_p = gD[jp], g_pp = gD[jpp];
-    // masks of row i at faces jm, j, jp
+    q.g_m = gD[jm]; q.g_j = gD[j]; q.g_p = gD[jp]; q.g_pp = gD[jpp];
     const uint8_t *wq = w.w + r;
-    const int wy_j = wq[j], wy_m = wq[jm], wy_p = wq[jp];
-    // mask of row 0 at face j (Q1)
-    const int w0 = w.w[j];
+    q.wy_j = wq[j]; q.wy_m = wq[jm]; q.wy_p = wq[jp];
+    q.w0 = w.w[j];
+    q.zs0 = s.Zs[r + j]; q.zs1 = s.Zs[r + jp];
+    q.dqx0 = s.dQx[r + j]; q.dqx1 = s.dQx[r + jp];
+    q.u0j = s.U[r + j]; q.u0p = s.U[r + jp]; q.u1j = s.U[r + ny + j]; q.u1p = s.U[r + ny + jp];
+    q.uc0 = s.Uc[r + j]; q.uc1 = s.Uc[r + jp];
+    q.vm = s.V[r + ny + jm]; q.vj = s.V[r + ny + j]; q.vp = s.V[r + ny + jp];
+    q.Fy = rf.Fy[r + j]; q.Qy1 = s.Qy[r + ny + j]; q.dnuDy_dx = rf.dnuDy_dx[r + j]; q.nuD_y = rf.nuD_y[r + j];
+    q.cD = rf.cD_y[r + j]; q.Urms = rf.Urms_y[r + j]; q.dnuD_dy = rf.dnuD_dy[r + j];
+    return q;
+}
+
+// Elements j = e and e + 1 of row i (e even, ny even), from loads of column pairs.
+__host__ __device__ __forceinline__ void qyy_load2(const Consts &k, const State &s, const RunFields &rf, const Work &w,
+                                                   int i, int e, QyyIn &a, QyyIn &b)
+{
+    const int ny = k.ny;
+    const long r = (long)i * ny;
+    const int em = (e == 0) ? ny - 2 : e - 2, ep = (e + 2 == ny) ? 0 : e + 2;
+    double x, gm, g0, g1, gp, gpp;
+    ld_pair(w.gD, r + em, x, gm);
+    ld_pair(w.gD, r + e, g0, g1);
+    ld_pair(w.gD, r + ep, gp, gpp);
+    a.g_m = gm; a.g_j = g0; a.g_p = g1; a.g_pp = gp;
+    b.g_m = g0; b.g_j = g1; b.g_p = gp; b.g_pp = gpp;
+    int wmm, wm, w0, w1, wp, wpp;
+    ld_mask_pair(w.w, r + em, wmm, wm);
+    ld_mask_pair(w.w, r + e, w0, w1);
+    ld_mask_pair(w.w, r + ep, wp, wpp);
+    a.wy_m = wm; a.wy_j = w0; a.wy_p = w1;
+    b.wy_m = w0; b.wy_j = w1; b.wy_p = wp;
+    ld_mask_pair(w.w, e, a.w0, b.w0);
+    double z0, z1, z2, q0, q1, q2;
+    ld_pair(s.Zs, r + e, z0, z1);
+    ld_pair(s.Zs, r + ep, z2, x);
+    ld_pair(s.dQx, r + e, q0, q1);
+    ld_pair(s.dQx, r + ep, q2, x);
+    a.zs0 = z0; a.zs1 = z1; b.zs0 = z1; b.zs1 = z2;
+    a.dqx0 = q0; a.dqx1 = q1; b.dqx0 = q1; b.dqx1 = q2;
+    double u00, u01, u02, u10, u11, u12;
+    ld_pair(s.U, r + e, u00, u01);
+    ld_pair(s.U, r + ep, u02, x);
+    ld_pair(s.U, r + ny + e, u10, u11);
+    ld_pair(s.U, r + ny + ep, u12, x);
+    a.u0j = u00; a.u0p = u01; a.u1j = u10; a.u1p = u11;
+    b.u0j = u01; b.u0p = u02; b.u1j = u11; b.u1p = u12;
+    double c0, c1, c2;
+    ld_pair(s.Uc, r + e, c0, c1);
+    ld_pair(s.Uc, r + ep, c2, x);
+    a.uc0 = c0; a.uc1 = c1; b.uc0 = c1; b.uc1 = c2;
+    double vm, v0, v1, vp;
+    ld_pair(s.V, r + ny + em, x, vm);
+    ld_pair(s.V, r + ny + e, v0, v1);
+    ld_pair(s.V, r + ny + ep, vp, x);
+    a.vm = vm; a.vj = v0; a.vp = v1;
+    b.vm = v0; b.vj = v1; b.vp = vp;
+    ld_pair(rf.Fy, r + e, a.Fy, b.Fy);
+    ld_pair(s.Qy, r + ny + e, a.Qy1, b.Qy1);
+    ld_pair(rf.dnuDy_dx, r + e, a.dnuDy_dx, b.dnuDy_dx);
+    ld_pair(rf.nuD_y, r + e, a.nuD_y, b.nuD_y);
+    ld_pair(rf.cD_y, r + e, a.cD, b.cD);
+    ld_pair(rf.Urms_y, r + e, a.Urms, b.Urms);
+    ld_pair(rf.dnuD_dy, r + e, a.dnuD_dy, b.dnuD_dy);
+}
+
+__host__ __device__ __forceinline__ void asm_qyy_vals(const Consts &k, const Metrics &m, const QyyIn &q, int i,
+                                                      double &a_out, double &b_out, double &c_out, double &f_out,
+                                                      double &drag_out)
+{
+    const double dt = k.dt;
+    const double g_m = q.g_m, g_j = q.g_j, g_p = q.g_p, g_pp = q.g_pp;
+    const int wy_j = q.wy_j, wy_m = q.wy_m, wy_p = q.wy_p;
+    const int w0 = q.w0;
     // Di row i+1 (getD_y(i)) at j, jm, jp
     const double Dy = 0.5 * (g_p + g_j);
     const double invDy = 1 / Dy;
     const double invDy_m = 1 / (0.5 * (g_j + g_m));
     const double invDy_p = 1 / (0.5 * (g_pp + g_p));
-    const double nud_y = rf.nuD_y[r + j];
+    const double nud_y = q.nuD_y;
     double fv;
     {   // setSolverQyy_f (:124-158)
         const double inv_dx = m.inv_dxx[i];
         const double x2c_d = m.x2c_d[i], x2c_up = m.x2c_up[i];
-        const double *zs = s.Zs + r, *dqx_dx = s.dQx + r, *u = s.U + r;
-        const double dzs_dy = k.inv_dy * (zs[jp] - zs[j]);
-        const double d2qx_dxy = k.inv_dy * (dqx_dx[jp] - dqx_dx[j]);
-        const double du_dy = k.inv_dy * (x2c_up * (u[ny + jp] - u[ny + j]) + x2c_d * (u[jp] - u[j]));
-        const double d2u_dxy = inv_dx * k.inv_dy * ((u[ny + jp] - u[ny + j]) - (u[jp] - u[j]));
-        fv = w0 * (rf.Fy[r + j] + s.Qy[r + ny + j] * k.inv_dt + G * Dy * (dt * d2qx_dxy - dzs_dy) +
-                   rf.dnuDy_dx[r + j] * du_dy + nud_y * d2u_dxy);
+        const double dzs_dy = k.inv_dy * (q.zs1 - q.zs0);
+        const double d2qx_dxy = k.inv_dy * (q.dqx1 - q.dqx0);
+        const double du_dy = k.inv_dy * (x2c_up * (q.u1p - q.u1j) + x2c_d * (q.u0p - q.u0j));
+        const double d2u_dxy = inv_dx * k.inv_dy * ((q.u1p - q.u1j) - (q.u0p - q.u0j));
+        fv = w0 * (q.Fy + q.Qy1 * k.inv_dt + G * Dy * (dt * d2qx_dxy - dzs_dy) + q.dnuDy_dx * du_dy +
+                   nud_y * d2u_dxy);
     }
     // Ui_j = C2Y(U_c row i) and the drag with V row i+1 (old)
-    const double uy = 0.5 * (s.Uc[r + jp] + s.Uc[r + j]);
-    const double *v = s.V + r + ny;
-    const double drag = drag_term(rf.cD_y[r + j], rf.Urms_y[r + j], uy, v[j], invDy, k);
+    const double uy = 0.5 * (q.uc1 + q.uc0);
+    const double drag = drag_term(q.cD, q.Urms, uy, q.vj, invDy, k);
     // setSolverQyy_d (:160-181)
     const int isl = !w0;
     const double dv = (k.inv_dt + drag + G * Dy * dt * k.two_inv_dy2 + 2 * nud_y * invDy * k.two_inv_dy2) * w0 + isl;
-    const double dnud_dy = rf.dnuD_dy[r + j];
+    const double dnud_dy = q.dnuD_dy;
     // setSolverQyy_low (:206-230)
     const double wwm = wy_j && wy_m;
-    const double lv = (0.5 * k.inv_dy * (-v[jm] + 2 * dnud_dy * invDy_m) -
+    const double lv = (0.5 * k.inv_dy * (-q.vm + 2 * dnud_dy * invDy_m) -
                        k.inv_dy2_yy * (G * Dy * dt + 2 * nud_y * invDy_m)) * wwm;
     // setSolverQyy_up (:183-204)
     const double wwp = wy_j && wy_p;
-    const double uv = (0.5 * k.inv_dy * (v[jp] - 2 * dnud_dy * invDy_p) -
+    const double uv = (0.5 * k.inv_dy * (q.vp - 2 * dnud_dy * invDy_p) -
                        k.inv_dy2_yy * (G * Dy * dt + 2 * nud_y * invDy_p)) * wwp;
     a_out = dv;
     b_out = lv;
     c_out = uv;
     f_out = fv;
-    w.drag[r + j] = drag;
+    drag_out = drag;
+}
+
+__host__ __device__ __forceinline__ void asm_qyy(const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
+                                        const Work &w, int i, int j, double &a_out, double &b_out, double &c_out,
+                                        double &f_out)
+{
+    double drag;
+    asm_qyy_vals(k, m, qyy_load(k, s, rf, w, i, j), i, a_out, b_out, c_out, f_out, drag);
+    w.drag[(long)i * k.ny + j] = drag;
 }
 
 // Qy projector, setSolverQyx_* (src/HydroQy.cpp:235-362) and setSolverQyx_BC

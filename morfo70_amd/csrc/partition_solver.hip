@@ -55,10 +55,25 @@ struct ProvArrays {   // coefficients stored in four arrays
         a = A[e]; b = B[e]; c = C[e]; f = F[e];
     }
 };
+// Row providers with get2() also yield elements kk and kk + 1 together (kk even, even
+// line length) from loads of column pairs.
+__device__ __forceinline__ void st_pair(double *p, long e, double a, double b)
+{
+    *reinterpret_cast<double2 *>(p + e) = make_double2(a, b);
+}
+
 struct ProvQxy {      // Qx projector, lines along y (rows): line = i, k = j
     Consts k; State s; RunFields rf; Work w;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qxy(k, s, rf, w, line, kk, a, b, c, f); }
+    __device__ __forceinline__ void get2(int line, int kk, double &a0, double &b0, double &c0, double &f0, double &a1,
+                                         double &b1, double &c1, double &f1) const
+    {
+        QxyIn q0, q1;
+        qxy_load2(k, s, rf, w, line, kk, q0, q1);
+        asm_qxy_vals(k, q0, a0, b0, c0, f0);
+        asm_qxy_vals(k, q1, a1, b1, c1, f1);
+    }
 };
 constexpr int CM = 8;   // rows per column segment in the three-pass column solver
 
@@ -130,6 +145,16 @@ struct ProvQyy {      // Qy predictor, rows
     Consts k; Metrics m; State s; RunFields rf; Work w;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qyy(k, m, s, rf, w, line, kk, a, b, c, f); }
+    __device__ __forceinline__ void get2(int line, int kk, double &a0, double &b0, double &c0, double &f0, double &a1,
+                                         double &b1, double &c1, double &f1) const
+    {
+        QyyIn q0, q1;
+        qyy_load2(k, s, rf, w, line, kk, q0, q1);
+        double d0, d1;
+        asm_qyy_vals(k, m, q0, line, a0, b0, c0, f0, d0);
+        asm_qyy_vals(k, m, q1, line, a1, b1, c1, f1, d1);
+        st_pair(w.drag, (long)line * k.ny + kk, d0, d1);
+    }
 };
 
 // ---------------------------------------------------------------------------------
@@ -344,20 +369,44 @@ template <int RB> struct RowsLds {     // RB threads per block, lines up to RB*R
 };
 __device__ __forceinline__ int lpad(int e) { return e + (e >> 3); }
 
+template <class P, class = void> struct HasGet2 : std::false_type {};
+template <class P> struct HasGet2<P, std::void_t<decltype(&P::get2)>> : std::true_type {};
+
 template <int RB, class Prov>
 __device__ __forceinline__ void rows_lds_body(const Prov &prov, double *__restrict__ Y, long ldy, int n, int S,
                                               int periodic, int line, double (*sh)[RowsLds<RB>::LDS])
 {
     const int t = threadIdx.x;
     double cu[RM], cf[RM];   // upper and rhs of the elements this thread assembles
+    // Elements of pass it: t + it*RB, or with get2() and an even line the pairs
+    // 2*(t + it*RB) + {0, 1} (passes it < RM/2).
+    bool pairs = false;
+    if constexpr (HasGet2<Prov>::value) {
+        pairs = (n & 1) == 0;
+        if (pairs) {
 #pragma unroll
-    for (int it = 0; it < RM; ++it) {
-        const int e = t + it * RB;
-        if (e < n) {
-            double a, b;
-            prov.get(line, e, a, b, cu[it], cf[it]);
-            const int p = lpad(e);
-            sh[0][p] = a; sh[1][p] = b;
+            for (int it = 0; it < RM / 2; ++it) {
+                const int e = 2 * (t + it * RB);
+                if (e < n) {
+                    double a0, b0, a1, b1;
+                    prov.get2(line, e, a0, b0, cu[2 * it], cf[2 * it], a1, b1, cu[2 * it + 1], cf[2 * it + 1]);
+                    const int p0 = lpad(e), p1 = lpad(e + 1);
+                    sh[0][p0] = a0; sh[1][p0] = b0;
+                    sh[0][p1] = a1; sh[1][p1] = b1;
+                }
+            }
+        }
+    }
+    if (!pairs) {
+#pragma unroll
+        for (int it = 0; it < RM; ++it) {
+            const int e = t + it * RB;
+            if (e < n) {
+                double a, b;
+                prov.get(line, e, a, b, cu[it], cf[it]);
+                const int p = lpad(e);
+                sh[0][p] = a; sh[1][p] = b;
+            }
         }
     }
     __syncthreads();
@@ -377,12 +426,24 @@ __device__ __forceinline__ void rows_lds_body(const Prov &prov, double *__restri
         }
     }
     __syncthreads();
+    if (pairs) {
 #pragma unroll
-    for (int it = 0; it < RM; ++it) {
-        const int e = t + it * RB;
-        if (e < n) {
-            const int p = lpad(e);
-            sh[0][p] = cu[it]; sh[1][p] = cf[it];
+        for (int it = 0; it < RM / 2; ++it) {
+            const int e = 2 * (t + it * RB);
+            if (e < n) {
+                const int p0 = lpad(e), p1 = lpad(e + 1);
+                sh[0][p0] = cu[2 * it]; sh[1][p0] = cf[2 * it];
+                sh[0][p1] = cu[2 * it + 1]; sh[1][p1] = cf[2 * it + 1];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int it = 0; it < RM; ++it) {
+            const int e = t + it * RB;
+            if (e < n) {
+                const int p = lpad(e);
+                sh[0][p] = cu[it]; sh[1][p] = cf[it];
+            }
         }
     }
     __syncthreads();
@@ -714,11 +775,29 @@ struct GProvQxy {
     const ScenDesc *d;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qxy(d->k, d->s, d->rf, d->w, line, kk, a, b, c, f); }
+    __device__ __forceinline__ void get2(int line, int kk, double &a0, double &b0, double &c0, double &f0, double &a1,
+                                         double &b1, double &c1, double &f1) const
+    {
+        QxyIn q0, q1;
+        qxy_load2(d->k, d->s, d->rf, d->w, line, kk, q0, q1);
+        asm_qxy_vals(d->k, q0, a0, b0, c0, f0);
+        asm_qxy_vals(d->k, q1, a1, b1, c1, f1);
+    }
 };
 struct GProvQyy {
     const ScenDesc *d;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qyy(d->k, d->m, d->s, d->rf, d->w, line, kk, a, b, c, f); }
+    __device__ __forceinline__ void get2(int line, int kk, double &a0, double &b0, double &c0, double &f0, double &a1,
+                                         double &b1, double &c1, double &f1) const
+    {
+        QyyIn q0, q1;
+        qyy_load2(d->k, d->s, d->rf, d->w, line, kk, q0, q1);
+        double d0, d1;
+        asm_qyy_vals(d->k, d->m, q0, line, a0, b0, c0, f0, d0);
+        asm_qyy_vals(d->k, d->m, q1, line, a1, b1, c1, f1, d1);
+        st_pair(d->w.drag, (long)line * d->k.ny + kk, d0, d1);
+    }
 };
 
 template <int WHICH, int RB>   // WHICH 0: Qx projector rows, 1: Qy predictor rows
