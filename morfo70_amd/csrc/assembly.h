@@ -167,7 +167,7 @@ struct QxyIn {
     double drag, nud_x, dnudx_dy;    // DragTerm of the predictor, nuD_x, dnuDx_dy
     double Dj0, Dj1, Dp0, Dp1, Dm0, Dm1;          // D rows i, i+1 at columns j, j+1, j-1
     double Vcp0, Vcp1, Vcm0, Vcm1;               // V_c rows i, i+1 at columns j+1, j-1
-    double qx;                       // Qx* row i+1
+    double qx;                       // Qx* row i+1 (filled by the caller)
 };
 
 __host__ __device__ __forceinline__ QxyIn qxy_load(const Consts &k, const State &s, const RunFields &rf, const Work &w,
@@ -185,7 +185,7 @@ __host__ __device__ __forceinline__ QxyIn qxy_load(const Consts &k, const State 
     q.Dm0 = s.D[r + jm]; q.Dm1 = s.D[r + ny + jm];
     q.Vcp0 = s.Vc[r + jp]; q.Vcp1 = s.Vc[r + ny + jp];
     q.Vcm0 = s.Vc[r + jm]; q.Vcm1 = s.Vc[r + ny + jm];
-    q.qx = s.Qx[r + ny + j];
+    q.qx = 0.0;
     return q;
 }
 
@@ -223,7 +223,7 @@ __host__ __device__ __forceinline__ void qxy_load2(const Consts &k, const State 
     ld_pair(s.Vc, r + ny + ep, v1p, x);
     a.Vcp0 = v0n; a.Vcp1 = v1n; a.Vcm0 = v0m; a.Vcm1 = v1m;
     b.Vcp0 = v0p; b.Vcp1 = v1p; b.Vcm0 = v0; b.Vcm1 = v1;
-    ld_pair(s.Qx, r + ny + e, a.qx, b.qx);
+    a.qx = b.qx = 0.0;
 }
 
 __host__ __device__ __forceinline__ void asm_qxy_vals(const Consts &k, const QxyIn &q, double &a_out, double &b_out,
@@ -251,7 +251,9 @@ __host__ __device__ __forceinline__ void asm_qxy_vals(const Consts &k, const Qxy
 __host__ __device__ __forceinline__ void asm_qxy(const Consts &k, const State &s, const RunFields &rf, const Work &w, int i,
                                         int j, double &a_out, double &b_out, double &c_out, double &f_out)
 {
-    asm_qxy_vals(k, qxy_load(k, s, rf, w, i, j), a_out, b_out, c_out, f_out);
+    QxyIn q = qxy_load(k, s, rf, w, i, j);
+    q.qx = s.Qx[(long)(i + 1) * k.ny + j];
+    asm_qxy_vals(k, q, a_out, b_out, c_out, f_out);
 }
 
 // Qy predictor, src/HydroQy.cpp:26-41 with :92-230; f and d use IsWater row 0

@@ -184,12 +184,14 @@ int run_step(m70h *h)
     hipStream_t st = h->stream;
     if (h->solver == M70H_SOLVER_FUSED) {
         // x-sweeps: three coalesced passes (assemble+eliminate, reduced solve, back-sub);
-        // y-sweeps: one launch each, the row's coefficients staged through LDS.
-        for (int pass = COLS_ELIM; pass <= COLS_BACKSUB; ++pass)
+        // y-sweeps: one launch each, the row's coefficients staged through LDS. The Qx
+        // projector forms the Qx predictor's solution itself from the first two passes.
+        for (int pass = COLS_ELIM; pass <= COLS_REDUCED; ++pass)
             if ((rc = timed(h, K_COLS_QX_ELIM + pass,
                             [&] { return launch_cols3_qxx(st, k, h->m, s, h->rf, w, f0, h->cw, h->tile_cols, pass); })))
                 return rc;
-        if ((rc = timed(h, K_FUSED_QXY, [&] { return launch_lds_qxy(st, k, s, h->rf, w, h->rows_block); }))) return rc;
+        if ((rc = timed(h, K_FUSED_QXY, [&] { return launch_lds_qxy_bs(st, k, s, h->rf, w, h->cw, h->rows_block); })))
+            return rc;
         if ((rc = timed(h, K_QX_TAIL, [&] { return launch_qx_tail(st, k, h->m, s, w, f0); }))) return rc;
         if ((rc = timed(h, K_FUSED_QYY, [&] { return launch_lds_qyy(st, k, h->m, s, h->rf, w, h->rows_block); }))) return rc;
         for (int pass = COLS_ELIM; pass <= COLS_BACKSUB; ++pass)

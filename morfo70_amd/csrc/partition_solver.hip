@@ -71,6 +71,73 @@ struct ProvQxy {      // Qx projector, lines along y (rows): line = i, k = j
     {
         QxyIn q0, q1;
         qxy_load2(k, s, rf, w, line, kk, q0, q1);
+        ld_pair(s.Qx, (long)(line + 1) * k.ny + kk, q0.qx, q1.qx);
+        asm_qxy_vals(k, q0, a0, b0, c0, f0);
+        asm_qxy_vals(k, q1, a1, b1, c1, f1);
+    }
+};
+
+// Unknown kk of column j of the three-pass column solve, formed as the back-substitution
+// pass forms it (k_cols_backsub): the segment's boundary values come from the reduced
+// solution X, interior ones from the eliminated segment E0..E2. n unknowns per column.
+__device__ __forceinline__ void cols_segment_of(const ColsWork &cw, int n, int kk, int &sg, int &q, int &m)
+{
+    const int nseg = cw.nseg;
+    sg = (int)(((long)kk * nseg) / n);
+    if ((int)(((long)(sg + 1) * n) / nseg) <= kk) ++sg;
+    const int k0 = (int)(((long)sg * n) / nseg), k1 = (int)(((long)(sg + 1) * n) / nseg);
+    q = kk - k0;
+    m = k1 - k0;
+}
+__device__ __forceinline__ void cols_value2(const ColsWork &cw, int n, int ny, int kk, int j, double &x0, double &x1)
+{
+    int sg, q, m;
+    cols_segment_of(cw, n, kk, sg, q, m);
+    if (q == 0) {
+        ld_pair(cw.X, (long)(2 * sg) * ny + j, x0, x1);
+    } else if (q == m - 1) {
+        ld_pair(cw.X, (long)(2 * sg + 1) * ny + j, x0, x1);
+    } else {
+        double F0, F1, L0, L1, e00, e01, e10, e11, e20, e21;
+        ld_pair(cw.X, (long)(2 * sg) * ny + j, F0, F1);
+        ld_pair(cw.X, (long)(2 * sg + 1) * ny + j, L0, L1);
+        const long o = (long)kk * ny + j;
+        ld_pair(cw.E0, o, e00, e01);
+        ld_pair(cw.E1, o, e10, e11);
+        ld_pair(cw.E2, o, e20, e21);
+        x0 = e20 - e00 * F0 - e10 * L0;
+        x1 = e21 - e01 * F1 - e11 * L1;
+    }
+}
+__device__ __forceinline__ double cols_value(const ColsWork &cw, int n, int ny, int kk, int j)
+{
+    int sg, q, m;
+    cols_segment_of(cw, n, kk, sg, q, m);
+    const double xF = cw.X[(long)(2 * sg) * ny + j];
+    const double xL = cw.X[(long)(2 * sg + 1) * ny + j];
+    if (q == 0) return xF;
+    if (q == m - 1) return xL;
+    const long o = (long)kk * ny + j;
+    return cw.E2[o] - cw.E0[o] * xF - cw.E1[o] * xL;
+}
+
+// Qx projector rows with Qx* (the predictor's solution, row i+1 = unknown i of each column)
+// formed from the predictor's three-pass column solve instead of read back after a
+// back-substitution pass: the FUSED step skips that pass and never stores Qx*.
+struct ProvQxyBs {
+    Consts k; State s; RunFields rf; Work w; ColsWork cw;
+    __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
+    {
+        QxyIn q = qxy_load(k, s, rf, w, line, kk);
+        q.qx = cols_value(cw, k.nx - 1, k.ny, line, kk);
+        asm_qxy_vals(k, q, a, b, c, f);
+    }
+    __device__ __forceinline__ void get2(int line, int kk, double &a0, double &b0, double &c0, double &f0, double &a1,
+                                         double &b1, double &c1, double &f1) const
+    {
+        QxyIn q0, q1;
+        qxy_load2(k, s, rf, w, line, kk, q0, q1);
+        cols_value2(cw, k.nx - 1, k.ny, line, kk, q0.qx, q1.qx);
         asm_qxy_vals(k, q0, a0, b0, c0, f0);
         asm_qxy_vals(k, q1, a1, b1, c1, f1);
     }
@@ -780,6 +847,7 @@ struct GProvQxy {
     {
         QxyIn q0, q1;
         qxy_load2(d->k, d->s, d->rf, d->w, line, kk, q0, q1);
+        ld_pair(d->s.Qx, (long)(line + 1) * d->k.ny + kk, q0.qx, q1.qx);
         asm_qxy_vals(d->k, q0, a0, b0, c0, f0);
         asm_qxy_vals(d->k, q1, a1, b1, c1, f1);
     }
@@ -937,6 +1005,11 @@ static hipError_t launch_rows_lds(hipStream_t st, const Prov &prov, double *Y, l
 hipError_t launch_lds_qxy(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w, int rb)
 {
     return launch_rows_lds(st, ProvQxy{k, s, rf, w}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true, rb);
+}
+hipError_t launch_lds_qxy_bs(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w,
+                             const ColsWork &cw, int rb)
+{
+    return launch_rows_lds(st, ProvQxyBs{k, s, rf, w, cw}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true, rb);
 }
 hipError_t launch_lds_qyy(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
                           const Work &w, int rb)
