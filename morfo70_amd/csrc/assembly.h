@@ -160,6 +160,50 @@ __host__ __device__ __forceinline__ void ld_mask_pair(const uint8_t *p, long e, 
     b = v >> 8;
 }
 
+// Unknown kk of column j of the three-pass column solve, formed as the back-substitution
+// pass forms it (k_cols_backsub): the segment's boundary values come from the reduced
+// solution X, interior ones from the eliminated segment E0..E2. n unknowns per column.
+__host__ __device__ __forceinline__ void cols_segment_of(const ColsWork &cw, int n, int kk, int &sg, int &q, int &m)
+{
+    const int nseg = cw.nseg;
+    sg = (int)(((long)kk * nseg) / n);
+    if ((int)(((long)(sg + 1) * n) / nseg) <= kk) ++sg;
+    const int k0 = (int)(((long)sg * n) / nseg), k1 = (int)(((long)(sg + 1) * n) / nseg);
+    q = kk - k0;
+    m = k1 - k0;
+}
+__host__ __device__ __forceinline__ void cols_value2(const ColsWork &cw, int n, int ny, int kk, int j, double &x0, double &x1)
+{
+    int sg, q, m;
+    cols_segment_of(cw, n, kk, sg, q, m);
+    if (q == 0) {
+        ld_pair(cw.X, (long)(2 * sg) * ny + j, x0, x1);
+    } else if (q == m - 1) {
+        ld_pair(cw.X, (long)(2 * sg + 1) * ny + j, x0, x1);
+    } else {
+        double F0, F1, L0, L1, e00, e01, e10, e11, e20, e21;
+        ld_pair(cw.X, (long)(2 * sg) * ny + j, F0, F1);
+        ld_pair(cw.X, (long)(2 * sg + 1) * ny + j, L0, L1);
+        const long o = (long)kk * ny + j;
+        ld_pair(cw.E0, o, e00, e01);
+        ld_pair(cw.E1, o, e10, e11);
+        ld_pair(cw.E2, o, e20, e21);
+        x0 = e20 - e00 * F0 - e10 * L0;
+        x1 = e21 - e01 * F1 - e11 * L1;
+    }
+}
+__host__ __device__ __forceinline__ double cols_value(const ColsWork &cw, int n, int ny, int kk, int j)
+{
+    int sg, q, m;
+    cols_segment_of(cw, n, kk, sg, q, m);
+    const double xF = cw.X[(long)(2 * sg) * ny + j];
+    const double xL = cw.X[(long)(2 * sg + 1) * ny + j];
+    if (q == 0) return xF;
+    if (q == m - 1) return xL;
+    const long o = (long)kk * ny + j;
+    return cw.E2[o] - cw.E0[o] * xF - cw.E1[o] * xL;
+}
+
 // Qx projector, setSolverQxy_f/_d/_up/_low, src/HydroQx.cpp:262-349, in two parts: the
 // values element (i, j) reads (QxyIn) and the arithmetic on them (asm_qxy_vals).
 struct QxyIn {

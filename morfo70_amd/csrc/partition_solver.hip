@@ -77,50 +77,6 @@ struct ProvQxy {      // Qx projector, lines along y (rows): line = i, k = j
     }
 };
 
-// Unknown kk of column j of the three-pass column solve, formed as the back-substitution
-// pass forms it (k_cols_backsub): the segment's boundary values come from the reduced
-// solution X, interior ones from the eliminated segment E0..E2. n unknowns per column.
-__device__ __forceinline__ void cols_segment_of(const ColsWork &cw, int n, int kk, int &sg, int &q, int &m)
-{
-    const int nseg = cw.nseg;
-    sg = (int)(((long)kk * nseg) / n);
-    if ((int)(((long)(sg + 1) * n) / nseg) <= kk) ++sg;
-    const int k0 = (int)(((long)sg * n) / nseg), k1 = (int)(((long)(sg + 1) * n) / nseg);
-    q = kk - k0;
-    m = k1 - k0;
-}
-__device__ __forceinline__ void cols_value2(const ColsWork &cw, int n, int ny, int kk, int j, double &x0, double &x1)
-{
-    int sg, q, m;
-    cols_segment_of(cw, n, kk, sg, q, m);
-    if (q == 0) {
-        ld_pair(cw.X, (long)(2 * sg) * ny + j, x0, x1);
-    } else if (q == m - 1) {
-        ld_pair(cw.X, (long)(2 * sg + 1) * ny + j, x0, x1);
-    } else {
-        double F0, F1, L0, L1, e00, e01, e10, e11, e20, e21;
-        ld_pair(cw.X, (long)(2 * sg) * ny + j, F0, F1);
-        ld_pair(cw.X, (long)(2 * sg + 1) * ny + j, L0, L1);
-        const long o = (long)kk * ny + j;
-        ld_pair(cw.E0, o, e00, e01);
-        ld_pair(cw.E1, o, e10, e11);
-        ld_pair(cw.E2, o, e20, e21);
-        x0 = e20 - e00 * F0 - e10 * L0;
-        x1 = e21 - e01 * F1 - e11 * L1;
-    }
-}
-__device__ __forceinline__ double cols_value(const ColsWork &cw, int n, int ny, int kk, int j)
-{
-    int sg, q, m;
-    cols_segment_of(cw, n, kk, sg, q, m);
-    const double xF = cw.X[(long)(2 * sg) * ny + j];
-    const double xL = cw.X[(long)(2 * sg + 1) * ny + j];
-    if (q == 0) return xF;
-    if (q == m - 1) return xL;
-    const long o = (long)kk * ny + j;
-    return cw.E2[o] - cw.E0[o] * xF - cw.E1[o] * xL;
-}
-
 // Qx projector rows with Qx* (the predictor's solution, row i+1 = unknown i of each column)
 // formed from the predictor's three-pass column solve instead of read back after a
 // back-substitution pass: the FUSED step skips that pass and never stores Qx*.
