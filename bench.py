@@ -35,7 +35,7 @@ KERNEL_BYTES_PER_CELL = {
     "thomas_qx": 8 * 5,                     # a,b,c,f in; Qx* out
     "qxy_assemble": 8 * 8 + 1 + 8 * 4,      # D,Vc,Qx*,nuD_x,dnuDx_dy,drag (+2 rows) in, mask in; a,b,c,f out
     "periodic_qx": 8 * 5,                   # a,b,c,f in; Qx out
-    "qx_tail": 8 * 4 + 8 * 6 + 1,           # Qx,D,dQy,Zs in; Qx0,dQx,gZs,gD,U,Uc out, Qy mask out (nx rows)
+    "qx_tail": 8 * 4 + 8 * 5 + 1,           # Qx,D,dQy,Zs in; Qx0,dQx,gD,U,Uc out, Qy mask out (nx rows)
     "qyy_assemble": 8 * 14 + 1 + 8 * 5,     # gD,Zs,dQx,U(2),Qy,Uc,V,Fy,nuD_y,dnuDy_dx,dnuD_dy,cD_y,Urms_y, mask in; a,b,c,f,drag out
     "periodic_qy": 8 * 5,
     "qyx_assemble": 8 * 11 + 1 + 8 * 4,

@@ -34,7 +34,7 @@ struct StepTape {
     double *f;                    // right-hand side scratch of the taped solves
     double *dragx, *dragy;        // DragTerm of the x and y half-steps
     uint8_t *wx, *wy;             // IsWater of the x and y half-steps
-    double *gD, *gZs;             // guess_D, guess_Zs
+    double *gD;                   // guess_D
 };
 
 // Adjoints of the ten state fields (same shapes as State).

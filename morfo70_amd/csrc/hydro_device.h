@@ -75,7 +75,8 @@ struct State {
 // Per-step work arrays.
 struct Work {
     double *a, *b, *c, *f;     // tridiagonal coefficients (diag, low, up, rhs), row-major F
-    double *gZs, *gD;          // guess_Zs, guess_D (src/Hydro.cpp:230-250)
+    double *gD;                // guess_D (src/Hydro.cpp:230-250); guess_Zs is used only for the
+                               // Qy predictor's IsWater, which the Qx tail forms, so it is not kept
     double *drag;              // DragTerm of the predictor, reused by the projector
     uint8_t *w;                // IsWater of the predictor, reused by the projector
 };

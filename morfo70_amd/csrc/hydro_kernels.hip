@@ -177,6 +177,7 @@ __global__ void __launch_bounds__(256) k_qxy_assemble(Consts k, State s, RunFiel
 // (src/HydroQx.cpp:80), solveZs_guess (src/Hydro.cpp:230-250), getU0/getU
 // (src/HydroQx.cpp:420-451), X2C (include/MeshOperations.h:36-47), and the Qy
 // predictor's IsWater (setWaterNodes_y, src/HydroQy.cpp:92-122) into w.w. Rows 0..nx-1.
+// guess_Zs enters only that mask and is not stored.
 // ------------------------------------------------------------------------------------
 __host__ __device__ __forceinline__ double qx_at(const State &s, const Consts &k, long rowoff, int row, int j,
                                         double f0)
@@ -236,7 +237,6 @@ __host__ __device__ __forceinline__ void qx_tail_body(const Consts &k, const Met
         w.w[r + j] = (uint8_t)wet_rule(gd[1], gd_p, gzs_i, gzs_p, s.dQy[r + j], s.dQy[r + jp], k.Dmask);
     }
     s.dQx[r + j] = dq_i;
-    w.gZs[r + j] = gzs_i;
     w.gD[r + j] = gd[1];
     s.U[r + j] = ui;
     s.Uc[r + j] = m.x2c_d[i] * ui + m.x2c_up[i] * ui1;
@@ -412,7 +412,6 @@ __global__ void __launch_bounds__(256) k_qx_tail2(Consts k, Metrics m, State s, 
     w.w[r + j0] = (uint8_t)wa;
     w.w[r + j0 + 1] = (uint8_t)wb;
     st2(s.dQx, r + j0, dq_i.x, dq_i.y);
-    st2(w.gZs, r + j0, gza, gzb);
     st2(w.gD, r + j0, gd[1].x, gd[1].y);
     st2(s.U, r + j0, ua, ub);
     st2(s.Uc, r + j0, m.x2c_d[i] * ua + m.x2c_up[i] * u1a, m.x2c_d[i] * ub + m.x2c_up[i] * u1b);

@@ -440,7 +440,7 @@ int m70h_create(int nx, int ny, const double *x, double dy, const m70h_params *p
                       &h->rf.dnuDy_dx};
     for (auto q : rfp)
         if ((e = dalloc(h, q, F)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc run fields"));
-    double **wk[] = {&h->w.a, &h->w.b, &h->w.c, &h->w.f, &h->w.gZs, &h->w.gD, &h->w.drag};
+    double **wk[] = {&h->w.a, &h->w.b, &h->w.c, &h->w.f, &h->w.gD, &h->w.drag};
     for (auto q : wk)
         if ((e = dalloc(h, q, F)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc work"));
     if ((e = dalloc(h, &h->w.w, F)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc mask"));
@@ -1075,7 +1075,7 @@ hipError_t adj_alloc_tape(AdjCtx *a, StepTape &T, long F, double *shared_f)
 {
     hipError_t e;
     double **tp[] = {&T.a1, &T.b1, &T.c1, &T.qxs, &T.a2, &T.b2, &T.c2, &T.a3, &T.b3, &T.c3, &T.qys,
-                     &T.a4, &T.b4, &T.c4, &T.dragx, &T.dragy, &T.gD, &T.gZs};
+                     &T.a4, &T.b4, &T.c4, &T.dragx, &T.dragy, &T.gD};
     for (auto q : tp)
         if ((e = adj_alloc(a, q, F)) != hipSuccess) return e;
     double *mk = nullptr;
@@ -1213,7 +1213,6 @@ int adj_taped_step(m70h *h, int t, const StepTape &T)
     const long nF = (long)(nx - 1) * ny;
     Work w = h->w;
     w.gD = T.gD;
-    w.gZs = T.gZs;
     Work wx = w, wy = w;
     wx.w = T.wx; wx.drag = T.dragx;
     wy.w = T.wy; wy.drag = T.dragy;
