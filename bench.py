@@ -42,7 +42,8 @@ KERNEL_BYTES_PER_CELL = {
     "thomas_qy": 8 * 5,
     "qy_tail": 8 * 5 + 8 * 5,               # Qy,dQx,D,Zs in; dQy,Zs,D,V,Vc out (nx rows)
     # FUSED family: y-sweeps in one launch (assembly inputs in, solution out)
-    "fused_qxy": 8 * 8 + 1 + 8,             # D(2),Vc(2),Qx*,nuD_x,dnuDx_dy,drag in, mask in; Qx out
+    # the Qx projector forms Qx* from the predictor's column passes: E0..E2 and X (2 B) in
+    "fused_qxy": 8 * 10 + 2 + 1 + 8,        # D(2),Vc(2),E0..E2,nuD_x,dnuDx_dy,drag in, X, mask in; Qx out
     "fused_qyy": 8 * 14 + 1 + 8 * 2,        # the qyy_assemble inputs; Qy*, drag out
     # FUSED x-sweeps, pass 1: assembly inputs in; drag, mask, E0..E2 out, plus the two
     # reduced rows (4 values each) per 8-row segment = 8 B per cell
