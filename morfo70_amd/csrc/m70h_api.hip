@@ -1219,11 +1219,16 @@ int adj_taped_step(m70h *h, int t, const StepTape &T)
     hipError_t e;
     wx.a = T.a1; wx.b = T.b1; wx.c = T.c1; wx.f = T.f;
     if ((e = launch_qxx_assemble(st, k, h->m, s, h->rf, wx, f0)) != hipSuccess) return hip_fail(h, e, "adjoint tape qxx");
-    if ((e = launch_partition_cols(st, T.a1, T.b1, T.c1, T.f, s.Qx + ny, nx - 1, ny, ny, h->tile_cols)) != hipSuccess)
+    // the predictors solve straight onto the tape (Qx*, Qy*: unknown i is row i+1); the
+    // projector assemblies read them there through a state whose Qx (Qy) points one row
+    // before the tape, since they read rows 1..nx-1 only
+    (void)nF;
+    if ((e = launch_partition_cols(st, T.a1, T.b1, T.c1, T.f, T.qxs, nx - 1, ny, ny, h->tile_cols)) != hipSuccess)
         return hip_fail(h, e, "adjoint tape thomas x");
-    HIPCHK(h, hipMemcpyAsync(T.qxs, s.Qx + ny, (size_t)nF * sizeof(double), hipMemcpyDeviceToDevice, st));
     wx.a = T.a2; wx.b = T.b2; wx.c = T.c2;
-    if ((e = launch_qxy_assemble(st, k, s, h->rf, wx)) != hipSuccess) return hip_fail(h, e, "adjoint tape qxy");
+    State sx = s;
+    sx.Qx = T.qxs - ny;
+    if ((e = launch_qxy_assemble(st, k, sx, h->rf, wx)) != hipSuccess) return hip_fail(h, e, "adjoint tape qxy");
     if ((e = launch_partition_rows(st, T.a2, T.b2, T.c2, T.f, ny, s.Qx + ny, ny, nx - 1, ny, true, h->tile_rows)) !=
         hipSuccess)
         return hip_fail(h, e, "adjoint tape periodic x");
@@ -1234,12 +1239,13 @@ int adj_taped_step(m70h *h, int t, const StepTape &T)
     }
     wy.a = T.a3; wy.b = T.b3; wy.c = T.c3; wy.f = T.f;
     if ((e = launch_qyy_assemble(st, k, h->m, s, h->rf, wy)) != hipSuccess) return hip_fail(h, e, "adjoint tape qyy");
-    if ((e = launch_partition_rows(st, T.a3, T.b3, T.c3, T.f, ny, s.Qy + ny, ny, nx - 1, ny, true, h->tile_rows)) !=
+    if ((e = launch_partition_rows(st, T.a3, T.b3, T.c3, T.f, ny, T.qys, ny, nx - 1, ny, true, h->tile_rows)) !=
         hipSuccess)
         return hip_fail(h, e, "adjoint tape periodic y");
-    HIPCHK(h, hipMemcpyAsync(T.qys, s.Qy + ny, (size_t)nF * sizeof(double), hipMemcpyDeviceToDevice, st));
     wy.a = T.a4; wy.b = T.b4; wy.c = T.c4;
-    if ((e = launch_qyx_assemble(st, k, h->m, s, h->rf, wy)) != hipSuccess) return hip_fail(h, e, "adjoint tape qyx");
+    State sy = s;
+    sy.Qy = T.qys - ny;
+    if ((e = launch_qyx_assemble(st, k, h->m, sy, h->rf, wy)) != hipSuccess) return hip_fail(h, e, "adjoint tape qyx");
     if ((e = launch_partition_cols(st, T.a4, T.b4, T.c4, T.f, s.Qy + ny, nx - 1, ny, ny, h->tile_cols)) != hipSuccess)
         return hip_fail(h, e, "adjoint tape thomas y");
     if ((e = launch_qy_tail(st, k, s)) != hipSuccess) return hip_fail(h, e, "adjoint tape qy tail");

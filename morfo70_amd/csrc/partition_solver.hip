@@ -579,6 +579,10 @@ static hipError_t dispatch_cols(hipStream_t st, const Prov &prov, double *Y, lon
     return launch_partition<true, 256, 16>(st, prov, Y, ldy, n, nL, false);
 }
 
+template <class Prov>
+static hipError_t launch_rows_lds(hipStream_t st, const Prov &prov, double *Y, long ldy, int n, int nL, bool periodic,
+                                  int rb);
+
 // Row lines (y-sweeps): tile = unknowns per thread (8 or 16; 0 = default 8).
 template <class Prov>
 static hipError_t dispatch_rows(hipStream_t st, const Prov &prov, double *Y, long ldy, int n, int nL, bool periodic,
@@ -586,6 +590,9 @@ static hipError_t dispatch_rows(hipStream_t st, const Prov &prov, double *Y, lon
 {
     if (tile == 16 && pick_segments(n, 16, 256) > 0)
         return launch_partition<false, 256, 16>(st, prov, Y, ldy, n, nL, periodic);
+    // rows up to 4096: staged through LDS (coalesced loads, two workgroups per CU); the
+    // same segments and arithmetic as the partition kernel below
+    if (pick_segments(n, RM, 512) > 0) return launch_rows_lds(st, prov, Y, ldy, n, nL, periodic, 0);
     if (pick_segments(n, 8, 512) > 0) return launch_partition<false, 512, 8>(st, prov, Y, ldy, n, nL, periodic);
     return launch_partition<false, 256, 16>(st, prov, Y, ldy, n, nL, periodic);
 }
