@@ -66,6 +66,6 @@ hipError_t launch_rows_grouped(hipStream_t st, int which, int rb, const ScenDesc
                                int blocks);
 hipError_t launch_cols3_grouped(hipStream_t st, int which, const ScenDesc *D, const int *start_cols,
                                 const int *start_red, const int *start_redp, int nscen, int blocks_cols, int blocks_red,
-                                int blocks_redp);
+                                int blocks_redp, bool backsub = true);
 
 }  // namespace m70

@@ -819,8 +819,9 @@ static hipError_t batch_step(m70h_batch *b, int odd)
     const int *st_redp = b->d_start + GS_REDP * (n + 1);
     hipStream_t st = b->stream;
     hipError_t e;
+    // the Qx projector rows form the Qx predictor's solution from its first two passes
     if ((e = launch_cols3_grouped(st, 0, D, st_cols, st_red, st_redp, n, b->blocks[GS_COLS], b->blocks[GS_RED],
-                                  b->blocks[GS_REDP])) != hipSuccess)
+                                  b->blocks[GS_REDP], false)) != hipSuccess)
         return e;
     if ((e = launch_rows_grouped(st, 0, b->rows_block, D, st_rows, n, b->blocks[GS_ROWS])) != hipSuccess) return e;
     if ((e = launch_cells_grouped(st, 2, D, st_cells, n, b->blocks[GS_CELLS])) != hipSuccess) return e;

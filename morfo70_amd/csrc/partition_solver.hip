@@ -801,20 +801,6 @@ struct GProvQyx {
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qyx(d->k, d->m, d->s, d->rf, d->w, kk, line, a, b, c, f); }
 };
-struct GProvQxy {
-    const ScenDesc *d;
-    __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
-    { asm_qxy(d->k, d->s, d->rf, d->w, line, kk, a, b, c, f); }
-    __device__ __forceinline__ void get2(int line, int kk, double &a0, double &b0, double &c0, double &f0, double &a1,
-                                         double &b1, double &c1, double &f1) const
-    {
-        QxyIn q0, q1;
-        qxy_load2(d->k, d->s, d->rf, d->w, line, kk, q0, q1);
-        ld_pair(d->s.Qx, (long)(line + 1) * d->k.ny + kk, q0.qx, q1.qx);
-        asm_qxy_vals(d->k, q0, a0, b0, c0, f0);
-        asm_qxy_vals(d->k, q1, a1, b1, c1, f1);
-    }
-};
 struct GProvQyy {
     const ScenDesc *d;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
@@ -831,6 +817,27 @@ struct GProvQyy {
     }
 };
 
+// Qx projector rows of a grouped step, with Qx* formed from the scenario's column passes
+// (as ProvQxyBs).
+struct GProvQxyBs {
+    const ScenDesc *d;
+    __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
+    {
+        QxyIn q = qxy_load(d->k, d->s, d->rf, d->w, line, kk);
+        q.qx = cols_value(d->cw, d->k.nx - 1, d->k.ny, line, kk);
+        asm_qxy_vals(d->k, q, a, b, c, f);
+    }
+    __device__ __forceinline__ void get2(int line, int kk, double &a0, double &b0, double &c0, double &f0, double &a1,
+                                         double &b1, double &c1, double &f1) const
+    {
+        QxyIn q0, q1;
+        qxy_load2(d->k, d->s, d->rf, d->w, line, kk, q0, q1);
+        cols_value2(d->cw, d->k.nx - 1, d->k.ny, line, kk, q0.qx, q1.qx);
+        asm_qxy_vals(d->k, q0, a0, b0, c0, f0);
+        asm_qxy_vals(d->k, q1, a1, b1, c1, f1);
+    }
+};
+
 template <int WHICH, int RB>   // WHICH 0: Qx projector rows, 1: Qy predictor rows
 __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_rows_lds_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start, int nscen)
@@ -841,7 +848,7 @@ k_rows_lds_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start
     const ScenDesc *d = D + sid;
     const int ny = d->k.ny;
     const int line = b - start[sid];
-    if (WHICH == 0) rows_lds_body<RB>(GProvQxy{d}, d->s.Qx + ny, ny, ny, d->S_rows, 1, line, sh);
+    if (WHICH == 0) rows_lds_body<RB>(GProvQxyBs{d}, d->s.Qx + ny, ny, ny, d->S_rows, 1, line, sh);
     else rows_lds_body<RB>(GProvQyy{d}, d->s.Qy + ny, ny, ny, d->S_rows, 1, line, sh);
 }
 
@@ -929,7 +936,7 @@ hipError_t launch_rows_grouped(hipStream_t st, int which, int rb, const ScenDesc
 
 hipError_t launch_cols3_grouped(hipStream_t st, int which, const ScenDesc *D, const int *start_cols,
                                 const int *start_red, const int *start_redp, int nscen, int blocks_cols, int blocks_red,
-                                int blocks_redp)
+                                int blocks_redp, bool backsub)
 {
     if (which == 0) hipLaunchKernelGGL(k_cols_elim_grouped<0>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
     else hipLaunchKernelGGL(k_cols_elim_grouped<1>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
@@ -943,6 +950,7 @@ hipError_t launch_cols3_grouped(hipStream_t st, int which, const ScenDesc *D, co
         hipLaunchKernelGGL(k_red_pcr_grouped<1024>, dim3(blocks_redp), dim3(1024), 0, st, D, start_redp, nscen);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
+    if (!backsub) return hipSuccess;
     if (which == 0) hipLaunchKernelGGL(k_cols_backsub_grouped<0>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
     else hipLaunchKernelGGL(k_cols_backsub_grouped<1>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
     return hipGetLastError();
