@@ -105,7 +105,7 @@ __device__ void w_breaking(const m70w_params &p, double h, double sigma, double 
         const double dcc_dh = inv_D * inv_gamma_b;
         const double cc = h * dcc_dh;
         const double ff = (1 + cc * cc);
-        const double ff1 = pow(ff, -2.5);
+        const double ff1 = 1 / (ff * ff * sqrt(ff));   // ff^-2.5
         const double ff2 = (1 - ff1);
         const double ff3 = ff1 / ff;
         dw = kappa * g3 * ff2;
