@@ -105,9 +105,9 @@ __device__ void w_breaking(const m70w_params &p, double h, double sigma, double 
         const double dcc_dh = inv_D * inv_gamma_b;
         const double cc = h * dcc_dh;
         const double ff = (1 + cc * cc);
-        const double ff1 = 1 / (ff * ff * sqrt(ff));   // ff^-2.5
+        const double ff3 = 1 / (ff * ff * ff * sqrt(ff));   // ff^-3.5
+        const double ff1 = ff3 * ff;                        // ff^-2.5
         const double ff2 = (1 - ff1);
-        const double ff3 = ff1 / ff;
         dw = kappa * g3 * ff2;
         ddw_dh = kappa * g2 * (3 * ff2 * inv_D + 2.5 * gamma * ff3 * 2 * cc * dcc_dh);
     } else {
