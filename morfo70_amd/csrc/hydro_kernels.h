@@ -49,7 +49,6 @@ hipError_t launch_cols3_qyx(hipStream_t st, const Consts &k, const Metrics &m, c
                             const Work &w, const ColsWork &cw, int tile, int pass);
 bool rows_lds_supported(int n);
 int rows_block_for(int n, int rb);
-hipError_t launch_lds_qxy(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w, int rb);
 // Qx projector rows reading the Qx predictor's solution from its elimination and reduced
 // passes (E0..E2, X of cw) in place of a back-substitution pass.
 hipError_t launch_lds_qxy_bs(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w,

@@ -62,21 +62,6 @@ __device__ __forceinline__ void st_pair(double *p, long e, double a, double b)
     *reinterpret_cast<double2 *>(p + e) = make_double2(a, b);
 }
 
-struct ProvQxy {      // Qx projector, lines along y (rows): line = i, k = j
-    Consts k; State s; RunFields rf; Work w;
-    __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
-    { asm_qxy(k, s, rf, w, line, kk, a, b, c, f); }
-    __device__ __forceinline__ void get2(int line, int kk, double &a0, double &b0, double &c0, double &f0, double &a1,
-                                         double &b1, double &c1, double &f1) const
-    {
-        QxyIn q0, q1;
-        qxy_load2(k, s, rf, w, line, kk, q0, q1);
-        ld_pair(s.Qx, (long)(line + 1) * k.ny + kk, q0.qx, q1.qx);
-        asm_qxy_vals(k, q0, a0, b0, c0, f0);
-        asm_qxy_vals(k, q1, a1, b1, c1, f1);
-    }
-};
-
 // Qx projector rows with Qx* (the predictor's solution, row i+1 = unknown i of each column)
 // formed from the predictor's three-pass column solve instead of read back after a
 // back-substitution pass: the FUSED step skips that pass and never stores Qx*.
@@ -973,10 +958,6 @@ static hipError_t launch_rows_lds(hipStream_t st, const Prov &prov, double *Y, l
 }
 
 // Row sweeps with the assembly staged through LDS (one row per block).
-hipError_t launch_lds_qxy(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w, int rb)
-{
-    return launch_rows_lds(st, ProvQxy{k, s, rf, w}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true, rb);
-}
 hipError_t launch_lds_qxy_bs(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w,
                              const ColsWork &cw, int rb)
 {
