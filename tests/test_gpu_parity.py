@@ -92,14 +92,18 @@ def test_config1_1000_steps():
 
 
 @pytest.mark.parametrize("solver", [1, 2])
-@pytest.mark.parametrize("case", ["models", "shapes", "tide", "config1"])
+@pytest.mark.parametrize("case", ["models", "shapes", "odd", "tide", "config1"])
 def test_partition_solver_parity(case, solver):
-    """The PARTITION (1) and FUSED (2) line solvers against the oracle at the 1e-10 bar."""
+    """The PARTITION (1) and FUSED (2) line solvers against the oracle at the 1e-10 bar.
+    "odd": an odd alongshore count, which takes the one-element paths of the row kernels
+    and the one-column tails (even counts take the column-pair paths)."""
     if case == "models":
         ph = S.Physics(drag_model=S.DRAG["logdepth"], visc_model=S.VISC["hrms"], dt=0.25)
         sc, steps, runs = nonuniform_beach(physics=ph, seed=11), 25, 2
     elif case == "shapes":
         sc, steps, runs = nonuniform_beach(nx=67, ny=70, seed=4), 20, 1
+    elif case == "odd":
+        sc, steps, runs = nonuniform_beach(nx=31, ny=37, seed=9), 20, 1
     elif case == "tide":
         ph = S.Physics(offshore_bc=S.OFFSHORE_BC["constant_zs"], dt=0.25)
         sc, steps, runs = nonuniform_beach(physics=ph, tide=True, steps=40, seed=6), 20, 2
