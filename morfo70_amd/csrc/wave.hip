@@ -325,6 +325,43 @@ __global__ void __launch_bounds__(WB) k_wave_first(WaveArgs a, double hOff, doub
     }
 }
 
+// The previous row's values one node's step reads (up = j+1, low = j-1, periodic): read
+// at the start of the launch, so that their latency overlaps the control read and the
+// dispersion and direction instead of stalling each phase.
+struct PrevRow {
+    double tt_p, tt_m, lk_j, lk_p, lk_m;                                       // direction
+    double cg_p, cg_m, u_p, u_m, u_j, v_p, v_m, v_j, dw_p, dw_m, df_p, df_m;  // energy
+    double le_p, le_m, le_j, wc_p, wc_m, cgx_j, cgy_p, cgy_m, h_j;
+    double c_p, c_m, wcr_p, wcr_m, cx_j, cy_p, cy_m, er_p, er_m, er_j, dr_p, dr_m, Dw_p, Dw_m;   // rollers
+};
+
+__device__ __forceinline__ PrevRow load_prev(const WRow &r, int j, int jp, int jm, bool rollers)
+{
+    PrevRow P;
+    P.tt_p = r.TanTheta[jp]; P.tt_m = r.TanTheta[jm];
+    P.lk_j = r.lK[j]; P.lk_p = r.lK[jp]; P.lk_m = r.lK[jm];
+    P.cg_p = r.Cg[jp]; P.cg_m = r.Cg[jm];
+    P.u_p = r.U[jp]; P.u_m = r.U[jm]; P.u_j = r.U[j];
+    P.v_p = r.V[jp]; P.v_m = r.V[jm]; P.v_j = r.V[j];
+    P.dw_p = r.dw[jp]; P.dw_m = r.dw[jm]; P.df_p = r.df[jp]; P.df_m = r.df[jm];
+    P.le_p = r.lE[jp]; P.le_m = r.lE[jm]; P.le_j = r.lE[j];
+    P.wc_p = r.WC[jp]; P.wc_m = r.WC[jm];
+    P.cgx_j = r.Cgx[j]; P.cgy_p = r.Cgy[jp]; P.cgy_m = r.Cgy[jm];
+    P.h_j = r.H[j];
+    if (rollers) {
+        P.c_p = r.C[jp]; P.c_m = r.C[jm];
+        P.wcr_p = r.WC_r[jp]; P.wcr_m = r.WC_r[jm];
+        P.cx_j = r.Cx[j]; P.cy_p = r.Cy[jp]; P.cy_m = r.Cy[jm];
+        P.er_p = r.E_r[jp]; P.er_m = r.E_r[jm]; P.er_j = r.E_r[j];
+        P.dr_p = r.d_r[jp]; P.dr_m = r.d_r[jm];
+        P.Dw_p = r.Dw[jp]; P.Dw_m = r.Dw[jm];
+    } else {
+        P.c_p = P.c_m = P.wcr_p = P.wcr_m = P.cx_j = P.cy_p = P.cy_m = 0.0;
+        P.er_p = P.er_m = P.er_j = P.dr_p = P.dr_m = P.Dw_p = P.Dw_m = 0.0;
+    }
+    return P;
+}
+
 // One marching step s (rows s&1 -> (s&1)^1): Wave::getDx (src/Wave.cpp:267-290),
 // WaveRow::initRow (src/WaveRow.cpp:235-296), dispersion (src/WaveParametrizations.cpp:
 // 14-60), solveDir (src/WaveDir.cpp:12-126), energyPreprocessor and solveEnergy
@@ -334,15 +371,23 @@ __global__ void __launch_bounds__(WB) k_wave_first(WaveArgs a, double hOff, doub
 __global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, WaveCtl *ctl, unsigned long long *acc,
                                                   int *anyv)
 {
-    const WaveCtl c = ctl[s & 1];
     const int t = threadIdx.x, k = blockIdx.x;
+    const int ny = a.ny;
+    const int j0 = k * WNI;
+    const int nint = min(WNI, ny - j0);
+    int j = (j0 + t - 2) % ny;
+    if (j < 0) j += ny;
+    const int jp = (j == ny - 1) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;
+    const WRow &rm1 = a.rows[s & 1];
+    const PrevRow P = load_prev(rm1, j, jp, jm, a.p.rollers != 0);
+    const WaveCtl c = ctl[s & 1];
     if (c.done) {   // pass the final state on to the launches that follow
         if (k == 0 && t == 0) ctl[(s + 1) & 1] = c;
         return;
     }
     __shared__ double sl_lK[WB], sl_U[WB], sl_V[WB], sl_Cgy[WB], sl_Cy[WB];
     __shared__ double s_sigma0;
-    const int nx = a.nx, ny = a.ny;
+    const int nx = a.nx;
     const m70w_params &p = a.p;
     const double inv_2dy = a.inv_2dy;
     const double w = 2 * WPI / T;
@@ -364,15 +409,9 @@ __global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, W
             store = 1;
         }
     }
-    const WRow &rm1 = a.rows[s & 1];
     const WRow &r = a.rows[(s & 1) ^ 1];
     const long index = (long)(c.xIndex + 1) * ny, index_m1 = (long)c.xIndex * ny;
     const double dx_dx = dx_row / a.dx[c.xIndex];
-    const int j0 = k * WNI;
-    const int nint = min(WNI, ny - j0);
-    int j = (j0 + t - 2) % ny;
-    if (j < 0) j += ny;
-    const int jp = (j == ny - 1) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;
     WNode n{};
     // initRow + dispersion
     auto init_node = [&](int q, double &d, double &u, double &v, double &uw) {
@@ -419,8 +458,8 @@ __global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, W
     const double dx2 = dx * dx;
     const double inv_dx = 1 / dx;
     if (t >= 1 && t < nint + 3) {
-        double tt_p1 = rm1.TanTheta[jp];
-        double tt_m1 = rm1.TanTheta[jm];
+        double tt_p1 = P.tt_p;
+        double tt_m1 = P.tt_m;
         double dttheta_dy = (tt_p1 - tt_m1) * inv_2dy;
         double ttheta_med = (tt_p1 + tt_m1) * 0.5;
         double y_i = -ttheta_med / (dttheta_dy + inv_dx);
@@ -431,8 +470,8 @@ __global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, W
         double theta_i = asin(sinTheta_i);
         n.Y_i = y_i;
         n.inv_s_i = inv_s_i;
-        double dlk_dx = (n.lK - rm1.lK[j]) * inv_dx;
-        double dlk_dy_c = 0.5 * inv_2dy * (sl_lK[t + 1] + rm1.lK[jp] - sl_lK[t - 1] - rm1.lK[jm]);
+        double dlk_dx = (n.lK - P.lk_j) * inv_dx;
+        double dlk_dy_c = 0.5 * inv_2dy * (sl_lK[t + 1] + P.lk_p - sl_lK[t - 1] - P.lk_m);
         double dlk_ds = dlk_dx * cosTheta_i + dlk_dy_c * sinTheta_i;
         double dlk_dst = -dlk_dx * sinTheta_i + dlk_dy_c * cosTheta_i;
         double theta = theta_i + dlk_dst / (inv_s_i + dlk_ds * 0.5);
@@ -448,38 +487,37 @@ __global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, W
     double m = -999;
     int any = 0;
     if (t >= 2 && t < nint + 2) {
-        const int up = jp, low = jm;
         const double y_i_dy = n.Y_i * inv_2dy;
         const double inv_s_i = n.inv_s_i;
         const double U_up = sl_U[t + 1], U_low = sl_U[t - 1], V_up = sl_V[t + 1], V_low = sl_V[t - 1];
         double G, Gp;
         {   // energyPreprocessor (src/WaveEnergy.cpp:181-219)
-            double cg_c = 0.5 * (n.Cg + at_ray(rm1.Cg[up], rm1.Cg[low], y_i_dy));
-            double u_c = 0.5 * (n.U + at_ray(rm1.U[up], rm1.U[low], y_i_dy));
-            double v_c = 0.5 * (n.V + at_ray(rm1.V[up], rm1.V[low], y_i_dy));
-            double dw_i = at_ray(rm1.dw[up], rm1.dw[low], y_i_dy);
-            double df_i = at_ray(rm1.df[up], rm1.df[low], y_i_dy);
-            double le_i = at_ray(rm1.lE[up], rm1.lE[low], y_i_dy);
-            double du_dx = (n.U - rm1.U[j]) * inv_dx;
+            double cg_c = 0.5 * (n.Cg + at_ray(P.cg_p, P.cg_m, y_i_dy));
+            double u_c = 0.5 * (n.U + at_ray(P.u_p, P.u_m, y_i_dy));
+            double v_c = 0.5 * (n.V + at_ray(P.v_p, P.v_m, y_i_dy));
+            double dw_i = at_ray(P.dw_p, P.dw_m, y_i_dy);
+            double df_i = at_ray(P.df_p, P.df_m, y_i_dy);
+            double le_i = at_ray(P.le_p, P.le_m, y_i_dy);
+            double du_dx = (n.U - P.u_j) * inv_dx;
             double du_dy = (U_up - U_low) * inv_2dy;
-            double dv_dx = (n.V - rm1.V[j]) * inv_dx;
+            double dv_dx = (n.V - P.v_j) * inv_dx;
             double dv_dy = (V_up - V_low) * inv_2dy;
             double wc = 2 * n.sxx * du_dx + n.sxy * (du_dy + dv_dx) + 2 * n.syy * dv_dy;
-            double wc_c = 0.5 * (wc + at_ray(rm1.WC[up], rm1.WC[low], y_i_dy));
+            double wc_c = 0.5 * (wc + at_ray(P.wc_p, P.wc_m, y_i_dy));
             n.WC = wc;
             n.WC_r = n.sxx_r * du_dx + n.sxy_r * (du_dy + dv_dx) + n.syy_r * dv_dy;
-            double dcgxu_dx = (n.Cgx - rm1.Cgx[j]) * inv_dx + du_dx;
-            double dcgyv_dy_c = 0.5 * (sl_Cgy[t + 1] + rm1.Cgy[up] - sl_Cgy[t - 1] - rm1.Cgy[low]) * inv_2dy + dv_dy;
+            double dcgxu_dx = (n.Cgx - P.cgx_j) * inv_dx + du_dx;
+            double dcgyv_dy_c = 0.5 * (sl_Cgy[t + 1] + P.cgy_p - sl_Cgy[t - 1] - P.cgy_m) * inv_2dy + dv_dy;
             double div_cgv_c = dcgxu_dx + dcgyv_dy_c;
-            double dle_dy_c = (rm1.lE[up] - rm1.lE[low]) * inv_2dy;
+            double dle_dy_c = (P.le_p - P.le_m) * inv_2dy;
             G = v_c * dle_dy_c + wc_c + div_cgv_c + 0.5 * (dw_i + df_i) - cg_c * inv_s_i * le_i -
-                u_c * rm1.lE[j] * inv_dx;
+                u_c * P.le_j * inv_dx;
             Gp = cg_c * inv_s_i + u_c * inv_dx;
         }
         {   // solveEnergy: Newton-Raphson on log(E); the Jacobian's doubled Gp is the
             // reference's (src/WaveEnergy.cpp:97)
             const double logCteEnergy = log(CTE_ENERGY);
-            double le_old = rm1.lE[j], H = rm1.H[j], E = 0.0, Dw = 0.0;
+            double le_old = P.le_j, H = P.h_j, E = 0.0, Dw = 0.0;
             double dw = 0.0, ddw_dh = 0.0, df = 0.0, ddf_dh = 0.0;
             for (int it = 0; it < p.max_iterations; it++) {
                 w_breaking(p, H, n.Sigma, n.D, dw, ddw_dh);
@@ -507,22 +545,22 @@ __global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, W
         }
         if (p.rollers) {   // solveRollers (src/WaveRollers.cpp:57-100)
             const double d_r_div_c = 2 * WG * p.sinBeta_r;
-            double c_c = 0.5 * (n.C + at_ray(rm1.C[up], rm1.C[low], y_i_dy));
-            double u_c = 0.5 * (n.U + at_ray(rm1.U[up], rm1.U[low], y_i_dy));
-            double v_c = 0.5 * (n.V + at_ray(rm1.V[up], rm1.V[low], y_i_dy));
-            double du_dx = (n.U - rm1.U[j]) * inv_dx;
+            double c_c = 0.5 * (n.C + at_ray(P.c_p, P.c_m, y_i_dy));
+            double u_c = 0.5 * (n.U + at_ray(P.u_p, P.u_m, y_i_dy));
+            double v_c = 0.5 * (n.V + at_ray(P.v_p, P.v_m, y_i_dy));
+            double du_dx = (n.U - P.u_j) * inv_dx;
             double dv_dy = (V_up - V_low) * inv_2dy;
-            double wc_r_c = 0.5 * (n.WC_r + at_ray(rm1.WC_r[up], rm1.WC_r[low], y_i_dy));
-            double dcxu_dx = (n.Cx - rm1.Cx[j]) * inv_dx + du_dx;
-            double dcyv_dy_c = 0.5 * (sl_Cy[t + 1] + rm1.Cy[up] - sl_Cy[t - 1] - rm1.Cy[low]) * inv_2dy + dv_dy;
+            double wc_r_c = 0.5 * (n.WC_r + at_ray(P.wcr_p, P.wcr_m, y_i_dy));
+            double dcxu_dx = (n.Cx - P.cx_j) * inv_dx + du_dx;
+            double dcyv_dy_c = 0.5 * (sl_Cy[t + 1] + P.cy_p - sl_Cy[t - 1] - P.cy_m) * inv_2dy + dv_dy;
             double div_cv_c = dcxu_dx + dcyv_dy_c;
-            double der_dy_c = (rm1.E_r[up] - rm1.E_r[low]) * inv_2dy;
+            double der_dy_c = (P.er_p - P.er_m) * inv_2dy;
             const double d_r = d_r_div_c / n.C;
             n.d_r = d_r;
-            double d_r_c = 0.5 * (d_r + at_ray(rm1.d_r[up], rm1.d_r[low], y_i_dy));
-            double Dw_c = 0.5 * (n.Dw + at_ray(rm1.Dw[up], rm1.Dw[low], y_i_dy));
-            double er_i = at_ray(rm1.E_r[up], rm1.E_r[low], y_i_dy);
-            double upv = c_c * inv_s_i * er_i + u_c * rm1.E_r[j] * inv_dx - v_c * der_dy_c + 0.5 * p.alpha_r * Dw_c;
+            double d_r_c = 0.5 * (d_r + at_ray(P.dr_p, P.dr_m, y_i_dy));
+            double Dw_c = 0.5 * (n.Dw + at_ray(P.Dw_p, P.Dw_m, y_i_dy));
+            double er_i = at_ray(P.er_p, P.er_m, y_i_dy);
+            double upv = c_c * inv_s_i * er_i + u_c * P.er_j * inv_dx - v_c * der_dy_c + 0.5 * p.alpha_r * Dw_c;
             double lowv = c_c * inv_s_i + u_c * inv_dx + div_cv_c + 0.5 * wc_r_c + 0.5 * d_r_c;
             n.E_r = upv / lowv;
         }
