@@ -2,8 +2,8 @@
 
 Knobs (all results-neutral except the reduced-system tile, which changes rounding by a
 few ulps, well inside the 1e-10 bar):
-  * rows_block: threads per block of the y-sweep solver, 256 (ny <= 2048, two blocks per
-    CU) or 512 (one block per CU);
+  * rows_block: threads per block of the y-sweep solver, 256 (ny <= 2048, 36 KB of LDS,
+    four blocks per CU) or 512 (72 KB, two blocks per CU);
   * tile_cols: unknowns per thread of the reduced column systems, 8 or 16;
   * graph (batches only): replay the step from a hipGraph or launch eagerly.
 
