@@ -345,12 +345,11 @@ __device__ __forceinline__ void st2(double *p, long e, double a, double b)
     *reinterpret_cast<double2 *>(p + e) = make_double2(a, b);
 }
 
-__global__ void __launch_bounds__(256) k_qx_tail2(Consts k, Metrics m, State s, Work w, double f0)
+// Columns j0, j0 + 1 of row i (j0 even, ny even).
+__device__ __forceinline__ void qx_tail2_body(const Consts &k, const Metrics &m, const State &s, const Work &w,
+                                              double f0, int i, int j0)
 {
     const int ny = k.ny, nx = k.nx;
-    const int j0 = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int i = blockIdx.y;
-    if (j0 >= ny || i >= nx) return;
     const long r = (long)i * ny;
     const int c2 = (j0 + 2 == ny) ? 0 : j0 + 2;
     const double dt = k.dt;
@@ -418,12 +417,10 @@ __global__ void __launch_bounds__(256) k_qx_tail2(Consts k, Metrics m, State s, 
     if (i == 0) st2(s.Qx, j0, q[1].x, q[1].y);
 }
 
-__global__ void __launch_bounds__(256) k_qy_tail2(Consts k, State s)
+// Columns j0, j0 + 1 of row i (j0 even, ny even).
+__device__ __forceinline__ void qy_tail2_body(const Consts &k, const State &s, int i, int j0)
 {
     const int ny = k.ny, nx = k.nx;
-    const int j0 = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int i = blockIdx.y;
-    if (j0 >= ny || i >= nx) return;
     const long r = (long)i * ny;
     const int cm = (j0 == 0) ? ny - 2 : j0 - 2, cp = (j0 + 2 == ny) ? 0 : j0 + 2;
     // qy_row(i+1, .) at columns cm, cm+1 | j0, j0+1 | cp (row nx is Qy[nx-1]*inv_3)
@@ -465,6 +462,22 @@ __global__ void __launch_bounds__(256) k_qy_tail2(Consts k, State s)
     }
 }
 
+__global__ void __launch_bounds__(256) k_qx_tail2(Consts k, Metrics m, State s, Work w, double f0)
+{
+    const int j0 = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = blockIdx.y;
+    if (j0 >= k.ny || i >= k.nx) return;
+    qx_tail2_body(k, m, s, w, f0, i, j0);
+}
+
+__global__ void __launch_bounds__(256) k_qy_tail2(Consts k, State s)
+{
+    const int j0 = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = blockIdx.y;
+    if (j0 >= k.ny || i >= k.nx) return;
+    qy_tail2_body(k, s, i, j0);
+}
+
 // ------------------------------------------------------------------------------------
 // Non-finite scan of a field; sets *flag to 1 if any element is NaN or Inf.
 // ------------------------------------------------------------------------------------
@@ -490,7 +503,9 @@ __global__ void __launch_bounds__(256) k_init_zs(double *Zs, const double *D, co
 
 // ------------------------------------------------------------------------------------
 // Grouped cell kernels: one launch covers every scenario of a batch. Block b belongs to
-// scenario scen_of(start, nscen, b); within it blocks tile (ceil(ny/256) x nx).
+// scenario scen_of(start, nscen, b); within it blocks tile (ceil(ny/256) x nx). The tails
+// of a scenario with an even ny take two columns per thread (ceil(ny/512) x nx blocks;
+// the scenario's other blocks return at once).
 // ------------------------------------------------------------------------------------
 template <int OP>
 __global__ void __launch_bounds__(256) k_cells_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
@@ -501,6 +516,15 @@ __global__ void __launch_bounds__(256) k_cells_grouped(const ScenDesc *__restric
     const ScenDesc &d = D[sid];
     const int ny = d.k.ny;
     const int lb = b - start[sid];
+    if ((OP == 2 || OP == 3) && ny % 2 == 0 && ny >= 4) {
+        const int gx2 = (ny + 511) / 512;
+        const int i = lb / gx2;
+        const int j0 = 2 * ((lb - i * gx2) * 256 + (int)threadIdx.x);
+        if (j0 >= ny || i >= d.k.nx) return;
+        if (OP == 2) qx_tail2_body(d.k, d.m, d.s, d.w, d.f0, i, j0);
+        else qy_tail2_body(d.k, d.s, i, j0);
+        return;
+    }
     const int gx = (ny + 255) / 256;
     const int i = lb / gx;
     const int j = (lb - i * gx) * 256 + threadIdx.x;
