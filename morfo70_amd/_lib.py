@@ -46,6 +46,7 @@ SIGNATURES = {
                            [ctypes.c_long, _vp, ctypes.c_long, ctypes.c_int]),
     "m70h_check_finite": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "m70h_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "m70h_set_profiling_only": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "m70h_profile": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), _dp,
                                     ctypes.POINTER(ctypes.c_long)]),
     "m70h_state_ptr": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),

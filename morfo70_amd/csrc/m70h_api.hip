@@ -71,6 +71,7 @@ struct m70h {
     long run_gen = 0;        // bumped by every Run and forcing change
     // profiling
     bool prof = false;
+    int prof_only = -1;   // kernel id to time alone, -1 for every launch
     struct Rec { int kid; hipEvent_t a, b; };
     std::vector<Rec> recs;
     std::vector<hipEvent_t> ev_pool;
@@ -124,14 +125,14 @@ template <typename Fn>
 int timed(m70h *h, int kid, Fn &&fn)
 {
     hipEvent_t a = nullptr, b = nullptr;
-    if (h->prof) {
+    if (h->prof && (h->prof_only < 0 || h->prof_only == kid)) {
         a = next_event(h);
         b = next_event(h);
         if (a && b) (void)hipEventRecord(a, h->stream);
     }
     hipError_t e = fn();
     if (e != hipSuccess) return hip_fail(h, e, kKernelNames[kid]);
-    if (h->prof && a && b) {
+    if (a && b) {
         (void)hipEventRecord(b, h->stream);
         h->recs.push_back({kid, a, b});
     }
@@ -729,6 +730,23 @@ int m70h_set_profiling(m70h *h, int enable)
     if (!h) return M70H_EINVAL;
     int rc = collect_profile(h);
     h->prof = enable != 0;
+    h->prof_only = -1;
+    for (int q = 0; q < K_COUNT; ++q) { h->prof_ms[q] = 0; h->prof_n[q] = 0; }
+    return rc;
+}
+
+int m70h_set_profiling_only(m70h *h, const char *kernel)
+{
+    if (!h) return M70H_EINVAL;
+    int kid = -1;
+    if (kernel) {
+        for (int q = 0; q < K_COUNT; ++q)
+            if (std::strcmp(kKernelNames[q], kernel) == 0) kid = q;
+        if (kid < 0) return fail(h, M70H_EINVAL, std::string("unknown kernel ") + kernel);
+    }
+    int rc = collect_profile(h);
+    h->prof = true;
+    h->prof_only = kid;
     for (int q = 0; q < K_COUNT; ++q) { h->prof_ms[q] = 0; h->prof_n[q] = 0; }
     return rc;
 }

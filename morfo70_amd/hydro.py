@@ -179,8 +179,12 @@ class Hydro:
     def _stream_ptr(self) -> int:
         return getattr(self, "_stream", 0)
 
-    def set_profiling(self, on: bool):
-        self._chk(self._L.m70h_set_profiling(self._h, 1 if on else 0))
+    def set_profiling(self, on: bool, only: str | None = None):
+        """Per-kernel HIP-event timing; `only` restricts it to the launches of one kernel."""
+        if on and only is not None:
+            self._chk(self._L.m70h_set_profiling_only(self._h, only.encode()))
+        else:
+            self._chk(self._L.m70h_set_profiling(self._h, 1 if on else 0))
 
     def profile(self) -> dict:
         out = {}

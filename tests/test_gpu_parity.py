@@ -142,3 +142,30 @@ def test_state_roundtrip_and_finite_check():
     st2 = g.state()
     for k in st:
         assert np.array_equal(st[k], st2[k]), k
+
+
+def test_profiling_one_kernel_leaves_results_unchanged():
+    """m70h_set_profiling_only times the launches of one kernel and none of the others;
+    the events do not change the step's results."""
+    sc = nonuniform_beach(nx=40, ny=36, seed=5)
+    from morfo70_amd import _lib
+    from morfo70_amd.hydro import from_scenario
+    print("[profiling] one kernel, FUSED 40x36", flush=True)
+    a = from_scenario(sc, solver=_lib.SOLVER_FUSED)
+    b = from_scenario(sc, solver=_lib.SOLVER_FUSED)
+    a.set_profiling(True, only="qx_tail")
+    a.Run(4)
+    b.Run(4)
+    prof = a.profile()
+    assert prof["qx_tail"][1] == 4
+    assert all(n == 0 for k, (ms, n) in prof.items() if k != "qx_tail")
+    a.set_profiling(True)
+    a.Run(1)
+    assert sum(n for ms, n in a.profile().values()) > 1
+    a.set_profiling(False)
+    b.Run(1)
+    with pytest.raises(RuntimeError):
+        a.set_profiling(True, only="no_such_kernel")
+    sa, sb = a.state(), b.state()
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
