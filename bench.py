@@ -44,10 +44,13 @@ KERNEL_BYTES_PER_CELL = {
     # FUSED family: y-sweeps in one launch (assembly inputs in, solution out)
     # the Qx projector forms Qx* from the predictor's column passes: E0..E2 and X (2 B) in
     "fused_qxy": 8 * 10 + 2 + 1 + 8,        # D(2),Vc(2),E0..E2,nuD_x,dnuDx_dy,drag in, X, mask in; Qx out
-    "fused_qyy": 8 * 14 + 1 + 8 * 2,        # the qyy_assemble inputs; Qy*, drag out
+    # the qyy_assemble inputs with nuD in place of nuD_y and dnuD_dy, and no cD_y (the
+    # constant drag model of this workload makes it cD0); Qy*, drag out
+    "fused_qyy": 8 * 12 + 1 + 8 * 2,
     # FUSED x-sweeps, pass 1: assembly inputs in; drag, mask, E0..E2 out, plus the two
-    # reduced rows (4 values each) per 8-row segment = 8 B per cell
-    "cols_qx_elim": 8 * 14 + 8 + 1 + 8 * 3 + 8,
+    # reduced rows (4 values each) per 8-row segment = 8 B per cell. The Qx predictor reads
+    # nuD in place of nuD_x, dnuD_dx and dnuDx_dy, and no cD_x (constant drag): 11 fields
+    "cols_qx_elim": 8 * 11 + 8 + 1 + 8 * 3 + 8,
     "cols_qy_elim": 8 * 11 + 1 + 8 * 3 + 8,
     # pass 3: E0..E2 and the two interface values per segment in, the solution out
     "cols_qx_backsub": 8 * 3 + 2 + 8,

@@ -90,15 +90,22 @@ constexpr int CM = 8;   // rows per column segment in the three-pass column solv
 // window: each element loads only the rows no earlier element of the segment loaded
 // (15 values instead of about 35). Rows of the nx-row fields past nx-1 are read only
 // where asm_qxx_vals does not use them, so they are clamped to row nx-1.
+// nuD_x, dnuD_dx and dnuDx_dy are formed from nuD rows i, i+1 at columns j, j+-1 by
+// run_fields_body's expressions (one array in the window instead of three), and with the
+// constant drag model (CDC) cD_x is cD0 itself (0.5*(cD0+cD0) is exact), not read.
+template <bool CDC>
 __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, const State &s, const RunFields &rf,
                                             const Work &w, double f0, int j, int k0, int mm, double (&lo)[CM],
                                             double (&di)[CM], double (&up)[CM], double (&rh)[CM])
 {
     const int ny = k.ny, nx = k.nx;
-    const int jm = wrap_dec(j, ny);
+    const int jm = wrap_dec(j, ny), jp = wrap_inc(j, ny);
     auto at = [&](int i) { return (long)i * ny + j; };
     auto atm = [&](int i) { return (long)i * ny + jm; };
+    auto atp = [&](int i) { return (long)i * ny + jp; };
     auto cl = [&](int i) { return i < nx ? i : nx - 1; };
+    const double *nuD = rf.nuD;
+    double n0 = nuD[at(k0)], n0p = nuD[atp(k0)], n0m = nuD[atm(k0)];   // nuD row i
     QxxIn q;
     {
         const int i = k0, im = i > 0 ? i - 1 : 0, i2 = cl(i + 2);
@@ -125,8 +132,14 @@ __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, 
                 q.v1j = q.v2j; q.v1m = q.v2m; q.v2j = s.V[at(i + 2)]; q.v2m = s.V[atm(i + 2)];
             }
             q.Qx1 = s.Qx[at(i + 1)];
-            q.cD = rf.cD_x[at(i)]; q.Urms = rf.Urms_x[at(i)]; q.Fx = rf.Fx[at(i)];
-            q.dnuDx_dy = rf.dnuDx_dy[at(i)]; q.nuD_x = rf.nuD_x[at(i)]; q.dnuD_dx = rf.dnuD_dx[at(i)];
+            q.cD = CDC ? k.cD0 : rf.cD_x[at(i)];
+            q.Urms = rf.Urms_x[at(i)]; q.Fx = rf.Fx[at(i)];
+            const double n1 = nuD[at(i + 1)], n1p = nuD[atp(i + 1)], n1m = nuD[atm(i + 1)];
+            q.nuD_x = 0.5 * (n0 + n1);
+            q.dnuD_dx = mt.inv_dx[i] * (n1 - n0);
+            const double nux_p = 0.5 * (n0p + n1p), nux_m = 0.5 * (n0m + n1m);
+            q.dnuDx_dy = k.inv_2dy_c2c * (nux_p - nux_m);
+            n0 = n1; n0p = n1p; n0m = n1m;
             int wi;
             double drag;
             asm_qxx_vals(k, mt, q, f0, i, di[t], lo[t], up[t], rh[t], wi, drag);
@@ -136,19 +149,38 @@ __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, 
     }
 }
 
+template <bool CDC>
 struct ProvQxx {      // Qx predictor, lines along x (columns): line = j, k = i
     Consts k; Metrics m; State s; RunFields rf; Work w; double f0;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qxx(k, m, s, rf, w, f0, kk, line, a, b, c, f); }
     __device__ __forceinline__ void segment(int line, int k0, int mm, double (&lo)[CM], double (&di)[CM],
                                             double (&up)[CM], double (&rh)[CM]) const
-    { qxx_segment(k, m, s, rf, w, f0, line, k0, mm, lo, di, up, rh); }
+    { qxx_segment<CDC>(k, m, s, rf, w, f0, line, k0, mm, lo, di, up, rh); }
 };
 struct ProvQyx {      // Qy projector, columns
     Consts k; Metrics m; State s; RunFields rf; Work w;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qyx(k, m, s, rf, w, kk, line, a, b, c, f); }
 };
+// nuD_y and dnuD_dy of elements e, e+1 of row i formed from nuD row i by run_fields_body's
+// expressions, in place of qyy_load2's loads of the two arrays (which become dead).
+__device__ __forceinline__ void qyy_nud_pair(const Consts &k, const RunFields &rf, int i, int e, QyyIn &a, QyyIn &b)
+{
+    const int ny = k.ny;
+    const long r = (long)i * ny;
+    const int ep = (e + 2 == ny) ? 0 : e + 2;
+    double n0, n1, n2, x;
+    ld_pair(rf.nuD, r + e, n0, n1);
+    ld_pair(rf.nuD, r + ep, n2, x);
+    a.nuD_y = 0.5 * (n1 + n0);
+    a.dnuD_dy = (n1 - n0) * k.inv_dy;
+    b.nuD_y = 0.5 * (n2 + n1);
+    b.dnuD_dy = (n2 - n1) * k.inv_dy;
+}
+
+// CDC: constant drag model, cD_y is cD0 itself (0.5*(cD0+cD0) is exact) and is not read.
+template <bool CDC>
 struct ProvQyy {      // Qy predictor, rows
     Consts k; Metrics m; State s; RunFields rf; Work w;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
@@ -158,6 +190,8 @@ struct ProvQyy {      // Qy predictor, rows
     {
         QyyIn q0, q1;
         qyy_load2(k, s, rf, w, line, kk, q0, q1);
+        qyy_nud_pair(k, rf, line, kk, q0, q1);
+        if (CDC) q0.cD = q1.cD = k.cD0;
         double d0, d1;
         asm_qyy_vals(k, m, q0, line, a0, b0, c0, f0, d0);
         asm_qyy_vals(k, m, q1, line, a1, b1, c1, f1, d1);
@@ -750,7 +784,9 @@ static hipError_t launch_cols3(hipStream_t st, const Prov &prov, const ColsWork 
 hipError_t launch_cols3_qxx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
                             const Work &w, double f0, const ColsWork &cw, int tile, int pass)
 {
-    return launch_cols3(st, ProvQxx{k, m, s, rf, w, f0}, cw, s.Qx + k.ny, k.nx - 1, k.ny, tile, pass);
+    if (k.drag_model == 0)
+        return launch_cols3(st, ProvQxx<true>{k, m, s, rf, w, f0}, cw, s.Qx + k.ny, k.nx - 1, k.ny, tile, pass);
+    return launch_cols3(st, ProvQxx<false>{k, m, s, rf, w, f0}, cw, s.Qx + k.ny, k.nx - 1, k.ny, tile, pass);
 }
 hipError_t launch_cols3_qyx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
                             const Work &w, const ColsWork &cw, int tile, int pass)
@@ -779,7 +815,7 @@ struct GProvQxx {
     { asm_qxx(d->k, d->m, d->s, d->rf, d->w, d->f0, kk, line, a, b, c, f); }
     __device__ __forceinline__ void segment(int line, int k0, int mm, double (&lo)[CM], double (&di)[CM],
                                             double (&up)[CM], double (&rh)[CM]) const
-    { qxx_segment(d->k, d->m, d->s, d->rf, d->w, d->f0, line, k0, mm, lo, di, up, rh); }
+    { qxx_segment<false>(d->k, d->m, d->s, d->rf, d->w, d->f0, line, k0, mm, lo, di, up, rh); }
 };
 struct GProvQyx {
     const ScenDesc *d;
@@ -795,6 +831,7 @@ struct GProvQyy {
     {
         QyyIn q0, q1;
         qyy_load2(d->k, d->s, d->rf, d->w, line, kk, q0, q1);
+        qyy_nud_pair(d->k, d->rf, line, kk, q0, q1);
         double d0, d1;
         asm_qyy_vals(d->k, d->m, q0, line, a0, b0, c0, f0, d0);
         asm_qyy_vals(d->k, d->m, q1, line, a1, b1, c1, f1, d1);
@@ -966,7 +1003,9 @@ hipError_t launch_lds_qxy_bs(hipStream_t st, const Consts &k, const State &s, co
 hipError_t launch_lds_qyy(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
                           const Work &w, int rb)
 {
-    return launch_rows_lds(st, ProvQyy{k, m, s, rf, w}, s.Qy + k.ny, k.ny, k.ny, k.nx - 1, true, rb);
+    if (k.drag_model == 0)
+        return launch_rows_lds(st, ProvQyy<true>{k, m, s, rf, w}, s.Qy + k.ny, k.ny, k.ny, k.nx - 1, true, rb);
+    return launch_rows_lds(st, ProvQyy<false>{k, m, s, rf, w}, s.Qy + k.ny, k.ny, k.ny, k.nx - 1, true, rb);
 }
 
 }  // namespace m70
