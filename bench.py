@@ -35,10 +35,11 @@ KERNEL_BYTES_PER_CELL = {
     "thomas_qx": 8 * 5,                     # a,b,c,f in; Qx* out
     "qxy_assemble": 8 * 8 + 1 + 8 * 4,      # D,Vc,Qx*,nuD_x,dnuDx_dy,drag (+2 rows) in, mask in; a,b,c,f out
     "periodic_qx": 8 * 5,                   # a,b,c,f in; Qx out
-    "qx_tail": 8 * 4 + 8 * 5 + 1,           # Qx,D,dQy,Zs in; Qx0,dQx,gD,U,Uc out, Qy mask out (nx rows)
-    "qyy_assemble": 8 * 14 + 1 + 8 * 5,     # gD,Zs,dQx,U(2),Qy,Uc,V,Fy,nuD_y,dnuDy_dx,dnuD_dy,cD_y,Urms_y, mask in; a,b,c,f,drag out
+    # Qx,D,dQy,Zs in; Qx0,dQx,gD,U out, Qy mask out (nx rows); U_c only on a Run's last step
+    "qx_tail": 8 * 4 + 8 * 4 + 1,
+    "qyy_assemble": 8 * 13 + 1 + 8 * 5,     # gD,Zs,dQx,U(2),Qy,V,Fy,nuD_y,dnuDy_dx,dnuD_dy,cD_y,Urms_y, mask in; a,b,c,f,drag out
     "periodic_qy": 8 * 5,
-    "qyx_assemble": 8 * 11 + 1 + 8 * 4,
+    "qyx_assemble": 8 * 10 + 1 + 8 * 4,     # U_c formed from U
     "thomas_qy": 8 * 5,
     "qy_tail": 8 * 5 + 8 * 5,               # Qy,dQx,D,Zs in; dQy,Zs,D,V,Vc out (nx rows)
     # FUSED family: y-sweeps in one launch (assembly inputs in, solution out)
@@ -46,12 +47,12 @@ KERNEL_BYTES_PER_CELL = {
     "fused_qxy": 8 * 10 + 2 + 1 + 8,        # D(2),Vc(2),E0..E2,nuD_x,dnuDx_dy,drag in, X, mask in; Qx out
     # the qyy_assemble inputs with nuD in place of nuD_y and dnuD_dy, and no cD_y (the
     # constant drag model of this workload makes it cD0); Qy*, drag out
-    "fused_qyy": 8 * 12 + 1 + 8 * 2,
+    "fused_qyy": 8 * 11 + 1 + 8 * 2,         # ... and U_c formed from U
     # FUSED x-sweeps, pass 1: assembly inputs in; drag, mask, E0..E2 out, plus the two
     # reduced rows (4 values each) per 8-row segment = 8 B per cell. The Qx predictor reads
     # nuD in place of nuD_x, dnuD_dx and dnuDx_dy, and no cD_x (constant drag): 11 fields
     "cols_qx_elim": 8 * 11 + 8 + 1 + 8 * 3 + 8,
-    "cols_qy_elim": 8 * 11 + 1 + 8 * 3 + 8,
+    "cols_qy_elim": 8 * 10 + 1 + 8 * 3 + 8,  # U_c formed from U
     # pass 3: E0..E2 and the two interface values per segment in, the solution out
     "cols_qx_backsub": 8 * 3 + 2 + 8,
     "cols_qy_backsub": 8 * 3 + 2 + 8,

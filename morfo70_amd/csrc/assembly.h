@@ -308,8 +308,7 @@ struct QyyIn {
     double g_m, g_j, g_p, g_pp;      // guess_D row i at columns j-1, j, j+1, j+2
     int wy_m, wy_j, wy_p, w0;        // IsWater row i at faces j-1, j, j+1; row 0 at face j (Q1)
     double zs0, zs1, dqx0, dqx1;     // Zs, dQx_dx row i at columns j, j+1
-    double u0j, u0p, u1j, u1p;       // U rows i, i+1 at columns j, j+1
-    double uc0, uc1;                 // U_c row i at columns j, j+1
+    double u0j, u0p, u1j, u1p;       // U rows i, i+1 at columns j, j+1 (U_c row i is formed from them)
     double vm, vj, vp;               // V row i+1 at columns j-1, j, j+1
     double Fy, Qy1, dnuDy_dx, nuD_y, cD, Urms, dnuD_dy;
 };
@@ -329,7 +328,6 @@ __host__ __device__ __forceinline__ QyyIn qyy_load(const Consts &k, const State 
     q.zs0 = s.Zs[r + j]; q.zs1 = s.Zs[r + jp];
     q.dqx0 = s.dQx[r + j]; q.dqx1 = s.dQx[r + jp];
     q.u0j = s.U[r + j]; q.u0p = s.U[r + jp]; q.u1j = s.U[r + ny + j]; q.u1p = s.U[r + ny + jp];
-    q.uc0 = s.Uc[r + j]; q.uc1 = s.Uc[r + jp];
     q.vm = s.V[r + ny + jm]; q.vj = s.V[r + ny + j]; q.vp = s.V[r + ny + jp];
     q.Fy = rf.Fy[r + j]; q.Qy1 = s.Qy[r + ny + j]; q.dnuDy_dx = rf.dnuDy_dx[r + j]; q.nuD_y = rf.nuD_y[r + j];
     q.cD = rf.cD_y[r + j]; q.Urms = rf.Urms_y[r + j]; q.dnuD_dy = rf.dnuD_dy[r + j];
@@ -370,10 +368,6 @@ __host__ __device__ __forceinline__ void qyy_load2(const Consts &k, const State 
     ld_pair(s.U, r + ny + ep, u12, x);
     a.u0j = u00; a.u0p = u01; a.u1j = u10; a.u1p = u11;
     b.u0j = u01; b.u0p = u02; b.u1j = u11; b.u1p = u12;
-    double c0, c1, c2;
-    ld_pair(s.Uc, r + e, c0, c1);
-    ld_pair(s.Uc, r + ep, c2, x);
-    a.uc0 = c0; a.uc1 = c1; b.uc0 = c1; b.uc1 = c2;
     double vm, v0, v1, vp;
     ld_pair(s.V, r + ny + em, x, vm);
     ld_pair(s.V, r + ny + e, v0, v1);
@@ -415,7 +409,11 @@ __host__ __device__ __forceinline__ void asm_qyy_vals(const Consts &k, const Met
                    nud_y * d2u_dxy);
     }
     // Ui_j = C2Y(U_c row i) and the drag with V row i+1 (old)
-    const double uy = 0.5 * (q.uc1 + q.uc0);
+    // U_c row i at columns j, j+1: X2C of U, the Qx tail's own expression for U_c, from the
+    // U rows loaded here; the same bits as the stored U_c, which a step need not write.
+    const double uc0 = m.x2c_d[i] * q.u0j + m.x2c_up[i] * q.u1j;
+    const double uc1 = m.x2c_d[i] * q.u0p + m.x2c_up[i] * q.u1p;
+    const double uy = 0.5 * (uc1 + uc0);
     const double drag = drag_term(q.cD, q.Urms, uy, q.vj, invDy, k);
     // setSolverQyy_d (:160-181)
     const int isl = !w0;
@@ -475,7 +473,11 @@ __host__ __device__ __forceinline__ void asm_qyx(const Consts &k, const Metrics 
         const double u_y = 0.5 * (x2c_up * (u[ny + jp] + u[ny + j]) + x2c_d * (u[jp] + u[j]));
         dv = wi * ((1 + dt * drag) + dt * (u_y * dx_d + du_dx) - dt * (inv_d * nud * d2x_d + inv_d * dnud_dx * dx_d)) + isl;
     }
-    const double uy = 0.5 * (s.Uc[r + jp] + s.Uc[r + j]);   // Ui_j row i (C2Y of U_c)
+    // Ui_j row i (C2Y of U_c), U_c formed from U rows i, i+1 as the Qx tail forms it
+    const double *uu = s.U + r;
+    const double uc_j = m.x2c_d[i] * uu[j] + m.x2c_up[i] * uu[ny + j];
+    const double uc_p = m.x2c_d[i] * uu[jp] + m.x2c_up[i] * uu[ny + jp];
+    const double uy = 0.5 * (uc_p + uc_j);
     // setSolverQyx_up (:287-311); inv_Di row i+2 = getD_y(i+1); IsWater row nx-1 is false
     double uv;
     {

@@ -13,8 +13,10 @@ hipError_t launch_qxx_assemble(hipStream_t st, const Consts &k, const Metrics &m
                                const RunFields &rf, const Work &w, double f0);
 hipError_t launch_qxy_assemble(hipStream_t st, const Consts &k, const State &s, const RunFields &rf,
                                const Work &w);
+// write_uc = false skips the U_c store: the step's own readers form U_c from U, so only
+// the last step of a Run needs it in memory.
 hipError_t launch_qx_tail(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const Work &w,
-                          double f0);
+                          double f0, bool write_uc = true);
 hipError_t launch_qyy_assemble(hipStream_t st, const Consts &k, const Metrics &m, const State &s,
                                const RunFields &rf, const Work &w);
 hipError_t launch_qyx_assemble(hipStream_t st, const Consts &k, const Metrics &m, const State &s,

@@ -186,7 +186,7 @@ __host__ __device__ __forceinline__ double qx_at(const State &s, const Consts &k
 }
 
 __host__ __device__ __forceinline__ void qx_tail_body(const Consts &k, const Metrics &m, const State &s, const Work &w, double f0,
-                                             int i, int j)
+                                             int i, int j, bool write_uc = true)
 {
     const int ny = k.ny;
     const long r = (long)i * ny;
@@ -239,14 +239,14 @@ __host__ __device__ __forceinline__ void qx_tail_body(const Consts &k, const Met
     s.dQx[r + j] = dq_i;
     w.gD[r + j] = gd[1];
     s.U[r + j] = ui;
-    s.Uc[r + j] = m.x2c_d[i] * ui + m.x2c_up[i] * ui1;
+    if (write_uc) s.Uc[r + j] = m.x2c_d[i] * ui + m.x2c_up[i] * ui1;
     if (i == 0) s.Qx[j] = q[1];
 }
 
-__global__ void __launch_bounds__(256) k_qx_tail(Consts k, Metrics m, State s, Work w, double f0)
+__global__ void __launch_bounds__(256) k_qx_tail(Consts k, Metrics m, State s, Work w, double f0, int write_uc)
 {
     CELL_IJ_ONLY(k.nx)
-    qx_tail_body(k, m, s, w, f0, i, j);
+    qx_tail_body(k, m, s, w, f0, i, j, write_uc != 0);
 }
 
 // ------------------------------------------------------------------------------------
@@ -347,7 +347,7 @@ __device__ __forceinline__ void st2(double *p, long e, double a, double b)
 
 // Columns j0, j0 + 1 of row i (j0 even, ny even).
 __device__ __forceinline__ void qx_tail2_body(const Consts &k, const Metrics &m, const State &s, const Work &w,
-                                              double f0, int i, int j0)
+                                              double f0, int i, int j0, bool write_uc = true)
 {
     const int ny = k.ny, nx = k.nx;
     const long r = (long)i * ny;
@@ -413,7 +413,7 @@ __device__ __forceinline__ void qx_tail2_body(const Consts &k, const Metrics &m,
     st2(s.dQx, r + j0, dq_i.x, dq_i.y);
     st2(w.gD, r + j0, gd[1].x, gd[1].y);
     st2(s.U, r + j0, ua, ub);
-    st2(s.Uc, r + j0, m.x2c_d[i] * ua + m.x2c_up[i] * u1a, m.x2c_d[i] * ub + m.x2c_up[i] * u1b);
+    if (write_uc) st2(s.Uc, r + j0, m.x2c_d[i] * ua + m.x2c_up[i] * u1a, m.x2c_d[i] * ub + m.x2c_up[i] * u1b);
     if (i == 0) st2(s.Qx, j0, q[1].x, q[1].y);
 }
 
@@ -462,12 +462,12 @@ __device__ __forceinline__ void qy_tail2_body(const Consts &k, const State &s, i
     }
 }
 
-__global__ void __launch_bounds__(256) k_qx_tail2(Consts k, Metrics m, State s, Work w, double f0)
+__global__ void __launch_bounds__(256) k_qx_tail2(Consts k, Metrics m, State s, Work w, double f0, int write_uc)
 {
     const int j0 = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int i = blockIdx.y;
     if (j0 >= k.ny || i >= k.nx) return;
-    qx_tail2_body(k, m, s, w, f0, i, j0);
+    qx_tail2_body(k, m, s, w, f0, i, j0, write_uc != 0);
 }
 
 __global__ void __launch_bounds__(256) k_qy_tail2(Consts k, State s)
@@ -578,12 +578,12 @@ hipError_t launch_qxy_assemble(hipStream_t st, const Consts &k, const State &s, 
     return hipGetLastError();
 }
 hipError_t launch_qx_tail(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const Work &w,
-                          double f0)
+                          double f0, bool write_uc)
 {
     if (k.ny % 2 == 0 && k.ny >= 4)
-        hipLaunchKernelGGL(k_qx_tail2, cell_grid(k.ny / 2, k.nx), dim3(256), 0, st, k, m, s, w, f0);
+        hipLaunchKernelGGL(k_qx_tail2, cell_grid(k.ny / 2, k.nx), dim3(256), 0, st, k, m, s, w, f0, (int)write_uc);
     else
-        hipLaunchKernelGGL(k_qx_tail, cell_grid(k.ny, k.nx), dim3(256), 0, st, k, m, s, w, f0);
+        hipLaunchKernelGGL(k_qx_tail, cell_grid(k.ny, k.nx), dim3(256), 0, st, k, m, s, w, f0, (int)write_uc);
     return hipGetLastError();
 }
 hipError_t launch_qyy_assemble(hipStream_t st, const Consts &k, const Metrics &m, const State &s,

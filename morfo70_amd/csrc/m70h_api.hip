@@ -173,7 +173,8 @@ int run_preloop(m70h *h)
 }
 
 // One ADI step: solveQx (src/HydroQx.cpp:18-93) then solveQy (src/HydroQy.cpp:17-85).
-int run_step(m70h *h)
+// last: the final step of a Run, which leaves every public state field in memory.
+int run_step(m70h *h, bool last = true)
 {
     const double f0 = f0_for_step(h);
     h->f0_pos++;
@@ -193,7 +194,7 @@ int run_step(m70h *h)
                 return rc;
         if ((rc = timed(h, K_FUSED_QXY, [&] { return launch_lds_qxy_bs(st, k, s, h->rf, w, h->cw, h->rows_block); })))
             return rc;
-        if ((rc = timed(h, K_QX_TAIL, [&] { return launch_qx_tail(st, k, h->m, s, w, f0); }))) return rc;
+        if ((rc = timed(h, K_QX_TAIL, [&] { return launch_qx_tail(st, k, h->m, s, w, f0, last); }))) return rc;
         if ((rc = timed(h, K_FUSED_QYY, [&] { return launch_lds_qyy(st, k, h->m, s, h->rf, w, h->rows_block); }))) return rc;
         for (int pass = COLS_ELIM; pass <= COLS_BACKSUB; ++pass)
             if ((rc = timed(h, K_COLS_QY_ELIM + pass,
@@ -623,7 +624,7 @@ int m70h_run_async(m70h *h, int nIter)
     int rc = run_preloop(h);
     if (rc) return rc;
     for (int it = 0; it < nIter; ++it)
-        if ((rc = run_step(h))) return rc;
+        if ((rc = run_step(h, it == nIter - 1))) return rc;
     return M70H_OK;
 }
 
