@@ -41,17 +41,22 @@ KERNEL_BYTES_PER_CELL = {
     "periodic_qy": 8 * 5,
     "qyx_assemble": 8 * 10 + 1 + 8 * 4,     # U_c formed from U
     "thomas_qy": 8 * 5,
-    "qy_tail": 8 * 5 + 8 * 5,               # Qy,dQx,D,Zs in; dQy,Zs,D,V,Vc out (nx rows)
+    # Qy,dQx,D,Zs in; dQy,Zs,D,V out (nx rows); the FUSED step stores V_c only on a Run's
+    # last step (the PARTITION and EXACT steps on every step, 8 B more)
+    "qy_tail": 8 * 5 + 8 * 4,
     # FUSED family: y-sweeps in one launch (assembly inputs in, solution out)
     # the Qx projector forms Qx* from the predictor's column passes: E0..E2 and X (2 B) in
-    "fused_qxy": 8 * 10 + 2 + 1 + 8,        # D(2),Vc(2),E0..E2,nuD_x,dnuDx_dy,drag in, X, mask in; Qx out
+    # D(2), V(2) (V_c formed from it after a Run's first step), E0..E2, nuD_x, dnuDx_dy,
+    # drag in, X, mask in; Qx out
+    "fused_qxy": 8 * 10 + 2 + 1 + 8,
     # the qyy_assemble inputs with nuD in place of nuD_y and dnuD_dy, and no cD_y (the
     # constant drag model of this workload makes it cD0); Qy*, drag out
     "fused_qyy": 8 * 11 + 1 + 8 * 2,         # ... and U_c formed from U
     # FUSED x-sweeps, pass 1: assembly inputs in; drag, mask, E0..E2 out, plus the two
     # reduced rows (4 values each) per 8-row segment = 8 B per cell. The Qx predictor reads
-    # nuD in place of nuD_x, dnuD_dx and dnuDx_dy, and no cD_x (constant drag): 11 fields
-    "cols_qx_elim": 8 * 11 + 8 + 1 + 8 * 3 + 8,
+    # nuD in place of nuD_x, dnuD_dx and dnuDx_dy, no cD_x (constant drag) and, after a
+    # Run's first step, V_c formed from the V it reads: 10 fields
+    "cols_qx_elim": 8 * 10 + 8 + 1 + 8 * 3 + 8,
     "cols_qy_elim": 8 * 10 + 1 + 8 * 3 + 8,  # U_c formed from U
     # pass 3: E0..E2 and the two interface values per segment in, the solution out
     "cols_qx_backsub": 8 * 3 + 2 + 8,

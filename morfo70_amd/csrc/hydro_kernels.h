@@ -21,7 +21,8 @@ hipError_t launch_qyy_assemble(hipStream_t st, const Consts &k, const Metrics &m
                                const RunFields &rf, const Work &w);
 hipError_t launch_qyx_assemble(hipStream_t st, const Consts &k, const Metrics &m, const State &s,
                                const RunFields &rf, const Work &w);
-hipError_t launch_qy_tail(hipStream_t st, const Consts &k, const State &s);
+// write_vc = false skips the V_c store, for steps whose successors in the Run form V_c from V.
+hipError_t launch_qy_tail(hipStream_t st, const Consts &k, const State &s, bool write_vc = true);
 hipError_t launch_nonfinite(hipStream_t st, const double *x, long n, int *flag);
 hipError_t launch_init_zs(hipStream_t st, double *Zs, const double *D, const double *Z, const double *hb, long n);
 
@@ -45,8 +46,10 @@ hipError_t launch_transposed_rows(hipStream_t st, const double *A, const double 
 
 int cols_segments(int n);
 enum ColsPass { COLS_ELIM = 0, COLS_REDUCED = 1, COLS_BACKSUB = 2 };
+// vc_from_v: the Qx sweeps form V_c from V (exact once a step of this Run has stored both;
+// see launch_qy_tail), so they need no V_c in memory.
 hipError_t launch_cols3_qxx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                            const Work &w, double f0, const ColsWork &cw, int tile, int pass);
+                            const Work &w, double f0, const ColsWork &cw, int tile, int pass, bool vc_from_v = false);
 hipError_t launch_cols3_qyx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
                             const Work &w, const ColsWork &cw, int tile, int pass);
 bool rows_lds_supported(int n);
@@ -54,7 +57,7 @@ int rows_block_for(int n, int rb);
 // Qx projector rows reading the Qx predictor's solution from its elimination and reduced
 // passes (E0..E2, X of cw) in place of a back-substitution pass.
 hipError_t launch_lds_qxy_bs(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w,
-                             const ColsWork &cw, int rb);
+                             const ColsWork &cw, int rb, bool vc_from_v = false);
 hipError_t launch_lds_qyy(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
                           const Work &w, int rb);
 

@@ -298,7 +298,7 @@ __host__ __device__ __forceinline__ double new_depth(const State &s, const Const
     return s.D[q] + incrZs;
 }
 
-__host__ __device__ __forceinline__ void qy_tail_body(const Consts &k, const State &s, int i, int j)
+__host__ __device__ __forceinline__ void qy_tail_body(const Consts &k, const State &s, int i, int j, bool write_vc = true)
 {
     const int ny = k.ny;
     const long r = (long)i * ny;
@@ -317,7 +317,7 @@ __host__ __device__ __forceinline__ void qy_tail_body(const Consts &k, const Sta
     s.Zs[r + j] = s.Zs[r + j] + incrZs;
     s.Dn[r + j] = dn;
     s.V[r + ny + j] = v_j;
-    s.Vc[r + j] = 0.5 * (v_j + v_m);
+    if (write_vc) s.Vc[r + j] = 0.5 * (v_j + v_m);
     if (i == 0) {
         const double qy0 = k.bcQy * s.Qy[ny + j];
         const double d_y0 = 2 * dn - new_depth(s, k, 1, j);
@@ -327,10 +327,10 @@ __host__ __device__ __forceinline__ void qy_tail_body(const Consts &k, const Sta
     if (i == nx - 1) s.Qy[(long)nx * ny + j] = s.Qy[(long)(nx - 1) * ny + j] * k.inv_3;
 }
 
-__global__ void __launch_bounds__(256) k_qy_tail(Consts k, State s)
+__global__ void __launch_bounds__(256) k_qy_tail(Consts k, State s, int write_vc)
 {
     CELL_IJ_ONLY(k.nx)
-    qy_tail_body(k, s, i, j);
+    qy_tail_body(k, s, i, j, write_vc != 0);
 }
 
 // ------------------------------------------------------------------------------------
@@ -418,7 +418,7 @@ __device__ __forceinline__ void qx_tail2_body(const Consts &k, const Metrics &m,
 }
 
 // Columns j0, j0 + 1 of row i (j0 even, ny even).
-__device__ __forceinline__ void qy_tail2_body(const Consts &k, const State &s, int i, int j0)
+__device__ __forceinline__ void qy_tail2_body(const Consts &k, const State &s, int i, int j0, bool write_vc = true)
 {
     const int ny = k.ny, nx = k.nx;
     const long r = (long)i * ny;
@@ -449,7 +449,7 @@ __device__ __forceinline__ void qy_tail2_body(const Consts &k, const State &s, i
     st2(s.Zs, r + j0, zs.x + inc_a, zs.y + inc_b);
     st2(s.Dn, r + j0, dn_a, dn_b);
     st2(s.V, r + ny + j0, v_a, v_b);
-    st2(s.Vc, r + j0, 0.5 * (v_a + v_m1), 0.5 * (v_b + v_a));
+    if (write_vc) st2(s.Vc, r + j0, 0.5 * (v_a + v_m1), 0.5 * (v_b + v_a));
     if (i == 0) {
         const double2 q1 = ld2(s.Qy, ny + j0);
         const double qa = k.bcQy * q1.x, qb = k.bcQy * q1.y;
@@ -470,12 +470,12 @@ __global__ void __launch_bounds__(256) k_qx_tail2(Consts k, Metrics m, State s, 
     qx_tail2_body(k, m, s, w, f0, i, j0, write_uc != 0);
 }
 
-__global__ void __launch_bounds__(256) k_qy_tail2(Consts k, State s)
+__global__ void __launch_bounds__(256) k_qy_tail2(Consts k, State s, int write_vc)
 {
     const int j0 = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int i = blockIdx.y;
     if (j0 >= k.ny || i >= k.nx) return;
-    qy_tail2_body(k, s, i, j0);
+    qy_tail2_body(k, s, i, j0, write_vc != 0);
 }
 
 // ------------------------------------------------------------------------------------
@@ -598,12 +598,12 @@ hipError_t launch_qyx_assemble(hipStream_t st, const Consts &k, const Metrics &m
     hipLaunchKernelGGL(k_qyx_assemble, cell_grid(k.ny, k.nx - 1), dim3(256), 0, st, k, m, s, rf, w);
     return hipGetLastError();
 }
-hipError_t launch_qy_tail(hipStream_t st, const Consts &k, const State &s)
+hipError_t launch_qy_tail(hipStream_t st, const Consts &k, const State &s, bool write_vc)
 {
     if (k.ny % 2 == 0 && k.ny >= 4)
-        hipLaunchKernelGGL(k_qy_tail2, cell_grid(k.ny / 2, k.nx), dim3(256), 0, st, k, s);
+        hipLaunchKernelGGL(k_qy_tail2, cell_grid(k.ny / 2, k.nx), dim3(256), 0, st, k, s, (int)write_vc);
     else
-        hipLaunchKernelGGL(k_qy_tail, cell_grid(k.ny, k.nx), dim3(256), 0, st, k, s);
+        hipLaunchKernelGGL(k_qy_tail, cell_grid(k.ny, k.nx), dim3(256), 0, st, k, s, (int)write_vc);
     return hipGetLastError();
 }
 hipError_t launch_nonfinite(hipStream_t st, const double *x, long n, int *flag)

@@ -174,7 +174,9 @@ int run_preloop(m70h *h)
 
 // One ADI step: solveQx (src/HydroQx.cpp:18-93) then solveQy (src/HydroQy.cpp:17-85).
 // last: the final step of a Run, which leaves every public state field in memory.
-int run_step(m70h *h, bool last = true)
+// first: the first step of a Run, which reads V_c as the caller left it; the FUSED steps
+// after it form V_c from V, so only a Run's last step stores V_c and U_c.
+int run_step(m70h *h, bool last = true, bool first = true)
 {
     const double f0 = f0_for_step(h);
     h->f0_pos++;
@@ -190,9 +192,9 @@ int run_step(m70h *h, bool last = true)
         // projector forms the Qx predictor's solution itself from the first two passes.
         for (int pass = COLS_ELIM; pass <= COLS_REDUCED; ++pass)
             if ((rc = timed(h, K_COLS_QX_ELIM + pass,
-                            [&] { return launch_cols3_qxx(st, k, h->m, s, h->rf, w, f0, h->cw, h->tile_cols, pass); })))
+                            [&] { return launch_cols3_qxx(st, k, h->m, s, h->rf, w, f0, h->cw, h->tile_cols, pass, !first); })))
                 return rc;
-        if ((rc = timed(h, K_FUSED_QXY, [&] { return launch_lds_qxy_bs(st, k, s, h->rf, w, h->cw, h->rows_block); })))
+        if ((rc = timed(h, K_FUSED_QXY, [&] { return launch_lds_qxy_bs(st, k, s, h->rf, w, h->cw, h->rows_block, !first); })))
             return rc;
         if ((rc = timed(h, K_QX_TAIL, [&] { return launch_qx_tail(st, k, h->m, s, w, f0, last); }))) return rc;
         if ((rc = timed(h, K_FUSED_QYY, [&] { return launch_lds_qyy(st, k, h->m, s, h->rf, w, h->rows_block); }))) return rc;
@@ -200,7 +202,7 @@ int run_step(m70h *h, bool last = true)
             if ((rc = timed(h, K_COLS_QY_ELIM + pass,
                             [&] { return launch_cols3_qyx(st, k, h->m, s, h->rf, w, h->cw, h->tile_cols, pass); })))
                 return rc;
-        if ((rc = timed(h, K_QY_TAIL, [&] { return launch_qy_tail(st, k, s); }))) return rc;
+        if ((rc = timed(h, K_QY_TAIL, [&] { return launch_qy_tail(st, k, s, last); }))) return rc;
         std::swap(s.D, s.Dn);
         return M70H_OK;
     }
@@ -624,7 +626,7 @@ int m70h_run_async(m70h *h, int nIter)
     int rc = run_preloop(h);
     if (rc) return rc;
     for (int it = 0; it < nIter; ++it)
-        if ((rc = run_step(h, it == nIter - 1))) return rc;
+        if ((rc = run_step(h, it == nIter - 1, it == 0))) return rc;
     return M70H_OK;
 }
 

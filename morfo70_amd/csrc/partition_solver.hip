@@ -65,11 +65,46 @@ __device__ __forceinline__ void st_pair(double *p, long e, double a, double b)
 // Qx projector rows with Qx* (the predictor's solution, row i+1 = unknown i of each column)
 // formed from the predictor's three-pass column solve instead of read back after a
 // back-substitution pass: the FUSED step skips that pass and never stores Qx*.
+// V_c row i at column c is 0.5*(V[i+1][c] + V[i+1][c-1]) (periodic in c), the Qy tail's own
+// expression for it. With VCF a sweep forms the V_c values it needs from V rows i+1, i+2 instead
+// of reading V_c rows i, i+1: the same bits wherever V_c was stored by a step, which holds
+// from a Run's second step on, so the steps between need not store V_c.
+__device__ __forceinline__ void qxy_vc_from_v(const Consts &k, const State &s, int i, int j, QxyIn &q)
+{
+    const int ny = k.ny;
+    const int jp = wrap_inc(j, ny), jm = wrap_dec(j, ny), jmm = wrap_dec(jm, ny);
+    const double *v1 = s.V + (long)(i + 1) * ny, *v2 = v1 + ny;
+    q.Vcp0 = 0.5 * (v1[jp] + v1[j]); q.Vcp1 = 0.5 * (v2[jp] + v2[j]);
+    q.Vcm0 = 0.5 * (v1[jm] + v1[jmm]); q.Vcm1 = 0.5 * (v2[jm] + v2[jmm]);
+}
+// Elements e, e+1 (e even, ny even): V columns e-2..e+2 from three pair loads per row.
+__device__ __forceinline__ void qxy_vc_from_v2(const Consts &k, const State &s, int i, int e, QxyIn &a, QxyIn &b)
+{
+    const int ny = k.ny;
+    const int em = (e == 0) ? ny - 2 : e - 2, ep = (e + 2 == ny) ? 0 : e + 2;
+    const long r1 = (long)(i + 1) * ny, r2 = r1 + ny;
+    double x, p2, p1, p0, q1, q2;   // V row i+1 at columns e-2, e-1, e, e+1, e+2
+    ld_pair(s.V, r1 + em, p2, p1);
+    ld_pair(s.V, r1 + e, p0, q1);
+    ld_pair(s.V, r1 + ep, q2, x);
+    double t2, t1, t0, u1, u2;      // V row i+2
+    ld_pair(s.V, r2 + em, t2, t1);
+    ld_pair(s.V, r2 + e, t0, u1);
+    ld_pair(s.V, r2 + ep, u2, x);
+    // V_c at columns e-1 (m), e (0), e+1 (n), e+2 (p) of rows i and i+1
+    const double c0m = 0.5 * (p1 + p2), c00 = 0.5 * (p0 + p1), c0n = 0.5 * (q1 + p0), c0p = 0.5 * (q2 + q1);
+    const double c1m = 0.5 * (t1 + t2), c10 = 0.5 * (t0 + t1), c1n = 0.5 * (u1 + t0), c1p = 0.5 * (u2 + u1);
+    a.Vcp0 = c0n; a.Vcp1 = c1n; a.Vcm0 = c0m; a.Vcm1 = c1m;
+    b.Vcp0 = c0p; b.Vcp1 = c1p; b.Vcm0 = c00; b.Vcm1 = c10;
+}
+
+template <bool VCF>
 struct ProvQxyBs {
     Consts k; State s; RunFields rf; Work w; ColsWork cw;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     {
         QxyIn q = qxy_load(k, s, rf, w, line, kk);
+        if (VCF) qxy_vc_from_v(k, s, line, kk, q);
         q.qx = cols_value(cw, k.nx - 1, k.ny, line, kk);
         asm_qxy_vals(k, q, a, b, c, f);
     }
@@ -78,6 +113,7 @@ struct ProvQxyBs {
     {
         QxyIn q0, q1;
         qxy_load2(k, s, rf, w, line, kk, q0, q1);
+        if (VCF) qxy_vc_from_v2(k, s, line, kk, q0, q1);
         cols_value2(cw, k.nx - 1, k.ny, line, kk, q0.qx, q1.qx);
         asm_qxy_vals(k, q0, a0, b0, c0, f0);
         asm_qxy_vals(k, q1, a1, b1, c1, f1);
@@ -93,7 +129,7 @@ constexpr int CM = 8;   // rows per column segment in the three-pass column solv
 // nuD_x, dnuD_dx and dnuDx_dy are formed from nuD rows i, i+1 at columns j, j+-1 by
 // run_fields_body's expressions (one array in the window instead of three), and with the
 // constant drag model (CDC) cD_x is cD0 itself (0.5*(cD0+cD0) is exact), not read.
-template <bool CDC>
+template <bool CDC, bool VCF>
 __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, const State &s, const RunFields &rf,
                                             const Work &w, double f0, int j, int k0, int mm, double (&lo)[CM],
                                             double (&di)[CM], double (&up)[CM], double (&rh)[CM])
@@ -112,7 +148,7 @@ __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, 
         q.Dm = s.D[at(im)]; q.D0 = s.D[at(i)]; q.D1 = s.D[at(i + 1)]; q.D2 = s.D[at(i2)];
         q.Zm = s.Zs[at(im)]; q.Z0 = s.Zs[at(i)]; q.Z1 = s.Zs[at(i + 1)]; q.Z2 = s.Zs[at(i2)];
         q.Qm = s.dQx[at(im)]; q.Q0 = s.dQx[at(i)]; q.Q1 = s.dQx[at(i + 1)]; q.Q2 = s.dQx[at(i2)];
-        q.Vc0 = s.Vc[at(i)]; q.Vc1 = s.Vc[at(i + 1)];
+        if (!VCF) { q.Vc0 = s.Vc[at(i)]; q.Vc1 = s.Vc[at(i + 1)]; }
         q.U0 = s.U[at(i)]; q.U1 = s.U[at(i + 1)]; q.U2 = s.U[at(i + 2)];
         q.dqy0 = s.dQy[at(i)]; q.dqy1 = s.dQy[at(i + 1)];
         q.v1j = s.V[at(i + 1)]; q.v1m = s.V[atm(i + 1)]; q.v2j = s.V[at(i + 2)]; q.v2m = s.V[atm(i + 2)];
@@ -126,10 +162,14 @@ __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, 
                 q.Dm = q.D0; q.D0 = q.D1; q.D1 = q.D2; q.D2 = s.D[at(i2)];
                 q.Zm = q.Z0; q.Z0 = q.Z1; q.Z1 = q.Z2; q.Z2 = s.Zs[at(i2)];
                 q.Qm = q.Q0; q.Q0 = q.Q1; q.Q1 = q.Q2; q.Q2 = s.dQx[at(i2)];
-                q.Vc0 = q.Vc1; q.Vc1 = s.Vc[at(i + 1)];
+                if (!VCF) { q.Vc0 = q.Vc1; q.Vc1 = s.Vc[at(i + 1)]; }
                 q.U0 = q.U1; q.U1 = q.U2; q.U2 = s.U[at(i + 2)];
                 q.dqy0 = q.dqy1; q.dqy1 = s.dQy[at(i + 1)];
                 q.v1j = q.v2j; q.v1m = q.v2m; q.v2j = s.V[at(i + 2)]; q.v2m = s.V[atm(i + 2)];
+            }
+            if (VCF) {   // V_c rows i, i+1 from the V window (rows i+1, i+2 at j, j-1)
+                q.Vc0 = 0.5 * (q.v1j + q.v1m);
+                q.Vc1 = 0.5 * (q.v2j + q.v2m);
             }
             q.Qx1 = s.Qx[at(i + 1)];
             q.cD = CDC ? k.cD0 : rf.cD_x[at(i)];
@@ -149,14 +189,14 @@ __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, 
     }
 }
 
-template <bool CDC>
+template <bool CDC, bool VCF>
 struct ProvQxx {      // Qx predictor, lines along x (columns): line = j, k = i
     Consts k; Metrics m; State s; RunFields rf; Work w; double f0;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qxx(k, m, s, rf, w, f0, kk, line, a, b, c, f); }
     __device__ __forceinline__ void segment(int line, int k0, int mm, double (&lo)[CM], double (&di)[CM],
                                             double (&up)[CM], double (&rh)[CM]) const
-    { qxx_segment<CDC>(k, m, s, rf, w, f0, line, k0, mm, lo, di, up, rh); }
+    { qxx_segment<CDC, VCF>(k, m, s, rf, w, f0, line, k0, mm, lo, di, up, rh); }
 };
 struct ProvQyx {      // Qy projector, columns
     Consts k; Metrics m; State s; RunFields rf; Work w;
@@ -782,11 +822,15 @@ static hipError_t launch_cols3(hipStream_t st, const Prov &prov, const ColsWork 
 }
 
 hipError_t launch_cols3_qxx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                            const Work &w, double f0, const ColsWork &cw, int tile, int pass)
+                            const Work &w, double f0, const ColsWork &cw, int tile, int pass, bool vc_from_v)
 {
+    double *Y = s.Qx + k.ny;
+    const int n = k.nx - 1, ny = k.ny;
     if (k.drag_model == 0)
-        return launch_cols3(st, ProvQxx<true>{k, m, s, rf, w, f0}, cw, s.Qx + k.ny, k.nx - 1, k.ny, tile, pass);
-    return launch_cols3(st, ProvQxx<false>{k, m, s, rf, w, f0}, cw, s.Qx + k.ny, k.nx - 1, k.ny, tile, pass);
+        return vc_from_v ? launch_cols3(st, ProvQxx<true, true>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass)
+                         : launch_cols3(st, ProvQxx<true, false>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass);
+    return vc_from_v ? launch_cols3(st, ProvQxx<false, true>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass)
+                     : launch_cols3(st, ProvQxx<false, false>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass);
 }
 hipError_t launch_cols3_qyx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
                             const Work &w, const ColsWork &cw, int tile, int pass)
@@ -815,7 +859,7 @@ struct GProvQxx {
     { asm_qxx(d->k, d->m, d->s, d->rf, d->w, d->f0, kk, line, a, b, c, f); }
     __device__ __forceinline__ void segment(int line, int k0, int mm, double (&lo)[CM], double (&di)[CM],
                                             double (&up)[CM], double (&rh)[CM]) const
-    { qxx_segment<false>(d->k, d->m, d->s, d->rf, d->w, d->f0, line, k0, mm, lo, di, up, rh); }
+    { qxx_segment<false, false>(d->k, d->m, d->s, d->rf, d->w, d->f0, line, k0, mm, lo, di, up, rh); }
 };
 struct GProvQyx {
     const ScenDesc *d;
@@ -996,9 +1040,11 @@ static hipError_t launch_rows_lds(hipStream_t st, const Prov &prov, double *Y, l
 
 // Row sweeps with the assembly staged through LDS (one row per block).
 hipError_t launch_lds_qxy_bs(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w,
-                             const ColsWork &cw, int rb)
+                             const ColsWork &cw, int rb, bool vc_from_v)
 {
-    return launch_rows_lds(st, ProvQxyBs{k, s, rf, w, cw}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true, rb);
+    if (vc_from_v)
+        return launch_rows_lds(st, ProvQxyBs<true>{k, s, rf, w, cw}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true, rb);
+    return launch_rows_lds(st, ProvQxyBs<false>{k, s, rf, w, cw}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true, rb);
 }
 hipError_t launch_lds_qyy(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
                           const Work &w, int rb)
