@@ -173,9 +173,9 @@ def test_profiling_one_kernel_leaves_results_unchanged():
 
 @pytest.mark.parametrize("solver", [0, 2])
 def test_run_starts_from_the_callers_derived_fields(solver):
-    """A Run's first step takes U_c and V_c as the caller left them, also where they do not
-    match U and V: the FUSED steps after a Run's first form V_c from V and U_c from U, and
-    store both only on the Run's last step. Against the oracle given the same perturbed
+    """A Run's first step takes U_c, V_c, dQx_dx and dQy_dy as the caller left them, also
+    where they do not match U, V, Qx and Qy: the FUSED steps after a Run's first form V_c
+    and U_c from V and U, and store them only on the Run's last step. Against the oracle given the same perturbed
     state, over two Runs: bit for bit for EXACT, at the 1e-10 bar for FUSED (whose column
     elimination rounds differently by a few ulps). The perturbation moves the oracle's result
     far above that bar, so a step that ignored the caller's V_c would fail."""
@@ -191,8 +191,8 @@ def test_run_starts_from_the_callers_derived_fields(solver):
     st = o.get_state()
     rng = np.random.default_rng(3)
     wet = st["D"] > 0.05
-    for k in ("U_c", "V_c"):
-        st[k] = st[k] + 1e-3 * rng.standard_normal(st[k].shape) * wet
+    for k in ("U_c", "V_c", "dQx_dx", "dQy_dy"):
+        st[k] = st[k] + 1e-4 * rng.standard_normal(st[k].shape) * wet
     ctl = orc.OracleHydro(sc)
     ctl.set_state(o.get_state())
     o.set_state(st)
