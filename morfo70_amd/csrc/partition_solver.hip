@@ -726,7 +726,9 @@ template <class Prov>
 __device__ __forceinline__ void cols_elim_body(const Prov &prov, const ColsWork &cw, int n, int ny, int bx, int by)
 {
     const int j = bx * 64 + threadIdx.x;
-    const int sg = by * 4 + threadIdx.y;
+    // a wavefront is one threadIdx.y of the 64x4 workgroup: its segment, and so its rows,
+    // are uniform, which keeps the per-row metrics in scalar loads
+    const int sg = __builtin_amdgcn_readfirstlane(by * 4 + (int)threadIdx.y);
     const int nseg = cw.nseg;
     if (j >= ny || sg >= nseg) return;
     const int k0 = (int)(((long)sg * n) / nseg);
@@ -765,7 +767,9 @@ __device__ __forceinline__ void cols_backsub_body(const ColsWork &cw, double *__
                                                   int by)
 {
     const int j = bx * 64 + threadIdx.x;
-    const int sg = by * 4 + threadIdx.y;
+    // a wavefront is one threadIdx.y of the 64x4 workgroup: its segment, and so its rows,
+    // are uniform, which keeps the per-row metrics in scalar loads
+    const int sg = __builtin_amdgcn_readfirstlane(by * 4 + (int)threadIdx.y);
     const int nseg = cw.nseg;
     if (j >= ny || sg >= nseg) return;
     const int k0 = (int)(((long)sg * n) / nseg);
