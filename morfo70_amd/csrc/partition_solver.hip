@@ -143,6 +143,8 @@ __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, 
     const double *nuD = rf.nuD;
     double n0 = nuD[at(k0)], n0p = nuD[atp(k0)], n0m = nuD[atm(k0)];   // nuD row i
     QxxIn q;
+    int wis[CM];
+    double drags[CM];
     {
         const int i = k0, im = i > 0 ? i - 1 : 0, i2 = cl(i + 2);
         q.Dm = s.D[at(im)]; q.D0 = s.D[at(i)]; q.D1 = s.D[at(i + 1)]; q.D2 = s.D[at(i2)];
@@ -180,11 +182,16 @@ __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, 
             const double nux_p = 0.5 * (n0p + n1p), nux_m = 0.5 * (n0m + n1m);
             q.dnuDx_dy = k.inv_2dy_c2c * (nux_p - nux_m);
             n0 = n1; n0p = n1p; n0m = n1m;
-            int wi;
-            double drag;
-            asm_qxx_vals(k, mt, q, f0, i, di[t], lo[t], up[t], rh[t], wi, drag);
-            w.w[at(i)] = (uint8_t)wi;
-            w.drag[at(i)] = drag;
+            asm_qxx_vals(k, mt, q, f0, i, di[t], lo[t], up[t], rh[t], wis[t], drags[t]);
+        }
+    }
+    // the side outputs go out after every load of the segment: stores in the loop would
+    // keep later rows' loads (and the per-row metrics' scalar loads) behind them
+#pragma unroll
+    for (int t = 0; t < CM; ++t) {
+        if (t < mm) {
+            w.w[at(k0 + t)] = (uint8_t)wis[t];
+            w.drag[at(k0 + t)] = drags[t];
         }
     }
 }
