@@ -346,78 +346,6 @@ __device__ __forceinline__ void st2(double *p, long e, double a, double b)
 }
 
 // Columns j0, j0 + 1 of row i (j0 even, ny even).
-__device__ __forceinline__ void qx_tail2_body(const Consts &k, const Metrics &m, const State &s, const Work &w,
-                                              double f0, int i, int j0, bool write_uc = true)
-{
-    const int ny = k.ny, nx = k.nx;
-    const long r = (long)i * ny;
-    const int c2 = (j0 + 2 == ny) ? 0 : j0 + 2;
-    const double dt = k.dt;
-    // Qx rows i-1..i+2 at (j0, j0+1), through the row-0 BC (qx_at)
-    auto qrow = [&](int row, int c) -> double2 {
-        if (row < 0 || row > nx) return make_double2(0.0, 0.0);
-        if (row == 0) {
-            const double2 v = ld2(s.Qx, ny + c);
-            return make_double2(k.f1 * v.x + f0, k.f1 * v.y + f0);
-        }
-        return ld2(s.Qx, (long)row * ny + c);
-    };
-    double2 q[4];
-    for (int t = 0; t < 4; ++t) q[t] = qrow(i - 1 + t, j0);
-    double2 gd[3];
-    double2 dq_i = make_double2(0.0, 0.0), inc_i = make_double2(0.0, 0.0);
-    for (int t = 0; t < 3; ++t) {
-        const int row = i - 1 + t;
-        if (row < 0 || row > nx - 1) { gd[t] = make_double2(0.0, 0.0); continue; }
-        const double idx = m.inv_dxx[row];
-        const double2 dqy = ld2(s.dQy, (long)row * ny + j0), dd = ld2(s.D, (long)row * ny + j0);
-        const double dqa = idx * (q[t + 1].x - q[t].x), dqb = idx * (q[t + 1].y - q[t].y);
-        const double ia = -dt * (dqa + dqy.x), ib = -dt * (dqb + dqy.y);
-        gd[t] = make_double2(dd.x + ia, dd.y + ib);
-        if (t == 1) { dq_i = make_double2(dqa, dqb); inc_i = make_double2(ia, ib); }
-    }
-    double ua, ub;
-    if (i == 0) {
-        ua = q[1].x / (1.5 * gd[1].x - 0.5 * gd[2].x);
-        ub = q[1].y / (1.5 * gd[1].y - 0.5 * gd[2].y);
-    } else {
-        ua = q[1].x / (0.5 * (gd[0].x + gd[1].x));
-        ub = q[1].y / (0.5 * (gd[0].y + gd[1].y));
-    }
-    double u1a, u1b;
-    if (i + 1 <= nx - 1) {
-        u1a = q[2].x / (0.5 * (gd[1].x + gd[2].x));
-        u1b = q[2].y / (0.5 * (gd[1].y + gd[2].y));
-    } else {
-        const double2 un = ld2(s.U, (long)nx * ny + j0);   // U row nx is never written by the reference
-        u1a = un.x;
-        u1b = un.y;
-    }
-    const double2 zs = ld2(s.Zs, r + j0);
-    const double gza = zs.x + inc_i.x, gzb = zs.y + inc_i.y;
-    const double2 dqy_i = ld2(s.dQy, r + j0);
-    // guess_D / guess_Zs of column j0+2 (the right neighbour of j0+1)
-    double gd_c2, gz_c2, dqy_c2;
-    {
-        const double qa = qrow(i, c2).x, qb = qrow(i + 1, c2).x;
-        const double dq = m.inv_dxx[i] * (qb - qa);
-        dqy_c2 = s.dQy[r + c2];
-        const double incrZs = -dt * (dq + dqy_c2);
-        gd_c2 = s.D[r + c2] + incrZs;
-        gz_c2 = s.Zs[r + c2] + incrZs;
-    }
-    const int wa = wet_rule(gd[1].x, gd[1].y, gza, gzb, dqy_i.x, dqy_i.y, k.Dmask);
-    const int wb = wet_rule(gd[1].y, gd_c2, gzb, gz_c2, dqy_i.y, dqy_c2, k.Dmask);
-    w.w[r + j0] = (uint8_t)wa;
-    w.w[r + j0 + 1] = (uint8_t)wb;
-    st2(s.dQx, r + j0, dq_i.x, dq_i.y);
-    st2(w.gD, r + j0, gd[1].x, gd[1].y);
-    st2(s.U, r + j0, ua, ub);
-    if (write_uc) st2(s.Uc, r + j0, m.x2c_d[i] * ua + m.x2c_up[i] * u1a, m.x2c_d[i] * ub + m.x2c_up[i] * u1b);
-    if (i == 0) st2(s.Qx, j0, q[1].x, q[1].y);
-}
-
-// Columns j0, j0 + 1 of row i (j0 even, ny even).
 __device__ __forceinline__ void qy_tail2_body(const Consts &k, const State &s, int i, int j0, bool write_vc = true)
 {
     const int ny = k.ny, nx = k.nx;
@@ -462,13 +390,115 @@ __device__ __forceinline__ void qy_tail2_body(const Consts &k, const State &s, i
     }
 }
 
-__global__ void __launch_bounds__(256) k_qx_tail2(Consts k, Metrics m, State s, Work w, double f0, int write_uc)
+// Rows i0..i1-1 of columns j0, j0 + 1 (j0 even, ny even): the Qx tail over a run of rows,
+// with the Qx rows i-1..i+2 and the guess depths of rows i-1..i+1 held in registers from
+// one row to the next, so each row of Qx, D and dQy_dy is loaded once per run instead of
+// four or three times. The same expressions on the same values: the same bits.
+__device__ __forceinline__ void qx_tail2_rows(const Consts &k, const Metrics &m, const State &s, const Work &w,
+                                              double f0, int i0, int i1, int j0, bool write_uc)
+{
+    const int ny = k.ny, nx = k.nx;
+    const int c2 = (j0 + 2 == ny) ? 0 : j0 + 2;
+    const double dt = k.dt;
+    auto qrow = [&](int row) -> double2 {   // Qx row `row` at (j0, j0+1), through the row-0 BC
+        if (row < 0 || row > nx) return make_double2(0.0, 0.0);
+        if (row == 0) {
+            const double2 v = ld2(s.Qx, ny + j0);
+            return make_double2(k.f1 * v.x + f0, k.f1 * v.y + f0);
+        }
+        return ld2(s.Qx, (long)row * ny + j0);
+    };
+    auto qrow_c2 = [&](int row) -> double {   // the same at column j0+2
+        if (row < 0 || row > nx) return 0.0;
+        if (row == 0) return k.f1 * s.Qx[ny + c2] + f0;
+        return s.Qx[(long)row * ny + c2];
+    };
+    struct G { double2 gd, dq, inc, dqy; };   // guess_D, dQx_dx, its depth increment, dQy_dy
+    auto grow = [&](int row, double2 lo, double2 hi) -> G {
+        G g;
+        g.gd = g.dq = g.inc = g.dqy = make_double2(0.0, 0.0);
+        if (row < 0 || row > nx - 1) return g;
+        const double idx = m.inv_dxx[row];
+        const double2 dqy = ld2(s.dQy, (long)row * ny + j0), dd = ld2(s.D, (long)row * ny + j0);
+        const double dqa = idx * (hi.x - lo.x), dqb = idx * (hi.y - lo.y);
+        const double ia = -dt * (dqa + dqy.x), ib = -dt * (dqb + dqy.y);
+        g.gd = make_double2(dd.x + ia, dd.y + ib);
+        g.dq = make_double2(dqa, dqb);
+        g.inc = make_double2(ia, ib);
+        g.dqy = dqy;
+        return g;
+    };
+    double2 qm = qrow(i0 - 1), q0 = qrow(i0), q1 = qrow(i0 + 1), q2 = qrow(i0 + 2);
+    G gm = grow(i0 - 1, qm, q0), g0 = grow(i0, q0, q1), g1 = grow(i0 + 1, q1, q2);
+    double cqa = qrow_c2(i0), cqb = qrow_c2(i0 + 1);
+    for (int i = i0; i < i1; ++i) {
+        const long r = (long)i * ny;
+        double ua, ub;
+        if (i == 0) {
+            ua = q0.x / (1.5 * g0.gd.x - 0.5 * g1.gd.x);
+            ub = q0.y / (1.5 * g0.gd.y - 0.5 * g1.gd.y);
+        } else {
+            ua = q0.x / (0.5 * (gm.gd.x + g0.gd.x));
+            ub = q0.y / (0.5 * (gm.gd.y + g0.gd.y));
+        }
+        double u1a, u1b;
+        if (i + 1 <= nx - 1) {
+            u1a = q1.x / (0.5 * (g0.gd.x + g1.gd.x));
+            u1b = q1.y / (0.5 * (g0.gd.y + g1.gd.y));
+        } else {
+            const double2 un = ld2(s.U, (long)nx * ny + j0);   // U row nx is never written by the reference
+            u1a = un.x;
+            u1b = un.y;
+        }
+        const double2 zs = ld2(s.Zs, r + j0);
+        const double gza = zs.x + g0.inc.x, gzb = zs.y + g0.inc.y;
+        double gd_c2, gz_c2, dqy_c2;
+        {   // guess_D / guess_Zs of column j0+2 (the right neighbour of j0+1)
+            const double dq = m.inv_dxx[i] * (cqb - cqa);
+            dqy_c2 = s.dQy[r + c2];
+            const double incrZs = -dt * (dq + dqy_c2);
+            gd_c2 = s.D[r + c2] + incrZs;
+            gz_c2 = s.Zs[r + c2] + incrZs;
+        }
+        const int wa = wet_rule(g0.gd.x, g0.gd.y, gza, gzb, g0.dqy.x, g0.dqy.y, k.Dmask);
+        const int wb = wet_rule(g0.gd.y, gd_c2, gzb, gz_c2, g0.dqy.y, dqy_c2, k.Dmask);
+        w.w[r + j0] = (uint8_t)wa;
+        w.w[r + j0 + 1] = (uint8_t)wb;
+        st2(s.dQx, r + j0, g0.dq.x, g0.dq.y);
+        st2(w.gD, r + j0, g0.gd.x, g0.gd.y);
+        st2(s.U, r + j0, ua, ub);
+        if (write_uc) st2(s.Uc, r + j0, m.x2c_d[i] * ua + m.x2c_up[i] * u1a, m.x2c_d[i] * ub + m.x2c_up[i] * u1b);
+        if (i == 0) st2(s.Qx, j0, q0.x, q0.y);
+        if (i + 1 < i1) {
+            const double2 q3 = qrow(i + 3);
+            gm = g0;
+            g0 = g1;
+            g1 = grow(i + 2, q2, q3);
+            qm = q0; q0 = q1; q1 = q2; q2 = q3;
+            cqa = cqb;
+            cqb = qrow_c2(i + 2);
+        }
+    }
+}
+
+// Row i alone (the batched step's tails, one row per thread).
+__device__ __forceinline__ void qx_tail2_body(const Consts &k, const Metrics &m, const State &s, const Work &w,
+                                              double f0, int i, int j0, bool write_uc = true)
+{
+    qx_tail2_rows(k, m, s, w, f0, i, i + 1, j0, write_uc);
+}
+
+// QXT_ROWS rows per thread, two columns (ny even): grid (ceil(ny/512), ceil(nx/QXT_ROWS)).
+constexpr int QXT_ROWS = 8;
+__global__ void __launch_bounds__(256) k_qx_tail2r(Consts k, Metrics m, State s, Work w, double f0, int write_uc)
 {
     const int j0 = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int i = blockIdx.y;
-    if (j0 >= k.ny || i >= k.nx) return;
-    qx_tail2_body(k, m, s, w, f0, i, j0, write_uc != 0);
+    const int i0 = (int)blockIdx.y * QXT_ROWS;
+    if (j0 >= k.ny || i0 >= k.nx) return;
+    const int i1 = i0 + QXT_ROWS < k.nx ? i0 + QXT_ROWS : k.nx;
+    qx_tail2_rows(k, m, s, w, f0, i0, i1, j0, write_uc != 0);
 }
+
 
 __global__ void __launch_bounds__(256) k_qy_tail2(Consts k, State s, int write_vc)
 {
@@ -581,7 +611,8 @@ hipError_t launch_qx_tail(hipStream_t st, const Consts &k, const Metrics &m, con
                           double f0, bool write_uc)
 {
     if (k.ny % 2 == 0 && k.ny >= 4)
-        hipLaunchKernelGGL(k_qx_tail2, cell_grid(k.ny / 2, k.nx), dim3(256), 0, st, k, m, s, w, f0, (int)write_uc);
+        hipLaunchKernelGGL(k_qx_tail2r, cell_grid(k.ny / 2, (k.nx + QXT_ROWS - 1) / QXT_ROWS), dim3(256), 0, st, k, m,
+                           s, w, f0, (int)write_uc);
     else
         hipLaunchKernelGGL(k_qx_tail, cell_grid(k.ny, k.nx), dim3(256), 0, st, k, m, s, w, f0, (int)write_uc);
     return hipGetLastError();
