@@ -64,6 +64,34 @@ __host__ __device__ __forceinline__ void acc(double *p, double v)
 #endif
 }
 
+// own at column j, left at j-1 and right at j+1 (periodic) of one row. In a kernel whose
+// wavefronts are 64 consecutive columns of a row (one lane per column), the parts that
+// lanes l-1 and l+1 send to column j join lane l's own part through lane shuffles, so one
+// atomic adds all three. A lane adds a part itself where the receiving lane is not in its
+// wavefront: lane 0 or column 0 (left), lane 63 or column ny-1 (right). Must be reached
+// by every lane of the wavefront that has a column; a lane's parts may be zero.
+__host__ __device__ __forceinline__ void scatter_row(double *row, int j, int ny, double own, double left,
+                                                     double right)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const int lane = (int)(threadIdx.x & 63);
+    const double from_r = __shfl_down(left, 1);   // lane l+1's part for column j
+    const double from_l = __shfl_up(right, 1);    // lane l-1's part for column j
+    const bool has_r = lane != 63 && j + 1 < ny;
+    const bool has_l = lane != 0 && j > 0;
+    double v = own;
+    if (has_r) v += from_r;
+    if (has_l) v += from_l;
+    acc(row + j, v);
+    if (!has_l) acc(row + wrap_dec(j, ny), left);
+    if (!has_r) acc(row + wrap_inc(j, ny), right);
+#else
+    acc(row + j, own);
+    acc(row + wrap_dec(j, ny), left);
+    acc(row + wrap_inc(j, ny), right);
+#endif
+}
+
 // Reverse of drag_term (src/HydroParametrizations.cpp:19-28):
 // drag = cd*urms/sqrt2*sqrt(af^2 + 2(v^2+u^2)/urms^2)*inv_d; g = adjoint of drag.
 __host__ __device__ __forceinline__ void drag_term_rev(double g, double cd, double urms, double u, double v,
@@ -176,6 +204,8 @@ __host__ __device__ __forceinline__ void adj_qyx(const Consts &k, const Metrics 
     const int jp = wrap_inc(j, ny);
     const double dt = k.dt;
     const CoefBar cb = coef_bar_cols(W.lam, S1.Qy + ny, i, j, nx - 1, ny);
+    // scatter values, applied at the end through scatter_row (zero where a branch is not taken)
+    double s_cu = 0, s_cd = 0, s_da = 0, s_du = 0, s_dl = 0;
     double lv_b = cb.b, uv_b = cb.c;
     const double dv_b = cb.a;
     if (i == 0) lv_b += k.bcQy * cb.a;
@@ -199,14 +229,10 @@ __host__ __device__ __forceinline__ void adj_qyx(const Consts &k, const Metrics 
         dnud_b -= dv_b * dt * inv_d * dx_d;
         const double cu = 0.5 * x2c_up * uyl_b + 0.5 * inv_dx * dudx_b;
         const double cd = 0.5 * x2c_d * uyl_b - 0.5 * inv_dx * dudx_b;
-        double *UN = W.UN + r;
-        acc(&UN[ny + jp], cu);
-        acc(&UN[ny + j], cu);
-        acc(&UN[jp], cd);
-        acc(&UN[j], cd);
+        s_cu = cu;
+        s_cd = cd;
         const double da_b = -invd_b * inv_d * inv_d;
-        acc(&W.gD[r + jp], 0.5 * da_b);
-        acc(&W.gD[r + j], 0.5 * da_b);
+        s_da = 0.5 * da_b;
     }
     {   // up: inv_Di row i+2; IsWater row nx-1 is false
         const int wu = (i + 1 <= nx - 2) ? T.wy[r + ny + j] : 0;
@@ -219,8 +245,7 @@ __host__ __device__ __forceinline__ void adj_qyx(const Consts &k, const Metrics 
             dnud_b -= e * dx_up * inv_up;
             const double iu_b = -e * (nud * d2x_up + dnud_dx * dx_up);
             const double du_b = -iu_b * inv_up * inv_up;
-            acc(&W.gD[r + ny + jp], 0.5 * du_b);
-            acc(&W.gD[r + ny + j], 0.5 * du_b);
+            s_du = 0.5 * du_b;
         }
     }
     {   // low
@@ -256,17 +281,20 @@ __host__ __device__ __forceinline__ void adj_qyx(const Consts &k, const Metrics 
                     acc(&W.gD[2 * ny - 1], -0.5 * dl_b);
                 }
             } else {
-                acc(&W.gD[r - ny + jp], 0.5 * dl_b);
-                acc(&W.gD[r - ny + j], 0.5 * dl_b);
+                s_dl = 0.5 * dl_b;
             }
         }
     }
-    acc(&W.UcN[r + jp], 0.5 * uy_b);
-    acc(&W.UcN[r + j], 0.5 * uy_b);
     W.rhs[r + j] = qb;
     W.dragy[r + j] = drag_b;
     R.nuD_y[r + j] += nud_b;
     R.dnuDy_dx[r + j] += dnud_b;
+    scatter_row(W.UN + r + ny, j, ny, s_cu, 0.0, s_cu);
+    scatter_row(W.UN + r, j, ny, s_cd, 0.0, s_cd);
+    scatter_row(W.gD + r, j, ny, s_da, 0.0, s_da);
+    if (i + 1 <= nx - 2) scatter_row(W.gD + r + ny, j, ny, s_du, 0.0, s_du);
+    if (i > 0) scatter_row(W.gD + r - ny, j, ny, s_dl, 0.0, s_dl);
+    scatter_row(W.UcN + r, j, ny, 0.5 * uy_b, 0.0, 0.5 * uy_b);
 }
 
 // ----------------------------------------------------------------------------------
@@ -293,6 +321,8 @@ __host__ __device__ __forceinline__ void adj_qyy(const Consts &k, const Metrics 
     const double dnud_dy = rf.dnuD_dy[r + j];
     double Dy_b = 0, invDy_b = 0, invm_b = 0, invp_b = 0, nud_b = 0, dnud_b = 0;
     double drag_b = W.dragy[r + j];
+    // scatter values, applied after the last load (an atomic ahead of a load holds the load)
+    double s_qy = 0, s_dq = 0, s_dz = 0, r_dy = 0, s_u1p = 0, s_u1j = 0, s_u0p = 0, s_u0j = 0, s_vm = 0, s_vp = 0;
     if (w0) {
         const double g = cb.f;
         const double inv_dx = m.inv_dxx[i];
@@ -302,24 +332,19 @@ __host__ __device__ __forceinline__ void adj_qyy(const Consts &k, const Metrics 
         const double d2qx_dxy = k.inv_dy * (dqx[jp] - dqx[j]);
         const double du_dy = k.inv_dy * (x2c_up * (u[ny + jp] - u[ny + j]) + x2c_d * (u[jp] - u[j]));
         const double d2u_dxy = inv_dx * k.inv_dy * ((u[ny + jp] - u[ny + j]) - (u[jp] - u[j]));
-        acc(&A.Qy[r + ny + j], g * k.inv_dt);
+        s_qy = g * k.inv_dt;
         Dy_b += g * G * (dt * d2qx_dxy - dzs_dy);
         const double t = g * G * Dy;
-        const double dq_b = t * dt * k.inv_dy;
-        acc(&W.dQxN[r + jp], dq_b);
-        acc(&W.dQxN[r + j], -dq_b);
-        const double dz_b = -t * k.inv_dy;
-        acc(&A.Zs[r + jp], dz_b);
-        acc(&A.Zs[r + j], -dz_b);
-        R.dnuDy_dx[r + j] += g * du_dy;
+        s_dq = t * dt * k.inv_dy;
+        s_dz = -t * k.inv_dy;
+        r_dy = g * du_dy;
         nud_b += g * d2u_dxy;
         const double du_b = g * rf.dnuDy_dx[r + j] * k.inv_dy;
         const double c2 = g * nud_y * inv_dx * k.inv_dy;
-        double *UN = W.UN + r;
-        acc(&UN[ny + jp], x2c_up * du_b + c2);
-        acc(&UN[ny + j], -x2c_up * du_b - c2);
-        acc(&UN[jp], x2c_d * du_b - c2);
-        acc(&UN[j], -x2c_d * du_b + c2);
+        s_u1p = x2c_up * du_b + c2;
+        s_u1j = -x2c_up * du_b - c2;
+        s_u0p = x2c_d * du_b - c2;
+        s_u0j = -x2c_d * du_b + c2;
         const double h = cb.a;
         drag_b += h;
         Dy_b += h * G * dt * k.two_inv_dy2;
@@ -329,7 +354,7 @@ __host__ __device__ __forceinline__ void adj_qyy(const Consts &k, const Metrics 
     const double *v = S0.V + r + ny;
     if (wy_j && wy_m && cb.b != 0.0) {
         const double e = cb.b;
-        acc(&A.V[r + ny + jm], -0.5 * k.inv_dy * e);
+        s_vm = -0.5 * k.inv_dy * e;
         dnud_b += e * k.inv_dy * invDy_m;
         invm_b += e * (k.inv_dy * dnud_dy - k.inv_dy2_yy * 2 * nud_y);
         Dy_b -= e * k.inv_dy2_yy * G * dt;
@@ -337,7 +362,7 @@ __host__ __device__ __forceinline__ void adj_qyy(const Consts &k, const Metrics 
     }
     if (wy_j && wy_p && cb.c != 0.0) {
         const double e = cb.c;
-        acc(&A.V[r + ny + jp], 0.5 * k.inv_dy * e);
+        s_vp = 0.5 * k.inv_dy * e;
         dnud_b -= e * k.inv_dy * invDy_p;
         invp_b += e * (-k.inv_dy * dnud_dy - k.inv_dy2_yy * 2 * nud_y);
         Dy_b -= e * k.inv_dy2_yy * G * dt;
@@ -347,20 +372,23 @@ __host__ __device__ __forceinline__ void adj_qyy(const Consts &k, const Metrics 
         const double uy = 0.5 * (S1.Uc[r + jp] + S1.Uc[r + j]);
         double cd_b = 0, uy_b = 0, v_b = 0;
         drag_term_rev(drag_b, rf.cD_y[r + j], rf.Urms_y[r + j], uy, v[j], invDy, k, cd_b, uy_b, v_b, invDy_b);
+        Dy_b -= invDy_b * invDy * invDy;
+        const double m_b = -invm_b * invDy_m * invDy_m;
+        const double p_b = -invp_b * invDy_p * invDy_p;
+        if (w0) R.dnuDy_dx[r + j] += r_dy;
         R.cD_y[r + j] += cd_b;
-        acc(&W.UcN[r + jp], 0.5 * uy_b);
-        acc(&W.UcN[r + j], 0.5 * uy_b);
-        acc(&A.V[r + ny + j], v_b);
+        R.nuD_y[r + j] += nud_b;
+        R.dnuD_dy[r + j] += dnud_b;
+        acc(&A.Qy[r + ny + j], s_qy);
+        scatter_row(W.dQxN + r, j, ny, -s_dq, 0.0, s_dq);
+        scatter_row(A.Zs + r, j, ny, -s_dz, 0.0, s_dz);
+        scatter_row(W.UN + r + ny, j, ny, s_u1j, 0.0, s_u1p);
+        scatter_row(W.UN + r, j, ny, s_u0j, 0.0, s_u0p);
+        scatter_row(A.V + r + ny, j, ny, v_b, s_vm, s_vp);
+        scatter_row(W.UcN + r, j, ny, 0.5 * uy_b, 0.0, 0.5 * uy_b);
+        scatter_row(W.gD + r, j, ny, 0.5 * Dy_b + 0.5 * m_b, 0.5 * m_b, 0.5 * Dy_b + 0.5 * p_b);
+        acc(&W.gD[r + jpp], 0.5 * p_b);
     }
-    Dy_b -= invDy_b * invDy * invDy;
-    const double m_b = -invm_b * invDy_m * invDy_m;
-    const double p_b = -invp_b * invDy_p * invDy_p;
-    acc(&W.gD[r + jp], 0.5 * Dy_b + 0.5 * p_b);
-    acc(&W.gD[r + j], 0.5 * Dy_b + 0.5 * m_b);
-    acc(&W.gD[r + jm], 0.5 * m_b);
-    acc(&W.gD[r + jpp], 0.5 * p_b);
-    R.nuD_y[r + j] += nud_b;
-    R.dnuD_dy[r + j] += dnud_b;
 }
 
 // ----------------------------------------------------------------------------------
@@ -476,16 +504,10 @@ __host__ __device__ __forceinline__ void adj_qxy(const Consts &k, const State &S
         }
     }
     const double dj = -0.5 * ij_b * inv_j * inv_j, dp = -0.5 * ip_b * inv_p * inv_p, dm = -0.5 * im_b * inv_m * inv_m;
-    acc(&A.D[r + j], dj);
-    acc(&A.D[r + ny + j], dj);
-    acc(&A.D[r + jp], dp);
-    acc(&A.D[r + ny + jp], dp);
-    acc(&A.D[r + jm], dm);
-    acc(&A.D[r + ny + jm], dm);
-    acc(&A.Vc[r + jp], 0.5 * vp_b);
-    acc(&A.Vc[r + ny + jp], 0.5 * vp_b);
-    acc(&A.Vc[r + jm], 0.5 * vm_b);
-    acc(&A.Vc[r + ny + jm], 0.5 * vm_b);
+    scatter_row(A.D + r, j, ny, dj, dm, dp);
+    scatter_row(A.D + r + ny, j, ny, dj, dm, dp);
+    scatter_row(A.Vc + r, j, ny, 0.0, 0.5 * vm_b, 0.5 * vp_b);
+    scatter_row(A.Vc + r + ny, j, ny, 0.0, 0.5 * vm_b, 0.5 * vp_b);
     W.rhs[r + j] = qb;
     W.dragx[r + j] = drag_b;
     R.nuD_x[r + j] += nud_b;
@@ -516,6 +538,8 @@ __host__ __device__ __forceinline__ void adj_qxx(const Consts &k, const Metrics 
     if (i == 0) lv_b += k.f1 * cb.a - f0 * cb.f;   // setSolverQxx_BC
     double Dx1_b = 0, invDx1_b = 0, u1_b = 0, dnud_b = 0, nud_b = 0, drag_b = W.dragx[r + j];
     double dD_m = 0, dD_0 = 0, dD_1 = 0, dD_2 = 0;
+    // scatter values, applied after the last load (an atomic ahead of a load holds the load)
+    double s_qx = 0, s_dq = 0, s_dz = 0, r_dy = 0, s_v1j = 0, s_v1m = 0, s_v0j = 0, s_v0m = 0, s_ul = 0, s_uu = 0;
     if (wi) {
         const double g = cb.f;
         const double inv_dx = m.inv_dx[i];
@@ -524,24 +548,19 @@ __host__ __device__ __forceinline__ void adj_qxx(const Consts &k, const Metrics 
         const double d2qy_dxy = inv_dx * (dqy[ny + j] - dqy[j]);
         const double dV_dx = 0.5 * inv_dx * ((v[ny + j] + v[ny + jm]) - (v[j] + v[jm]));
         const double d2V_dxy = inv_dx * k.inv_dy * ((v[ny + j] - v[ny + jm]) - (v[j] - v[jm]));
-        acc(&A.Qx[r + ny + j], g * k.inv_dt);
+        s_qx = g * k.inv_dt;
         Dx1_b += g * G * (dt * d2qy_dxy - dzs_dx);
         const double t = g * G * Dx1;
-        const double dq_b = t * dt * inv_dx;
-        acc(&A.dQy[r + ny + j], dq_b);
-        acc(&A.dQy[r + j], -dq_b);
-        const double dz_b = -t * inv_dx;
-        acc(&A.Zs[r + ny + j], dz_b);
-        acc(&A.Zs[r + j], -dz_b);
-        R.dnuDx_dy[r + j] += g * dV_dx;
+        s_dq = t * dt * inv_dx;
+        s_dz = -t * inv_dx;
+        r_dy = g * dV_dx;
         nud_b += g * d2V_dxy;
         const double dv_b = g * rf.dnuDx_dy[r + j] * 0.5 * inv_dx;
         const double c2 = g * nud_x * inv_dx * k.inv_dy;
-        double *AV = A.V + r + ny;
-        acc(&AV[ny + j], dv_b + c2);
-        acc(&AV[ny + jm], dv_b - c2);
-        acc(&AV[j], -dv_b - c2);
-        acc(&AV[jm], -dv_b + c2);
+        s_v1j = dv_b + c2;
+        s_v1m = dv_b - c2;
+        s_v0j = -dv_b - c2;
+        s_v0m = -dv_b + c2;
         const double h = cb.a;
         drag_b += h;
         u1_b += h * m.difx_x_d[i];
@@ -554,7 +573,7 @@ __host__ __device__ __forceinline__ void adj_qxx(const Consts &k, const Metrics 
     if (i == 0) {   // row-0 low is unmasked (src/HydroQx.cpp:214-229)
         const double e = lv_b;
         const double Dx0 = 1.5 * Di - 0.5 * Di1, invDx0 = 1.0 / Dx0;
-        acc(&A.U[j], e * m.difx_x_low[0]);
+        s_ul = e * m.difx_x_low[0];   // A.U row 0
         dnud_b += e * m.difx_x_low[0] * (-2 * invDx0);
         const double i0_b = e * m.difx_x_low[0] * (-2 * dnud_dx) - e * m.dif2x_x_low[0] * 2 * nud_x;
         Dx1_b -= e * m.dif2x_x_low[0] * G * dt;
@@ -565,7 +584,7 @@ __host__ __device__ __forceinline__ void adj_qxx(const Consts &k, const Metrics 
     } else if (wi && T.wx[r - ny + j]) {
         const double e = lv_b;
         const double Dxi = 0.5 * (D[r - ny + j] + Di), invDxi = 1 / Dxi;
-        acc(&A.U[r + j], e * m.difx_x_low[i]);
+        s_ul = e * m.difx_x_low[i];
         dnud_b += e * m.difx_x_low[i] * (-2 * invDxi);
         const double ii_b = e * m.difx_x_low[i] * (-2 * dnud_dx) - e * m.dif2x_x_low[i] * 2 * nud_x;
         Dx1_b -= e * m.dif2x_x_low[i] * G * dt;
@@ -577,7 +596,7 @@ __host__ __device__ __forceinline__ void adj_qxx(const Consts &k, const Metrics 
     if (i < nx - 2 && wi && T.wx[r + ny + j]) {
         const double e = cb.c;
         const double Dx2 = 0.5 * (Di1 + D[r + 2L * ny + j]), invDx2 = 1 / Dx2;
-        acc(&A.U[r + 2L * ny + j], e * m.difx_x_up[i]);
+        s_uu = e * m.difx_x_up[i];
         dnud_b += e * m.difx_x_up[i] * (-2 * invDx2);
         const double i2_b = e * m.difx_x_up[i] * (-2 * dnud_dx) - e * m.dif2x_x_up[i] * 2 * nud_x;
         Dx1_b -= e * m.dif2x_x_up[i] * G * dt;
@@ -586,23 +605,31 @@ __host__ __device__ __forceinline__ void adj_qxx(const Consts &k, const Metrics 
         dD_1 += 0.5 * d2_b;
         dD_2 += 0.5 * d2_b;
     }
-    {
-        double cd_b = 0, vx_b = 0;
-        drag_term_rev(drag_b, rf.cD_x[r + j], rf.Urms_x[r + j], u1, vx, invDx1, k, cd_b, u1_b, vx_b, invDx1_b);
-        R.cD_x[r + j] += cd_b;
-        acc(&A.Vc[r + j], 0.5 * vx_b);
-        acc(&A.Vc[r + ny + j], 0.5 * vx_b);
-    }
-    acc(&A.U[r + ny + j], u1_b);
+    double cd_b = 0, vx_b = 0;
+    drag_term_rev(drag_b, rf.cD_x[r + j], rf.Urms_x[r + j], u1, vx, invDx1, k, cd_b, u1_b, vx_b, invDx1_b);
     Dx1_b -= invDx1_b * invDx1 * invDx1;
     dD_0 += 0.5 * Dx1_b;
     dD_1 += 0.5 * Dx1_b;
+    if (wi) R.dnuDx_dy[r + j] += r_dy;
+    R.cD_x[r + j] += cd_b;
+    R.dnuD_dx[r + j] += dnud_b;
+    R.nuD_x[r + j] += nud_b;
+    acc(&A.Qx[r + ny + j], s_qx);
+    acc(&A.dQy[r + ny + j], s_dq);
+    acc(&A.dQy[r + j], -s_dq);
+    acc(&A.Zs[r + ny + j], s_dz);
+    acc(&A.Zs[r + j], -s_dz);
+    scatter_row(A.V + r + 2L * ny, j, ny, s_v1j, s_v1m, 0.0);
+    scatter_row(A.V + r + ny, j, ny, s_v0j, s_v0m, 0.0);
+    acc(&A.U[r + j], s_ul);
+    if (i < nx - 2) acc(&A.U[r + 2L * ny + j], s_uu);
+    acc(&A.Vc[r + j], 0.5 * vx_b);
+    acc(&A.Vc[r + ny + j], 0.5 * vx_b);
+    acc(&A.U[r + ny + j], u1_b);
     if (i > 0) acc(&A.D[r - ny + j], dD_m);
     acc(&A.D[r + j], dD_0);
     acc(&A.D[r + ny + j], dD_1);
     if (i < nx - 2) acc(&A.D[r + 2L * ny + j], dD_2);
-    R.dnuD_dx[r + j] += dnud_b;
-    R.nuD_x[r + j] += nud_b;
 }
 
 // ----------------------------------------------------------------------------------

@@ -488,15 +488,24 @@ __device__ __forceinline__ void qx_tail2_body(const Consts &k, const Metrics &m,
     qx_tail2_rows(k, m, s, w, f0, i, i + 1, j0, write_uc);
 }
 
-// QXT_ROWS rows per thread, two columns (ny even): grid (ceil(ny/512), ceil(nx/QXT_ROWS)).
-constexpr int QXT_ROWS = 8;
-__global__ void __launch_bounds__(256) k_qx_tail2r(Consts k, Metrics m, State s, Work w, double f0, int write_uc)
+// `rows` rows per thread, two columns (ny even): grid (ceil(ny/512), ceil(nx/rows)).
+__global__ void __launch_bounds__(256) k_qx_tail2r(Consts k, Metrics m, State s, Work w, double f0, int write_uc,
+                                                   int rows)
 {
     const int j0 = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int i0 = (int)blockIdx.y * QXT_ROWS;
+    const int i0 = (int)blockIdx.y * rows;
     if (j0 >= k.ny || i0 >= k.nx) return;
-    const int i1 = i0 + QXT_ROWS < k.nx ? i0 + QXT_ROWS : k.nx;
+    const int i1 = i0 + rows < k.nx ? i0 + rows : k.nx;
     qx_tail2_rows(k, m, s, w, f0, i0, i1, j0, write_uc != 0);
+}
+
+// Rows per thread of k_qx_tail2r: up to 8, while the launch keeps at least 2048 workgroups
+// (eight per CU); small grids take one row per thread.
+static int qx_tail_rows(const Consts &k)
+{
+    const long bx = (k.ny / 2 + 255) / 256;
+    long r = (long)k.nx * bx / 2048;
+    return r < 1 ? 1 : (r > 8 ? 8 : (int)r);
 }
 
 
@@ -610,11 +619,13 @@ hipError_t launch_qxy_assemble(hipStream_t st, const Consts &k, const State &s, 
 hipError_t launch_qx_tail(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const Work &w,
                           double f0, bool write_uc)
 {
-    if (k.ny % 2 == 0 && k.ny >= 4)
-        hipLaunchKernelGGL(k_qx_tail2r, cell_grid(k.ny / 2, (k.nx + QXT_ROWS - 1) / QXT_ROWS), dim3(256), 0, st, k, m,
-                           s, w, f0, (int)write_uc);
-    else
+    if (k.ny % 2 == 0 && k.ny >= 4) {
+        const int rows = qx_tail_rows(k);
+        hipLaunchKernelGGL(k_qx_tail2r, cell_grid(k.ny / 2, (k.nx + rows - 1) / rows), dim3(256), 0, st, k, m, s, w, f0,
+                           (int)write_uc, rows);
+    } else {
         hipLaunchKernelGGL(k_qx_tail, cell_grid(k.ny, k.nx), dim3(256), 0, st, k, m, s, w, f0, (int)write_uc);
+    }
     return hipGetLastError();
 }
 hipError_t launch_qyy_assemble(hipStream_t st, const Consts &k, const Metrics &m, const State &s,
