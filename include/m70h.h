@@ -173,15 +173,17 @@ int m70h_tridiag_solve(void *hip_stream, int solver, int lines_are_rows, int per
  * M70H_ENONFINITE when a non-finite value is found. */
 int m70h_check_finite(m70h *h, int *finite);
 
-/* Per-kernel timing with HIP events on the handle's stream. When enabled, run records an
- * event pair around every launch; m70h_profile reports kernel k's name, summed
- * milliseconds and launch count (returns M70H_EINVAL past the last kernel). */
+/* Per-kernel timing with HIP events on the handle's stream (no counterpart in the
+ * reference; measurement only). When enabled, run records an event pair around every
+ * launch; m70h_profile reports kernel k's name, summed milliseconds and launch count
+ * (returns M70H_EINVAL past the last kernel). */
 int m70h_set_profiling(m70h *h, int enable);
-/* Profiling of the launches of one kernel only (its name as m70h_profile reports it;
- * NULL for every launch): two events per launch of that kernel instead of two per launch,
- * so the timing of one kernel disturbs the step less. M70H_EINVAL for an unknown name. */
-int m70h_set_profiling_only(m70h *h, const char *kernel);
 int m70h_profile(m70h *h, int k, const char **name, double *ms, long *launches);
+/* Profiling of the launches of one kernel only (its name as m70h_profile reports it;
+ * NULL for every launch): an event pair around each launch of that kernel alone, so
+ * timing one kernel inside a timed region disturbs the step less. M70H_EINVAL for an
+ * unknown name. */
+int m70h_set_profiling_only(m70h *h, const char *kernel);
 
 /* Device pointer of the current state field `name` ("Zs","D","Qx","Qy","U","V","U_c",
  * "V_c","dQx_dx","dQy_dy"), valid until the next run. */
