@@ -3,6 +3,8 @@
 * EXACT and PARTITION solver families agree to the 1e-10 bar after 100 steps (EXACT is
   the reference's own recurrences and is pinned bit for bit to the oracle at small sizes
   in test_gpu_parity);
+* FUSED against the oracle over a few steps (the oracle's OpenMP build takes seconds for
+  them);
 * mass budget: sum(dx_x * dy * dZs) equals dt*dy*(sum of offshore inflow Qx[0] minus the
   periodic-free lateral terms, which cancel) step by step, i.e. the step conserves
   volume up to the boundary flux (src/Hydro.cpp:253-271 with src/HydroMeshOperations.cpp
@@ -56,6 +58,25 @@ def test_fused_vs_exact_full_size(c2):
     print({k: v["rel"] for k, v in res.items()}, flush=True)
     for k, r in res.items():
         assert r["nan"] == 0 and r["rel"] <= 1e-10, (k, r)
+
+
+def test_fused_vs_oracle_full_size(c2):
+    """FUSED against the oracle itself at full size, over two Runs of three steps: the paths
+    only this size takes (the Qx tail's multi-row runs, the derived V_c of a Run's later
+    steps, the constant-drag sweeps, 512-thread row workgroups) at the 1e-10 bar."""
+    import oracle as orc
+    from morfo70_amd.hydro import from_scenario
+    print("[fullsize] 2048x4096 tide, 2 x Run(3), fused vs oracle", flush=True)
+    o = orc.OracleHydro(c2, threads=16)
+    g = from_scenario(c2, solver=_lib.SOLVER_FUSED)
+    for _ in range(2):
+        o.run(3)
+        g.Run(3)
+    res = compare_states(g.state(), o.get_state())
+    g.close()
+    print({k: v["rel"] for k, v in res.items()}, flush=True)
+    for k, r in res.items():
+        assert r["nan"] == r["ref_nan"] == 0 and r["rel"] <= 1e-10, (k, r)
 
 
 def test_mass_budget_full_size(c2):
