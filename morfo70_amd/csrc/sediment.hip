@@ -98,7 +98,8 @@ __global__ void __launch_bounds__(256) k_sed_coeffs(SedArgs a)
     const int i = blockIdx.y;
     if (j >= a.ny || i >= a.nx) return;
     const long q = (long)i * a.ny + j;
-    const double d = a.D[q];
+    // every input is loaded before the first store, which would otherwise hold later loads
+    const double d = a.D[q], zb = a.Zb[q], zs_sh = a.Zs_sh[j], max_dw = a.maxDw[j];
     {   // getWaveStirringCWS
         const double alpha0_min = a.p.alpha0_min;
         const double incrAlpha = a.p.alpha0_max - alpha0_min;
@@ -122,9 +123,9 @@ __global__ void __launch_bounds__(256) k_sed_coeffs(SedArgs a)
         const double inv_rho_g_sm1 = 1 / (G * RHO * (S_REL - 1));
         const double _2_Doff_on = 2 / (A_swash - Dmin);
         const int isDry = (d <= Dmin);
-        const double d_dry = (1 - isDry) * d + isDry * (fabs(a.Zb[q] + a.Zs_sh[j]));
+        const double d_dry = (1 - isDry) * d + isDry * (fabs(zb + zs_sh));
         const double sf_swash = 1 - one_plus_tanh(PI_C * (_2_Doff_on * (d_dry - Dmin) - 1)) * 0.5;
-        a.Gamma[q] = G0 * inv_rho_g_sm1 * sf_swash * a.maxDw[j];
+        a.Gamma[q] = G0 * inv_rho_g_sm1 * sf_swash * max_dw;
     }
 }
 
@@ -135,22 +136,29 @@ __global__ void __launch_bounds__(256) k_sed_fluxes(SedArgs a)
     const int ny = a.ny;
     if (j >= ny || i >= a.nx) return;
     const long q = (long)i * ny + j;
-    if (i < a.nx - 1) {   // getqx: rows 0..nx-2, U row i+1
+    // both fluxes are formed before either is stored: a store ahead of the qy loads would
+    // hold them behind it
+    const bool has_qx = i < a.nx - 1;
+    double qx = 0.0;
+    if (has_qx) {   // getqx: rows 0..nx-2, U row i+1
         const double inv_dx = a.inv_dx[i];
         double A_x = 0.5 * (a.Alpha[q] + a.Alpha[q + ny]);
         double g_x = 0.5 * (a.gamma[q] + a.gamma[q + ny]);
         double G_x = 0.5 * (a.Gamma[q] + a.Gamma[q + ny]);
         double dh_dx = (a.h[q + ny] - a.h[q]) * inv_dx;
-        a.qx[q] = A_x * (a.U[q + ny] - g_x * dh_dx) - G_x * dh_dx;
+        qx = A_x * (a.U[q + ny] - g_x * dh_dx) - G_x * dh_dx;
     }
+    double qy;
     {   // getqy: rows 0..nx-1, V row i+1, periodic in j
         const long qp = (long)i * ny + wrap_inc(j, ny);
         double A_y = 0.5 * (a.Alpha[q] + a.Alpha[qp]);
         double g_y = 0.5 * (a.gamma[q] + a.gamma[qp]);
         double G_y = 0.5 * (a.Gamma[q] + a.Gamma[qp]);
         double dh_dy = (a.h[qp] - a.h[q]) * a.inv_dy;
-        a.qy[q] = A_y * (a.V[q + ny] - g_y * dh_dy) - G_y * dh_dy;
+        qy = A_y * (a.V[q + ny] - g_y * dh_dy) - G_y * dh_dy;
     }
+    if (has_qx) a.qx[q] = qx;
+    a.qy[q] = qy;
 }
 
 __global__ void __launch_bounds__(256) k_sed_update(SedArgs a, double dt_2, double _1_m1p)
