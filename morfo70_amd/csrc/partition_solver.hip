@@ -774,8 +774,7 @@ __device__ __forceinline__ void cols_backsub_body(const ColsWork &cw, double *__
                                                   int by)
 {
     const int j = bx * 64 + threadIdx.x;
-    // a wavefront is one threadIdx.y of the 64x4 workgroup: its segment, and so its rows,
-    // are uniform, which keeps the per-row metrics in scalar loads
+    // a wavefront is one threadIdx.y of the 64x4 workgroup: its segment bounds are scalars
     const int sg = __builtin_amdgcn_readfirstlane(by * 4 + (int)threadIdx.y);
     const int nseg = cw.nseg;
     if (j >= ny || sg >= nseg) return;
