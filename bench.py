@@ -66,6 +66,20 @@ KERNEL_BYTES_PER_CELL = {
     "cols_qy_reduced": 10,
 }
 
+# Algorithmic bytes per flux face of each of the step's four line-solve sweeps: the
+# compulsory FP64 field accesses of the sweep (S1 Qx predictor: reads Zs, D, Qx, Qy, U,
+# Sxx, Sxy, Urms, cD, nuD, writes Qx*; S2 Qx projector: reads Qx*, Zs, D, Qx, Qy, U, Urms,
+# cD, nuD, writes Qx, U; S3 Qy predictor as S1 along y; S4 Qy projector: reads 9, writes
+# Qy, Zs, D), 45 accesses = 360 B per cell-step in all. The kernel that carries a sweep's
+# line solve is priced at its sweep's figure: the dominant kernel's roofline fraction is
+# then sweep bytes / launch time.
+SWEEP_BYTES_PER_CELL = {
+    "cols_qx_elim": 88.0, "qxx_assemble": 88.0,       # S1
+    "fused_qxy": 88.0, "qxy_assemble": 88.0,          # S2
+    "fused_qyy": 88.0, "qyy_assemble": 88.0,          # S3
+    "cols_qy_elim": 96.0, "qyx_assemble": 96.0,       # S4
+}
+
 # Sediment transport and bed update (src/Sediment.cpp), per cell of the nx x ny grid
 # (the bed update covers rows 1..nx-1): compulsory FP64 fields read once, written once.
 SED_BYTES_PER_CELL = {
@@ -131,21 +145,46 @@ def measured_traffic(kernel: str, nx: int, ny: int, solver: str, workload: str =
     return None if k is None else k["hbm_bytes"]
 
 
+def host_threads(requested: int):
+    """OpenMP threads for a CPU leg: every host core this process may use (affinity mask,
+    bounded by a cgroup quota), close binding, one thread per core; and the host record."""
+    import oracle
+    oracle.set_omp_binding()
+    info = oracle.host_cpus()
+    return (requested or info["threads"]), info
+
+
 def cpu_baseline(sc, steps: int, threads: int) -> dict:
     """The oracle (CPU restatement of the reference, OpenMP over independent lines) on
-    the host cores, on a bounded sample of the same workload."""
+    every host core, on a bounded sample of the same workload: the median of three
+    Run(steps) calls, each after one untimed step; beside it the 1-thread rate from one
+    shorter Run."""
     import oracle
-    threads = threads or min(16, os.cpu_count() or 1)
+    threads, info = host_threads(threads)
     o = oracle.OracleHydro(sc, threads=threads)
     o.run(1)                               # untimed: per-Run setup + one step
-    t0 = time.perf_counter()
-    o.run(steps)
-    dt = time.perf_counter() - t0
-    del o
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        o.run(steps)
+        times.append(time.perf_counter() - t0)
+    dt = sorted(times)[1]
     rate = sc.nx * sc.ny * steps / dt
+    serial_steps = max(1, steps // 20)
+    oracle.lib().orc_set_threads(1)
+    t0 = time.perf_counter()
+    o.run(serial_steps)
+    sdt = time.perf_counter() - t0
+    oracle.lib().orc_set_threads(threads)
+    del o
     return {"value": rate, "unit": "cell-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{sc.nx}x{sc.ny} grid, one Run({steps}) after one warm step, "
-                      f"{threads} OpenMP threads, {dt:.2f} s"}
+            "sample": f"{sc.nx}x{sc.ny} grid, median of 3 x Run({steps}) after one warm step, "
+                      f"{threads} OpenMP threads (OMP_PROC_BIND=close, OMP_PLACES=cores), "
+                      f"{', '.join(f'{t:.2f}' for t in times)} s",
+            "cpu_model": info["model"], "affinity_cores": info["affinity"],
+            "cgroup_quota_cpus": info["cgroup_quota"],
+            "serial": {"value": sc.nx * sc.ny * serial_steps / sdt, "threads": 1,
+                       "sample": f"one Run({serial_steps}), {sdt:.2f} s"}}
 
 
 def bench_sediment(args):
@@ -221,7 +260,7 @@ def bench_sediment(args):
     }
     if not args.no_cpu:
         import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads, info = host_threads(args.cpu_threads)
         n = args.cpu_steps or 150
         o = oracle.OracleHydro(sc, threads=threads)
         o.run(1)
@@ -236,7 +275,8 @@ def bench_sediment(args):
         del osd, o
         rec["cpu_baseline"] = {"value": cells * n / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
                                "sample": f"{sc.nx}x{sc.ny} grid, {n} x (fluxes + bed update) after one untimed, "
-                                         f"{threads} OpenMP threads, {dt:.2f} s"}
+                                         f"{threads} OpenMP threads, {dt:.2f} s",
+                               "cpu_model": info["model"], "affinity_cores": info["affinity"]}
         rec["gpu_over_cpu"] = value / rec["cpu_baseline"]["value"]
     else:
         rec["cpu_baseline"] = None
@@ -307,7 +347,7 @@ def bench_wave(args):
     }
     if not args.no_cpu:
         import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads, info = host_threads(args.cpu_threads)
         oracle.lib().orc_set_threads(threads)
         o = oracle.OracleWave(sc.x, sc.ny, sc.dy, p)
         o.set_depth(D)
@@ -319,7 +359,8 @@ def bench_wave(args):
         del o
         rec["cpu_baseline"] = {"value": cells / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
                                "sample": f"{sc.nx}x{sc.ny} field, {n} fields, OpenMP over the nodes of a row "
-                                         f"(up to {threads} threads), {dt:.2f} s per field"}
+                                         f"(up to {threads} threads), {dt:.2f} s per field",
+                               "cpu_model": info["model"], "affinity_cores": info["affinity"]}
         rec["gpu_over_cpu"] = value / rec["cpu_baseline"]["value"]
     else:
         rec["cpu_baseline"] = None
@@ -378,7 +419,7 @@ def bench_morpho(args):
     }
     if not args.no_cpu:
         import oracle
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads, info = host_threads(args.cpu_threads)
         o = oracle.OracleHydro(sc, threads=threads)
         ow = oracle.OracleWave(sc.x, sc.ny, sc.dy, wp)
         osd = oracle.OracleSediment(o, sp)
@@ -394,7 +435,8 @@ def bench_morpho(args):
         cdt = (time.perf_counter() - t1) / k
         rec["cpu_baseline"] = {"value": 1.0 / cdt, "unit": "outer-steps/s", "cores": threads, "kind": "port",
                                "sample": f"{sc.nx}x{sc.ny}, {k} outer steps of the oracle's wave, hydro and "
-                                         f"sediment ({threads} OpenMP threads), {cdt:.2f} s per step"}
+                                         f"sediment ({threads} OpenMP threads), {cdt:.2f} s per step",
+                               "cpu_model": info["model"], "affinity_cores": info["affinity"]}
         rec["gpu_over_cpu"] = value * cdt
     else:
         rec["cpu_baseline"] = None
@@ -665,7 +707,8 @@ def main():
         dom = max((k for k in per_kernel if k in KERNEL_BYTES_PER_CELL), key=lambda k: per_kernel[k][2])
     avg_ms = per_kernel[dom][0]
     rows = sc.nx - 1 if dom not in ("qx_tail", "qy_tail") else sc.nx
-    dom_bytes = KERNEL_BYTES_PER_CELL[dom] * rows * sc.ny
+    io_bytes = KERNEL_BYTES_PER_CELL[dom] * rows * sc.ny
+    dom_bytes = SWEEP_BYTES_PER_CELL.get(dom, KERNEL_BYTES_PER_CELL[dom]) * rows * sc.ny
     achieved = dom_bytes / (avg_ms * 1e-3) / 1e9
     step_gbs = B_ALG_STEP * value / 1e9
     traffic = measured_traffic(dom, args.nx, args.ny, args.solver)
@@ -689,7 +732,10 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "avg_launch_ms": avg_ms,
-                     "alg_bytes_per_launch": dom_bytes},
+                     "alg_bytes_per_launch": dom_bytes,
+                     "alg_bytes_basis": "the sweep's compulsory field accesses (88 B per face for the Qx predictor)",
+                     "kernel_io_bytes_per_launch": io_bytes,
+                     "kernel_io_frac": io_bytes / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "step_roofline": {"bytes_per_cell_step": B_ALG_STEP, "achieved": step_gbs,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
         "kernels_avg_ms_warmup": {k: round(v[0], 5) for k, v in warm.items() if v[1]},
@@ -697,7 +743,7 @@ def main():
     }
     g.close()
     if not args.no_cpu:
-        record["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps or 300, args.cpu_threads)
+        record["cpu_baseline"] = cpu_baseline(sc, args.cpu_steps or 100, args.cpu_threads)
         record["gpu_over_cpu"] = value / record["cpu_baseline"]["value"]
     else:
         record["cpu_baseline"] = None

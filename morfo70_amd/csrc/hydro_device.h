@@ -101,12 +101,26 @@ struct ScenDesc {
     Forcing fw;
     Work w;
     ColsWork cw;
-    double f0;
+    double f0;            // offshore offset when the scenario has no series
+    const double *f0s;    // the scenario's f0 series on the device, or NULL
+    long f0_n;            // series length
+    long f0_base;         // series index of the batch run's first step
+    const int *step;      // steps done in the current batch run (one counter per batch)
     int S_rows;   // segments per row line in the LDS row solver
     int S_red;    // segments per reduced column system
     int L_red;    // reduced column systems per workgroup
     int pad;
 };
+
+// Offshore offset of the scenario's current step: series[min(base + step, n - 1)], the
+// value m70h_run would use at the same step.
+__device__ __forceinline__ double scen_f0(const ScenDesc &d)
+{
+    if (!d.f0s) return d.f0;
+    long q = d.f0_base + *d.step;
+    if (q > d.f0_n - 1) q = d.f0_n - 1;
+    return d.f0s[q];
+}
 
 // Stages of a grouped step; start[stage][s] is the first block of scenario s.
 // GS_RED / GS_REDP: reduced column systems in 256- / 1024-thread workgroups.

@@ -24,6 +24,8 @@ hipError_t launch_qyx_assemble(hipStream_t st, const Consts &k, const Metrics &m
 // write_vc = false skips the V_c store, for steps whose successors in the Run form V_c from V.
 hipError_t launch_qy_tail(hipStream_t st, const Consts &k, const State &s, bool write_vc = true);
 hipError_t launch_nonfinite(hipStream_t st, const double *x, long n, int *flag);
+// *flag = 0 by a one-thread kernel (capturable in a graph, unlike a memset node).
+hipError_t launch_flag_clear(hipStream_t st, int *flag);
 hipError_t launch_init_zs(hipStream_t st, double *Zs, const double *D, const double *Z, const double *hb, long n);
 
 hipError_t launch_thomas_cols(hipStream_t st, const double *A, const double *B, const double *C, const double *F,

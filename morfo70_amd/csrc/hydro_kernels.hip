@@ -533,6 +533,11 @@ __global__ void __launch_bounds__(256) k_nonfinite(const double *__restrict__ x,
     if (bad) atomicOr(flag, 1);
 }
 
+__global__ void __launch_bounds__(64) k_flag_clear(int *flag)
+{
+    if (threadIdx.x == 0) *flag = 0;
+}
+
 // Zs = D - Z + h, src/Hydro.cpp:35-37.
 __global__ void __launch_bounds__(256) k_init_zs(double *Zs, const double *D, const double *Z, const double *hb,
                                                  long n)
@@ -552,6 +557,9 @@ __global__ void __launch_bounds__(256) k_cells_grouped(const ScenDesc *__restric
                                                        int nscen)
 {
     const int b = blockIdx.x;
+    // the Qy tail ends a step: count it for the offshore offset series (scen_f0); every
+    // reader of the counter is in a later launch
+    if (OP == 3 && b == 0 && threadIdx.x == 0) atomicAdd((int *)D[0].step, 1);
     const int sid = scen_of(start, nscen, b);
     const ScenDesc &d = D[sid];
     const int ny = d.k.ny;
@@ -561,7 +569,7 @@ __global__ void __launch_bounds__(256) k_cells_grouped(const ScenDesc *__restric
         const int i = lb / gx2;
         const int j0 = 2 * ((lb - i * gx2) * 256 + (int)threadIdx.x);
         if (j0 >= ny || i >= d.k.nx) return;
-        if (OP == 2) qx_tail2_body(d.k, d.m, d.s, d.w, d.f0, i, j0);
+        if (OP == 2) qx_tail2_body(d.k, d.m, d.s, d.w, scen_f0(d), i, j0);
         else qy_tail2_body(d.k, d.s, i, j0);
         return;
     }
@@ -571,7 +579,7 @@ __global__ void __launch_bounds__(256) k_cells_grouped(const ScenDesc *__restric
     if (j >= ny || i >= d.k.nx) return;
     if (OP == 0) drag_viscosity_body(d.k, d.s.D, d.fw, d.rf, i, j);
     else if (OP == 1) run_fields_body(d.k, d.m, d.fw, d.rf, i, j);
-    else if (OP == 2) qx_tail_body(d.k, d.m, d.s, d.w, d.f0, i, j);
+    else if (OP == 2) qx_tail_body(d.k, d.m, d.s, d.w, scen_f0(d), i, j);
     else qy_tail_body(d.k, d.s, i, j);
 }
 
@@ -655,6 +663,12 @@ hipError_t launch_nonfinite(hipStream_t st, const double *x, long n, int *flag)
     if (blocks > 4096) blocks = 4096;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_nonfinite, dim3((unsigned)blocks), dim3(256), 0, st, x, n, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_flag_clear(hipStream_t st, int *flag)
+{
+    hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(64), 0, st, flag);
     return hipGetLastError();
 }
 

@@ -67,6 +67,11 @@ struct m70h {
     double *sr = nullptr, *sv = nullptr, *sk = nullptr;
     ColsWork cw{};
     int *flag = nullptr;
+    int *run_flag = nullptr;      // device: a Run's non-finite flag
+    int *run_flag_host = nullptr; // pinned copy of it, read by m70h_synchronize
+    bool run_check_pending = false;
+    double *f0_dev = nullptr;     // device copy of f0_series (read by batched steps)
+    long f0_dev_n = 0;
     AdjCtx *adj = nullptr;   // checkpoints and work of the adjoint (m70h_adjoint_*)
     long run_gen = 0;        // bumped by every Run and forcing change
     // profiling
@@ -454,6 +459,10 @@ int m70h_create(int nx, int ny, const double *x, double dy, const m70h_params *p
         (e = dalloc(h, &h->sk, scr)) != hipSuccess)
         return bail(hip_fail(h, e, "hipMalloc scratch"));
     if ((e = dalloc(h, &h->flag, 1)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc flag"));
+    if ((e = dalloc(h, &h->run_flag, 1)) != hipSuccess) return bail(hip_fail(h, e, "hipMalloc flag"));
+    if ((e = hipHostMalloc((void **)&h->run_flag_host, sizeof(int), hipHostMallocDefault)) != hipSuccess)
+        return bail(hip_fail(h, e, "hipHostMalloc flag"));
+    *h->run_flag_host = 0;
     {   // three-pass column solver: E0..E2 reuse the coefficient arrays a, b, c
         ColsWork &cw = h->cw;
         cw.nseg = cols_segments(nx - 1);
@@ -473,6 +482,8 @@ void m70h_destroy(m70h *h)
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     adj_release(h);
     for (void *q : h->allocs) (void)hipFree(q);
+    if (h->f0_dev) (void)hipFree(h->f0_dev);
+    if (h->run_flag_host) (void)hipHostFree(h->run_flag_host);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     delete h;
@@ -605,6 +616,20 @@ int m70h_set_offshore_f0_series(m70h *h, const double *f0, long n)
     if (n < 0 || (n > 0 && !f0)) return fail(h, M70H_EINVAL, "bad f0 series");
     h->f0_series.assign(f0, f0 + n);
     h->f0_pos = 0;
+    // device copy for the batched steps, which read the step's value on the device
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (h->f0_dev && h->f0_dev_n < n) {
+        HIPCHK(h, hipFree(h->f0_dev));
+        h->f0_dev = nullptr;
+        h->f0_dev_n = 0;
+    }
+    if (n > 0) {
+        if (!h->f0_dev) {
+            HIPCHK(h, hipMalloc(&h->f0_dev, n * sizeof(double)));
+            h->f0_dev_n = n;
+        }
+        HIPCHK(h, hipMemcpy(h->f0_dev, f0, n * sizeof(double), hipMemcpyHostToDevice));
+    }
     return M70H_OK;
 }
 
@@ -617,7 +642,12 @@ int m70h_bc_constants(const m70h *h, double *f1, double *f0, double *bcQy)
     return M70H_OK;
 }
 
-int m70h_run_async(m70h *h, int nIter)
+}  // extern "C"
+
+// Run(nIter) enqueued on the handle's stream. With `check`, a scan of the final state for
+// NaN/Inf follows; its flag lands in pinned host memory and m70h_synchronize reports it.
+// The device outer loop enqueues runs without the scan.
+static int run_enqueue(m70h *h, int nIter, bool check)
 {
     if (!h) return M70H_EINVAL;
     if (nIter < 0) return fail(h, M70H_EINVAL, "nIter < 0");
@@ -627,14 +657,35 @@ int m70h_run_async(m70h *h, int nIter)
     if (rc) return rc;
     for (int it = 0; it < nIter; ++it)
         if ((rc = run_step(h, it == nIter - 1, it == 0))) return rc;
+    if (check) {
+        // a fill kernel rather than a memset, so that a caller may capture the Run in a graph
+        HIPCHK(h, launch_flag_clear(h->stream, h->run_flag));
+        State &s = h->s;
+        const double *f[] = {s.Zs, s.D, s.Qx, s.Qy, s.U, s.V, s.Uc, s.Vc, s.dQx, s.dQy};
+        const long n[] = {h->F, h->F, h->Fp, h->Fp, h->Fp, h->Fp, h->F, h->F, h->F, h->F};
+        for (int q = 0; q < 10; ++q) HIPCHK(h, launch_nonfinite(h->stream, f[q], n[q], h->run_flag));
+        HIPCHK(h, hipMemcpyAsync(h->run_flag_host, h->run_flag, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+        h->run_check_pending = true;
+    }
     return M70H_OK;
 }
+
+extern "C" {
+
+int m70h_run_async(m70h *h, int nIter) { return run_enqueue(h, nIter, true); }
 
 int m70h_synchronize(m70h *h)
 {
     if (!h) return M70H_EINVAL;
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    if (h->prof) return collect_profile(h);
+    if (h->prof) {
+        int rc = collect_profile(h);
+        if (rc) return rc;
+    }
+    if (h->run_check_pending) {
+        h->run_check_pending = false;
+        if (*h->run_flag_host) return fail(h, M70H_ENONFINITE, "non-finite value in the state after Run");
+    }
     return M70H_OK;
 }
 
@@ -788,6 +839,7 @@ struct m70h_batch {
     hipStream_t stream = nullptr;
     ScenDesc *d_desc = nullptr;   // [2][nscen]: even steps (D, Dn) and odd steps (swapped)
     int *d_start = nullptr;       // [GS_COUNT][nscen+1]
+    int *d_step = nullptr;        // steps done in the current run (offshore offset series)
     int blocks[GS_COUNT] = {};
     int cell_blocks = 0;
     bool use_graph = true;
@@ -808,13 +860,17 @@ static int bfail(m70h_batch *b, int code, const std::string &msg)
         if (e_ != hipSuccess) return bfail(b, M70H_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
     } while (0)
 
-static void fill_desc(const m70h *h, ScenDesc &d, bool odd, int rb)
+static void fill_desc(const m70h *h, ScenDesc &d, bool odd, int rb, const int *step)
 {
     int rows, cols, red, redp, S_rows, S_red, L_red;
     grouped_stage_blocks(h->nx, h->ny, rb, &rows, &cols, &red, &redp, &S_rows, &S_red, &L_red);
     d.k = h->k; d.m = h->m; d.s = h->s; d.rf = h->rf; d.fw = h->fw; d.w = h->w; d.cw = h->cw;
     if (odd) std::swap(d.s.D, d.s.Dn);
-    d.f0 = h->f0_series.empty() ? h->f0 : h->f0_series.back();
+    d.f0 = h->f0;
+    d.f0s = h->f0_series.empty() ? nullptr : h->f0_dev;
+    d.f0_n = (long)h->f0_series.size();
+    d.f0_base = h->f0_pos;
+    d.step = step;
     d.S_rows = S_rows; d.S_red = S_red; d.L_red = L_red; d.pad = 0;
 }
 
@@ -823,10 +879,11 @@ static int batch_upload(m70h_batch *b)
     const int n = (int)b->hs.size();
     std::vector<ScenDesc> host(2 * (size_t)n);
     for (int q = 0; q < n; ++q) {
-        fill_desc(b->hs[q], host[q], false, b->rows_block);
-        fill_desc(b->hs[q], host[n + q], true, b->rows_block);
+        fill_desc(b->hs[q], host[q], false, b->rows_block, b->d_step);
+        fill_desc(b->hs[q], host[n + q], true, b->rows_block, b->d_step);
     }
     BHIPCHK(b, hipMemcpyAsync(b->d_desc, host.data(), host.size() * sizeof(ScenDesc), hipMemcpyHostToDevice, b->stream));
+    BHIPCHK(b, launch_flag_clear(b->stream, b->d_step));
     BHIPCHK(b, hipStreamSynchronize(b->stream));
     return M70H_OK;
 }
@@ -890,6 +947,7 @@ int m70h_batch_create(int nscen, m70h *const *handles, m70h_batch **out)
     if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipMalloc(&b->d_desc, 2 * (size_t)nscen * sizeof(ScenDesc))) != hipSuccess ||
         (e = hipMalloc(&b->d_start, starts.size() * sizeof(int))) != hipSuccess ||
+        (e = hipMalloc(&b->d_step, sizeof(int))) != hipSuccess ||
         (e = hipMemcpy(b->d_start, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess)
         return bail(bfail(b, M70H_EHIP, std::string("batch allocation: ") + hipGetErrorString(e)));
     *out = b;
@@ -903,6 +961,7 @@ void m70h_batch_destroy(m70h_batch *b)
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->d_desc) (void)hipFree(b->d_desc);
     if (b->d_start) (void)hipFree(b->d_start);
+    if (b->d_step) (void)hipFree(b->d_step);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -2206,7 +2265,7 @@ int m70o_run(m70o *o, double stopTime, long *steps)
         // the forcing also carries Dw to the sediment, which reads wave->Dw in the reference
         if ((rc = m70w_push_forcing(o->w))) return ofail(o, rc, o->w->err);
         if (p.calculate_hydro) {
-            if ((rc = m70h_run_async(h, hydroIterations))) return ofail(o, rc, h->err);
+            if ((rc = run_enqueue(h, hydroIterations, false))) return ofail(o, rc, h->err);
             if (p.wave_current_interaction && (rc = m70w_set_uv_from_hydro(o->w))) return ofail(o, rc, o->w->err);
         }
         if (p.calculate_sediment && o->s && (rc = m70s_calculate_fluxes_async(o->s))) return ofail(o, rc, o->s->err);

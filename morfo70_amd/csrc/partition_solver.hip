@@ -866,10 +866,10 @@ int rows_block_for(int n, int rb)
 struct GProvQxx {
     const ScenDesc *d;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
-    { asm_qxx(d->k, d->m, d->s, d->rf, d->w, d->f0, kk, line, a, b, c, f); }
+    { asm_qxx(d->k, d->m, d->s, d->rf, d->w, scen_f0(*d), kk, line, a, b, c, f); }
     __device__ __forceinline__ void segment(int line, int k0, int mm, double (&lo)[CM], double (&di)[CM],
                                             double (&up)[CM], double (&rh)[CM]) const
-    { qxx_segment<false, false>(d->k, d->m, d->s, d->rf, d->w, d->f0, line, k0, mm, lo, di, up, rh); }
+    { qxx_segment<false, false>(d->k, d->m, d->s, d->rf, d->w, scen_f0(*d), line, k0, mm, lo, di, up, rh); }
 };
 struct GProvQyx {
     const ScenDesc *d;

@@ -120,6 +120,10 @@ void Hydro::Run(int nIter)
     check(m70h_set_state(h_, Zs, bathy->D, Qx, Qy, U, V, U_c, V_c, dQx_dx, dQy_dy));
     check(m70h_set_params(h_, &m70::hydroConfig().params));   // read at call time, as the reference
     check(m70h_set_forcing(h_, wave->Sxx, wave->Sxy, wave->Syy, wave->Urms, wave->Dw, wave->D_r, wave->H));
-    check(m70h_run(h_, nIter));
-    check(m70h_get_state(h_, Zs, bathy->D, Qx, Qy, U, V, U_c, V_c, dQx_dx, dQy_dy));
+    const int rc = m70h_run(h_, nIter);
+    // a Run that ends with NaN/Inf in the state still leaves that state in the public
+    // arrays, as the reference does; the facade then reports it
+    if (rc == M70H_OK || rc == M70H_ENONFINITE)
+        check(m70h_get_state(h_, Zs, bathy->D, Qx, Qy, U, V, U_c, V_c, dQx_dx, dQy_dy));
+    check(rc);
 }

@@ -154,9 +154,11 @@ class Hydro:
         if push:
             self.push()
         self.set_forcing()
-        self._chk(self._L.m70h_run(self._h, int(nIter)))
-        if pull:
+        rc = self._L.m70h_run(self._h, int(nIter))
+        # a non-finite state is still the Run's result: mirror it, then report it
+        if pull and rc in (_lib.M70H_OK, _lib.M70H_ENONFINITE):
             self.pull()
+        self._chk(rc)
 
     # -- device-resident use (benchmarks, batched drivers) --------------------------
     def run_async(self, nIter: int):

@@ -445,3 +445,36 @@ def wave_breaking(params, H: float, sigma: float, D: float):
     _wave_lib().orc_wave_breaking(ctypes.byref(p), H, sigma, D, ctypes.cast(ctypes.byref(dw), _dp),
                                   ctypes.cast(ctypes.byref(ddw), _dp))
     return dw.value, ddw.value
+
+
+def host_cpus() -> dict:
+    """The host cores this process may run on: the affinity mask, bounded by a cgroup CPU
+    quota when one is set (cgroup v2 cpu.max), and the CPU model. `threads` is the
+    OpenMP thread count that uses all of them."""
+    aff = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = aff if quota is None else max(1, min(aff, int(quota)))
+    return {"threads": threads, "affinity": aff, "cgroup_quota": quota, "model": model}
+
+
+def set_omp_binding() -> None:
+    """OMP_PROC_BIND=close and OMP_PLACES=cores for the oracle's OpenMP runtime; takes
+    effect when the oracle library is first loaded."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")

@@ -96,3 +96,45 @@ def test_mass_budget_full_size(c2):
     scale = c2.physics.dt * c2.dy * float(np.sum(np.abs(after["Qx"][0]))) + 1e-30
     print(f"[mass] dV={vol:.6e} flux={flux:.6e} scale={scale:.3e}", flush=True)
     assert abs(vol - flux) <= 1e-9 * max(scale, abs(vol)) + 1e-9
+
+
+@pytest.fixture(scope="module")
+def c2_2000():
+    return S.config2(steps=2000)
+
+
+def test_fused_vs_exact_2000_steps(c2_2000):
+    """The plain workload as configured: 2048 x 4096, tide, 2000 steps. FUSED against
+    EXACT (the reference's own recurrences, equal to the oracle bit for bit where no
+    transcendental function enters, and to 1e-10 here) at the 1e-10 bar."""
+    from morfo70_amd.hydro import from_scenario
+    print("[fullsize] 2048x4096 tide, Run(2000), fused vs exact", flush=True)
+    out = []
+    for solver in (_lib.SOLVER_EXACT, _lib.SOLVER_FUSED):
+        g = from_scenario(c2_2000, solver=solver)
+        g.Run(2000)
+        out.append(g.state())
+        g.close()
+        print(f"[fullsize] solver {solver} done", flush=True)
+    res = compare_states(out[1], out[0])
+    print({k: v["rel"] for k, v in res.items()}, flush=True)
+    for k, r in res.items():
+        assert r["nan"] == 0 and r["rel"] <= 1e-10, (k, r)
+
+
+def test_fused_vs_oracle_200_steps(c2_2000):
+    """FUSED against the oracle over the first 200 steps of the configured workload (the
+    oracle's OpenMP build takes about 10 s for them on 16 host threads)."""
+    import oracle as orc
+    from morfo70_amd.hydro import from_scenario
+    threads = orc.host_cpus()["threads"]
+    print(f"[fullsize] 2048x4096 tide, Run(200), fused vs oracle on {threads} threads", flush=True)
+    o = orc.OracleHydro(c2_2000, threads=threads)
+    g = from_scenario(c2_2000, solver=_lib.SOLVER_FUSED)
+    g.Run(200)
+    o.run(200)
+    res = compare_states(g.state(), o.get_state())
+    g.close()
+    print({k: v["rel"] for k, v in res.items()}, flush=True)
+    for k, r in res.items():
+        assert r["nan"] == r["ref_nan"] == 0 and r["rel"] <= 1e-10, (k, r)

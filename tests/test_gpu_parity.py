@@ -13,6 +13,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
+from morfo70_amd import _lib
 from morfo70_amd import scenarios as S
 from tests.helpers import compare_states, max_rel, nonuniform_beach
 
@@ -207,3 +208,31 @@ def test_run_starts_from_the_callers_derived_fields(solver):
     for _ in range(2):
         ctl.run(3)
     assert max_rel(ctl.get_state()["Qx"], o.get_state()["Qx"]) > 1e-7
+
+
+def test_run_reports_a_nonfinite_state():
+    """Urms = 0 with the fluxes at rest makes the drag term 0/0 (src/HydroParametrizations.cpp:25),
+    which the reference lets pass silently. m70h_run returns M70H_ENONFINITE; the state,
+    NaN included, still reaches the host mirrors; a clean Run afterwards reports success."""
+    from morfo70_amd.hydro import from_scenario
+    sc = nonuniform_beach(nx=30, ny=24, steps=4)
+    g = from_scenario(sc, solver=_lib.SOLVER_FUSED)
+    g.Run(2)                                   # clean
+    urms = g.wave.Urms.copy()
+    g.wave.Urms[10:14, 5:9] = 0.0
+    g.Qx[...] = 0.0
+    g.Qy[...] = 0.0
+    g.U[...] = 0.0
+    g.V[...] = 0.0
+    g.U_c[...] = 0.0
+    g.V_c[...] = 0.0
+    with pytest.raises(_lib.M70HError) as ei:
+        g.Run(3)
+    assert ei.value.code == _lib.M70H_ENONFINITE
+    assert np.isnan(g.Qx).any()
+    g2 = from_scenario(sc, solver=_lib.SOLVER_EXACT)
+    g2.wave.Urms[...] = urms
+    g2.Run(3)                                  # no exception
+    assert np.isfinite(g2.Qx).all()
+    g.close()
+    g2.close()
