@@ -172,6 +172,22 @@ int m70h_set_rows_block(m70h *h, int threads);
 int m70h_tridiag_solve(void *hip_stream, int solver, int lines_are_rows, int periodic, int n, int nlines, long ld,
                        const double *a_dev, const double *b_dev, const double *c_dev, const double *f_dev,
                        double *y_dev, long ldy, double *scratch_dev, long scratch_len, int tile);
+/* m70h_tridiag_solve without the workspace argument, and without any device allocation:
+ * no counterpart in the reference. It is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The EXACT family's scratch lives in a static device array of the library,
+ * M70H_NOALLOC_SCRATCH_DOUBLES doubles (32 MiB) on every GPU the library runs on, held
+ * until the process ends; its address is looked up once per device on that device's
+ * first call. Sizes whose scratch would exceed it (EXACT rows+periodic:
+ * 3*n*roundup(nlines,64); EXACT columns: n*ld) return M70H_EINVAL with a message in
+ * m70h_last_error(NULL). All calls on one GPU share that GPU's array: calls that overlap
+ * on one GPU (different streams or threads without ordering between them) race. The
+ * array keeps the last call's contents, and no result depends on them. PARTITION needs no
+ * scratch. Same results as m70h_tridiag_solve. */
+#define M70H_NOALLOC_SCRATCH_DOUBLES 4194304L
+int m70h_tridiag_solve_noalloc(void *hip_stream, int solver, int lines_are_rows, int periodic, int n, int nlines,
+                               long ld, const double *a_dev, const double *b_dev, const double *c_dev,
+                               const double *f_dev, double *y_dev, long ldy, int tile);
 
 
 /* Scans the state for NaN/Inf on the device; *finite = 1 if none. Returns

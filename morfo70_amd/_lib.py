@@ -44,6 +44,9 @@ SIGNATURES = {
     "m70h_tridiag_solve": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.c_long] + [_vp] * 5 +
                            [ctypes.c_long, _vp, ctypes.c_long, ctypes.c_int]),
+    "m70h_tridiag_solve_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_int, ctypes.c_long] + [_vp] * 5 +
+                                   [ctypes.c_long, ctypes.c_int]),
     "m70h_check_finite": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_int)]),
     "m70h_set_profiling": (ctypes.c_int, [_vp, ctypes.c_int]),
     "m70h_set_profiling_only": (ctypes.c_int, [_vp, ctypes.c_char_p]),

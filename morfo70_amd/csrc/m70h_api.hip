@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -762,6 +763,73 @@ int m70h_tridiag_solve(void *hip_stream, int solver, int lines_are_rows, int per
         return e == hipErrorInvalidValue ? M70H_EINVAL : M70H_EHIP;
     }
     return M70H_OK;
+}
+
+}  // extern "C"
+
+// Static scratch of m70h_tridiag_solve_noalloc: one copy per GPU, in the code object, so
+// the entry point never allocates. Host code uses only the per-device address that
+// hipGetSymbolAddress returns (the symbol's name is a host-side stand-in).
+static __device__ __align__(256) double scratch[M70H_NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+// Address of `scratch` on device `dev`, looked up once with that device current.
+int noalloc_scratch(int dev, double **out)
+{
+    if (dev < 0 || dev >= kMaxDevices) {
+        g_create_error = "m70h_tridiag_solve_noalloc: device ordinal out of range";
+        return M70H_EINVAL;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != dev) e = hipSetDevice(dev);
+        void *p = nullptr;
+        if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != dev) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) {
+            g_create_error = std::string("m70h_tridiag_solve_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e);
+            return M70H_EHIP;
+        }
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return M70H_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int m70h_tridiag_solve_noalloc(void *hip_stream, int solver, int lines_are_rows, int periodic, int n, int nlines,
+                               long ld, const double *a_dev, const double *b_dev, const double *c_dev,
+                               const double *f_dev, double *y_dev, long ldy, int tile)
+{
+    if (n < 3 || nlines < 1) return M70H_EINVAL;
+    long need = 0;
+    if (solver == M70H_SOLVER_EXACT)
+        need = lines_are_rows ? 3L * n * periodic_rows_padded_lines(nlines) : (long)n * ld;
+    if (need > M70H_NOALLOC_SCRATCH_DOUBLES) {
+        g_create_error = "m70h_tridiag_solve_noalloc: the solve needs " + std::to_string(need) +
+                         " doubles of scratch, above the static maximum of " +
+                         std::to_string(M70H_NOALLOC_SCRATCH_DOUBLES);
+        return M70H_EINVAL;
+    }
+    int dev = 0;
+    hipError_t e = hip_stream ? hipStreamGetDevice((hipStream_t)hip_stream, &dev) : hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        g_create_error = std::string("m70h_tridiag_solve_noalloc: device of the stream: ") + hipGetErrorString(e);
+        return M70H_EHIP;
+    }
+    double *sp = nullptr;
+    int rc = noalloc_scratch(dev, &sp);
+    if (rc) return rc;
+    return m70h_tridiag_solve(hip_stream, solver, lines_are_rows, periodic, n, nlines, ld, a_dev, b_dev, c_dev, f_dev,
+                              y_dev, ldy, sp, M70H_NOALLOC_SCRATCH_DOUBLES, tile);
 }
 
 int m70h_check_finite(m70h *h, int *finite)
