@@ -32,34 +32,57 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def _bind(L):
+    """Sets the argument and result types of the oracle's API on a loaded build."""
+    L.orc_create.restype = ctypes.c_void_p
+    L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double,
+                             ctypes.POINTER(_Params), _dp, _dp, ctypes.c_double]
+    L.orc_destroy.argtypes = [ctypes.c_void_p]
+    L.orc_set_threads.argtypes = [ctypes.c_int]
+    L.orc_set_forcing.argtypes = [ctypes.c_void_p] + [_dp] * 7
+    L.orc_set_f0.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    L.orc_set_f0_series.argtypes = [ctypes.c_void_p, _dp, ctypes.c_long]
+    L.orc_bc_constants.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+    L.orc_get_state.argtypes = [ctypes.c_void_p] + [_dp] * 10
+    L.orc_set_state.argtypes = [ctypes.c_void_p] + [_dp] * 10
+    L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    L.orc_run.restype = ctypes.c_int
+    L.orc_get_field.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _dp]
+    L.orc_get_field.restype = ctypes.c_long
+    L.orc_grid_metrics.argtypes = [ctypes.c_int, _dp, _dp]
+    L.orc_tridiag.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]
+    L.orc_tridiag_periodic.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int, _dp]
+    L.orc_wet_rule.argtypes = [ctypes.c_double] * 6 + [ctypes.c_double]
+    L.orc_wet_rule.restype = ctypes.c_int
+    return L
+
+
+def load(path: str):
+    """A build of the oracle at `path` (the default one, an older copy, or a mutant)."""
+    return _bind(ctypes.CDLL(path))
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
-        L.orc_create.restype = ctypes.c_void_p
-        L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double,
-                                 ctypes.POINTER(_Params), _dp, _dp, ctypes.c_double]
-        L.orc_destroy.argtypes = [ctypes.c_void_p]
-        L.orc_set_threads.argtypes = [ctypes.c_int]
-        L.orc_set_forcing.argtypes = [ctypes.c_void_p] + [_dp] * 7
-        L.orc_set_f0.argtypes = [ctypes.c_void_p, ctypes.c_double]
-        L.orc_set_f0_series.argtypes = [ctypes.c_void_p, _dp, ctypes.c_long]
-        L.orc_bc_constants.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
-        L.orc_get_state.argtypes = [ctypes.c_void_p] + [_dp] * 10
-        L.orc_set_state.argtypes = [ctypes.c_void_p] + [_dp] * 10
-        L.orc_run.argtypes = [ctypes.c_void_p, ctypes.c_int]
-        L.orc_run.restype = ctypes.c_int
-        L.orc_get_field.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _dp]
-        L.orc_get_field.restype = ctypes.c_long
-        L.orc_grid_metrics.argtypes = [ctypes.c_int, _dp, _dp]
-        L.orc_tridiag.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]
-        L.orc_tridiag_periodic.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int, _dp]
-        L.orc_wet_rule.argtypes = [ctypes.c_double] * 6 + [ctypes.c_double]
-        L.orc_wet_rule.restype = ctypes.c_int
-        _lib = L
+        _lib = load(LIB_PATH)
     return _lib
+
+
+MUTANT_COUNT = 22
+MUTANT_DIR = os.path.join(_HERE, "build", "mutants")
+
+
+def mutant_lib(k: int):
+    """The oracle built with -DORC_MUTANT=k: term k of the assembly scaled by 1.1 (see the
+    mutation hooks in m70_oracle.c). Built on first use by `make mutants`."""
+    path = os.path.join(MUTANT_DIR, f"liboracle_mut{k}.so")
+    if not os.path.exists(path):
+        import subprocess
+        subprocess.run(["make", "-s", "-C", _HERE, "mutants"], check=True)
+    return load(path)
 
 
 def ref_lib():
@@ -107,8 +130,8 @@ def grid_metrics(x: np.ndarray) -> dict:
 class OracleHydro:
     """The reference Hydro object restated on the CPU (see m70_oracle.h)."""
 
-    def __init__(self, sc, threads: int = 0):
-        L = lib()
+    def __init__(self, sc, threads: int = 0, library=None):
+        L = self._L = library if library is not None else lib()
         if threads:
             L.orc_set_threads(threads)
         self.nx, self.ny = sc.nx, sc.ny
@@ -126,35 +149,35 @@ class OracleHydro:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().orc_destroy(self.h)
+            self._L.orc_destroy(self.h)
             self.h = None
 
     def set_forcing(self, f: dict):
         arrs = [np.ascontiguousarray(f[k], dtype=np.float64) for k in FORCING_FIELDS]
-        lib().orc_set_forcing(self.h, *[ptr(a) for a in arrs])
+        self._L.orc_set_forcing(self.h, *[ptr(a) for a in arrs])
 
     def set_f0(self, f0: float):
-        lib().orc_set_f0(self.h, f0)
+        self._L.orc_set_f0(self.h, f0)
 
     def set_f0_series(self, s):
         s = np.ascontiguousarray(s, dtype=np.float64)
         self._f0 = s
-        lib().orc_set_f0_series(self.h, ptr(s), s.size)
+        self._L.orc_set_f0_series(self.h, ptr(s), s.size)
 
     def bc_constants(self):
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
-        lib().orc_bc_constants(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        self._L.orc_bc_constants(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
         return a.value, b.value, c.value
 
     def run(self, n: int):
-        lib().orc_run(self.h, int(n))
+        self._L.orc_run(self.h, int(n))
 
     def _shape(self, name):
         return (self.nx + 1, self.ny) if name in PLUS_FIELDS else (self.nx, self.ny)
 
     def get_state(self) -> dict:
         out = {k: np.zeros(self._shape(k)) for k in STATE_FIELDS}
-        lib().orc_get_state(self.h, *[ptr(out[k]) for k in STATE_FIELDS])
+        self._L.orc_get_state(self.h, *[ptr(out[k]) for k in STATE_FIELDS])
         return out
 
     def set_state(self, st: dict):
@@ -162,14 +185,14 @@ class OracleHydro:
         for k in STATE_FIELDS:
             a = st.get(k)
             arrs.append(None if a is None else np.ascontiguousarray(a, dtype=np.float64))
-        lib().orc_set_state(self.h, *[ptr(a) if a is not None else None for a in arrs])
+        self._L.orc_set_state(self.h, *[ptr(a) if a is not None else None for a in arrs])
 
     def field(self, name: str) -> np.ndarray:
-        n = lib().orc_get_field(self.h, name.encode(), None)
+        n = self._L.orc_get_field(self.h, name.encode(), None)
         if n < 0:
             raise KeyError(name)
         out = np.zeros(n)
-        lib().orc_get_field(self.h, name.encode(), ptr(out))
+        self._L.orc_get_field(self.h, name.encode(), ptr(out))
         return out.reshape(-1, self.ny)
 
 

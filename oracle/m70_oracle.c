@@ -19,6 +19,23 @@
 #include <omp.h>
 #endif
 
+/* Mutation hooks for the oracle's own tests (tests/test_oracle_consistency.py). A build
+ * with -DORC_MUTANT=k scales term k of the assembly by 1.1; the default build multiplies
+ * by 1.0, which is exact, so its results are those of the unmarked expressions. Terms:
+ *   1 Qx predictor advection     2 Qx predictor x-viscosity    3 Qx predictor cross terms
+ *   4 Qx predictor dt*d(dQy_dy)/dx coupling                    5 Qx predictor 1/dt in d
+ *   6 Qx predictor implicit gravity                            7 Qx projector advection
+ *   8 Qx projector y-viscosity   9 Qy predictor advection     10 Qy predictor y-viscosity
+ *  11 Qy predictor cross terms  12 Qy predictor dt*d(dQx_dx)/dy coupling
+ *  13 Qy projector advection    14 Qy projector x-viscosity   15 drag term
+ *  16 Qx forcing                17 Qy forcing                 18 Qx explicit gravity
+ *  19 Qy explicit gravity       20 Qy predictor implicit gravity
+ *  21 Qy predictor 1/dt in d    22 Qx projector 1/dt form (1 + dt*drag) in d */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+#define MUT(k) ((ORC_MUTANT) == (k) ? 1.1 : 1.0)
+
 /* include/OptionsParametersConstants.h:14-19 */
 static const double g = 9.8100000000000000;
 static const double rho = 1025.00000000000;
@@ -366,8 +383,8 @@ static void calculateDragTerm(double *drag, const double *cd, const double *urms
     const double alpha_f2 = (alpha_f * alpha_f);
     const double inv_sqrt2 = 1 / sqrt_2;
     for (int j = 0; j < ny; j++)
-        drag[j] = cd[j] * urms[j] * inv_sqrt2 *
-                  sqrt(alpha_f2 + 2 * (v[j] * v[j] + u[j] * u[j]) / (urms[j] * urms[j])) * inv_d[j];
+        drag[j] = MUT(15) * (cd[j] * urms[j] * inv_sqrt2 *
+                  sqrt(alpha_f2 + 2 * (v[j] * v[j] + u[j] * u[j]) / (urms[j] * urms[j])) * inv_d[j]);
 }
 
 static void calculateDragAndViscosity(orc_hydro *H)
@@ -606,16 +623,18 @@ static void setSolverQxx_f(orc_hydro *H, int i)
         const double d2qy_dxy = inv_dx * (dqy_dy[ny + j] - dqy_dy[j]);
         const double dV_dx = 0.5 * inv_dx * ((v[ny + j] + v[ny + j - 1]) - (v[j] + v[j - 1]));
         const double d2V_dxy = inv_dx * inv_dy * ((v[ny + j] - v[ny + j - 1]) - (v[j] - v[j - 1]));
-        _f[j] = isWater[j] * (forcing_x[j] + qx[j] * inv_dt + g * d_x[j] * (dt * d2qy_dxy - dzs_dx) +
-                              dnud_dy[j] * dV_dx + nud_x[j] * d2V_dxy);
+        _f[j] = isWater[j] * (MUT(16) * forcing_x[j] + qx[j] * inv_dt +
+                              g * d_x[j] * (MUT(4) * (dt * d2qy_dxy) - MUT(18) * dzs_dx) +
+                              MUT(3) * (dnud_dy[j] * dV_dx) + MUT(3) * (nud_x[j] * d2V_dxy));
     }
     {
         const double dzs_dx = inv_dx * (zs[ny] - zs[0]);
         const double d2qy_dxy = inv_dx * (dqy_dy[ny] - dqy_dy[0]);
         const double dV_dx = 0.5 * inv_dx * ((v[ny] + v[2 * ny - 1]) - (v[0] + v[ny - 1]));
         const double d2V_dxy = inv_dx * inv_dy * ((v[ny] - v[2 * ny - 1]) - (v[0] - v[ny - 1]));
-        _f[0] = isWater[0] * (forcing_x[0] + qx[0] * inv_dt + g * d_x[0] * (dt * d2qy_dxy - dzs_dx) +
-                              dnud_dy[0] * dV_dx + nud_x[0] * d2V_dxy);
+        _f[0] = isWater[0] * (MUT(16) * forcing_x[0] + qx[0] * inv_dt +
+                              g * d_x[0] * (MUT(4) * (dt * d2qy_dxy) - MUT(18) * dzs_dx) +
+                              MUT(3) * (dnud_dy[0] * dV_dx) + MUT(3) * (nud_x[0] * d2V_dxy));
     }
 }
 
@@ -633,8 +652,8 @@ static void setSolverQxx_d(orc_hydro *H, int i)
     double *_d = H->d + (long)i * ny;
     for (int j = 0; j < ny; j++) {
         int isl = !isWater[j];
-        _d[j] = (inv_dt + dragTerm[j] + difx * (u[j] - 2 * dnud_dx[j] * inv_d_x[j]) -
-                 dif2x * (g * d_x[j] * dt + 2 * nud_x[j] * inv_d_x[j])) * isWater[j] + isl;
+        _d[j] = (MUT(5) * inv_dt + dragTerm[j] + difx * (MUT(1) * u[j] - MUT(2) * (2 * dnud_dx[j] * inv_d_x[j])) -
+                 dif2x * (MUT(6) * (g * d_x[j] * dt) + MUT(2) * (2 * nud_x[j] * inv_d_x[j]))) * isWater[j] + isl;
     }
 }
 
@@ -651,7 +670,8 @@ static void setSolverQxx_up(orc_hydro *H, int i)
     double *_up = H->up + (long)i * ny;
     for (int j = 0; j < ny; j++) {
         double _isWater = isWater[j] && isWater[j + ny];
-        _up[j] = (difx * (uup[j] - 2 * dnud_dx[j] * inv_d_xup[j]) - dif2x * (g * d_x[j] * dt + 2 * nud_x[j] * inv_d_xup[j])) *
+        _up[j] = (difx * (MUT(1) * uup[j] - MUT(2) * (2 * dnud_dx[j] * inv_d_xup[j])) -
+                  dif2x * (MUT(6) * (g * d_x[j] * dt) + MUT(2) * (2 * nud_x[j] * inv_d_xup[j]))) *
                  _isWater;
     }
 }
@@ -669,7 +689,8 @@ static void setSolverQxx_low(orc_hydro *H, int i)
     double *_low = H->low + (long)i * ny;
     for (int j = 0; j < ny; j++) {
         double _isWater = isWater[j] && isWater[j - ny];
-        _low[j] = (difx * (ulow[j] - 2 * dnud_dx[j] * inv_d_xlow[j]) - dif2x * (g * d_x[j] * dt + 2 * nud_x[j] * inv_d_xlow[j])) *
+        _low[j] = (difx * (MUT(1) * ulow[j] - MUT(2) * (2 * dnud_dx[j] * inv_d_xlow[j])) -
+                   dif2x * (MUT(6) * (g * d_x[j] * dt) + MUT(2) * (2 * nud_x[j] * inv_d_xlow[j]))) *
                   _isWater;
     }
 }
@@ -683,7 +704,8 @@ static void setSolverQxx0_low(orc_hydro *H)
     const double *ulow = H->U, *d_x = H->Di + ny, *inv_d_xlow = H->inv_Di, *nud_x = H->nuD_x, *dnud_dx = H->dnuD_dx;
     double *_low = H->low;
     for (int j = 0; j < ny; j++)
-        _low[j] = (difx * (ulow[j] - 2 * dnud_dx[j] * inv_d_xlow[j]) - dif2x * (g * d_x[j] * dt + 2 * nud_x[j] * inv_d_xlow[j]));
+        _low[j] = (difx * (MUT(1) * ulow[j] - MUT(2) * (2 * dnud_dx[j] * inv_d_xlow[j])) -
+                   dif2x * (MUT(6) * (g * d_x[j] * dt) + MUT(2) * (2 * nud_x[j] * inv_d_xlow[j])));
 }
 
 static void setSolverQxx_BC(orc_hydro *H)
@@ -717,7 +739,7 @@ static void setSolverQxy(orc_hydro *H, int i)
         const double _2inv_dy2 = 2 / (H->dy * H->dy);
         for (j = 0; j < ny; j++) {
             int isl = !isWater[j];
-            _d[j] = isWater[j] * ((1 + dt * dragTerm[j]) + dt * inv_d_x[j] * nud_x[j] * _2inv_dy2) + isl;
+            _d[j] = isWater[j] * (MUT(22) * (1 + dt * dragTerm[j]) + MUT(8) * (dt * inv_d_x[j] * nud_x[j] * _2inv_dy2)) + isl;
         }
     }
     {
@@ -726,21 +748,21 @@ static void setSolverQxy(orc_hydro *H, int i)
         /* setSolverQxy_up, :297-322 */
         for (j = 0; j < ny - 1; j++) {
             double _isWater = isWater[j] && isWater[j + 1];
-            _up[j] = dt * (v[j + 1] * inv_2dy - inv_d_x[j + 1] * (nud_x[j] * inv_dy2 + dnudx_dy[j] * inv_2dy)) * _isWater;
+            _up[j] = dt * (MUT(7) * (v[j + 1] * inv_2dy) - MUT(8) * (inv_d_x[j + 1] * (nud_x[j] * inv_dy2 + dnudx_dy[j] * inv_2dy))) * _isWater;
         }
         {
             double _isWater = isWater[ny - 1] && isWater[0];
-            _up[ny - 1] = dt * (v[0] * inv_2dy - inv_d_x[0] * (nud_x[ny - 1] * inv_dy2 + dnudx_dy[ny - 1] * inv_2dy)) *
+            _up[ny - 1] = dt * (MUT(7) * (v[0] * inv_2dy) - MUT(8) * (inv_d_x[0] * (nud_x[ny - 1] * inv_dy2 + dnudx_dy[ny - 1] * inv_2dy))) *
                           _isWater;
         }
         /* setSolverQxy_low, :324-349 */
         for (j = 1; j < ny; j++) {
             double _isWater = isWater[j] && isWater[j - 1];
-            _low[j] = dt * (-v[j - 1] * inv_2dy - inv_d_x[j - 1] * (nud_x[j] * inv_dy2 - dnudx_dy[j] * inv_2dy)) * _isWater;
+            _low[j] = dt * (MUT(7) * (-v[j - 1] * inv_2dy) - MUT(8) * (inv_d_x[j - 1] * (nud_x[j] * inv_dy2 - dnudx_dy[j] * inv_2dy))) * _isWater;
         }
         {
             double _isWater = isWater[0] && isWater[ny - 1];
-            _low[0] = dt * (-v[ny - 1] * inv_2dy - inv_d_x[ny - 1] * (nud_x[0] * inv_dy2 - dnudx_dy[0] * inv_2dy)) * _isWater;
+            _low[0] = dt * (MUT(7) * (-v[ny - 1] * inv_2dy) - MUT(8) * (inv_d_x[ny - 1] * (nud_x[0] * inv_dy2 - dnudx_dy[0] * inv_2dy))) * _isWater;
         }
     }
 }
@@ -915,16 +937,18 @@ static void setSolverQyy(orc_hydro *H, int i)
             const double d2qx_dxy = inv_dy * (dqx_dx[j + 1] - dqx_dx[j]);
             const double du_dy = inv_dy * (x2c_up * (u[ny + j + 1] - u[ny + j]) + x2c_d * (u[j + 1] - u[j]));
             const double d2u_dxy = inv_dx * inv_dy * ((u[ny + j + 1] - u[ny + j]) - (u[j + 1] - u[j]));
-            _f[j] = isWater0[j] * (forcing_y[j] + qy[j] * inv_dt + g * d_y[j] * (dt * d2qx_dxy - dzs_dy) +
-                                   dnudy_dx[j] * du_dy + nud_y[j] * d2u_dxy);
+            _f[j] = isWater0[j] * (MUT(17) * forcing_y[j] + qy[j] * inv_dt +
+                                   g * d_y[j] * (MUT(12) * (dt * d2qx_dxy) - MUT(19) * dzs_dy) +
+                                   MUT(11) * (dnudy_dx[j] * du_dy) + MUT(11) * (nud_y[j] * d2u_dxy));
         }
         {
             const double dzs_dy = inv_dy * (zs[0] - zs[ny - 1]);
             const double d2qx_dxy = inv_dy * (dqx_dx[0] - dqx_dx[ny - 1]);
             const double du_dy = inv_dy * (x2c_up * (u[ny] - u[2 * ny - 1]) + x2c_d * (u[0] - u[ny - 1]));
             const double d2u_dxy = inv_dx * inv_dy * ((u[ny] - u[2 * ny - 1]) - (u[0] - u[ny - 1]));
-            _f[ny - 1] = isWater0[ny - 1] * (forcing_y[ny - 1] + qy[ny - 1] * inv_dt + g * d_y[ny - 1] * (dt * d2qx_dxy - dzs_dy) +
-                                             dnudy_dx[ny - 1] * du_dy + nud_y[ny - 1] * d2u_dxy);
+            _f[ny - 1] = isWater0[ny - 1] * (MUT(17) * forcing_y[ny - 1] + qy[ny - 1] * inv_dt +
+                                             g * d_y[ny - 1] * (MUT(12) * (dt * d2qx_dxy) - MUT(19) * dzs_dy) +
+                                             MUT(11) * (dnudy_dx[ny - 1] * du_dy) + MUT(11) * (nud_y[ny - 1] * d2u_dxy));
         }
     }
     /* :34 C2Y(U_c) -> Ui_j (U at y-faces), :35 drag */
@@ -938,7 +962,8 @@ static void setSolverQyy(orc_hydro *H, int i)
                      *dragTerm = H->DragTerm + (long)i * ny, *nud_y = H->nuD_y + (long)i * ny;
         for (j = 0; j < ny; j++) {
             int isl = !isWater0[j];
-            _d[j] = (inv_dt + dragTerm[j] + g * d_y[j] * dt * _2inv_dy2 + 2 * nud_y[j] * inv_d_y[j] * _2inv_dy2) *
+            _d[j] = (MUT(21) * inv_dt + dragTerm[j] + MUT(20) * (g * d_y[j] * dt * _2inv_dy2) +
+                     MUT(10) * (2 * nud_y[j] * inv_d_y[j] * _2inv_dy2)) *
                         isWater0[j] + isl;
         }
     }
@@ -951,24 +976,24 @@ static void setSolverQyy(orc_hydro *H, int i)
         /* setSolverQyy_low, :206-230 */
         for (j = 1; j < ny; j++) {
             double _isWater = isWater[j] && isWater[j - 1];
-            _low[j] = (0.5 * inv_dy * (-v[j - 1] + 2 * dnud_dy[j] * inv_d_y[j - 1]) -
-                       inv_dy2 * (g * d_y[j] * dt + 2 * nud_y[j] * inv_d_y[j - 1])) * _isWater;
+            _low[j] = (0.5 * inv_dy * (MUT(9) * -v[j - 1] + MUT(10) * (2 * dnud_dy[j] * inv_d_y[j - 1])) -
+                       inv_dy2 * (MUT(20) * (g * d_y[j] * dt) + MUT(10) * (2 * nud_y[j] * inv_d_y[j - 1]))) * _isWater;
         }
         {
             double _isWater = isWater[0] && isWater[ny - 1];
-            _low[0] = (0.5 * inv_dy * (-v[ny - 1] + 2 * dnud_dy[0] * inv_d_y[ny - 1]) -
-                       inv_dy2 * (g * d_y[0] * dt + 2 * nud_y[0] * inv_d_y[ny - 1])) * _isWater;
+            _low[0] = (0.5 * inv_dy * (MUT(9) * -v[ny - 1] + MUT(10) * (2 * dnud_dy[0] * inv_d_y[ny - 1])) -
+                       inv_dy2 * (MUT(20) * (g * d_y[0] * dt) + MUT(10) * (2 * nud_y[0] * inv_d_y[ny - 1]))) * _isWater;
         }
         /* setSolverQyy_up, :183-204 */
         for (j = 0; j < ny - 1; j++) {
             double _isWater = isWater[j] && isWater[j + 1];
-            _up[j] = (0.5 * inv_dy * (v[j + 1] - 2 * dnud_dy[j] * inv_d_y[j + 1]) -
-                      inv_dy2 * (g * d_y[j] * dt + 2 * nud_y[j] * inv_d_y[j + 1])) * _isWater;
+            _up[j] = (0.5 * inv_dy * (MUT(9) * v[j + 1] - MUT(10) * (2 * dnud_dy[j] * inv_d_y[j + 1])) -
+                      inv_dy2 * (MUT(20) * (g * d_y[j] * dt) + MUT(10) * (2 * nud_y[j] * inv_d_y[j + 1]))) * _isWater;
         }
         {
             double _isWater = isWater[ny - 1] && isWater[0];
-            _up[ny - 1] = (0.5 * inv_dy * (v[0] - 2 * dnud_dy[ny - 1] * inv_d_y[0]) -
-                           inv_dy2 * (g * d_y[ny - 1] * dt + 2 * nud_y[ny - 1] * inv_d_y[0])) * _isWater;
+            _up[ny - 1] = (0.5 * inv_dy * (MUT(9) * v[0] - MUT(10) * (2 * dnud_dy[ny - 1] * inv_d_y[0])) -
+                           inv_dy2 * (MUT(20) * (g * d_y[ny - 1] * dt) + MUT(10) * (2 * nud_y[ny - 1] * inv_d_y[0]))) * _isWater;
         }
     }
 }
@@ -1001,15 +1026,15 @@ static void setSolverQyx(orc_hydro *H, int i)
             const int isl = !isWater[j];
             const double du_dx = 0.5 * inv_dx * ((u[j + ny + 1] + u[j + ny]) - (u[j + 1] + u[j]));
             const double u_y = 0.5 * (x2c_up * (u[ny + j + 1] + u[ny + j]) + x2c_d * (u[j + 1] + u[j]));
-            _d[j] = isWater[j] * ((1 + dt * dragTerm[j]) + dt * (u_y * dx_d + du_dx) -
-                                  dt * (inv_d[j] * nud[j] * d2x_d + inv_d[j] * dnud_dx[j] * dx_d)) + isl;
+            _d[j] = isWater[j] * ((1 + dt * dragTerm[j]) + MUT(13) * (dt * (u_y * dx_d + du_dx)) -
+                                  MUT(14) * (dt * (inv_d[j] * nud[j] * d2x_d + inv_d[j] * dnud_dx[j] * dx_d))) + isl;
         }
         {
             const int isl = !isWater[ny - 1];
             const double du_dx = 0.5 * inv_dx * ((u[ny] + u[2 * ny - 1]) - (u[0] + u[ny - 1]));
             const double u_y = 0.5 * (x2c_up * (u[ny] + u[2 * ny - 1]) + x2c_d * (u[0] + u[ny - 1]));
-            _d[ny - 1] = isWater[ny - 1] * ((1 + dt * dragTerm[ny - 1]) + dt * (u_y * dx_d + du_dx) -
-                                            dt * (inv_d[ny - 1] * nud[ny - 1] * d2x_d + inv_d[ny - 1] * dnud_dx[ny - 1] * dx_d)) +
+            _d[ny - 1] = isWater[ny - 1] * ((1 + dt * dragTerm[ny - 1]) + MUT(13) * (dt * (u_y * dx_d + du_dx)) -
+                                            MUT(14) * (dt * (inv_d[ny - 1] * nud[ny - 1] * d2x_d + inv_d[ny - 1] * dnud_dx[ny - 1] * dx_d))) +
                          isl;
         }
     }
@@ -1019,7 +1044,8 @@ static void setSolverQyx(orc_hydro *H, int i)
         const double d2x_up = H->dif2x_c_up[i];
         for (j = 0; j < ny; j++) {
             double _isWater = isWater[j] && isWater[j + ny];
-            _up[j] = _isWater * dt * (u[j] * dx_up - nud[j] * d2x_up * inv_d_y_up[j] - dnud_dx[j] * dx_up * inv_d_y_up[j]);
+            _up[j] = _isWater * dt * (MUT(13) * (u[j] * dx_up) - MUT(14) * (nud[j] * d2x_up * inv_d_y_up[j]) -
+                                      MUT(14) * (dnud_dx[j] * dx_up * inv_d_y_up[j]));
         }
     }
     { /* setSolverQyx_low, :313-338, and setSolverQyx0_low, :340-362 */
@@ -1029,11 +1055,13 @@ static void setSolverQyx(orc_hydro *H, int i)
         if (i > 0) {
             for (j = 0; j < ny; j++) {
                 double _isWater = isWater[j] && isWater[j - ny];
-                _low[j] = _isWater * dt * (u[j] * dx_low - nud[j] * d2x_low * inv_d_y_low[j] - dnud_dx[j] * dx_low * inv_d_y_low[j]);
+                _low[j] = _isWater * dt * (MUT(13) * (u[j] * dx_low) - MUT(14) * (nud[j] * d2x_low * inv_d_y_low[j]) -
+                                           MUT(14) * (dnud_dx[j] * dx_low * inv_d_y_low[j]));
             }
         } else {
             for (j = 0; j < ny; j++)
-                _low[j] = isWater[j] * dt * (u[j] * dx_low - nud[j] * d2x_low * inv_d_y_low[j] - dnud_dx[j] * dx_low * inv_d_y_low[j]);
+                _low[j] = isWater[j] * dt * (MUT(13) * (u[j] * dx_low) - MUT(14) * (nud[j] * d2x_low * inv_d_y_low[j]) -
+                                             MUT(14) * (dnud_dx[j] * dx_low * inv_d_y_low[j]));
         }
     }
 }

@@ -17,6 +17,19 @@
  * Layout: row-major, index i*ny+j; i is cross-shore (0 = offshore), j alongshore
  * (periodic). Fields with nx+1 rows (U, Qx, V, Qy) hold the offshore ghost in row 0
  * and the onshore wall in row nx.
+ *
+ * What pins it (the reference's Hydro sources cannot be compiled here, and the reference
+ * ships no expected outputs): its solvers equal the reference's own src/Math.cpp bit for
+ * bit (built into oracle/_ref); grid weights, masks, drag, viscosity and forcing stencils
+ * match closed forms; the composed step converges at second order to the continuous
+ * flux-form shallow-water equations and satisfies the exact semi-implicit gravity
+ * identities of its half-steps, and a build with any one of 22 assembly terms scaled by
+ * 1.1 (ORC_MUTANT) fails those checks (tests/test_oracle_pins.py,
+ * tests/test_oracle_consistency.py). Parity unpinned: the boundary and mask quirks no
+ * continuous equation determines -- the row-0 mask in the Qy predictor
+ * (src/HydroQy.cpp:140,170), the never-written mask row nx-1 (src/Hydro.cpp:91), the row-0
+ * masking asymmetry of the two lower diagonals (src/HydroQx.cpp:214-229,
+ * src/HydroQy.cpp:340-362) and the stale depth row nx (src/HydroQx.cpp:164-188).
  */
 #ifndef M70_ORACLE_H_
 #define M70_ORACLE_H_
