@@ -109,6 +109,20 @@ Hydro::~Hydro()
     delete[] dQx_dx; delete[] dQy_dy; delete[] Zs;
 }
 
+void Hydro::setHydroFromResumeFile(std::string resumeFile)
+{
+    const long F = (long)grid->size, ny = grid->ny;
+    const char *path = resumeFile.c_str();
+    struct { const char *var; double *dst; long n; } reads[] = {
+        {"U", U_c, F},          {"U0", U, ny},        {"Qx0", Qx, ny},       {"Qx_x", Qx + ny, F},
+        {"U_x", U + ny, F},     {"V", V_c, F},        {"V0", V, ny},         {"Qy0", Qy, ny},
+        {"Qy_y", Qy + ny, F},   {"V_y", V + ny, F},   {"Zs", Zs, F},         {"D", bathy->D, F},
+        {"dQx_dx", dQx_dx, F},  {"dQy_dy", dQy_dy, F}};
+    for (auto &r : reads)
+        if (m70nc_read(path, r.var, r.dst, r.n) != M70H_OK)
+            throw std::runtime_error(std::string("setHydroFromResumeFile: ") + m70nc_last_error());
+}
+
 void Hydro::setOffshoreF0Series(const std::vector<double> &f0)
 {
     check(m70h_set_offshore_f0_series(h_, f0.data(), (long)f0.size()));

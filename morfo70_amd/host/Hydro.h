@@ -100,6 +100,13 @@ public:
     /* Hydro::Run (include/Hydro.h:124). Throws std::runtime_error on a library error. */
     void Run(int nIter);
 
+    /* Hydro::setHydroFromResumeFile (include/Hydro.h:117; src/Hydro.cpp:204-227): the
+     * state fields of an m70_*.nc snapshot into the public arrays and bathy->D (U_c from
+     * "U", U from "U0" and "U_x", Qx from "Qx0" and "Qx_x", V_c, V, Qy alike, Zs, D,
+     * dQx_dx, dQy_dy). The next Run uploads them. Throws std::runtime_error when the
+     * file or a variable cannot be read. */
+    void setHydroFromResumeFile(std::string resumeFile);
+
     /* Time-varying offshore flux offset f0 (tide), one value per step from now on. */
     void setOffshoreF0Series(const std::vector<double> &f0);
 

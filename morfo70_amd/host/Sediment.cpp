@@ -2,6 +2,7 @@
 #include "Sediment.h"
 
 #include <stdexcept>
+#include <vector>
 
 namespace m70 {
 SedimentConfig &sedimentConfig()
@@ -56,6 +57,16 @@ void Sediment::updateBottom()
     pull();
     check(m70h_get_state(hydro->handle(), hydro->Zs, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                          nullptr, nullptr));
+}
+
+void Sediment::setSedimentFluxesFromResumeFile(std::string resumeFile)
+{
+    const long F = (long)grid->size;
+    std::vector<double> qx_in(F), qy_in(F);
+    if (m70nc_read(resumeFile.c_str(), "qx", qx_in.data(), F) != M70H_OK ||
+        m70nc_read(resumeFile.c_str(), "qy", qy_in.data(), F) != M70H_OK)
+        throw std::runtime_error(std::string("setSedimentFluxesFromResumeFile: ") + m70nc_last_error());
+    setSedimentFluxes(qx_in.data(), qy_in.data());
 }
 
 void Sediment::setSedimentFluxes(const double *qx_in, const double *qy_in)

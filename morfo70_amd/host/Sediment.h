@@ -48,8 +48,11 @@ public:
     void calculateSedimentFluxes();
     /* Sediment::updateBottom (include/Sediment.h:81). */
     void updateBottom();
-    /* The arithmetic of Sediment::setSedimentFluxesFromResumeFile (include/Sediment.h:87)
-     * on fluxes the caller has read (the NetCDF read stays in the reference). */
+    /* Sediment::setSedimentFluxesFromResumeFile (include/Sediment.h:60; src/Sediment.cpp:77-103):
+     * qx and qy of an m70_*.nc snapshot and the dh_dt they imply. Throws
+     * std::runtime_error when the file or a variable cannot be read. */
+    void setSedimentFluxesFromResumeFile(std::string resumeFile);
+    /* Its arithmetic alone, on fluxes the caller already holds. */
     void setSedimentFluxes(const double *qx_in, const double *qy_in);
 
 private:

@@ -6,15 +6,27 @@
 // does inside the bed evolution window (src/Morfo70.cpp:155-163):
 // calculateSedimentFluxes(), then updateBottom().
 //
+// With resume_runs > 0 it then writes the state as a resume file (the m70_*.nc variables
+// Hydro::setHydroFromResumeFile and Sediment::setSedimentFluxesFromResumeFile read,
+// src/Hydro.cpp:204-227, src/Sediment.cpp:77-103), builds a second Hydro (and Sediment)
+// from that file, and runs both the original and the resumed Hydro resume_runs more
+// times without sediment.
+//
 // Usage: facade_run <input.bin> <output.bin>
-//   input:  int32 nx, ny, runs, steps_per_run, drag, visc, bc, solver, sediment;
+//   input:  int32 nx, ny, runs, steps_per_run, drag, visc, bc, solver, sediment, resume_runs;
 //           float64 dy, zs0, dt, Dmin, alpha_f, cD0, nu0, M, ka, z0, alpha_r, bed_dt;
 //           x[nx]; Z[nx*ny]; h[nx*ny]; Sxx, Sxy, Syy, Urms, Dw, D_r, H [nx*ny each]
 //   output: Zs, D [nx*ny]; Qx, Qy, U, V [(nx+1)*ny]; U_c, V_c, dQx_dx, dQy_dy [nx*ny];
-//           with sediment: Alpha, gamma, Gamma, qx, qy, dh_dt, h, Zb [nx*ny each]
+//           with sediment: Alpha, gamma, Gamma, qx, qy, dh_dt, h, Zb [nx*ny each];
+//           with resume_runs > 0: the hydro fields above of the original, then of the
+//           resumed Hydro, after the extra runs; with sediment also qx, qy, dh_dt of the
+//           original Sediment when the file was written and of the resumed one just after
+//           reading it
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "Hydro.h"
@@ -23,6 +35,45 @@
 static void rd(FILE *f, void *p, size_t sz, size_t n)
 {
     if (fread(p, sz, n, f) != n) throw std::runtime_error("short read");
+}
+
+static void write_hydro(FILE *out, const Hydro &hy, const Bathymetry &b, size_t F, size_t ny)
+{
+    const size_t Fp = F + ny;
+    fwrite(hy.Zs, sizeof(double), F, out);
+    fwrite(b.D, sizeof(double), F, out);
+    fwrite(hy.Qx, sizeof(double), Fp, out);
+    fwrite(hy.Qy, sizeof(double), Fp, out);
+    fwrite(hy.U, sizeof(double), Fp, out);
+    fwrite(hy.V, sizeof(double), Fp, out);
+    fwrite(hy.U_c, sizeof(double), F, out);
+    fwrite(hy.V_c, sizeof(double), F, out);
+    fwrite(hy.dQx_dx, sizeof(double), F, out);
+    fwrite(hy.dQy_dy, sizeof(double), F, out);
+}
+
+// The resume variables of an m70_*.nc snapshot (src/Morfo70.cpp:225-276 writes them;
+// src/Hydro.cpp:204-227 and src/Sediment.cpp:77-103 read them).
+static void write_resume(const char *path, const Grid &grid, const Hydro &hy, const Bathymetry &b,
+                         const Sediment *sed)
+{
+    const int nx = grid.nx, ny = grid.ny;
+    std::vector<double> y(ny);
+    for (int j = 0; j < ny; ++j) y[j] = j * grid.dy;
+    std::vector<const char *> names = {"U", "U0", "U_x", "Qx0", "Qx_x", "V", "V0", "V_y", "Qy0", "Qy_y",
+                                       "Zs", "D", "dQx_dx", "dQy_dy"};
+    std::vector<const double *> data = {hy.U_c, hy.U, hy.U + ny, hy.Qx, hy.Qx + ny, hy.V_c, hy.V, hy.V + ny,
+                                        hy.Qy, hy.Qy + ny, hy.Zs, b.D, hy.dQx_dx, hy.dQy_dy};
+    std::vector<int> rows = {nx, 1, nx, 1, nx, nx, 1, nx, 1, nx, nx, nx, nx, nx};
+    if (sed) {
+        names.push_back("qx"); data.push_back(sed->qx); rows.push_back(nx);
+        names.push_back("qy"); data.push_back(sed->qy); rows.push_back(nx);
+    }
+    const char *att = "time";
+    const double t = 0.0;
+    if (m70nc_write(path, nx, ny, grid.x, y.data(), (int)names.size(), names.data(), data.data(), rows.data(), 1,
+                    &att, &t) != M70H_OK)
+        throw std::runtime_error(std::string("resume file: ") + m70nc_last_error());
 }
 
 int main(int argc, char **argv)
@@ -34,9 +85,9 @@ int main(int argc, char **argv)
     try {
         FILE *in = fopen(argv[1], "rb");
         if (!in) throw std::runtime_error("cannot open input");
-        int hdr[9];
-        rd(in, hdr, sizeof(int), 9);
-        const int nx = hdr[0], ny = hdr[1], runs = hdr[2], steps = hdr[3], with_sed = hdr[8];
+        int hdr[10];
+        rd(in, hdr, sizeof(int), 10);
+        const int nx = hdr[0], ny = hdr[1], runs = hdr[2], steps = hdr[3], with_sed = hdr[8], resume_runs = hdr[9];
         double sc[12];
         rd(in, sc, sizeof(double), 12);
         m70::sedimentConfig().dt = sc[11];
@@ -65,17 +116,34 @@ int main(int argc, char **argv)
         }
         FILE *out = fopen(argv[2], "wb");
         if (!out) throw std::runtime_error("cannot open output");
-        const size_t Fp = F + ny;
-        fwrite(hydro.Zs, sizeof(double), F, out);
-        fwrite(bathy.D, sizeof(double), F, out);
-        fwrite(hydro.Qx, sizeof(double), Fp, out);
-        fwrite(hydro.Qy, sizeof(double), Fp, out);
-        fwrite(hydro.U, sizeof(double), Fp, out);
-        fwrite(hydro.V, sizeof(double), Fp, out);
-        fwrite(hydro.U_c, sizeof(double), F, out);
-        fwrite(hydro.V_c, sizeof(double), F, out);
-        fwrite(hydro.dQx_dx, sizeof(double), F, out);
-        fwrite(hydro.dQy_dy, sizeof(double), F, out);
+        if (resume_runs > 0) {
+            const std::string path = std::string(argv[2]) + ".resume.nc";
+            write_resume(path.c_str(), grid, hydro, bathy, sed);
+            Bathymetry bathy2(Z.data(), h.data(), sc[1], &grid);   // the bed as the Sediment left it
+            std::copy(bathy.h, bathy.h + F, bathy2.h);
+            std::copy(bathy.Zb, bathy.Zb + F, bathy2.Zb);
+            Hydro hydro2(&grid, &bathy2, &wave);
+            hydro2.setHydroFromResumeFile(path);
+            std::vector<double> sed_before, sed_after;
+            if (sed) {
+                Sediment sed2(&hydro2, &wave, &bathy2, &grid);
+                sed2.setSedimentFluxesFromResumeFile(path);
+                for (const double *p : {sed->qx, sed->qy, sed->dh_dt}) sed_before.insert(sed_before.end(), p, p + F);
+                for (const double *p : {sed2.qx, sed2.qy, sed2.dh_dt}) sed_after.insert(sed_after.end(), p, p + F);
+            }
+            for (int r = 0; r < resume_runs; ++r) {
+                hydro.Run(steps);
+                hydro2.Run(steps);
+            }
+            write_hydro(out, hydro, bathy, F, ny);
+            write_hydro(out, hydro2, bathy2, F, ny);
+            fwrite(sed_before.data(), sizeof(double), sed_before.size(), out);
+            fwrite(sed_after.data(), sizeof(double), sed_after.size(), out);
+            delete sed;
+            fclose(out);
+            return 0;
+        }
+        write_hydro(out, hydro, bathy, F, ny);
         if (sed) {
             const double *sf[8] = {sed->Alpha, sed->gamma, sed->Gamma, sed->qx, sed->qy, sed->dh_dt, bathy.h, bathy.Zb};
             for (const double *p : sf) fwrite(p, sizeof(double), F, out);

@@ -160,6 +160,22 @@ class Hydro:
             self.pull()
         self._chk(rc)
 
+    def setHydroFromResumeFile(self, resumeFile: str):
+        """Hydro::setHydroFromResumeFile (include/Hydro.h:117; src/Hydro.cpp:204-227): the
+        state of an m70_*.nc snapshot into the public arrays and bathy.D; the next Run
+        uploads them."""
+        from . import netcdf
+        nx, ny = self.nx, self.ny
+        self.U_c = netcdf.read(resumeFile, "U", (nx, ny))
+        self.V_c = netcdf.read(resumeFile, "V", (nx, ny))
+        for full, row0, rest in (("U", "U0", "U_x"), ("Qx", "Qx0", "Qx_x"), ("V", "V0", "V_y"),
+                                 ("Qy", "Qy0", "Qy_y")):
+            setattr(self, full, np.vstack([netcdf.read(resumeFile, row0, (1, ny)),
+                                           netcdf.read(resumeFile, rest, (nx, ny))]))
+        for k in ("Zs", "dQx_dx", "dQy_dy"):
+            setattr(self, k, netcdf.read(resumeFile, k, (nx, ny)))
+        self.bathy.D[...] = netcdf.read(resumeFile, "D", (nx, ny))
+
     # -- device-resident use (benchmarks, batched drivers) --------------------------
     def run_async(self, nIter: int):
         self._chk(self._L.m70h_run_async(self._h, int(nIter)))

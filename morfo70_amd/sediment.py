@@ -101,6 +101,13 @@ class Sediment:
             self.pull()
             self.hydro.pull()
 
+    def setSedimentFluxesFromResumeFile(self, resumeFile: str, pull: bool = True):
+        """Sediment::setSedimentFluxesFromResumeFile (include/Sediment.h:60;
+        src/Sediment.cpp:77-103): qx, qy of an m70_*.nc snapshot and the dh_dt they imply."""
+        from . import netcdf
+        shape = (self.hydro.nx, self.hydro.ny)
+        self.setSedimentFluxes(netcdf.read(resumeFile, "qx", shape), netcdf.read(resumeFile, "qy", shape), pull)
+
     def setSedimentFluxes(self, qx, qy, pull: bool = True):
         """The arithmetic of Sediment::setSedimentFluxesFromResumeFile (src/Sediment.cpp:77-103)
         on given qx, qy (the NetCDF read stays with the caller)."""

@@ -16,11 +16,11 @@ from tests.helpers import FIELDS, compare_states, nonuniform_beach
 pytestmark = pytest.mark.gpu
 
 
-def write_input(path, sc, runs, steps, solver, sediment=0, bed_dt=0.1):
+def write_input(path, sc, runs, steps, solver, sediment=0, bed_dt=0.1, resume_runs=0):
     ph = sc.physics
     with open(path, "wb") as f:
-        np.array([sc.nx, sc.ny, runs, steps, ph.drag_model, ph.visc_model, ph.offshore_bc, solver, sediment],
-                 dtype=np.int32).tofile(f)
+        np.array([sc.nx, sc.ny, runs, steps, ph.drag_model, ph.visc_model, ph.offshore_bc, solver, sediment,
+                  resume_runs], dtype=np.int32).tofile(f)
         np.array([sc.dy, sc.zs0, ph.dt, ph.Dmin, ph.alpha_f, ph.cD0, ph.nu0, ph.M, ph.ka, ph.z0, ph.alpha_r,
                   bed_dt], dtype=np.float64).tofile(f)
         for a in [sc.x, sc.Z, sc.h] + [sc.forcing[k] for k in ("Sxx", "Sxy", "Syy", "Urms", "Dw", "D_r", "H")]:
@@ -30,11 +30,16 @@ def write_input(path, sc, runs, steps, solver, sediment=0, bed_dt=0.1):
 SED_FIELDS = ("Alpha", "gamma", "Gamma", "qx", "qy", "dh_dt", "h", "Zb")
 
 
-def read_output(path, nx, ny, sediment=False):
+def read_output(path, nx, ny, sediment=False, resume=False):
     raw = np.fromfile(path, dtype=np.float64)
     out, off = {}, 0
-    for k in FIELDS + (SED_FIELDS if sediment else ()):
-        rows = nx + 1 if k in ("Qx", "Qy", "U", "V") else nx
+    names = FIELDS + (SED_FIELDS if sediment else ())
+    if resume:
+        names = tuple(f"cont_{k}" for k in FIELDS) + tuple(f"res_{k}" for k in FIELDS)
+        if sediment:
+            names += ("cont_qx", "cont_qy", "cont_dh_dt", "res_qx", "res_qy", "res_dh_dt")
+    for k in names:
+        rows = nx + 1 if k.split("_")[-1] in ("Qx", "Qy", "U", "V") and not k.endswith("_dx") else nx
         out[k] = raw[off:off + rows * ny].reshape(rows, ny)
         off += rows * ny
     assert off == raw.size
@@ -84,3 +89,24 @@ def test_facade_sediment_outer_loop(tmp_path):
     for k, r in res.items():
         assert r["nan"] == 0 and r["rel"] <= 1e-10, (k, r)
     assert np.max(np.abs(got["dh_dt"])) > 0
+
+
+def test_facade_resume_continues_bit_for_bit(tmp_path):
+    """Hydro::setHydroFromResumeFile and Sediment::setSedimentFluxesFromResumeFile on the
+    facades (include/Hydro.h:117, include/Sediment.h:60): after three Runs with sediment,
+    the state is written as a resume file, a fresh Hydro and Sediment read it, and two
+    more Runs of the original and of the resumed Hydro agree bit for bit; the resumed
+    Sediment holds the original's qx, qy and the dh_dt they imply."""
+    ph = S.Physics(drag_model=S.DRAG["constant"], visc_model=S.VISC["constant"], dt=0.25)
+    sc = nonuniform_beach(nx=50, ny=44, physics=ph, seed=23)
+    inp, outp = tmp_path / "in.bin", tmp_path / "out.bin"
+    write_input(inp, sc, runs=3, steps=5, solver=2, sediment=1, bed_dt=2.0, resume_runs=2)
+    print("[facade] resume 50x44 3x(5 steps + bed), then 2x5 steps", flush=True)
+    subprocess.run([FACADE, str(inp), str(outp)], check=True, timeout=120)
+    got = read_output(outp, sc.nx, sc.ny, sediment=True, resume=True)
+    for k in FIELDS:
+        assert np.array_equal(got[f"res_{k}"], got[f"cont_{k}"]), k
+    assert np.max(np.abs(got["cont_Qx"])) > 0
+    for k in ("qx", "qy", "dh_dt"):
+        assert np.array_equal(got[f"res_{k}"][1:], got[f"cont_{k}"][1:]), k
+    assert np.max(np.abs(got["cont_qx"])) > 0

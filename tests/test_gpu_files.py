@@ -94,3 +94,44 @@ def test_hydro_on_reference_bathymetry():
     res = compare_states(g.state(), o.get_state())
     for k, r in res.items():
         assert r["nan"] == 0 and r["rel"] <= 1e-10, (k, r)
+
+
+def test_python_mirror_resume_members(tmp_path):
+    """Hydro.setHydroFromResumeFile and Sediment.setSedimentFluxesFromResumeFile of the
+    Python mirror (include/Hydro.h:117, include/Sediment.h:60): a fresh Hydro read from a
+    file of the running one's state continues bit for bit; the fresh Sediment holds the
+    file's fluxes and the dh_dt they imply, equal to the running Sediment's."""
+    from morfo70_amd import _lib, netcdf
+    from morfo70_amd.hydro import from_scenario
+    from morfo70_amd.sediment import Sediment
+    from tests.helpers import nonuniform_beach
+    sc = nonuniform_beach(nx=40, ny=36, steps=10, seed=31)
+    a = from_scenario(sc, solver=_lib.SOLVER_FUSED)
+    sa = Sediment(a)
+    a.Run(4)
+    sa.calculateSedimentFluxes()
+    sa.updateBottom(2.0)
+    st = a.state()
+    path = str(tmp_path / "m70_resume.nc")
+    y = np.arange(sc.ny) * sc.dy
+    netcdf.write(path, sc.x, y, {"U": st["U_c"], "U0": st["U"][0], "U_x": st["U"][1:], "Qx0": st["Qx"][0],
+                                 "Qx_x": st["Qx"][1:], "V": st["V_c"], "V0": st["V"][0], "V_y": st["V"][1:],
+                                 "Qy0": st["Qy"][0], "Qy_y": st["Qy"][1:], "Zs": st["Zs"], "D": st["D"],
+                                 "dQx_dx": st["dQx_dx"], "dQy_dy": st["dQy_dy"], "qx": sa.qx, "qy": sa.qy},
+                 {"time": 0.0})
+    b = from_scenario(sc, solver=_lib.SOLVER_FUSED)
+    b.setHydroFromResumeFile(path)
+    sb = Sediment(b)
+    sb.setSedimentFluxesFromResumeFile(path)
+    for k in ("qx", "qy"):
+        assert np.array_equal(getattr(sb, k), getattr(sa, k)), k
+    assert np.array_equal(sb.dh_dt[1:], sa.dh_dt[1:])
+    sb.close()
+    sa.close()
+    a.Run(5)
+    b.Run(5)
+    sa_, sb_ = a.state(), b.state()
+    for k in sa_:
+        assert np.array_equal(sb_[k], sa_[k]), k
+    a.close()
+    b.close()
