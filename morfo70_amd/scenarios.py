@@ -285,8 +285,24 @@ def seeded_beach(nx: int, ny: int, dx: float, dy: float, seed: int, steps: int,
     if tide:
         sc.f0_series = tide_f0_series(steps, dt, dx)  # uniform grid: dx_x[0] = dx
     sc.meta = dict(seed=seed, slope=slope, bar_amp=amp, wavelength=lam, Hrms=Hrms, T=T,
-                   theta=theta, tide=tide)
+                   theta=theta, tide=tide, bar_centre=centre, bar_width=30.0, bar_mod=0.3, phase=phase,
+                   x_shore=x[-1] - 0.08 * x[-1])
     return sc
+
+
+def bar_field(sc: Scenario, amp: float, centre: float):
+    """The bar of a seeded beach with amplitude ``amp`` (m) and crest ``centre`` metres
+    seaward of the shoreline: amp (1 + mod sin(2 pi y / wavelength + phase))
+    exp(-((s - centre)/width)^2), s the distance seaward of the shoreline (the bar of
+    _bar_beach). Returns (h, dh/d amp, dh/d centre), each (nx, ny)."""
+    m = sc.meta
+    s = (m["x_shore"] - sc.x)[:, None]
+    y = (np.arange(sc.ny) * sc.dy)[None, :]
+    shape = (1.0 + m["bar_mod"] * np.sin(2.0 * math.pi * y / m["wavelength"] + m["phase"]))
+    g = np.exp(-((s - centre) / m["bar_width"]) ** 2)
+    h = amp * shape * g
+    return (np.ascontiguousarray(h), np.ascontiguousarray(shape * g),
+            np.ascontiguousarray(h * 2.0 * (s - centre) / m["bar_width"] ** 2))
 
 
 def config2(steps: int = 2000, seed: int = 2, nx: int = 2048, ny: int = 4096) -> Scenario:
@@ -321,9 +337,11 @@ def config4(seed: int = 4, steps: int = 400, nx: int = 512, ny: int = 1024, dx: 
 def config5(count: int = 16, seed: int = 5, steps: int = 300, nx_range=(256, 1024), ny_range=(512, 2048),
             dx: float = 2.0, noise: float = 0.02):
     """Batched inversion set-up: ``count`` seeded beaches with nx ~ U[nx_range],
-    ny ~ U[ny_range] (Hrms U[0.5, 1.5] m, T U[6, 12] s, angle U[-20, 20] deg), each
-    with a true bar field h + dh (dh smooth, 0.2 m) and the noise level of its
-    observations. Returns (scenarios, dhs, rng)."""
+    ny ~ U[ny_range] (Hrms U[0.5, 1.5] m, T U[6, 12] s, angle U[-20, 20] deg). The
+    scenario's own bar is the first guess; the truth is that bar perturbed by a seed, its
+    amplitude scaled by U[0.75, 1.25] and its crest moved by U[-12, 12] m (sc.meta
+    true_amp, true_centre). Observations carry N(0, noise) noise. Returns (scenarios, dhs,
+    rng) with dhs the true bar minus the first guess."""
     rng = np.random.default_rng(seed)
     scs, dhs = [], []
     for k in range(count):
@@ -332,9 +350,11 @@ def config5(count: int = 16, seed: int = 5, steps: int = 300, nx_range=(256, 102
         sc = seeded_beach(nx, ny, dx, dx, seed * 1000 + k, steps, name=f"inv{k}_{nx}x{ny}")
         Hrms, T, theta = rng.uniform(0.5, 1.5), rng.uniform(6.0, 12.0), rng.uniform(-20.0, 20.0)
         sc.forcing = wave_forcing(sc.initial_depth(), sc.x, Hrms, T, theta, sc.physics.Dmin)
-        sc.meta.update(Hrms=Hrms, T=T, theta=theta)
+        true_amp = sc.meta["bar_amp"] * rng.uniform(0.75, 1.25)
+        true_centre = sc.meta["bar_centre"] + rng.uniform(-12.0, 12.0)
+        sc.meta.update(Hrms=Hrms, T=T, theta=theta, true_amp=true_amp, true_centre=true_centre, noise=noise)
         scs.append(sc)
-        dhs.append(smooth_field(nx, ny, rng, 0.2))
+        dhs.append(np.ascontiguousarray(bar_field(sc, true_amp, true_centre)[0] - sc.h))
     return scs, dhs, rng
 
 
