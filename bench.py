@@ -577,9 +577,9 @@ def bench_adjoint(args):
 def bench_inversion(args):
     """16 uneven beaches (nx U[256,1024], ny U[512,2048]), 300 steps each: recovery of each
     scenario's seeded perturbed bar (amplitude and crest position) from noisy final-surface
-    observations. One iteration = a checkpointed forward + reverse sweep for every
-    scenario still moving (concurrent, one stream each) and the plain forward Runs of its
-    line search; the misfits reported are the plain forward's."""
+    observations. One iteration = the checkpointed forward + reverse sweeps of every
+    scenario still moving, as grouped launches over all of them, and the plain forward
+    Runs of its line search; the misfits reported are the plain forward's."""
     import torch
     from morfo70_amd import scenarios as S
     from morfo70_amd.inversion import BarInversion
@@ -599,10 +599,15 @@ def bench_inversion(args):
               for sc, (a0, c0) in zip(scs, bi.theta0)]
     L_true = bi.plain_losses(range(len(scs)), z_true)
     zs0 = [np.zeros(2)] * len(scs)
+    h0 = [bi.h_of(q, zs0[q])[0] for q in range(len(scs))]
     bi.gradients(zs0)                      # warm-up: allocations
+    bi.inv.loss_and_grad(h0, grouped=False)
     t0 = time.perf_counter()
-    bi.inv.loss_and_grad([bi.h_of(q, zs0[q])[0] for q in range(len(scs))], concurrent=False)
+    bi.inv.loss_and_grad(h0, grouped=False, concurrent=False)
     t_seq_grad = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    bi.inv.loss_and_grad(h0, grouped=False, concurrent=True)
+    t_thr_grad = time.perf_counter() - t0
     t0 = time.perf_counter()
     bi.gradients(zs0)
     t_grad = time.perf_counter() - t0
@@ -630,7 +635,8 @@ def bench_inversion(args):
         "dtype": "f64", "data": "synthetic (16 seeded beaches, seeded perturbed bars, N(0, 0.02 m) noise)",
         "config": {"workload": "inversion16", "scenarios": len(scs), "cells_total": cells, "steps": n,
                    "checkpoint_buffers": slots, "iterations": args.gd_iters, "threads": bi.inv.workers},
-        "gradient_s_all_scenarios": t_grad, "gradient_s_one_after_another": t_seq_grad,
+        "gradient_s_grouped": t_grad, "gradient_s_threads_and_streams": t_thr_grad,
+        "gradient_s_one_after_another": t_seq_grad, "grouped_over_one_after_another": t_seq_grad / t_grad,
         "gradient_cell_steps_per_s": cells * n / t_grad, "fit_total_s": t_fit,
         "iterations_run": len(hist) - 1,
         "bar_rms_error_ratio": {"median": float(np.median(ratio)), "max": float(np.max(ratio)),

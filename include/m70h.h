@@ -447,6 +447,25 @@ int m70h_adjoint_reverse(m70h *h, const m70h_fields *seed_dev, const m70h_fields
 int m70h_adjoint_stats(const m70h *h, long *forward_steps, long *taped_steps, int *buffers);
 int m70h_adjoint_release(m70h *h);
 
+/* The adjoint of Run for many handles at once (no counterpart in the reference): every
+ * operation of the checkpointed forward and reverse sweeps -- state copies, untaped steps,
+ * taped steps and reverse steps -- is one launch per stage over all handles, as the
+ * forward batch (m70h_batch_*) advances them. The handles share the Run length and the
+ * checkpoint budget (so one binomial schedule), and each keeps its own checkpoints and
+ * adjoint buffers, as with m70h_adjoint_forward. Gradients equal each handle's own
+ * m70h_adjoint_reverse to rounding (the reverse adds through atomics). Requirements: the
+ * batch requirements, nx-1 up to 2048, default solver tiles, no offshore offset series,
+ * slots >= 1. seeds_dev and grads_dev hold one m70h_fields per handle (NULL: zero seeds /
+ * no gradients); dparams (host, may be NULL) receives 3 doubles per handle. Destroy the
+ * batch before its handles; the handles' buffers stay with them (m70h_adjoint_release). */
+typedef struct m70h_adjoint_batch m70h_adjoint_batch;
+int m70h_adjoint_batch_create(int nscen, m70h *const *handles, m70h_adjoint_batch **out);
+void m70h_adjoint_batch_destroy(m70h_adjoint_batch *b);
+const char *m70h_adjoint_batch_last_error(const m70h_adjoint_batch *b);
+int m70h_adjoint_batch_forward(m70h_adjoint_batch *b, int nIter, int slots);
+int m70h_adjoint_batch_reverse(m70h_adjoint_batch *b, const m70h_fields *seeds_dev, const m70h_fields *grads_dev,
+                               double *dparams);
+
 #ifdef __cplusplus
 }
 #endif

@@ -108,6 +108,11 @@ SIGNATURES = {
     "m70h_adjoint_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long),
                                           ctypes.POINTER(ctypes.c_int)]),
     "m70h_adjoint_release": (ctypes.c_int, [_vp]),
+    "m70h_adjoint_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "m70h_adjoint_batch_destroy": (None, [_vp]),
+    "m70h_adjoint_batch_last_error": (ctypes.c_char_p, [_vp]),
+    "m70h_adjoint_batch_forward": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "m70h_adjoint_batch_reverse": (ctypes.c_int, [_vp, _vp, _vp, _dp]),
 }
 
 STATE_FIELDS = ("Zs", "D", "Qx", "Qy", "U", "V", "U_c", "V_c", "dQx_dx", "dQy_dy")

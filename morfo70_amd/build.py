@@ -13,7 +13,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libm70h.so")
 
-SOURCES = ["hydro_kernels.hip", "line_solvers.hip", "partition_solver.hip", "adjoint.hip", "sediment.hip",
+SOURCES = ["hydro_kernels.hip", "line_solvers.hip", "partition_solver.hip", "adjoint.hip", "adjoint_batch.hip", "sediment.hip",
            "wave.hip", "outer.hip", "netcdf3.cpp", "m70h_api.hip"]
 
 # -ffp-contract=off: no fused multiply-adds, so every product and sum rounds as the

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/m70h.h"
+#include "adjoint_batch.h"
 #include "adjoint_kernels.h"
 #include "hydro_device.h"
 #include "hydro_kernels.h"
@@ -1595,6 +1596,464 @@ int m70h_adjoint_release(m70h *h)
 {
     if (!h) return M70H_EINVAL;
     adj_release(h);
+    return M70H_OK;
+}
+
+}  // extern "C"
+
+// =====================================================================================
+// Grouped adjoint of many handles (m70h_adjoint_batch_*): every handle keeps its own
+// checkpoints, tape and adjoint buffers (its AdjCtx), all handles share one Run length and
+// checkpoint budget and so one binomial schedule, and every operation of that schedule
+// (state copies, untaped steps, taped steps, reverse steps) is one launch per stage over
+// all handles. Untaped steps and the per-Run pre-loop are the forward batch's grouped
+// launches; the rest read a table of per-handle descriptors (AdjScen) that a pinned ring
+// stages to the device ahead of each operation.
+// =====================================================================================
+namespace {
+enum { GA_NXP1 = 0, GA_NX = 1, GA_NX1 = 2, GA_COLS = 3, GA_ROWS = 4, GA_COUNT = 5 };
+}
+
+struct m70h_adjoint_batch {
+    static constexpr int RING = 64;
+    std::vector<m70h *> hs;
+    m70h_batch *fb = nullptr;        // forward batch: per-Run pre-loop and untaped steps
+    hipStream_t stream = nullptr;    // the forward batch's stream
+    int rb = 0;                      // row-solver block size
+    int *d_start = nullptr;          // [GA_COUNT][n+1] first block of each handle per stage
+    int blocks[GA_COUNT] = {};
+    AdjScen *d_ring = nullptr, *h_ring = nullptr;   // RING tables of n descriptors
+    hipEvent_t ev[RING] = {};
+    bool ev_used[RING] = {};
+    long ring_pos = 0;
+    int N = 0, S = 0;
+    int parity = 0;                  // D/Dn swaps since the forward batch's tables were uploaded
+    int cur = 0;                     // A[cur] of every handle holds the adjoint of the current state
+    bool recorded = false;
+    std::string err;
+};
+
+namespace {
+
+int abfail(m70h_adjoint_batch *b, int code, const std::string &msg)
+{
+    if (b) b->err = msg; else g_create_error = msg;
+    return code;
+}
+
+#define ABCHK(b, expr)                                                                       \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) return abfail(b, M70H_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// Which state a copy reads or writes.
+enum { SEL_STATE, SEL_SLOT, SEL_S0BUF, SEL_FINAL };
+State ab_state(m70h *h, int sel, int q)
+{
+    switch (sel) {
+    case SEL_SLOT: return buf_state(h, h->adj->slot[q]);
+    case SEL_S0BUF: return buf_state(h, h->adj->s0buf);
+    case SEL_FINAL: return buf_state(h, h->adj->final_buf);
+    default: return h->s;
+    }
+}
+
+// Stages a table of descriptors (fill(handle index, descriptor) on top of the defaults).
+template <class Fill>
+int ab_table(m70h_adjoint_batch *b, Fill fill, const AdjScen **out)
+{
+    const int n = (int)b->hs.size();
+    const int r = (int)(b->ring_pos++ % m70h_adjoint_batch::RING);
+    if (b->ev_used[r]) ABCHK(b, hipEventSynchronize(b->ev[r]));   // its previous copy has left the host buffer
+    AdjScen *host = b->h_ring + (size_t)r * n;
+    for (int q = 0; q < n; ++q) {
+        m70h *h = b->hs[q];
+        AdjCtx *a = h->adj;
+        AdjScen &d = host[q];
+        d = AdjScen{};
+        d.k = h->k; d.m = h->m; d.rf = h->rf;
+        d.s = h->s; d.S0 = h->s; d.dst = h->s; d.src = h->s;
+        d.T = a->T; d.w = h->w; d.W = a->W; d.R = a->R;
+        d.Ain = a->A[b->cur]; d.Aout = a->A[1 - b->cur];
+        d.f0 = h->f0;
+        d.S_cols = adj_cols_segments(h->nx - 1);
+        d.L_cols = d.S_cols > 0 ? 1024 / d.S_cols : 0;
+        d.S_rows = adj_rows_segments(h->ny, b->rb);
+        fill(q, d);
+    }
+    AdjScen *dev = b->d_ring + (size_t)r * n;
+    ABCHK(b, hipMemcpyAsync(dev, host, (size_t)n * sizeof(AdjScen), hipMemcpyHostToDevice, b->stream));
+    ABCHK(b, hipEventRecord(b->ev[r], b->stream));
+    b->ev_used[r] = true;
+    *out = dev;
+    return M70H_OK;
+}
+
+const int *ab_start(m70h_adjoint_batch *b, int stage) { return b->d_start + (size_t)stage * (b->hs.size() + 1); }
+
+int ab_cells(m70h_adjoint_batch *b, int op, const AdjScen *D)
+{
+    const int stage = op == AGC_COPY || op == AGC_ZERO ? GA_NXP1 : (agc_rows(op, 4) == 4 ? GA_NX : GA_NX1);
+    ABCHK(b, launch_adj_cells_grouped(b->stream, op, D, ab_start(b, stage), (int)b->hs.size(), b->blocks[stage]));
+    return M70H_OK;
+}
+int ab_cols(m70h_adjoint_batch *b, int sys, int trans, const AdjScen *D)
+{
+    ABCHK(b, launch_adj_cols_grouped(b->stream, sys, trans, D, ab_start(b, GA_COLS), (int)b->hs.size(),
+                                     b->blocks[GA_COLS]));
+    return M70H_OK;
+}
+int ab_rows(m70h_adjoint_batch *b, int sys, int trans, const AdjScen *D)
+{
+    ABCHK(b, launch_adj_rows_grouped(b->stream, b->rb, sys, trans, D, ab_start(b, GA_ROWS), (int)b->hs.size(),
+                                     b->blocks[GA_ROWS]));
+    return M70H_OK;
+}
+
+// dst <- src for every handle.
+int ab_copy(m70h_adjoint_batch *b, int dsel, int dq, int ssel, int sq)
+{
+    const AdjScen *D;
+    int rc = ab_table(b, [&](int q, AdjScen &d) {
+        d.dst = ab_state(b->hs[q], dsel, dq);
+        d.src = ab_state(b->hs[q], ssel, sq);
+    }, &D);
+    if (rc) return rc;
+    return ab_cells(b, AGC_COPY, D);
+}
+
+void ab_swap_depths(m70h_adjoint_batch *b)
+{
+    for (m70h *h : b->hs) std::swap(h->s.D, h->s.Dn);
+    b->parity ^= 1;
+}
+
+// Uploads the forward batch's tables for the handles' current states (parity 0 from here).
+int ab_sync_forward_tables(m70h_adjoint_batch *b)
+{
+    int rc = batch_upload(b->fb);
+    if (rc) return abfail(b, rc, b->fb->err);
+    b->parity = 0;
+    return M70H_OK;
+}
+
+// n untaped steps of every handle (the forward batch's grouped step).
+int ab_advance(m70h_adjoint_batch *b, int n)
+{
+    for (int t = 0; t < n; ++t) {
+        ABCHK(b, batch_step(b->fb, b->parity));
+        ab_swap_depths(b);
+        for (m70h *h : b->hs) h->adj->fwd_steps++;
+    }
+    return M70H_OK;
+}
+
+// The checkpoint buffer every handle holds `step` in (all handles keep the same layout).
+int ab_find(m70h_adjoint_batch *b, int step) { return adj_find(b->hs[0], step); }
+
+int ab_store(m70h_adjoint_batch *b, int step)
+{
+    AdjCtx *a0 = b->hs[0]->adj;
+    for (size_t q = 0; q < a0->slot.size(); ++q)
+        if (a0->slot_step[q] < 0) {
+            int rc = ab_copy(b, SEL_SLOT, (int)q, SEL_STATE, 0);
+            if (rc) return rc;
+            for (m70h *h : b->hs) h->adj->slot_step[q] = step;
+            return M70H_OK;
+        }
+    return abfail(b, M70H_ESTATE, "adjoint batch: no free checkpoint buffer (schedule error)");
+}
+
+// One taped step t of every handle from its current state (as adj_taped_step).
+int ab_taped_step(m70h_adjoint_batch *b, int t)
+{
+    const AdjScen *D;
+    int rc = ab_table(b, [&](int q, AdjScen &d) {
+        m70h *h = b->hs[q];
+        d.f0 = f0_at(h, h->adj->f0_pos0 + t);
+    }, &D);
+    if (rc) return rc;
+    if ((rc = ab_cells(b, AGC_QXX_ASM, D)) || (rc = ab_cols(b, 1, 0, D)) || (rc = ab_cells(b, AGC_QXY_ASM, D)) ||
+        (rc = ab_rows(b, 2, 0, D)) || (rc = ab_cells(b, AGC_QX_TAIL, D)) || (rc = ab_cells(b, AGC_QYY_ASM, D)) ||
+        (rc = ab_rows(b, 3, 0, D)) || (rc = ab_cells(b, AGC_QYX_ASM, D)) || (rc = ab_cols(b, 4, 0, D)) ||
+        (rc = ab_cells(b, AGC_QY_TAIL, D)))
+        return rc;
+    ab_swap_depths(b);
+    for (m70h *h : b->hs) h->adj->taped_steps++;
+    return M70H_OK;
+}
+
+// Reverse kernels of step t (as adj_reverse_kernels): the current states hold s_{t+1}, the
+// buffers selected by s0sel/s0q hold s_t.
+int ab_reverse_kernels(m70h_adjoint_batch *b, int t, int s0sel, int s0q)
+{
+    const AdjScen *D;
+    int rc = ab_table(b, [&](int q, AdjScen &d) {
+        m70h *h = b->hs[q];
+        d.S0 = ab_state(h, s0sel, s0q);
+        d.f0 = f0_at(h, h->adj->f0_pos0 + t);
+    }, &D);
+    if (rc) return rc;
+    if ((rc = ab_cells(b, AGC_ZERO, D)) || (rc = ab_cells(b, AGC_QY_TAIL_A, D)) || (rc = ab_cells(b, AGC_QY_TAIL_B, D)) ||
+        (rc = ab_cols(b, 4, 1, D)) || (rc = ab_cells(b, AGC_QYX, D)) || (rc = ab_rows(b, 3, 1, D)) ||
+        (rc = ab_cells(b, AGC_QYY, D)) || (rc = ab_cells(b, AGC_QX_TAIL_A, D)) || (rc = ab_cells(b, AGC_QX_TAIL_B, D)) ||
+        (rc = ab_rows(b, 2, 1, D)) || (rc = ab_cells(b, AGC_QXY, D)) || (rc = ab_cols(b, 1, 1, D)) ||
+        (rc = ab_cells(b, AGC_QXX, D)))
+        return rc;
+    b->cur = 1 - b->cur;
+    return M70H_OK;
+}
+
+int ab_reverse_one(m70h_adjoint_batch *b, int t, int s0sel, int s0q)
+{
+    int rc = ab_taped_step(b, t);
+    if (rc) return rc;
+    return ab_reverse_kernels(b, t, s0sel, s0q);
+}
+
+// Steps a0..b0-1 in reverse (as adj_reverse_range): step a0's state is in a checkpoint
+// buffer and s buffers are free.
+int ab_reverse_range(m70h_adjoint_batch *b, int a0, int b0, int s)
+{
+    const int qa = ab_find(b, a0);
+    if (qa < 0) return abfail(b, M70H_ESTATE, "adjoint batch: missing checkpoint");
+    int rc;
+    if (b0 - a0 == 1) {
+        if ((rc = ab_copy(b, SEL_STATE, 0, SEL_SLOT, qa))) return rc;
+        return ab_reverse_one(b, a0, SEL_SLOT, qa);
+    }
+    if (s == 0) {
+        for (int t = b0 - 1; t >= a0; --t) {
+            if ((rc = ab_copy(b, SEL_STATE, 0, SEL_SLOT, qa))) return rc;
+            if (t == a0) {
+                if ((rc = ab_reverse_one(b, t, SEL_SLOT, qa))) return rc;
+            } else {
+                if ((rc = ab_advance(b, t - a0))) return rc;
+                if ((rc = ab_copy(b, SEL_S0BUF, 0, SEL_STATE, 0))) return rc;
+                if ((rc = ab_reverse_one(b, t, SEL_S0BUF, 0))) return rc;
+            }
+        }
+        return M70H_OK;
+    }
+    AdjCtx *a0c = b->hs[0]->adj;
+    const int j = a0c->split[(size_t)(b0 - a0) * (a0c->S + 1) + s];
+    const int m = a0 + j;
+    if (ab_find(b, m) < 0) {
+        if ((rc = ab_copy(b, SEL_STATE, 0, SEL_SLOT, qa))) return rc;
+        if ((rc = ab_advance(b, j))) return rc;
+        if ((rc = ab_store(b, m))) return rc;
+    }
+    if ((rc = ab_reverse_range(b, m, b0, s - 1))) return rc;
+    const int qm = ab_find(b, m);
+    for (m70h *h : b->hs) h->adj->slot_step[qm] = -1;
+    return ab_reverse_range(b, a0, m, s);
+}
+
+int ab_sync_handles(m70h_adjoint_batch *b)
+{
+    for (m70h *h : b->hs) ABCHK(b, hipStreamSynchronize(h->stream));
+    return M70H_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *m70h_adjoint_batch_last_error(const m70h_adjoint_batch *b)
+{
+    return b ? b->err.c_str() : g_create_error.c_str();
+}
+
+void m70h_adjoint_batch_destroy(m70h_adjoint_batch *b)
+{
+    if (!b) return;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto &e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->d_ring) (void)hipFree(b->d_ring);
+    if (b->h_ring) (void)hipHostFree(b->h_ring);
+    if (b->d_start) (void)hipFree(b->d_start);
+    m70h_batch_destroy(b->fb);
+    delete b;
+}
+
+int m70h_adjoint_batch_create(int nscen, m70h *const *handles, m70h_adjoint_batch **out)
+{
+    if (!out) return abfail(nullptr, M70H_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (nscen < 1 || !handles) return abfail(nullptr, M70H_EINVAL, "need at least one handle");
+    for (int q = 0; q < nscen; ++q) {
+        m70h *h = handles[q];
+        if (!h) return abfail(nullptr, M70H_EINVAL, "NULL handle");
+        if (!h->f0_series.empty())
+            return abfail(nullptr, M70H_EINVAL, "adjoint batch: handles with an offshore offset series are not supported");
+        if (h->tile_cols != 0 || h->tile_rows != 0)
+            return abfail(nullptr, M70H_EINVAL, "adjoint batch: handles must use the default solver tiles");
+        if (adj_cols_segments(h->nx - 1) <= 0)
+            return abfail(nullptr, M70H_EINVAL, "adjoint batch: nx-1 must be at most 2048");
+    }
+    m70h_adjoint_batch *b = new m70h_adjoint_batch();
+    auto bail = [&](int rc) { g_create_error = b->err; m70h_adjoint_batch_destroy(b); return rc; };
+    int rc = m70h_batch_create(nscen, handles, &b->fb);
+    if (rc) return bail(abfail(b, rc, g_create_error));
+    b->stream = b->fb->stream;
+    b->rb = b->fb->rows_block;
+    b->hs.assign(handles, handles + nscen);
+    std::vector<int> starts((size_t)GA_COUNT * (nscen + 1), 0);
+    for (int q = 0; q < nscen; ++q) {
+        m70h *h = handles[q];
+        const int gx = (h->ny + 255) / 256;
+        const int Sc = adj_cols_segments(h->nx - 1), Lc = 1024 / Sc;
+        if (adj_rows_segments(h->ny, b->rb) <= 0) return bail(abfail(b, M70H_EINVAL, "adjoint batch: ny too large"));
+        const int per[GA_COUNT] = {gx * (h->nx + 1), gx * h->nx, gx * (h->nx - 1), (h->ny + Lc - 1) / Lc, h->nx - 1};
+        for (int gs = 0; gs < GA_COUNT; ++gs)
+            starts[(size_t)gs * (nscen + 1) + q + 1] = starts[(size_t)gs * (nscen + 1) + q] + per[gs];
+    }
+    for (int gs = 0; gs < GA_COUNT; ++gs) b->blocks[gs] = starts[(size_t)gs * (nscen + 1) + nscen];
+    hipError_t e;
+    if ((e = hipMalloc(&b->d_start, starts.size() * sizeof(int))) != hipSuccess ||
+        (e = hipMemcpy(b->d_start, starts.data(), starts.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMalloc(&b->d_ring, (size_t)m70h_adjoint_batch::RING * nscen * sizeof(AdjScen))) != hipSuccess ||
+        (e = hipHostMalloc((void **)&b->h_ring, (size_t)m70h_adjoint_batch::RING * nscen * sizeof(AdjScen),
+                           hipHostMallocDefault)) != hipSuccess)
+        return bail(abfail(b, M70H_EHIP, std::string("adjoint batch allocation: ") + hipGetErrorString(e)));
+    for (auto &ev : b->ev)
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
+            return bail(abfail(b, M70H_EHIP, std::string("adjoint batch events: ") + hipGetErrorString(e)));
+    *out = b;
+    return M70H_OK;
+}
+
+int m70h_adjoint_batch_forward(m70h_adjoint_batch *b, int nIter, int slots)
+{
+    if (!b) return M70H_EINVAL;
+    if (nIter < 1) return abfail(b, M70H_EINVAL, "adjoint batch: nIter must be >= 1");
+    if (slots < 1) return abfail(b, M70H_EINVAL, "adjoint batch: slots must be >= 1");
+    if ((double)nIter * nIter * (slots < nIter ? slots : nIter) > 4e9)
+        return abfail(b, M70H_EINVAL, "adjoint batch: schedule too large (split the Run or use fewer slots)");
+    int rc = ab_sync_handles(b);
+    if (rc) return rc;
+    for (m70h *h : b->hs) {
+        if (!h->forcing_set) return abfail(b, M70H_ESTATE, "forcing not set on a handle");
+        if ((rc = adj_setup(h, nIter, slots, false))) {
+            std::string msg = h->err;
+            adj_release(h);
+            return abfail(b, rc, msg);
+        }
+        AdjCtx *a = h->adj;
+        h->run_gen++;
+        a->gen = h->run_gen;
+        a->f0_pos0 = h->f0_pos;
+    }
+    b->N = nIter;
+    b->S = b->hs[0]->adj->S;
+    b->recorded = false;
+    if ((rc = ab_sync_forward_tables(b))) return rc;
+    const int n = (int)b->hs.size();
+    // per-Run pre-loop of every handle (src/Hydro.cpp:175-192)
+    ABCHK(b, launch_cells_grouped(b->stream, 0, b->fb->d_desc, b->fb->d_start + GS_CELLS * (n + 1), n,
+                                  b->fb->blocks[GS_CELLS]));
+    ABCHK(b, launch_cells_grouped(b->stream, 1, b->fb->d_desc, b->fb->d_start + GS_CELLS * (n + 1), n,
+                                  b->fb->blocks[GS_CELLS]));
+    if ((rc = ab_store(b, 0))) return rc;
+    // first sweep: the chain of checkpoints the schedule starts with
+    AdjCtx *a0 = b->hs[0]->adj;
+    int pos = 0, l = nIter, s = b->S;
+    while (l > 1 && s > 0) {
+        const int j = a0->split[(size_t)l * (a0->S + 1) + s];
+        if ((rc = ab_advance(b, j))) return rc;
+        pos += j;
+        if ((rc = ab_store(b, pos))) return rc;
+        l -= j;
+        --s;
+    }
+    if ((rc = ab_advance(b, nIter - pos))) return rc;
+    if ((rc = ab_copy(b, SEL_FINAL, 0, SEL_STATE, 0))) return rc;
+    for (m70h *h : b->hs) h->f0_pos = h->adj->f0_pos0 + nIter;
+    ABCHK(b, hipStreamSynchronize(b->stream));
+    b->recorded = true;
+    return M70H_OK;
+}
+
+int m70h_adjoint_batch_reverse(m70h_adjoint_batch *b, const m70h_fields *seeds, const m70h_fields *grads,
+                               double *dparams)
+{
+    if (!b) return M70H_EINVAL;
+    if (!b->recorded) return abfail(b, M70H_ESTATE, "adjoint batch: no recorded Run (call m70h_adjoint_batch_forward)");
+    for (m70h *h : b->hs)
+        if (!h->adj || h->adj->gen != h->run_gen || h->adj->N != b->N)
+            return abfail(b, M70H_ESTATE, "adjoint batch: a handle ran or changed since the recording");
+    int rc = ab_sync_handles(b);
+    if (rc) return rc;
+    const int n = (int)b->hs.size();
+    hipStream_t st = b->stream;
+    // adjoint of the final states, zeroed per-Run accumulators
+    for (int q = 0; q < n; ++q) {
+        m70h *h = b->hs[q];
+        AdjCtx *a = h->adj;
+        const long F = h->F, Fp = h->Fp;
+        const AdjFields &A = a->A[0];
+        double *dst[10] = {A.Zs, A.D, A.Qx, A.Qy, A.U, A.V, A.Uc, A.Vc, A.dQx, A.dQy};
+        const long len[10] = {F, F, Fp, Fp, Fp, Fp, F, F, F, F};
+        const double *src[10] = {};
+        if (seeds) {
+            const m70h_fields &sd = seeds[q];
+            const double *s10[10] = {sd.Zs, sd.D, sd.Qx, sd.Qy, sd.U, sd.V, sd.U_c, sd.V_c, sd.dQx_dx, sd.dQy_dy};
+            for (int f = 0; f < 10; ++f) src[f] = s10[f];
+        }
+        for (int f = 0; f < 10; ++f) {
+            if (src[f]) ABCHK(b, hipMemcpyAsync(dst[f], src[f], (size_t)len[f] * sizeof(double), hipMemcpyDeviceToDevice, st));
+            else ABCHK(b, hipMemsetAsync(dst[f], 0, (size_t)len[f] * sizeof(double), st));
+        }
+        const AdjRun &R = a->R;
+        double *rz[] = {R.cD_x, R.nuD_x, R.dnuD_dx, R.dnuDx_dy, R.cD_y, R.nuD_y, R.dnuD_dy, R.dnuDy_dx, R.cD, R.nuD};
+        for (double *p : rz) ABCHK(b, hipMemsetAsync(p, 0, (size_t)F * sizeof(double), st));
+        ABCHK(b, hipMemsetAsync(a->params, 0, 4 * sizeof(double), st));
+        a->fwd_steps = 0;
+        a->taped_steps = 0;
+    }
+    b->cur = 0;
+    if ((rc = ab_sync_forward_tables(b))) return rc;
+    if ((rc = ab_reverse_range(b, 0, b->N, b->S))) return rc;
+    // per-Run pre-loop reverse, gradients out, handles back at their final states
+    std::vector<double> hp(4 * (size_t)n, 0.0);
+    double *hpp = nullptr;
+    ABCHK(b, hipHostMalloc((void **)&hpp, 4 * (size_t)n * sizeof(double), hipHostMallocDefault));
+    for (int q = 0; q < n; ++q) {
+        m70h *h = b->hs[q];
+        AdjCtx *a = h->adj;
+        a->cur = b->cur;
+        const int q0 = adj_find(h, 0);
+        if (q0 < 0) { (void)hipHostFree(hpp); return abfail(b, M70H_ESTATE, "adjoint batch: missing start checkpoint"); }
+        const State S0 = buf_state(h, a->slot[q0]);
+        const AdjFields &L = a->A[a->cur];
+        hipError_t e = launch_adj_preloop(st, h->k, h->m, S0.D, h->fw, a->R, L.D, a->params);
+        if (e != hipSuccess) { (void)hipHostFree(hpp); return abfail(b, M70H_EHIP, std::string("adjoint pre-loop: ") + hipGetErrorString(e)); }
+        if (grads) {
+            const m70h_fields &gd = grads[q];
+            double *dst[10] = {gd.Zs, gd.D, gd.Qx, gd.Qy, gd.U, gd.V, gd.U_c, gd.V_c, gd.dQx_dx, gd.dQy_dy};
+            const double *src[10] = {L.Zs, L.D, L.Qx, L.Qy, L.U, L.V, L.Uc, L.Vc, L.dQx, L.dQy};
+            const long F = h->F, Fp = h->Fp;
+            const long len[10] = {F, F, Fp, Fp, Fp, Fp, F, F, F, F};
+            for (int f = 0; f < 10; ++f)
+                if (dst[f]) {
+                    e = hipMemcpyAsync(dst[f], src[f], (size_t)len[f] * sizeof(double), hipMemcpyDeviceToDevice, st);
+                    if (e != hipSuccess) { (void)hipHostFree(hpp); return abfail(b, M70H_EHIP, hipGetErrorString(e)); }
+                }
+        }
+        e = hipMemcpyAsync(hpp + 4 * q, a->params, 4 * sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e != hipSuccess) { (void)hipHostFree(hpp); return abfail(b, M70H_EHIP, hipGetErrorString(e)); }
+        h->f0_pos = a->f0_pos0 + a->N;
+    }
+    if ((rc = ab_copy(b, SEL_STATE, 0, SEL_FINAL, 0))) { (void)hipHostFree(hpp); return rc; }
+    hipError_t e = hipStreamSynchronize(st);
+    if (e == hipSuccess && dparams)
+        for (int q = 0; q < n; ++q)
+            for (int k = 0; k < 3; ++k) dparams[3 * q + k] = hpp[4 * q + k];
+    (void)hipHostFree(hpp);
+    if (e != hipSuccess) return abfail(b, M70H_EHIP, std::string("adjoint batch: ") + hipGetErrorString(e));
+    b->recorded = false;   // the first sweep's chain is consumed
     return M70H_OK;
 }
 

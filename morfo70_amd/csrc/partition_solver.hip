@@ -19,6 +19,7 @@
 // ulps per solve.
 #include <type_traits>
 
+#include "adjoint_batch.h"
 #include "assembly.h"
 #include "hydro_device.h"
 #include "hydro_kernels.h"
@@ -975,6 +976,95 @@ __global__ void __launch_bounds__(BLOCK) k_red_pcr_grouped(const ScenDesc *__res
     partition_body<true, BLOCK, 8>(ProvArrays<true>{d->cw.R1, d->cw.R0, d->cw.R2, d->cw.R3, (long)ny}, d->cw.X, g,
                                    b - start[sid], red);
 }
+
+// ---------------------------------------------------------------------------------
+// Line solves of the grouped adjoint (adjoint_batch.h): the taped step's systems and their
+// transposes, every scenario with its own segments, as its single-scenario launch.
+// ---------------------------------------------------------------------------------
+template <int SYS, int TRANS>
+__global__ void __launch_bounds__(1024) k_adj_cols_grouped(const AdjScen *__restrict__ D, const int *__restrict__ start,
+                                                           int nscen)
+{
+    __shared__ double red[2][4][1024 * 2];
+    const int b = blockIdx.x;
+    const int sid = scen_of(start, nscen, b);
+    const AdjScen *d = D + sid;
+    const int ny = d->k.ny;
+    LineGeom g{};
+    g.ld = ny; g.ldy = ny; g.n = d->k.nx - 1; g.nL = ny; g.S = d->S_cols; g.lines_per_block = d->L_cols;
+    g.periodic = 0;
+    const double *A = SYS == 1 ? d->T.a1 : d->T.a4, *B = SYS == 1 ? d->T.b1 : d->T.b4;
+    const double *C = SYS == 1 ? d->T.c1 : d->T.c4;
+    if (TRANS) {
+        partition_body<true, 1024, 8>(ProvTransCols{A, B, C, d->W.rhs, (long)ny, g.n}, d->W.lam, g, b - start[sid], red);
+    } else {
+        double *Y = SYS == 1 ? d->T.qxs : d->s.Qy + ny;
+        partition_body<true, 1024, 8>(ProvArrays<true>{A, B, C, d->T.f, (long)ny}, Y, g, b - start[sid], red);
+    }
+}
+
+template <int RB, int SYS, int TRANS>
+__global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_adj_rows_grouped(const AdjScen *__restrict__ D, const int *__restrict__ start, int nscen)
+{
+    __shared__ double sh[2][RowsLds<RB>::LDS];
+    const int b = blockIdx.x;
+    const int sid = scen_of(start, nscen, b);
+    const AdjScen *d = D + sid;
+    const int ny = d->k.ny;
+    const int line = b - start[sid];
+    const double *A = SYS == 2 ? d->T.a2 : d->T.a3, *B = SYS == 2 ? d->T.b2 : d->T.b3;
+    const double *C = SYS == 2 ? d->T.c2 : d->T.c3;
+    if (TRANS) {
+        rows_lds_body<RB>(ProvTransRows{A, B, C, d->W.rhs, (long)ny, ny}, d->W.lam, ny, ny, d->S_rows, 1, line, sh);
+    } else {
+        double *Y = SYS == 2 ? d->s.Qx + ny : d->T.qys;
+        rows_lds_body<RB>(ProvArrays<false>{A, B, C, d->T.f, (long)ny}, Y, ny, ny, d->S_rows, 1, line, sh);
+    }
+}
+
+hipError_t launch_adj_cols_grouped(hipStream_t st, int sys, int trans, const AdjScen *D, const int *start, int nscen,
+                                   int blocks)
+{
+    if (blocks <= 0) return hipSuccess;
+    const dim3 gr((unsigned)blocks), bl(1024);
+    if (sys == 1 && !trans) hipLaunchKernelGGL((k_adj_cols_grouped<1, 0>), gr, bl, 0, st, D, start, nscen);
+    else if (sys == 1) hipLaunchKernelGGL((k_adj_cols_grouped<1, 1>), gr, bl, 0, st, D, start, nscen);
+    else if (sys == 4 && !trans) hipLaunchKernelGGL((k_adj_cols_grouped<4, 0>), gr, bl, 0, st, D, start, nscen);
+    else if (sys == 4) hipLaunchKernelGGL((k_adj_cols_grouped<4, 1>), gr, bl, 0, st, D, start, nscen);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_rows_grouped(hipStream_t st, int rb, int sys, int trans, const AdjScen *D, const int *start,
+                                   int nscen, int blocks)
+{
+    if (blocks <= 0) return hipSuccess;
+    const dim3 gr((unsigned)blocks);
+    if (sys != 2 && sys != 3) return hipErrorInvalidValue;
+    if (rb == 256) {
+        const dim3 bl(256);
+        if (sys == 2 && !trans) hipLaunchKernelGGL((k_adj_rows_grouped<256, 2, 0>), gr, bl, 0, st, D, start, nscen);
+        else if (sys == 2) hipLaunchKernelGGL((k_adj_rows_grouped<256, 2, 1>), gr, bl, 0, st, D, start, nscen);
+        else if (!trans) hipLaunchKernelGGL((k_adj_rows_grouped<256, 3, 0>), gr, bl, 0, st, D, start, nscen);
+        else hipLaunchKernelGGL((k_adj_rows_grouped<256, 3, 1>), gr, bl, 0, st, D, start, nscen);
+    } else if (rb == 512) {
+        const dim3 bl(512);
+        if (sys == 2 && !trans) hipLaunchKernelGGL((k_adj_rows_grouped<512, 2, 0>), gr, bl, 0, st, D, start, nscen);
+        else if (sys == 2) hipLaunchKernelGGL((k_adj_rows_grouped<512, 2, 1>), gr, bl, 0, st, D, start, nscen);
+        else if (!trans) hipLaunchKernelGGL((k_adj_rows_grouped<512, 3, 0>), gr, bl, 0, st, D, start, nscen);
+        else hipLaunchKernelGGL((k_adj_rows_grouped<512, 3, 1>), gr, bl, 0, st, D, start, nscen);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Segments of a grouped adjoint scenario: column lines of n = nx-1 unknowns as the single
+// scenario's column solves (8 unknowns per thread), row lines of ny as its LDS row solver
+// with rb threads; 0 when the scenario does not fit.
+int adj_cols_segments(int n) { return pick_segments(n, 8, 256); }
+int adj_rows_segments(int n, int rb) { return pick_segments(n, RM, rb); }
 
 // Block counts of the grouped stages for one scenario.
 void grouped_stage_blocks(int nx, int ny, int rb, int *rows, int *cols, int *red, int *redp, int *S_rows, int *S_red,

@@ -1,36 +1,98 @@
-"""Batched bathymetry inversion: gradients of many scenarios at once, and gradient
-descent on the bed perturbation h.
+"""Batched bathymetry inversion: gradients of many scenarios at once, and the recovery of
+each scenario's bar.
 
-Each scenario has its own Hydro handle, HIP stream and adjoint buffers. A gradient of
+Each scenario has its own Hydro handle, stream and adjoint buffers. A gradient of
 L_s = 1/2 sum (Zs_N - Zs_obs)^2 for every scenario s is one checkpointed forward sweep and
-one reverse sweep per scenario (morfo70_amd.adjoint); the scenarios' sweeps are issued
-from a pool of host threads (ctypes releases the GIL during the library calls), so the
-GPU runs several scenarios' kernels at the same time.
+one reverse sweep. By default all scenarios' sweeps run together as grouped launches
+(m70h_adjoint_batch_*: one launch per stage of the schedule over every scenario); with
+grouped=False each scenario's sweeps are issued from a pool of host threads instead
+(ctypes releases the GIL during the library calls), one stream per scenario.
 """
 from __future__ import annotations
 
+import ctypes
 import time
 from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
 from . import _lib
-from .adjoint import HydroModel, h_adjoint
+from .adjoint import HydroModel, _fields, h_adjoint
 from .hydro import from_scenario
 
 
 class BatchInversion:
     def __init__(self, scenarios, nIter: int, slots: int, cD0: float, workers: int = 4,
-                 solver: int = _lib.SOLVER_FUSED):
+                 solver: int = _lib.SOLVER_FUSED, grouped: bool = True):
         self.cD0 = float(cD0)
         self.nIter = int(nIter)
+        self.slots = int(slots)
         self.models, self.obs = [], []
         for sc in scenarios:
             hy = from_scenario(sc, solver=solver)
             hy.set_forcing()
             self.models.append(HydroModel(hy, sc.Z, sc.zs0, nIter, slots))
         self.workers = max(1, int(workers))
+        self.grouped = bool(grouped)
+        self._batches = {}                           # scenario subset -> m70h_adjoint_batch
         self.last_dcD0 = [0.0] * len(self.models)   # dL_q/dcD0 of the last gradient
+
+    def _adjoint_batch(self, qs):
+        key = tuple(qs)
+        if key not in self._batches:
+            L = self.models[0].hy._L
+            arr = (ctypes.c_void_p * len(qs))(*[self.models[q].hy.handle for q in qs])
+            b = ctypes.c_void_p()
+            rc = L.m70h_adjoint_batch_create(len(qs), arr, ctypes.byref(b))
+            if rc != _lib.M70H_OK:
+                raise _lib.M70HError(rc, L.m70h_adjoint_batch_last_error(None).decode())
+            self._batches[key] = b
+        return self._batches[key]
+
+    def _grouped(self, qs, hs):
+        """Losses and h gradients of the scenarios qs from one grouped forward and reverse
+        sweep over all of them."""
+        import torch
+        L = self.models[0].hy._L
+        b = self._adjoint_batch(qs)
+
+        def chk(rc):
+            if rc != _lib.M70H_OK:
+                raise _lib.M70HError(rc, L.m70h_adjoint_batch_last_error(b).decode())
+        for q, h in zip(qs, hs):
+            m = self.models[q]
+            m.set_drag(self.cD0)
+            m.load_initial(h)
+        torch.cuda.synchronize()
+        chk(L.m70h_adjoint_batch_forward(b, self.nIter, self.slots))
+        losses, seeds, grads, keep = [], [], [], []
+        for q in qs:
+            m = self.models[q]
+            eta = m.final_zs()
+            with torch.cuda.stream(m.stream):
+                r = (eta - self.obs[q]).contiguous()
+                losses.append(0.5 * torch.sum(r * r))
+                g = {"Zs": torch.zeros_like(r), "D": torch.zeros_like(r)}
+            keep.append((r, g))
+            seeds.append(_fields({"Zs": r}))
+            grads.append(_fields(g))
+        torch.cuda.synchronize()
+        seed_arr = (_lib.Fields * len(qs))(*seeds)
+        grad_arr = (_lib.Fields * len(qs))(*grads)
+        dp = (ctypes.c_double * (3 * len(qs)))()
+        chk(L.m70h_adjoint_batch_reverse(b, ctypes.cast(seed_arr, ctypes.c_void_p),
+                                         ctypes.cast(grad_arr, ctypes.c_void_p),
+                                         ctypes.cast(dp, ctypes.POINTER(ctypes.c_double))))
+        out_l, out_g = [], []
+        for k, (q, h) in enumerate(zip(qs, hs)):
+            m = self.models[q]
+            _, g = keep[k]
+            with torch.cuda.stream(m.stream):
+                out_g.append(h_adjoint(m.Z, h, m.zs0, m.hy._params.Dmin, g["D"], g["Zs"]))
+            self.last_dcD0[q] = dp[3 * k]
+            out_l.append(float(losses[k]))
+        torch.cuda.synchronize()
+        return out_l, out_g
 
     def final_zs(self, hs):
         """Zs after nIter steps from the initial state of each h (forward only)."""
@@ -65,13 +127,17 @@ class BatchInversion:
         self.last_dcD0[q] = dp["cD0"]
         return float(loss), gh
 
-    def loss_and_grad(self, hs, concurrent: bool = True, which=None):
+    def loss_and_grad(self, hs, concurrent: bool = True, which=None, grouped: bool | None = None):
         """hs: one device tensor per scenario (ready on the device). Returns the losses
         and the gradients dL_s/dh_s (dL_s/dcD0 in last_dcD0). `which`: the scenarios to
-        evaluate (default all; hs then holds one tensor per listed scenario)."""
+        evaluate (default all; hs then holds one tensor per listed scenario). grouped
+        (default: the instance's choice) runs all of them as grouped launches; otherwise
+        concurrent issues each scenario's sweeps from its own host thread and stream."""
         import torch
         qs = list(range(len(self.models))) if which is None else list(which)
         torch.cuda.synchronize()
+        if self.grouped if grouped is None else grouped:
+            return self._grouped(qs, hs)
         if concurrent and self.workers > 1 and len(qs) > 1:
             with ThreadPoolExecutor(self.workers) as ex:
                 res = list(ex.map(lambda k: self._one(qs[k], hs[k]), range(len(qs))))
@@ -100,6 +166,11 @@ class BatchInversion:
         return hs, hist
 
     def close(self):
+        if self._batches:
+            L = self.models[0].hy._L
+            for b in self._batches.values():
+                L.m70h_adjoint_batch_destroy(b)
+            self._batches = {}
         for m in self.models:
             m.adj.release()
             m.hy.close()

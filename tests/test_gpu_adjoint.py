@@ -225,8 +225,10 @@ def test_bathymetry_inversion_gradient_c4():
 def test_batched_gradients_match_the_tangent_linear_oracle_at_c5_sizes():
     """The batched inversion's gradients on three of the C5 scenarios at their own sizes
     (nx U[256,1024], ny U[512,2048]; 60 steps instead of 300 to bound the oracle's time):
-    dL/dh along a random smooth direction and dL/dcD0 against the tangent-linear oracle
-    (<= 1e-8), and the concurrent gradients equal the one-after-another ones to rounding."""
+    the grouped sweeps (one launch per stage over all scenarios) equal each scenario's
+    own sweeps, issued concurrently or one after another, to rounding (the reverse adds
+    through FP64 atomics), and dL/dh along a random smooth direction and dL/dcD0 agree
+    with the tangent-linear oracle (<= 1e-8)."""
     from morfo70_amd.inversion import BatchInversion
 
     steps = 60
@@ -237,19 +239,22 @@ def test_batched_gradients_match_the_tangent_linear_oracle_at_c5_sizes():
     truth = inv.final_zs([torch.as_tensor(sc.h + dh, device="cuda") for sc, dh in zip(scs, dhs)])
     inv.set_observations([z + torch.as_tensor(rng.normal(0, 0.02, size=z.shape), device="cuda") for z in truth])
     hs = [torch.as_tensor(np.ascontiguousarray(sc.h), device="cuda") for sc in scs]
-    l1, g1 = inv.loss_and_grad(hs, concurrent=True)
-    dc1 = list(inv.last_dcD0)
-    l2, g2 = inv.loss_and_grad(hs, concurrent=False)
-    # the reverse sweep adds through FP64 atomics, so only the summation order differs
-    for a, b, ga, gb in zip(l1, l2, g1, g2):
-        assert a == b
-        assert float(torch.max(torch.abs(ga - gb))) <= 1e-12 * float(torch.max(torch.abs(gb)))
-    for a, b in zip(inv.last_dcD0, dc1):
-        assert abs(a - b) <= 1e-12 * abs(b)
+    lg, gg = inv.loss_and_grad(hs)                                    # grouped
+    dcg = list(inv.last_dcD0)
+    lc, gc = inv.loss_and_grad(hs, grouped=False, concurrent=True)
+    dcc = list(inv.last_dcD0)
+    ls, gs = inv.loss_and_grad(hs, grouped=False, concurrent=False)
+    dcs = list(inv.last_dcD0)
+    for q in range(len(scs)):
+        assert lg[q] == lc[q] == ls[q], q
+        for other, dco in ((gc, dcc), (gs, dcs)):
+            scale = float(torch.max(torch.abs(other[q])))
+            assert float(torch.max(torch.abs(gg[q] - other[q]))) <= 1e-12 * scale, q
+            assert abs(dcg[q] - dco[q]) <= 1e-12 * abs(dco[q]), q
     threads = oracle.host_cpus()["threads"]
     for q, sc in enumerate(scs):
         obs = inv.obs[q].cpu().numpy()
-        gh = g1[q].cpu().numpy()
+        gh = gg[q].cpu().numpy()
         v = S.smooth_field(sc.nx, sc.ny, np.random.default_rng(70 + q), 0.01)
         tl = oracle.TangentHydro(sc, dh=v, threads=threads)
         tl.run(steps)
@@ -261,8 +266,8 @@ def test_batched_gradients_match_the_tangent_linear_oracle_at_c5_sizes():
         tc.run(steps)
         vals, tans = tc.get_state()
         lhs = float(np.sum((vals["Zs"] - obs) * tans["Zs"]))
-        print(f"[c5 tlm] {sc.name} cD0: tlm {lhs:.12e} adjoint {dc1[q]:.12e}", flush=True)
-        assert abs(lhs - dc1[q]) <= 1e-8 * max(abs(lhs), abs(dc1[q])), (sc.name, lhs, dc1[q])
+        print(f"[c5 tlm] {sc.name} cD0: tlm {lhs:.12e} adjoint {dcg[q]:.12e}", flush=True)
+        assert abs(lhs - dcg[q]) <= 1e-8 * max(abs(lhs), abs(dcg[q])), (sc.name, lhs, dcg[q])
     inv.close()
 
 
