@@ -114,6 +114,9 @@ def parse():
                     help="adjoint checkpoint buffers; 0 = tape every step in the forward sweep")
     ap.add_argument("--adj-steps", type=int, default=400)
     ap.add_argument("--batch-count", type=int, default=64)
+    ap.add_argument("--grouped-only", action="store_true",
+                    help="inversion16: skip the per-scenario (threads and streams, one after another) "
+                         "gradient timings, so every launch comes from one host thread")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-sized device buffer 5 times (reference kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -601,13 +604,15 @@ def bench_inversion(args):
     zs0 = [np.zeros(2)] * len(scs)
     h0 = [bi.h_of(q, zs0[q])[0] for q in range(len(scs))]
     bi.gradients(zs0)                      # warm-up: allocations
-    bi.inv.loss_and_grad(h0, grouped=False)
-    t0 = time.perf_counter()
-    bi.inv.loss_and_grad(h0, grouped=False, concurrent=False)
-    t_seq_grad = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    bi.inv.loss_and_grad(h0, grouped=False, concurrent=True)
-    t_thr_grad = time.perf_counter() - t0
+    t_seq_grad = t_thr_grad = None
+    if not args.grouped_only:
+        bi.inv.loss_and_grad(h0, grouped=False)
+        t0 = time.perf_counter()
+        bi.inv.loss_and_grad(h0, grouped=False, concurrent=False)
+        t_seq_grad = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        bi.inv.loss_and_grad(h0, grouped=False, concurrent=True)
+        t_thr_grad = time.perf_counter() - t0
     t0 = time.perf_counter()
     bi.gradients(zs0)
     t_grad = time.perf_counter() - t0
@@ -636,7 +641,8 @@ def bench_inversion(args):
         "config": {"workload": "inversion16", "scenarios": len(scs), "cells_total": cells, "steps": n,
                    "checkpoint_buffers": slots, "iterations": args.gd_iters, "threads": bi.inv.workers},
         "gradient_s_grouped": t_grad, "gradient_s_threads_and_streams": t_thr_grad,
-        "gradient_s_one_after_another": t_seq_grad, "grouped_over_one_after_another": t_seq_grad / t_grad,
+        "gradient_s_one_after_another": t_seq_grad,
+        "grouped_over_one_after_another": t_seq_grad / t_grad if t_seq_grad else None,
         "gradient_cell_steps_per_s": cells * n / t_grad, "fit_total_s": t_fit,
         "iterations_run": len(hist) - 1,
         "bar_rms_error_ratio": {"median": float(np.median(ratio)), "max": float(np.max(ratio)),
