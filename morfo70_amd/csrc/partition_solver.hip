@@ -309,8 +309,33 @@ __device__ __forceinline__ RedRows seg_eliminate(int m, double (&lo)[SEG_M], dou
     return r;
 }
 
+// One cyclic-reduction step of a row (lower lo, diagonal dg, upper up, rhs rh) against
+// its neighbours at -stride (m0..m3) and +stride (p0..p3), as (lower, diagonal, upper,
+// rhs): eliminates the couplings to them, coupling the row to rows at twice the stride.
+__device__ __forceinline__ void pcr_combine(double &lo, double &dg, double &up, double &rh, double m0, double m1,
+                                            double m2, double m3, double p0, double p1, double p2, double p3)
+{
+    const double k1m = lo / m1;
+    const double k2p = up / p1;
+    const double na = -m0 * k1m;
+    const double nc = -p2 * k2p;
+    const double nb = dg - m2 * k1m - p0 * k2p;
+    const double nd = rh - m3 * k1m - p3 * k2p;
+    lo = na; dg = nb; up = nc; rh = nd;
+}
+
+// Last step of cyclic PCR on 2S rows: rows i and i+S are coupled only to each other,
+//   bi x_i + ei x_j = di,  ej x_i + bj x_j = dj  (e = lower + upper);  returns x_i.
+__device__ __forceinline__ double pcr_pair(double bi, double ei, double di, double bj, double ej, double dj)
+{
+    return (di * bj - ei * dj) / (bi * bj - ei * ej);
+}
+
 // Cyclic PCR on N = 2S reduced rows of one line (lower, diagonal, upper, rhs), row i of
-// the line at base + i*rs of the four arrays P[0..3] or Q[0..3]: the steps alternate
+// the line at base + slot(i)*rs of the four arrays P[0..3] or Q[0..3], where slot(2s+w) =
+// w*S + s: the first rows of all segments, then their last rows, so that a wavefront's
+// accesses to one of a thread's two rows are consecutive words (no LDS bank conflicts;
+// rows 2s and 2s+1 side by side would put lanes 16 bytes apart). The steps alternate
 // between the two buffers, so one barrier per step orders every read before the next
 // overwrite, and each thread keeps its own two rows in registers, reading only the
 // neighbours' from LDS. Workgroups holding several lines interleave them (rs = lines per
@@ -321,13 +346,14 @@ __device__ __forceinline__ void pcr_cyclic(double *const (&P)[4], double *const 
                                            int S, bool active, const RedRows &r, double &xF, double &xL)
 {
     const int N = 2 * S;
+    auto at = [&](int i) { return base + ((i & 1) * S + (i >> 1)) * rs; };
     double lo[2] = {r.lo0, r.A_L}, dg[2] = {r.A_F, r.B_L}, up[2] = {r.C_F, r.upL}, rh[2] = {r.R_F, r.R_L};
     const double *c0 = P[0], *c1 = P[1], *c2 = P[2], *c3 = P[3];
     double *n0 = Q[0], *n1 = Q[1], *n2 = Q[2], *n3 = Q[3];
     if (active) {
 #pragma unroll
         for (int w = 0; w < 2; ++w) {
-            const int ii = base + (2 * seg + w) * rs;
+            const int ii = base + (w * S + seg) * rs;
             P[0][ii] = lo[w]; P[1][ii] = dg[w]; P[2][ii] = up[w]; P[3][ii] = rh[w];
         }
     }
@@ -337,18 +363,12 @@ __device__ __forceinline__ void pcr_cyclic(double *const (&P)[4], double *const 
 #pragma unroll
             for (int w = 0; w < 2; ++w) {
                 const int i = 2 * seg + w;
-                const int im = base + ((i - st) & (N - 1)) * rs, ip = base + ((i + st) & (N - 1)) * rs;
-                const double k1m = lo[w] / c1[im];
-                const double k2p = up[w] / c1[ip];
-                const double na = -c0[im] * k1m;
-                const double nc = -c2[ip] * k2p;
-                const double nb = dg[w] - c2[im] * k1m - c0[ip] * k2p;
-                const double nd = rh[w] - c3[im] * k1m - c3[ip] * k2p;
-                lo[w] = na; dg[w] = nb; up[w] = nc; rh[w] = nd;
+                const int im = at((i - st) & (N - 1)), ip = at((i + st) & (N - 1));
+                pcr_combine(lo[w], dg[w], up[w], rh[w], c0[im], c1[im], c2[im], c3[im], c0[ip], c1[ip], c2[ip], c3[ip]);
             }
 #pragma unroll
             for (int w = 0; w < 2; ++w) {
-                const int ii = base + (2 * seg + w) * rs;
+                const int ii = base + (w * S + seg) * rs;
                 n0[ii] = lo[w]; n1[ii] = dg[w]; n2[ii] = up[w]; n3[ii] = rh[w];
             }
         }
@@ -367,14 +387,81 @@ __device__ __forceinline__ void pcr_cyclic(double *const (&P)[4], double *const 
 #pragma unroll
         for (int w = 0; w < 2; ++w) {
             const int i = 2 * seg + w;
-            const int jj = base + ((i + S) & (N - 1)) * rs;
-            const double bi = dg[w], ei = lo[w] + up[w], di_ = rh[w];
-            const double bj = c1[jj], ej = c0[jj] + c2[jj], dj = c3[jj];
-            xs[w] = (di_ * bj - ei * dj) / (bi * bj - ei * ej);
+            const int jj = at((i + S) & (N - 1));
+            xs[w] = pcr_pair(dg[w], lo[w] + up[w], rh[w], c1[jj], c0[jj] + c2[jj], c3[jj]);
         }
         xF = xs[0];
         xL = xs[1];
     }
+}
+
+// Reads a double from lane `src` of the wavefront (the source lane must be active).
+__device__ __forceinline__ double lane_read(double v, int src)
+{
+    const int a = src << 2;
+    const int lo = __builtin_amdgcn_ds_bpermute(a, __double2loint(v));
+    const int hi = __builtin_amdgcn_ds_bpermute(a, __double2hiint(v));
+    return __hiloint2double(hi, lo);
+}
+
+// The steps of cyclic PCR after the first, inside one wavefront, with no LDS and no
+// barrier. Lane l (0..G-1, G a power of two, 2 <= G <= 64) holds the two rows (w = 0, 1)
+// of segment l, cyclic over the G lanes. The steps at strides 2, 4, ..., G/2 read the
+// rows of the same w from the lanes 1, 2, ..., G/4 away by cross-lane permutes and
+// combine them (pcr_combine); then rows G/2 lanes apart are coupled only to each other
+// (pcr_pair). Each row's arithmetic is pcr_cyclic's, step for step. Lanes G..63 take part
+// in the permutes with whatever they hold and return garbage. Every lane of the
+// wavefront must call it.
+__device__ __forceinline__ void pcr_lane_steps(int G, int l, double (&lo)[2], double (&dg)[2], double (&up)[2],
+                                               double (&rh)[2], double &x0, double &x1)
+{
+    const int msk = G - 1;
+    for (int d = 1; d < (G >> 1); d <<= 1) {
+        const int lm = (l - d) & msk, lp = (l + d) & msk;
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+            const double m0 = lane_read(lo[w], lm), m1 = lane_read(dg[w], lm), m2 = lane_read(up[w], lm),
+                         m3 = lane_read(rh[w], lm);
+            const double p0 = lane_read(lo[w], lp), p1 = lane_read(dg[w], lp), p2 = lane_read(up[w], lp),
+                         p3 = lane_read(rh[w], lp);
+            pcr_combine(lo[w], dg[w], up[w], rh[w], m0, m1, m2, m3, p0, p1, p2, p3);
+        }
+    }
+    const int pl = (l + (G >> 1)) & msk;
+    double xs[2];
+#pragma unroll
+    for (int w = 0; w < 2; ++w) {
+        const double e = lo[w] + up[w];
+        const double bj = lane_read(dg[w], pl), ej = lane_read(e, pl), dj = lane_read(rh[w], pl);
+        xs[w] = pcr_pair(dg[w], e, rh[w], bj, ej, dj);
+    }
+    x0 = xs[0];
+    x1 = xs[1];
+}
+
+// pcr_cyclic for S <= 64 segments held by lanes 0..S-1 of one wavefront (lane l, segment
+// l): the stride-1 step mixes a lane's two rows with its neighbours' (row 2l with rows
+// 2l-1 and 2l+1, row 2l+1 with rows 2l and 2l+2), the other steps are pcr_lane_steps.
+// Same steps and arithmetic as pcr_cyclic, so the same bits, without LDS or barriers.
+// Every lane of the wavefront calls it; lanes l >= S return garbage.
+__device__ __forceinline__ void pcr_wave(int S, int l, const RedRows &r, double &xF, double &xL)
+{
+    double lo[2] = {r.lo0, r.A_L}, dg[2] = {r.A_F, r.B_L}, up[2] = {r.C_F, r.upL}, rh[2] = {r.R_F, r.R_L};
+    if (S == 1) {   // rows 0 and 1 are the coupled pair
+        xF = pcr_pair(dg[0], lo[0] + up[0], rh[0], dg[1], lo[1] + up[1], rh[1]);
+        xL = pcr_pair(dg[1], lo[1] + up[1], rh[1], dg[0], lo[0] + up[0], rh[0]);
+        return;
+    }
+    const int msk = S - 1, sl = l & msk;
+    const int lm = (sl - 1) & msk, lp = (sl + 1) & msk;
+    const double m0 = lane_read(lo[1], lm), m1 = lane_read(dg[1], lm), m2 = lane_read(up[1], lm),
+                 m3 = lane_read(rh[1], lm);
+    const double p0 = lane_read(lo[0], lp), p1 = lane_read(dg[0], lp), p2 = lane_read(up[0], lp),
+                 p3 = lane_read(rh[0], lp);
+    const double a0 = lo[0], b0 = dg[0], c0 = up[0], d0 = rh[0];
+    pcr_combine(lo[0], dg[0], up[0], rh[0], m0, m1, m2, m3, lo[1], dg[1], up[1], rh[1]);
+    pcr_combine(lo[1], dg[1], up[1], rh[1], a0, b0, c0, d0, p0, p1, p2, p3);
+    pcr_lane_steps(S, sl, lo, dg, up, rh, xF, xL);
 }
 
 // Interior unknown q of an eliminated segment.
@@ -557,8 +644,11 @@ __device__ __forceinline__ void rows_lds_body(const Prov &prov, double *__restri
     __syncthreads();   // the coefficient tile is free from here on
     RedRows rr{};
     if (active) rr = seg_eliminate<RM>(m, lo, di, up, rh);
-    double xF, xL;
-    {
+    double xF = 0.0, xL = 0.0;
+    if (S <= 64) {
+        // every segment is in wavefront 0: PCR by cross-lane permutes, no LDS
+        if (t < 64) pcr_wave(S, t, rr, xF, xL);
+    } else {
         const int N = 2 * S;   // 4N <= 4*RB*2 < LDS doubles per array
         double *const P[4] = {sh[0], sh[0] + N, sh[0] + 2 * N, sh[0] + 3 * N};
         double *const Q[4] = {sh[1], sh[1] + N, sh[1] + 2 * N, sh[1] + 3 * N};

@@ -66,10 +66,13 @@ def gpu_solve(solver, rows, periodic, a, b, c, f, tile=0):
     return y if rows else np.ascontiguousarray(y.T)
 
 
-@pytest.mark.parametrize("n", [3, 4, 17, 64, 100, 257])
+@pytest.mark.parametrize("n", [3, 4, 9, 17, 64, 100, 257, 512])
 @pytest.mark.parametrize("periodic", [True, False])
 @pytest.mark.parametrize("rows", [True, False])
 def test_partition_vs_dense(n, periodic, rows):
+    """Row lines up to 512 unknowns (at most 64 segments of 8) run their reduced system's
+    PCR in one wavefront by cross-lane permutes (1, 2, 16 and 64 segments here: n = 3 and
+    4, 9, 100, 257 and 512); longer ones through LDS."""
     if periodic and not rows:
         pytest.skip("column lines are never periodic in the Hydro step")
     rng = np.random.default_rng(n + 7 * periodic + 3 * rows)
