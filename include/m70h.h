@@ -49,14 +49,14 @@ extern "C" {
 /* PARTITION splits every line over the threads of a workgroup: two-sided elimination of
  * per-thread segments in registers, cyclic reduction of the segment-boundary system in
  * LDS, back-substitution. Same systems, different rounding: results agree with EXACT to
- * a few ulps per solve. Line lengths up to 4096. */
+ * a few ulps per solve. Line lengths up to 8192. */
 #define M70H_SOLVER_PARTITION 1
 /* FUSED keeps the coefficients out of HBM where it can: the y-sweeps (lines along rows)
  * are assembled and solved in one launch with the row staged through LDS, and the
  * x-sweeps assemble and eliminate 8-row column segments in registers, solve the reduced
  * boundary systems and back-substitute, every access coalesced across 64 columns.
- * Same partition algorithm as PARTITION, different segment sizes on the columns. ny up
- * to 4096. */
+ * Same partition algorithm as PARTITION, different segment sizes on the columns. nx-1 and
+ * ny up to 8192. */
 #define M70H_SOLVER_FUSED 2
 
 /* Parameters read by Hydro at call time in the reference (Parameters::Hydro::dt, Dmin,
@@ -155,8 +155,9 @@ int m70h_set_solver(m70h *h, int solver);
  * x (columns) and along y (rows). This is the knob the autotuner sets. */
 int m70h_set_solver_tiles(m70h *h, int tile_cols, int tile_rows);
 
-/* FUSED row-solver block: 256 threads (ny up to 2048; two blocks per CU) or 512 (ny up
- * to 4096); 0 picks the smallest that fits. An autotuner knob. */
+/* FUSED row-solver block: 256 threads (ny up to 2048; four blocks per CU) or 512 (ny up
+ * to 4096; two blocks per CU); 0 picks the smallest that fits. Rows longer than 4096 (up
+ * to 8192) always take 1024-thread blocks. An autotuner knob. */
 int m70h_set_rows_block(m70h *h, int threads);
 
 /* Batched tridiagonal solve on device arrays, the counterpart of Math::TridiagonalSolver
@@ -167,7 +168,7 @@ int m70h_set_rows_block(m70h *h, int threads);
  * n-1 is the corner), f = right-hand side; y receives the solution with stride ldy.
  * EXACT supports rows+periodic (scratch >= 3*n*roundup(nlines,64) doubles) and
  * columns+non-periodic (ldy == ld, scratch >= n*ld doubles); PARTITION supports rows
- * (periodic or not) and columns (non-periodic), n <= 4096, no scratch. Enqueued on
+ * (periodic or not) and columns (non-periodic), n <= 8192, no scratch. Enqueued on
  * hip_stream (a hipStream_t); returns without waiting. */
 int m70h_tridiag_solve(void *hip_stream, int solver, int lines_are_rows, int periodic, int n, int nlines, long ld,
                        const double *a_dev, const double *b_dev, const double *c_dev, const double *f_dev,

@@ -704,9 +704,9 @@ int m70h_set_solver(m70h *h, int solver)
     if (solver != M70H_SOLVER_EXACT && solver != M70H_SOLVER_PARTITION && solver != M70H_SOLVER_FUSED)
         return fail(h, M70H_EINVAL, "unknown solver");
     if (solver != M70H_SOLVER_EXACT && (!partition_supported(h->nx - 1) || !partition_supported(h->ny)))
-        return fail(h, M70H_EINVAL, "partition solver supports line lengths up to 4096");
+        return fail(h, M70H_EINVAL, "partition solver supports line lengths up to 8192");
     if (solver == M70H_SOLVER_FUSED && !rows_lds_supported(h->ny))
-        return fail(h, M70H_EINVAL, "fused solver supports ny up to 4096");
+        return fail(h, M70H_EINVAL, "fused solver supports ny up to 8192");
     h->solver = solver;
     return M70H_OK;
 }

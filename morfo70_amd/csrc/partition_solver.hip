@@ -683,15 +683,20 @@ static int pick_segments(int n, int M, int smax)
     return S;
 }
 
-bool partition_supported(int n) { return pick_segments(n, 16, 256) > 0 || pick_segments(n, 8, 512) > 0; }
+// Lines up to 8192 unknowns: rows through the LDS row kernel (up to 1024 segments of 8),
+// columns through the partition kernel (up to 1024 segments of 8 in 1024-thread blocks).
+bool partition_supported(int n) { return pick_segments(n, 8, 1024) > 0; }
 
+// smax bounds the segment count (0: the default, BLOCK for rows and BLOCK/4 for columns).
 template <bool COLS, int BLOCK, int M, class Prov>
-static hipError_t launch_partition(hipStream_t st, const Prov &prov, double *Y, long ldy, int n, int nL, bool periodic)
+static hipError_t launch_partition(hipStream_t st, const Prov &prov, double *Y, long ldy, int n, int nL, bool periodic,
+                                   int smax = 0)
 {
     LineGeom g{};
     g.ld = 0; g.ldy = ldy; g.n = n; g.nL = nL; g.periodic = periodic ? 1 : 0;
-    // columns: keep at least 4 lines per block so a row access covers 32 contiguous bytes
-    g.S = pick_segments(n, M, COLS ? BLOCK / 4 : BLOCK);
+    // columns: by default at least 4 lines per block, so a row access covers 32
+    // contiguous bytes
+    g.S = pick_segments(n, M, smax > 0 ? smax : (COLS ? BLOCK / 4 : BLOCK));
     if (g.S == 0) return hipErrorInvalidValue;
     int lg = 0;
     while ((1 << lg) < g.S) ++lg;
@@ -733,7 +738,9 @@ static hipError_t dispatch_cols(hipStream_t st, const Prov &prov, double *Y, lon
         if (wg == 256) return launch_partition<true, 256, 8>(st, prov, Y, ldy, n, nL, false);
         return launch_partition<true, 1024, 8>(st, prov, Y, ldy, n, nL, false);
     }
-    return launch_partition<true, 256, 16>(st, prov, Y, ldy, n, nL, false);
+    // columns longer than 2048: up to 1024 segments, so fewer than 4 columns per
+    // 1024-thread workgroup (2 up to 4096, 1 up to 8192)
+    return launch_partition<true, 1024, 8>(st, prov, Y, ldy, n, nL, false, 1024);
 }
 
 template <class Prov>
@@ -747,9 +754,9 @@ static hipError_t dispatch_rows(hipStream_t st, const Prov &prov, double *Y, lon
 {
     if (tile == 16 && pick_segments(n, 16, 256) > 0)
         return launch_partition<false, 256, 16>(st, prov, Y, ldy, n, nL, periodic);
-    // rows up to 4096: staged through LDS (coalesced loads, two workgroups per CU); the
-    // same segments and arithmetic as the partition kernel below
-    if (pick_segments(n, RM, 512) > 0) return launch_rows_lds(st, prov, Y, ldy, n, nL, periodic, 0);
+    // rows up to 8192: staged through LDS (coalesced loads); the same segments and
+    // arithmetic as the partition kernel below
+    if (pick_segments(n, RM, 1024) > 0) return launch_rows_lds(st, prov, Y, ldy, n, nL, periodic, 0);
     if (pick_segments(n, 8, 512) > 0) return launch_partition<false, 512, 8>(st, prov, Y, ldy, n, nL, periodic);
     return launch_partition<false, 256, 16>(st, prov, Y, ldy, n, nL, periodic);
 }
@@ -939,12 +946,15 @@ hipError_t launch_cols3_qyx(hipStream_t st, const Consts &k, const Metrics &m, c
     return launch_cols3(st, ProvQyx{k, m, s, rf, w}, cw, s.Qy + k.ny, k.nx - 1, k.ny, tile, pass);
 }
 
-bool rows_lds_supported(int n) { return pick_segments(n, RM, 512) > 0; }
+bool rows_lds_supported(int n) { return pick_segments(n, RM, 1024) > 0; }
 
-// Row block: 256 threads (lines up to 2048, 36 KB of LDS) or 512 threads (lines up to
-// 4096, 72 KB). rb = 0 picks the smallest fit.
+// Row block: 256 threads (lines up to 2048, 36 KB of LDS; four blocks per CU), 512 threads
+// (lines up to 4096, 72 KB; two blocks per CU) or 1024 threads (lines up to 8192, 144 KB;
+// one block per CU). rb = 0 picks the smallest fit; a block too small for the line is
+// replaced by the smallest that fits.
 int rows_block_for(int n, int rb)
 {
+    if (n > RowsLds<512>::NMAX) return 1024;
     if (rb == 256 && pick_segments(n, RM, 256) > 0 && n <= RowsLds<256>::NMAX) return 256;
     if (rb == 512) return 512;
     return n <= RowsLds<256>::NMAX ? 256 : 512;
@@ -1222,8 +1232,11 @@ static hipError_t launch_rows_lds(hipStream_t st, const Prov &prov, double *Y, l
     if (rb == 256)
         hipLaunchKernelGGL((k_rows_lds<256, Prov>), dim3((unsigned)nL), dim3(256), 0, st, prov, Y, ldy, n, S,
                            periodic ? 1 : 0);
-    else
+    else if (rb == 512)
         hipLaunchKernelGGL((k_rows_lds<512, Prov>), dim3((unsigned)nL), dim3(512), 0, st, prov, Y, ldy, n, S,
+                           periodic ? 1 : 0);
+    else
+        hipLaunchKernelGGL((k_rows_lds<1024, Prov>), dim3((unsigned)nL), dim3(1024), 0, st, prov, Y, ldy, n, S,
                            periodic ? 1 : 0);
     return hipGetLastError();
 }

@@ -115,6 +115,20 @@ def test_partition_solver_parity(case, solver):
     print({k: v["rel"] for k, v in res.items()})
 
 
+@pytest.mark.parametrize("solver", [1, 2])
+@pytest.mark.parametrize("nx,ny", [(3000, 16), (5000, 12), (16, 6000), (12, 8192)])
+def test_long_lines_parity(nx, ny, solver):
+    """Lines longer than 2048 and 4096 against the oracle at the 1e-10 bar: columns of
+    2999 and 4999 unknowns (PARTITION with fewer than four columns per workgroup, FUSED's
+    reduced systems of 750 and 1250 rows) and rows of 6000 and 8192 (the 1024-thread LDS
+    row kernel)."""
+    ph = S.Physics(drag_model=S.DRAG["constant"], visc_model=S.VISC["constant"], dt=0.25)
+    sc = nonuniform_beach(nx=nx, ny=ny, physics=ph, seed=nx + ny)
+    got, ref = run_both(sc, 5, solver=solver)
+    assert_parity(got, ref, exact=False)
+    assert np.max(np.abs(got["Qx"])) > 0
+
+
 def test_lake_at_rest_matches_oracle_and_stays_at_rest():
     """Still water, zero forcing: Zs = D - Z + h is zero only to rounding, so the step
     moves it at the 1e-16 level; the GPU must match the oracle bit for bit and the

@@ -101,8 +101,13 @@ def test_exact_matches_oracle_solvers_bitwise(n, nl):
 
 
 @pytest.mark.parametrize("n,nl,rows,periodic", [(4096, 33, True, True), (2047, 300, False, False),
-                                                (1000, 50, True, False)])
+                                                (1000, 50, True, False), (3000, 40, False, False),
+                                                (6000, 9, True, True), (8192, 7, True, False),
+                                                (8192, 70, False, False)])
 def test_partition_long_lines_vs_exact(n, nl, rows, periodic):
+    """Up to 8192 unknowns: rows above 4096 take the 1024-thread LDS row kernel, columns
+    above 2048 the partition kernel with 2 columns (up to 4096) or 1 (up to 8192) per
+    1024-thread workgroup."""
     rng = np.random.default_rng(n + nl)
     a, b, c, f = dominant_system(rng, n, nl, periodic, dominance=2.05)
     import oracle
