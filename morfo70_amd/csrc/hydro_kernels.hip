@@ -180,9 +180,12 @@ __global__ void __launch_bounds__(256) k_qxy_assemble(Consts k, State s, RunFiel
 // predictor's IsWater (setWaterNodes_y, src/HydroQy.cpp:92-122) into w.w. Rows 0..nx-1.
 // guess_Zs enters only that mask and is not stored.
 // ------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_qx_tail(Consts k, Metrics m, State s, Work w, double f0, int write_uc)
+__global__ void __launch_bounds__(256) k_qx_tail(Consts k, Metrics m, State s, Work w, double f0, int write_uc,
+                                                 int rev)
 {
-    CELL_IJ_ONLY(k.nx)
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = rev ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;   // rev: rows last to first
+    if (j >= k.ny || i >= k.nx) return;
     qx_tail_body(k, m, s, w, f0, i, j, write_uc != 0);
 }
 
@@ -217,9 +220,11 @@ __global__ void __launch_bounds__(256) k_qyx_assemble(Consts k, Metrics m, State
 // (src/HydroQy.cpp:458-487), Y2C (src/HydroMeshOperations.cpp:88-97). Rows 0..nx-1.
 // Reads D, writes the new depth to Dn.
 // ------------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_qy_tail(Consts k, State s, int write_vc)
+__global__ void __launch_bounds__(256) k_qy_tail(Consts k, State s, int write_vc, int rev)
 {
-    CELL_IJ_ONLY(k.nx)
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = rev ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;   // rev: rows last to first
+    if (j >= k.ny || i >= k.nx) return;
     qy_tail_body(k, s, i, j, write_vc != 0);
 }
 
@@ -380,10 +385,10 @@ __device__ __forceinline__ void qx_tail2_body(const Consts &k, const Metrics &m,
 
 // `rows` rows per thread, two columns (ny even): grid (ceil(ny/512), ceil(nx/rows)).
 __global__ void __launch_bounds__(256) k_qx_tail2r(Consts k, Metrics m, State s, Work w, double f0, int write_uc,
-                                                   int rows)
+                                                   int rows, int rev)
 {
     const int j0 = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int i0 = (int)blockIdx.y * rows;
+    const int i0 = (rev ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y) * rows;   // rev: row runs last to first
     if (j0 >= k.ny || i0 >= k.nx) return;
     const int i1 = i0 + rows < k.nx ? i0 + rows : k.nx;
     qx_tail2_rows(k, m, s, w, f0, i0, i1, j0, write_uc != 0);
@@ -400,10 +405,10 @@ static int qx_tail_rows(const Consts &k)
 }
 
 
-__global__ void __launch_bounds__(256) k_qy_tail2(Consts k, State s, int write_vc)
+__global__ void __launch_bounds__(256) k_qy_tail2(Consts k, State s, int write_vc, int rev)
 {
     const int j0 = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    const int i = blockIdx.y;
+    const int i = rev ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;   // rev: rows last to first
     if (j0 >= k.ny || i >= k.nx) return;
     qy_tail2_body(k, s, i, j0, write_vc != 0);
 }
@@ -444,12 +449,13 @@ __global__ void __launch_bounds__(256) k_init_zs(double *Zs, const double *D, co
 // ------------------------------------------------------------------------------------
 template <int OP>
 __global__ void __launch_bounds__(256) k_cells_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
-                                                       int nscen)
+                                                       int nscen, int rev)
 {
-    const int b = blockIdx.x;
+    // rev: the last scenario's last row first
+    const int b = rev ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x;
     // the Qy tail ends a step: count it for the offshore offset series (scen_f0); every
     // reader of the counter is in a later launch
-    if (OP == 3 && b == 0 && threadIdx.x == 0) atomicAdd((int *)D[0].step, 1);
+    if (OP == 3 && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((int *)D[0].step, 1);
     const int sid = scen_of(start, nscen, b);
     const ScenDesc &d = D[sid];
     const int ny = d.k.ny;
@@ -475,13 +481,14 @@ __global__ void __launch_bounds__(256) k_cells_grouped(const ScenDesc *__restric
 
 int grouped_cell_blocks(int nx, int ny) { return ((ny + 255) / 256) * nx; }
 
-hipError_t launch_cells_grouped(hipStream_t st, int op, const ScenDesc *D, const int *start, int nscen, int blocks)
+hipError_t launch_cells_grouped(hipStream_t st, int op, const ScenDesc *D, const int *start, int nscen, int blocks,
+                                int rev)
 {
     switch (op) {
-    case 0: hipLaunchKernelGGL(k_cells_grouped<0>, dim3(blocks), dim3(256), 0, st, D, start, nscen); break;
-    case 1: hipLaunchKernelGGL(k_cells_grouped<1>, dim3(blocks), dim3(256), 0, st, D, start, nscen); break;
-    case 2: hipLaunchKernelGGL(k_cells_grouped<2>, dim3(blocks), dim3(256), 0, st, D, start, nscen); break;
-    default: hipLaunchKernelGGL(k_cells_grouped<3>, dim3(blocks), dim3(256), 0, st, D, start, nscen); break;
+    case 0: hipLaunchKernelGGL(k_cells_grouped<0>, dim3(blocks), dim3(256), 0, st, D, start, nscen, rev); break;
+    case 1: hipLaunchKernelGGL(k_cells_grouped<1>, dim3(blocks), dim3(256), 0, st, D, start, nscen, rev); break;
+    case 2: hipLaunchKernelGGL(k_cells_grouped<2>, dim3(blocks), dim3(256), 0, st, D, start, nscen, rev); break;
+    default: hipLaunchKernelGGL(k_cells_grouped<3>, dim3(blocks), dim3(256), 0, st, D, start, nscen, rev); break;
     }
     return hipGetLastError();
 }
@@ -516,14 +523,14 @@ hipError_t launch_qxy_assemble(hipStream_t st, const Consts &k, const State &s, 
     return hipGetLastError();
 }
 hipError_t launch_qx_tail(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const Work &w,
-                          double f0, bool write_uc)
+                          double f0, bool write_uc, int rev)
 {
     if (k.ny % 2 == 0 && k.ny >= 4) {
         const int rows = qx_tail_rows(k);
         hipLaunchKernelGGL(k_qx_tail2r, cell_grid(k.ny / 2, (k.nx + rows - 1) / rows), dim3(256), 0, st, k, m, s, w, f0,
-                           (int)write_uc, rows);
+                           (int)write_uc, rows, rev);
     } else {
-        hipLaunchKernelGGL(k_qx_tail, cell_grid(k.ny, k.nx), dim3(256), 0, st, k, m, s, w, f0, (int)write_uc);
+        hipLaunchKernelGGL(k_qx_tail, cell_grid(k.ny, k.nx), dim3(256), 0, st, k, m, s, w, f0, (int)write_uc, rev);
     }
     return hipGetLastError();
 }
@@ -539,12 +546,12 @@ hipError_t launch_qyx_assemble(hipStream_t st, const Consts &k, const Metrics &m
     hipLaunchKernelGGL(k_qyx_assemble, cell_grid(k.ny, k.nx - 1), dim3(256), 0, st, k, m, s, rf, w);
     return hipGetLastError();
 }
-hipError_t launch_qy_tail(hipStream_t st, const Consts &k, const State &s, bool write_vc)
+hipError_t launch_qy_tail(hipStream_t st, const Consts &k, const State &s, bool write_vc, int rev)
 {
     if (k.ny % 2 == 0 && k.ny >= 4)
-        hipLaunchKernelGGL(k_qy_tail2, cell_grid(k.ny / 2, k.nx), dim3(256), 0, st, k, s, (int)write_vc);
+        hipLaunchKernelGGL(k_qy_tail2, cell_grid(k.ny / 2, k.nx), dim3(256), 0, st, k, s, (int)write_vc, rev);
     else
-        hipLaunchKernelGGL(k_qy_tail, cell_grid(k.ny, k.nx), dim3(256), 0, st, k, s, (int)write_vc);
+        hipLaunchKernelGGL(k_qy_tail, cell_grid(k.ny, k.nx), dim3(256), 0, st, k, s, (int)write_vc, rev);
     return hipGetLastError();
 }
 hipError_t launch_nonfinite(hipStream_t st, const double *x, long n, int *flag)

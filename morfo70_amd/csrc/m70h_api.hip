@@ -51,6 +51,7 @@ struct m70h {
     double f0 = 0.0;
     std::vector<double> f0_series;
     long f0_pos = 0;
+    int sweep_rev = 0;   // row order of the next FUSED step's first kernel (run_step)
     int device = 0;
     hipStream_t own_stream = nullptr, stream = nullptr;
     int solver = M70H_SOLVER_EXACT;
@@ -197,19 +198,30 @@ int run_step(m70h *h, bool last = true, bool first = true)
         // x-sweeps: three coalesced passes (assemble+eliminate, reduced solve, back-sub);
         // y-sweeps: one launch each, the row's coefficients staged through LDS. The Qx
         // projector forms the Qx predictor's solution itself from the first two passes.
+        // Sweep order: each kernel takes the grid rows in the order opposite to the
+        // kernel before it, so it starts on the rows that kernel wrote last, which are
+        // still in the 256 MB MALL. Seven kernels of a step have a row order, so the
+        // order flips from step to step (sweep_rev). Results do not depend on it.
+        const int a = h->sweep_rev, b = 1 - a;
+        h->sweep_rev = b;
         for (int pass = COLS_ELIM; pass <= COLS_REDUCED; ++pass)
-            if ((rc = timed(h, K_COLS_QX_ELIM + pass,
-                            [&] { return launch_cols3_qxx(st, k, h->m, s, h->rf, w, f0, h->cw, h->tile_cols, pass, !first); })))
+            if ((rc = timed(h, K_COLS_QX_ELIM + pass, [&] {
+                     return launch_cols3_qxx(st, k, h->m, s, h->rf, w, f0, h->cw, h->tile_cols, pass, !first, a);
+                 })))
                 return rc;
-        if ((rc = timed(h, K_FUSED_QXY, [&] { return launch_lds_qxy_bs(st, k, s, h->rf, w, h->cw, h->rows_block, !first); })))
+        if ((rc = timed(h, K_FUSED_QXY,
+                        [&] { return launch_lds_qxy_bs(st, k, s, h->rf, w, h->cw, h->rows_block, !first, b); })))
             return rc;
-        if ((rc = timed(h, K_QX_TAIL, [&] { return launch_qx_tail(st, k, h->m, s, w, f0, last); }))) return rc;
-        if ((rc = timed(h, K_FUSED_QYY, [&] { return launch_lds_qyy(st, k, h->m, s, h->rf, w, h->rows_block); }))) return rc;
+        if ((rc = timed(h, K_QX_TAIL, [&] { return launch_qx_tail(st, k, h->m, s, w, f0, last, a); }))) return rc;
+        if ((rc = timed(h, K_FUSED_QYY, [&] { return launch_lds_qyy(st, k, h->m, s, h->rf, w, h->rows_block, b); })))
+            return rc;
         for (int pass = COLS_ELIM; pass <= COLS_BACKSUB; ++pass)
-            if ((rc = timed(h, K_COLS_QY_ELIM + pass,
-                            [&] { return launch_cols3_qyx(st, k, h->m, s, h->rf, w, h->cw, h->tile_cols, pass); })))
+            if ((rc = timed(h, K_COLS_QY_ELIM + pass, [&] {
+                     return launch_cols3_qyx(st, k, h->m, s, h->rf, w, h->cw, h->tile_cols, pass,
+                                             pass == COLS_BACKSUB ? b : a);
+                 })))
                 return rc;
-        if ((rc = timed(h, K_QY_TAIL, [&] { return launch_qy_tail(st, k, s, last); }))) return rc;
+        if ((rc = timed(h, K_QY_TAIL, [&] { return launch_qy_tail(st, k, s, last, a); }))) return rc;
         std::swap(s.D, s.Dn);
         return M70H_OK;
     }
@@ -966,17 +978,20 @@ static hipError_t batch_step(m70h_batch *b, int odd)
     const int *st_redp = b->d_start + GS_REDP * (n + 1);
     hipStream_t st = b->stream;
     hipError_t e;
+    // sweep order as run_step's: each stage starts where the one before it ended (those
+    // rows are still in the MALL), flipping from step to step with the D/Dn parity
+    const int ra = odd, rb = 1 - odd;
     // the Qx projector rows form the Qx predictor's solution from its first two passes
     if ((e = launch_cols3_grouped(st, 0, D, st_cols, st_red, st_redp, n, b->blocks[GS_COLS], b->blocks[GS_RED],
-                                  b->blocks[GS_REDP], false)) != hipSuccess)
+                                  b->blocks[GS_REDP], false, ra, rb)) != hipSuccess)
         return e;
-    if ((e = launch_rows_grouped(st, 0, b->rows_block, D, st_rows, n, b->blocks[GS_ROWS])) != hipSuccess) return e;
-    if ((e = launch_cells_grouped(st, 2, D, st_cells, n, b->blocks[GS_CELLS])) != hipSuccess) return e;
-    if ((e = launch_rows_grouped(st, 1, b->rows_block, D, st_rows, n, b->blocks[GS_ROWS])) != hipSuccess) return e;
+    if ((e = launch_rows_grouped(st, 0, b->rows_block, D, st_rows, n, b->blocks[GS_ROWS], rb)) != hipSuccess) return e;
+    if ((e = launch_cells_grouped(st, 2, D, st_cells, n, b->blocks[GS_CELLS], ra)) != hipSuccess) return e;
+    if ((e = launch_rows_grouped(st, 1, b->rows_block, D, st_rows, n, b->blocks[GS_ROWS], rb)) != hipSuccess) return e;
     if ((e = launch_cols3_grouped(st, 1, D, st_cols, st_red, st_redp, n, b->blocks[GS_COLS], b->blocks[GS_RED],
-                                  b->blocks[GS_REDP])) != hipSuccess)
+                                  b->blocks[GS_REDP], true, ra, rb)) != hipSuccess)
         return e;
-    return launch_cells_grouped(st, 3, D, st_cells, n, b->blocks[GS_CELLS]);
+    return launch_cells_grouped(st, 3, D, st_cells, n, b->blocks[GS_CELLS], ra);
 }
 
 extern "C" {

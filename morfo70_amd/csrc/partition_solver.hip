@@ -667,10 +667,11 @@ __device__ __forceinline__ void rows_lds_body(const Prov &prov, double *__restri
 
 template <int RB, class Prov>
 __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4, 4)))
-k_rows_lds(Prov prov, double *__restrict__ Y, long ldy, int n, int S, int periodic)
+k_rows_lds(Prov prov, double *__restrict__ Y, long ldy, int n, int S, int periodic, int rev)
 {
     __shared__ double sh[2][RowsLds<RB>::LDS];
-    rows_lds_body<RB>(prov, Y, ldy, n, S, periodic, (int)blockIdx.x, sh);
+    // rev: lines last to first
+    rows_lds_body<RB>(prov, Y, ldy, n, S, periodic, rev ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x, sh);
 }
 
 // Picks S (power of two, n >= 2S, ceil(n/S) <= M) for a line length n; 0 if none.
@@ -745,7 +746,7 @@ static hipError_t dispatch_cols(hipStream_t st, const Prov &prov, double *Y, lon
 
 template <class Prov>
 static hipError_t launch_rows_lds(hipStream_t st, const Prov &prov, double *Y, long ldy, int n, int nL, bool periodic,
-                                  int rb);
+                                  int rb, int rev = 0);
 
 // Row lines (y-sweeps): tile = unknowns per thread (8 or 16; 0 = default 8).
 template <class Prov>
@@ -894,35 +895,37 @@ __device__ __forceinline__ void cols_backsub_body(const ColsWork &cw, double *__
     }
 }
 
+// rev: take the segment groups last to first (see sweep order in m70h_api.hip).
 template <class Prov>
-__global__ void __launch_bounds__(256) k_cols_elim(Prov prov, ColsWork cw, int n, int ny)
+__global__ void __launch_bounds__(256) k_cols_elim(Prov prov, ColsWork cw, int n, int ny, int rev)
 {
-    cols_elim_body(prov, cw, n, ny, (int)blockIdx.x, (int)blockIdx.y);
+    cols_elim_body(prov, cw, n, ny, (int)blockIdx.x, rev ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y);
 }
 
-__global__ void __launch_bounds__(256) k_cols_backsub(ColsWork cw, double *__restrict__ Y, int n, int ny)
+__global__ void __launch_bounds__(256) k_cols_backsub(ColsWork cw, double *__restrict__ Y, int n, int ny, int rev)
 {
-    cols_backsub_body(cw, Y, n, ny, (int)blockIdx.x, (int)blockIdx.y);
+    cols_backsub_body(cw, Y, n, ny, (int)blockIdx.x, rev ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y);
 }
 
 // One pass of the three-pass column solve: COLS_ELIM (assemble + eliminate the segments),
 // COLS_REDUCED (the reduced systems, 2*nseg rows per column) or COLS_BACKSUB. The passes
-// run in this order on one stream; each is its own launch so it can be timed alone.
+// run in this order on one stream; each is its own launch so it can be timed alone. rev:
+// the elimination and back-substitution passes take the rows bottom to top.
 template <class Prov>
 static hipError_t launch_cols3(hipStream_t st, const Prov &prov, const ColsWork &cw, double *Y, int n, int ny, int tile,
-                               int pass)
+                               int pass, int rev)
 {
     if (n < 3 || cols_segments(n) != cw.nseg) return hipErrorInvalidValue;
     const dim3 grid((unsigned)((ny + 63) / 64), (unsigned)((cw.nseg + 3) / 4)), block(64, 4);
     switch (pass) {
     case COLS_ELIM:
-        hipLaunchKernelGGL((k_cols_elim<Prov>), grid, block, 0, st, prov, cw, n, ny);
+        hipLaunchKernelGGL((k_cols_elim<Prov>), grid, block, 0, st, prov, cw, n, ny, rev);
         return hipGetLastError();
     case COLS_REDUCED:
         // diagonal R1, lower R0, upper R2, rhs R3; 2*nseg rows per column
         return dispatch_cols(st, ProvArrays<true>{cw.R1, cw.R0, cw.R2, cw.R3, (long)ny}, cw.X, ny, 2 * cw.nseg, ny, tile);
     case COLS_BACKSUB:
-        hipLaunchKernelGGL(k_cols_backsub, grid, block, 0, st, cw, Y, n, ny);
+        hipLaunchKernelGGL(k_cols_backsub, grid, block, 0, st, cw, Y, n, ny, rev);
         return hipGetLastError();
     default:
         return hipErrorInvalidValue;
@@ -930,20 +933,20 @@ static hipError_t launch_cols3(hipStream_t st, const Prov &prov, const ColsWork 
 }
 
 hipError_t launch_cols3_qxx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                            const Work &w, double f0, const ColsWork &cw, int tile, int pass, bool vc_from_v)
+                            const Work &w, double f0, const ColsWork &cw, int tile, int pass, bool vc_from_v, int rev)
 {
     double *Y = s.Qx + k.ny;
     const int n = k.nx - 1, ny = k.ny;
     if (k.drag_model == 0)
-        return vc_from_v ? launch_cols3(st, ProvQxx<true, true>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass)
-                         : launch_cols3(st, ProvQxx<true, false>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass);
-    return vc_from_v ? launch_cols3(st, ProvQxx<false, true>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass)
-                     : launch_cols3(st, ProvQxx<false, false>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass);
+        return vc_from_v ? launch_cols3(st, ProvQxx<true, true>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass, rev)
+                         : launch_cols3(st, ProvQxx<true, false>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass, rev);
+    return vc_from_v ? launch_cols3(st, ProvQxx<false, true>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass, rev)
+                     : launch_cols3(st, ProvQxx<false, false>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass, rev);
 }
 hipError_t launch_cols3_qyx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                            const Work &w, const ColsWork &cw, int tile, int pass)
+                            const Work &w, const ColsWork &cw, int tile, int pass, int rev)
 {
-    return launch_cols3(st, ProvQyx{k, m, s, rf, w}, cw, s.Qy + k.ny, k.nx - 1, k.ny, tile, pass);
+    return launch_cols3(st, ProvQyx{k, m, s, rf, w}, cw, s.Qy + k.ny, k.nx - 1, k.ny, tile, pass, rev);
 }
 
 bool rows_lds_supported(int n) { return pick_segments(n, RM, 1024) > 0; }
@@ -1017,10 +1020,10 @@ struct GProvQxyBs {
 
 template <int WHICH, int RB>   // WHICH 0: Qx projector rows, 1: Qy predictor rows
 __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4, 4)))
-k_rows_lds_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start, int nscen)
+k_rows_lds_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start, int nscen, int rev)
 {
     __shared__ double sh[2][RowsLds<RB>::LDS];
-    const int b = blockIdx.x;
+    const int b = rev ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x;   // rev: last scenario's last row first
     const int sid = scen_of(start, nscen, b);
     const ScenDesc *d = D + sid;
     const int ny = d->k.ny;
@@ -1031,9 +1034,9 @@ k_rows_lds_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start
 
 template <int WHICH>   // 0: Qx predictor columns, 1: Qy projector columns
 __global__ void __launch_bounds__(256) k_cols_elim_grouped(const ScenDesc *__restrict__ D,
-                                                           const int *__restrict__ start, int nscen)
+                                                           const int *__restrict__ start, int nscen, int rev)
 {
-    const int b = blockIdx.x;
+    const int b = rev ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x;
     const int sid = scen_of(start, nscen, b);
     const ScenDesc *d = D + sid;
     const int ny = d->k.ny;
@@ -1046,9 +1049,9 @@ __global__ void __launch_bounds__(256) k_cols_elim_grouped(const ScenDesc *__res
 
 template <int WHICH>
 __global__ void __launch_bounds__(256) k_cols_backsub_grouped(const ScenDesc *__restrict__ D,
-                                                              const int *__restrict__ start, int nscen)
+                                                              const int *__restrict__ start, int nscen, int rev)
 {
-    const int b = blockIdx.x;
+    const int b = rev ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x;
     const int sid = scen_of(start, nscen, b);
     const ScenDesc *d = D + sid;
     const int ny = d->k.ny;
@@ -1188,24 +1191,27 @@ void grouped_stage_blocks(int nx, int ny, int rb, int *rows, int *cols, int *red
 }
 
 hipError_t launch_rows_grouped(hipStream_t st, int which, int rb, const ScenDesc *D, const int *start, int nscen,
-                               int blocks)
+                               int blocks, int rev)
 {
     if (rb == 256) {
-        if (which == 0) hipLaunchKernelGGL((k_rows_lds_grouped<0, 256>), dim3(blocks), dim3(256), 0, st, D, start, nscen);
-        else hipLaunchKernelGGL((k_rows_lds_grouped<1, 256>), dim3(blocks), dim3(256), 0, st, D, start, nscen);
+        if (which == 0)
+            hipLaunchKernelGGL((k_rows_lds_grouped<0, 256>), dim3(blocks), dim3(256), 0, st, D, start, nscen, rev);
+        else hipLaunchKernelGGL((k_rows_lds_grouped<1, 256>), dim3(blocks), dim3(256), 0, st, D, start, nscen, rev);
     } else {
-        if (which == 0) hipLaunchKernelGGL((k_rows_lds_grouped<0, 512>), dim3(blocks), dim3(512), 0, st, D, start, nscen);
-        else hipLaunchKernelGGL((k_rows_lds_grouped<1, 512>), dim3(blocks), dim3(512), 0, st, D, start, nscen);
+        if (which == 0)
+            hipLaunchKernelGGL((k_rows_lds_grouped<0, 512>), dim3(blocks), dim3(512), 0, st, D, start, nscen, rev);
+        else hipLaunchKernelGGL((k_rows_lds_grouped<1, 512>), dim3(blocks), dim3(512), 0, st, D, start, nscen, rev);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_cols3_grouped(hipStream_t st, int which, const ScenDesc *D, const int *start_cols,
                                 const int *start_red, const int *start_redp, int nscen, int blocks_cols, int blocks_red,
-                                int blocks_redp, bool backsub)
+                                int blocks_redp, bool backsub, int rev_elim, int rev_backsub)
 {
-    if (which == 0) hipLaunchKernelGGL(k_cols_elim_grouped<0>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
-    else hipLaunchKernelGGL(k_cols_elim_grouped<1>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
+    if (which == 0)
+        hipLaunchKernelGGL(k_cols_elim_grouped<0>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen, rev_elim);
+    else hipLaunchKernelGGL(k_cols_elim_grouped<1>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen, rev_elim);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (blocks_red > 0) {   // scenarios whose reduced solve fits 256-thread workgroups
@@ -1217,44 +1223,47 @@ hipError_t launch_cols3_grouped(hipStream_t st, int which, const ScenDesc *D, co
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (!backsub) return hipSuccess;
-    if (which == 0) hipLaunchKernelGGL(k_cols_backsub_grouped<0>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
-    else hipLaunchKernelGGL(k_cols_backsub_grouped<1>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen);
+    if (which == 0)
+        hipLaunchKernelGGL(k_cols_backsub_grouped<0>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen,
+                           rev_backsub);
+    else hipLaunchKernelGGL(k_cols_backsub_grouped<1>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen,
+                            rev_backsub);
     return hipGetLastError();
 }
 
 template <class Prov>
 static hipError_t launch_rows_lds(hipStream_t st, const Prov &prov, double *Y, long ldy, int n, int nL, bool periodic,
-                                  int rb)
+                                  int rb, int rev)
 {
     rb = rows_block_for(n, rb);
     const int S = pick_segments(n, RM, rb);
     if (S == 0) return hipErrorInvalidValue;
     if (rb == 256)
         hipLaunchKernelGGL((k_rows_lds<256, Prov>), dim3((unsigned)nL), dim3(256), 0, st, prov, Y, ldy, n, S,
-                           periodic ? 1 : 0);
+                           periodic ? 1 : 0, rev);
     else if (rb == 512)
         hipLaunchKernelGGL((k_rows_lds<512, Prov>), dim3((unsigned)nL), dim3(512), 0, st, prov, Y, ldy, n, S,
-                           periodic ? 1 : 0);
+                           periodic ? 1 : 0, rev);
     else
         hipLaunchKernelGGL((k_rows_lds<1024, Prov>), dim3((unsigned)nL), dim3(1024), 0, st, prov, Y, ldy, n, S,
-                           periodic ? 1 : 0);
+                           periodic ? 1 : 0, rev);
     return hipGetLastError();
 }
 
 // Row sweeps with the assembly staged through LDS (one row per block).
 hipError_t launch_lds_qxy_bs(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w,
-                             const ColsWork &cw, int rb, bool vc_from_v)
+                             const ColsWork &cw, int rb, bool vc_from_v, int rev)
 {
     if (vc_from_v)
-        return launch_rows_lds(st, ProvQxyBs<true>{k, s, rf, w, cw}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true, rb);
-    return launch_rows_lds(st, ProvQxyBs<false>{k, s, rf, w, cw}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true, rb);
+        return launch_rows_lds(st, ProvQxyBs<true>{k, s, rf, w, cw}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true, rb, rev);
+    return launch_rows_lds(st, ProvQxyBs<false>{k, s, rf, w, cw}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true, rb, rev);
 }
 hipError_t launch_lds_qyy(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                          const Work &w, int rb)
+                          const Work &w, int rb, int rev)
 {
     if (k.drag_model == 0)
-        return launch_rows_lds(st, ProvQyy<true>{k, m, s, rf, w}, s.Qy + k.ny, k.ny, k.ny, k.nx - 1, true, rb);
-    return launch_rows_lds(st, ProvQyy<false>{k, m, s, rf, w}, s.Qy + k.ny, k.ny, k.ny, k.nx - 1, true, rb);
+        return launch_rows_lds(st, ProvQyy<true>{k, m, s, rf, w}, s.Qy + k.ny, k.ny, k.ny, k.nx - 1, true, rb, rev);
+    return launch_rows_lds(st, ProvQyy<false>{k, m, s, rf, w}, s.Qy + k.ny, k.ny, k.ny, k.nx - 1, true, rb, rev);
 }
 
 }  // namespace m70
