@@ -2315,7 +2315,8 @@ hipError_t walloc(m70w *w, double **p, size_t n)
 }
 
 // Completes the pending field: reads the marching state after the last launched step
-// and launches more steps (intermediate rows) until the last mesh row is reached.
+// and launches more steps (intermediate rows) until the last mesh row is reached, then
+// the field-wide launch that forms the derived arrays.
 int wave_finish(m70w *w)
 {
     if (!w->pending) return M70H_OK;
@@ -2328,6 +2329,8 @@ int wave_finish(m70w *w)
             w->pending = false;
             w->last_steps = c.steps;
             if (c.err) return wfail(w, M70H_ENONFINITE, "wave field: the march took more than 1000*nx steps");
+            // the arrays storageRow derives from the marched ones, for the whole field
+            WHIPCHK(w, launch_wave_field_terms(w->stream, w->a));
             return M70H_OK;
         }
         if (w->next_s > limit) {
@@ -2384,19 +2387,14 @@ int m70w_create(int nx, int ny, const double *x, double dy, const m70w_params *p
         if ((e = walloc(w, &fld[q], F)) != hipSuccess) return bail(e);
     for (double **q : {&w->a.disp.lK, &w->a.disp.Sigma, &w->a.disp.C, &w->a.disp.Cg, &w->a.disp.Urms_H})
         if ((e = walloc(w, q, F)) != hipSuccess) return bail(e);
-    for (int r = 0; r < 2; ++r) {
-        double *buf = nullptr;
-        if ((e = walloc(w, &buf, (size_t)WROW_ARRAYS * ny)) != hipSuccess) return bail(e);
-        double **arr = &w->a.rows[r].D;
-        for (int q = 0; q < WROW_ARRAYS; ++q) arr[q] = buf + (size_t)q * ny;
-    }
+    for (int r = 0; r < 2; ++r)
+        if ((e = walloc(w, &w->a.rows[r], (size_t)WROW_ARRAYS * ny)) != hipSuccess) return bail(e);
     void *cbm = nullptr;
-    const size_t cb_bytes = 2 * sizeof(WaveCtl) + 3 * sizeof(unsigned long long) + 3 * sizeof(int);
+    const size_t cb_bytes = 2 * sizeof(WaveCtl) + 3 * sizeof(unsigned long long);
     if ((e = hipMalloc(&cbm, cb_bytes)) != hipSuccess) return bail(e);
     w->allocs.push_back(cbm);
     w->cb.ctl = (WaveCtl *)cbm;
     w->cb.acc = (unsigned long long *)((char *)cbm + 2 * sizeof(WaveCtl));
-    w->cb.anyv = (int *)((char *)w->cb.acc + 3 * sizeof(unsigned long long));
     WaveArgs &a = w->a;
     a.nx = nx;
     a.ny = ny;
@@ -2471,8 +2469,8 @@ int m70w_calculate_async(m70w *w, double hOff, double thetaOff, double T)
     if (w->hydro) w->a.D = w->hydro->s.D;
     // both rows start from zero, as the reference builds them afresh for every field
     for (int r = 0; r < 2; ++r)
-        WHIPCHK(w, hipMemsetAsync(w->a.rows[r].D, 0, (size_t)WROW_ARRAYS * w->ny * sizeof(double), w->stream));
-    WHIPCHK(w, hipMemsetAsync(w->cb.acc, 0, 3 * sizeof(unsigned long long) + 3 * sizeof(int), w->stream));
+        WHIPCHK(w, hipMemsetAsync(w->a.rows[r], 0, (size_t)WROW_ARRAYS * w->ny * sizeof(double), w->stream));
+    WHIPCHK(w, hipMemsetAsync(w->cb.acc, 0, 3 * sizeof(unsigned long long), w->stream));
     w->T = T;
     WHIPCHK(w, launch_wave_dispersion(w->stream, w->a, T));
     WHIPCHK(w, launch_wave_first(w->stream, w->a, hOff, thetaOff, T, w->cb));

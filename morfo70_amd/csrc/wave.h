@@ -7,13 +7,16 @@
 
 namespace m70 {
 
-// One WaveRow (include/WaveRow.h:22-75): per-node arrays of one cross-shore position.
-struct WRow {
-    double *D, *lK, *U, *V, *U_w, *TanTheta, *CosTheta, *SinTheta, *Theta, *Sigma, *C, *Cg, *dw, *df, *sxx, *sxy,
-        *syy, *sxx_r, *sxy_r, *syy_r, *lE, *E, *Dw, *H, *E_r, *d_r, *Cx, *Cy, *Cgx, *Cgy, *Urms_H, *inv_S_i, *Y_i, *G,
-        *Gp, *WC, *WC_r;
+// The arrays of a WaveRow (include/WaveRow.h:22-75) that the next marching step reads as
+// the previous row, ny each: array q of row buffer r at rows[r] + q*ny. The friction term
+// is kept only with bottom friction on, the rollers' arrays only with rollers on (a buffer
+// is zeroed before each field, and a term that is off is zero).
+enum WRowArr {
+    WR_H, WR_lE, WR_TanTheta, WR_lK, WR_Cg, WR_U, WR_V, WR_dw, WR_WC, WR_Cgx, WR_Cgy,   // always
+    WR_df,                                                                           // friction
+    WR_C, WR_WC_r, WR_Cx, WR_Cy, WR_E_r, WR_d_r, WR_Dw,                               // rollers
+    WROW_ARRAYS
 };
-constexpr int WROW_ARRAYS = 37;
 
 // The public arrays of class Wave (include/Wave.h:33-53), nx*ny each.
 struct WField {
@@ -36,7 +39,7 @@ struct WaveArgs {
     m70w_params p;
     const double *D, *U, *V, *U_w;     // depth and currents, nx*ny
     WField f;
-    WRow rows[2];
+    double *rows[2];                   // row buffers, WROW_ARRAYS x ny each
     WDisp disp;
 };
 
@@ -56,19 +59,23 @@ struct WaveCtl {
 };
 struct WaveCtlBufs {
     WaveCtl *ctl;               // [2]
-    unsigned long long *acc;    // [3] bit patterns of max |tan(theta)| of a row
-    int *anyv;                  // [3] the row had a non-NaN |tan(theta)|
+    unsigned long long *acc;    // [3] 1 + bit pattern of max |tan(theta)| of a row; 0: no value
 };
 
 // Wave::calculateWaveField (src/Wave.cpp:80-172): the first row (with ctl[0], acc[0]),
 // then marching steps s0 .. s0+count-1, one launch each. The caller zeroes both row
-// buffers, acc and anyv before the first row, launches at least nx-1 steps and reads
-// ctl[next & 1] to see whether more are needed (intermediate rows).
+// buffers and acc before the first row, launches at least nx-1 steps and reads
+// ctl[next & 1] to see whether more are needed (intermediate rows). The march stores
+// Theta, H, E, Dw and E_r of every mesh row; once it is done, launch_wave_field_terms
+// forms the field's other arrays.
 hipError_t launch_wave_first(hipStream_t st, const WaveArgs &a, double hOff, double thetaOff, double T,
                              const WaveCtlBufs &b);
 // WaveRow::dispersion (src/WaveParametrizations.cpp:14-60) of every mesh node into a.disp.
 hipError_t launch_wave_dispersion(hipStream_t st, const WaveArgs &a, double T);
 hipError_t launch_wave_steps(hipStream_t st, const WaveArgs &a, double T, int s0, int count, const WaveCtlBufs &b);
+// The rest of Wave::storageRow (src/Wave.cpp:175-264) over the whole field: CosTheta,
+// SinTheta, Cg, K, the radiation stresses, D_r and Urms from the stored arrays and a.disp.
+hipError_t launch_wave_field_terms(hipStream_t st, const WaveArgs &a);
 // Wave::setUV (src/Wave.cpp:293-302): U, V copied in; U_w from the stored direction.
 hipError_t launch_wave_set_uv(hipStream_t st, const WaveArgs &a, double *U, double *V, double *U_w,
                               const double *Usrc, const double *Vsrc, long n);

@@ -9,10 +9,17 @@
 // and the energy balance the neighbours' directions and currents, so a workgroup also
 // computes a halo of two nodes each side for the dispersion and one for the direction
 // (the same arithmetic as the owner's, so the values agree bit for bit) and needs no
-// other workgroup within the step. The largest |tan(theta)| of the new row is reduced by
-// an atomic maximum (non-negative doubles order as their bit patterns) into a three-slot
-// accumulator, and the step's control state (x0, mesh row, done) is double-buffered in
-// device memory, so that launch s reads what launch s-1 wrote. Control flow (mesh rows,
+// other workgroup within the step. The largest |tan(theta)| of the new row is reduced
+// across the wavefront by lane shuffles as soon as the direction is known, and folded by
+// one atomic maximum (non-negative doubles order as their bit patterns) into a three-slot
+// accumulator while the energy balance runs; the step's control state (x0, mesh row,
+// done) is double-buffered in device memory, so that launch s reads what launch s-1
+// wrote, and is written at the start of the step, which has all its inputs. A mesh row
+// stores the node values only the march computes (Theta, H, E, Dw, E_r); the arrays
+// Wave::storageRow derives from them and from the mesh node's dispersion are formed for
+// the whole field by one launch after the march (k_wave_field_terms), with the same
+// arithmetic. Each marching kernel is instantiated per breaking model and friction
+// switch, so the energy iteration is one branch-free block. Control flow (mesh rows,
 // intermediate rows, the step count) follows src/Wave.cpp:80-172, with the two row
 // buffers swapped after every step (the reference's brace-less `else` at :158-163 swaps
 // them only after intermediate rows, aliasing the buffers otherwise). Each node's
@@ -71,10 +78,13 @@ __device__ void w_dispersion(double d, double u_w, double w, double z0, double &
 }
 
 // Breaking dissipation Dw/E and d(Dw/E)/dH: Church-Thornton (:155-203), Thornton-Guza
-// (:206-251), regular waves (:254-302).
-__device__ void w_breaking(const m70w_params &p, double h, double sigma, double D, double &dw, double &ddw_dh)
+// (:206-251), regular waves (:254-302). BM is the model, or 0 to select it from p.
+template <int BM>
+__device__ __forceinline__ void w_breaking(const m70w_params &p, double h, double sigma, double D, double &dw,
+                                           double &ddw_dh)
 {
-    if (p.breaking_model == M70W_BREAK_CT) {
+    const int model = BM ? BM : p.breaking_model;
+    if (model == M70W_BREAK_CT) {
         double inv_gamma_b = 1 / p.gamma_b;
         double inv_4sqrtpi = 0.25 / WSQRT_PI;
         double B33 = 3 * p.B3;
@@ -94,7 +104,7 @@ __device__ void w_breaking(const m70w_params &p, double h, double sigma, double 
         double df3_h = (1 - tanh8ccm1_2) * 8. * dcc_dh;
         dw = h * f1 * f2 * f3;
         ddw_dh = f1 * (f2 * f3 + h * (df2_h * f3 + df3_h * f2));
-    } else if (p.breaking_model == M70W_BREAK_TG) {
+    } else if (model == M70W_BREAK_TG) {
         const double inv_gamma_b = 1 / p.gamma_b;
         const double kappa_sigma = 3 * p.B3 / (4 * WSQRT_PI * p.gamma_b * p.gamma_b);
         const double kappa = kappa_sigma * sigma;
@@ -133,7 +143,8 @@ __device__ void w_breaking(const m70w_params &p, double h, double sigma, double 
 
 // Bottom friction Df/E and d(Df/E)/dH, constant fw (:305-339) or Soulsby 1997 (:342-388);
 // sigma0 is the frequency of the row's first node, as the reference passes it.
-__device__ void w_friction(const m70w_params &p, double h, double sigma0, double urms_h, double &df, double &ddf_dh)
+__device__ __forceinline__ void w_friction(const m70w_params &p, double h, double sigma0, double urms_h, double &df,
+                                           double &ddf_dh)
 {
     if (p.friction_model == M70W_FRIC_CONSTANT) {
         double _64fw_6pig = 21.333333333 * p.fw0 * WINV_2PI / WG;
@@ -188,104 +199,87 @@ __device__ __forceinline__ void node_dir_terms(WNode &n, double theta, double co
     n.TanTheta = sinTheta / cosTheta;
 }
 
-// The arrays of a row that the next step reads as the previous row.
-__device__ __forceinline__ void write_row(const WRow &r, int j, const WNode &n)
+// The arrays of a row that the next step reads as the previous row (row buffer r).
+__device__ __forceinline__ void write_row(double *r, int ny, int j, const WNode &n, bool rollers, bool friction)
 {
-    r.H[j] = n.H;
-    r.lE[j] = n.lE;
-    r.TanTheta[j] = n.TanTheta;
-    r.lK[j] = n.lK;
-    r.Cg[j] = n.Cg;
-    r.C[j] = n.C;
-    r.U[j] = n.U;
-    r.V[j] = n.V;
-    r.dw[j] = n.dw;
-    r.df[j] = n.df;
-    r.Dw[j] = n.Dw;
-    r.WC[j] = n.WC;
-    r.WC_r[j] = n.WC_r;
-    r.Cx[j] = n.Cx;
-    r.Cy[j] = n.Cy;
-    r.Cgx[j] = n.Cgx;
-    r.Cgy[j] = n.Cgy;
-    r.E_r[j] = n.E_r;
-    r.d_r[j] = n.d_r;
+    auto at = [&](int q) -> double & { return r[(long)q * ny + j]; };
+    at(WR_H) = n.H;
+    at(WR_lE) = n.lE;
+    at(WR_TanTheta) = n.TanTheta;
+    at(WR_lK) = n.lK;
+    at(WR_Cg) = n.Cg;
+    at(WR_U) = n.U;
+    at(WR_V) = n.V;
+    at(WR_dw) = n.dw;
+    at(WR_WC) = n.WC;
+    at(WR_Cgx) = n.Cgx;
+    at(WR_Cgy) = n.Cgy;
+    if (friction) at(WR_df) = n.df;
+    if (rollers) {
+        at(WR_C) = n.C;
+        at(WR_WC_r) = n.WC_r;
+        at(WR_Cx) = n.Cx;
+        at(WR_Cy) = n.Cy;
+        at(WR_E_r) = n.E_r;
+        at(WR_d_r) = n.d_r;
+        at(WR_Dw) = n.Dw;
+    }
 }
 
-// Wave::storageRow (src/Wave.cpp:175-264) of one node into element o of the field.
-__device__ __forceinline__ void store_node(const WField &f, long o, const WNode &n)
+// The node values of Wave::storageRow (src/Wave.cpp:175-264) that only the march computes,
+// into element o of the field (E_r is zero without rollers, as allocated).
+__device__ __forceinline__ void store_node(const WField &f, long o, const WNode &n, bool rollers)
 {
     f.Theta[o] = n.Theta;
-    f.CosTheta[o] = n.CosTheta;
-    f.SinTheta[o] = n.SinTheta;
-    f.Cg[o] = n.Cg;
     f.H[o] = n.H;
+    f.E[o] = n.E;
     f.Dw[o] = n.Dw;
-    f.K[o] = n.lK;
-    const double e = n.E, e_r = n.E_r;
-    const double sxx_w = n.sxx * e, sxy_w = n.sxy * e, syy_w = n.syy * e;
-    const double sxx_r = n.sxx_r * e_r, sxy_r = n.sxy_r * e_r, syy_r = n.syy_r * e_r;
-    f.E[o] = e;
-    f.Sxx_w[o] = sxx_w;
-    f.Sxy_w[o] = sxy_w;
-    f.Syy_w[o] = syy_w;
-    f.E_r[o] = e_r;
-    f.D_r[o] = n.d_r * e_r;
-    f.Sxx_r[o] = sxx_r;
-    f.Sxy_r[o] = sxy_r;
-    f.Syy_r[o] = syy_r;
-    f.Sxx[o] = sxx_w + sxx_r;
-    f.Sxy[o] = sxy_w + sxy_r;
-    f.Syy[o] = syy_w + syy_r;
-    f.Urms[o] = n.Urms_H * n.H;
+    if (rollers) f.E_r[o] = n.E_r;
 }
 
-// Block maximum of |tan(theta)| over the valid values of the workgroup's nodes, folded
-// into acc (bit pattern of a non-negative double) and any (a value was valid).
-__device__ __forceinline__ void fold_tan_max(double m, int any, unsigned long long *acc, int *anyv)
+// Wavefront maximum of v over its lanes (v >= 0 for a valid |tan(theta)|, -1 for a NaN or
+// a lane without an owned node), folded by lane 0 into acc as 1 + its bit pattern: non-
+// negative doubles order as their bit patterns, and 0 stays "no valid value". Every lane
+// of the wavefront calls it.
+__device__ __forceinline__ void fold_tan_max(double v, unsigned long long *acc)
 {
-    __shared__ double s_max[WB];
-    __shared__ int s_any[WB];
-    const int t = threadIdx.x;
-    s_max[t] = m;
-    s_any[t] = any;
-    __syncthreads();
     for (int o = WB / 2; o > 0; o >>= 1) {
-        if (t < o) {
-            if (s_max[t + o] > s_max[t]) s_max[t] = s_max[t + o];
-            s_any[t] |= s_any[t + o];
-        }
-        __syncthreads();
+        const double w = __shfl_xor(v, o, WB);
+        v = (w > v) ? w : v;
     }
-    if (t == 0 && s_any[0]) {
-        atomicMax(acc, (unsigned long long)__double_as_longlong(s_max[0]));
-        atomicOr(anyv, 1);
-    }
+    if (threadIdx.x == 0 && v >= 0) atomicMax(acc, (unsigned long long)__double_as_longlong(v) + 1ull);
+}
+
+// The largest |tan(theta)| of the row a launch reads its step length from (-999 when the
+// row had no valid value, the reference's initial value, src/Wave.cpp:267-290).
+__device__ __forceinline__ double tan_max_of(unsigned long long v)
+{
+    return v ? __longlong_as_double((long long)(v - 1ull)) : -999.0;
 }
 
 // WaveRow::initFirstRow (src/WaveRow.cpp:129-232) and storageRow(0): one thread per
 // node of the workgroup's WNI nodes.
 __global__ void __launch_bounds__(WB) k_wave_first(WaveArgs a, double hOff, double thetaOff, double T, WaveCtl *ctl,
-                                                   unsigned long long *acc, int *anyv)
+                                                   unsigned long long *acc)
 {
     __shared__ double s_dw0, s_df0;
     const int t = threadIdx.x, k = blockIdx.x;
     const int ny = a.ny;
     const m70w_params &p = a.p;
+    const bool rollers = p.rollers != 0, friction = p.bottom_friction != 0;
     const double w = 2 * WPI / T;
     if (t == 0) {   // node 0's depth, the absolute frequency and the offshore height (:143-146)
         const double Uh = a.disp.Urms_H[0];
         double dw0 = 0, ddw_dh0 = 0, df0 = 0, ddf_dh0 = 0;
-        w_breaking(p, hOff, w, a.D[0], dw0, ddw_dh0);
-        if (p.bottom_friction) w_friction(p, hOff, w, Uh, df0, ddf_dh0);
+        w_breaking<0>(p, hOff, w, a.D[0], dw0, ddw_dh0);
+        if (friction) w_friction(p, hOff, w, Uh, df0, ddf_dh0);
         s_dw0 = dw0;
         s_df0 = df0;
     }
     __syncthreads();
     const int j0 = k * WNI;
     const int nint = min(WNI, ny - j0);
-    double m = -999;
-    int any = 0;
+    double m = -1;
     if (t < nint) {
         const int j = j0 + t;
         WNode n{};
@@ -308,15 +302,12 @@ __global__ void __launch_bounds__(WB) k_wave_first(WaveArgs a, double hOff, doub
         n.lE = log(EOff);
         n.dw = s_dw0;
         n.df = s_df0;
-        write_row(a.rows[0], j, n);
-        store_node(a.f, j, n);
+        write_row(a.rows[0], ny, j, n, rollers, friction);
+        store_node(a.f, j, n, rollers);
         const double tt = fabs(n.TanTheta);
-        if (tt > m) {
-            m = tt;
-            any = 1;
-        }
+        if (tt >= 0) m = tt;
     }
-    fold_tan_max(m, any, &acc[0], &anyv[0]);
+    fold_tan_max(m, &acc[0]);
     if (k == 0 && t == 0) {
         WaveCtl c{};
         c.x0 = a.x[0];
@@ -335,26 +326,33 @@ struct PrevRow {
     double c_p, c_m, wcr_p, wcr_m, cx_j, cy_p, cy_m, er_p, er_m, er_j, dr_p, dr_m, Dw_p, Dw_m;   // rollers
 };
 
-__device__ __forceinline__ PrevRow load_prev(const WRow &r, int j, int jp, int jm, bool rollers)
+__device__ __forceinline__ PrevRow load_prev(const double *rb, int ny, int j, int jp, int jm, bool rollers,
+                                             bool friction)
 {
+    auto r = [&](int q) { return rb + (long)q * ny; };
     PrevRow P;
-    P.tt_p = r.TanTheta[jp]; P.tt_m = r.TanTheta[jm];
-    P.lk_j = r.lK[j]; P.lk_p = r.lK[jp]; P.lk_m = r.lK[jm];
-    P.cg_p = r.Cg[jp]; P.cg_m = r.Cg[jm];
-    P.u_p = r.U[jp]; P.u_m = r.U[jm]; P.u_j = r.U[j];
-    P.v_p = r.V[jp]; P.v_m = r.V[jm]; P.v_j = r.V[j];
-    P.dw_p = r.dw[jp]; P.dw_m = r.dw[jm]; P.df_p = r.df[jp]; P.df_m = r.df[jm];
-    P.le_p = r.lE[jp]; P.le_m = r.lE[jm]; P.le_j = r.lE[j];
-    P.wc_p = r.WC[jp]; P.wc_m = r.WC[jm];
-    P.cgx_j = r.Cgx[j]; P.cgy_p = r.Cgy[jp]; P.cgy_m = r.Cgy[jm];
-    P.h_j = r.H[j];
+    P.tt_p = r(WR_TanTheta)[jp]; P.tt_m = r(WR_TanTheta)[jm];
+    P.lk_j = r(WR_lK)[j]; P.lk_p = r(WR_lK)[jp]; P.lk_m = r(WR_lK)[jm];
+    P.cg_p = r(WR_Cg)[jp]; P.cg_m = r(WR_Cg)[jm];
+    P.u_p = r(WR_U)[jp]; P.u_m = r(WR_U)[jm]; P.u_j = r(WR_U)[j];
+    P.v_p = r(WR_V)[jp]; P.v_m = r(WR_V)[jm]; P.v_j = r(WR_V)[j];
+    P.dw_p = r(WR_dw)[jp]; P.dw_m = r(WR_dw)[jm];
+    if (friction) {
+        P.df_p = r(WR_df)[jp]; P.df_m = r(WR_df)[jm];
+    } else {   // df is zero on every row without friction
+        P.df_p = P.df_m = 0.0;
+    }
+    P.le_p = r(WR_lE)[jp]; P.le_m = r(WR_lE)[jm]; P.le_j = r(WR_lE)[j];
+    P.wc_p = r(WR_WC)[jp]; P.wc_m = r(WR_WC)[jm];
+    P.cgx_j = r(WR_Cgx)[j]; P.cgy_p = r(WR_Cgy)[jp]; P.cgy_m = r(WR_Cgy)[jm];
+    P.h_j = r(WR_H)[j];
     if (rollers) {
-        P.c_p = r.C[jp]; P.c_m = r.C[jm];
-        P.wcr_p = r.WC_r[jp]; P.wcr_m = r.WC_r[jm];
-        P.cx_j = r.Cx[j]; P.cy_p = r.Cy[jp]; P.cy_m = r.Cy[jm];
-        P.er_p = r.E_r[jp]; P.er_m = r.E_r[jm]; P.er_j = r.E_r[j];
-        P.dr_p = r.d_r[jp]; P.dr_m = r.d_r[jm];
-        P.Dw_p = r.Dw[jp]; P.Dw_m = r.Dw[jm];
+        P.c_p = r(WR_C)[jp]; P.c_m = r(WR_C)[jm];
+        P.wcr_p = r(WR_WC_r)[jp]; P.wcr_m = r(WR_WC_r)[jm];
+        P.cx_j = r(WR_Cx)[j]; P.cy_p = r(WR_Cy)[jp]; P.cy_m = r(WR_Cy)[jm];
+        P.er_p = r(WR_E_r)[jp]; P.er_m = r(WR_E_r)[jm]; P.er_j = r(WR_E_r)[j];
+        P.dr_p = r(WR_d_r)[jp]; P.dr_m = r(WR_d_r)[jm];
+        P.Dw_p = r(WR_Dw)[jp]; P.Dw_m = r(WR_Dw)[jm];
     } else {
         P.c_p = P.c_m = P.wcr_p = P.wcr_m = P.cx_j = P.cy_p = P.cy_m = 0.0;
         P.er_p = P.er_m = P.er_j = P.dr_p = P.dr_m = P.Dw_p = P.Dw_m = 0.0;
@@ -368,8 +366,9 @@ __device__ __forceinline__ PrevRow load_prev(const WRow &r, int j, int jp, int j
 // (src/WaveEnergy.cpp), solveRollers (src/WaveRollers.cpp) and storageRow when the row is a
 // mesh row. Thread t of workgroup k holds node (k*WNI + t - 2) mod ny: slots 0..nint+3
 // take the dispersion, 1..nint+2 the direction, 2..nint+1 (the owned nodes) the rest.
-__global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, WaveCtl *ctl, unsigned long long *acc,
-                                                  int *anyv)
+// BM is the breaking model, FRIC the bottom friction switch.
+template <int BM, bool FRIC>
+__global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, WaveCtl *ctl, unsigned long long *acc)
 {
     const int t = threadIdx.x, k = blockIdx.x;
     const int ny = a.ny;
@@ -378,9 +377,10 @@ __global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, W
     int j = (j0 + t - 2) % ny;
     if (j < 0) j += ny;
     const int jp = (j == ny - 1) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;
-    const WRow &rm1 = a.rows[s & 1];
-    const PrevRow P = load_prev(rm1, j, jp, jm, a.p.rollers != 0);
+    const bool rollers = a.p.rollers != 0;
+    const PrevRow P = load_prev(a.rows[s & 1], ny, j, jp, jm, rollers, FRIC);
     const WaveCtl c = ctl[s & 1];
+    const unsigned long long acc_prev = acc[s % 3];
     if (c.done) {   // pass the final state on to the launches that follow
         if (k == 0 && t == 0) ctl[(s + 1) & 1] = c;
         return;
@@ -392,7 +392,7 @@ __global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, W
     const double inv_2dy = a.inv_2dy;
     const double w = 2 * WPI / T;
     // getDx
-    const double tanThetaMax = anyv[s % 3] ? __longlong_as_double((long long)acc[s % 3]) : -999.0;
+    const double tanThetaMax = tan_max_of(acc_prev);
     const double dx_remain = a.x[c.xIndex + 1] - c.x0;
     double dx_row;
     int store;
@@ -409,7 +409,20 @@ __global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, W
             store = 1;
         }
     }
-    const WRow &r = a.rows[(s & 1) ^ 1];
+    if (k == 0 && t == 0) {   // the next launch's control state: its inputs are all known here
+        WaveCtl nc = c;
+        nc.x0 = c.x0 + dx_row;
+        if (store) nc.xIndex = c.xIndex + 1;
+        nc.steps = c.steps + 1;
+        nc.done = (nc.xIndex >= nx - 1);
+        if (nc.steps > 1000L * nx + 1000) {
+            nc.done = 1;
+            nc.err = 1;
+        }
+        ctl[(s + 1) & 1] = nc;
+        acc[(s + 2) % 3] = 0;   // the slot launch s+1 folds into
+    }
+    double *const r = a.rows[(s & 1) ^ 1];
     const long index = (long)(c.xIndex + 1) * ny, index_m1 = (long)c.xIndex * ny;
     const double dx_dx = dx_row / a.dx[c.xIndex];
     WNode n{};
@@ -442,7 +455,7 @@ __global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, W
         sl_U[t] = n.U;
         sl_V[t] = n.V;
     }
-    if (t == 0 && p.bottom_friction && p.friction_model == M70W_FRIC_SOULSBY97) {
+    if (FRIC && t == 0 && p.friction_model == M70W_FRIC_SOULSBY97) {
         double d0, u0, v0, uw0, lK0, sg0, C0, Cg0, Uh0;   // Sigma of the row's first node
         if (store) {
             sg0 = a.disp.Sigma[index];
@@ -484,8 +497,14 @@ __global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, W
         sl_Cy[t] = n.Cy;
     }
     __syncthreads();
-    double m = -999;
-    int any = 0;
+    {   // the new row's largest |tan(theta)| (owned nodes), folded while the energy runs
+        double v = -1;
+        if (t >= 2 && t < nint + 2) {
+            const double tt = fabs(n.TanTheta);
+            if (tt >= 0) v = tt;
+        }
+        fold_tan_max(v, &acc[(s + 1) % 3]);
+    }
     if (t >= 2 && t < nint + 2) {
         const double y_i_dy = n.Y_i * inv_2dy;
         const double inv_s_i = n.inv_s_i;
@@ -520,8 +539,8 @@ __global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, W
             double le_old = P.le_j, H = P.h_j, E = 0.0, Dw = 0.0;
             double dw = 0.0, ddw_dh = 0.0, df = 0.0, ddf_dh = 0.0;
             for (int it = 0; it < p.max_iterations; it++) {
-                w_breaking(p, H, n.Sigma, n.D, dw, ddw_dh);
-                if (p.bottom_friction) w_friction(p, H, s_sigma0, n.Urms_H, df, ddf_dh);
+                w_breaking<BM>(p, H, n.Sigma, n.D, dw, ddw_dh);
+                if (FRIC) w_friction(p, H, s_sigma0, n.Urms_H, df, ddf_dh);
                 double gg = G;
                 double gp = Gp;
                 double le = le_old;
@@ -564,29 +583,51 @@ __global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, W
             double lowv = c_c * inv_s_i + u_c * inv_dx + div_cv_c + 0.5 * wc_r_c + 0.5 * d_r_c;
             n.E_r = upv / lowv;
         }
-        write_row(r, j, n);
-        if (store) store_node(a.f, index + j, n);
-        const double tt = fabs(n.TanTheta);
-        if (tt > m) {
-            m = tt;
-            any = 1;
-        }
+        write_row(r, ny, j, n, rollers, FRIC);
+        if (store) store_node(a.f, index + j, n, rollers);
     }
-    fold_tan_max(m, any, &acc[(s + 1) % 3], &anyv[(s + 1) % 3]);
-    if (k == 0 && t == 0) {
-        WaveCtl nc = c;
-        nc.x0 = c.x0 + dx_row;
-        if (store) nc.xIndex = c.xIndex + 1;
-        nc.steps = c.steps + 1;
-        nc.done = (nc.xIndex >= nx - 1);
-        if (nc.steps > 1000L * nx + 1000) {
-            nc.done = 1;
-            nc.err = 1;
-        }
-        ctl[(s + 1) & 1] = nc;
-        acc[(s + 2) % 3] = 0;   // the slot launch s+1 folds into
-        anyv[(s + 2) % 3] = 0;
+}
+
+// The arrays Wave::storageRow (src/Wave.cpp:175-264) derives, for every node of the field:
+// the direction's cosine and sine, the mesh node's Cg and log k, the radiation stresses
+// from E and E_r, the rollers' dissipation and Urms. The node terms are node_dir_terms',
+// on the stored Theta and the mesh node's C and Cg (a mesh row's dispersion is a.disp's),
+// so each value is the one storageRow forms from the marching node; d_r is zero on the
+// first row and without rollers.
+__global__ void __launch_bounds__(256) k_wave_field_terms(WaveArgs a, long n)
+{
+    const long o = (long)blockIdx.x * 256 + threadIdx.x;
+    if (o >= n) return;
+    const WField &f = a.f;
+    WNode nd{};
+    nd.C = a.disp.C[o];
+    nd.Cg = a.disp.Cg[o];
+    const double theta = f.Theta[o];
+    const double cosTheta = cos(theta), sinTheta = sin(theta);
+    node_dir_terms(nd, theta, cosTheta, sinTheta, cosTheta * sinTheta);
+    double d_r = 0.0;
+    if (a.p.rollers && o >= a.ny) {   // solveRollers (src/WaveRollers.cpp:57-100)
+        const double d_r_div_c = 2 * WG * a.p.sinBeta_r;
+        d_r = d_r_div_c / nd.C;
     }
+    const double e = f.E[o], e_r = f.E_r[o];
+    const double sxx_w = nd.sxx * e, sxy_w = nd.sxy * e, syy_w = nd.syy * e;
+    const double sxx_r = nd.sxx_r * e_r, sxy_r = nd.sxy_r * e_r, syy_r = nd.syy_r * e_r;
+    f.CosTheta[o] = cosTheta;
+    f.SinTheta[o] = sinTheta;
+    f.Cg[o] = nd.Cg;
+    f.K[o] = a.disp.lK[o];
+    f.Sxx_w[o] = sxx_w;
+    f.Sxy_w[o] = sxy_w;
+    f.Syy_w[o] = syy_w;
+    f.D_r[o] = d_r * e_r;
+    f.Sxx_r[o] = sxx_r;
+    f.Sxy_r[o] = sxy_r;
+    f.Syy_r[o] = syy_r;
+    f.Sxx[o] = sxx_w + sxx_r;
+    f.Sxy[o] = sxy_w + sxy_r;
+    f.Syy[o] = syy_w + syy_r;
+    f.Urms[o] = a.disp.Urms_H[o] * f.H[o];
 }
 
 __global__ void __launch_bounds__(256) k_wave_dispersion(WaveArgs a, double w, long n)
@@ -614,7 +655,7 @@ hipError_t launch_wave_first(hipStream_t st, const WaveArgs &a, double hOff, dou
                              const WaveCtlBufs &b)
 {
     hipLaunchKernelGGL(k_wave_first, dim3((unsigned)wave_blocks(a.ny)), dim3(WB), 0, st, a, hOff, thetaOff, T, b.ctl,
-                       b.acc, b.anyv);
+                       b.acc);
     return hipGetLastError();
 }
 
@@ -625,11 +666,31 @@ hipError_t launch_wave_dispersion(hipStream_t st, const WaveArgs &a, double T)
     return hipGetLastError();
 }
 
+using WaveStepKernel = void (*)(WaveArgs, double, int, WaveCtl *, unsigned long long *);
+
+template <bool FRIC>
+WaveStepKernel wave_step_kernel(int breaking_model)
+{
+    switch (breaking_model) {
+    case M70W_BREAK_CT: return k_wave_step<M70W_BREAK_CT, FRIC>;
+    case M70W_BREAK_TG: return k_wave_step<M70W_BREAK_TG, FRIC>;
+    default: return k_wave_step<M70W_BREAK_REGULAR, FRIC>;
+    }
+}
+
 hipError_t launch_wave_steps(hipStream_t st, const WaveArgs &a, double T, int s0, int count, const WaveCtlBufs &b)
 {
+    const WaveStepKernel kern =
+        a.p.bottom_friction ? wave_step_kernel<true>(a.p.breaking_model) : wave_step_kernel<false>(a.p.breaking_model);
     for (int s = s0; s < s0 + count; ++s)
-        hipLaunchKernelGGL(k_wave_step, dim3((unsigned)wave_blocks(a.ny)), dim3(WB), 0, st, a, T, s, b.ctl, b.acc,
-                           b.anyv);
+        hipLaunchKernelGGL(kern, dim3((unsigned)wave_blocks(a.ny)), dim3(WB), 0, st, a, T, s, b.ctl, b.acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_wave_field_terms(hipStream_t st, const WaveArgs &a)
+{
+    const long n = (long)a.nx * a.ny;
+    hipLaunchKernelGGL(k_wave_field_terms, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, n);
     return hipGetLastError();
 }
 
