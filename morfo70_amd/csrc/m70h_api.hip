@@ -2283,7 +2283,7 @@ struct m70w {
     double *D = nullptr, *U = nullptr, *V = nullptr, *U_w = nullptr;   // owned inputs
     WaveCtlBufs cb{};
     double T = 0.0;      // wave period of the pending field
-    int next_s = 0;      // marching steps launched for the pending field
+    int next_s = 0;      // next marching launch of the pending field (launch 0 is the first row)
     long last_steps = -1;
     bool pending = false;
     m70h *hydro = nullptr;
@@ -2322,8 +2322,9 @@ int wave_finish(m70w *w)
     if (!w->pending) return M70H_OK;
     const long limit = 1000L * w->nx + 1000 + 64;
     for (;;) {
-        WaveCtl c{};
-        WHIPCHK(w, hipMemcpyAsync(&c, w->cb.ctl + (w->next_s & 1), sizeof(c), hipMemcpyDeviceToHost, w->stream));
+        WaveCtl c{};   // written by the last launch
+        WHIPCHK(w, hipMemcpyAsync(&c, w->cb.ctl + ((w->next_s - 1) & 1), sizeof(c), hipMemcpyDeviceToHost,
+                                  w->stream));
         WHIPCHK(w, hipStreamSynchronize(w->stream));
         if (c.done) {
             w->pending = false;
@@ -2337,8 +2338,8 @@ int wave_finish(m70w *w)
             w->pending = false;
             return wfail(w, M70H_ENONFINITE, "wave field: the march did not finish");
         }
-        WHIPCHK(w, launch_wave_steps(w->stream, w->a, w->T, w->next_s, 64, w->cb));
-        w->next_s += 64;
+        WHIPCHK(w, launch_wave_steps(w->stream, w->a, w->T, w->next_s, 16, w->cb));
+        w->next_s += 16;
     }
 }
 
@@ -2388,9 +2389,9 @@ int m70w_create(int nx, int ny, const double *x, double dy, const m70w_params *p
     for (double **q : {&w->a.disp.lK, &w->a.disp.Sigma, &w->a.disp.C, &w->a.disp.Cg, &w->a.disp.Urms_H})
         if ((e = walloc(w, q, F)) != hipSuccess) return bail(e);
     for (int r = 0; r < 2; ++r)
-        if ((e = walloc(w, &w->a.rows[r], (size_t)WROW_ARRAYS * ny)) != hipSuccess) return bail(e);
+        if ((e = walloc(w, &w->a.rows[r], (size_t)WKMAX * WROW_ARRAYS * ny)) != hipSuccess) return bail(e);
     void *cbm = nullptr;
-    const size_t cb_bytes = 2 * sizeof(WaveCtl) + 3 * sizeof(unsigned long long);
+    const size_t cb_bytes = 2 * sizeof(WaveCtl) + 3 * WACC_SLOTS * sizeof(unsigned long long);
     if ((e = hipMalloc(&cbm, cb_bytes)) != hipSuccess) return bail(e);
     w->allocs.push_back(cbm);
     w->cb.ctl = (WaveCtl *)cbm;
@@ -2467,16 +2468,17 @@ int m70w_calculate_async(m70w *w, double hOff, double thetaOff, double T)
     int rc = wave_finish(w);
     if (rc) return rc;
     if (w->hydro) w->a.D = w->hydro->s.D;
-    // both rows start from zero, as the reference builds them afresh for every field
-    for (int r = 0; r < 2; ++r)
-        WHIPCHK(w, hipMemsetAsync(w->a.rows[r], 0, (size_t)WROW_ARRAYS * w->ny * sizeof(double), w->stream));
-    WHIPCHK(w, hipMemsetAsync(w->cb.acc, 0, 3 * sizeof(unsigned long long), w->stream));
+    // every row a step reads was written by the step before it (the first row by launch 0),
+    // so the row buffers need no clearing; the accumulators do
+    WHIPCHK(w, hipMemsetAsync(w->cb.acc, 0, 3 * WACC_SLOTS * sizeof(unsigned long long), w->stream));
     w->T = T;
     WHIPCHK(w, launch_wave_dispersion(w->stream, w->a, T));
     WHIPCHK(w, launch_wave_first(w->stream, w->a, hOff, thetaOff, T, w->cb));
-    // every field has at least nx-1 steps (one per mesh row); wave_finish adds the rest
-    WHIPCHK(w, launch_wave_steps(w->stream, w->a, T, 0, w->nx - 1, w->cb));
-    w->next_s = w->nx - 1;
+    // the launches a field of mesh rows only takes; wave_finish adds what intermediate
+    // rows need
+    const int n0 = wave_march_launches(w->nx);
+    WHIPCHK(w, launch_wave_steps(w->stream, w->a, T, 1, n0, w->cb));
+    w->next_s = 1 + n0;
     w->pending = true;
     return M70H_OK;
 }

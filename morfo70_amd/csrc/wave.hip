@@ -28,6 +28,8 @@
 // operation order (-ffp-contract=off); transcendental functions are the device library's.
 #include "wave.h"
 
+#include <algorithm>
+
 namespace m70 {
 
 // include/OptionsParametersConstants.h:14-21
@@ -258,7 +260,8 @@ __device__ __forceinline__ double tan_max_of(unsigned long long v)
 }
 
 // WaveRow::initFirstRow (src/WaveRow.cpp:129-232) and storageRow(0): one thread per
-// node of the workgroup's WNI nodes.
+// node of the workgroup's WNI nodes. This is launch 0 of the march: its one row goes to
+// row buffer set 0, slot 1, its largest |tan(theta)| to accumulator set 0, slot 1.
 __global__ void __launch_bounds__(WB) k_wave_first(WaveArgs a, double hOff, double thetaOff, double T, WaveCtl *ctl,
                                                    unsigned long long *acc)
 {
@@ -307,18 +310,17 @@ __global__ void __launch_bounds__(WB) k_wave_first(WaveArgs a, double hOff, doub
         const double tt = fabs(n.TanTheta);
         if (tt >= 0) m = tt;
     }
-    fold_tan_max(m, &acc[0]);
+    fold_tan_max(m, &acc[1]);
     if (k == 0 && t == 0) {
         WaveCtl c{};
         c.x0 = a.x[0];
         c.done = (a.nx - 1 <= 0);
+        c.K = 1;
         ctl[0] = c;
     }
 }
 
-// The previous row's values one node's step reads (up = j+1, low = j-1, periodic): read
-// at the start of the launch, so that their latency overlaps the control read and the
-// dispersion and direction instead of stalling each phase.
+// The previous row's values one node's step reads (up = j+1, low = j-1, periodic).
 struct PrevRow {
     double tt_p, tt_m, lk_j, lk_p, lk_m;                                       // direction
     double cg_p, cg_m, u_p, u_m, u_j, v_p, v_m, v_j, dw_p, dw_m, df_p, df_m;  // energy
@@ -326,76 +328,42 @@ struct PrevRow {
     double c_p, c_m, wcr_p, wcr_m, cx_j, cy_p, cy_m, er_p, er_m, er_j, dr_p, dr_m, Dw_p, Dw_m;   // rollers
 };
 
-__device__ __forceinline__ PrevRow load_prev(const double *rb, int ny, int j, int jp, int jm, bool rollers,
-                                             bool friction)
+// A node's own previous-row values (the rest of PrevRow is read from the neighbours).
+struct PrevOwn {
+    double lk, u, v, le, cgx, h, cx, er;
+};
+
+// The previous row's values the neighbours read, one LDS array per value, indexed by lane.
+enum PrevLds { PL_TT, PL_LK, PL_CG, PL_U, PL_V, PL_DW, PL_DF, PL_LE, PL_WC, PL_CGY,   // always
+               PL_C, PL_WCR, PL_CY, PL_ER, PL_DR, PL_DWE,                          // rollers
+               PL_N };
+
+// A mesh node's inputs: depth, currents and the dispersion solved before the march.
+struct MeshNode {
+    double D, U, V, U_w, lK, Sigma, C, Cg, Urms_H;
+};
+
+__device__ __forceinline__ MeshNode load_mesh(const WaveArgs &a, long o)
 {
-    auto r = [&](int q) { return rb + (long)q * ny; };
-    PrevRow P;
-    P.tt_p = r(WR_TanTheta)[jp]; P.tt_m = r(WR_TanTheta)[jm];
-    P.lk_j = r(WR_lK)[j]; P.lk_p = r(WR_lK)[jp]; P.lk_m = r(WR_lK)[jm];
-    P.cg_p = r(WR_Cg)[jp]; P.cg_m = r(WR_Cg)[jm];
-    P.u_p = r(WR_U)[jp]; P.u_m = r(WR_U)[jm]; P.u_j = r(WR_U)[j];
-    P.v_p = r(WR_V)[jp]; P.v_m = r(WR_V)[jm]; P.v_j = r(WR_V)[j];
-    P.dw_p = r(WR_dw)[jp]; P.dw_m = r(WR_dw)[jm];
-    if (friction) {
-        P.df_p = r(WR_df)[jp]; P.df_m = r(WR_df)[jm];
-    } else {   // df is zero on every row without friction
-        P.df_p = P.df_m = 0.0;
-    }
-    P.le_p = r(WR_lE)[jp]; P.le_m = r(WR_lE)[jm]; P.le_j = r(WR_lE)[j];
-    P.wc_p = r(WR_WC)[jp]; P.wc_m = r(WR_WC)[jm];
-    P.cgx_j = r(WR_Cgx)[j]; P.cgy_p = r(WR_Cgy)[jp]; P.cgy_m = r(WR_Cgy)[jm];
-    P.h_j = r(WR_H)[j];
-    if (rollers) {
-        P.c_p = r(WR_C)[jp]; P.c_m = r(WR_C)[jm];
-        P.wcr_p = r(WR_WC_r)[jp]; P.wcr_m = r(WR_WC_r)[jm];
-        P.cx_j = r(WR_Cx)[j]; P.cy_p = r(WR_Cy)[jp]; P.cy_m = r(WR_Cy)[jm];
-        P.er_p = r(WR_E_r)[jp]; P.er_m = r(WR_E_r)[jm]; P.er_j = r(WR_E_r)[j];
-        P.dr_p = r(WR_d_r)[jp]; P.dr_m = r(WR_d_r)[jm];
-        P.Dw_p = r(WR_Dw)[jp]; P.Dw_m = r(WR_Dw)[jm];
-    } else {
-        P.c_p = P.c_m = P.wcr_p = P.wcr_m = P.cx_j = P.cy_p = P.cy_m = 0.0;
-        P.er_p = P.er_m = P.er_j = P.dr_p = P.dr_m = P.Dw_p = P.Dw_m = 0.0;
-    }
-    return P;
+    MeshNode m;
+    m.D = a.D[o];
+    m.U = a.U[o];
+    m.V = a.V[o];
+    m.U_w = a.U_w[o];
+    m.lK = a.disp.lK[o];
+    m.Sigma = a.disp.Sigma[o];
+    m.C = a.disp.C[o];
+    m.Cg = a.disp.Cg[o];
+    m.Urms_H = a.disp.Urms_H[o];
+    return m;
 }
 
-// One marching step s (rows s&1 -> (s&1)^1): Wave::getDx (src/Wave.cpp:267-290),
-// WaveRow::initRow (src/WaveRow.cpp:235-296), dispersion (src/WaveParametrizations.cpp:
-// 14-60), solveDir (src/WaveDir.cpp:12-126), energyPreprocessor and solveEnergy
-// (src/WaveEnergy.cpp), solveRollers (src/WaveRollers.cpp) and storageRow when the row is a
-// mesh row. Thread t of workgroup k holds node (k*WNI + t - 2) mod ny: slots 0..nint+3
-// take the dispersion, 1..nint+2 the direction, 2..nint+1 (the owned nodes) the rest.
-// BM is the breaking model, FRIC the bottom friction switch.
-template <int BM, bool FRIC>
-__global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, WaveCtl *ctl, unsigned long long *acc)
+// Wave::getDx (src/Wave.cpp:267-290): the step length from the previous row's largest
+// |tan(theta)|; store = 1 when the step reaches the next mesh row.
+__device__ __forceinline__ void get_dx(const WaveArgs &a, const WaveCtl &c, double tanThetaMax, double &dx_row,
+                                       int &store)
 {
-    const int t = threadIdx.x, k = blockIdx.x;
-    const int ny = a.ny;
-    const int j0 = k * WNI;
-    const int nint = min(WNI, ny - j0);
-    int j = (j0 + t - 2) % ny;
-    if (j < 0) j += ny;
-    const int jp = (j == ny - 1) ? 0 : j + 1, jm = (j == 0) ? ny - 1 : j - 1;
-    const bool rollers = a.p.rollers != 0;
-    const PrevRow P = load_prev(a.rows[s & 1], ny, j, jp, jm, rollers, FRIC);
-    const WaveCtl c = ctl[s & 1];
-    const unsigned long long acc_prev = acc[s % 3];
-    if (c.done) {   // pass the final state on to the launches that follow
-        if (k == 0 && t == 0) ctl[(s + 1) & 1] = c;
-        return;
-    }
-    __shared__ double sl_lK[WB], sl_U[WB], sl_V[WB], sl_Cgy[WB], sl_Cy[WB];
-    __shared__ double s_sigma0;
-    const int nx = a.nx;
-    const m70w_params &p = a.p;
-    const double inv_2dy = a.inv_2dy;
-    const double w = 2 * WPI / T;
-    // getDx
-    const double tanThetaMax = tan_max_of(acc_prev);
     const double dx_remain = a.x[c.xIndex + 1] - c.x0;
-    double dx_row;
-    int store;
     if (tanThetaMax == 0) {
         dx_row = dx_remain;
         store = 1;
@@ -409,182 +377,370 @@ __global__ void __launch_bounds__(WB) k_wave_step(WaveArgs a, double T, int s, W
             store = 1;
         }
     }
-    if (k == 0 && t == 0) {   // the next launch's control state: its inputs are all known here
-        WaveCtl nc = c;
-        nc.x0 = c.x0 + dx_row;
-        if (store) nc.xIndex = c.xIndex + 1;
-        nc.steps = c.steps + 1;
-        nc.done = (nc.xIndex >= nx - 1);
-        if (nc.steps > 1000L * nx + 1000) {
-            nc.done = 1;
-            nc.err = 1;
-        }
-        ctl[(s + 1) & 1] = nc;
-        acc[(s + 2) % 3] = 0;   // the slot launch s+1 folds into
+}
+
+// The marching state after a step of length dx_row (src/Wave.cpp:150-170).
+__device__ __forceinline__ WaveCtl advance(const WaveCtl &c, double dx_row, int store, int nx)
+{
+    WaveCtl nc = c;
+    nc.x0 = c.x0 + dx_row;
+    if (store) nc.xIndex = c.xIndex + 1;
+    nc.steps = c.steps + 1;
+    nc.done = (nc.xIndex >= nx - 1);
+    if (nc.steps > 1000L * nx + 1000) {
+        nc.done = 1;
+        nc.err = 1;
     }
-    double *const r = a.rows[(s & 1) ^ 1];
-    const long index = (long)(c.xIndex + 1) * ny, index_m1 = (long)c.xIndex * ny;
-    const double dx_dx = dx_row / a.dx[c.xIndex];
-    WNode n{};
-    // initRow + dispersion
-    auto init_node = [&](int q, double &d, double &u, double &v, double &uw) {
-        if (store) {
-            d = a.D[index + q];
-            u = a.U[index + q];
-            v = a.V[index + q];
-            uw = a.U_w[index + q];
-        } else {
-            d = a.D[index_m1 + q] + (a.D[index + q] - a.D[index_m1 + q]) * dx_dx;
-            u = a.U[index_m1 + q] + (a.U[index + q] - a.U[index_m1 + q]) * dx_dx;
-            v = a.V[index_m1 + q] + (a.V[index + q] - a.V[index_m1 + q]) * dx_dx;
-            uw = a.U_w[index_m1 + q] + (a.U_w[index + q] - a.U_w[index_m1 + q]) * dx_dx;
+    return nc;
+}
+
+// Marching launch L: Wave::getDx, WaveRow::initRow (src/WaveRow.cpp:235-296), dispersion
+// (src/WaveParametrizations.cpp:14-60), solveDir (src/WaveDir.cpp:12-126),
+// energyPreprocessor and solveEnergy (src/WaveEnergy.cpp), solveRollers
+// (src/WaveRollers.cpp) and storageRow of mesh rows, for up to WKMAX rows.
+//
+// Every workgroup first replays the previous launch's rows 2..K under the step rule, from
+// the largest |tan(theta)| of each of its rows: the rows up to the first one the rule
+// would have made an intermediate row hold (they were computed as mesh rows), and the
+// march goes on from the last of them, v. Row 1 of this launch then takes the rule's own
+// step from row v, and rows 2..K are again taken to be mesh rows (K doubles after a launch
+// whose rows all held, up to WKMAX, and is 1 after one whose rows did not). Each row is
+// folded into its own accumulator slot as soon as its direction is known. A row's values
+// go to row buffer set L&1, slot r, so that any of them can be the next launch's previous
+// row; the fields of mesh rows go to the field arrays (a mesh row computed past the
+// point that holds is computed again later, and stored again).
+//
+// Lane t holds the same node through the launch; the previous row of row 1 is loaded from
+// row v, each later one is the lanes' own values of the row before, exchanged through
+// LDS. All lanes compute every row; values near the workgroup's edges go stale one lane
+// per row (their neighbours are missing), and the owned lanes stay exact through WKMAX
+// rows. Only owned lanes store or fold. BM is the breaking model, FRIC the bottom friction
+// switch.
+template <int BM, bool FRIC>
+__global__ void __launch_bounds__(WB) k_wave_block(WaveArgs a, double T, int L, WaveCtl *ctl, unsigned long long *acc)
+{
+    const int t = threadIdx.x, k = blockIdx.x;
+    const int ny = a.ny, nx = a.nx;
+    const int j0 = k * WNB;
+    const int nint = min(WNB, ny - j0);
+    const bool owned = t >= WHALO && t < WHALO + nint;
+    int j = (j0 + t - WHALO) % ny;
+    if (j < 0) j += ny;
+    const int tm = (t == 0) ? 0 : t - 1, tp = (t == WB - 1) ? WB - 1 : t + 1;
+    const bool rollers = a.p.rollers != 0;
+    const m70w_params &p = a.p;
+    const WaveCtl cp = ctl[(L - 1) & 1];
+    if (cp.done) {   // pass the final state on to the launches that follow
+        if (k == 0 && t == 0) {
+            WaveCtl nc = cp;
+            nc.K = 0;
+            ctl[L & 1] = nc;
         }
-    };
-    if (t < nint + 4) {
-        init_node(j, n.D, n.U, n.V, n.U_w);
-        if (store) {   // a mesh row: solved before the march, from the same inputs
-            n.lK = a.disp.lK[index + j];
-            n.Sigma = a.disp.Sigma[index + j];
-            n.C = a.disp.C[index + j];
-            n.Cg = a.disp.Cg[index + j];
-            n.Urms_H = a.disp.Urms_H[index + j];
+        return;
+    }
+    // the previous launch's rows 2..K under the step rule
+    const unsigned long long *accp = acc + ((L - 1) % 3) * WACC_SLOTS;
+    WaveCtl c = cp;
+    int v = 1;
+    while (v < cp.K && !c.done) {
+        double dxr;
+        int st;
+        get_dx(a, c, tan_max_of(accp[v]), dxr, st);
+        if (!st) break;   // row v+1 should have been an intermediate row
+        c = advance(c, dxr, 1, nx);
+        ++v;
+    }
+    if (c.done) {   // the previous launch's rows reached the last mesh row
+        if (k == 0 && t == 0) {
+            WaveCtl nc = c;
+            nc.K = 0;
+            ctl[L & 1] = nc;
+        }
+        return;
+    }
+    const bool all_held = (v == cp.K);
+    // this launch's row 1
+    double dx_row;
+    int store;
+    get_dx(a, c, tan_max_of(accp[v]), dx_row, store);
+    const WaveCtl c1 = advance(c, dx_row, store, nx);
+    int K = all_held ? min(2 * cp.K, WKMAX) : 1;
+    K = c1.done ? 1 : min(K, 1 + (nx - 1 - c1.xIndex));
+    if (k == 0 && t == 0) {
+        WaveCtl nc = c1;
+        nc.K = K;
+        ctl[L & 1] = nc;
+        unsigned long long *accn = acc + ((L + 1) % 3) * WACC_SLOTS;   // the set launch L+1 folds into
+        for (int r = 0; r < WACC_SLOTS; ++r) accn[r] = 0;
+    }
+    unsigned long long *const accl = acc + (L % 3) * WACC_SLOTS;
+    const long rstride = (long)WROW_ARRAYS * ny;
+    double *const rows_out = a.rows[L & 1];
+
+    __shared__ double sp[PL_N][WB];
+    __shared__ double sl_lK[WB], sl_U[WB], sl_V[WB], sl_Cgy[WB], sl_Cy[WB];
+    __shared__ double s_sigma0;
+    const double inv_2dy = a.inv_2dy;
+    const double w = 2 * WPI / T;
+
+    // the previous row of row 1: row v of the previous launch
+    PrevOwn own;
+    {
+        const double *rb = a.rows[(L - 1) & 1] + (long)(v - 1) * rstride;
+        auto r = [&](int q) { return rb + (long)q * ny; };
+        own.lk = r(WR_lK)[j];
+        own.u = r(WR_U)[j];
+        own.v = r(WR_V)[j];
+        own.le = r(WR_lE)[j];
+        own.cgx = r(WR_Cgx)[j];
+        own.h = r(WR_H)[j];
+        sp[PL_TT][t] = r(WR_TanTheta)[j];
+        sp[PL_LK][t] = own.lk;
+        sp[PL_CG][t] = r(WR_Cg)[j];
+        sp[PL_U][t] = own.u;
+        sp[PL_V][t] = own.v;
+        sp[PL_DW][t] = r(WR_dw)[j];
+        sp[PL_DF][t] = FRIC ? r(WR_df)[j] : 0.0;   // df is zero on every row without friction
+        sp[PL_LE][t] = own.le;
+        sp[PL_WC][t] = r(WR_WC)[j];
+        sp[PL_CGY][t] = r(WR_Cgy)[j];
+        if (rollers) {
+            own.cx = r(WR_Cx)[j];
+            own.er = r(WR_E_r)[j];
+            sp[PL_C][t] = r(WR_C)[j];
+            sp[PL_WCR][t] = r(WR_WC_r)[j];
+            sp[PL_CY][t] = r(WR_Cy)[j];
+            sp[PL_ER][t] = own.er;
+            sp[PL_DR][t] = r(WR_d_r)[j];
+            sp[PL_DWE][t] = r(WR_Dw)[j];
         } else {
+            own.cx = own.er = 0.0;
+        }
+    }
+    MeshNode next_m{};
+    if (store) next_m = load_mesh(a, (long)(c.xIndex + 1) * ny + j);
+    __syncthreads();
+
+    for (int row = 1; row <= K; ++row) {
+        if (row > 1) {   // taken to be a mesh row
+            dx_row = a.x[c.xIndex + 1] - c.x0;
+            store = 1;
+        }
+        const WaveCtl cn = advance(c, dx_row, store, nx);
+        const long index = (long)(c.xIndex + 1) * ny, index_m1 = (long)c.xIndex * ny;
+        const MeshNode m = next_m;
+        if (row < K) next_m = load_mesh(a, (long)(cn.xIndex + 1) * ny + j);   // the next row's, early
+        WNode n{};
+        // initRow + dispersion
+        if (store) {   // a mesh row: solved before the march, from the same inputs
+            n.D = m.D;
+            n.U = m.U;
+            n.V = m.V;
+            n.U_w = m.U_w;
+            n.lK = m.lK;
+            n.Sigma = m.Sigma;
+            n.C = m.C;
+            n.Cg = m.Cg;
+            n.Urms_H = m.Urms_H;
+        } else {
+            const double dx_dx = dx_row / a.dx[c.xIndex];
+            auto interp = [&](const double *f, long q) { return f[index_m1 + q] + (f[index + q] - f[index_m1 + q]) * dx_dx; };
+            n.D = interp(a.D, j);
+            n.U = interp(a.U, j);
+            n.V = interp(a.V, j);
+            n.U_w = interp(a.U_w, j);
             w_dispersion(n.D, n.U_w, w, p.z0, n.lK, n.Sigma, n.C, n.Cg, n.Urms_H);
         }
         sl_lK[t] = n.lK;
         sl_U[t] = n.U;
         sl_V[t] = n.V;
-    }
-    if (FRIC && t == 0 && p.friction_model == M70W_FRIC_SOULSBY97) {
-        double d0, u0, v0, uw0, lK0, sg0, C0, Cg0, Uh0;   // Sigma of the row's first node
-        if (store) {
-            sg0 = a.disp.Sigma[index];
-        } else {
-            init_node(0, d0, u0, v0, uw0);
-            w_dispersion(d0, uw0, w, p.z0, lK0, sg0, C0, Cg0, Uh0);
-        }
-        s_sigma0 = sg0;
-    }
-    __syncthreads();
-    // solveDir
-    const double dx = dx_row;
-    const double dx2 = dx * dx;
-    const double inv_dx = 1 / dx;
-    if (t >= 1 && t < nint + 3) {
-        double tt_p1 = P.tt_p;
-        double tt_m1 = P.tt_m;
-        double dttheta_dy = (tt_p1 - tt_m1) * inv_2dy;
-        double ttheta_med = (tt_p1 + tt_m1) * 0.5;
-        double y_i = -ttheta_med / (dttheta_dy + inv_dx);
-        double s_i = sqrt(dx2 + y_i * y_i);
-        double inv_s_i = 1 / s_i;
-        double sinTheta_i = -y_i * inv_s_i;
-        double cosTheta_i = dx * inv_s_i;
-        double theta_i = asin(sinTheta_i);
-        n.Y_i = y_i;
-        n.inv_s_i = inv_s_i;
-        double dlk_dx = (n.lK - P.lk_j) * inv_dx;
-        double dlk_dy_c = 0.5 * inv_2dy * (sl_lK[t + 1] + P.lk_p - sl_lK[t - 1] - P.lk_m);
-        double dlk_ds = dlk_dx * cosTheta_i + dlk_dy_c * sinTheta_i;
-        double dlk_dst = -dlk_dx * sinTheta_i + dlk_dy_c * cosTheta_i;
-        double theta = theta_i + dlk_dst / (inv_s_i + dlk_ds * 0.5);
-        theta = smin_(theta, p.theta_max);
-        theta = smax_(theta, -p.theta_max);
-        const double sinTheta = sin(theta);
-        const double cosTheta = cos(theta);
-        node_dir_terms(n, theta, cosTheta, sinTheta, cosTheta * sinTheta);
-        sl_Cgy[t] = n.Cgy;
-        sl_Cy[t] = n.Cy;
-    }
-    __syncthreads();
-    {   // the new row's largest |tan(theta)| (owned nodes), folded while the energy runs
-        double v = -1;
-        if (t >= 2 && t < nint + 2) {
-            const double tt = fabs(n.TanTheta);
-            if (tt >= 0) v = tt;
-        }
-        fold_tan_max(v, &acc[(s + 1) % 3]);
-    }
-    if (t >= 2 && t < nint + 2) {
-        const double y_i_dy = n.Y_i * inv_2dy;
-        const double inv_s_i = n.inv_s_i;
-        const double U_up = sl_U[t + 1], U_low = sl_U[t - 1], V_up = sl_V[t + 1], V_low = sl_V[t - 1];
-        double G, Gp;
-        {   // energyPreprocessor (src/WaveEnergy.cpp:181-219)
-            double cg_c = 0.5 * (n.Cg + at_ray(P.cg_p, P.cg_m, y_i_dy));
-            double u_c = 0.5 * (n.U + at_ray(P.u_p, P.u_m, y_i_dy));
-            double v_c = 0.5 * (n.V + at_ray(P.v_p, P.v_m, y_i_dy));
-            double dw_i = at_ray(P.dw_p, P.dw_m, y_i_dy);
-            double df_i = at_ray(P.df_p, P.df_m, y_i_dy);
-            double le_i = at_ray(P.le_p, P.le_m, y_i_dy);
-            double du_dx = (n.U - P.u_j) * inv_dx;
-            double du_dy = (U_up - U_low) * inv_2dy;
-            double dv_dx = (n.V - P.v_j) * inv_dx;
-            double dv_dy = (V_up - V_low) * inv_2dy;
-            double wc = 2 * n.sxx * du_dx + n.sxy * (du_dy + dv_dx) + 2 * n.syy * dv_dy;
-            double wc_c = 0.5 * (wc + at_ray(P.wc_p, P.wc_m, y_i_dy));
-            n.WC = wc;
-            n.WC_r = n.sxx_r * du_dx + n.sxy_r * (du_dy + dv_dx) + n.syy_r * dv_dy;
-            double dcgxu_dx = (n.Cgx - P.cgx_j) * inv_dx + du_dx;
-            double dcgyv_dy_c = 0.5 * (sl_Cgy[t + 1] + P.cgy_p - sl_Cgy[t - 1] - P.cgy_m) * inv_2dy + dv_dy;
-            double div_cgv_c = dcgxu_dx + dcgyv_dy_c;
-            double dle_dy_c = (P.le_p - P.le_m) * inv_2dy;
-            G = v_c * dle_dy_c + wc_c + div_cgv_c + 0.5 * (dw_i + df_i) - cg_c * inv_s_i * le_i -
-                u_c * P.le_j * inv_dx;
-            Gp = cg_c * inv_s_i + u_c * inv_dx;
-        }
-        {   // solveEnergy: Newton-Raphson on log(E); the Jacobian's doubled Gp is the
-            // reference's (src/WaveEnergy.cpp:97)
-            const double logCteEnergy = log(CTE_ENERGY);
-            double le_old = P.le_j, H = P.h_j, E = 0.0, Dw = 0.0;
-            double dw = 0.0, ddw_dh = 0.0, df = 0.0, ddf_dh = 0.0;
-            for (int it = 0; it < p.max_iterations; it++) {
-                w_breaking<BM>(p, H, n.Sigma, n.D, dw, ddw_dh);
-                if (FRIC) w_friction(p, H, s_sigma0, n.Urms_H, df, ddf_dh);
-                double gg = G;
-                double gp = Gp;
-                double le = le_old;
-                gg += gp * le + 0.5 * (dw + df);
-                gp += gp + 0.25 * (ddf_dh + ddw_dh) * H;
-                le -= gg / gp;
-                double h = exp((le - logCteEnergy) * 0.5);
-                H = h;
-                E = CTE_ENERGY * h * h;
-                const bool more = fabs(le - le_old) > p.tolerance;
-                le_old = le;
-                Dw = dw * E;
-                if (!more) break;
+        if (FRIC && t == 0 && p.friction_model == M70W_FRIC_SOULSBY97) {
+            double sg0;   // Sigma of the row's first node
+            if (store) {
+                sg0 = a.disp.Sigma[index];
+            } else {
+                const double dx_dx = dx_row / a.dx[c.xIndex];
+                auto interp = [&](const double *f) { return f[index_m1] + (f[index] - f[index_m1]) * dx_dx; };
+                double lK0, C0, Cg0, Uh0;
+                w_dispersion(interp(a.D), interp(a.U_w), w, p.z0, lK0, sg0, C0, Cg0, Uh0);
             }
-            n.lE = le_old;
-            n.E = E;
-            n.H = H;
-            n.dw = dw;
-            n.df = df;
-            n.Dw = Dw;
+            s_sigma0 = sg0;
         }
-        if (p.rollers) {   // solveRollers (src/WaveRollers.cpp:57-100)
-            const double d_r_div_c = 2 * WG * p.sinBeta_r;
-            double c_c = 0.5 * (n.C + at_ray(P.c_p, P.c_m, y_i_dy));
-            double u_c = 0.5 * (n.U + at_ray(P.u_p, P.u_m, y_i_dy));
-            double v_c = 0.5 * (n.V + at_ray(P.v_p, P.v_m, y_i_dy));
-            double du_dx = (n.U - P.u_j) * inv_dx;
-            double dv_dy = (V_up - V_low) * inv_2dy;
-            double wc_r_c = 0.5 * (n.WC_r + at_ray(P.wcr_p, P.wcr_m, y_i_dy));
-            double dcxu_dx = (n.Cx - P.cx_j) * inv_dx + du_dx;
-            double dcyv_dy_c = 0.5 * (sl_Cy[t + 1] + P.cy_p - sl_Cy[t - 1] - P.cy_m) * inv_2dy + dv_dy;
-            double div_cv_c = dcxu_dx + dcyv_dy_c;
-            double der_dy_c = (P.er_p - P.er_m) * inv_2dy;
-            const double d_r = d_r_div_c / n.C;
-            n.d_r = d_r;
-            double d_r_c = 0.5 * (d_r + at_ray(P.dr_p, P.dr_m, y_i_dy));
-            double Dw_c = 0.5 * (n.Dw + at_ray(P.Dw_p, P.Dw_m, y_i_dy));
-            double er_i = at_ray(P.er_p, P.er_m, y_i_dy);
-            double upv = c_c * inv_s_i * er_i + u_c * P.er_j * inv_dx - v_c * der_dy_c + 0.5 * p.alpha_r * Dw_c;
-            double lowv = c_c * inv_s_i + u_c * inv_dx + div_cv_c + 0.5 * wc_r_c + 0.5 * d_r_c;
-            n.E_r = upv / lowv;
+        __syncthreads();
+        PrevRow P;
+        P.tt_p = sp[PL_TT][tp];
+        P.tt_m = sp[PL_TT][tm];
+        P.lk_j = own.lk;
+        P.lk_p = sp[PL_LK][tp];
+        P.lk_m = sp[PL_LK][tm];
+        // solveDir
+        const double dx = dx_row;
+        const double dx2 = dx * dx;
+        const double inv_dx = 1 / dx;
+        {
+            double tt_p1 = P.tt_p;
+            double tt_m1 = P.tt_m;
+            double dttheta_dy = (tt_p1 - tt_m1) * inv_2dy;
+            double ttheta_med = (tt_p1 + tt_m1) * 0.5;
+            double y_i = -ttheta_med / (dttheta_dy + inv_dx);
+            double s_i = sqrt(dx2 + y_i * y_i);
+            double inv_s_i = 1 / s_i;
+            double sinTheta_i = -y_i * inv_s_i;
+            double cosTheta_i = dx * inv_s_i;
+            double theta_i = asin(sinTheta_i);
+            n.Y_i = y_i;
+            n.inv_s_i = inv_s_i;
+            double dlk_dx = (n.lK - P.lk_j) * inv_dx;
+            double dlk_dy_c = 0.5 * inv_2dy * (sl_lK[tp] + P.lk_p - sl_lK[tm] - P.lk_m);
+            double dlk_ds = dlk_dx * cosTheta_i + dlk_dy_c * sinTheta_i;
+            double dlk_dst = -dlk_dx * sinTheta_i + dlk_dy_c * cosTheta_i;
+            double theta = theta_i + dlk_dst / (inv_s_i + dlk_ds * 0.5);
+            theta = smin_(theta, p.theta_max);
+            theta = smax_(theta, -p.theta_max);
+            const double sinTheta = sin(theta);
+            const double cosTheta = cos(theta);
+            node_dir_terms(n, theta, cosTheta, sinTheta, cosTheta * sinTheta);
+            sl_Cgy[t] = n.Cgy;
+            sl_Cy[t] = n.Cy;
         }
-        write_row(r, ny, j, n, rollers, FRIC);
-        if (store) store_node(a.f, index + j, n, rollers);
+        __syncthreads();
+        {   // the row's largest |tan(theta)| over the owned nodes, folded while the energy runs
+            double mv = -1;
+            if (owned) {
+                const double tt = fabs(n.TanTheta);
+                if (tt >= 0) mv = tt;
+            }
+            fold_tan_max(mv, &accl[row]);
+        }
+        P.cg_p = sp[PL_CG][tp]; P.cg_m = sp[PL_CG][tm];
+        P.u_p = sp[PL_U][tp]; P.u_m = sp[PL_U][tm]; P.u_j = own.u;
+        P.v_p = sp[PL_V][tp]; P.v_m = sp[PL_V][tm]; P.v_j = own.v;
+        P.dw_p = sp[PL_DW][tp]; P.dw_m = sp[PL_DW][tm];
+        P.df_p = sp[PL_DF][tp]; P.df_m = sp[PL_DF][tm];
+        P.le_p = sp[PL_LE][tp]; P.le_m = sp[PL_LE][tm]; P.le_j = own.le;
+        P.wc_p = sp[PL_WC][tp]; P.wc_m = sp[PL_WC][tm];
+        P.cgx_j = own.cgx; P.cgy_p = sp[PL_CGY][tp]; P.cgy_m = sp[PL_CGY][tm];
+        P.h_j = own.h;
+        if (rollers) {
+            P.c_p = sp[PL_C][tp]; P.c_m = sp[PL_C][tm];
+            P.wcr_p = sp[PL_WCR][tp]; P.wcr_m = sp[PL_WCR][tm];
+            P.cx_j = own.cx; P.cy_p = sp[PL_CY][tp]; P.cy_m = sp[PL_CY][tm];
+            P.er_p = sp[PL_ER][tp]; P.er_m = sp[PL_ER][tm]; P.er_j = own.er;
+            P.dr_p = sp[PL_DR][tp]; P.dr_m = sp[PL_DR][tm];
+            P.Dw_p = sp[PL_DWE][tp]; P.Dw_m = sp[PL_DWE][tm];
+        }
+        {
+            const double y_i_dy = n.Y_i * inv_2dy;
+            const double inv_s_i = n.inv_s_i;
+            const double U_up = sl_U[tp], U_low = sl_U[tm], V_up = sl_V[tp], V_low = sl_V[tm];
+            double G, Gp;
+            {   // energyPreprocessor (src/WaveEnergy.cpp:181-219)
+                double cg_c = 0.5 * (n.Cg + at_ray(P.cg_p, P.cg_m, y_i_dy));
+                double u_c = 0.5 * (n.U + at_ray(P.u_p, P.u_m, y_i_dy));
+                double v_c = 0.5 * (n.V + at_ray(P.v_p, P.v_m, y_i_dy));
+                double dw_i = at_ray(P.dw_p, P.dw_m, y_i_dy);
+                double df_i = at_ray(P.df_p, P.df_m, y_i_dy);
+                double le_i = at_ray(P.le_p, P.le_m, y_i_dy);
+                double du_dx = (n.U - P.u_j) * inv_dx;
+                double du_dy = (U_up - U_low) * inv_2dy;
+                double dv_dx = (n.V - P.v_j) * inv_dx;
+                double dv_dy = (V_up - V_low) * inv_2dy;
+                double wc = 2 * n.sxx * du_dx + n.sxy * (du_dy + dv_dx) + 2 * n.syy * dv_dy;
+                double wc_c = 0.5 * (wc + at_ray(P.wc_p, P.wc_m, y_i_dy));
+                n.WC = wc;
+                n.WC_r = n.sxx_r * du_dx + n.sxy_r * (du_dy + dv_dx) + n.syy_r * dv_dy;
+                double dcgxu_dx = (n.Cgx - P.cgx_j) * inv_dx + du_dx;
+                double dcgyv_dy_c = 0.5 * (sl_Cgy[tp] + P.cgy_p - sl_Cgy[tm] - P.cgy_m) * inv_2dy + dv_dy;
+                double div_cgv_c = dcgxu_dx + dcgyv_dy_c;
+                double dle_dy_c = (P.le_p - P.le_m) * inv_2dy;
+                G = v_c * dle_dy_c + wc_c + div_cgv_c + 0.5 * (dw_i + df_i) - cg_c * inv_s_i * le_i -
+                    u_c * P.le_j * inv_dx;
+                Gp = cg_c * inv_s_i + u_c * inv_dx;
+            }
+            {   // solveEnergy: Newton-Raphson on log(E); the Jacobian's doubled Gp is the
+                // reference's (src/WaveEnergy.cpp:97)
+                const double logCteEnergy = log(CTE_ENERGY);
+                double le_old = P.le_j, H = P.h_j, E = 0.0, Dw = 0.0;
+                double dw = 0.0, ddw_dh = 0.0, df = 0.0, ddf_dh = 0.0;
+                for (int it = 0; it < p.max_iterations; it++) {
+                    w_breaking<BM>(p, H, n.Sigma, n.D, dw, ddw_dh);
+                    if (FRIC) w_friction(p, H, s_sigma0, n.Urms_H, df, ddf_dh);
+                    double gg = G;
+                    double gp = Gp;
+                    double le = le_old;
+                    gg += gp * le + 0.5 * (dw + df);
+                    gp += gp + 0.25 * (ddf_dh + ddw_dh) * H;
+                    le -= gg / gp;
+                    double h = exp((le - logCteEnergy) * 0.5);
+                    H = h;
+                    E = CTE_ENERGY * h * h;
+                    const bool more = fabs(le - le_old) > p.tolerance;
+                    le_old = le;
+                    Dw = dw * E;
+                    if (!more) break;
+                }
+                n.lE = le_old;
+                n.E = E;
+                n.H = H;
+                n.dw = dw;
+                n.df = df;
+                n.Dw = Dw;
+            }
+            if (rollers) {   // solveRollers (src/WaveRollers.cpp:57-100)
+                const double d_r_div_c = 2 * WG * p.sinBeta_r;
+                double c_c = 0.5 * (n.C + at_ray(P.c_p, P.c_m, y_i_dy));
+                double u_c = 0.5 * (n.U + at_ray(P.u_p, P.u_m, y_i_dy));
+                double v_c = 0.5 * (n.V + at_ray(P.v_p, P.v_m, y_i_dy));
+                double du_dx = (n.U - P.u_j) * inv_dx;
+                double dv_dy = (V_up - V_low) * inv_2dy;
+                double wc_r_c = 0.5 * (n.WC_r + at_ray(P.wcr_p, P.wcr_m, y_i_dy));
+                double dcxu_dx = (n.Cx - P.cx_j) * inv_dx + du_dx;
+                double dcyv_dy_c = 0.5 * (sl_Cy[tp] + P.cy_p - sl_Cy[tm] - P.cy_m) * inv_2dy + dv_dy;
+                double div_cv_c = dcxu_dx + dcyv_dy_c;
+                double der_dy_c = (P.er_p - P.er_m) * inv_2dy;
+                const double d_r = d_r_div_c / n.C;
+                n.d_r = d_r;
+                double d_r_c = 0.5 * (d_r + at_ray(P.dr_p, P.dr_m, y_i_dy));
+                double Dw_c = 0.5 * (n.Dw + at_ray(P.Dw_p, P.Dw_m, y_i_dy));
+                double er_i = at_ray(P.er_p, P.er_m, y_i_dy);
+                double upv = c_c * inv_s_i * er_i + u_c * P.er_j * inv_dx - v_c * der_dy_c + 0.5 * p.alpha_r * Dw_c;
+                double lowv = c_c * inv_s_i + u_c * inv_dx + div_cv_c + 0.5 * wc_r_c + 0.5 * d_r_c;
+                n.E_r = upv / lowv;
+            }
+        }
+        if (owned) {
+            write_row(rows_out + (long)(row - 1) * rstride, ny, j, n, rollers, FRIC);
+            if (store) store_node(a.f, index + j, n, rollers);
+        }
+        __syncthreads();   // every lane has read the previous row
+        own.lk = n.lK;
+        own.u = n.U;
+        own.v = n.V;
+        own.le = n.lE;
+        own.cgx = n.Cgx;
+        own.h = n.H;
+        sp[PL_TT][t] = n.TanTheta;
+        sp[PL_LK][t] = n.lK;
+        sp[PL_CG][t] = n.Cg;
+        sp[PL_U][t] = n.U;
+        sp[PL_V][t] = n.V;
+        sp[PL_DW][t] = n.dw;
+        sp[PL_DF][t] = n.df;
+        sp[PL_LE][t] = n.lE;
+        sp[PL_WC][t] = n.WC;
+        sp[PL_CGY][t] = n.Cgy;
+        if (rollers) {
+            own.cx = n.Cx;
+            own.er = n.E_r;
+            sp[PL_C][t] = n.C;
+            sp[PL_WCR][t] = n.WC_r;
+            sp[PL_CY][t] = n.Cy;
+            sp[PL_ER][t] = n.E_r;
+            sp[PL_DR][t] = n.d_r;
+            sp[PL_DWE][t] = n.Dw;
+        }
+        __syncthreads();
+        c = cn;
     }
 }
 
@@ -650,6 +806,17 @@ __global__ void __launch_bounds__(256) k_wave_set_uv(double *U, double *V, doubl
 }
 
 int wave_blocks(int ny) { return (ny + WNI - 1) / WNI; }
+int wave_blocks_march(int ny) { return (ny + WNB - 1) / WNB; }
+
+int wave_march_launches(int nx)
+{
+    int n = 0, K = 1;
+    for (long rows = 0; rows < nx - 1; ++n) {
+        K = std::min(2 * K, WKMAX);
+        rows += K;
+    }
+    return n + 1;
+}
 
 hipError_t launch_wave_first(hipStream_t st, const WaveArgs &a, double hOff, double thetaOff, double T,
                              const WaveCtlBufs &b)
@@ -672,9 +839,9 @@ template <bool FRIC>
 WaveStepKernel wave_step_kernel(int breaking_model)
 {
     switch (breaking_model) {
-    case M70W_BREAK_CT: return k_wave_step<M70W_BREAK_CT, FRIC>;
-    case M70W_BREAK_TG: return k_wave_step<M70W_BREAK_TG, FRIC>;
-    default: return k_wave_step<M70W_BREAK_REGULAR, FRIC>;
+    case M70W_BREAK_CT: return k_wave_block<M70W_BREAK_CT, FRIC>;
+    case M70W_BREAK_TG: return k_wave_block<M70W_BREAK_TG, FRIC>;
+    default: return k_wave_block<M70W_BREAK_REGULAR, FRIC>;
     }
 }
 
@@ -682,8 +849,8 @@ hipError_t launch_wave_steps(hipStream_t st, const WaveArgs &a, double T, int s0
 {
     const WaveStepKernel kern =
         a.p.bottom_friction ? wave_step_kernel<true>(a.p.breaking_model) : wave_step_kernel<false>(a.p.breaking_model);
-    for (int s = s0; s < s0 + count; ++s)
-        hipLaunchKernelGGL(kern, dim3((unsigned)wave_blocks(a.ny)), dim3(WB), 0, st, a, T, s, b.ctl, b.acc);
+    for (int L = s0; L < s0 + count; ++L)
+        hipLaunchKernelGGL(kern, dim3((unsigned)wave_blocks_march(a.ny)), dim3(WB), 0, st, a, T, L, b.ctl, b.acc);
     return hipGetLastError();
 }
 
