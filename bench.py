@@ -343,9 +343,9 @@ def bench_wave(args):
         "roofline": {"bound": "hbm", "kernel": "wave_field", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": ms,
                      "alg_bytes_per_launch": alg,
-                     "note": "a serial march over the rows, one launch per marching step over ~70 workgroups; "
-                             "a step is bound by the latency of each node's Newton-Raphson chains "
-                             "(dispersion, energy), not by HBM"},
+                     "note": "a serial march over the rows, up to 12 rows per launch over one-wavefront "
+                             "workgroups of 38 owned nodes; a row is bound by the FP64 instruction "
+                             "stream of each node's Newton-Raphson chains (dispersion, energy), not by HBM"},
         "finite": finite,
     }
     if not args.no_cpu:

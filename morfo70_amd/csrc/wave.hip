@@ -3,22 +3,22 @@
 // src/WaveRollers.cpp) for gfx950.
 //
 // The field marches offshore to onshore, one row after the other: a row needs the whole
-// previous row (the step length takes the largest |tan(theta)| over it). Each marching
-// step is one launch over the row, split across workgroups of WB threads that own WNI
-// consecutive nodes each. Within a row the direction needs the neighbours' wave numbers
-// and the energy balance the neighbours' directions and currents, so a workgroup also
-// computes a halo of two nodes each side for the dispersion and one for the direction
-// (the same arithmetic as the owner's, so the values agree bit for bit) and needs no
-// other workgroup within the step. The largest |tan(theta)| of the new row is reduced
-// across the wavefront by lane shuffles as soon as the direction is known, and folded by
-// one atomic maximum (non-negative doubles order as their bit patterns) into a three-slot
-// accumulator while the energy balance runs; the step's control state (x0, mesh row,
-// done) is double-buffered in device memory, so that launch s reads what launch s-1
-// wrote, and is written at the start of the step, which has all its inputs. A mesh row
-// stores the node values only the march computes (Theta, H, E, Dw, E_r); the arrays
-// Wave::storageRow derives from them and from the mesh node's dispersion are formed for
-// the whole field by one launch after the march (k_wave_field_terms), with the same
-// arithmetic. Each marching kernel is instantiated per breaking model and friction
+// previous row (the step length takes the largest |tan(theta)| over it). A marching launch
+// computes up to WKMAX rows over workgroups of one wavefront each: lane t holds the same
+// node for every row, and a node needs the previous row at its neighbours and the
+// neighbours' new directions, so the values near a workgroup's edges go stale one lane per
+// row and WHALO halo lanes each side keep the owned lanes exact (the same arithmetic as the
+// neighbouring workgroup's owner, so the values agree bit for bit). A launch's first row
+// follows the reference's step rule; its later rows are taken to be mesh rows, and the next
+// launch checks that against the rule from each row's largest |tan(theta)| (reduced across
+// the wavefront by lane shuffles as soon as the direction is known and folded by one atomic
+// maximum on the bit patterns of non-negative doubles into the row's accumulator slot),
+// keeping the rows that hold. The control state (x0, mesh row, step count, done, rows
+// per launch) is double-buffered in device memory, so that launch L reads what launch L-1
+// wrote. A mesh row stores the node values only the march computes (Theta, H, E, Dw, E_r);
+// the arrays Wave::storageRow derives from them and from the mesh node's dispersion are
+// formed for the whole field by one launch after the march (k_wave_field_terms), with the
+// same arithmetic. Each marching kernel is instantiated per breaking model and friction
 // switch, so the energy iteration is one branch-free block. Control flow (mesh rows,
 // intermediate rows, the step count) follows src/Wave.cpp:80-172, with the two row
 // buffers swapped after every step (the reference's brace-less `else` at :158-163 swaps
