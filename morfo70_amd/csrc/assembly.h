@@ -142,20 +142,21 @@ __host__ __device__ __forceinline__ void asm_qxx(const Consts &k, const Metrics 
 
 // Loads of two adjacent doubles (an even column of a row whose length is even: 16-byte
 // aligned) and of two adjacent masks.
+template <bool NT = false>
 __host__ __device__ __forceinline__ void ld_pair(const double *p, long e, double &a, double &b)
 {
-#if defined(__HIP_DEVICE_COMPILE__)
-    const double2 v = *reinterpret_cast<const double2 *>(p + e);
-    a = v.x;
-    b = v.y;
-#else
-    a = p[e];
-    b = p[e + 1];
-#endif
+    ldn2<NT>(p + e, a, b);
 }
+template <bool NT = false>
 __host__ __device__ __forceinline__ void ld_mask_pair(const uint8_t *p, long e, int &a, int &b)
 {
-    const uint16_t v = *reinterpret_cast<const uint16_t *>(p + e);
+    uint16_t v;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (NT) v = __builtin_nontemporal_load(reinterpret_cast<const uint16_t *>(p + e));
+    else v = *reinterpret_cast<const uint16_t *>(p + e);
+#else
+    v = *reinterpret_cast<const uint16_t *>(p + e);
+#endif
     a = v & 0xff;
     b = v >> 8;
 }
@@ -172,22 +173,25 @@ __host__ __device__ __forceinline__ void cols_segment_of(const ColsWork &cw, int
     q = kk - k0;
     m = k1 - k0;
 }
+template <bool STR = false>
 __host__ __device__ __forceinline__ void cols_value2(const ColsWork &cw, int n, int ny, int kk, int j, double &x0, double &x1)
 {
+    constexpr bool NTL = Nt<NT_QXY_ROWS, STR>::L;
+    constexpr bool NTF = NTL || Nt<NT_QXY_STREAM, STR>::L;   // single-use fields
     int sg, q, m;
     cols_segment_of(cw, n, kk, sg, q, m);
     if (q == 0) {
-        ld_pair(cw.X, (long)(2 * sg) * ny + j, x0, x1);
+        ld_pair<NTL>(cw.X, (long)(2 * sg) * ny + j, x0, x1);
     } else if (q == m - 1) {
-        ld_pair(cw.X, (long)(2 * sg + 1) * ny + j, x0, x1);
+        ld_pair<NTL>(cw.X, (long)(2 * sg + 1) * ny + j, x0, x1);
     } else {
         double F0, F1, L0, L1, e00, e01, e10, e11, e20, e21;
-        ld_pair(cw.X, (long)(2 * sg) * ny + j, F0, F1);
-        ld_pair(cw.X, (long)(2 * sg + 1) * ny + j, L0, L1);
+        ld_pair<NTL>(cw.X, (long)(2 * sg) * ny + j, F0, F1);
+        ld_pair<NTL>(cw.X, (long)(2 * sg + 1) * ny + j, L0, L1);
         const long o = (long)kk * ny + j;
-        ld_pair(cw.E0, o, e00, e01);
-        ld_pair(cw.E1, o, e10, e11);
-        ld_pair(cw.E2, o, e20, e21);
+        ld_pair<NTF>(cw.E0, o, e00, e01);
+        ld_pair<NTF>(cw.E1, o, e10, e11);
+        ld_pair<NTF>(cw.E2, o, e20, e21);
         x0 = e20 - e00 * F0 - e10 * L0;
         x1 = e21 - e01 * F1 - e11 * L1;
     }
@@ -234,37 +238,40 @@ __host__ __device__ __forceinline__ QxyIn qxy_load(const Consts &k, const State 
 }
 
 // Elements j = e and e + 1 of row i (e even, ny even), from loads of column pairs.
+template <bool STR = false>
 __host__ __device__ __forceinline__ void qxy_load2(const Consts &k, const State &s, const RunFields &rf, const Work &w,
                                                    int i, int e, QxyIn &a, QxyIn &b)
 {
+    constexpr bool NTL = Nt<NT_QXY_ROWS, STR>::L;
+    constexpr bool NTF = NTL || Nt<NT_QXY_STREAM, STR>::L;   // single-use fields
     const int ny = k.ny;
     const long r = (long)i * ny;
     const int em = (e == 0) ? ny - 2 : e - 2, ep = (e + 2 == ny) ? 0 : e + 2;
     int wmm, wm, w0, w1, wp, wpp;
-    ld_mask_pair(w.w, r + em, wmm, wm);
-    ld_mask_pair(w.w, r + e, w0, w1);
-    ld_mask_pair(w.w, r + ep, wp, wpp);
+    ld_mask_pair<NTL>(w.w, r + em, wmm, wm);
+    ld_mask_pair<NTL>(w.w, r + e, w0, w1);
+    ld_mask_pair<NTL>(w.w, r + ep, wp, wpp);
     a.wj = w0; a.wp = w1; a.wm = wm;
     b.wj = w1; b.wp = wp; b.wm = w0;
-    ld_pair(w.drag, r + e, a.drag, b.drag);
-    ld_pair(rf.nuD_x, r + e, a.nud_x, b.nud_x);
-    ld_pair(rf.dnuDx_dy, r + e, a.dnudx_dy, b.dnudx_dy);
+    ld_pair<NTF>(w.drag, r + e, a.drag, b.drag);
+    ld_pair<NTF>(rf.nuD_x, r + e, a.nud_x, b.nud_x);
+    ld_pair<NTF>(rf.dnuDx_dy, r + e, a.dnudx_dy, b.dnudx_dy);
     double d0m, d0, d0n, d0p, d1m, d1, d1n, d1p, x;
-    ld_pair(s.D, r + em, x, d0m);
-    ld_pair(s.D, r + e, d0, d0n);
-    ld_pair(s.D, r + ep, d0p, x);
-    ld_pair(s.D, r + ny + em, x, d1m);
-    ld_pair(s.D, r + ny + e, d1, d1n);
-    ld_pair(s.D, r + ny + ep, d1p, x);
+    ld_pair<NTL>(s.D, r + em, x, d0m);
+    ld_pair<NTL>(s.D, r + e, d0, d0n);
+    ld_pair<NTL>(s.D, r + ep, d0p, x);
+    ld_pair<NTL>(s.D, r + ny + em, x, d1m);
+    ld_pair<NTL>(s.D, r + ny + e, d1, d1n);
+    ld_pair<NTL>(s.D, r + ny + ep, d1p, x);
     a.Dj0 = d0; a.Dj1 = d1; a.Dp0 = d0n; a.Dp1 = d1n; a.Dm0 = d0m; a.Dm1 = d1m;
     b.Dj0 = d0n; b.Dj1 = d1n; b.Dp0 = d0p; b.Dp1 = d1p; b.Dm0 = d0; b.Dm1 = d1;
     double v0m, v0, v0n, v0p, v1m, v1, v1n, v1p;
-    ld_pair(s.Vc, r + em, x, v0m);
-    ld_pair(s.Vc, r + e, v0, v0n);
-    ld_pair(s.Vc, r + ep, v0p, x);
-    ld_pair(s.Vc, r + ny + em, x, v1m);
-    ld_pair(s.Vc, r + ny + e, v1, v1n);
-    ld_pair(s.Vc, r + ny + ep, v1p, x);
+    ld_pair<NTL>(s.Vc, r + em, x, v0m);
+    ld_pair<NTL>(s.Vc, r + e, v0, v0n);
+    ld_pair<NTL>(s.Vc, r + ep, v0p, x);
+    ld_pair<NTL>(s.Vc, r + ny + em, x, v1m);
+    ld_pair<NTL>(s.Vc, r + ny + e, v1, v1n);
+    ld_pair<NTL>(s.Vc, r + ny + ep, v1p, x);
     a.Vcp0 = v0n; a.Vcp1 = v1n; a.Vcm0 = v0m; a.Vcm1 = v1m;
     b.Vcp0 = v0p; b.Vcp1 = v1p; b.Vcm0 = v0; b.Vcm1 = v1;
     a.qx = b.qx = 0.0;
@@ -335,52 +342,55 @@ __host__ __device__ __forceinline__ QyyIn qyy_load(const Consts &k, const State 
 }
 
 // Elements j = e and e + 1 of row i (e even, ny even), from loads of column pairs.
+template <bool STR = false>
 __host__ __device__ __forceinline__ void qyy_load2(const Consts &k, const State &s, const RunFields &rf, const Work &w,
                                                    int i, int e, QyyIn &a, QyyIn &b)
 {
+    constexpr bool NTL = Nt<NT_QYY_ROWS, STR>::L;
+    constexpr bool NTF = NTL || Nt<NT_QYY_STREAM, STR>::L;   // single-use fields
     const int ny = k.ny;
     const long r = (long)i * ny;
     const int em = (e == 0) ? ny - 2 : e - 2, ep = (e + 2 == ny) ? 0 : e + 2;
     double x, gm, g0, g1, gp, gpp;
-    ld_pair(w.gD, r + em, x, gm);
-    ld_pair(w.gD, r + e, g0, g1);
-    ld_pair(w.gD, r + ep, gp, gpp);
+    ld_pair<NTL>(w.gD, r + em, x, gm);
+    ld_pair<NTL>(w.gD, r + e, g0, g1);
+    ld_pair<NTL>(w.gD, r + ep, gp, gpp);
     a.g_m = gm; a.g_j = g0; a.g_p = g1; a.g_pp = gp;
     b.g_m = g0; b.g_j = g1; b.g_p = gp; b.g_pp = gpp;
     int wmm, wm, w0, w1, wp, wpp;
-    ld_mask_pair(w.w, r + em, wmm, wm);
-    ld_mask_pair(w.w, r + e, w0, w1);
-    ld_mask_pair(w.w, r + ep, wp, wpp);
+    ld_mask_pair<NTL>(w.w, r + em, wmm, wm);
+    ld_mask_pair<NTL>(w.w, r + e, w0, w1);
+    ld_mask_pair<NTL>(w.w, r + ep, wp, wpp);
     a.wy_m = wm; a.wy_j = w0; a.wy_p = w1;
     b.wy_m = w0; b.wy_j = w1; b.wy_p = wp;
-    ld_mask_pair(w.w, e, a.w0, b.w0);
+    ld_mask_pair<NTL>(w.w, e, a.w0, b.w0);
     double z0, z1, z2, q0, q1, q2;
-    ld_pair(s.Zs, r + e, z0, z1);
-    ld_pair(s.Zs, r + ep, z2, x);
-    ld_pair(s.dQx, r + e, q0, q1);
-    ld_pair(s.dQx, r + ep, q2, x);
+    ld_pair<NTL>(s.Zs, r + e, z0, z1);
+    ld_pair<NTL>(s.Zs, r + ep, z2, x);
+    ld_pair<NTL>(s.dQx, r + e, q0, q1);
+    ld_pair<NTL>(s.dQx, r + ep, q2, x);
     a.zs0 = z0; a.zs1 = z1; b.zs0 = z1; b.zs1 = z2;
     a.dqx0 = q0; a.dqx1 = q1; b.dqx0 = q1; b.dqx1 = q2;
     double u00, u01, u02, u10, u11, u12;
-    ld_pair(s.U, r + e, u00, u01);
-    ld_pair(s.U, r + ep, u02, x);
-    ld_pair(s.U, r + ny + e, u10, u11);
-    ld_pair(s.U, r + ny + ep, u12, x);
+    ld_pair<NTL>(s.U, r + e, u00, u01);
+    ld_pair<NTL>(s.U, r + ep, u02, x);
+    ld_pair<NTL>(s.U, r + ny + e, u10, u11);
+    ld_pair<NTL>(s.U, r + ny + ep, u12, x);
     a.u0j = u00; a.u0p = u01; a.u1j = u10; a.u1p = u11;
     b.u0j = u01; b.u0p = u02; b.u1j = u11; b.u1p = u12;
     double vm, v0, v1, vp;
-    ld_pair(s.V, r + ny + em, x, vm);
-    ld_pair(s.V, r + ny + e, v0, v1);
-    ld_pair(s.V, r + ny + ep, vp, x);
+    ld_pair<NTL>(s.V, r + ny + em, x, vm);
+    ld_pair<NTL>(s.V, r + ny + e, v0, v1);
+    ld_pair<NTL>(s.V, r + ny + ep, vp, x);
     a.vm = vm; a.vj = v0; a.vp = v1;
     b.vm = v0; b.vj = v1; b.vp = vp;
-    ld_pair(rf.Fy, r + e, a.Fy, b.Fy);
-    ld_pair(s.Qy, r + ny + e, a.Qy1, b.Qy1);
-    ld_pair(rf.dnuDy_dx, r + e, a.dnuDy_dx, b.dnuDy_dx);
-    ld_pair(rf.nuD_y, r + e, a.nuD_y, b.nuD_y);
-    ld_pair(rf.cD_y, r + e, a.cD, b.cD);
-    ld_pair(rf.Urms_y, r + e, a.Urms, b.Urms);
-    ld_pair(rf.dnuD_dy, r + e, a.dnuD_dy, b.dnuD_dy);
+    ld_pair<NTF>(rf.Fy, r + e, a.Fy, b.Fy);
+    ld_pair<NTF>(s.Qy, r + ny + e, a.Qy1, b.Qy1);
+    ld_pair<NTF>(rf.dnuDy_dx, r + e, a.dnuDy_dx, b.dnuDy_dx);
+    ld_pair<NTF>(rf.nuD_y, r + e, a.nuD_y, b.nuD_y);
+    ld_pair<NTF>(rf.cD_y, r + e, a.cD, b.cD);
+    ld_pair<NTF>(rf.Urms_y, r + e, a.Urms, b.Urms);
+    ld_pair<NTF>(rf.dnuD_dy, r + e, a.dnuD_dy, b.dnuD_dy);
 }
 
 __host__ __device__ __forceinline__ void asm_qyy_vals(const Consts &k, const Metrics &m, const QyyIn &q, int i,
@@ -445,6 +455,7 @@ __host__ __device__ __forceinline__ void asm_qyy(const Consts &k, const Metrics 
 
 // Qy projector, setSolverQyx_* (src/HydroQy.cpp:235-362) and setSolverQyx_BC
 // (src/HydroBoundaryConditions.cpp:41-60).
+template <bool STR = false>
 __host__ __device__ __forceinline__ void asm_qyx(const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
                                         const Work &w, int i, int j, double &a_out, double &b_out, double &c_out,
                                         double &f_out)
@@ -455,11 +466,13 @@ __host__ __device__ __forceinline__ void asm_qyx(const Consts &k, const Metrics 
     const int jp = wrap_inc(j, ny);
     const double dt = k.dt;
     const double *gD = w.gD;
-    const int wi = w.w[r + j];
-    const double drag = w.drag[r + j];
-    const double nud = rf.nuD_y[r + j], dnud_dx = rf.dnuDy_dx[r + j];
+    constexpr bool NTL = Nt<NT_QYX, STR>::L;
+    constexpr bool NTF = NTL || Nt<NT_QYX_STREAM, STR>::L;   // single-use fields
+    const int wi = ldn<NTL>(w.w + r + j);
+    const double drag = ldn<NTF>(w.drag + r + j);
+    const double nud = ldn<NTF>(rf.nuD_y + r + j), dnud_dx = ldn<NTF>(rf.dnuDy_dx + r + j);
     // setSolverQyx_f (:235-248)
-    const double fv = wi * s.Qy[r + ny + j] * (1 + dt * drag);
+    const double fv = wi * ldn<NTF>(s.Qy + r + ny + j) * (1 + dt * drag);
     // setSolverQyx_d (:250-285)
     double dv;
     {
@@ -467,22 +480,23 @@ __host__ __device__ __forceinline__ void asm_qyx(const Consts &k, const Metrics 
         const double *u = s.U + r;
         const double dx_d = m.difx_c_d[i], d2x_d = m.dif2x_c_d[i];
         const double inv_dx = m.inv_dxx[i];
-        const double inv_d = 1 / (0.5 * (gD[r + jp] + gD[r + j]));   // inv_Di row i+1
+        const double inv_d = 1 / (0.5 * (ldn<NTL>(gD + r + jp) + ldn<NTL>(gD + r + j)));   // inv_Di row i+1
         const int isl = !wi;
-        const double du_dx = 0.5 * inv_dx * ((u[ny + jp] + u[ny + j]) - (u[jp] + u[j]));
-        const double u_y = 0.5 * (x2c_up * (u[ny + jp] + u[ny + j]) + x2c_d * (u[jp] + u[j]));
+        const double u1p = ldn<NTL>(u + ny + jp), u1j = ldn<NTL>(u + ny + j), u0p = ldn<NTL>(u + jp), u0j = ldn<NTL>(u + j);
+        const double du_dx = 0.5 * inv_dx * ((u1p + u1j) - (u0p + u0j));
+        const double u_y = 0.5 * (x2c_up * (u1p + u1j) + x2c_d * (u0p + u0j));
         dv = wi * ((1 + dt * drag) + dt * (u_y * dx_d + du_dx) - dt * (inv_d * nud * d2x_d + inv_d * dnud_dx * dx_d)) + isl;
     }
     // Ui_j row i (C2Y of U_c), U_c formed from U rows i, i+1 as the Qx tail forms it
     const double *uu = s.U + r;
-    const double uc_j = m.x2c_d[i] * uu[j] + m.x2c_up[i] * uu[ny + j];
-    const double uc_p = m.x2c_d[i] * uu[jp] + m.x2c_up[i] * uu[ny + jp];
+    const double uc_j = m.x2c_d[i] * ldn<NTL>(uu + j) + m.x2c_up[i] * ldn<NTL>(uu + ny + j);
+    const double uc_p = m.x2c_d[i] * ldn<NTL>(uu + jp) + m.x2c_up[i] * ldn<NTL>(uu + ny + jp);
     const double uy = 0.5 * (uc_p + uc_j);
     // setSolverQyx_up (:287-311); inv_Di row i+2 = getD_y(i+1); IsWater row nx-1 is false
     double uv;
     {
-        const double inv_up = 1 / (0.5 * (gD[r + ny + jp] + gD[r + ny + j]));
-        const int wu = (i + 1 <= nx - 2) ? w.w[r + ny + j] : 0;
+        const double inv_up = 1 / (0.5 * (ldn<NTL>(gD + r + ny + jp) + ldn<NTL>(gD + r + ny + j)));
+        const int wu = (i + 1 <= nx - 2) ? ldn<NTL>(w.w + r + ny + j) : 0;
         const double ww = wi && wu;
         const double dx_up = m.difx_c_up[i], d2x_up = m.dif2x_c_up[i];
         uv = ww * dt * (uy * dx_up - nud * d2x_up * inv_up - dnud_dx * dx_up * inv_up);
@@ -502,8 +516,8 @@ __host__ __device__ __forceinline__ void asm_qyx(const Consts &k, const Metrics 
             const double inv_low = 1 / Dy0;
             lv = wi * dt * (uy * dx_low - nud * d2x_low * inv_low - dnud_dx * dx_low * inv_low);
         } else {
-            const double inv_low = 1 / (0.5 * (gD[r - ny + jp] + gD[r - ny + j]));
-            const int wl = w.w[r - ny + j];
+            const double inv_low = 1 / (0.5 * (ldn<NTL>(gD + r - ny + jp) + ldn<NTL>(gD + r - ny + j)));
+            const int wl = ldn<NTL>(w.w + r - ny + j);
             const double ww = wi && wl;
             lv = ww * dt * (uy * dx_low - nud * d2x_low * inv_low - dnud_dx * dx_low * inv_low);
         }

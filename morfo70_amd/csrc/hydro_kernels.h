@@ -15,15 +15,17 @@ hipError_t launch_qxy_assemble(hipStream_t st, const Consts &k, const State &s, 
                                const Work &w);
 // write_uc = false skips the U_c store: the step's own readers form U_c from U, so only
 // the last step of a Run needs it in memory. rev (here and in the FUSED sweeps below):
-// take the rows last to first; results do not depend on it.
+// take the rows last to first; results do not depend on it. stream (here and in the FUSED
+// sweeps below): the streaming cache policy (stream_policy); results do not depend on it.
 hipError_t launch_qx_tail(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const Work &w,
-                          double f0, bool write_uc = true, int rev = 0);
+                          double f0, bool write_uc = true, int rev = 0, bool stream = false);
 hipError_t launch_qyy_assemble(hipStream_t st, const Consts &k, const Metrics &m, const State &s,
                                const RunFields &rf, const Work &w);
 hipError_t launch_qyx_assemble(hipStream_t st, const Consts &k, const Metrics &m, const State &s,
                                const RunFields &rf, const Work &w);
 // write_vc = false skips the V_c store, for steps whose successors in the Run form V_c from V.
-hipError_t launch_qy_tail(hipStream_t st, const Consts &k, const State &s, bool write_vc = true, int rev = 0);
+hipError_t launch_qy_tail(hipStream_t st, const Consts &k, const State &s, bool write_vc = true, int rev = 0,
+                          bool stream = false);
 hipError_t launch_nonfinite(hipStream_t st, const double *x, long n, int *flag);
 // *flag = 0 by a one-thread kernel (capturable in a graph, unlike a memset node).
 hipError_t launch_flag_clear(hipStream_t st, int *flag);
@@ -53,17 +55,17 @@ enum ColsPass { COLS_ELIM = 0, COLS_REDUCED = 1, COLS_BACKSUB = 2 };
 // see launch_qy_tail), so they need no V_c in memory.
 hipError_t launch_cols3_qxx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
                             const Work &w, double f0, const ColsWork &cw, int tile, int pass, bool vc_from_v = false,
-                            int rev = 0);
+                            int rev = 0, bool stream = false);
 hipError_t launch_cols3_qyx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                            const Work &w, const ColsWork &cw, int tile, int pass, int rev = 0);
+                            const Work &w, const ColsWork &cw, int tile, int pass, int rev = 0, bool stream = false);
 bool rows_lds_supported(int n);
 int rows_block_for(int n, int rb);
 // Qx projector rows reading the Qx predictor's solution from its elimination and reduced
 // passes (E0..E2, X of cw) in place of a back-substitution pass.
 hipError_t launch_lds_qxy_bs(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w,
-                             const ColsWork &cw, int rb, bool vc_from_v = false, int rev = 0);
+                             const ColsWork &cw, int rb, bool vc_from_v = false, int rev = 0, bool stream = false);
 hipError_t launch_lds_qyy(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                          const Work &w, int rb, int rev = 0);
+                          const Work &w, int rb, int rev = 0, bool stream = false);
 
 // grouped (multi-scenario) launches; rev*: take the blocks last to first (the last
 // scenario's last rows first); results do not depend on it

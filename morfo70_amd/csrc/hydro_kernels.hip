@@ -234,27 +234,36 @@ __global__ void __launch_bounds__(256) k_qy_tail(Consts k, State s, int write_vc
 // instead of one load per neighbour and column. Every value is formed by the same
 // expressions as qx_tail_body / qy_tail_body, so the results are the same bits.
 // ------------------------------------------------------------------------------------
-__device__ __forceinline__ double2 ld2(const double *p, long e) { return *reinterpret_cast<const double2 *>(p + e); }
+template <bool NT = false>
+__device__ __forceinline__ double2 ld2(const double *p, long e)
+{
+    double2 v;
+    ldn2<NT>(p + e, v.x, v.y);
+    return v;
+}
+template <bool NT = false>
 __device__ __forceinline__ void st2(double *p, long e, double a, double b)
 {
-    *reinterpret_cast<double2 *>(p + e) = make_double2(a, b);
+    stn2<NT>(p + e, a, b);
 }
 
 // Columns j0, j0 + 1 of row i (j0 even, ny even).
+template <bool STR = false>
 __device__ __forceinline__ void qy_tail2_body(const Consts &k, const State &s, int i, int j0, bool write_vc = true)
 {
+    constexpr bool NTL = Nt<NT_QY_TAIL, STR>::L, NTS = Nt<NT_QY_TAIL, STR>::S;
     const int ny = k.ny, nx = k.nx;
     const long r = (long)i * ny;
     const int cm = (j0 == 0) ? ny - 2 : j0 - 2, cp = (j0 + 2 == ny) ? 0 : j0 + 2;
     // qy_row(i+1, .) at columns cm, cm+1 | j0, j0+1 | cp (row nx is Qy[nx-1]*inv_3)
     auto qyr = [&](int c) -> double2 {
-        if (i + 1 < nx) return ld2(s.Qy, (long)(i + 1) * ny + c);
-        const double2 v = ld2(s.Qy, (long)(nx - 1) * ny + c);
+        if (i + 1 < nx) return ld2<NTL>(s.Qy, (long)(i + 1) * ny + c);
+        const double2 v = ld2<NTL>(s.Qy, (long)(nx - 1) * ny + c);
         return make_double2(v.x * k.inv_3, v.y * k.inv_3);
     };
     const double2 qm = qyr(cm), q0 = qyr(j0), qp = qyr(cp);
-    const double2 xm = ld2(s.dQx, r + cm), x0 = ld2(s.dQx, r + j0), xp = ld2(s.dQx, r + cp);
-    const double2 dm = ld2(s.D, r + cm), d0 = ld2(s.D, r + j0), dp = ld2(s.D, r + cp);
+    const double2 xm = ld2<NTL>(s.dQx, r + cm), x0 = ld2<NTL>(s.dQx, r + j0), xp = ld2<NTL>(s.dQx, r + cp);
+    const double2 dm = ld2<NTL>(s.D, r + cm), d0 = ld2<NTL>(s.D, r + j0), dp = ld2<NTL>(s.D, r + cp);
     // columns j0-1 (m1), j0 (a), j0+1 (b), j0+2 (p1)
     const double dq_m1 = (qm.y - qm.x) * k.inv_dy;
     const double dq_a = (q0.x - qm.y) * k.inv_dy;
@@ -267,21 +276,21 @@ __device__ __forceinline__ void qy_tail2_body(const Consts &k, const State &s, i
     const double v_m1 = qm.y / (0.5 * (dn_m1 + dn_a));
     const double v_a = q0.x / (0.5 * (dn_a + dn_b));
     const double v_b = q0.y / (0.5 * (dn_b + dn_p1));
-    st2(s.dQy, r + j0, dq_a, dq_b);
-    const double2 zs = ld2(s.Zs, r + j0);
-    st2(s.Zs, r + j0, zs.x + inc_a, zs.y + inc_b);
-    st2(s.Dn, r + j0, dn_a, dn_b);
-    st2(s.V, r + ny + j0, v_a, v_b);
-    if (write_vc) st2(s.Vc, r + j0, 0.5 * (v_a + v_m1), 0.5 * (v_b + v_a));
+    st2<NTS>(s.dQy, r + j0, dq_a, dq_b);
+    const double2 zs = ld2<NTL>(s.Zs, r + j0);
+    st2<NTS>(s.Zs, r + j0, zs.x + inc_a, zs.y + inc_b);
+    st2<NTS>(s.Dn, r + j0, dn_a, dn_b);
+    st2<NTS>(s.V, r + ny + j0, v_a, v_b);
+    if (write_vc) st2<NTS>(s.Vc, r + j0, 0.5 * (v_a + v_m1), 0.5 * (v_b + v_a));
     if (i == 0) {
-        const double2 q1 = ld2(s.Qy, ny + j0);
+        const double2 q1 = ld2<NTL>(s.Qy, ny + j0);
         const double qa = k.bcQy * q1.x, qb = k.bcQy * q1.y;
-        st2(s.Qy, j0, qa, qb);
-        st2(s.V, j0, qa / (2 * dn_a - new_depth(s, k, 1, j0)), qb / (2 * dn_b - new_depth(s, k, 1, j0 + 1)));
+        st2<NTS>(s.Qy, j0, qa, qb);
+        st2<NTS>(s.V, j0, qa / (2 * dn_a - new_depth(s, k, 1, j0)), qb / (2 * dn_b - new_depth(s, k, 1, j0 + 1)));
     }
     if (i == nx - 1) {
-        const double2 ql = ld2(s.Qy, (long)(nx - 1) * ny + j0);
-        st2(s.Qy, (long)nx * ny + j0, ql.x * k.inv_3, ql.y * k.inv_3);
+        const double2 ql = ld2<NTL>(s.Qy, (long)(nx - 1) * ny + j0);
+        st2<NTS>(s.Qy, (long)nx * ny + j0, ql.x * k.inv_3, ql.y * k.inv_3);
     }
 }
 
@@ -289,24 +298,26 @@ __device__ __forceinline__ void qy_tail2_body(const Consts &k, const State &s, i
 // with the Qx rows i-1..i+2 and the guess depths of rows i-1..i+1 held in registers from
 // one row to the next, so each row of Qx, D and dQy_dy is loaded once per run instead of
 // four or three times. The same expressions on the same values: the same bits.
+template <bool STR = false>
 __device__ __forceinline__ void qx_tail2_rows(const Consts &k, const Metrics &m, const State &s, const Work &w,
                                               double f0, int i0, int i1, int j0, bool write_uc)
 {
+    constexpr bool NTL = Nt<NT_QX_TAIL, STR>::L, NTS = Nt<NT_QX_TAIL, STR>::S;
     const int ny = k.ny, nx = k.nx;
     const int c2 = (j0 + 2 == ny) ? 0 : j0 + 2;
     const double dt = k.dt;
     auto qrow = [&](int row) -> double2 {   // Qx row `row` at (j0, j0+1), through the row-0 BC
         if (row < 0 || row > nx) return make_double2(0.0, 0.0);
         if (row == 0) {
-            const double2 v = ld2(s.Qx, ny + j0);
+            const double2 v = ld2<NTL>(s.Qx, ny + j0);
             return make_double2(k.f1 * v.x + f0, k.f1 * v.y + f0);
         }
-        return ld2(s.Qx, (long)row * ny + j0);
+        return ld2<NTL>(s.Qx, (long)row * ny + j0);
     };
     auto qrow_c2 = [&](int row) -> double {   // the same at column j0+2
         if (row < 0 || row > nx) return 0.0;
-        if (row == 0) return k.f1 * s.Qx[ny + c2] + f0;
-        return s.Qx[(long)row * ny + c2];
+        if (row == 0) return k.f1 * ldn<NTL>(s.Qx + ny + c2) + f0;
+        return ldn<NTL>(s.Qx + (long)row * ny + c2);
     };
     struct G { double2 gd, dq, inc, dqy; };   // guess_D, dQx_dx, its depth increment, dQy_dy
     auto grow = [&](int row, double2 lo, double2 hi) -> G {
@@ -314,7 +325,7 @@ __device__ __forceinline__ void qx_tail2_rows(const Consts &k, const Metrics &m,
         g.gd = g.dq = g.inc = g.dqy = make_double2(0.0, 0.0);
         if (row < 0 || row > nx - 1) return g;
         const double idx = m.inv_dxx[row];
-        const double2 dqy = ld2(s.dQy, (long)row * ny + j0), dd = ld2(s.D, (long)row * ny + j0);
+        const double2 dqy = ld2<NTL>(s.dQy, (long)row * ny + j0), dd = ld2<NTL>(s.D, (long)row * ny + j0);
         const double dqa = idx * (hi.x - lo.x), dqb = idx * (hi.y - lo.y);
         const double ia = -dt * (dqa + dqy.x), ib = -dt * (dqb + dqy.y);
         g.gd = make_double2(dd.x + ia, dd.y + ib);
@@ -341,29 +352,29 @@ __device__ __forceinline__ void qx_tail2_rows(const Consts &k, const Metrics &m,
             u1a = q1.x / (0.5 * (g0.gd.x + g1.gd.x));
             u1b = q1.y / (0.5 * (g0.gd.y + g1.gd.y));
         } else {
-            const double2 un = ld2(s.U, (long)nx * ny + j0);   // U row nx is never written by the reference
+            const double2 un = ld2<NTL>(s.U, (long)nx * ny + j0);   // U row nx is never written by the reference
             u1a = un.x;
             u1b = un.y;
         }
-        const double2 zs = ld2(s.Zs, r + j0);
+        const double2 zs = ld2<NTL>(s.Zs, r + j0);
         const double gza = zs.x + g0.inc.x, gzb = zs.y + g0.inc.y;
         double gd_c2, gz_c2, dqy_c2;
         {   // guess_D / guess_Zs of column j0+2 (the right neighbour of j0+1)
             const double dq = m.inv_dxx[i] * (cqb - cqa);
-            dqy_c2 = s.dQy[r + c2];
+            dqy_c2 = ldn<NTL>(s.dQy + r + c2);
             const double incrZs = -dt * (dq + dqy_c2);
-            gd_c2 = s.D[r + c2] + incrZs;
-            gz_c2 = s.Zs[r + c2] + incrZs;
+            gd_c2 = ldn<NTL>(s.D + r + c2) + incrZs;
+            gz_c2 = ldn<NTL>(s.Zs + r + c2) + incrZs;
         }
         const int wa = wet_rule(g0.gd.x, g0.gd.y, gza, gzb, g0.dqy.x, g0.dqy.y, k.Dmask);
         const int wb = wet_rule(g0.gd.y, gd_c2, gzb, gz_c2, g0.dqy.y, dqy_c2, k.Dmask);
-        w.w[r + j0] = (uint8_t)wa;
-        w.w[r + j0 + 1] = (uint8_t)wb;
-        st2(s.dQx, r + j0, g0.dq.x, g0.dq.y);
-        st2(w.gD, r + j0, g0.gd.x, g0.gd.y);
-        st2(s.U, r + j0, ua, ub);
-        if (write_uc) st2(s.Uc, r + j0, m.x2c_d[i] * ua + m.x2c_up[i] * u1a, m.x2c_d[i] * ub + m.x2c_up[i] * u1b);
-        if (i == 0) st2(s.Qx, j0, q0.x, q0.y);
+        stn<NTS>(w.w + r + j0, (uint8_t)wa);
+        stn<NTS>(w.w + r + j0 + 1, (uint8_t)wb);
+        st2<NTS>(s.dQx, r + j0, g0.dq.x, g0.dq.y);
+        st2<NTS>(w.gD, r + j0, g0.gd.x, g0.gd.y);
+        st2<NTS>(s.U, r + j0, ua, ub);
+        if (write_uc) st2<NTS>(s.Uc, r + j0, m.x2c_d[i] * ua + m.x2c_up[i] * u1a, m.x2c_d[i] * ub + m.x2c_up[i] * u1b);
+        if (i == 0) st2<NTS>(s.Qx, j0, q0.x, q0.y);
         if (i + 1 < i1) {
             const double2 q3 = qrow(i + 3);
             gm = g0;
@@ -377,13 +388,15 @@ __device__ __forceinline__ void qx_tail2_rows(const Consts &k, const Metrics &m,
 }
 
 // Row i alone (the batched step's tails, one row per thread).
+template <bool STR = false>
 __device__ __forceinline__ void qx_tail2_body(const Consts &k, const Metrics &m, const State &s, const Work &w,
                                               double f0, int i, int j0, bool write_uc = true)
 {
-    qx_tail2_rows(k, m, s, w, f0, i, i + 1, j0, write_uc);
+    qx_tail2_rows<STR>(k, m, s, w, f0, i, i + 1, j0, write_uc);
 }
 
 // `rows` rows per thread, two columns (ny even): grid (ceil(ny/512), ceil(nx/rows)).
+template <bool STR>
 __global__ void __launch_bounds__(256) k_qx_tail2r(Consts k, Metrics m, State s, Work w, double f0, int write_uc,
                                                    int rows, int rev)
 {
@@ -391,7 +404,7 @@ __global__ void __launch_bounds__(256) k_qx_tail2r(Consts k, Metrics m, State s,
     const int i0 = (rev ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y) * rows;   // rev: row runs last to first
     if (j0 >= k.ny || i0 >= k.nx) return;
     const int i1 = i0 + rows < k.nx ? i0 + rows : k.nx;
-    qx_tail2_rows(k, m, s, w, f0, i0, i1, j0, write_uc != 0);
+    qx_tail2_rows<STR>(k, m, s, w, f0, i0, i1, j0, write_uc != 0);
 }
 
 // Rows per thread of k_qx_tail2r: up to 8, while the launch keeps at least 8192 workgroups
@@ -405,12 +418,13 @@ static int qx_tail_rows(const Consts &k)
 }
 
 
+template <bool STR>
 __global__ void __launch_bounds__(256) k_qy_tail2(Consts k, State s, int write_vc, int rev)
 {
     const int j0 = 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
     const int i = rev ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;   // rev: rows last to first
     if (j0 >= k.ny || i >= k.nx) return;
-    qy_tail2_body(k, s, i, j0, write_vc != 0);
+    qy_tail2_body<STR>(k, s, i, j0, write_vc != 0);
 }
 
 // ------------------------------------------------------------------------------------
@@ -465,8 +479,8 @@ __global__ void __launch_bounds__(256) k_cells_grouped(const ScenDesc *__restric
         const int i = lb / gx2;
         const int j0 = 2 * ((lb - i * gx2) * 256 + (int)threadIdx.x);
         if (j0 >= ny || i >= d.k.nx) return;
-        if (OP == 2) qx_tail2_body(d.k, d.m, d.s, d.w, scen_f0(d), i, j0);
-        else qy_tail2_body(d.k, d.s, i, j0);
+        if (OP == 2) qx_tail2_body<GROUPED_STREAM>(d.k, d.m, d.s, d.w, scen_f0(d), i, j0);
+        else qy_tail2_body<GROUPED_STREAM>(d.k, d.s, i, j0);
         return;
     }
     const int gx = (ny + 255) / 256;
@@ -523,12 +537,12 @@ hipError_t launch_qxy_assemble(hipStream_t st, const Consts &k, const State &s, 
     return hipGetLastError();
 }
 hipError_t launch_qx_tail(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const Work &w,
-                          double f0, bool write_uc, int rev)
+                          double f0, bool write_uc, int rev, bool stream)
 {
     if (k.ny % 2 == 0 && k.ny >= 4) {
         const int rows = qx_tail_rows(k);
-        hipLaunchKernelGGL(k_qx_tail2r, cell_grid(k.ny / 2, (k.nx + rows - 1) / rows), dim3(256), 0, st, k, m, s, w, f0,
-                           (int)write_uc, rows, rev);
+        hipLaunchKernelGGL(stream ? k_qx_tail2r<true> : k_qx_tail2r<false>, cell_grid(k.ny / 2, (k.nx + rows - 1) / rows),
+                           dim3(256), 0, st, k, m, s, w, f0, (int)write_uc, rows, rev);
     } else {
         hipLaunchKernelGGL(k_qx_tail, cell_grid(k.ny, k.nx), dim3(256), 0, st, k, m, s, w, f0, (int)write_uc, rev);
     }
@@ -546,10 +560,11 @@ hipError_t launch_qyx_assemble(hipStream_t st, const Consts &k, const Metrics &m
     hipLaunchKernelGGL(k_qyx_assemble, cell_grid(k.ny, k.nx - 1), dim3(256), 0, st, k, m, s, rf, w);
     return hipGetLastError();
 }
-hipError_t launch_qy_tail(hipStream_t st, const Consts &k, const State &s, bool write_vc, int rev)
+hipError_t launch_qy_tail(hipStream_t st, const Consts &k, const State &s, bool write_vc, int rev, bool stream)
 {
     if (k.ny % 2 == 0 && k.ny >= 4)
-        hipLaunchKernelGGL(k_qy_tail2, cell_grid(k.ny / 2, k.nx), dim3(256), 0, st, k, s, (int)write_vc, rev);
+        hipLaunchKernelGGL(stream ? k_qy_tail2<true> : k_qy_tail2<false>, cell_grid(k.ny / 2, k.nx), dim3(256), 0, st, k,
+                           s, (int)write_vc, rev);
     else
         hipLaunchKernelGGL(k_qy_tail, cell_grid(k.ny, k.nx), dim3(256), 0, st, k, s, (int)write_vc, rev);
     return hipGetLastError();

@@ -204,24 +204,28 @@ int run_step(m70h *h, bool last = true, bool first = true)
         // order flips from step to step (sweep_rev). Results do not depend on it.
         const int a = h->sweep_rev, b = 1 - a;
         h->sweep_rev = b;
+        // cache policy by grid size: large grids stream (non-temporal accesses, see
+        // stream_policy), grids whose kernels fit the MALL keep plain ones
+        const bool str = stream_policy(nx, ny);
         for (int pass = COLS_ELIM; pass <= COLS_REDUCED; ++pass)
             if ((rc = timed(h, K_COLS_QX_ELIM + pass, [&] {
-                     return launch_cols3_qxx(st, k, h->m, s, h->rf, w, f0, h->cw, h->tile_cols, pass, !first, a);
+                     return launch_cols3_qxx(st, k, h->m, s, h->rf, w, f0, h->cw, h->tile_cols, pass, !first, a, str);
                  })))
                 return rc;
         if ((rc = timed(h, K_FUSED_QXY,
-                        [&] { return launch_lds_qxy_bs(st, k, s, h->rf, w, h->cw, h->rows_block, !first, b); })))
+                        [&] { return launch_lds_qxy_bs(st, k, s, h->rf, w, h->cw, h->rows_block, !first, b, str); })))
             return rc;
-        if ((rc = timed(h, K_QX_TAIL, [&] { return launch_qx_tail(st, k, h->m, s, w, f0, last, a); }))) return rc;
-        if ((rc = timed(h, K_FUSED_QYY, [&] { return launch_lds_qyy(st, k, h->m, s, h->rf, w, h->rows_block, b); })))
+        if ((rc = timed(h, K_QX_TAIL, [&] { return launch_qx_tail(st, k, h->m, s, w, f0, last, a, str); }))) return rc;
+        if ((rc = timed(h, K_FUSED_QYY,
+                        [&] { return launch_lds_qyy(st, k, h->m, s, h->rf, w, h->rows_block, b, str); })))
             return rc;
         for (int pass = COLS_ELIM; pass <= COLS_BACKSUB; ++pass)
             if ((rc = timed(h, K_COLS_QY_ELIM + pass, [&] {
                      return launch_cols3_qyx(st, k, h->m, s, h->rf, w, h->cw, h->tile_cols, pass,
-                                             pass == COLS_BACKSUB ? b : a);
+                                             pass == COLS_BACKSUB ? b : a, str);
                  })))
                 return rc;
-        if ((rc = timed(h, K_QY_TAIL, [&] { return launch_qy_tail(st, k, s, last, a); }))) return rc;
+        if ((rc = timed(h, K_QY_TAIL, [&] { return launch_qy_tail(st, k, s, last, a, str); }))) return rc;
         std::swap(s.D, s.Dn);
         return M70H_OK;
     }

@@ -46,16 +46,27 @@ __device__ __forceinline__ long elem_index(const LineGeom &g, int line, int k, l
 }
 
 // Coefficient providers: get(line, k, a, b, c, f) yields element k of line `line`.
-template <bool COLS>
+template <bool COLS, int SITE = NT_NONE, bool STR = false>
 struct ProvArrays {   // coefficients stored in four arrays
+    static constexpr int nt_site = SITE;
+    static constexpr bool nt_stream = STR;
     const double *A, *B, *C, *F;
     long ld;
     __device__ __forceinline__ void get(int line, int k, double &a, double &b, double &c, double &f) const
     {
+        constexpr bool NTL = Nt<SITE, STR>::L;
         const long e = COLS ? (long)k * ld + line : (long)line * ld + k;
-        a = A[e]; b = B[e]; c = C[e]; f = F[e];
+        a = ldn<NTL>(A + e); b = ldn<NTL>(B + e); c = ldn<NTL>(C + e); f = ldn<NTL>(F + e);
     }
 };
+// The cache-policy site of a coefficient provider (NtSite, NT_NONE if it names none) and
+// whether it runs under the streaming policy.
+template <class P, class = void> struct ProvSite { static constexpr int value = NT_NONE; };
+template <class P> struct ProvSite<P, std::void_t<decltype(P::nt_site)>> { static constexpr int value = P::nt_site; };
+template <class P, class = void> struct ProvStream { static constexpr bool value = false; };
+template <class P> struct ProvStream<P, std::void_t<decltype(P::nt_stream)>> { static constexpr bool value = P::nt_stream; };
+// The store policy of a provider's kernel.
+template <class P> constexpr bool prov_nts() { return Nt<ProvSite<P>::value, ProvStream<P>::value>::S; }
 // Row providers with get2() also yield elements kk and kk + 1 together (kk even, even
 // line length) from loads of column pairs.
 __device__ __forceinline__ void st_pair(double *p, long e, double a, double b)
@@ -79,19 +90,21 @@ __device__ __forceinline__ void qxy_vc_from_v(const Consts &k, const State &s, i
     q.Vcm0 = 0.5 * (v1[jm] + v1[jmm]); q.Vcm1 = 0.5 * (v2[jm] + v2[jmm]);
 }
 // Elements e, e+1 (e even, ny even): V columns e-2..e+2 from three pair loads per row.
+template <bool STR>
 __device__ __forceinline__ void qxy_vc_from_v2(const Consts &k, const State &s, int i, int e, QxyIn &a, QxyIn &b)
 {
+    constexpr bool NTL = Nt<NT_QXY_ROWS, STR>::L;
     const int ny = k.ny;
     const int em = (e == 0) ? ny - 2 : e - 2, ep = (e + 2 == ny) ? 0 : e + 2;
     const long r1 = (long)(i + 1) * ny, r2 = r1 + ny;
     double x, p2, p1, p0, q1, q2;   // V row i+1 at columns e-2, e-1, e, e+1, e+2
-    ld_pair(s.V, r1 + em, p2, p1);
-    ld_pair(s.V, r1 + e, p0, q1);
-    ld_pair(s.V, r1 + ep, q2, x);
+    ld_pair<NTL>(s.V, r1 + em, p2, p1);
+    ld_pair<NTL>(s.V, r1 + e, p0, q1);
+    ld_pair<NTL>(s.V, r1 + ep, q2, x);
     double t2, t1, t0, u1, u2;      // V row i+2
-    ld_pair(s.V, r2 + em, t2, t1);
-    ld_pair(s.V, r2 + e, t0, u1);
-    ld_pair(s.V, r2 + ep, u2, x);
+    ld_pair<NTL>(s.V, r2 + em, t2, t1);
+    ld_pair<NTL>(s.V, r2 + e, t0, u1);
+    ld_pair<NTL>(s.V, r2 + ep, u2, x);
     // V_c at columns e-1 (m), e (0), e+1 (n), e+2 (p) of rows i and i+1
     const double c0m = 0.5 * (p1 + p2), c00 = 0.5 * (p0 + p1), c0n = 0.5 * (q1 + p0), c0p = 0.5 * (q2 + q1);
     const double c1m = 0.5 * (t1 + t2), c10 = 0.5 * (t0 + t1), c1n = 0.5 * (u1 + t0), c1p = 0.5 * (u2 + u1);
@@ -99,8 +112,10 @@ __device__ __forceinline__ void qxy_vc_from_v2(const Consts &k, const State &s, 
     b.Vcp0 = c0p; b.Vcp1 = c1p; b.Vcm0 = c00; b.Vcm1 = c10;
 }
 
-template <bool VCF>
+template <bool VCF, bool STR>
 struct ProvQxyBs {
+    static constexpr int nt_site = NT_QXY_ROWS;
+    static constexpr bool nt_stream = STR;
     Consts k; State s; RunFields rf; Work w; ColsWork cw;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     {
@@ -113,9 +128,9 @@ struct ProvQxyBs {
                                          double &b1, double &c1, double &f1) const
     {
         QxyIn q0, q1;
-        qxy_load2(k, s, rf, w, line, kk, q0, q1);
-        if (VCF) qxy_vc_from_v2(k, s, line, kk, q0, q1);
-        cols_value2(cw, k.nx - 1, k.ny, line, kk, q0.qx, q1.qx);
+        qxy_load2<STR>(k, s, rf, w, line, kk, q0, q1);
+        if (VCF) qxy_vc_from_v2<STR>(k, s, line, kk, q0, q1);
+        cols_value2<STR>(cw, k.nx - 1, k.ny, line, kk, q0.qx, q1.qx);
         asm_qxy_vals(k, q0, a0, b0, c0, f0);
         asm_qxy_vals(k, q1, a1, b1, c1, f1);
     }
@@ -130,7 +145,7 @@ constexpr int CM = 8;   // rows per column segment in the three-pass column solv
 // nuD_x, dnuD_dx and dnuDx_dy are formed from nuD rows i, i+1 at columns j, j+-1 by
 // run_fields_body's expressions (one array in the window instead of three), and with the
 // constant drag model (CDC) cD_x is cD0 itself (0.5*(cD0+cD0) is exact), not read.
-template <bool CDC, bool VCF>
+template <bool CDC, bool VCF, bool STR>
 __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, const State &s, const RunFields &rf,
                                             const Work &w, double f0, int j, int k0, int mm, double (&lo)[CM],
                                             double (&di)[CM], double (&up)[CM], double (&rh)[CM])
@@ -141,20 +156,24 @@ __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, 
     auto atm = [&](int i) { return (long)i * ny + jm; };
     auto atp = [&](int i) { return (long)i * ny + jp; };
     auto cl = [&](int i) { return i < nx ? i : nx - 1; };
+    constexpr bool NTL = Nt<NT_QXX, STR>::L, NTS = Nt<NT_QXX, STR>::S;
+    constexpr bool NTF = NTL || Nt<NT_QXX_STREAM, STR>::L;   // single-use fields
+    auto ld = [&](const double *f, long e) { return ldn<NTL>(f + e); };    // stencil-shared fields
+    auto ldf = [&](const double *f, long e) { return ldn<NTF>(f + e); };   // single-use fields
     const double *nuD = rf.nuD;
-    double n0 = nuD[at(k0)], n0p = nuD[atp(k0)], n0m = nuD[atm(k0)];   // nuD row i
+    double n0 = ld(nuD, at(k0)), n0p = ld(nuD, atp(k0)), n0m = ld(nuD, atm(k0));   // nuD row i
     QxxIn q;
     int wis[CM];
     double drags[CM];
     {
         const int i = k0, im = i > 0 ? i - 1 : 0, i2 = cl(i + 2);
-        q.Dm = s.D[at(im)]; q.D0 = s.D[at(i)]; q.D1 = s.D[at(i + 1)]; q.D2 = s.D[at(i2)];
-        q.Zm = s.Zs[at(im)]; q.Z0 = s.Zs[at(i)]; q.Z1 = s.Zs[at(i + 1)]; q.Z2 = s.Zs[at(i2)];
-        q.Qm = s.dQx[at(im)]; q.Q0 = s.dQx[at(i)]; q.Q1 = s.dQx[at(i + 1)]; q.Q2 = s.dQx[at(i2)];
-        if (!VCF) { q.Vc0 = s.Vc[at(i)]; q.Vc1 = s.Vc[at(i + 1)]; }
-        q.U0 = s.U[at(i)]; q.U1 = s.U[at(i + 1)]; q.U2 = s.U[at(i + 2)];
-        q.dqy0 = s.dQy[at(i)]; q.dqy1 = s.dQy[at(i + 1)];
-        q.v1j = s.V[at(i + 1)]; q.v1m = s.V[atm(i + 1)]; q.v2j = s.V[at(i + 2)]; q.v2m = s.V[atm(i + 2)];
+        q.Dm = ld(s.D, at(im)); q.D0 = ld(s.D, at(i)); q.D1 = ld(s.D, at(i + 1)); q.D2 = ld(s.D, at(i2));
+        q.Zm = ld(s.Zs, at(im)); q.Z0 = ld(s.Zs, at(i)); q.Z1 = ld(s.Zs, at(i + 1)); q.Z2 = ld(s.Zs, at(i2));
+        q.Qm = ld(s.dQx, at(im)); q.Q0 = ld(s.dQx, at(i)); q.Q1 = ld(s.dQx, at(i + 1)); q.Q2 = ld(s.dQx, at(i2));
+        if (!VCF) { q.Vc0 = ld(s.Vc, at(i)); q.Vc1 = ld(s.Vc, at(i + 1)); }
+        q.U0 = ld(s.U, at(i)); q.U1 = ld(s.U, at(i + 1)); q.U2 = ld(s.U, at(i + 2));
+        q.dqy0 = ldf(s.dQy, at(i)); q.dqy1 = ldf(s.dQy, at(i + 1));
+        q.v1j = ld(s.V, at(i + 1)); q.v1m = ld(s.V, atm(i + 1)); q.v2j = ld(s.V, at(i + 2)); q.v2m = ld(s.V, atm(i + 2));
     }
 #pragma unroll
     for (int t = 0; t < CM; ++t) {
@@ -162,22 +181,22 @@ __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, 
             const int i = k0 + t;
             if (t > 0) {
                 const int i2 = cl(i + 2);
-                q.Dm = q.D0; q.D0 = q.D1; q.D1 = q.D2; q.D2 = s.D[at(i2)];
-                q.Zm = q.Z0; q.Z0 = q.Z1; q.Z1 = q.Z2; q.Z2 = s.Zs[at(i2)];
-                q.Qm = q.Q0; q.Q0 = q.Q1; q.Q1 = q.Q2; q.Q2 = s.dQx[at(i2)];
-                if (!VCF) { q.Vc0 = q.Vc1; q.Vc1 = s.Vc[at(i + 1)]; }
-                q.U0 = q.U1; q.U1 = q.U2; q.U2 = s.U[at(i + 2)];
-                q.dqy0 = q.dqy1; q.dqy1 = s.dQy[at(i + 1)];
-                q.v1j = q.v2j; q.v1m = q.v2m; q.v2j = s.V[at(i + 2)]; q.v2m = s.V[atm(i + 2)];
+                q.Dm = q.D0; q.D0 = q.D1; q.D1 = q.D2; q.D2 = ld(s.D, at(i2));
+                q.Zm = q.Z0; q.Z0 = q.Z1; q.Z1 = q.Z2; q.Z2 = ld(s.Zs, at(i2));
+                q.Qm = q.Q0; q.Q0 = q.Q1; q.Q1 = q.Q2; q.Q2 = ld(s.dQx, at(i2));
+                if (!VCF) { q.Vc0 = q.Vc1; q.Vc1 = ld(s.Vc, at(i + 1)); }
+                q.U0 = q.U1; q.U1 = q.U2; q.U2 = ld(s.U, at(i + 2));
+                q.dqy0 = q.dqy1; q.dqy1 = ldf(s.dQy, at(i + 1));
+                q.v1j = q.v2j; q.v1m = q.v2m; q.v2j = ld(s.V, at(i + 2)); q.v2m = ld(s.V, atm(i + 2));
             }
             if (VCF) {   // V_c rows i, i+1 from the V window (rows i+1, i+2 at j, j-1)
                 q.Vc0 = 0.5 * (q.v1j + q.v1m);
                 q.Vc1 = 0.5 * (q.v2j + q.v2m);
             }
-            q.Qx1 = s.Qx[at(i + 1)];
-            q.cD = CDC ? k.cD0 : rf.cD_x[at(i)];
-            q.Urms = rf.Urms_x[at(i)]; q.Fx = rf.Fx[at(i)];
-            const double n1 = nuD[at(i + 1)], n1p = nuD[atp(i + 1)], n1m = nuD[atm(i + 1)];
+            q.Qx1 = ldf(s.Qx, at(i + 1));
+            q.cD = CDC ? k.cD0 : ldf(rf.cD_x, at(i));
+            q.Urms = ldf(rf.Urms_x, at(i)); q.Fx = ldf(rf.Fx, at(i));
+            const double n1 = ld(nuD, at(i + 1)), n1p = ld(nuD, atp(i + 1)), n1m = ld(nuD, atm(i + 1));
             q.nuD_x = 0.5 * (n0 + n1);
             q.dnuD_dx = mt.inv_dx[i] * (n1 - n0);
             const double nux_p = 0.5 * (n0p + n1p), nux_m = 0.5 * (n0m + n1m);
@@ -191,36 +210,43 @@ __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, 
 #pragma unroll
     for (int t = 0; t < CM; ++t) {
         if (t < mm) {
-            w.w[at(k0 + t)] = (uint8_t)wis[t];
-            w.drag[at(k0 + t)] = drags[t];
+            stn<NTS>(w.w + at(k0 + t), (uint8_t)wis[t]);
+            stn<NTS>(w.drag + at(k0 + t), drags[t]);
         }
     }
 }
 
-template <bool CDC, bool VCF>
+template <bool CDC, bool VCF, bool STR>
 struct ProvQxx {      // Qx predictor, lines along x (columns): line = j, k = i
+    static constexpr int nt_site = NT_QXX;
+    static constexpr bool nt_stream = STR;
     Consts k; Metrics m; State s; RunFields rf; Work w; double f0;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qxx(k, m, s, rf, w, f0, kk, line, a, b, c, f); }
     __device__ __forceinline__ void segment(int line, int k0, int mm, double (&lo)[CM], double (&di)[CM],
                                             double (&up)[CM], double (&rh)[CM]) const
-    { qxx_segment<CDC, VCF>(k, m, s, rf, w, f0, line, k0, mm, lo, di, up, rh); }
+    { qxx_segment<CDC, VCF, STR>(k, m, s, rf, w, f0, line, k0, mm, lo, di, up, rh); }
 };
+template <bool STR>
 struct ProvQyx {      // Qy projector, columns
+    static constexpr int nt_site = NT_QYX;
+    static constexpr bool nt_stream = STR;
     Consts k; Metrics m; State s; RunFields rf; Work w;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
-    { asm_qyx(k, m, s, rf, w, kk, line, a, b, c, f); }
+    { asm_qyx<STR>(k, m, s, rf, w, kk, line, a, b, c, f); }
 };
 // nuD_y and dnuD_dy of elements e, e+1 of row i formed from nuD row i by run_fields_body's
 // expressions, in place of qyy_load2's loads of the two arrays (which become dead).
+template <bool STR>
 __device__ __forceinline__ void qyy_nud_pair(const Consts &k, const RunFields &rf, int i, int e, QyyIn &a, QyyIn &b)
 {
+    constexpr bool NTL = Nt<NT_QYY_ROWS, STR>::L;
     const int ny = k.ny;
     const long r = (long)i * ny;
     const int ep = (e + 2 == ny) ? 0 : e + 2;
     double n0, n1, n2, x;
-    ld_pair(rf.nuD, r + e, n0, n1);
-    ld_pair(rf.nuD, r + ep, n2, x);
+    ld_pair<NTL>(rf.nuD, r + e, n0, n1);
+    ld_pair<NTL>(rf.nuD, r + ep, n2, x);
     a.nuD_y = 0.5 * (n1 + n0);
     a.dnuD_dy = (n1 - n0) * k.inv_dy;
     b.nuD_y = 0.5 * (n2 + n1);
@@ -228,8 +254,10 @@ __device__ __forceinline__ void qyy_nud_pair(const Consts &k, const RunFields &r
 }
 
 // CDC: constant drag model, cD_y is cD0 itself (0.5*(cD0+cD0) is exact) and is not read.
-template <bool CDC>
+template <bool CDC, bool STR>
 struct ProvQyy {      // Qy predictor, rows
+    static constexpr int nt_site = NT_QYY_ROWS;
+    static constexpr bool nt_stream = STR;
     Consts k; Metrics m; State s; RunFields rf; Work w;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qyy(k, m, s, rf, w, line, kk, a, b, c, f); }
@@ -237,13 +265,13 @@ struct ProvQyy {      // Qy predictor, rows
                                          double &b1, double &c1, double &f1) const
     {
         QyyIn q0, q1;
-        qyy_load2(k, s, rf, w, line, kk, q0, q1);
-        qyy_nud_pair(k, rf, line, kk, q0, q1);
+        qyy_load2<STR>(k, s, rf, w, line, kk, q0, q1);
+        qyy_nud_pair<STR>(k, rf, line, kk, q0, q1);
         if (CDC) q0.cD = q1.cD = k.cD0;
         double d0, d1;
         asm_qyy_vals(k, m, q0, line, a0, b0, c0, f0, d0);
         asm_qyy_vals(k, m, q1, line, a1, b1, c1, f1, d1);
-        st_pair(w.drag, (long)line * k.ny + kk, d0, d1);
+        stn2<Nt<NT_QYY_ROWS, STR>::S>(w.drag + (long)line * k.ny + kk, d0, d1);
     }
 };
 
@@ -518,7 +546,9 @@ __device__ __forceinline__ void partition_body(const Prov &prov, double *__restr
     if (active) {
 #pragma unroll
         for (int q = 0; q < SEG_M; ++q)
-            if (q < m) Y[elem_index<COLS>(g, line, k0 + q, g.ldy)] = seg_value<SEG_M>(q, m, lo, di, up, rh, xF, xL);
+            if (q < m)
+                stn<prov_nts<Prov>()>(Y + elem_index<COLS>(g, line, k0 + q, g.ldy),
+                                                   seg_value<SEG_M>(q, m, lo, di, up, rh, xF, xL));
     }
 }
 
@@ -662,7 +692,7 @@ __device__ __forceinline__ void rows_lds_body(const Prov &prov, double *__restri
     }
     __syncthreads();
     double *yl = Y + (long)line * ldy;
-    for (int e = t; e < n; e += RB) yl[e] = sh[0][lpad(e)];
+    for (int e = t; e < n; e += RB) stn<prov_nts<Prov>()>(yl + e, sh[0][lpad(e)]);
 }
 
 template <int RB, class Prov>
@@ -855,20 +885,22 @@ __device__ __forceinline__ void cols_elim_body(const Prov &prov, const ColsWork 
             if (q == m - 1) up[q] = 0.0;
     }
     const RedRows r = seg_eliminate<CM>(m, lo, di, up, rh);
+    constexpr bool NTS = prov_nts<Prov>();
 #pragma unroll
     for (int q = 1; q < CM - 1; ++q) {
         if (q < m - 1) {
             const long o = (long)(k0 + q) * ny + j;
-            cw.E0[o] = lo[q] * di[q];
-            cw.E1[o] = up[q] * di[q];
-            cw.E2[o] = rh[q] * di[q];
+            stn<NTS>(cw.E0 + o, lo[q] * di[q]);
+            stn<NTS>(cw.E1 + o, up[q] * di[q]);
+            stn<NTS>(cw.E2 + o, rh[q] * di[q]);
         }
     }
     const long oF = (long)(2 * sg) * ny + j, oL = oF + ny;
-    cw.R0[oF] = r.lo0; cw.R1[oF] = r.A_F; cw.R2[oF] = r.C_F; cw.R3[oF] = r.R_F;
-    cw.R0[oL] = r.A_L; cw.R1[oL] = r.B_L; cw.R2[oL] = r.upL; cw.R3[oL] = r.R_L;
+    stn<NTS>(cw.R0 + oF, r.lo0); stn<NTS>(cw.R1 + oF, r.A_F); stn<NTS>(cw.R2 + oF, r.C_F); stn<NTS>(cw.R3 + oF, r.R_F);
+    stn<NTS>(cw.R0 + oL, r.A_L); stn<NTS>(cw.R1 + oL, r.B_L); stn<NTS>(cw.R2 + oL, r.upL); stn<NTS>(cw.R3 + oL, r.R_L);
 }
 
+template <bool STR>
 __device__ __forceinline__ void cols_backsub_body(const ColsWork &cw, double *__restrict__ Y, int n, int ny, int bx,
                                                   int by)
 {
@@ -880,8 +912,9 @@ __device__ __forceinline__ void cols_backsub_body(const ColsWork &cw, double *__
     const int k0 = (int)(((long)sg * n) / nseg);
     const int k1 = (int)(((long)(sg + 1) * n) / nseg);
     const int m = k1 - k0;
-    const double xF = cw.X[(long)(2 * sg) * ny + j];
-    const double xL = cw.X[(long)(2 * sg + 1) * ny + j];
+    constexpr bool NTL = Nt<NT_BACKSUB, STR>::L, NTS = Nt<NT_BACKSUB, STR>::S;
+    const double xF = ldn<NTL>(cw.X + (long)(2 * sg) * ny + j);
+    const double xL = ldn<NTL>(cw.X + (long)(2 * sg + 1) * ny + j);
 #pragma unroll
     for (int q = 0; q < CM; ++q) {
         if (q < m) {
@@ -889,8 +922,8 @@ __device__ __forceinline__ void cols_backsub_body(const ColsWork &cw, double *__
             double x;
             if (q == 0) x = xF;
             else if (q == m - 1) x = xL;
-            else x = cw.E2[o] - cw.E0[o] * xF - cw.E1[o] * xL;
-            Y[o] = x;
+            else x = ldn<NTL>(cw.E2 + o) - ldn<NTL>(cw.E0 + o) * xF - ldn<NTL>(cw.E1 + o) * xL;
+            stn<NTS>(Y + o, x);
         }
     }
 }
@@ -902,9 +935,10 @@ __global__ void __launch_bounds__(256) k_cols_elim(Prov prov, ColsWork cw, int n
     cols_elim_body(prov, cw, n, ny, (int)blockIdx.x, rev ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y);
 }
 
+template <bool STR>
 __global__ void __launch_bounds__(256) k_cols_backsub(ColsWork cw, double *__restrict__ Y, int n, int ny, int rev)
 {
-    cols_backsub_body(cw, Y, n, ny, (int)blockIdx.x, rev ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y);
+    cols_backsub_body<STR>(cw, Y, n, ny, (int)blockIdx.x, rev ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y);
 }
 
 // One pass of the three-pass column solve: COLS_ELIM (assemble + eliminate the segments),
@@ -923,30 +957,41 @@ static hipError_t launch_cols3(hipStream_t st, const Prov &prov, const ColsWork 
         return hipGetLastError();
     case COLS_REDUCED:
         // diagonal R1, lower R0, upper R2, rhs R3; 2*nseg rows per column
-        return dispatch_cols(st, ProvArrays<true>{cw.R1, cw.R0, cw.R2, cw.R3, (long)ny}, cw.X, ny, 2 * cw.nseg, ny, tile);
+        return dispatch_cols(st, ProvArrays<true, NT_REDUCED, ProvStream<Prov>::value>{cw.R1, cw.R0, cw.R2, cw.R3, (long)ny},
+                             cw.X, ny, 2 * cw.nseg, ny, tile);
     case COLS_BACKSUB:
-        hipLaunchKernelGGL(k_cols_backsub, grid, block, 0, st, cw, Y, n, ny, rev);
+        hipLaunchKernelGGL(k_cols_backsub<ProvStream<Prov>::value>, grid, block, 0, st, cw, Y, n, ny, rev);
         return hipGetLastError();
     default:
         return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_cols3_qxx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                            const Work &w, double f0, const ColsWork &cw, int tile, int pass, bool vc_from_v, int rev)
+template <bool CDC, bool VCF>
+static hipError_t launch_cols3_qxx_t(hipStream_t st, const Consts &k, const Metrics &m, const State &s,
+                                     const RunFields &rf, const Work &w, double f0, const ColsWork &cw, int tile,
+                                     int pass, int rev, bool stream)
 {
     double *Y = s.Qx + k.ny;
     const int n = k.nx - 1, ny = k.ny;
+    return stream ? launch_cols3(st, ProvQxx<CDC, VCF, true>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass, rev)
+                  : launch_cols3(st, ProvQxx<CDC, VCF, false>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass, rev);
+}
+hipError_t launch_cols3_qxx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
+                            const Work &w, double f0, const ColsWork &cw, int tile, int pass, bool vc_from_v, int rev,
+                            bool stream)
+{
     if (k.drag_model == 0)
-        return vc_from_v ? launch_cols3(st, ProvQxx<true, true>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass, rev)
-                         : launch_cols3(st, ProvQxx<true, false>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass, rev);
-    return vc_from_v ? launch_cols3(st, ProvQxx<false, true>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass, rev)
-                     : launch_cols3(st, ProvQxx<false, false>{k, m, s, rf, w, f0}, cw, Y, n, ny, tile, pass, rev);
+        return vc_from_v ? launch_cols3_qxx_t<true, true>(st, k, m, s, rf, w, f0, cw, tile, pass, rev, stream)
+                         : launch_cols3_qxx_t<true, false>(st, k, m, s, rf, w, f0, cw, tile, pass, rev, stream);
+    return vc_from_v ? launch_cols3_qxx_t<false, true>(st, k, m, s, rf, w, f0, cw, tile, pass, rev, stream)
+                     : launch_cols3_qxx_t<false, false>(st, k, m, s, rf, w, f0, cw, tile, pass, rev, stream);
 }
 hipError_t launch_cols3_qyx(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                            const Work &w, const ColsWork &cw, int tile, int pass, int rev)
+                            const Work &w, const ColsWork &cw, int tile, int pass, int rev, bool stream)
 {
-    return launch_cols3(st, ProvQyx{k, m, s, rf, w}, cw, s.Qy + k.ny, k.nx - 1, k.ny, tile, pass, rev);
+    return stream ? launch_cols3(st, ProvQyx<true>{k, m, s, rf, w}, cw, s.Qy + k.ny, k.nx - 1, k.ny, tile, pass, rev)
+                  : launch_cols3(st, ProvQyx<false>{k, m, s, rf, w}, cw, s.Qy + k.ny, k.nx - 1, k.ny, tile, pass, rev);
 }
 
 bool rows_lds_supported(int n) { return pick_segments(n, RM, 1024) > 0; }
@@ -968,19 +1013,25 @@ int rows_block_for(int n, int rb)
 // descriptor in place.
 // ---------------------------------------------------------------------------------
 struct GProvQxx {
+    static constexpr int nt_site = NT_QXX;
+    static constexpr bool nt_stream = GROUPED_STREAM;
     const ScenDesc *d;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qxx(d->k, d->m, d->s, d->rf, d->w, scen_f0(*d), kk, line, a, b, c, f); }
     __device__ __forceinline__ void segment(int line, int k0, int mm, double (&lo)[CM], double (&di)[CM],
                                             double (&up)[CM], double (&rh)[CM]) const
-    { qxx_segment<false, false>(d->k, d->m, d->s, d->rf, d->w, scen_f0(*d), line, k0, mm, lo, di, up, rh); }
+    { qxx_segment<false, false, GROUPED_STREAM>(d->k, d->m, d->s, d->rf, d->w, scen_f0(*d), line, k0, mm, lo, di, up, rh); }
 };
 struct GProvQyx {
+    static constexpr int nt_site = NT_QYX;
+    static constexpr bool nt_stream = GROUPED_STREAM;
     const ScenDesc *d;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
-    { asm_qyx(d->k, d->m, d->s, d->rf, d->w, kk, line, a, b, c, f); }
+    { asm_qyx<GROUPED_STREAM>(d->k, d->m, d->s, d->rf, d->w, kk, line, a, b, c, f); }
 };
 struct GProvQyy {
+    static constexpr int nt_site = NT_QYY_ROWS;
+    static constexpr bool nt_stream = GROUPED_STREAM;
     const ScenDesc *d;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qyy(d->k, d->m, d->s, d->rf, d->w, line, kk, a, b, c, f); }
@@ -988,18 +1039,20 @@ struct GProvQyy {
                                          double &b1, double &c1, double &f1) const
     {
         QyyIn q0, q1;
-        qyy_load2(d->k, d->s, d->rf, d->w, line, kk, q0, q1);
-        qyy_nud_pair(d->k, d->rf, line, kk, q0, q1);
+        qyy_load2<GROUPED_STREAM>(d->k, d->s, d->rf, d->w, line, kk, q0, q1);
+        qyy_nud_pair<GROUPED_STREAM>(d->k, d->rf, line, kk, q0, q1);
         double d0, d1;
         asm_qyy_vals(d->k, d->m, q0, line, a0, b0, c0, f0, d0);
         asm_qyy_vals(d->k, d->m, q1, line, a1, b1, c1, f1, d1);
-        st_pair(d->w.drag, (long)line * d->k.ny + kk, d0, d1);
+        stn2<Nt<NT_QYY_ROWS, GROUPED_STREAM>::S>(d->w.drag + (long)line * d->k.ny + kk, d0, d1);
     }
 };
 
 // Qx projector rows of a grouped step, with Qx* formed from the scenario's column passes
 // (as ProvQxyBs).
 struct GProvQxyBs {
+    static constexpr int nt_site = NT_QXY_ROWS;
+    static constexpr bool nt_stream = GROUPED_STREAM;
     const ScenDesc *d;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     {
@@ -1011,8 +1064,8 @@ struct GProvQxyBs {
                                          double &b1, double &c1, double &f1) const
     {
         QxyIn q0, q1;
-        qxy_load2(d->k, d->s, d->rf, d->w, line, kk, q0, q1);
-        cols_value2(d->cw, d->k.nx - 1, d->k.ny, line, kk, q0.qx, q1.qx);
+        qxy_load2<GROUPED_STREAM>(d->k, d->s, d->rf, d->w, line, kk, q0, q1);
+        cols_value2<GROUPED_STREAM>(d->cw, d->k.nx - 1, d->k.ny, line, kk, q0.qx, q1.qx);
         asm_qxy_vals(d->k, q0, a0, b0, c0, f0);
         asm_qxy_vals(d->k, q1, a1, b1, c1, f1);
     }
@@ -1058,7 +1111,7 @@ __global__ void __launch_bounds__(256) k_cols_backsub_grouped(const ScenDesc *__
     const int lb = b - start[sid];
     const int gx = (ny + 63) / 64;
     const int by = lb / gx, bx = lb - by * gx;
-    cols_backsub_body(d->cw, (WHICH == 0 ? d->s.Qx : d->s.Qy) + ny, d->k.nx - 1, ny, bx, by);
+    cols_backsub_body<GROUPED_STREAM>(d->cw, (WHICH == 0 ? d->s.Qx : d->s.Qy) + ny, d->k.nx - 1, ny, bx, by);
 }
 
 // Reduced column systems of every scenario (partition solver, 8 unknowns per thread).
@@ -1076,7 +1129,7 @@ __global__ void __launch_bounds__(BLOCK) k_red_pcr_grouped(const ScenDesc *__res
     LineGeom g{};
     g.ld = ny; g.ldy = ny; g.n = 2 * d->cw.nseg; g.nL = ny; g.S = d->S_red; g.lines_per_block = d->L_red;
     g.periodic = 0;
-    partition_body<true, BLOCK, 8>(ProvArrays<true>{d->cw.R1, d->cw.R0, d->cw.R2, d->cw.R3, (long)ny}, d->cw.X, g,
+    partition_body<true, BLOCK, 8>(ProvArrays<true, NT_REDUCED, GROUPED_STREAM>{d->cw.R1, d->cw.R0, d->cw.R2, d->cw.R3, (long)ny}, d->cw.X, g,
                                    b - start[sid], red);
 }
 
@@ -1251,19 +1304,34 @@ static hipError_t launch_rows_lds(hipStream_t st, const Prov &prov, double *Y, l
 }
 
 // Row sweeps with the assembly staged through LDS (one row per block).
-hipError_t launch_lds_qxy_bs(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w,
-                             const ColsWork &cw, int rb, bool vc_from_v, int rev)
+template <bool STR>
+static hipError_t launch_lds_qxy_bs_t(hipStream_t st, const Consts &k, const State &s, const RunFields &rf,
+                                      const Work &w, const ColsWork &cw, int rb, bool vc_from_v, int rev)
 {
     if (vc_from_v)
-        return launch_rows_lds(st, ProvQxyBs<true>{k, s, rf, w, cw}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true, rb, rev);
-    return launch_rows_lds(st, ProvQxyBs<false>{k, s, rf, w, cw}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true, rb, rev);
+        return launch_rows_lds(st, ProvQxyBs<true, STR>{k, s, rf, w, cw}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true, rb,
+                               rev);
+    return launch_rows_lds(st, ProvQxyBs<false, STR>{k, s, rf, w, cw}, s.Qx + k.ny, k.ny, k.ny, k.nx - 1, true, rb,
+                           rev);
 }
-hipError_t launch_lds_qyy(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
-                          const Work &w, int rb, int rev)
+hipError_t launch_lds_qxy_bs(hipStream_t st, const Consts &k, const State &s, const RunFields &rf, const Work &w,
+                             const ColsWork &cw, int rb, bool vc_from_v, int rev, bool stream)
+{
+    return stream ? launch_lds_qxy_bs_t<true>(st, k, s, rf, w, cw, rb, vc_from_v, rev)
+                  : launch_lds_qxy_bs_t<false>(st, k, s, rf, w, cw, rb, vc_from_v, rev);
+}
+template <bool STR>
+static hipError_t launch_lds_qyy_t(hipStream_t st, const Consts &k, const Metrics &m, const State &s,
+                                   const RunFields &rf, const Work &w, int rb, int rev)
 {
     if (k.drag_model == 0)
-        return launch_rows_lds(st, ProvQyy<true>{k, m, s, rf, w}, s.Qy + k.ny, k.ny, k.ny, k.nx - 1, true, rb, rev);
-    return launch_rows_lds(st, ProvQyy<false>{k, m, s, rf, w}, s.Qy + k.ny, k.ny, k.ny, k.nx - 1, true, rb, rev);
+        return launch_rows_lds(st, ProvQyy<true, STR>{k, m, s, rf, w}, s.Qy + k.ny, k.ny, k.ny, k.nx - 1, true, rb, rev);
+    return launch_rows_lds(st, ProvQyy<false, STR>{k, m, s, rf, w}, s.Qy + k.ny, k.ny, k.ny, k.nx - 1, true, rb, rev);
+}
+hipError_t launch_lds_qyy(hipStream_t st, const Consts &k, const Metrics &m, const State &s, const RunFields &rf,
+                          const Work &w, int rb, int rev, bool stream)
+{
+    return stream ? launch_lds_qyy_t<true>(st, k, m, s, rf, w, rb, rev) : launch_lds_qyy_t<false>(st, k, m, s, rf, w, rb, rev);
 }
 
 }  // namespace m70
