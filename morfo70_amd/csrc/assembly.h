@@ -344,7 +344,8 @@ __host__ __device__ __forceinline__ QyyIn qyy_load(const Consts &k, const State 
 // Elements j = e and e + 1 of row i (e even, ny even), from loads of column pairs.
 template <bool STR = false>
 __host__ __device__ __forceinline__ void qyy_load2(const Consts &k, const State &s, const RunFields &rf, const Work &w,
-                                                   int i, int e, QyyIn &a, QyyIn &b)
+                                                   int i, int e, QyyIn &a, QyyIn &b,
+                                                   bool load_cd = true)
 {
     constexpr bool NTL = Nt<NT_QYY_ROWS, STR>::L;
     constexpr bool NTF = NTL || Nt<NT_QYY_STREAM, STR>::L;   // single-use fields
@@ -388,7 +389,8 @@ __host__ __device__ __forceinline__ void qyy_load2(const Consts &k, const State 
     ld_pair<NTF>(s.Qy, r + ny + e, a.Qy1, b.Qy1);
     ld_pair<NTF>(rf.dnuDy_dx, r + e, a.dnuDy_dx, b.dnuDy_dx);
     ld_pair<NTF>(rf.nuD_y, r + e, a.nuD_y, b.nuD_y);
-    ld_pair<NTF>(rf.cD_y, r + e, a.cD, b.cD);
+    if (load_cd) ld_pair<NTF>(rf.cD_y, r + e, a.cD, b.cD);
+    else a.cD = b.cD = k.cD0;   // constant drag model: 0.5*(cD0+cD0) is cD0 exactly
     ld_pair<NTF>(rf.Urms_y, r + e, a.Urms, b.Urms);
     ld_pair<NTF>(rf.dnuD_dy, r + e, a.dnuD_dy, b.dnuD_dy);
 }

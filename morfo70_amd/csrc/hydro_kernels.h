@@ -70,14 +70,18 @@ hipError_t launch_lds_qyy(hipStream_t st, const Consts &k, const Metrics &m, con
 // grouped (multi-scenario) launches; rev*: take the blocks last to first (the last
 // scenario's last rows first); results do not depend on it
 int grouped_cell_blocks(int nx, int ny);
+// write_derived = false: the tails skip the U_c / V_c stores (a batched Run's steps before
+// its last form them from U and V)
 hipError_t launch_cells_grouped(hipStream_t st, int op, const ScenDesc *D, const int *start, int nscen, int blocks,
-                                int rev = 0);
+                                int rev = 0, bool write_derived = true);
 void grouped_stage_blocks(int nx, int ny, int rb, int *rows, int *cols, int *red, int *redp, int *S_rows, int *S_red,
                           int *L_red);
+// vc_from_v: the Qx sweeps form V_c from V (a batched Run's steps after its first)
 hipError_t launch_rows_grouped(hipStream_t st, int which, int rb, const ScenDesc *D, const int *start, int nscen,
-                               int blocks, int rev = 0);
+                               int blocks, int rev = 0, bool vc_from_v = false);
 hipError_t launch_cols3_grouped(hipStream_t st, int which, const ScenDesc *D, const int *start_cols,
                                 const int *start_red, const int *start_redp, int nscen, int blocks_cols, int blocks_red,
-                                int blocks_redp, bool backsub = true, int rev_elim = 0, int rev_backsub = 0);
+                                int blocks_redp, bool backsub = true, int rev_elim = 0, int rev_backsub = 0,
+                                bool vc_from_v = false);
 
 }  // namespace m70

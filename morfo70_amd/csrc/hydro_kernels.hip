@@ -461,7 +461,8 @@ __global__ void __launch_bounds__(256) k_init_zs(double *Zs, const double *D, co
 // of a scenario with an even ny take two columns per thread (ceil(ny/512) x nx blocks;
 // the scenario's other blocks return at once).
 // ------------------------------------------------------------------------------------
-template <int OP>
+// WD: the tails store U_c (OP 2) and V_c (OP 3)
+template <int OP, bool WD>
 __global__ void __launch_bounds__(256) k_cells_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
                                                        int nscen, int rev)
 {
@@ -479,8 +480,8 @@ __global__ void __launch_bounds__(256) k_cells_grouped(const ScenDesc *__restric
         const int i = lb / gx2;
         const int j0 = 2 * ((lb - i * gx2) * 256 + (int)threadIdx.x);
         if (j0 >= ny || i >= d.k.nx) return;
-        if (OP == 2) qx_tail2_body<GROUPED_STREAM>(d.k, d.m, d.s, d.w, scen_f0(d), i, j0);
-        else qy_tail2_body<GROUPED_STREAM>(d.k, d.s, i, j0);
+        if (OP == 2) qx_tail2_body<GROUPED_STREAM>(d.k, d.m, d.s, d.w, scen_f0(d), i, j0, WD);
+        else qy_tail2_body<GROUPED_STREAM>(d.k, d.s, i, j0, WD);
         return;
     }
     const int gx = (ny + 255) / 256;
@@ -489,20 +490,27 @@ __global__ void __launch_bounds__(256) k_cells_grouped(const ScenDesc *__restric
     if (j >= ny || i >= d.k.nx) return;
     if (OP == 0) drag_viscosity_body(d.k, d.s.D, d.fw, d.rf, i, j);
     else if (OP == 1) run_fields_body(d.k, d.m, d.fw, d.rf, i, j);
-    else if (OP == 2) qx_tail_body(d.k, d.m, d.s, d.w, scen_f0(d), i, j);
-    else qy_tail_body(d.k, d.s, i, j);
+    else if (OP == 2) qx_tail_body(d.k, d.m, d.s, d.w, scen_f0(d), i, j, WD);
+    else qy_tail_body(d.k, d.s, i, j, WD);
 }
 
 int grouped_cell_blocks(int nx, int ny) { return ((ny + 255) / 256) * nx; }
 
 hipError_t launch_cells_grouped(hipStream_t st, int op, const ScenDesc *D, const int *start, int nscen, int blocks,
-                                int rev)
+                                int rev, bool write_derived)
 {
+    const dim3 g((unsigned)blocks), b(256);
     switch (op) {
-    case 0: hipLaunchKernelGGL(k_cells_grouped<0>, dim3(blocks), dim3(256), 0, st, D, start, nscen, rev); break;
-    case 1: hipLaunchKernelGGL(k_cells_grouped<1>, dim3(blocks), dim3(256), 0, st, D, start, nscen, rev); break;
-    case 2: hipLaunchKernelGGL(k_cells_grouped<2>, dim3(blocks), dim3(256), 0, st, D, start, nscen, rev); break;
-    default: hipLaunchKernelGGL(k_cells_grouped<3>, dim3(blocks), dim3(256), 0, st, D, start, nscen, rev); break;
+    case 0: hipLaunchKernelGGL((k_cells_grouped<0, true>), g, b, 0, st, D, start, nscen, rev); break;
+    case 1: hipLaunchKernelGGL((k_cells_grouped<1, true>), g, b, 0, st, D, start, nscen, rev); break;
+    case 2:
+        if (write_derived) hipLaunchKernelGGL((k_cells_grouped<2, true>), g, b, 0, st, D, start, nscen, rev);
+        else hipLaunchKernelGGL((k_cells_grouped<2, false>), g, b, 0, st, D, start, nscen, rev);
+        break;
+    default:
+        if (write_derived) hipLaunchKernelGGL((k_cells_grouped<3, true>), g, b, 0, st, D, start, nscen, rev);
+        else hipLaunchKernelGGL((k_cells_grouped<3, false>), g, b, 0, st, D, start, nscen, rev);
+        break;
     }
     return hipGetLastError();
 }

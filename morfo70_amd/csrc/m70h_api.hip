@@ -973,7 +973,11 @@ static int batch_upload(m70h_batch *b)
     return M70H_OK;
 }
 
-static hipError_t batch_step(m70h_batch *b, int odd)
+// One grouped step. first: the first step of a batched Run, which reads V_c as the handles
+// hold it; later steps form it from V. last: the last step, which stores U_c and V_c (the
+// steps before it leave them to be formed from U and V, as run_step does). The defaults
+// give a step that reads and leaves every field in memory.
+static hipError_t batch_step(m70h_batch *b, int odd, bool first = true, bool last = true)
 {
     const int n = (int)b->hs.size();
     const ScenDesc *D = b->d_desc + (odd ? n : 0);
@@ -987,15 +991,16 @@ static hipError_t batch_step(m70h_batch *b, int odd)
     const int ra = odd, rb = 1 - odd;
     // the Qx projector rows form the Qx predictor's solution from its first two passes
     if ((e = launch_cols3_grouped(st, 0, D, st_cols, st_red, st_redp, n, b->blocks[GS_COLS], b->blocks[GS_RED],
-                                  b->blocks[GS_REDP], false, ra, rb)) != hipSuccess)
+                                  b->blocks[GS_REDP], false, ra, rb, !first)) != hipSuccess)
         return e;
-    if ((e = launch_rows_grouped(st, 0, b->rows_block, D, st_rows, n, b->blocks[GS_ROWS], rb)) != hipSuccess) return e;
-    if ((e = launch_cells_grouped(st, 2, D, st_cells, n, b->blocks[GS_CELLS], ra)) != hipSuccess) return e;
+    if ((e = launch_rows_grouped(st, 0, b->rows_block, D, st_rows, n, b->blocks[GS_ROWS], rb, !first)) != hipSuccess)
+        return e;
+    if ((e = launch_cells_grouped(st, 2, D, st_cells, n, b->blocks[GS_CELLS], ra, last)) != hipSuccess) return e;
     if ((e = launch_rows_grouped(st, 1, b->rows_block, D, st_rows, n, b->blocks[GS_ROWS], rb)) != hipSuccess) return e;
     if ((e = launch_cols3_grouped(st, 1, D, st_cols, st_red, st_redp, n, b->blocks[GS_COLS], b->blocks[GS_RED],
                                   b->blocks[GS_REDP], true, ra, rb)) != hipSuccess)
         return e;
-    return launch_cells_grouped(st, 3, D, st_cells, n, b->blocks[GS_CELLS], ra);
+    return launch_cells_grouped(st, 3, D, st_cells, n, b->blocks[GS_CELLS], ra, last);
 }
 
 extern "C" {
@@ -1103,13 +1108,20 @@ int m70h_batch_run_async(m70h_batch *b, int nIter)
     // per-Run pre-loop (src/Hydro.cpp:175-192) of every scenario
     BHIPCHK(b, launch_cells_grouped(st, 0, b->d_desc, b->d_start + GS_CELLS * (n + 1), n, b->blocks[GS_CELLS]));
     BHIPCHK(b, launch_cells_grouped(st, 1, b->d_desc, b->d_start + GS_CELLS * (n + 1), n, b->blocks[GS_CELLS]));
+    // the first step reads V_c as the handles hold it and the last one stores U_c and V_c;
+    // the steps between form them from U and V, two at a time from a graph: odd (Dn -> D)
+    // then even (D -> Dn)
     int it = 0;
-    if (b->use_graph && nIter >= 2) {
-        if (!b->gexec) {   // two steps: even (D -> Dn) and odd (Dn -> D)
+    if (nIter >= 1) {
+        BHIPCHK(b, batch_step(b, 0, true, nIter == 1));
+        it = 1;
+    }
+    if (b->use_graph && it + 2 <= nIter - 1) {
+        if (!b->gexec) {
             hipGraph_t g = nullptr;
             BHIPCHK(b, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-            hipError_t e1 = batch_step(b, 0);
-            hipError_t e2 = e1 == hipSuccess ? batch_step(b, 1) : e1;
+            hipError_t e1 = batch_step(b, 1, false, false);
+            hipError_t e2 = e1 == hipSuccess ? batch_step(b, 0, false, false) : e1;
             hipError_t e3 = hipStreamEndCapture(st, &g);
             if (e2 != hipSuccess || e3 != hipSuccess) {
                 if (g) (void)hipGraphDestroy(g);
@@ -1122,9 +1134,9 @@ int m70h_batch_run_async(m70h_batch *b, int nIter)
                 return bfail(b, M70H_EHIP, std::string("graph instantiate: ") + hipGetErrorString(e4));
             }
         }
-        for (; it + 2 <= nIter; it += 2) BHIPCHK(b, hipGraphLaunch(b->gexec, st));
+        for (; it + 2 <= nIter - 1; it += 2) BHIPCHK(b, hipGraphLaunch(b->gexec, st));
     }
-    for (; it < nIter; ++it) BHIPCHK(b, batch_step(b, it & 1));
+    for (; it < nIter; ++it) BHIPCHK(b, batch_step(b, it & 1, false, it == nIter - 1));
     if (nIter & 1)
         for (m70h *h : b->hs) std::swap(h->s.D, h->s.Dn);
     for (m70h *h : b->hs) h->f0_pos += nIter;

@@ -148,7 +148,8 @@ constexpr int CM = 8;   // rows per column segment in the three-pass column solv
 template <bool CDC, bool VCF, bool STR>
 __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, const State &s, const RunFields &rf,
                                             const Work &w, double f0, int j, int k0, int mm, double (&lo)[CM],
-                                            double (&di)[CM], double (&up)[CM], double (&rh)[CM])
+                                            double (&di)[CM], double (&up)[CM], double (&rh)[CM],
+                                            bool cdc_rt = false)
 {
     const int ny = k.ny, nx = k.nx;
     const int jm = wrap_dec(j, ny), jp = wrap_inc(j, ny);
@@ -194,7 +195,7 @@ __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, 
                 q.Vc1 = 0.5 * (q.v2j + q.v2m);
             }
             q.Qx1 = ldf(s.Qx, at(i + 1));
-            q.cD = CDC ? k.cD0 : ldf(rf.cD_x, at(i));
+            q.cD = (CDC || cdc_rt) ? k.cD0 : ldf(rf.cD_x, at(i));
             q.Urms = ldf(rf.Urms_x, at(i)); q.Fx = ldf(rf.Fx, at(i));
             const double n1 = ld(nuD, at(i + 1)), n1p = ld(nuD, atp(i + 1)), n1m = ld(nuD, atm(i + 1));
             q.nuD_x = 0.5 * (n0 + n1);
@@ -265,9 +266,8 @@ struct ProvQyy {      // Qy predictor, rows
                                          double &b1, double &c1, double &f1) const
     {
         QyyIn q0, q1;
-        qyy_load2<STR>(k, s, rf, w, line, kk, q0, q1);
+        qyy_load2<STR>(k, s, rf, w, line, kk, q0, q1, !CDC);
         qyy_nud_pair<STR>(k, rf, line, kk, q0, q1);
-        if (CDC) q0.cD = q1.cD = k.cD0;
         double d0, d1;
         asm_qyy_vals(k, m, q0, line, a0, b0, c0, f0, d0);
         asm_qyy_vals(k, m, q1, line, a1, b1, c1, f1, d1);
@@ -1012,6 +1012,7 @@ int rows_block_for(int n, int rb)
 // Grouped (multi-scenario) versions of the FUSED stages. Providers read the scenario's
 // descriptor in place.
 // ---------------------------------------------------------------------------------
+template <bool VCF>   // VCF: V_c formed from V (see ProvQxx)
 struct GProvQxx {
     static constexpr int nt_site = NT_QXX;
     static constexpr bool nt_stream = GROUPED_STREAM;
@@ -1020,7 +1021,10 @@ struct GProvQxx {
     { asm_qxx(d->k, d->m, d->s, d->rf, d->w, scen_f0(*d), kk, line, a, b, c, f); }
     __device__ __forceinline__ void segment(int line, int k0, int mm, double (&lo)[CM], double (&di)[CM],
                                             double (&up)[CM], double (&rh)[CM]) const
-    { qxx_segment<false, false, GROUPED_STREAM>(d->k, d->m, d->s, d->rf, d->w, scen_f0(*d), line, k0, mm, lo, di, up, rh); }
+    {
+        qxx_segment<false, VCF, GROUPED_STREAM>(d->k, d->m, d->s, d->rf, d->w, scen_f0(*d), line, k0, mm, lo, di, up, rh,
+                                                d->k.drag_model == 0);
+    }
 };
 struct GProvQyx {
     static constexpr int nt_site = NT_QYX;
@@ -1039,7 +1043,7 @@ struct GProvQyy {
                                          double &b1, double &c1, double &f1) const
     {
         QyyIn q0, q1;
-        qyy_load2<GROUPED_STREAM>(d->k, d->s, d->rf, d->w, line, kk, q0, q1);
+        qyy_load2<GROUPED_STREAM>(d->k, d->s, d->rf, d->w, line, kk, q0, q1, d->k.drag_model != 0);
         qyy_nud_pair<GROUPED_STREAM>(d->k, d->rf, line, kk, q0, q1);
         double d0, d1;
         asm_qyy_vals(d->k, d->m, q0, line, a0, b0, c0, f0, d0);
@@ -1050,6 +1054,7 @@ struct GProvQyy {
 
 // Qx projector rows of a grouped step, with Qx* formed from the scenario's column passes
 // (as ProvQxyBs).
+template <bool VCF>
 struct GProvQxyBs {
     static constexpr int nt_site = NT_QXY_ROWS;
     static constexpr bool nt_stream = GROUPED_STREAM;
@@ -1057,6 +1062,7 @@ struct GProvQxyBs {
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     {
         QxyIn q = qxy_load(d->k, d->s, d->rf, d->w, line, kk);
+        if (VCF) qxy_vc_from_v(d->k, d->s, line, kk, q);
         q.qx = cols_value(d->cw, d->k.nx - 1, d->k.ny, line, kk);
         asm_qxy_vals(d->k, q, a, b, c, f);
     }
@@ -1065,13 +1071,15 @@ struct GProvQxyBs {
     {
         QxyIn q0, q1;
         qxy_load2<GROUPED_STREAM>(d->k, d->s, d->rf, d->w, line, kk, q0, q1);
+        if (VCF) qxy_vc_from_v2<GROUPED_STREAM>(d->k, d->s, line, kk, q0, q1);
         cols_value2<GROUPED_STREAM>(d->cw, d->k.nx - 1, d->k.ny, line, kk, q0.qx, q1.qx);
         asm_qxy_vals(d->k, q0, a0, b0, c0, f0);
         asm_qxy_vals(d->k, q1, a1, b1, c1, f1);
     }
 };
 
-template <int WHICH, int RB>   // WHICH 0: Qx projector rows, 1: Qy predictor rows
+// WHICH 0: Qx projector rows, 1: Qy predictor rows; VCF: the Qx projector forms V_c from V
+template <int WHICH, int RB, bool VCF>
 __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_rows_lds_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start, int nscen, int rev)
 {
@@ -1081,11 +1089,13 @@ k_rows_lds_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start
     const ScenDesc *d = D + sid;
     const int ny = d->k.ny;
     const int line = b - start[sid];
-    if (WHICH == 0) rows_lds_body<RB>(GProvQxyBs{d}, d->s.Qx + ny, ny, ny, d->S_rows, 1, line, sh);
+    if (WHICH == 0) rows_lds_body<RB>(GProvQxyBs<VCF>{d}, d->s.Qx + ny, ny, ny, d->S_rows, 1, line, sh);
     else rows_lds_body<RB>(GProvQyy{d}, d->s.Qy + ny, ny, ny, d->S_rows, 1, line, sh);
 }
 
-template <int WHICH>   // 0: Qx predictor columns, 1: Qy projector columns
+// WHICH 0: Qx predictor columns, 1: Qy projector columns; VCF: the Qx predictor forms V_c
+// from V
+template <int WHICH, bool VCF>
 __global__ void __launch_bounds__(256) k_cols_elim_grouped(const ScenDesc *__restrict__ D,
                                                            const int *__restrict__ start, int nscen, int rev)
 {
@@ -1096,7 +1106,7 @@ __global__ void __launch_bounds__(256) k_cols_elim_grouped(const ScenDesc *__res
     const int lb = b - start[sid];
     const int gx = (ny + 63) / 64;
     const int by = lb / gx, bx = lb - by * gx;
-    if (WHICH == 0) cols_elim_body(GProvQxx{d}, d->cw, d->k.nx - 1, ny, bx, by);
+    if (WHICH == 0) cols_elim_body(GProvQxx<VCF>{d}, d->cw, d->k.nx - 1, ny, bx, by);
     else cols_elim_body(GProvQyx{d}, d->cw, d->k.nx - 1, ny, bx, by);
 }
 
@@ -1244,27 +1254,31 @@ void grouped_stage_blocks(int nx, int ny, int rb, int *rows, int *cols, int *red
 }
 
 hipError_t launch_rows_grouped(hipStream_t st, int which, int rb, const ScenDesc *D, const int *start, int nscen,
-                               int blocks, int rev)
+                               int blocks, int rev, bool vc_from_v)
 {
+    const dim3 g((unsigned)blocks);
     if (rb == 256) {
-        if (which == 0)
-            hipLaunchKernelGGL((k_rows_lds_grouped<0, 256>), dim3(blocks), dim3(256), 0, st, D, start, nscen, rev);
-        else hipLaunchKernelGGL((k_rows_lds_grouped<1, 256>), dim3(blocks), dim3(256), 0, st, D, start, nscen, rev);
+        if (which == 1) hipLaunchKernelGGL((k_rows_lds_grouped<1, 256, false>), g, dim3(256), 0, st, D, start, nscen, rev);
+        else if (vc_from_v)
+            hipLaunchKernelGGL((k_rows_lds_grouped<0, 256, true>), g, dim3(256), 0, st, D, start, nscen, rev);
+        else hipLaunchKernelGGL((k_rows_lds_grouped<0, 256, false>), g, dim3(256), 0, st, D, start, nscen, rev);
     } else {
-        if (which == 0)
-            hipLaunchKernelGGL((k_rows_lds_grouped<0, 512>), dim3(blocks), dim3(512), 0, st, D, start, nscen, rev);
-        else hipLaunchKernelGGL((k_rows_lds_grouped<1, 512>), dim3(blocks), dim3(512), 0, st, D, start, nscen, rev);
+        if (which == 1) hipLaunchKernelGGL((k_rows_lds_grouped<1, 512, false>), g, dim3(512), 0, st, D, start, nscen, rev);
+        else if (vc_from_v)
+            hipLaunchKernelGGL((k_rows_lds_grouped<0, 512, true>), g, dim3(512), 0, st, D, start, nscen, rev);
+        else hipLaunchKernelGGL((k_rows_lds_grouped<0, 512, false>), g, dim3(512), 0, st, D, start, nscen, rev);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_cols3_grouped(hipStream_t st, int which, const ScenDesc *D, const int *start_cols,
                                 const int *start_red, const int *start_redp, int nscen, int blocks_cols, int blocks_red,
-                                int blocks_redp, bool backsub, int rev_elim, int rev_backsub)
+                                int blocks_redp, bool backsub, int rev_elim, int rev_backsub, bool vc_from_v)
 {
-    if (which == 0)
-        hipLaunchKernelGGL(k_cols_elim_grouped<0>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen, rev_elim);
-    else hipLaunchKernelGGL(k_cols_elim_grouped<1>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen, rev_elim);
+    const dim3 g((unsigned)blocks_cols), blk(64, 4);
+    if (which == 1) hipLaunchKernelGGL((k_cols_elim_grouped<1, false>), g, blk, 0, st, D, start_cols, nscen, rev_elim);
+    else if (vc_from_v) hipLaunchKernelGGL((k_cols_elim_grouped<0, true>), g, blk, 0, st, D, start_cols, nscen, rev_elim);
+    else hipLaunchKernelGGL((k_cols_elim_grouped<0, false>), g, blk, 0, st, D, start_cols, nscen, rev_elim);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (blocks_red > 0) {   // scenarios whose reduced solve fits 256-thread workgroups
