@@ -123,8 +123,9 @@ __device__ __forceinline__ double scen_f0(const ScenDesc &d)
 }
 
 // Stages of a grouped step; start[stage][s] is the first block of scenario s.
-// GS_RED / GS_REDP: reduced column systems in 256- / 1024-thread workgroups.
-enum GroupStage { GS_CELLS = 0, GS_ROWS = 1, GS_COLS = 2, GS_RED = 3, GS_REDP = 4, GS_COUNT = 5 };
+// GS_RED / GS_REDP: reduced column systems in 256- / 1024-thread workgroups; GS_TX / GS_TY:
+// the Qx and Qy tails.
+enum GroupStage { GS_CELLS = 0, GS_ROWS = 1, GS_COLS = 2, GS_RED = 3, GS_REDP = 4, GS_TX = 5, GS_TY = 6, GS_COUNT = 7 };
 
 // Scenario owning block b: start[] has nscen+1 ascending entries, start[0] = 0.
 __device__ __forceinline__ int scen_of(const int *__restrict__ start, int nscen, int b)

@@ -70,9 +70,12 @@ hipError_t launch_lds_qyy(hipStream_t st, const Consts &k, const Metrics &m, con
 // grouped (multi-scenario) launches; rev*: take the blocks last to first (the last
 // scenario's last rows first); results do not depend on it
 int grouped_cell_blocks(int nx, int ny);
-// write_derived = false: the tails skip the U_c / V_c stores (a batched Run's steps before
-// its last form them from U and V)
-hipError_t launch_cells_grouped(hipStream_t st, int op, const ScenDesc *D, const int *start, int nscen, int blocks,
+// op 0: drag and viscosity, op 1: the per-Run fields
+hipError_t launch_cells_grouped(hipStream_t st, int op, const ScenDesc *D, const int *start, int nscen, int blocks);
+// op 2: the Qx tail, op 3: the Qy tail. write_derived = false skips the U_c / V_c stores (a
+// batched Run's steps before its last form them from U and V).
+int grouped_tail_blocks(int nx, int ny, int op);
+hipError_t launch_tails_grouped(hipStream_t st, int op, const ScenDesc *D, const int *start, int nscen, int blocks,
                                 int rev = 0, bool write_derived = true);
 void grouped_stage_blocks(int nx, int ny, int rb, int *rows, int *cols, int *red, int *redp, int *S_rows, int *S_red,
                           int *L_red);

@@ -457,60 +457,93 @@ __global__ void __launch_bounds__(256) k_init_zs(double *Zs, const double *D, co
 
 // ------------------------------------------------------------------------------------
 // Grouped cell kernels: one launch covers every scenario of a batch. Block b belongs to
-// scenario scen_of(start, nscen, b); within it blocks tile (ceil(ny/256) x nx). The tails
-// of a scenario with an even ny take two columns per thread (ceil(ny/512) x nx blocks;
-// the scenario's other blocks return at once).
+// scenario scen_of(start, nscen, b); within it blocks tile (ceil(ny/256) x nx).
 // ------------------------------------------------------------------------------------
-// WD: the tails store U_c (OP 2) and V_c (OP 3)
-template <int OP, bool WD>
+template <int OP>
 __global__ void __launch_bounds__(256) k_cells_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
+                                                       int nscen)
+{
+    const int b = (int)blockIdx.x;
+    const int sid = scen_of(start, nscen, b);
+    const ScenDesc &d = D[sid];
+    const int ny = d.k.ny;
+    const int lb = b - start[sid];
+    const int gx = (ny + 255) / 256;
+    const int i = lb / gx;
+    const int j = (lb - i * gx) * 256 + threadIdx.x;
+    if (j >= ny || i >= d.k.nx) return;
+    if (OP == 0) drag_viscosity_body(d.k, d.s.D, d.fw, d.rf, i, j);
+    else run_fields_body(d.k, d.m, d.fw, d.rf, i, j);
+}
+
+int grouped_cell_blocks(int nx, int ny) { return ((ny + 255) / 256) * nx; }
+
+hipError_t launch_cells_grouped(hipStream_t st, int op, const ScenDesc *D, const int *start, int nscen, int blocks)
+{
+    if (op == 0) hipLaunchKernelGGL(k_cells_grouped<0>, dim3(blocks), dim3(256), 0, st, D, start, nscen);
+    else hipLaunchKernelGGL(k_cells_grouped<1>, dim3(blocks), dim3(256), 0, st, D, start, nscen);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------
+// Grouped tails (OP 2: Qx, OP 3: Qy). A scenario's work items are numbered in one line and
+// taken 256 to a workgroup, so short rows leave no lanes idle: with an even ny an item is
+// a column pair (qx_tail2_rows / qy_tail2_body), over GT_ROWS rows for the Qx tail; with an
+// odd ny it is one cell. WD: the tails store U_c (OP 2) and V_c (OP 3).
+// ------------------------------------------------------------------------------------
+constexpr int GT_ROWS = 2;
+
+static long grouped_tail_items(int nx, int ny, int op)
+{
+    if (ny % 2 == 0 && ny >= 4) return op == 2 ? (long)((nx + GT_ROWS - 1) / GT_ROWS) * (ny / 2) : (long)nx * (ny / 2);
+    return (long)nx * ny;
+}
+int grouped_tail_blocks(int nx, int ny, int op) { return (int)((grouped_tail_items(nx, ny, op) + 255) / 256); }
+
+template <int OP, bool WD>
+__global__ void __launch_bounds__(256) k_tails_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
                                                        int nscen, int rev)
 {
-    // rev: the last scenario's last row first
+    // rev: the last scenario's last rows first
     const int b = rev ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x;
     // the Qy tail ends a step: count it for the offshore offset series (scen_f0); every
     // reader of the counter is in a later launch
     if (OP == 3 && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((int *)D[0].step, 1);
     const int sid = scen_of(start, nscen, b);
     const ScenDesc &d = D[sid];
-    const int ny = d.k.ny;
-    const int lb = b - start[sid];
-    if ((OP == 2 || OP == 3) && ny % 2 == 0 && ny >= 4) {
-        const int gx2 = (ny + 511) / 512;
-        const int i = lb / gx2;
-        const int j0 = 2 * ((lb - i * gx2) * 256 + (int)threadIdx.x);
-        if (j0 >= ny || i >= d.k.nx) return;
-        if (OP == 2) qx_tail2_body<GROUPED_STREAM>(d.k, d.m, d.s, d.w, scen_f0(d), i, j0, WD);
-        else qy_tail2_body<GROUPED_STREAM>(d.k, d.s, i, j0, WD);
+    const int nx = d.k.nx, ny = d.k.ny;
+    const long q = (long)(b - start[sid]) * 256 + threadIdx.x;
+    if (ny % 2 == 0 && ny >= 4) {
+        const int np = ny / 2;
+        const long r = q / np;
+        const int j0 = 2 * (int)(q - r * np);
+        if (OP == 2) {
+            if (r * GT_ROWS >= nx) return;
+            const int i0 = (int)r * GT_ROWS, i1 = i0 + GT_ROWS < nx ? i0 + GT_ROWS : nx;
+            qx_tail2_rows<GROUPED_STREAM>(d.k, d.m, d.s, d.w, scen_f0(d), i0, i1, j0, WD);
+        } else {
+            if (r >= nx) return;
+            qy_tail2_body<GROUPED_STREAM>(d.k, d.s, (int)r, j0, WD);
+        }
         return;
     }
-    const int gx = (ny + 255) / 256;
-    const int i = lb / gx;
-    const int j = (lb - i * gx) * 256 + threadIdx.x;
-    if (j >= ny || i >= d.k.nx) return;
-    if (OP == 0) drag_viscosity_body(d.k, d.s.D, d.fw, d.rf, i, j);
-    else if (OP == 1) run_fields_body(d.k, d.m, d.fw, d.rf, i, j);
-    else if (OP == 2) qx_tail_body(d.k, d.m, d.s, d.w, scen_f0(d), i, j, WD);
-    else qy_tail_body(d.k, d.s, i, j, WD);
+    const long i = q / ny;
+    const int j = (int)(q - i * ny);
+    if (i >= nx) return;
+    if (OP == 2) qx_tail_body(d.k, d.m, d.s, d.w, scen_f0(d), (int)i, j, WD);
+    else qy_tail_body(d.k, d.s, (int)i, j, WD);
 }
 
-int grouped_cell_blocks(int nx, int ny) { return ((ny + 255) / 256) * nx; }
-
-hipError_t launch_cells_grouped(hipStream_t st, int op, const ScenDesc *D, const int *start, int nscen, int blocks,
+hipError_t launch_tails_grouped(hipStream_t st, int op, const ScenDesc *D, const int *start, int nscen, int blocks,
                                 int rev, bool write_derived)
 {
     const dim3 g((unsigned)blocks), b(256);
-    switch (op) {
-    case 0: hipLaunchKernelGGL((k_cells_grouped<0, true>), g, b, 0, st, D, start, nscen, rev); break;
-    case 1: hipLaunchKernelGGL((k_cells_grouped<1, true>), g, b, 0, st, D, start, nscen, rev); break;
-    case 2:
-        if (write_derived) hipLaunchKernelGGL((k_cells_grouped<2, true>), g, b, 0, st, D, start, nscen, rev);
-        else hipLaunchKernelGGL((k_cells_grouped<2, false>), g, b, 0, st, D, start, nscen, rev);
-        break;
-    default:
-        if (write_derived) hipLaunchKernelGGL((k_cells_grouped<3, true>), g, b, 0, st, D, start, nscen, rev);
-        else hipLaunchKernelGGL((k_cells_grouped<3, false>), g, b, 0, st, D, start, nscen, rev);
-        break;
+    if (op == 2) {
+        if (write_derived) hipLaunchKernelGGL((k_tails_grouped<2, true>), g, b, 0, st, D, start, nscen, rev);
+        else hipLaunchKernelGGL((k_tails_grouped<2, false>), g, b, 0, st, D, start, nscen, rev);
+    } else {
+        if (write_derived) hipLaunchKernelGGL((k_tails_grouped<3, true>), g, b, 0, st, D, start, nscen, rev);
+        else hipLaunchKernelGGL((k_tails_grouped<3, false>), g, b, 0, st, D, start, nscen, rev);
     }
     return hipGetLastError();
 }

@@ -981,9 +981,10 @@ static hipError_t batch_step(m70h_batch *b, int odd, bool first = true, bool las
 {
     const int n = (int)b->hs.size();
     const ScenDesc *D = b->d_desc + (odd ? n : 0);
-    const int *st_cells = b->d_start + GS_CELLS * (n + 1), *st_rows = b->d_start + GS_ROWS * (n + 1);
+    const int *st_rows = b->d_start + GS_ROWS * (n + 1);
     const int *st_cols = b->d_start + GS_COLS * (n + 1), *st_red = b->d_start + GS_RED * (n + 1);
     const int *st_redp = b->d_start + GS_REDP * (n + 1);
+    const int *st_tx = b->d_start + GS_TX * (n + 1), *st_ty = b->d_start + GS_TY * (n + 1);
     hipStream_t st = b->stream;
     hipError_t e;
     // sweep order as run_step's: each stage starts where the one before it ended (those
@@ -995,12 +996,12 @@ static hipError_t batch_step(m70h_batch *b, int odd, bool first = true, bool las
         return e;
     if ((e = launch_rows_grouped(st, 0, b->rows_block, D, st_rows, n, b->blocks[GS_ROWS], rb, !first)) != hipSuccess)
         return e;
-    if ((e = launch_cells_grouped(st, 2, D, st_cells, n, b->blocks[GS_CELLS], ra, last)) != hipSuccess) return e;
+    if ((e = launch_tails_grouped(st, 2, D, st_tx, n, b->blocks[GS_TX], ra, last)) != hipSuccess) return e;
     if ((e = launch_rows_grouped(st, 1, b->rows_block, D, st_rows, n, b->blocks[GS_ROWS], rb)) != hipSuccess) return e;
     if ((e = launch_cols3_grouped(st, 1, D, st_cols, st_red, st_redp, n, b->blocks[GS_COLS], b->blocks[GS_RED],
                                   b->blocks[GS_REDP], true, ra, rb)) != hipSuccess)
         return e;
-    return launch_cells_grouped(st, 3, D, st_cells, n, b->blocks[GS_CELLS], ra, last);
+    return launch_tails_grouped(st, 3, D, st_ty, n, b->blocks[GS_TY], ra, last);
 }
 
 extern "C" {
@@ -1030,7 +1031,8 @@ int m70h_batch_create(int nscen, m70h *const *handles, m70h_batch **out)
         int rows, cols, red, redp, S_rows, S_red, L_red;
         grouped_stage_blocks(h->nx, h->ny, b->rows_block, &rows, &cols, &red, &redp, &S_rows, &S_red, &L_red);
         if (S_rows <= 0 || S_red <= 0) return bail(bfail(b, M70H_EINVAL, "grid too small for the batched step"));
-        const int per[GS_COUNT] = {grouped_cell_blocks(h->nx, h->ny), rows, cols, red, redp};
+        const int per[GS_COUNT] = {grouped_cell_blocks(h->nx, h->ny), rows, cols, red, redp,
+                                   grouped_tail_blocks(h->nx, h->ny, 2), grouped_tail_blocks(h->nx, h->ny, 3)};
         for (int gs = 0; gs < GS_COUNT; ++gs)
             starts[(size_t)gs * (nscen + 1) + q + 1] = starts[(size_t)gs * (nscen + 1) + q] + per[gs];
         b->hs.push_back(h);
