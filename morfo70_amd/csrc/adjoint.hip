@@ -55,11 +55,20 @@ __global__ void __launch_bounds__(256) k_adj_qxy(Consts k, State S0, State S1, R
     ADJ_CELL(k.nx - 1)
     adj_qxy(k, S0, S1, rf, T, A, W, R, i, j);
 }
-__global__ void __launch_bounds__(256) k_adj_qxx(Consts k, Metrics m, State S0, RunFields rf, StepTape T, AdjFields A,
-                                                 AdjWork W, AdjRun R, double f0)
+// The Qx predictor's reverse by runs of ADJ_RUN rows of a column (adj_qxx_rows): workgroups
+// of 64 x 4 threads, a wavefront being 64 consecutive columns of one run.
+// 4 rows: 2 and 8 took 0.5% and 3% longer over the C4 gradient.
+constexpr int ADJ_RUN = 4;
+__global__ void __launch_bounds__(256) k_adj_qxx_rows(Consts k, Metrics m, State S0, RunFields rf, StepTape T,
+                                                      AdjFields A, AdjWork W, AdjRun R, double f0)
 {
-    ADJ_CELL(k.nx - 1)
-    adj_qxx(k, m, S0, rf, T, A, W, R, f0, i, j);
+    const int j = blockIdx.x * 64 + threadIdx.x;
+    const int run = __builtin_amdgcn_readfirstlane((int)blockIdx.y * 4 + (int)threadIdx.y);
+    const int n = k.nx - 1;
+    const int k0 = run * ADJ_RUN;
+    if (j >= k.ny || k0 >= n) return;
+    const int k1 = k0 + ADJ_RUN < n ? k0 + ADJ_RUN : n;
+    adj_qxx_rows(k, m, S0, rf, T, A, W, R, f0, k0, k1, j);
 }
 __global__ void __launch_bounds__(256) k_adj_run_fields(Consts k, Metrics m, AdjRun R)
 {
@@ -175,7 +184,9 @@ hipError_t launch_adj_x(hipStream_t st, const Consts &k, const Metrics &m, const
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // X2: A1^T lam = rhs (columns)
     if ((e = launch_transposed_cols(st, T.a1, T.b1, T.c1, W.rhs, W.lam, nx - 1, ny, ny, tile)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_adj_qxx, cell_grid(k, nx - 1), dim3(256), 0, st, k, m, S0, rf, T, A, W, R, f0);
+    const int runs = (nx - 1 + ADJ_RUN - 1) / ADJ_RUN;
+    hipLaunchKernelGGL(k_adj_qxx_rows, dim3((unsigned)((ny + 63) / 64), (unsigned)((runs + 3) / 4)), dim3(64, 4), 0, st, k,
+                       m, S0, rf, T, A, W, R, f0);
     return hipGetLastError();
 }
 

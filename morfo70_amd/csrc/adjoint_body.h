@@ -517,9 +517,19 @@ __host__ __device__ __forceinline__ void adj_qxy(const Consts &k, const State &S
 // ----------------------------------------------------------------------------------
 // X1: Qx predictor assembly (asm_qxx), rows 0..nx-2. lam = transposed Thomas solution.
 // ----------------------------------------------------------------------------------
-__host__ __device__ __forceinline__ void adj_qxx(const Consts &k, const Metrics &m, const State &S0,
-                                                 const RunFields &rf, const StepTape &T, const AdjFields &A,
-                                                 const AdjWork &W, const AdjRun &R, double f0, int i, int j)
+// The adjoint parts element (i, j) of the Qx predictor sends to the state before the step;
+// the per-Run field adjoints it owns (R) are updated in place.
+struct QxxParts {
+    double qx;                  // A.Qx row i+1
+    double dq, dz;              // A.dQy and A.Zs: +dq/+dz at row i+1, -dq/-dz at row i
+    double v1j, v1m, v0j, v0m;  // A.V row i+2 (v1) and row i+1 (v0) at columns j and j-1
+    double ul, u1, uu;          // A.U rows i, i+1, i+2
+    double vc;                  // A.Vc rows i and i+1, each
+    double dm, d0, d1, d2;      // A.D rows i-1, i, i+1, i+2
+};
+__host__ __device__ __forceinline__ QxxParts adj_qxx_parts(const Consts &k, const Metrics &m, const State &S0,
+                                                           const RunFields &rf, const StepTape &T, const AdjWork &W,
+                                                           const AdjRun &R, double f0, int i, int j)
 {
     const int ny = k.ny, nx = k.nx;
     const long r = (long)i * ny;
@@ -614,22 +624,94 @@ __host__ __device__ __forceinline__ void adj_qxx(const Consts &k, const Metrics 
     R.cD_x[r + j] += cd_b;
     R.dnuD_dx[r + j] += dnud_b;
     R.nuD_x[r + j] += nud_b;
-    acc(&A.Qx[r + ny + j], s_qx);
-    acc(&A.dQy[r + ny + j], s_dq);
-    acc(&A.dQy[r + j], -s_dq);
-    acc(&A.Zs[r + ny + j], s_dz);
-    acc(&A.Zs[r + j], -s_dz);
-    scatter_row(A.V + r + 2L * ny, j, ny, s_v1j, s_v1m, 0.0);
-    scatter_row(A.V + r + ny, j, ny, s_v0j, s_v0m, 0.0);
-    acc(&A.U[r + j], s_ul);
-    if (i < nx - 2) acc(&A.U[r + 2L * ny + j], s_uu);
-    acc(&A.Vc[r + j], 0.5 * vx_b);
-    acc(&A.Vc[r + ny + j], 0.5 * vx_b);
-    acc(&A.U[r + ny + j], u1_b);
-    if (i > 0) acc(&A.D[r - ny + j], dD_m);
-    acc(&A.D[r + j], dD_0);
-    acc(&A.D[r + ny + j], dD_1);
-    if (i < nx - 2) acc(&A.D[r + 2L * ny + j], dD_2);
+    QxxParts p;
+    p.qx = s_qx;
+    p.dq = s_dq;
+    p.dz = s_dz;
+    p.v1j = s_v1j; p.v1m = s_v1m; p.v0j = s_v0j; p.v0m = s_v0m;
+    p.ul = s_ul;
+    p.u1 = u1_b;
+    p.uu = (i < nx - 2) ? s_uu : 0.0;
+    p.vc = 0.5 * vx_b;
+    p.dm = (i > 0) ? dD_m : 0.0;
+    p.d0 = dD_0;
+    p.d1 = dD_1;
+    p.d2 = (i < nx - 2) ? dD_2 : 0.0;
+    return p;
+}
+__host__ __device__ __forceinline__ void adj_qxx(const Consts &k, const Metrics &m, const State &S0,
+                                                 const RunFields &rf, const StepTape &T, const AdjFields &A,
+                                                 const AdjWork &W, const AdjRun &R, double f0, int i, int j)
+{
+    const int ny = k.ny, nx = k.nx;
+    const long r = (long)i * ny;
+    const QxxParts p = adj_qxx_parts(k, m, S0, rf, T, W, R, f0, i, j);
+    acc(&A.Qx[r + ny + j], p.qx);
+    acc(&A.dQy[r + ny + j], p.dq);
+    acc(&A.dQy[r + j], -p.dq);
+    acc(&A.Zs[r + ny + j], p.dz);
+    acc(&A.Zs[r + j], -p.dz);
+    scatter_row(A.V + r + 2L * ny, j, ny, p.v1j, p.v1m, 0.0);
+    scatter_row(A.V + r + ny, j, ny, p.v0j, p.v0m, 0.0);
+    acc(&A.U[r + j], p.ul);
+    if (i < nx - 2) acc(&A.U[r + 2L * ny + j], p.uu);
+    acc(&A.Vc[r + j], p.vc);
+    acc(&A.Vc[r + ny + j], p.vc);
+    acc(&A.U[r + ny + j], p.u1);
+    if (i > 0) acc(&A.D[r - ny + j], p.dm);
+    acc(&A.D[r + j], p.d0);
+    acc(&A.D[r + ny + j], p.d1);
+    if (i < nx - 2) acc(&A.D[r + 2L * ny + j], p.d2);
+}
+
+// The Qx predictor's reverse over rows k0..k1-1 of column j (one lane per column, a
+// wavefront 64 consecutive columns). Element i sends parts to rows i-1..i+2; the row no
+// later element of the run reaches is added once, with every part the run has for it, so
+// a row takes one atomic per field instead of up to four (rows the run shares with the
+// runs above and below it take one from each). Every lane of the wavefront that has a
+// column must call it with the same rows.
+__host__ __device__ __forceinline__ void adj_qxx_rows(const Consts &k, const Metrics &m, const State &S0,
+                                                      const RunFields &rf, const StepTape &T, const AdjFields &A,
+                                                      const AdjWork &W, const AdjRun &R, double f0, int k0, int k1,
+                                                      int j)
+{
+    const int ny = k.ny, nx = k.nx;
+    auto at = [&](int row) { return (long)row * ny + j; };
+    // pending parts by row: dQy, Zs, Vc rows i, i+1; U rows i..i+2; D rows i-1..i+2;
+    // V rows i+1, i+2 (own column and the one to the left)
+    double dq0 = 0, dq1 = 0, dz0 = 0, dz1 = 0, vc0 = 0, vc1 = 0, u0 = 0, u1 = 0, u2 = 0;
+    double dm = 0, d0 = 0, d1 = 0, d2 = 0, v0 = 0, v0l = 0, v1 = 0, v1l = 0;
+    for (int i = k0; i < k1; ++i) {
+        const QxxParts p = adj_qxx_parts(k, m, S0, rf, T, W, R, f0, i, j);
+        acc(&A.Qx[at(i + 1)], p.qx);
+        dq0 -= p.dq; dq1 += p.dq;
+        dz0 -= p.dz; dz1 += p.dz;
+        vc0 += p.vc; vc1 += p.vc;
+        u0 += p.ul; u1 += p.u1; u2 += p.uu;
+        dm += p.dm; d0 += p.d0; d1 += p.d1; d2 += p.d2;
+        v0 += p.v0j; v0l += p.v0m; v1 += p.v1j; v1l += p.v1m;
+        // rows element i+1 no longer reaches: i for dQy, Zs, Vc and U; i-1 for D; i+1 for V
+        acc(&A.dQy[at(i)], dq0);
+        acc(&A.Zs[at(i)], dz0);
+        acc(&A.Vc[at(i)], vc0);
+        acc(&A.U[at(i)], u0);
+        if (i > 0) acc(&A.D[at(i - 1)], dm);
+        scatter_row(A.V + at(i + 1) - j, j, ny, v0, v0l, 0.0);
+        dq0 = dq1; dq1 = 0; dz0 = dz1; dz1 = 0; vc0 = vc1; vc1 = 0;
+        u0 = u1; u1 = u2; u2 = 0;
+        dm = d0; d0 = d1; d1 = d2; d2 = 0;
+        v0 = v1; v0l = v1l; v1 = 0; v1l = 0;
+    }
+    // the rows past the run: k1 for dQy, Zs, Vc; k1, k1+1 for U; k1-1..k1+1 for D; k1+1 for V
+    acc(&A.dQy[at(k1)], dq0);
+    acc(&A.Zs[at(k1)], dz0);
+    acc(&A.Vc[at(k1)], vc0);
+    acc(&A.U[at(k1)], u0);
+    if (k1 + 1 <= nx) acc(&A.U[at(k1 + 1)], u1);
+    if (k1 > 0) acc(&A.D[at(k1 - 1)], dm);
+    if (k1 < nx) acc(&A.D[at(k1)], d0);
+    if (k1 + 1 < nx) acc(&A.D[at(k1 + 1)], d1);
+    scatter_row(A.V + at(k1 + 1) - j, j, ny, v0, v0l, 0.0);
 }
 
 // ----------------------------------------------------------------------------------
