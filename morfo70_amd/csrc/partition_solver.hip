@@ -228,6 +228,107 @@ struct ProvQxx {      // Qx predictor, lines along x (columns): line = j, k = i
                                             double (&up)[CM], double (&rh)[CM]) const
     { qxx_segment<CDC, VCF, STR>(k, m, s, rf, w, f0, line, k0, mm, lo, di, up, rh); }
 };
+// Column segment i = k0..k0+mm-1 of the Qy projector at column j (asm_qyx's arithmetic,
+// src/HydroQy.cpp:235-362), with the rows the elements share held in a register window:
+// element i reads guess_D rows i-1..i+1 at columns j and j+1, U rows i and i+1 and IsWater
+// rows i-1..i+1, so each element loads only row i+1's (9 loads instead of 17), and the
+// reciprocal of guess_D's face value of a row serves the three elements that use it. The
+// same expressions on the same values: the same bits as asm_qyx.
+template <bool STR>
+__device__ __forceinline__ void qyx_segment(const Consts &k, const Metrics &mt, const State &s, const RunFields &rf,
+                                            const Work &w, int j, int k0, int mm, double (&lo)[CM], double (&di)[CM],
+                                            double (&up)[CM], double (&rh)[CM])
+{
+    constexpr bool NTL = Nt<NT_QYX, STR>::L;
+    constexpr bool NTF = NTL || Nt<NT_QYX_STREAM, STR>::L;   // single-use fields
+    const int ny = k.ny, nx = k.nx;
+    const int jp = wrap_inc(j, ny);
+    const double dt = k.dt;
+    const double *gD = w.gD, *U = s.U;
+    auto ld = [&](const double *f, long e) { return ldn<NTL>(f + e); };
+    auto ldf = [&](const double *f, long e) { return ldn<NTF>(f + e); };
+    auto inv_face = [&](int i) {   // inv_Di row i+1 (getD_y(i)) at column j
+        const long r = (long)i * ny;
+        return 1 / (0.5 * (ld(gD, r + jp) + ld(gD, r + j)));
+    };
+    // window: rows i-1 (m), i (0), i+1 (1)
+    double invm = 0.0, inv0 = inv_face(k0), inv1 = inv_face(k0 + 1);
+    int wm = k0 > 0 ? (int)ldn<NTL>(w.w + (long)(k0 - 1) * ny + j) : 0;
+    int w0 = ldn<NTL>(w.w + (long)k0 * ny + j), w1 = ldn<NTL>(w.w + (long)(k0 + 1) * ny + j);
+    if (k0 > 0) invm = inv_face(k0 - 1);
+    const long r00 = (long)k0 * ny;
+    double u0j = ld(U, r00 + j), u0p = ld(U, r00 + jp), u1j = ld(U, r00 + ny + j), u1p = ld(U, r00 + ny + jp);
+#pragma unroll
+    for (int t = 0; t < CM; ++t) {
+        if (t < mm) {
+            const int i = k0 + t;
+            const long r = (long)i * ny;
+            if (t > 0) {   // rows i-1..i+1 from the window, row i+1 loaded
+                invm = inv0; inv0 = inv1; inv1 = inv_face(i + 1);
+                wm = w0; w0 = w1; w1 = ldn<NTL>(w.w + r + ny + j);
+                u0j = u1j; u0p = u1p; u1j = ld(U, r + ny + j); u1p = ld(U, r + ny + jp);
+            }
+            const int wi = w0;
+            const double drag = ldf(w.drag, r + j);
+            const double nud = ldf(rf.nuD_y, r + j), dnud_dx = ldf(rf.dnuDy_dx, r + j);
+            // setSolverQyx_f (:235-248)
+            const double fv = wi * ldf(s.Qy, r + ny + j) * (1 + dt * drag);
+            // setSolverQyx_d (:250-285)
+            const double x2c_d = mt.x2c_d[i], x2c_up = mt.x2c_up[i];
+            double dv;
+            {
+                const double dx_d = mt.difx_c_d[i], d2x_d = mt.dif2x_c_d[i];
+                const double inv_dx = mt.inv_dxx[i];
+                const double inv_d = inv0;
+                const int isl = !wi;
+                const double du_dx = 0.5 * inv_dx * ((u1p + u1j) - (u0p + u0j));
+                const double u_y = 0.5 * (x2c_up * (u1p + u1j) + x2c_d * (u0p + u0j));
+                dv = wi * ((1 + dt * drag) + dt * (u_y * dx_d + du_dx) -
+                           dt * (inv_d * nud * d2x_d + inv_d * dnud_dx * dx_d)) + isl;
+            }
+            // Ui_j row i (C2Y of U_c), U_c formed from U rows i, i+1 as the Qx tail forms it
+            const double uc_j = x2c_d * u0j + x2c_up * u1j;
+            const double uc_p = x2c_d * u0p + x2c_up * u1p;
+            const double uy = 0.5 * (uc_p + uc_j);
+            // setSolverQyx_up (:287-311); inv_Di row i+2 = getD_y(i+1); IsWater row nx-1 is false
+            double uv;
+            {
+                const double inv_up = inv1;
+                const int wu = (i + 1 <= nx - 2) ? w1 : 0;
+                const double ww = wi && wu;
+                const double dx_up = mt.difx_c_up[i], d2x_up = mt.dif2x_c_up[i];
+                uv = ww * dt * (uy * dx_up - nud * d2x_up * inv_up - dnud_dx * dx_up * inv_up);
+            }
+            // setSolverQyx0_low (:340-362) / setSolverQyx_low (:313-338)
+            double lv;
+            {
+                const double dx_low = mt.difx_c_low[i], d2x_low = mt.dif2x_c_low[i];
+                if (i == 0) {
+                    // getD_y0 (:426-440); the wrap element reads d[0] where the interior
+                    // reads row 1 (:438)
+                    double Dy0;
+                    if (j < ny - 1)
+                        Dy0 = 1 * (gD[j + 1] + gD[j]) - 0.5 * (gD[ny + j + 1] + gD[ny + j]);
+                    else
+                        Dy0 = 1 * (gD[0] + gD[ny - 1]) - 0.5 * (gD[0] + gD[2 * ny - 1]);
+                    const double inv_low = 1 / Dy0;
+                    lv = wi * dt * (uy * dx_low - nud * d2x_low * inv_low - dnud_dx * dx_low * inv_low);
+                } else {
+                    const double inv_low = invm;
+                    const double ww = wi && wm;
+                    lv = ww * dt * (uy * dx_low - nud * d2x_low * inv_low - dnud_dx * dx_low * inv_low);
+                }
+            }
+            if (i == 0) dv += k.bcQy * lv;
+            if (i == nx - 2) dv += uv * k.inv_3;
+            di[t] = dv;
+            lo[t] = lv;
+            up[t] = uv;
+            rh[t] = fv;
+        }
+    }
+}
+
 template <bool STR>
 struct ProvQyx {      // Qy projector, columns
     static constexpr int nt_site = NT_QYX;
@@ -235,6 +336,9 @@ struct ProvQyx {      // Qy projector, columns
     Consts k; Metrics m; State s; RunFields rf; Work w;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qyx<STR>(k, m, s, rf, w, kk, line, a, b, c, f); }
+    __device__ __forceinline__ void segment(int line, int k0, int mm, double (&lo)[CM], double (&di)[CM],
+                                            double (&up)[CM], double (&rh)[CM]) const
+    { qyx_segment<STR>(k, m, s, rf, w, line, k0, mm, lo, di, up, rh); }
 };
 // nuD_y and dnuD_dy of elements e, e+1 of row i formed from nuD row i by run_fields_body's
 // expressions, in place of qyy_load2's loads of the two arrays (which become dead).
@@ -1032,6 +1136,9 @@ struct GProvQyx {
     const ScenDesc *d;
     __device__ __forceinline__ void get(int line, int kk, double &a, double &b, double &c, double &f) const
     { asm_qyx<GROUPED_STREAM>(d->k, d->m, d->s, d->rf, d->w, kk, line, a, b, c, f); }
+    __device__ __forceinline__ void segment(int line, int k0, int mm, double (&lo)[CM], double (&di)[CM],
+                                            double (&up)[CM], double (&rh)[CM]) const
+    { qyx_segment<GROUPED_STREAM>(d->k, d->m, d->s, d->rf, d->w, line, k0, mm, lo, di, up, rh); }
 };
 struct GProvQyy {
     static constexpr int nt_site = NT_QYY_ROWS;
