@@ -85,6 +85,6 @@ hipError_t launch_rows_grouped(hipStream_t st, int which, int rb, const ScenDesc
 hipError_t launch_cols3_grouped(hipStream_t st, int which, const ScenDesc *D, const int *start_cols,
                                 const int *start_red, const int *start_redp, int nscen, int blocks_cols, int blocks_red,
                                 int blocks_redp, bool backsub = true, int rev_elim = 0, int rev_backsub = 0,
-                                bool vc_from_v = false);
+                                bool vc_from_v = false, bool const_drag = false);
 
 }  // namespace m70

@@ -929,6 +929,7 @@ struct m70h_batch {
     int cell_blocks = 0;
     bool use_graph = true;
     int rows_block = 0;   // row-solver block size for every scenario of the batch
+    bool const_drag = true;   // every scenario has the constant drag model
     hipGraphExec_t gexec = nullptr;
     std::string err;
 };
@@ -992,7 +993,7 @@ static hipError_t batch_step(m70h_batch *b, int odd, bool first = true, bool las
     const int ra = odd, rb = 1 - odd;
     // the Qx projector rows form the Qx predictor's solution from its first two passes
     if ((e = launch_cols3_grouped(st, 0, D, st_cols, st_red, st_redp, n, b->blocks[GS_COLS], b->blocks[GS_RED],
-                                  b->blocks[GS_REDP], false, ra, rb, !first)) != hipSuccess)
+                                  b->blocks[GS_REDP], false, ra, rb, !first, b->const_drag)) != hipSuccess)
         return e;
     if ((e = launch_rows_grouped(st, 0, b->rows_block, D, st_rows, n, b->blocks[GS_ROWS], rb, !first)) != hipSuccess)
         return e;
@@ -1036,6 +1037,7 @@ int m70h_batch_create(int nscen, m70h *const *handles, m70h_batch **out)
         for (int gs = 0; gs < GS_COUNT; ++gs)
             starts[(size_t)gs * (nscen + 1) + q + 1] = starts[(size_t)gs * (nscen + 1) + q] + per[gs];
         b->hs.push_back(h);
+        if (h->p.drag_model != 0) b->const_drag = false;
     }
     for (int gs = 0; gs < GS_COUNT; ++gs) b->blocks[gs] = starts[(size_t)gs * (nscen + 1) + nscen];
     hipError_t e;
