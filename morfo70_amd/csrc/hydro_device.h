@@ -198,10 +198,12 @@ template <int SITE, bool STR> struct Nt {
 // the kernel after it, which non-temporal stores would forgo. Measured step times, plain
 // against streaming policy (ms): 768 x 1024 0.139 / 0.152, 1024 x 1024 0.160 / 0.171,
 // 1024 x 2048 0.276 / 0.275, 2048 x 2048 0.517 / 0.502, 2048 x 4096 0.955 / 0.911.
-// Grouped launches over many scenarios keep plain accesses: with the streaming policy the
-// 64-scenario batch stepped 1.3% slower (6.72e9 -> 6.63e9 cell-steps/s).
+// Grouped launches over many scenarios take the streaming policy, together with 1024-thread
+// workgroups for their reduced column systems (grouped_stage_blocks): 64 uneven beaches
+// 7.39e9 -> 7.42e9 cell-steps/s, one 2048 x 4096 beach 1.145 -> 1.036 ms per step. With
+// 256-thread reduced-system workgroups the streaming policy made the 64 beaches 1.5% slower.
 constexpr long STREAM_MIN_CELLS = 2L << 20;
-constexpr bool GROUPED_STREAM = false;
+constexpr bool GROUPED_STREAM = true;
 __host__ __device__ __forceinline__ bool stream_policy(int nx, int ny) { return (long)nx * ny >= STREAM_MIN_CELLS; }
 
 typedef double m70_v2d __attribute__((ext_vector_type(2)));

@@ -1352,8 +1352,11 @@ void grouped_stage_blocks(int nx, int ny, int rb, int *rows, int *cols, int *red
     const int nseg = cols_segments(nx - 1);
     *cols = ((ny + 63) / 64) * ((nseg + 3) / 4);
     *S_red = pick_segments(2 * nseg, 8, 256);
-    // 256-thread workgroups when the segment count allows it (2*nseg <= 512), else 1024
-    const int B = (*S_red > 0 && pick_segments(2 * nseg, 8, 64) == *S_red) ? 256 : 1024;
+    // 1024-thread workgroups of 1024/S columns (rows read in runs of 8*1024/S bytes), or
+    // 256 threads when the scenario has fewer columns than such a workgroup holds (and at
+    // most 64 segments). For a 2048 x 4096 beach 1024 threads take 25 us where 256 (four
+    // columns per workgroup) took 46 us.
+    const int B = (*S_red > 0 && *S_red <= 64 && ny < 1024 / *S_red) ? 256 : 1024;
     *L_red = *S_red > 0 ? B / *S_red : 0;
     if (B == 256) {
         *red = (ny + *L_red - 1) / *L_red;
