@@ -31,6 +31,8 @@ struct AdjScen {
     int S_rows;        // segments per row line (LDS row solver)
     int pad;
 };
+// The kernels read the table through the constant address space (see cScenDesc).
+typedef __attribute__((address_space(4))) const AdjScen cAdjScen;
 
 // Cell stages. Rows: COPY and ZERO nx+1 (the F+ extent), the two tails and the tails'
 // first reverses nx, the others nx-1.

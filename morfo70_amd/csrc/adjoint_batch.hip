@@ -11,12 +11,12 @@
 namespace m70 {
 
 template <int OP>
-__global__ void __launch_bounds__(256) k_adj_cells_grouped(const AdjScen *__restrict__ D, const int *__restrict__ start,
+__global__ void __launch_bounds__(256) k_adj_cells_grouped(cAdjScen *D, const int *__restrict__ start,
                                                            int nscen)
 {
     const int b = blockIdx.x;
     const int sid = scen_of(start, nscen, b);
-    const AdjScen &d = D[sid];
+    const AdjScen &d = *(const AdjScen *)(D + sid);
     const Consts &k = d.k;
     const int ny = k.ny;
     const int gx = (ny + 255) / 256;
@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(256) k_adj_cells_grouped(const AdjScen *__rest
 template <int OP>
 static void launch_one(hipStream_t st, const AdjScen *D, const int *start, int nscen, int blocks)
 {
-    hipLaunchKernelGGL(k_adj_cells_grouped<OP>, dim3((unsigned)blocks), dim3(256), 0, st, D, start, nscen);
+    hipLaunchKernelGGL(k_adj_cells_grouped<OP>, dim3((unsigned)blocks), dim3(256), 0, st, (cAdjScen *)D, start, nscen);
 }
 
 hipError_t launch_adj_cells_grouped(hipStream_t st, int op, const AdjScen *D, const int *start, int nscen, int blocks)

@@ -112,6 +112,13 @@ struct ScenDesc {
     int pad;
 };
 
+// Grouped kernels take their ScenDesc table through the constant address space: pointers
+// loaded from it then count as global ones, so loads through them are global loads and the
+// per-row metrics, uniform in a wavefront, scalar loads. Through a generic table pointer
+// they were flat vector loads (194 of them per wavefront in the Qx predictor, against 121
+// global loads in the single-scenario kernel).
+typedef __attribute__((address_space(4))) const ScenDesc cScenDesc;
+
 // Offshore offset of the scenario's current step: series[min(base + step, n - 1)], the
 // value m70h_run would use at the same step.
 __device__ __forceinline__ double scen_f0(const ScenDesc &d)

@@ -460,12 +460,12 @@ __global__ void __launch_bounds__(256) k_init_zs(double *Zs, const double *D, co
 // scenario scen_of(start, nscen, b); within it blocks tile (ceil(ny/256) x nx).
 // ------------------------------------------------------------------------------------
 template <int OP>
-__global__ void __launch_bounds__(256) k_cells_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
+__global__ void __launch_bounds__(256) k_cells_grouped(cScenDesc *D, const int *__restrict__ start,
                                                        int nscen)
 {
     const int b = (int)blockIdx.x;
     const int sid = scen_of(start, nscen, b);
-    const ScenDesc &d = D[sid];
+    const ScenDesc &d = *(const ScenDesc *)(D + sid);
     const int ny = d.k.ny;
     const int lb = b - start[sid];
     const int gx = (ny + 255) / 256;
@@ -480,8 +480,8 @@ int grouped_cell_blocks(int nx, int ny) { return ((ny + 255) / 256) * nx; }
 
 hipError_t launch_cells_grouped(hipStream_t st, int op, const ScenDesc *D, const int *start, int nscen, int blocks)
 {
-    if (op == 0) hipLaunchKernelGGL(k_cells_grouped<0>, dim3(blocks), dim3(256), 0, st, D, start, nscen);
-    else hipLaunchKernelGGL(k_cells_grouped<1>, dim3(blocks), dim3(256), 0, st, D, start, nscen);
+    if (op == 0) hipLaunchKernelGGL(k_cells_grouped<0>, dim3(blocks), dim3(256), 0, st, (cScenDesc *)D, start, nscen);
+    else hipLaunchKernelGGL(k_cells_grouped<1>, dim3(blocks), dim3(256), 0, st, (cScenDesc *)D, start, nscen);
     return hipGetLastError();
 }
 
@@ -501,16 +501,16 @@ static long grouped_tail_items(int nx, int ny, int op)
 int grouped_tail_blocks(int nx, int ny, int op) { return (int)((grouped_tail_items(nx, ny, op) + 255) / 256); }
 
 template <int OP, bool WD>
-__global__ void __launch_bounds__(256) k_tails_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
+__global__ void __launch_bounds__(256) k_tails_grouped(cScenDesc *D, const int *__restrict__ start,
                                                        int nscen, int rev)
 {
     // rev: the last scenario's last rows first
     const int b = rev ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x;
     // the Qy tail ends a step: count it for the offshore offset series (scen_f0); every
     // reader of the counter is in a later launch
-    if (OP == 3 && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((int *)D[0].step, 1);
+    if (OP == 3 && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd((int *)((const ScenDesc *)D)->step, 1);
     const int sid = scen_of(start, nscen, b);
-    const ScenDesc &d = D[sid];
+    const ScenDesc &d = *(const ScenDesc *)(D + sid);
     const int nx = d.k.nx, ny = d.k.ny;
     const long q = (long)(b - start[sid]) * 256 + threadIdx.x;
     if (ny % 2 == 0 && ny >= 4) {
@@ -539,11 +539,11 @@ hipError_t launch_tails_grouped(hipStream_t st, int op, const ScenDesc *D, const
 {
     const dim3 g((unsigned)blocks), b(256);
     if (op == 2) {
-        if (write_derived) hipLaunchKernelGGL((k_tails_grouped<2, true>), g, b, 0, st, D, start, nscen, rev);
-        else hipLaunchKernelGGL((k_tails_grouped<2, false>), g, b, 0, st, D, start, nscen, rev);
+        if (write_derived) hipLaunchKernelGGL((k_tails_grouped<2, true>), g, b, 0, st, (cScenDesc *)D, start, nscen, rev);
+        else hipLaunchKernelGGL((k_tails_grouped<2, false>), g, b, 0, st, (cScenDesc *)D, start, nscen, rev);
     } else {
-        if (write_derived) hipLaunchKernelGGL((k_tails_grouped<3, true>), g, b, 0, st, D, start, nscen, rev);
-        else hipLaunchKernelGGL((k_tails_grouped<3, false>), g, b, 0, st, D, start, nscen, rev);
+        if (write_derived) hipLaunchKernelGGL((k_tails_grouped<3, true>), g, b, 0, st, (cScenDesc *)D, start, nscen, rev);
+        else hipLaunchKernelGGL((k_tails_grouped<3, false>), g, b, 0, st, (cScenDesc *)D, start, nscen, rev);
     }
     return hipGetLastError();
 }

@@ -1190,12 +1190,12 @@ struct GProvQxyBs {
 // WHICH 0: Qx projector rows, 1: Qy predictor rows; VCF: the Qx projector forms V_c from V
 template <int WHICH, int RB, bool VCF>
 __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4, 4)))
-k_rows_lds_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start, int nscen, int rev)
+k_rows_lds_grouped(cScenDesc *D, const int *__restrict__ start, int nscen, int rev)
 {
     __shared__ double sh[2][RowsLds<RB>::LDS];
     const int b = rev ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x;   // rev: last scenario's last row first
     const int sid = scen_of(start, nscen, b);
-    const ScenDesc *d = D + sid;
+    const ScenDesc *d = (const ScenDesc *)(D + sid);
     const int ny = d->k.ny;
     const int line = b - start[sid];
     if (WHICH == 0) rows_lds_body<RB>(GProvQxyBs<VCF>{d}, d->s.Qx + ny, ny, ny, d->S_rows, 1, line, sh);
@@ -1207,12 +1207,12 @@ k_rows_lds_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start
 // here (two waves per SIMD); held to three waves per SIMD it spills about 60 bytes per lane
 // and the 64-scenario batch still steps 0.8% faster (7.34e9 -> 7.40e9 cell-steps/s).
 template <int WHICH, bool VCF, bool CDC>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_cols_elim_grouped(const ScenDesc *__restrict__ D,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_cols_elim_grouped(cScenDesc *D,
                                                            const int *__restrict__ start, int nscen, int rev)
 {
     const int b = rev ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x;
     const int sid = scen_of(start, nscen, b);
-    const ScenDesc *d = D + sid;
+    const ScenDesc *d = (const ScenDesc *)(D + sid);
     const int ny = d->k.ny;
     const int lb = b - start[sid];
     const int gx = (ny + 63) / 64;
@@ -1222,12 +1222,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 }
 
 template <int WHICH>
-__global__ void __launch_bounds__(256) k_cols_backsub_grouped(const ScenDesc *__restrict__ D,
+__global__ void __launch_bounds__(256) k_cols_backsub_grouped(cScenDesc *D,
                                                               const int *__restrict__ start, int nscen, int rev)
 {
     const int b = rev ? (int)(gridDim.x - 1 - blockIdx.x) : (int)blockIdx.x;
     const int sid = scen_of(start, nscen, b);
-    const ScenDesc *d = D + sid;
+    const ScenDesc *d = (const ScenDesc *)(D + sid);
     const int ny = d->k.ny;
     const int lb = b - start[sid];
     const int gx = (ny + 63) / 64;
@@ -1239,13 +1239,13 @@ __global__ void __launch_bounds__(256) k_cols_backsub_grouped(const ScenDesc *__
 // Reduced systems of the FUSED x-sweeps for a batch: the partition solver with the
 // scenario's segment count, RED_BLOCK-thread workgroups of RED_BLOCK/S lines.
 template <int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_red_pcr_grouped(const ScenDesc *__restrict__ D, const int *__restrict__ start,
+__global__ void __launch_bounds__(BLOCK) k_red_pcr_grouped(cScenDesc *D, const int *__restrict__ start,
                                                            int nscen)
 {
     __shared__ double red[2][4][BLOCK * 2];
     const int b = blockIdx.x;
     const int sid = scen_of(start, nscen, b);
-    const ScenDesc *d = D + sid;
+    const ScenDesc *d = (const ScenDesc *)(D + sid);
     const int ny = d->k.ny;
     LineGeom g{};
     g.ld = ny; g.ldy = ny; g.n = 2 * d->cw.nseg; g.nL = ny; g.S = d->S_red; g.lines_per_block = d->L_red;
@@ -1259,13 +1259,13 @@ __global__ void __launch_bounds__(BLOCK) k_red_pcr_grouped(const ScenDesc *__res
 // transposes, every scenario with its own segments, as its single-scenario launch.
 // ---------------------------------------------------------------------------------
 template <int SYS, int TRANS>
-__global__ void __launch_bounds__(1024) k_adj_cols_grouped(const AdjScen *__restrict__ D, const int *__restrict__ start,
+__global__ void __launch_bounds__(1024) k_adj_cols_grouped(cAdjScen *D, const int *__restrict__ start,
                                                            int nscen)
 {
     __shared__ double red[2][4][1024 * 2];
     const int b = blockIdx.x;
     const int sid = scen_of(start, nscen, b);
-    const AdjScen *d = D + sid;
+    const AdjScen *d = (const AdjScen *)(D + sid);
     const int ny = d->k.ny;
     LineGeom g{};
     g.ld = ny; g.ldy = ny; g.n = d->k.nx - 1; g.nL = ny; g.S = d->S_cols; g.lines_per_block = d->L_cols;
@@ -1282,12 +1282,12 @@ __global__ void __launch_bounds__(1024) k_adj_cols_grouped(const AdjScen *__rest
 
 template <int RB, int SYS, int TRANS>
 __global__ void __launch_bounds__(RB) __attribute__((amdgpu_waves_per_eu(4, 4)))
-k_adj_rows_grouped(const AdjScen *__restrict__ D, const int *__restrict__ start, int nscen)
+k_adj_rows_grouped(cAdjScen *D, const int *__restrict__ start, int nscen)
 {
     __shared__ double sh[2][RowsLds<RB>::LDS];
     const int b = blockIdx.x;
     const int sid = scen_of(start, nscen, b);
-    const AdjScen *d = D + sid;
+    const AdjScen *d = (const AdjScen *)(D + sid);
     const int ny = d->k.ny;
     const int line = b - start[sid];
     const double *A = SYS == 2 ? d->T.a2 : d->T.a3, *B = SYS == 2 ? d->T.b2 : d->T.b3;
@@ -1305,10 +1305,10 @@ hipError_t launch_adj_cols_grouped(hipStream_t st, int sys, int trans, const Adj
 {
     if (blocks <= 0) return hipSuccess;
     const dim3 gr((unsigned)blocks), bl(1024);
-    if (sys == 1 && !trans) hipLaunchKernelGGL((k_adj_cols_grouped<1, 0>), gr, bl, 0, st, D, start, nscen);
-    else if (sys == 1) hipLaunchKernelGGL((k_adj_cols_grouped<1, 1>), gr, bl, 0, st, D, start, nscen);
-    else if (sys == 4 && !trans) hipLaunchKernelGGL((k_adj_cols_grouped<4, 0>), gr, bl, 0, st, D, start, nscen);
-    else if (sys == 4) hipLaunchKernelGGL((k_adj_cols_grouped<4, 1>), gr, bl, 0, st, D, start, nscen);
+    if (sys == 1 && !trans) hipLaunchKernelGGL((k_adj_cols_grouped<1, 0>), gr, bl, 0, st, (cAdjScen *)D, start, nscen);
+    else if (sys == 1) hipLaunchKernelGGL((k_adj_cols_grouped<1, 1>), gr, bl, 0, st, (cAdjScen *)D, start, nscen);
+    else if (sys == 4 && !trans) hipLaunchKernelGGL((k_adj_cols_grouped<4, 0>), gr, bl, 0, st, (cAdjScen *)D, start, nscen);
+    else if (sys == 4) hipLaunchKernelGGL((k_adj_cols_grouped<4, 1>), gr, bl, 0, st, (cAdjScen *)D, start, nscen);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
@@ -1321,16 +1321,16 @@ hipError_t launch_adj_rows_grouped(hipStream_t st, int rb, int sys, int trans, c
     if (sys != 2 && sys != 3) return hipErrorInvalidValue;
     if (rb == 256) {
         const dim3 bl(256);
-        if (sys == 2 && !trans) hipLaunchKernelGGL((k_adj_rows_grouped<256, 2, 0>), gr, bl, 0, st, D, start, nscen);
-        else if (sys == 2) hipLaunchKernelGGL((k_adj_rows_grouped<256, 2, 1>), gr, bl, 0, st, D, start, nscen);
-        else if (!trans) hipLaunchKernelGGL((k_adj_rows_grouped<256, 3, 0>), gr, bl, 0, st, D, start, nscen);
-        else hipLaunchKernelGGL((k_adj_rows_grouped<256, 3, 1>), gr, bl, 0, st, D, start, nscen);
+        if (sys == 2 && !trans) hipLaunchKernelGGL((k_adj_rows_grouped<256, 2, 0>), gr, bl, 0, st, (cAdjScen *)D, start, nscen);
+        else if (sys == 2) hipLaunchKernelGGL((k_adj_rows_grouped<256, 2, 1>), gr, bl, 0, st, (cAdjScen *)D, start, nscen);
+        else if (!trans) hipLaunchKernelGGL((k_adj_rows_grouped<256, 3, 0>), gr, bl, 0, st, (cAdjScen *)D, start, nscen);
+        else hipLaunchKernelGGL((k_adj_rows_grouped<256, 3, 1>), gr, bl, 0, st, (cAdjScen *)D, start, nscen);
     } else if (rb == 512) {
         const dim3 bl(512);
-        if (sys == 2 && !trans) hipLaunchKernelGGL((k_adj_rows_grouped<512, 2, 0>), gr, bl, 0, st, D, start, nscen);
-        else if (sys == 2) hipLaunchKernelGGL((k_adj_rows_grouped<512, 2, 1>), gr, bl, 0, st, D, start, nscen);
-        else if (!trans) hipLaunchKernelGGL((k_adj_rows_grouped<512, 3, 0>), gr, bl, 0, st, D, start, nscen);
-        else hipLaunchKernelGGL((k_adj_rows_grouped<512, 3, 1>), gr, bl, 0, st, D, start, nscen);
+        if (sys == 2 && !trans) hipLaunchKernelGGL((k_adj_rows_grouped<512, 2, 0>), gr, bl, 0, st, (cAdjScen *)D, start, nscen);
+        else if (sys == 2) hipLaunchKernelGGL((k_adj_rows_grouped<512, 2, 1>), gr, bl, 0, st, (cAdjScen *)D, start, nscen);
+        else if (!trans) hipLaunchKernelGGL((k_adj_rows_grouped<512, 3, 0>), gr, bl, 0, st, (cAdjScen *)D, start, nscen);
+        else hipLaunchKernelGGL((k_adj_rows_grouped<512, 3, 1>), gr, bl, 0, st, (cAdjScen *)D, start, nscen);
     } else {
         return hipErrorInvalidValue;
     }
@@ -1371,16 +1371,17 @@ hipError_t launch_rows_grouped(hipStream_t st, int which, int rb, const ScenDesc
                                int blocks, int rev, bool vc_from_v)
 {
     const dim3 g((unsigned)blocks);
+    cScenDesc *Dc = (cScenDesc *)D;
     if (rb == 256) {
-        if (which == 1) hipLaunchKernelGGL((k_rows_lds_grouped<1, 256, false>), g, dim3(256), 0, st, D, start, nscen, rev);
+        if (which == 1) hipLaunchKernelGGL((k_rows_lds_grouped<1, 256, false>), g, dim3(256), 0, st, Dc, start, nscen, rev);
         else if (vc_from_v)
-            hipLaunchKernelGGL((k_rows_lds_grouped<0, 256, true>), g, dim3(256), 0, st, D, start, nscen, rev);
-        else hipLaunchKernelGGL((k_rows_lds_grouped<0, 256, false>), g, dim3(256), 0, st, D, start, nscen, rev);
+            hipLaunchKernelGGL((k_rows_lds_grouped<0, 256, true>), g, dim3(256), 0, st, Dc, start, nscen, rev);
+        else hipLaunchKernelGGL((k_rows_lds_grouped<0, 256, false>), g, dim3(256), 0, st, Dc, start, nscen, rev);
     } else {
-        if (which == 1) hipLaunchKernelGGL((k_rows_lds_grouped<1, 512, false>), g, dim3(512), 0, st, D, start, nscen, rev);
+        if (which == 1) hipLaunchKernelGGL((k_rows_lds_grouped<1, 512, false>), g, dim3(512), 0, st, Dc, start, nscen, rev);
         else if (vc_from_v)
-            hipLaunchKernelGGL((k_rows_lds_grouped<0, 512, true>), g, dim3(512), 0, st, D, start, nscen, rev);
-        else hipLaunchKernelGGL((k_rows_lds_grouped<0, 512, false>), g, dim3(512), 0, st, D, start, nscen, rev);
+            hipLaunchKernelGGL((k_rows_lds_grouped<0, 512, true>), g, dim3(512), 0, st, Dc, start, nscen, rev);
+        else hipLaunchKernelGGL((k_rows_lds_grouped<0, 512, false>), g, dim3(512), 0, st, Dc, start, nscen, rev);
     }
     return hipGetLastError();
 }
@@ -1391,30 +1392,31 @@ hipError_t launch_cols3_grouped(hipStream_t st, int which, const ScenDesc *D, co
                                 bool const_drag)
 {
     const dim3 g((unsigned)blocks_cols), blk(64, 4);
+    cScenDesc *Dc = (cScenDesc *)D;
     auto qxx = [&](auto vcf, auto cdc) {
-        hipLaunchKernelGGL((k_cols_elim_grouped<0, decltype(vcf)::value, decltype(cdc)::value>), g, blk, 0, st, D,
+        hipLaunchKernelGGL((k_cols_elim_grouped<0, decltype(vcf)::value, decltype(cdc)::value>), g, blk, 0, st, Dc,
                            start_cols, nscen, rev_elim);
     };
     using T = std::true_type;
     using F = std::false_type;
-    if (which == 1) hipLaunchKernelGGL((k_cols_elim_grouped<1, false, false>), g, blk, 0, st, D, start_cols, nscen, rev_elim);
+    if (which == 1) hipLaunchKernelGGL((k_cols_elim_grouped<1, false, false>), g, blk, 0, st, Dc, start_cols, nscen, rev_elim);
     else if (vc_from_v) const_drag ? qxx(T{}, T{}) : qxx(T{}, F{});
     else const_drag ? qxx(F{}, T{}) : qxx(F{}, F{});
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (blocks_red > 0) {   // scenarios whose reduced solve fits 256-thread workgroups
-        hipLaunchKernelGGL(k_red_pcr_grouped<256>, dim3(blocks_red), dim3(256), 0, st, D, start_red, nscen);
+        hipLaunchKernelGGL(k_red_pcr_grouped<256>, dim3(blocks_red), dim3(256), 0, st, Dc, start_red, nscen);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (blocks_redp > 0) {  // the others, 1024-thread workgroups
-        hipLaunchKernelGGL(k_red_pcr_grouped<1024>, dim3(blocks_redp), dim3(1024), 0, st, D, start_redp, nscen);
+        hipLaunchKernelGGL(k_red_pcr_grouped<1024>, dim3(blocks_redp), dim3(1024), 0, st, Dc, start_redp, nscen);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (!backsub) return hipSuccess;
     if (which == 0)
-        hipLaunchKernelGGL(k_cols_backsub_grouped<0>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen,
+        hipLaunchKernelGGL(k_cols_backsub_grouped<0>, dim3(blocks_cols), dim3(64, 4), 0, st, Dc, start_cols, nscen,
                            rev_backsub);
-    else hipLaunchKernelGGL(k_cols_backsub_grouped<1>, dim3(blocks_cols), dim3(64, 4), 0, st, D, start_cols, nscen,
+    else hipLaunchKernelGGL(k_cols_backsub_grouped<1>, dim3(blocks_cols), dim3(64, 4), 0, st, Dc, start_cols, nscen,
                             rev_backsub);
     return hipGetLastError();
 }
