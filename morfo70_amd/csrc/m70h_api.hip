@@ -1776,8 +1776,10 @@ int ab_sync_forward_tables(m70h_adjoint_batch *b)
 // n untaped steps of every handle (the forward batch's grouped step).
 int ab_advance(m70h_adjoint_batch *b, int n)
 {
+    // one Run of n steps: its first step reads V_c as stored, the steps after it form V_c
+    // from V, and only its last stores U_c and V_c (the same bits as n one-step Runs)
     for (int t = 0; t < n; ++t) {
-        ABCHK(b, batch_step(b->fb, b->parity));
+        ABCHK(b, batch_step(b->fb, b->parity, t == 0, t == n - 1));
         ab_swap_depths(b);
         for (m70h *h : b->hs) h->adj->fwd_steps++;
     }
