@@ -84,8 +84,8 @@ SWEEP_BYTES_PER_CELL = {
 # (the bed update covers rows 1..nx-1): compulsory FP64 fields read once, written once.
 SED_BYTES_PER_CELL = {
     "sed_columns": 8 * 2,                   # Dw, D in (running max, shoreline scan; Zs of one row per column)
-    "sed_coeffs": 8 * 2 + 8 * 3,            # D, Zb in; Alpha, gamma, Gamma out
-    "sed_fluxes": 8 * 6 + 8 * 2,            # Alpha, gamma, Gamma, h, U, V in; qx, qy out
+    # coefficients and fluxes in one pass: D, Zb, h, U, V in; Alpha, gamma, Gamma, qx, qy out
+    "sed_fluxes": 8 * 5 + 8 * 5,
     "sed_update": 8 * 6 + 8 * 4,            # qx, qy, dh_dt, h, Zb, Zs in; h, Zb, Zs, dh_dt out
 }
 B_ALG_SED = float(sum(SED_BYTES_PER_CELL.values()))

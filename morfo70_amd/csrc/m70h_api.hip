@@ -32,13 +32,13 @@ enum KernelId {
     K_DRAG_VISC, K_RUN_FIELDS, K_QXX_ASM, K_THOMAS_QX, K_QXY_ASM, K_PERIODIC_QX, K_QX_TAIL,
     K_QYY_ASM, K_PERIODIC_QY, K_QYX_ASM, K_THOMAS_QY, K_QY_TAIL, K_FUSED_QXY, K_FUSED_QYY,
     K_COLS_QX_ELIM, K_COLS_QX_RED, K_COLS_QX_BACK, K_COLS_QY_ELIM, K_COLS_QY_RED, K_COLS_QY_BACK,
-    K_SED_COLUMNS, K_SED_COEFFS, K_SED_FLUXES, K_SED_UPDATE, K_COUNT
+    K_SED_COLUMNS, K_SED_FLUXES, K_SED_UPDATE, K_COUNT
 };
 const char *const kKernelNames[K_COUNT] = {
     "drag_viscosity", "run_fields", "qxx_assemble", "thomas_qx", "qxy_assemble", "periodic_qx", "qx_tail",
     "qyy_assemble", "periodic_qy", "qyx_assemble", "thomas_qy", "qy_tail", "fused_qxy", "fused_qyy",
     "cols_qx_elim", "cols_qx_reduced", "cols_qx_backsub", "cols_qy_elim", "cols_qy_reduced", "cols_qy_backsub",
-    "sed_columns", "sed_coeffs", "sed_fluxes", "sed_update"};
+    "sed_columns", "sed_fluxes", "sed_update"};
 
 }  // namespace
 
@@ -2232,8 +2232,7 @@ int m70s_calculate_fluxes_async(m70s *s)
     const SedArgs a = sed_args(s);
     int rc;
     if ((rc = timed(h, K_SED_COLUMNS, [&] { return launch_sed_columns(h->stream, a); })) ||
-        (rc = timed(h, K_SED_COEFFS, [&] { return launch_sed_coeffs(h->stream, a); })) ||
-        (rc = timed(h, K_SED_FLUXES, [&] { return launch_sed_qxqy(h->stream, a); })))
+        (rc = timed(h, K_SED_FLUXES, [&] { return launch_sed_coeffs_fluxes(h->stream, a); })))
         return sfail(s, rc, h->err);
     return M70H_OK;
 }

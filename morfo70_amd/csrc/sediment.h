@@ -27,12 +27,11 @@ constexpr int SED_MAX_CHUNKS = 128;
 // rows each); *rows_per_chunk receives the chunk height.
 int sed_row_chunks(int nx, int *rows_per_chunk);
 
-// Sediment::calculateSedimentFluxes (src/Sediment.cpp:47-65) is the three launches below,
-// in order: the per-column running max of Dw and shoreline scan, the per-cell
-// coefficients, the fluxes.
+// Sediment::calculateSedimentFluxes (src/Sediment.cpp:47-65) is the two launches below,
+// in order: the per-column running max of Dw and shoreline scan, then the per-cell
+// coefficients with the fluxes.
 hipError_t launch_sed_columns(hipStream_t st, const SedArgs &a);
-hipError_t launch_sed_coeffs(hipStream_t st, const SedArgs &a);
-hipError_t launch_sed_qxqy(hipStream_t st, const SedArgs &a);
+hipError_t launch_sed_coeffs_fluxes(hipStream_t st, const SedArgs &a);
 // Sediment::updateBottom (src/Sediment.cpp:67-75,105-151); dt = Parameters::Run::dt.
 hipError_t launch_sed_update(hipStream_t st, const SedArgs &a, double dt);
 // dh_dt from the fluxes in a.qx, a.qy (src/Sediment.cpp:77-103).

@@ -1,14 +1,15 @@
 // sediment.hip -- sediment transport and bed update (class Sediment, src/Sediment.cpp)
 // for gfx950.
 //
-// Four kernels per flux evaluation and one per bed update, all on the hydro handle's
+// Three kernels per flux evaluation and one per bed update, all on the hydro handle's
 // stream and reading its device state:
 //   k_sed_col_*    the running maximum of the breaking dissipation over x
 //                  (getMaxWaveBreakingDissipation, :252-262) and the shoreline surface
 //                  level scan (getShorelineZs, :264-283): per chunk of rows, then per
 //                  column;
-//   k_sed_coeffs   per cell: stirring, downslope and swash coefficients (:194-250);
-//   k_sed_fluxes   per cell: cross-shore (rows 0..nx-2) and alongshore fluxes (:156-192);
+//   k_sed_coeffs_fluxes  per cell, by runs of rows: stirring, downslope and swash
+//                  coefficients (:194-250), then the cross-shore (rows 0..nx-2) and
+//                  alongshore fluxes (:156-192);
 //   k_sed_update   per cell, rows 1..nx-1: Adams-Bashforth-2 bed update, moving the
 //                  hydro's Zs with the bed (:105-151).
 // Arithmetic keeps the reference's operation order (built with -ffp-contract=off), except
@@ -92,21 +93,20 @@ __global__ void __launch_bounds__(256) k_sed_col_final(SedArgs a, int chunks)
 // in double; 0 and 2 at the saturated ends, as 1 + tanh gives there.
 __device__ __forceinline__ double one_plus_tanh(double z) { return 2.0 / (1.0 + exp(-2.0 * z)); }
 
-__global__ void __launch_bounds__(256) k_sed_coeffs(SedArgs a)
+// The three coefficients of cell (i, j) (src/Sediment.cpp:194-250): stirring Alpha,
+// downslope gamma, swash Gamma, from its depth and bed level and the column's shoreline
+// level and running maximum of Dw.
+struct SedCoef { double alpha, gamma, Gamma; };
+__device__ __forceinline__ SedCoef sed_coef(const SedArgs &a, double d, double zb, double zs_sh, double max_dw)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
-    if (j >= a.ny || i >= a.nx) return;
-    const long q = (long)i * a.ny + j;
-    // every input is loaded before the first store, which would otherwise hold later loads
-    const double d = a.D[q], zb = a.Zb[q], zs_sh = a.Zs_sh[j], max_dw = a.maxDw[j];
+    SedCoef c;
     {   // getWaveStirringCWS
         const double alpha0_min = a.p.alpha0_min;
         const double incrAlpha = a.p.alpha0_max - alpha0_min;
         const double Dmax = a.p.D_alpha0_max;
         const double Dmax_min = a.p.D_alpha0_min - Dmax;
         double x = 1 + 2 * (Dmax - d) / Dmax_min;
-        a.Alpha[q] = alpha0_min + incrAlpha * 0.5 * one_plus_tanh(PI_C * x);
+        c.alpha = alpha0_min + incrAlpha * 0.5 * one_plus_tanh(PI_C * x);
     }
     {   // getGammaDownslopeCWS (the EGU form)
         const double inv_tanPhi_i = 1 / TANPHI_I;
@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(256) k_sed_coeffs(SedArgs a)
         const double Dmin = a.Dmin;
         const double A_swash = a.p.A_swash;
         const double u_bsl = one_plus_tanh(PI_C * (2 * (d - Dmin) / (A_swash - Dmin) - 1)) / 2;
-        a.gamma[q] = u_bsl * gamma0_tanPhi;
+        c.gamma = u_bsl * gamma0_tanPhi;
     }
     {   // getGammaSwash
         const double G0 = a.p.Gamma0;
@@ -125,40 +125,79 @@ __global__ void __launch_bounds__(256) k_sed_coeffs(SedArgs a)
         const int isDry = (d <= Dmin);
         const double d_dry = (1 - isDry) * d + isDry * (fabs(zb + zs_sh));
         const double sf_swash = 1 - one_plus_tanh(PI_C * (_2_Doff_on * (d_dry - Dmin) - 1)) * 0.5;
-        a.Gamma[q] = G0 * inv_rho_g_sm1 * sf_swash * max_dw;
+        c.Gamma = G0 * inv_rho_g_sm1 * sf_swash * max_dw;
     }
+    return c;
 }
 
-__global__ void __launch_bounds__(256) k_sed_fluxes(SedArgs a)
+// Coefficients and fluxes in one pass (src/Sediment.cpp:56-65, 156-250): thread j takes a
+// run of SED_RUN rows of column j. Each cell's coefficients are formed once and stored
+// (they are public outputs); the qx flux of row i reads row i+1's from the run's registers,
+// the qy flux reads column j+1's from lane l+1 (cross-lane, no memory access), and only
+// the last lane of a wavefront and column ny-1 (whose right neighbour is column 0) form
+// their right neighbour's themselves. The values are the ones the coefficient arrays would
+// hold, so the fluxes are those of the two-pass form, bit for bit. Every lane of a
+// wavefront holds the same rows (the runs are uniform per workgroup).
+// Rows per run, measured at 2048 x 4096 (ms per sediment step, fluxes + bed update):
+// 2 rows 0.341, 4 rows 0.327, 8 rows 0.331, 16 rows 0.351; the two-pass form took 0.370.
+constexpr int SED_RUN = 4;
+__global__ void __launch_bounds__(256) k_sed_coeffs_fluxes(SedArgs a)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
-    const int ny = a.ny;
-    if (j >= ny || i >= a.nx) return;
-    const long q = (long)i * ny + j;
-    // both fluxes are formed before either is stored: a store ahead of the qy loads would
-    // hold them behind it
-    const bool has_qx = i < a.nx - 1;
-    double qx = 0.0;
-    if (has_qx) {   // getqx: rows 0..nx-2, U row i+1
-        const double inv_dx = a.inv_dx[i];
-        double A_x = 0.5 * (a.Alpha[q] + a.Alpha[q + ny]);
-        double g_x = 0.5 * (a.gamma[q] + a.gamma[q + ny]);
-        double G_x = 0.5 * (a.Gamma[q] + a.Gamma[q + ny]);
-        double dh_dx = (a.h[q + ny] - a.h[q]) * inv_dx;
-        qx = A_x * (a.U[q + ny] - g_x * dh_dx) - G_x * dh_dx;
+    const int ny = a.ny, nx = a.nx;
+    const int i0 = blockIdx.y * SED_RUN;
+    if (j >= ny || i0 >= nx) return;
+    const int i1 = min(i0 + SED_RUN, nx);
+    const int jp = wrap_inc(j, ny);
+    const bool own_p = (threadIdx.x & 63) < 63 && j + 1 < ny;   // column j+1 is lane l+1's
+    const double zs_sh = a.Zs_sh[j], max_dw = a.maxDw[j];
+    const double zs_sh_p = own_p ? 0.0 : a.Zs_sh[jp], max_dw_p = own_p ? 0.0 : a.maxDw[jp];
+    long q = (long)i0 * ny + j;
+    SedCoef c0 = sed_coef(a, a.D[q], a.Zb[q], zs_sh, max_dw);
+    double h0 = a.h[q];
+    for (int i = i0; i < i1; ++i, q += ny) {
+        const bool has_qx = i < nx - 1;
+        const long qp = (long)i * ny + jp;
+        // loads of this row first: a store ahead of them would hold them behind it
+        double d1 = 0.0, zb1 = 0.0, h1 = 0.0, u1 = 0.0;
+        if (has_qx) { d1 = a.D[q + ny]; zb1 = a.Zb[q + ny]; h1 = a.h[q + ny]; u1 = a.U[q + ny]; }
+        const double v1 = a.V[q + ny];
+        double dp = 0.0, zbp = 0.0, hp_own = 0.0;
+        if (!own_p) { dp = a.D[qp]; zbp = a.Zb[qp]; hp_own = a.h[qp]; }
+        // column j+1 of row i from lane l+1 (every lane takes part)
+        const double al_l = __shfl_down(c0.alpha, 1), ga_l = __shfl_down(c0.gamma, 1);
+        const double Ga_l = __shfl_down(c0.Gamma, 1), h_l = __shfl_down(h0, 1);
+        SedCoef cp;
+        double hp;
+        if (own_p) { cp.alpha = al_l; cp.gamma = ga_l; cp.Gamma = Ga_l; hp = h_l; }
+        else { cp = sed_coef(a, dp, zbp, zs_sh_p, max_dw_p); hp = hp_own; }
+        SedCoef c1 = c0;
+        double qx = 0.0;
+        if (has_qx) {   // getqx: rows 0..nx-2, U row i+1
+            c1 = sed_coef(a, d1, zb1, zs_sh, max_dw);
+            const double inv_dx = a.inv_dx[i];
+            double A_x = 0.5 * (c0.alpha + c1.alpha);
+            double g_x = 0.5 * (c0.gamma + c1.gamma);
+            double G_x = 0.5 * (c0.Gamma + c1.Gamma);
+            double dh_dx = (h1 - h0) * inv_dx;
+            qx = A_x * (u1 - g_x * dh_dx) - G_x * dh_dx;
+        }
+        double qy;
+        {   // getqy: rows 0..nx-1, V row i+1, periodic in j
+            double A_y = 0.5 * (c0.alpha + cp.alpha);
+            double g_y = 0.5 * (c0.gamma + cp.gamma);
+            double G_y = 0.5 * (c0.Gamma + cp.Gamma);
+            double dh_dy = (hp - h0) * a.inv_dy;
+            qy = A_y * (v1 - g_y * dh_dy) - G_y * dh_dy;
+        }
+        a.Alpha[q] = c0.alpha;
+        a.gamma[q] = c0.gamma;
+        a.Gamma[q] = c0.Gamma;
+        if (has_qx) a.qx[q] = qx;
+        a.qy[q] = qy;
+        c0 = c1;
+        h0 = h1;
     }
-    double qy;
-    {   // getqy: rows 0..nx-1, V row i+1, periodic in j
-        const long qp = (long)i * ny + wrap_inc(j, ny);
-        double A_y = 0.5 * (a.Alpha[q] + a.Alpha[qp]);
-        double g_y = 0.5 * (a.gamma[q] + a.gamma[qp]);
-        double G_y = 0.5 * (a.Gamma[q] + a.Gamma[qp]);
-        double dh_dy = (a.h[qp] - a.h[q]) * a.inv_dy;
-        qy = A_y * (a.V[q + ny] - g_y * dh_dy) - G_y * dh_dy;
-    }
-    if (has_qx) a.qx[q] = qx;
-    a.qy[q] = qy;
 }
 
 __global__ void __launch_bounds__(256) k_sed_update(SedArgs a, double dt_2, double _1_m1p)
@@ -211,15 +250,9 @@ hipError_t launch_sed_columns(hipStream_t st, const SedArgs &a)
     return hipGetLastError();
 }
 
-hipError_t launch_sed_coeffs(hipStream_t st, const SedArgs &a)
+hipError_t launch_sed_coeffs_fluxes(hipStream_t st, const SedArgs &a)
 {
-    hipLaunchKernelGGL(k_sed_coeffs, sed_grid(a, a.nx), dim3(256), 0, st, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_sed_qxqy(hipStream_t st, const SedArgs &a)
-{
-    hipLaunchKernelGGL(k_sed_fluxes, sed_grid(a, a.nx), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(k_sed_coeffs_fluxes, sed_grid(a, (a.nx + SED_RUN - 1) / SED_RUN), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
