@@ -162,6 +162,9 @@ __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, 
     auto ld = [&](const double *f, long e) { return ldn<NTL>(f + e); };    // stencil-shared fields
     auto ldf = [&](const double *f, long e) { return ldn<NTF>(f + e); };   // single-use fields
     const double *nuD = rf.nuD;
+    // cD0 in a register: selected against a load, a field of k (which may live in a constant
+    // table) would make the select one of pointers in two address spaces, a flat load
+    const double cD0 = k.cD0;
     double n0 = ld(nuD, at(k0)), n0p = ld(nuD, atp(k0)), n0m = ld(nuD, atm(k0));   // nuD row i
     QxxIn q;
     int wis[CM];
@@ -195,7 +198,7 @@ __device__ __forceinline__ void qxx_segment(const Consts &k, const Metrics &mt, 
                 q.Vc1 = 0.5 * (q.v2j + q.v2m);
             }
             q.Qx1 = ldf(s.Qx, at(i + 1));
-            q.cD = (CDC || cdc_rt) ? k.cD0 : ldf(rf.cD_x, at(i));
+            q.cD = (CDC || cdc_rt) ? cD0 : ldf(rf.cD_x, at(i));
             q.Urms = ldf(rf.Urms_x, at(i)); q.Fx = ldf(rf.Fx, at(i));
             const double n1 = ld(nuD, at(i + 1)), n1p = ld(nuD, atp(i + 1)), n1m = ld(nuD, atm(i + 1));
             q.nuD_x = 0.5 * (n0 + n1);
