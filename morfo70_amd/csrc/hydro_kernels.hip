@@ -297,7 +297,9 @@ __device__ __forceinline__ void qy_tail2_body(const Consts &k, const State &s, i
 // Rows i0..i1-1 of columns j0, j0 + 1 (j0 even, ny even): the Qx tail over a run of rows,
 // with the Qx rows i-1..i+2 and the guess depths of rows i-1..i+1 held in registers from
 // one row to the next, so each row of Qx, D and dQy_dy is loaded once per run instead of
-// four or three times. The same expressions on the same values: the same bits.
+// four or three times. The same expressions on the same values: the same bits. Lane l of a
+// wavefront must hold the pair after lane l-1's (or lie past the row's last pair) over the
+// same rows, and every lane that holds a pair must call it (cross-lane reads).
 template <bool STR = false>
 __device__ __forceinline__ void qx_tail2_rows(const Consts &k, const Metrics &m, const State &s, const Work &w,
                                               double f0, int i0, int i1, int j0, bool write_uc)
@@ -336,7 +338,14 @@ __device__ __forceinline__ void qx_tail2_rows(const Consts &k, const Metrics &m,
     };
     double2 qm = qrow(i0 - 1), q0 = qrow(i0), q1 = qrow(i0 + 1), q2 = qrow(i0 + 2);
     G gm = grow(i0 - 1, qm, q0), g0 = grow(i0, q0, q1), g1 = grow(i0 + 1, q1, q2);
-    double cqa = qrow_c2(i0), cqb = qrow_c2(i0 + 1);
+    // Column j0+2 is lane l+1's j0 (a wavefront's lanes take consecutive pairs of a row run):
+    // its guess depth, guess level and dQy_dy come across lanes, the same expressions on the
+    // same values; only a lane whose right neighbour is in another wavefront or across the
+    // periodic seam loads what they are formed from.
+    const int lane = (int)(threadIdx.x & 63);
+    const bool own_p = lane < 63 && j0 + 2 < ny;
+    double cqa = 0.0, cqb = 0.0;
+    if (!own_p) { cqa = qrow_c2(i0); cqb = qrow_c2(i0 + 1); }
     for (int i = i0; i < i1; ++i) {
         const long r = (long)i * ny;
         double ua, ub;
@@ -357,14 +366,18 @@ __device__ __forceinline__ void qx_tail2_rows(const Consts &k, const Metrics &m,
             u1b = un.y;
         }
         const double2 zs = ld2<NTL>(s.Zs, r + j0);
+        double e_dqy = 0.0, e_d = 0.0, e_zs = 0.0;
+        if (!own_p) { e_dqy = ldn<NTL>(s.dQy + r + c2); e_d = ldn<NTL>(s.D + r + c2); e_zs = ldn<NTL>(s.Zs + r + c2); }
         const double gza = zs.x + g0.inc.x, gzb = zs.y + g0.inc.y;
-        double gd_c2, gz_c2, dqy_c2;
-        {   // guess_D / guess_Zs of column j0+2 (the right neighbour of j0+1)
+        // guess_D / guess_Zs of column j0+2 (the right neighbour of j0+1); every lane takes
+        // part in the cross-lane reads
+        double gd_c2 = __shfl_down(g0.gd.x, 1), gz_c2 = __shfl_down(gza, 1), dqy_c2 = __shfl_down(g0.dqy.x, 1);
+        if (!own_p) {
             const double dq = m.inv_dxx[i] * (cqb - cqa);
-            dqy_c2 = ldn<NTL>(s.dQy + r + c2);
+            dqy_c2 = e_dqy;
             const double incrZs = -dt * (dq + dqy_c2);
-            gd_c2 = ldn<NTL>(s.D + r + c2) + incrZs;
-            gz_c2 = ldn<NTL>(s.Zs + r + c2) + incrZs;
+            gd_c2 = e_d + incrZs;
+            gz_c2 = e_zs + incrZs;
         }
         const int wa = wet_rule(g0.gd.x, g0.gd.y, gza, gzb, g0.dqy.x, g0.dqy.y, k.Dmask);
         const int wb = wet_rule(g0.gd.y, gd_c2, gzb, gz_c2, g0.dqy.y, dqy_c2, k.Dmask);
@@ -381,8 +394,10 @@ __device__ __forceinline__ void qx_tail2_rows(const Consts &k, const Metrics &m,
             g0 = g1;
             g1 = grow(i + 2, q2, q3);
             qm = q0; q0 = q1; q1 = q2; q2 = q3;
-            cqa = cqb;
-            cqb = qrow_c2(i + 2);
+            if (!own_p) {
+                cqa = cqb;
+                cqb = qrow_c2(i + 2);
+            }
         }
     }
 }
