@@ -81,23 +81,27 @@ __host__ __device__ __forceinline__ void run_fields_body(const Consts &k, const 
     const int nx = k.nx;
     const int jp = wrap_inc(j, ny), jm = wrap_dec(j, ny);
     const double *nuD = rf.nuD;
-    if (i < nx - 1) {
+    // every value is formed before the first store: a store ahead of a load would hold it
+    // (the output arrays may alias the inputs as far as the compiler knows)
+    const bool xrow = i < nx - 1;
+    double Fx = 0.0, Urms_x = 0.0, cD_x = 0.0, nuD_x = 0.0, dnuD_dx = 0.0, dnuDx_dy = 0.0;
+    if (xrow) {
         const double *Sxy = fw.Sxy + r, *Sxy_up = Sxy + ny, *Sxx = fw.Sxx + r, *Sxx_up = Sxx + ny;
         // getForcing_x, src/HydroQx.cpp:371-386 (j = 0 and ny-1 wrap, :378-379)
-        rf.Fx[r + j] = -(k.inv_4dy_rho * (Sxy[jp] - Sxy[jm] + Sxy_up[jp] - Sxy_up[jm]) +
+        Fx = -(k.inv_4dy_rho * (Sxy[jp] - Sxy[jm] + Sxy_up[jp] - Sxy_up[jm]) +
                          (Sxx_up[j] - Sxx[j]) * m.inv_rhodx[i]);
         // C2X (include/MeshOperations.h:18-27) of Urms, cD, nuD: src/Hydro.cpp:177,182,186
-        rf.Urms_x[r + j] = 0.5 * (fw.Urms[r + j] + fw.Urms[r + ny + j]);
-        rf.cD_x[r + j] = 0.5 * (rf.cD[r + j] + rf.cD[r + ny + j]);
-        const double nux = 0.5 * (nuD[r + j] + nuD[r + ny + j]);
-        rf.nuD_x[r + j] = nux;
+        Urms_x = 0.5 * (fw.Urms[r + j] + fw.Urms[r + ny + j]);
+        cD_x = 0.5 * (rf.cD[r + j] + rf.cD[r + ny + j]);
+        nuD_x = 0.5 * (nuD[r + j] + nuD[r + ny + j]);
         // difC2X(dnuD_dx, nuD, dx), src/HydroMeshOperations.cpp:99-114
-        rf.dnuD_dx[r + j] = m.inv_dx[i] * (nuD[r + ny + j] - nuD[r + j]);
+        dnuD_dx = m.inv_dx[i] * (nuD[r + ny + j] - nuD[r + j]);
         // difC2C_y(dnuDx_dy, nuD_x), src/HydroMeshOperations.cpp:202-221
         const double nux_p = 0.5 * (nuD[r + jp] + nuD[r + ny + jp]);
         const double nux_m = 0.5 * (nuD[r + jm] + nuD[r + ny + jm]);
-        rf.dnuDx_dy[r + j] = k.inv_2dy_c2c * (nux_p - nux_m);
+        dnuDx_dy = k.inv_2dy_c2c * (nux_p - nux_m);
     }
+    double Fy;
     {   // getForcing_y, src/HydroQy.cpp:367-424. Row 0 uses Sxy rows 0,1,2 with the row-0
         // coefficients (:385-393); the last row has no "up" term (:413-421).
         const double d_dx_d = m.difx_c_d[i] * k.inv_rho;
@@ -116,18 +120,17 @@ __host__ __device__ __forceinline__ void run_fields_body(const Consts &k, const 
             val = -(k.inv_dy_rho * (Syy[jp] - Syy[j]) +
                     0.5 * ((Sxy[j] + Sxy[jp]) * d_dx_d + (Sxy_low[j] + Sxy_low[jp]) * d_dx_low));
         }
-        rf.Fy[r + j] = val;
+        Fy = val;
     }
     // C2Y (src/HydroMeshOperations.cpp:47-61) of Urms, cD, nuD: src/Hydro.cpp:180,184,187
-    rf.Urms_y[r + j] = 0.5 * (fw.Urms[r + jp] + fw.Urms[r + j]);
-    rf.cD_y[r + j] = 0.5 * (rf.cD[r + jp] + rf.cD[r + j]);
+    const double Urms_y = 0.5 * (fw.Urms[r + jp] + fw.Urms[r + j]);
+    const double cD_y = 0.5 * (rf.cD[r + jp] + rf.cD[r + j]);
     const double nuy = 0.5 * (nuD[r + jp] + nuD[r + j]);
-    rf.nuD_y[r + j] = nuy;
     // difC2Y(dnuD_dy, nuD), src/HydroMeshOperations.cpp:116-130
-    rf.dnuD_dy[r + j] = (nuD[r + jp] - nuD[r + j]) * k.inv_dy;
+    const double dnuD_dy = (nuD[r + jp] - nuD[r + j]) * k.inv_dy;
     // difC2C_x_addBoundary(dnuDy_dx, nuD_y), src/HydroMeshOperations.cpp:161-200
+    double out;
     {
-        double out;
         if (i == 0) {
             const double vu = 0.5 * (nuD[r + ny + jp] + nuD[r + ny + j]);
             out = m.difx_c_low[0] * (2 * nuy - vu) + m.difx_c_d[0] * nuy + vu * m.difx_c_up[0];
@@ -139,8 +142,21 @@ __host__ __device__ __forceinline__ void run_fields_body(const Consts &k, const 
             const double vl = 0.5 * (nuD[r - ny + jp] + nuD[r - ny + j]);
             out = m.difx_c_low[nx - 1] * vl + m.difx_c_d[nx - 1] * nuy;
         }
-        rf.dnuDy_dx[r + j] = out;
     }
+    if (xrow) {
+        rf.Fx[r + j] = Fx;
+        rf.Urms_x[r + j] = Urms_x;
+        rf.cD_x[r + j] = cD_x;
+        rf.nuD_x[r + j] = nuD_x;
+        rf.dnuD_dx[r + j] = dnuD_dx;
+        rf.dnuDx_dy[r + j] = dnuDx_dy;
+    }
+    rf.Fy[r + j] = Fy;
+    rf.Urms_y[r + j] = Urms_y;
+    rf.cD_y[r + j] = cD_y;
+    rf.nuD_y[r + j] = nuy;
+    rf.dnuD_dy[r + j] = dnuD_dy;
+    rf.dnuDy_dx[r + j] = out;
 }
 
 __global__ void __launch_bounds__(256) k_run_fields(Consts k, Metrics m, Forcing fw, RunFields rf)
