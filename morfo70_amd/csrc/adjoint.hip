@@ -57,8 +57,6 @@ __global__ void __launch_bounds__(256) k_adj_qxy(Consts k, State S0, State S1, R
 }
 // The Qx predictor's reverse by runs of ADJ_RUN rows of a column (adj_qxx_rows): workgroups
 // of 64 x 4 threads, a wavefront being 64 consecutive columns of one run.
-// 4 rows: 2 and 8 took 0.5% and 3% longer over the C4 gradient.
-constexpr int ADJ_RUN = 4;
 __global__ void __launch_bounds__(256) k_adj_qxx_rows(Consts k, Metrics m, State S0, RunFields rf, StepTape T,
                                                       AdjFields A, AdjWork W, AdjRun R, double f0)
 {

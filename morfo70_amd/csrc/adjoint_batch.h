@@ -39,7 +39,7 @@ typedef __attribute__((address_space(4))) const AdjScen cAdjScen;
 enum AdjGroupedCell {
     AGC_COPY, AGC_ZERO,
     AGC_QXX_ASM, AGC_QXY_ASM, AGC_QX_TAIL, AGC_QYY_ASM, AGC_QYX_ASM, AGC_QY_TAIL,
-    AGC_QY_TAIL_A, AGC_QY_TAIL_B, AGC_QYX, AGC_QYY, AGC_QX_TAIL_A, AGC_QX_TAIL_B, AGC_QXY, AGC_QXX,
+    AGC_QY_TAIL_A, AGC_QY_TAIL_B, AGC_QYX, AGC_QYY, AGC_QX_TAIL_A, AGC_QX_TAIL_B, AGC_QXY,
     AGC_COUNT
 };
 // Grid rows a cell stage covers.
@@ -51,6 +51,10 @@ __host__ __device__ inline int agc_rows(int op, int nx)
 }
 
 hipError_t launch_adj_cells_grouped(hipStream_t st, int op, const AdjScen *D, const int *start, int nscen, int blocks);
+// The Qx predictor's reverse (the last reverse stage) by runs of rows of a column;
+// adj_qxx_rows_blocks gives one scenario's workgroups.
+int adj_qxx_rows_blocks(int nx, int ny);
+hipError_t launch_adj_qxx_rows_grouped(hipStream_t st, const AdjScen *D, const int *start, int nscen, int blocks);
 // Column line solves of every scenario (n = nx-1 unknowns, ny lines), 1024-thread
 // workgroups: sys 1 (the Qx predictor, A1) or 4 (the Qy projector, A4); trans 0 solves
 // the taped system (right-hand side T.f, solution onto the tape or the state), trans 1

@@ -96,9 +96,41 @@ __global__ void __launch_bounds__(256) k_adj_cells_grouped(cAdjScen *D, const in
         adj_qx_tail_b(k, d.m, d.Ain, d.Aout, d.W, i, j);
     } else if constexpr (OP == AGC_QXY) {
         adj_qxy(k, d.S0, d.s, d.rf, d.T, d.Aout, d.W, d.R, i, j);
-    } else if constexpr (OP == AGC_QXX) {
-        adj_qxx(k, d.m, d.S0, d.rf, d.T, d.Aout, d.W, d.R, d.f0, i, j);
+
     }
+}
+
+// The Qx predictor's reverse of every scenario by runs of ADJ_RUN rows of a column, as the
+// single-scenario k_adj_qxx_rows: workgroups of 64 x 4 threads, each scenario's workgroups
+// tiling ceil(ny/64) columns by ceil(runs/4) run groups.
+__global__ void __launch_bounds__(256) k_adj_qxx_rows_grouped(cAdjScen *D, const int *__restrict__ start, int nscen)
+{
+    const int b = blockIdx.x;
+    const int sid = scen_of(start, nscen, b);
+    const AdjScen &d = *(const AdjScen *)(D + sid);
+    const int ny = d.k.ny, n = d.k.nx - 1;
+    const int gx = (ny + 63) / 64;
+    const int lb = b - start[sid];
+    const int by = lb / gx, bx = lb - by * gx;
+    const int j = bx * 64 + (int)threadIdx.x;
+    const int run = __builtin_amdgcn_readfirstlane(by * 4 + (int)threadIdx.y);
+    const int k0 = run * ADJ_RUN;
+    if (j >= ny || k0 >= n) return;
+    const int k1 = k0 + ADJ_RUN < n ? k0 + ADJ_RUN : n;
+    adj_qxx_rows(d.k, d.m, d.S0, d.rf, d.T, d.Aout, d.W, d.R, d.f0, k0, k1, j);
+}
+
+int adj_qxx_rows_blocks(int nx, int ny)
+{
+    const int runs = (nx - 1 + ADJ_RUN - 1) / ADJ_RUN;
+    return ((ny + 63) / 64) * ((runs + 3) / 4);
+}
+
+hipError_t launch_adj_qxx_rows_grouped(hipStream_t st, const AdjScen *D, const int *start, int nscen, int blocks)
+{
+    if (blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adj_qxx_rows_grouped, dim3((unsigned)blocks), dim3(64, 4), 0, st, (cAdjScen *)D, start, nscen);
+    return hipGetLastError();
 }
 
 template <int OP>
@@ -126,7 +158,6 @@ hipError_t launch_adj_cells_grouped(hipStream_t st, int op, const AdjScen *D, co
     case AGC_QX_TAIL_A: launch_one<AGC_QX_TAIL_A>(st, D, start, nscen, blocks); break;
     case AGC_QX_TAIL_B: launch_one<AGC_QX_TAIL_B>(st, D, start, nscen, blocks); break;
     case AGC_QXY: launch_one<AGC_QXY>(st, D, start, nscen, blocks); break;
-    case AGC_QXX: launch_one<AGC_QXX>(st, D, start, nscen, blocks); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -1646,7 +1646,7 @@ int m70h_adjoint_release(m70h *h)
 // stages to the device ahead of each operation.
 // =====================================================================================
 namespace {
-enum { GA_NXP1 = 0, GA_NX = 1, GA_NX1 = 2, GA_COLS = 3, GA_ROWS = 4, GA_COUNT = 5 };
+enum { GA_NXP1 = 0, GA_NX = 1, GA_NX1 = 2, GA_COLS = 3, GA_ROWS = 4, GA_QXR = 5, GA_COUNT = 6 };
 }
 
 struct m70h_adjoint_batch {
@@ -1737,6 +1737,11 @@ int ab_cols(m70h_adjoint_batch *b, int sys, int trans, const AdjScen *D)
 {
     ABCHK(b, launch_adj_cols_grouped(b->stream, sys, trans, D, ab_start(b, GA_COLS), (int)b->hs.size(),
                                      b->blocks[GA_COLS]));
+    return M70H_OK;
+}
+int ab_qxx_rows(m70h_adjoint_batch *b, const AdjScen *D)
+{
+    ABCHK(b, launch_adj_qxx_rows_grouped(b->stream, D, ab_start(b, GA_QXR), (int)b->hs.size(), b->blocks[GA_QXR]));
     return M70H_OK;
 }
 int ab_rows(m70h_adjoint_batch *b, int sys, int trans, const AdjScen *D)
@@ -1836,7 +1841,7 @@ int ab_reverse_kernels(m70h_adjoint_batch *b, int t, int s0sel, int s0q)
         (rc = ab_cols(b, 4, 1, D)) || (rc = ab_cells(b, AGC_QYX, D)) || (rc = ab_rows(b, 3, 1, D)) ||
         (rc = ab_cells(b, AGC_QYY, D)) || (rc = ab_cells(b, AGC_QX_TAIL_A, D)) || (rc = ab_cells(b, AGC_QX_TAIL_B, D)) ||
         (rc = ab_rows(b, 2, 1, D)) || (rc = ab_cells(b, AGC_QXY, D)) || (rc = ab_cols(b, 1, 1, D)) ||
-        (rc = ab_cells(b, AGC_QXX, D)))
+        (rc = ab_qxx_rows(b, D)))
         return rc;
     b->cur = 1 - b->cur;
     return M70H_OK;
@@ -1943,7 +1948,8 @@ int m70h_adjoint_batch_create(int nscen, m70h *const *handles, m70h_adjoint_batc
         const int gx = (h->ny + 255) / 256;
         const int Sc = adj_cols_segments(h->nx - 1), Lc = 1024 / Sc;
         if (adj_rows_segments(h->ny, b->rb) <= 0) return bail(abfail(b, M70H_EINVAL, "adjoint batch: ny too large"));
-        const int per[GA_COUNT] = {gx * (h->nx + 1), gx * h->nx, gx * (h->nx - 1), (h->ny + Lc - 1) / Lc, h->nx - 1};
+        const int per[GA_COUNT] = {gx * (h->nx + 1), gx * h->nx, gx * (h->nx - 1), (h->ny + Lc - 1) / Lc, h->nx - 1,
+                                   adj_qxx_rows_blocks(h->nx, h->ny)};
         for (int gs = 0; gs < GA_COUNT; ++gs)
             starts[(size_t)gs * (nscen + 1) + q + 1] = starts[(size_t)gs * (nscen + 1) + q] + per[gs];
     }
